@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-step vortex-element time step on one MI355X.
+
+Workload: circular cylinder, radius 0.5, 1000 equal panels, U = (1, 0),
+Re = 1000 (nu = 1e-3), dt = 0.005, blob radius = panel length, with a
+synthetic wake of N = 2^20 vortices (uniform in [-0.6, 10] x [-2, 2] minus the
+disk, circulations +-U(0, 2 * eps) alternating in sign, seed 1) resident in
+HBM.  One step = one full time step: birth from the panel system, Biot-Savart
++ nearest-3, vortex and wall diffusive velocities, move / absorb / delete,
+force sums, compaction and two collapse kinds (up to 2 passes each).
+
+Prints one JSON line: steps/s of the mixed-precision path (value), its
+pair-interactions/s, the plain FP64 GPU path's steps/s on the same state, the
+drag-coefficient difference of one step from the identical state, the
+roofline of the dominant kernel and the FP64 CPU oracle timed on a bounded
+sample on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
+PEAK_FP64 = 78.6   # TFLOP/s, MI355X vector FP64
+BS_FLOP_PER_PAIR = 12
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_case(prec, n, P=1000, dt=0.005):
+    from vortex_amd import geometry as geo
+    from vortex_amd.solver import Case, make_params
+    from vortex_amd.state import uniform_wake
+    pan = geo.circle(P, 0.5)
+    eps = float(pan["length"][0])
+    prm = make_params(pan, dt=dt, ve_size=eps, nu=1e-3, rho=1.0, vinf=(1.0, 0.0),
+                      box=(-2.0, 10.0, -10.0, 10.0), r_col_diff_sign=0.3, r_col_same_sign=0.3,
+                      max_ve_g=2 * eps, n_col=2, precision=prec)
+    x, y, g = uniform_wake(n, seed=1, max_ve_g=2 * eps)
+    c = Case(prm, pan)
+    c.reserve(n + 8 * P + 64)
+    return c, (x, y, g), prm
+
+
+def run_path(prec, n, steps, warmup, M=None):
+    c, (x, y, g), prm = make_case(prec, n)
+    if M is None:
+        c.build_matrix()
+        M = c.matrix()
+    else:
+        c.set_matrix(M)
+    c.set_state(x, y, g)
+    c.enable_timing(True)
+    vinf = (1.0, 0.0)
+    first = None
+    for _ in range(warmup):
+        f = c.time_step(vinf)
+        first = f if first is None else first
+    c.sync()
+    t0 = c.timings()
+    n_start = c.n
+    t = time.perf_counter()
+    forces = []
+    for _ in range(steps):
+        forces.append(c.time_step(vinf))
+    c.sync()
+    wall = time.perf_counter() - t
+    t1 = c.timings()
+    res = {
+        "wall": wall, "n_start": n_start, "n_end": c.n,
+        "bs_ms": (t1["bs_ms"] - t0["bs_ms"]) / max(1, t1["bs_launches"] - t0["bs_launches"]),
+        "diff_ms": (t1["diff_ms"] - t0["diff_ms"]) / max(1, steps),
+        "speed_ms": (t1["speed_ms"] - t0["speed_ms"]) / max(1, steps),
+        "creation_ms": (t1["creation_ms"] - t0["creation_ms"]) / max(1, steps),
+        "step_ms": (t1["step_ms"] - t0["step_ms"]) / max(1, steps),
+        "first_forces": first, "forces": forces,
+    }
+    c.close()
+    return res, M
+
+
+def one_step_cd(prec, n, M):
+    """Drag of one full step from the synthetic state (Cd = Fx / (0.5 rho U^2 D), D = 1)."""
+    c, (x, y, g), prm = make_case(prec, n)
+    c.set_matrix(M)
+    c.set_state(x, y, g)
+    f = c.time_step((1.0, 0.0))
+    c.close()
+    return f[1] / (0.5 * 1.0 * 1.0 * 1.0)
+
+
+def cpu_baseline(n, sample_targets, threads):
+    """FP64 oracle (multithreaded C) on the host: Biot-Savart + diffusion for a
+    sample of targets against all n sources, scaled to a full step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import vx_ref
+    from vortex_amd.state import uniform_wake
+    eps = math.pi / 1000
+    x, y, g = uniform_wake(n, seed=1, max_ve_g=2 * eps)
+    vx_ref.lib().vxr_set_num_threads(threads)
+    m = sample_targets
+    t = time.perf_counter()
+    u, v, d = vx_ref.biot_savart(x, y, g, eps * eps, (1.0, 0.0), n_tgt=m)
+    t_bs = time.perf_counter() - t
+    t = time.perf_counter()
+    dfull = np.full(n, 0.005)
+    dfull[:m] = d
+    vx_ref.diffusion(x, y, g, dfull, 1e-3, eps, u, v, n_tgt=m)
+    t_diff = time.perf_counter() - t
+    scale = n / m
+    step_s = (t_bs + t_diff) * scale
+    return {
+        "value": 1.0 / step_s, "unit": "steps/s", "cores": threads, "kind": "port",
+        "sample": (f"FP64 oracle Biot-Savart + nearest-3 and vortex diffusion for the first {m} "
+                   f"targets against all {n} sources ({t_bs + t_diff:.1f} s), scaled by N/{m} to one "
+                   f"step; panel, wall and O(N) stages omitted (they add under 1% at this N)"),
+        "pairs_per_s": m * (n - 1) / t_bs,
+    }
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    counter summary, if present (FETCH_SIZE + WRITE_SIZE, KiB, copy-calibrated)."""
+    p = os.path.join(ROOT, "profiles", "biot_savart_mixed_1M_counters.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split()[1])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--f64-steps", type=int, default=2)
+    ap.add_argument("--cpu-targets", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by targets, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from vortex_amd.solver import VX_PREC_F64, VX_PREC_MIXED
+    n = args.n
+    log(f"[bench] mixed path: N={n}, warmup={args.warmup}, steps={args.steps}")
+    mixed, M = run_path(VX_PREC_MIXED, n, args.steps, args.warmup)
+    sps = args.steps / mixed["wall"]
+    n_avg = 0.5 * (mixed["n_start"] + mixed["n_end"])
+    s_bs = math.ceil(n_avg / 32) * 32
+    pairs = s_bs * (s_bs - 1)
+    bs_s = mixed["bs_ms"] / 1e3
+    achieved = BS_FLOP_PER_PAIR * pairs / bs_s / 1e12
+    log(f"[bench] mixed: {sps:.4f} steps/s, BS {mixed['bs_ms']:.1f} ms, diffusion {mixed['diff_ms']:.1f} ms, "
+        f"speed {mixed['speed_ms']:.1f} ms, creation {mixed['creation_ms']:.2f} ms, step {mixed['step_ms']:.2f} ms")
+
+    f64 = None
+    dcd = None
+    if not args.no_f64 and args.f64_steps > 0:
+        log(f"[bench] FP64 path: steps={args.f64_steps}")
+        f64, _ = run_path(VX_PREC_F64, n, args.f64_steps, 0, M=M)
+        log(f"[bench] f64: {args.f64_steps / f64['wall']:.4f} steps/s, BS {f64['bs_ms']:.1f} ms")
+        cd_m = one_step_cd(VX_PREC_MIXED, n, M)
+        cd_f = one_step_cd(VX_PREC_F64, n, M)
+        dcd = abs(cd_m - cd_f) / max(abs(cd_f), 1e-300)
+        log(f"[bench] one-step Cd mixed {cd_m:.9g} f64 {cd_f:.9g} rel diff {dcd:.3g}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] CPU oracle sample: {args.cpu_targets} targets x {n} sources, {threads} threads")
+        cpu = cpu_baseline(n, args.cpu_targets, threads)
+        log(f"[bench] cpu: {cpu['value']:.3g} steps/s ({cpu['pairs_per_s']:.3g} BS pairs/s)")
+
+    out = {
+        "metric": "time steps/s at N=1M (mixed precision, direct sums)",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / sps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic (uniform wake, seed 1)",
+        "config": {"workload": "cylinder P=1000, Re=1000, dt=0.005, eps=pi/1000, N=2^20 wake",
+                   "n_vortices": int(n_avg), "panels": 1000, "parallelism": "single GPU"},
+        "pair_interactions_per_s": pairs / bs_s,
+        "f64_steps_per_s": (args.f64_steps / f64["wall"]) if f64 else None,
+        "mixed_over_f64": (sps / (args.f64_steps / f64["wall"])) if f64 else None,
+        "delta_cd_rel_one_step": dcd,
+        "stage_ms": {k: mixed[k] for k in ("creation_ms", "speed_ms", "bs_ms", "diff_ms", "step_ms")},
+        "roofline": {"kernel": "k_biot_savart<float,4>", "bound": "valu", "achieved": achieved,
+                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": traffic_from_profiles(),
+                     "flop_per_pair": BS_FLOP_PER_PAIR, "pairs_per_launch": pairs},
+        "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

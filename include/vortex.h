@@ -1,0 +1,142 @@
+/*
+ * vortex.h -- C-ABI of the MI355X vortex-element time step (libvortex_amd.so).
+ *
+ * One vx_case holds one flow case: parameters, panels, the panel-system
+ * inverse and the vortex state, all resident in HBM as structure-of-arrays,
+ * plus its own HIP stream.  Every entry point returns 0 on success, 1 on a
+ * HIP/runtime error and 2 on bad arguments (the reference's convention,
+ * include/main.h:20-26 and src/unit_1.cu:459-461); vx_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Host arrays passed in or out are plain double/int pointers; panels are
+ * arrays of the reference's 120-byte tPanel layout (vx_panel below).
+ *
+ * Entry points and the reference routine each one replaces
+ * (grechkin-pogrebnyakov/vortex, path:line):
+ *   vx_case_create            init_device_conf_values    src/unit_1.cu:1220-1252
+ *                             + panel upload             src/main.c:541-552
+ *   vx_case_build_matrix      matr_creation              src/unit_1.cu:256-402
+ *   vx_case_set_matrix        load_matrix                src/unit_1.cu:404-456
+ *   vx_case_reserve           incr_vort_quant / allocate_arrays / randomize_tail
+ *                                                        src/unit_1.cu:458-558
+ *   vx_vort_creation          vort_creation              src/unit_1.cu:560-595
+ *   vx_speed                  Speed                      src/unit_1.cu:963-1075
+ *   vx_step                   Step                       src/unit_1.cu:1077-1218
+ *   vx_velocity_probe         velocity_control           src/unit_1.cu:1254-1269
+ *   vx_time_step              one iteration of the loop  src/main.c:652-785
+ * Stage-level entry points (vx_right_part, vx_birth, vx_biot_savart,
+ * vx_diffusion, vx_diffusion_wall, vx_step_move, vx_compact, vx_setka,
+ * vx_collapse) run one kernel group each, as the reference kernels of the
+ * same names do (src/kernel.cu:779-1271), for parity testing and profiling.
+ */
+#ifndef VORTEX_AMD_H_
+#define VORTEX_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VX_ABI_VERSION 1
+
+/* Byte-identical to the reference tPanel (include/definitions.h:92-113). */
+typedef struct vx_panel {
+    unsigned n;
+    double left[2];
+    double right[2];
+    double contr[2];
+    double birth[2];
+    double norm[2];
+    double tang[2];
+    double length;
+    unsigned n_of_lpanel;
+    unsigned n_of_rpanel;
+} vx_panel;
+
+/* Precision of the pairwise stages.  VX_PREC_F64: the reference method in
+ * double precision throughout.  VX_PREC_MIXED: pairwise and far-field sums in
+ * FP32 with per-tile FP64 accumulation; panel system, reductions, moves,
+ * compaction and collapse stay FP64. */
+enum { VX_PREC_F64 = 0, VX_PREC_MIXED = 1 };
+
+/* The reference's device constants and conf fields (src/unit_1.cu:20-44,
+ * include/definitions.h:115-142).  birth_offset_rel = 0 keeps the reference
+ * birth point (panel midpoint + 1e-6 normal); > 0 places newborns at
+ * midpoint + birth_offset_rel * length * normal. */
+typedef struct vx_params {
+    double dt, ve_size, nu, rho;
+    double vinf_x, vinf_y;
+    double x_min, x_max, y_min, y_max;          /* deletion box                  */
+    double px_min, px_max, py_min, py_max;      /* profile box (left end points) */
+    double r_col_diff_sign, r_col_same_sign;    /* collapse radii / ve_size      */
+    double max_ve_g;
+    double rc_x, rc_y;
+    double birth_offset_rel;
+    int64_t n_of_points;                        /* wall quadrature points        */
+    int64_t n_col;                              /* collapse passes per kind      */
+    int64_t inc_step;                           /* capacity growth quantum       */
+    int32_t precision;                          /* VX_PREC_*                     */
+    int32_t tree_min_n;                         /* Morton-tree velocities at N >= this (0: never) */
+} vx_params;
+
+typedef struct vx_case vx_case;
+
+int         vx_abi_version(void);
+const char *vx_last_error(void);
+int         vx_device_count(int *count);
+
+/* ---- case lifetime and state ------------------------------------------- */
+int vx_case_create(const vx_params *prm, const vx_panel *panels, int64_t P, vx_case **out);
+int vx_case_destroy(vx_case *c);
+int vx_case_build_matrix(vx_case *c);                        /* L on the GPU + Gauss-Jordan */
+int vx_case_set_matrix(vx_case *c, const double *M, int64_t m); /* m = P+1, row-major      */
+int vx_case_get_matrix(vx_case *c, double *M, int64_t m);
+int vx_case_get_panel_matrix(vx_case *c, double *L, int64_t m);  /* L before inversion     */
+int vx_case_reserve(vx_case *c, int64_t capacity);
+int vx_case_set_state(vx_case *c, const double *x, const double *y, const double *g, int64_t n);
+/* Copies slots [0, count) (count may exceed n: the tail beyond n is included). */
+int vx_case_get_state(vx_case *c, double *x, double *y, double *g, int64_t count);
+int vx_case_get_velocity(vx_case *c, double *u, double *v, double *d, int64_t count);
+int vx_case_set_velocity(vx_case *c, const double *u, const double *v, const double *d, int64_t count);
+int64_t vx_case_n(const vx_case *c);
+int64_t vx_case_capacity(const vx_case *c);
+int vx_case_set_n(vx_case *c, int64_t n);
+double  vx_case_d_g(const vx_case *c);
+int vx_case_set_d_g(vx_case *c, double d_g);
+int vx_case_sync(vx_case *c);
+
+/* ---- per-step stages (device resident) ---------------------------------- */
+int vx_vort_creation(vx_case *c, double vinf_x, double vinf_y);
+int vx_speed(vx_case *c, double vinf_x, double vinf_y);
+int vx_step(vx_case *c, double out4[4]);
+int vx_time_step(vx_case *c, double vinf_x, double vinf_y, double forces4[4]);
+int vx_run_steps(vx_case *c, int64_t steps, double vinf_x, double vinf_y, double *forces4_per_step);
+int vx_velocity_probe(vx_case *c, const double *px, const double *py, int64_t m,
+                      double vinf_x, double vinf_y, double *u, double *v);
+
+/* ---- stage-level entry points (parity tests, profiling) ------------------ */
+int vx_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y, double *R_p);
+int vx_birth(vx_case *c, const double *R_p, double d_g, double *gsheet);   /* writes [n, n+P) */
+int vx_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, double vinf_y);
+int vx_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src);
+int vx_diffusion_wall(vx_case *c, int64_t n_tgt);
+int vx_step_move(vx_case *c, double out4[4]);
+int vx_compact(vx_case *c);
+int vx_setka(vx_case *c, int kind, int32_t *COL);                           /* COL[n] out */
+int vx_collapse(vx_case *c, int kind, const int32_t *COL);
+
+/* ---- timing of the last stages on the case stream (milliseconds) -------- */
+typedef struct vx_timings {
+    double creation_ms, speed_ms, step_ms;      /* cumulative, as the timings line */
+    double bs_ms, diff_ms, wall_ms;             /* cumulative, per Speed kernel    */
+    int64_t bs_launches;
+} vx_timings;
+int vx_case_timings(vx_case *c, vx_timings *t);
+int vx_case_enable_timing(vx_case *c, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VORTEX_AMD_H_ */

@@ -1,0 +1,445 @@
+// Host orchestration behind the C-ABI of include/vortex.h.
+//
+// The call order of one time step follows the reference loop
+// (src/main.c:652-785): birth (vort_creation), velocities over the padded
+// count s = ceil32(n) (Speed), then move/absorb/delete, sums, compaction and
+// the collapse passes (Step).  All device work of a case runs on its own HIP
+// stream; the host reads back only the four sums and the vortex count, as the
+// reference does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vx_internal.h"
+
+namespace vx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+static int64_t ceil32(int64_t n) { return (n + 31) / 32 * 32; }
+
+// Grow capacity in inc_step quanta (incr_vort_quant, src/unit_1.cu:511-558):
+// positions are kept, velocities need not be, new slots get the far tail.
+static int reserve(vx_case *c, int64_t want) {
+    if (want <= c->cap) return 0;
+    const int64_t inc = c->prm.inc_step > 0 ? c->prm.inc_step : 8192;
+    int64_t cap = c->cap;
+    while (cap < want) cap += inc;
+    const int64_t old = c->cap;
+    hipStream_t s = c->stream;
+    if (c->x.ensure((size_t)cap, s, true) || c->y.ensure((size_t)cap, s, true) || c->g.ensure((size_t)cap, s, true) ||
+        c->u.ensure((size_t)cap, s, true) || c->v.ensure((size_t)cap, s, true) || c->d.ensure((size_t)cap, s, true))
+        return 1;
+    c->cap = cap;
+    if (launch_fill_tail(c, old, cap)) return 1;
+    VX_CHECK(hipMemsetAsync(c->u.p + old, 0, sizeof(double) * (cap - old), s));
+    VX_CHECK(hipMemsetAsync(c->v.p + old, 0, sizeof(double) * (cap - old), s));
+    VX_CHECK(hipMemsetAsync(c->d.p + old, 0, sizeof(double) * (cap - old), s));
+    return 0;
+}
+
+struct Timer {
+    vx_case *c;
+    double *acc;
+    bool on;
+    Timer(vx_case *cc, double *a, int slot) : c(cc), acc(a), on(cc->timing) {
+        if (on) (void)hipEventRecord(c->ev[slot * 2], c->stream);
+        this->slot = slot;
+    }
+    int slot;
+    void stop() {
+        if (!on) return;
+        (void)hipEventRecord(c->ev[slot * 2 + 1], c->stream);
+        (void)hipEventSynchronize(c->ev[slot * 2 + 1]);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev[slot * 2], c->ev[slot * 2 + 1]);
+        *acc += ms;
+        on = false;
+    }
+    ~Timer() { stop(); }
+};
+
+}  // namespace vx
+
+using namespace vx;
+
+extern "C" {
+
+int vx_abi_version(void) { return VX_ABI_VERSION; }
+const char *vx_last_error(void) { return g_err.c_str(); }
+
+int vx_device_count(int *count) {
+    VX_ARG(count, "count");
+    int n = 0;
+    VX_CHECK(hipGetDeviceCount(&n));
+    *count = n;
+    return 0;
+}
+
+int vx_case_create(const vx_params *prm, const vx_panel *panels, int64_t P, vx_case **out) {
+    VX_ARG(prm && panels && out, "null pointer");
+    VX_ARG(P >= 2, "need at least two panels");
+    VX_ARG(prm->dt > 0 && prm->ve_size > 0, "dt and ve_size must be positive");
+    VX_ARG(prm->precision == VX_PREC_F64 || prm->precision == VX_PREC_MIXED, "precision");
+    for (int64_t i = 0; i < P; ++i)
+        VX_ARG(panels[i].n_of_rpanel < (unsigned)P && panels[i].n_of_lpanel < (unsigned)P, "panel neighbour index");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error("no HIP device");
+        return 1;
+    }
+    vx_case *c = new vx_case();
+    c->prm = *prm;
+    if (c->prm.n_col < 0) c->prm.n_col = 0;
+    if (c->prm.n_of_points <= 0) c->prm.n_of_points = 20;
+    c->P = P;
+    VX_CHECK(hipGetDevice(&c->device));
+    VX_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    VX_CHECK(hipHostMalloc((void **)&c->h_count, 64, hipHostMallocDefault));
+    VX_CHECK(hipHostMalloc((void **)&c->h_red, 64, hipHostMallocDefault));
+    for (auto &e : c->ev) VX_CHECK(hipEventCreate(&e));
+    // panels -> SoA, one allocation: 13 double columns + rp
+    std::vector<double> cols((size_t)(13 * P));
+    std::vector<int32_t> rp((size_t)P);
+    double bxmin = 1e300, bxmax = -1e300, bymin = 1e300, bymax = -1e300, lmax = 0;
+    for (int64_t i = 0; i < P; ++i) {
+        const vx_panel &p = panels[i];
+        const double v[13] = {p.left[0], p.left[1], p.right[0], p.right[1], p.contr[0], p.contr[1], p.birth[0],
+                              p.birth[1], p.norm[0], p.norm[1], p.tang[0], p.tang[1], p.length};
+        for (int k = 0; k < 13; ++k) cols[(size_t)(k * P + i)] = v[k];
+        rp[(size_t)i] = (int32_t)p.n_of_rpanel;
+        bxmin = std::min(bxmin, p.birth[0]); bxmax = std::max(bxmax, p.birth[0]);
+        bymin = std::min(bymin, p.birth[1]); bymax = std::max(bymax, p.birth[1]);
+        lmax = std::max(lmax, p.length);
+    }
+    c->wall_box = make_double4(bxmin - 5 * lmax, bxmax + 5 * lmax, bymin - 5 * lmax, bymax + 5 * lmax);
+    const size_t bytes = sizeof(double) * 13 * P + sizeof(int32_t) * P;
+    VX_CHECK(hipMalloc(&c->pan_block, bytes));
+    VX_CHECK(hipMemcpy(c->pan_block, cols.data(), sizeof(double) * 13 * P, hipMemcpyHostToDevice));
+    VX_CHECK(hipMemcpy((char *)c->pan_block + sizeof(double) * 13 * P, rp.data(), sizeof(int32_t) * P,
+                       hipMemcpyHostToDevice));
+    double *b = (double *)c->pan_block;
+    double **f[13] = {&c->pan.lx, &c->pan.ly, &c->pan.rx, &c->pan.ry, &c->pan.cx, &c->pan.cy, &c->pan.bx,
+                      &c->pan.by, &c->pan.nx, &c->pan.ny, &c->pan.tx, &c->pan.ty, &c->pan.len};
+    for (int k = 0; k < 13; ++k) *f[k] = b + (size_t)k * P;
+    c->pan.rp = (int32_t *)((char *)c->pan_block + sizeof(double) * 13 * P);
+    c->pan.P = P;
+    if (reserve(c, c->prm.inc_step > 0 ? c->prm.inc_step : 8192)) {
+        vx_case_destroy(c);
+        return 1;
+    }
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    *out = c;
+    return 0;
+}
+
+int vx_case_destroy(vx_case *c) {
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    vx::DBuf<double> *db[] = {&c->L, &c->M, &c->x, &c->y, &c->g, &c->u, &c->v, &c->d, &c->rp, &c->gsheet,
+                              &c->partial, &c->red, &c->tx, &c->ty, &c->tg, &c->px, &c->py, &c->pu, &c->pv};
+    for (auto *p : db) p->release();
+    c->xf.release(); c->yf.release(); c->gf.release();
+    c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();
+    c->keys.release(); c->keys_alt.release(); c->vals.release(); c->vals_alt.release();
+    c->cub_tmp.release(); c->dcount.release();
+    if (c->pan_block) (void)hipFree(c->pan_block);
+    if (c->h_count) (void)hipHostFree(c->h_count);
+    if (c->h_red) (void)hipHostFree(c->h_red);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int vx_case_build_matrix(vx_case *c) {
+    VX_ARG(c, "case");
+    if (launch_panel_matrix(c) || launch_invert(c)) return 1;
+    c->have_matrix = true;
+    return 0;
+}
+
+int vx_case_set_matrix(vx_case *c, const double *M, int64_t m) {
+    VX_ARG(c && M, "null pointer");
+    VX_ARG(m == c->P + 1, "matrix size must be P+1");
+    if (c->M.ensure((size_t)(m * m), c->stream)) return 1;
+    VX_CHECK(hipMemcpyAsync(c->M.p, M, sizeof(double) * m * m, hipMemcpyHostToDevice, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    c->have_matrix = true;
+    return 0;
+}
+
+int vx_case_get_matrix(vx_case *c, double *M, int64_t m) {
+    VX_ARG(c && M, "null pointer");
+    VX_ARG(m == c->P + 1 && c->have_matrix, "no matrix of that size");
+    VX_CHECK(hipMemcpyAsync(M, c->M.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_case_get_panel_matrix(vx_case *c, double *L, int64_t m) {
+    VX_ARG(c && L, "null pointer");
+    VX_ARG(m == c->P + 1, "matrix size must be P+1");
+    if (!c->L.p && launch_panel_matrix(c)) return 1;
+    VX_CHECK(hipMemcpyAsync(L, c->L.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_case_reserve(vx_case *c, int64_t capacity) {
+    VX_ARG(c && capacity >= 0, "case/capacity");
+    if (reserve(c, capacity)) return 1;
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_case_set_state(vx_case *c, const double *x, const double *y, const double *g, int64_t n) {
+    VX_ARG(c && n >= 0 && (n == 0 || (x && y && g)), "state arrays");
+    if (reserve(c, ceil32(n + c->P))) return 1;
+    if (n) {
+        VX_CHECK(hipMemcpyAsync(c->x.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        VX_CHECK(hipMemcpyAsync(c->y.p, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        VX_CHECK(hipMemcpyAsync(c->g.p, g, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    c->n = n;
+    return 0;
+}
+
+int vx_case_get_state(vx_case *c, double *x, double *y, double *g, int64_t count) {
+    VX_ARG(c && count >= 0 && count <= c->cap, "count beyond capacity");
+    if (x) VX_CHECK(hipMemcpyAsync(x, c->x.p, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    if (y) VX_CHECK(hipMemcpyAsync(y, c->y.p, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    if (g) VX_CHECK(hipMemcpyAsync(g, c->g.p, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_case_get_velocity(vx_case *c, double *u, double *v, double *d, int64_t count) {
+    VX_ARG(c && count >= 0 && count <= c->cap, "count beyond capacity");
+    if (u) VX_CHECK(hipMemcpyAsync(u, c->u.p, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    if (v) VX_CHECK(hipMemcpyAsync(v, c->v.p, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    if (d) VX_CHECK(hipMemcpyAsync(d, c->d.p, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_case_set_velocity(vx_case *c, const double *u, const double *v, const double *d, int64_t count) {
+    VX_ARG(c && count >= 0 && count <= c->cap, "count beyond capacity");
+    if (u) VX_CHECK(hipMemcpyAsync(c->u.p, u, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
+    if (v) VX_CHECK(hipMemcpyAsync(c->v.p, v, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
+    if (d) VX_CHECK(hipMemcpyAsync(c->d.p, d, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int64_t vx_case_n(const vx_case *c) { return c ? c->n : -1; }
+int64_t vx_case_capacity(const vx_case *c) { return c ? c->cap : -1; }
+double vx_case_d_g(const vx_case *c) { return c ? c->d_g : 0.0; }
+
+int vx_case_set_n(vx_case *c, int64_t n) {
+    VX_ARG(c && n >= 0 && n <= c->cap, "n beyond capacity");
+    c->n = n;
+    return 0;
+}
+
+int vx_case_set_d_g(vx_case *c, double d_g) {
+    VX_ARG(c, "case");
+    c->d_g = d_g;
+    return 0;
+}
+
+int vx_case_sync(vx_case *c) {
+    VX_ARG(c, "case");
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_case_enable_timing(vx_case *c, int on) {
+    VX_ARG(c, "case");
+    c->timing = on != 0;
+    return 0;
+}
+
+int vx_case_timings(vx_case *c, vx_timings *t) {
+    VX_ARG(c && t, "null pointer");
+    *t = c->tm;
+    return 0;
+}
+
+// ---- stages ----------------------------------------------------------------
+int vx_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y, double *R_p) {
+    VX_ARG(c && n_src >= 0 && n_src <= c->cap, "n_src beyond capacity");
+    if (launch_right_part(c, n_src, vinf_x, vinf_y)) return 1;
+    if (R_p) VX_CHECK(hipMemcpyAsync(R_p, c->rp.p, sizeof(double) * c->P, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_birth(vx_case *c, const double *R_p, double d_g, double *gsheet) {
+    VX_ARG(c && c->have_matrix, "no panel matrix");
+    if (reserve(c, ceil32(c->n + c->P))) return 1;
+    if (R_p) {
+        if (c->rp.ensure((size_t)c->P, c->stream)) return 1;
+        VX_CHECK(hipMemcpyAsync(c->rp.p, R_p, sizeof(double) * c->P, hipMemcpyHostToDevice, c->stream));
+    }
+    if (launch_birth(c, d_g)) return 1;
+    if (gsheet)
+        VX_CHECK(hipMemcpyAsync(gsheet, c->gsheet.p, sizeof(double) * c->P, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    c->n += c->P;
+    return 0;
+}
+
+int vx_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, double vinf_y) {
+    VX_ARG(c && n_tgt >= 0 && n_tgt <= n_src && n_src <= c->cap, "need n_tgt <= n_src <= capacity");
+    if (launch_biot_savart(c, n_tgt, n_src, vinf_x, vinf_y)) return 1;
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src) {
+    VX_ARG(c && n_tgt >= 0 && n_tgt <= c->cap && n_src <= c->cap, "counts beyond capacity");
+    if (launch_diffusion(c, n_tgt, n_src)) return 1;
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_diffusion_wall(vx_case *c, int64_t n_tgt) {
+    VX_ARG(c && n_tgt >= 0 && n_tgt <= c->cap, "count beyond capacity");
+    if (launch_diffusion_wall(c, n_tgt)) return 1;
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_step_move(vx_case *c, double out4[4]) {
+    VX_ARG(c && out4, "null pointer");
+    return launch_step_move(c, out4);
+}
+
+int vx_compact(vx_case *c) {
+    VX_ARG(c, "case");
+    int64_t n = c->n;
+    if (launch_compact(c, &n)) return 1;
+    c->n = n;
+    return 0;
+}
+
+int vx_setka(vx_case *c, int kind, int32_t *COL) {
+    VX_ARG(c && (kind == 1 || kind == 2), "kind must be 1 or 2");
+    if (launch_setka(c, kind)) return 1;
+    if (COL && c->n) VX_CHECK(hipMemcpyAsync(COL, c->col.p, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_collapse(vx_case *c, int kind, const int32_t *COL) {
+    VX_ARG(c && (kind == 1 || kind == 2), "kind must be 1 or 2");
+    if (COL) {
+        for (int64_t i = 0; i < c->n; ++i)
+            VX_ARG(COL[i] == -1 || (COL[i] > i && COL[i] < c->n), "COL entries must be -1 or in (i, n)");
+        if (c->col.ensure((size_t)(c->n > 0 ? c->n : 1), c->stream)) return 1;
+        if (c->n) VX_CHECK(hipMemcpyAsync(c->col.p, COL, sizeof(int32_t) * c->n, hipMemcpyHostToDevice, c->stream));
+    }
+    if (launch_collapse(c, kind)) return 1;
+    int32_t err = 0;
+    VX_CHECK(hipMemcpyAsync(&err, (int32_t *)(c->dcount.p + 3), sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    if (err) {
+        set_error("collapse resolution did not converge");
+        return 1;
+    }
+    return 0;
+}
+
+// vort_creation (src/unit_1.cu:560-595): right-hand side over the 32-padded
+// source count, then the newborns at [n, n+P).
+int vx_vort_creation(vx_case *c, double vinf_x, double vinf_y) {
+    VX_ARG(c && c->have_matrix, "no panel matrix");
+    Timer t(c, &c->tm.creation_ms, 0);
+    if (reserve(c, ceil32(c->n + c->P))) return 1;
+    if (launch_right_part(c, ceil32(c->n), vinf_x, vinf_y) || launch_birth(c, c->d_g)) return 1;
+    c->n += c->P;
+    return 0;
+}
+
+// Speed (src/unit_1.cu:963-1075): over s = ceil32(n) targets and sources.
+int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
+    VX_ARG(c, "case");
+    const int64_t s = ceil32(c->n);
+    if (reserve(c, s)) return 1;
+    Timer t(c, &c->tm.speed_ms, 1);
+    {
+        Timer tb(c, &c->tm.bs_ms, 2);
+        if (launch_biot_savart(c, s, s, vinf_x, vinf_y)) return 1;
+        c->tm.bs_launches++;
+    }
+    {
+        Timer td(c, &c->tm.diff_ms, 3);
+        if (launch_diffusion(c, s, s)) return 1;
+    }
+    if (launch_diffusion_wall(c, s)) return 1;
+    return 0;
+}
+
+// Step (src/unit_1.cu:1077-1218).
+int vx_step(vx_case *c, double out4[4]) {
+    VX_ARG(c && out4, "null pointer");
+    Timer t(c, &c->tm.step_ms, 4);
+    if (launch_step_move(c, out4)) return 1;
+    int64_t n = c->n;
+    if (launch_compact(c, &n)) return 1;
+    c->n = n;
+    for (int kind = 1; kind <= 2; ++kind) {
+        for (int64_t cc = 0; cc < c->prm.n_col; ++cc) {
+            if (launch_setka(c, kind) || launch_collapse(c, kind)) return 1;
+            int64_t nt = c->n;
+            if (launch_compact(c, &nt)) return 1;
+            if (nt == c->n) break;
+            c->n = nt;
+        }
+    }
+    c->d_g = out4[0];
+    return 0;
+}
+
+int vx_time_step(vx_case *c, double vinf_x, double vinf_y, double forces4[4]) {
+    VX_ARG(c && forces4, "null pointer");
+    if (vx_vort_creation(c, vinf_x, vinf_y) || vx_speed(c, vinf_x, vinf_y) || vx_step(c, forces4)) return 1;
+    return 0;
+}
+
+int vx_run_steps(vx_case *c, int64_t steps, double vinf_x, double vinf_y, double *f4) {
+    VX_ARG(c && steps >= 0, "steps");
+    double tmp[4];
+    for (int64_t k = 0; k < steps; ++k)
+        if (vx_time_step(c, vinf_x, vinf_y, f4 ? f4 + 4 * k : tmp)) return 1;
+    return 0;
+}
+
+// velocity_control (src/unit_1.cu:1254-1269): sources [0, ceil32(n)).
+int vx_velocity_probe(vx_case *c, const double *px, const double *py, int64_t m, double vinf_x, double vinf_y,
+                      double *u, double *v) {
+    VX_ARG(c && px && py && u && v && m >= 0, "null pointer");
+    if (!m) return 0;
+    hipStream_t s = c->stream;
+    if (c->px.ensure((size_t)m, s) || c->py.ensure((size_t)m, s) || c->pu.ensure((size_t)m, s) ||
+        c->pv.ensure((size_t)m, s))
+        return 1;
+    VX_CHECK(hipMemcpyAsync(c->px.p, px, sizeof(double) * m, hipMemcpyHostToDevice, s));
+    VX_CHECK(hipMemcpyAsync(c->py.p, py, sizeof(double) * m, hipMemcpyHostToDevice, s));
+    if (launch_probe(c, m, std::min(ceil32(c->n), c->cap), vinf_x, vinf_y)) return 1;
+    VX_CHECK(hipMemcpyAsync(u, c->pu.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipMemcpyAsync(v, c->pv.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// Internal declarations shared by the HIP translation units of libvortex_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "vortex.h"
+
+namespace vx {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kDelt = 1e-12;        // include/definitions.h:36
+constexpr int kBlock = 256;
+
+void set_error(const std::string &msg);
+
+#define VX_CHECK(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ::vx::set_error(std::string(hipGetErrorString(e_)) + " (" __FILE__ ":" +     \
+                            std::to_string(__LINE__) + ")");                             \
+            return 1;                                                                    \
+        }                                                                                \
+    } while (0)
+
+#define VX_ARG(cond, msg)                                                                \
+    do {                                                                                 \
+        if (!(cond)) {                                                                   \
+            ::vx::set_error(std::string("bad argument: ") + (msg));                      \
+            return 2;                                                                    \
+        }                                                                                \
+    } while (0)
+
+// Panels on the device as structure-of-arrays (one allocation).
+struct PanelsSoA {
+    double *lx, *ly, *rx, *ry, *cx, *cy, *bx, *by, *nx, *ny, *tx, *ty, *len;
+    int32_t *rp;
+    int64_t P;
+};
+
+// Growable device buffer.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t want, hipStream_t s, bool keep = false) {
+        if (want <= n) return 0;
+        size_t cap = want < 1024 ? 1024 : want;
+        T *q = nullptr;
+        hipError_t e = hipMalloc((void **)&q, cap * sizeof(T));
+        if (e != hipSuccess) {
+            set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+            return 1;
+        }
+        if (keep && p && n) {
+            e = hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                set_error(std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+                (void)hipFree(q);
+                return 1;
+            }
+        }
+        if (p) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+        }
+        p = q;
+        n = cap;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace vx
+
+struct vx_case {
+    vx_params prm;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    vx::PanelsSoA pan{};
+    void *pan_block = nullptr;
+    int64_t P = 0;
+    double4 wall_box{};             // birth-point box grown by 5 max(length)
+    // panel system
+    vx::DBuf<double> L, M;          // (P+1)^2
+    bool have_matrix = false;
+    // vortex state, FP64 SoA
+    int64_t n = 0, cap = 0;
+    double d_g = 0.0;               // circulation absorbed by the last Step (host copy)
+    vx::DBuf<double> x, y, g, u, v, d;
+    // FP32 copies of the sources for the mixed precision pairwise stages
+    vx::DBuf<float> xf, yf, gf;
+    // scratch
+    vx::DBuf<double> rp, gsheet, partial, red;       // RHS, sheet, split-K and reductions
+    vx::DBuf<double> tx, ty, tg;                      // compaction/collapse staging
+    vx::DBuf<int32_t> flags, scan, lists, col, misc;
+    vx::DBuf<uint64_t> keys, keys_alt;
+    vx::DBuf<int32_t> vals, vals_alt;
+    vx::DBuf<unsigned char> cub_tmp;
+    vx::DBuf<int64_t> dcount;                         // device counters
+    int64_t *h_count = nullptr;                       // pinned read-back slot
+    double *h_red = nullptr;                          // pinned reduction read-back
+    // probes
+    vx::DBuf<double> px, py, pu, pv;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[10] = {};
+    vx_timings tm{};
+};
+
+namespace vx {
+
+// ---- launchers (kernels in the .hip units) --------------------------------
+// vx_state.hip
+int launch_fill_tail(vx_case *c, int64_t from, int64_t to);
+int launch_pack_f32(vx_case *c, int64_t count);
+// vx_pairwise.hip
+int launch_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, double vinf_y);
+int launch_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src);
+int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y);  // -> c->rp
+int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vinf_y); // c->px.. -> c->pu,pv
+// vx_wall.hip
+int launch_diffusion_wall(vx_case *c, int64_t n_tgt);
+// vx_panel.hip
+int launch_panel_matrix(vx_case *c);                   // -> c->L
+int launch_invert(vx_case *c);                         // c->L -> c->M (L destroyed copy)
+int launch_birth(vx_case *c, double d_g);              // c->rp -> c->gsheet, newborns at [n, n+P)
+// vx_step.hip
+int launch_step_move(vx_case *c, double out4[4]);      // synchronous read-back of the sums
+int launch_compact(vx_case *c, int64_t *n_inout);      // synchronous read-back of n
+int launch_setka(vx_case *c, int kind);                // -> c->col
+int launch_collapse(vx_case *c, int kind);             // uses c->col
+
+}  // namespace vx

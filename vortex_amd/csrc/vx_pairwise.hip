@@ -1,0 +1,453 @@
+// Pairwise O(N^2) stages: Biot-Savart velocity with the nearest-3 distance,
+// vortex-vortex diffusive velocity, the birth right-hand side and the probe
+// velocities.  Each is templated on the arithmetic type:
+//   double  -- the reference formulas in FP64 (VX_PREC_F64);
+//   float   -- FP32 pair arithmetic, FP32 partial sums over one 256-source
+//              LDS tile, FP64 accumulation of the tile sums (VX_PREC_MIXED).
+// Sources are staged through LDS in 256-wide tiles (one wave64-aligned load
+// per thread, broadcast reads in the inner loop); each thread owns TPT
+// targets spaced one block apart so that target loads and stores coalesce.
+#include <cfloat>
+
+#include "vx_internal.h"
+
+namespace vx {
+
+template <typename T>
+struct Src4 {
+    T x, y, g, pad;
+};
+
+template <typename T>
+struct Arith;
+
+template <>
+struct Arith<double> {
+    static constexpr double kFar = 1e30;  // padding sources: squared distance stays finite
+    __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+    __device__ static __forceinline__ double med3(double a, double b, double c) {
+        return fmax(fmin(a, b), fmin(fmax(a, b), c));
+    }
+    __device__ static __forceinline__ double min_(double a, double b) { return fmin(a, b); }
+    __device__ static __forceinline__ double max_(double a, double b) { return fmax(a, b); }
+};
+
+template <>
+struct Arith<float> {
+    static constexpr float kFar = 1e18f;
+    __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static __forceinline__ float med3(float a, float b, float c) {
+        return __builtin_amdgcn_fmed3f(a, b, c);
+    }
+    __device__ static __forceinline__ float min_(float a, float b) { return fminf(a, b); }
+    __device__ static __forceinline__ float max_(float a, float b) { return fmaxf(a, b); }
+};
+
+constexpr int kTile = 256;
+
+template <typename T>
+__device__ __forceinline__ void stage_tile(Src4<T> *ls, const T *__restrict__ sx, const T *__restrict__ sy,
+                                           const T *__restrict__ sg, int64_t base, int64_t n_src) {
+    const int64_t j = base + threadIdx.x;
+    Src4<T> s;
+    if (j < n_src) {
+        s.x = sx[j];
+        s.y = sy[j];
+        s.g = sg[j];
+    } else {
+        s.x = Arith<T>::kFar;
+        s.y = Arith<T>::kFar;
+        s.g = T(0);
+    }
+    s.pad = T(0);
+    ls[threadIdx.x] = s;
+}
+
+// ---- A6: src/kernel.cu:875-983 -------------------------------------------
+// u_i = V_inf + (1/2pi) sum_j g_j (-(y_i-y_j), x_i-x_j) / max(r_ij^2, eps^2), j != i
+// d_i = mean of sqrt of the three smallest r_ij^2 over j != i (sentinel 1e5).
+// The self pair contributes exactly 0 to the sums (dx = dy = 0); for d the
+// four smallest r^2 including the self 0 are kept and the smallest one is
+// dropped, which equals the three smallest over j != i.
+template <typename T, int TPT>
+__global__ __launch_bounds__(kTile) void k_biot_savart(
+    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, int64_t n_src,
+    int64_t n_tgt, T eps2, double vinf_x, double vinf_y, double *__restrict__ u,
+    double *__restrict__ v, double *__restrict__ d) {
+    __shared__ Src4<T> ls[kTile];
+    using A = Arith<T>;
+    const int64_t t0 = (int64_t)blockIdx.x * (kTile * TPT) + threadIdx.x;
+    T ax[TPT], ay[TPT], m0[TPT], m1[TPT], m2[TPT], m3[TPT], su[TPT], sv[TPT];
+    double au[TPT], av[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int64_t i = t0 + (int64_t)k * kTile;
+        ax[k] = i < n_tgt ? sx[i] : T(0);
+        ay[k] = i < n_tgt ? sy[i] : T(0);
+        m0[k] = m1[k] = m2[k] = m3[k] = T(1e5);
+        su[k] = sv[k] = T(0);
+        au[k] = av[k] = 0.0;
+    }
+    for (int64_t base = 0; base < n_src; base += kTile) {
+        __syncthreads();
+        stage_tile<T>(ls, sx, sy, sg, base, n_src);
+        __syncthreads();
+#pragma unroll 4
+        for (int jj = 0; jj < kTile; ++jj) {
+            const Src4<T> s = ls[jj];
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) {
+                const T dx = ax[k] - s.x;
+                const T dy = ay[k] - s.y;
+                const T r2 = dx * dx + dy * dy;
+                m3[k] = A::med3(m2[k], m3[k], r2);
+                m2[k] = A::med3(m1[k], m2[k], r2);
+                m1[k] = A::med3(m0[k], m1[k], r2);
+                m0[k] = A::min_(m0[k], r2);
+                const T w = A::div(s.g, A::max_(r2, eps2));
+                sv[k] += w * dx;
+                su[k] -= w * dy;
+            }
+        }
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) {
+                au[k] += (double)su[k];
+                av[k] += (double)sv[k];
+                su[k] = sv[k] = T(0);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int64_t i = t0 + (int64_t)k * kTile;
+        if (i >= n_tgt) continue;
+        if constexpr (sizeof(T) == 8) {
+            au[k] = (double)su[k];
+            av[k] = (double)sv[k];
+        }
+        u[i] = au[k] / (2 * kPi) + vinf_x;
+        v[i] = av[k] / (2 * kPi) + vinf_y;
+        if (d) d[i] = (sqrt((double)m1[k]) + sqrt((double)m2[k]) + sqrt((double)m3[k])) / 3;
+    }
+}
+
+// ---- A7: src/kernel.cu:985-1029 -------------------------------------------
+// Over j with r_ij > 0.001 ve_size: I2 += -g_j (r_i - r_j)/r e^{-r/d_i},
+// I1 += g_j e^{-r/d_i}; if |I1| > 1e-5: V_i += -nu I2 / (I1 d_i).
+template <typename T, int TPT>
+__global__ __launch_bounds__(kTile) void k_diffusion(
+    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, int64_t n_src,
+    int64_t n_tgt, const double *__restrict__ dd, double nu, double thr,
+    double *__restrict__ u, double *__restrict__ v) {
+    __shared__ Src4<T> ls[kTile];
+    const int64_t t0 = (int64_t)blockIdx.x * (kTile * TPT) + threadIdx.x;
+    T ax[TPT], ay[TPT], kd[TPT], s2x[TPT], s2y[TPT], s1[TPT];
+    double a2x[TPT], a2y[TPT], a1[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int64_t i = t0 + (int64_t)k * kTile;
+        ax[k] = i < n_tgt ? sx[i] : T(0);
+        ay[k] = i < n_tgt ? sy[i] : T(0);
+        const double di = i < n_tgt ? dd[i] : 1.0;
+        if constexpr (sizeof(T) == 4)
+            kd[k] = (T)(-1.4426950408889634 / di);  // exp(-r/d) = exp2(r * kd)
+        else
+            kd[k] = (T)(1.0 / di);
+        s2x[k] = s2y[k] = s1[k] = T(0);
+        a2x[k] = a2y[k] = a1[k] = 0.0;
+    }
+    const T thr2 = (T)(thr * thr);
+    for (int64_t base = 0; base < n_src; base += kTile) {
+        __syncthreads();
+        stage_tile<T>(ls, sx, sy, sg, base, n_src);
+        __syncthreads();
+#pragma unroll 4
+        for (int jj = 0; jj < kTile; ++jj) {
+            const Src4<T> s = ls[jj];
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) {
+                const T dx = ax[k] - s.x;
+                const T dy = ay[k] - s.y;
+                const T r2 = dx * dx + dy * dy;
+                if constexpr (sizeof(T) == 4) {
+                    const T r2c = r2 > thr2 ? r2 : T(1e30f);
+                    const T rs = __builtin_amdgcn_rsqf(r2c);
+                    const T dist = r2c * rs;
+                    const T e = __builtin_amdgcn_exp2f(dist * kd[k]);
+                    const T ge = s.g * e;
+                    const T w = ge * rs;
+                    s2x[k] += w * dx;
+                    s2y[k] += w * dy;
+                    s1[k] += ge;
+                } else {
+                    const T dist = sqrt(r2);
+                    if (dist > (T)thr) {
+                        const T w = s.g / dist;
+                        const T e = exp(-dist * kd[k]);
+                        s2x[k] += w * dx * e;
+                        s2y[k] += w * dy * e;
+                        s1[k] += s.g * e;
+                    }
+                }
+            }
+        }
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) {
+                a2x[k] += (double)s2x[k];
+                a2y[k] += (double)s2y[k];
+                a1[k] += (double)s1[k];
+                s2x[k] = s2y[k] = s1[k] = T(0);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int64_t i = t0 + (int64_t)k * kTile;
+        if (i >= n_tgt) continue;
+        if constexpr (sizeof(T) == 8) {
+            a2x[k] = s2x[k];
+            a2y[k] = s2y[k];
+            a1[k] = s1[k];
+        }
+        // I2 = -sum w (r_i - r_j) e, so -nu I2/(I1 d) = nu a2/(I1 d)
+        if (fabs(a1[k]) > 1e-5) {
+            const double di = dd[i];
+            u[i] += nu * a2x[k] / (a1[k] * di);
+            v[i] += nu * a2y[k] / (a1[k] * di);
+        }
+    }
+}
+
+// ---- A3: src/kernel.cu:779-847, split over source chunks -----------------
+// For panel i with end points l, r, d = r - l:
+//   y += g_j (alpha d + beta k x d), alpha = atan(s0.d/z) - atan(s1.d/z),
+//   beta = 0.5 ln(|s1|^2/|s0|^2), s0 = l - r_j, s1 = r - r_j, z = d x s0.
+// In FP32 alpha is evaluated as one atan2(-|d|^2 z, z^2 + (s0.d)(s1.d)),
+// the same angle difference.
+template <typename T>
+__global__ __launch_bounds__(kTile) void k_right_part_partial(
+    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, int64_t n_src,
+    int64_t chunk, PanelsSoA pan, double *__restrict__ part) {
+    __shared__ Src4<T> ls[kTile];
+    const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    const int64_t lo = c * chunk, hi = min(n_src, lo + chunk);
+    const bool live = i < pan.P;
+    const T l0 = live ? (T)pan.lx[i] : T(0), l1 = live ? (T)pan.ly[i] : T(0);
+    const T r0 = live ? (T)pan.rx[i] : T(1), r1 = live ? (T)pan.ry[i] : T(0);
+    const T d0 = r0 - l0, d1 = r1 - l1;
+    const T dd2 = d0 * d0 + d1 * d1;
+    T y0 = 0, y1 = 0;
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t base = lo; base < hi; base += kTile) {
+        __syncthreads();
+        stage_tile<T>(ls, sx, sy, sg, base, hi);
+        __syncthreads();
+        const int cnt = (int)min((int64_t)kTile, hi - base);
+        for (int jj = 0; jj < cnt; ++jj) {
+            const Src4<T> s = ls[jj];
+            const T s00 = l0 - s.x, s01 = l1 - s.y;
+            const T s10 = r0 - s.x, s11 = r1 - s.y;
+            const T z = d0 * s01 - d1 * s00;
+            const T A = s00 * d0 + s01 * d1;
+            const T B = s10 * d0 + s11 * d1;
+            T alpha, beta;
+            if constexpr (sizeof(T) == 4) {
+                alpha = atan2f(-dd2 * z, z * z + A * B);
+                const T q0 = s00 * s00 + s01 * s01, q1 = s10 * s10 + s11 * s11;
+                beta = 0.34657359027997264f * __builtin_amdgcn_logf(q1 * __builtin_amdgcn_rcpf(q0));
+            } else {
+                alpha = atan(A / z) - atan(B / z);
+                beta = 0.5 * log((s10 * s10 + s11 * s11) / (s00 * s00 + s01 * s01));
+            }
+            y0 += s.g * (alpha * d0 - beta * d1);
+            y1 += s.g * (alpha * d1 + beta * d0);
+        }
+        if constexpr (sizeof(T) == 4) {
+            a0 += (double)y0;
+            a1 += (double)y1;
+            y0 = y1 = T(0);
+        }
+    }
+    if constexpr (sizeof(T) == 8) {
+        a0 = y0;
+        a1 = y1;
+    }
+    if (live) {
+        part[(c * 2 + 0) * pan.P + i] = a0;
+        part[(c * 2 + 1) * pan.P + i] = a1;
+    }
+}
+
+__global__ void k_right_part_final(const double *__restrict__ part, int64_t nchunk, PanelsSoA pan,
+                                   double vinf_x, double vinf_y, double *__restrict__ R) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pan.P) return;
+    double y0 = 0.0, y1 = 0.0;
+    for (int64_t c = 0; c < nchunk; ++c) {
+        y0 += part[(c * 2 + 0) * pan.P + i];
+        y1 += part[(c * 2 + 1) * pan.P + i];
+    }
+    const double d0 = pan.rx[i] - pan.lx[i], d1 = pan.ry[i] - pan.ly[i];
+    const double mnog = 1 / (2 * kPi * (d0 * d0 + d1 * d1));
+    R[i] = -((mnog * y0 + vinf_x) * pan.tx[i] + (mnog * y1 + vinf_y) * pan.ty[i]);
+}
+
+// ---- A19: src/kernel.cu:1434-1475, split over source chunks ---------------
+template <typename T>
+__global__ __launch_bounds__(kTile) void k_probe_partial(
+    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, int64_t n_src,
+    int64_t chunk, const double *__restrict__ px, const double *__restrict__ py, int64_t m, T eps2,
+    double *__restrict__ part) {
+    __shared__ Src4<T> ls[kTile];
+    using A = Arith<T>;
+    const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    const int64_t lo = c * chunk, hi = min(n_src, lo + chunk);
+    const T a0 = i < m ? (T)px[i] : T(0), a1 = i < m ? (T)py[i] : T(0);
+    T su = 0, sv = 0;
+    double au = 0.0, av = 0.0;
+    for (int64_t base = lo; base < hi; base += kTile) {
+        __syncthreads();
+        stage_tile<T>(ls, sx, sy, sg, base, hi);
+        __syncthreads();
+        const int cnt = (int)min((int64_t)kTile, hi - base);
+        for (int jj = 0; jj < cnt; ++jj) {
+            const Src4<T> s = ls[jj];
+            const T dx = a0 - s.x, dy = a1 - s.y;
+            const T w = A::div(s.g, A::max_(dx * dx + dy * dy, eps2));
+            sv += w * dx;
+            su -= w * dy;
+        }
+        if constexpr (sizeof(T) == 4) {
+            au += (double)su;
+            av += (double)sv;
+            su = sv = T(0);
+        }
+    }
+    if constexpr (sizeof(T) == 8) {
+        au = su;
+        av = sv;
+    }
+    if (i < m) {
+        part[(c * 2 + 0) * m + i] = au;
+        part[(c * 2 + 1) * m + i] = av;
+    }
+}
+
+__global__ void k_probe_final(const double *__restrict__ part, int64_t nchunk, int64_t m,
+                              double vinf_x, double vinf_y, double *__restrict__ u,
+                              double *__restrict__ v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double a = 0.0, b = 0.0;
+    for (int64_t c = 0; c < nchunk; ++c) {
+        a += part[(c * 2 + 0) * m + i];
+        b += part[(c * 2 + 1) * m + i];
+    }
+    u[i] = a / (2 * kPi) + vinf_x;
+    v[i] = b / (2 * kPi) + vinf_y;
+}
+
+// ---------------------------------------------------------------------------
+static bool mixed(const vx_case *c) { return c->prm.precision == VX_PREC_MIXED; }
+
+// Split [0, n_src) into chunks of whole tiles so that tgt_blocks * chunks
+// gives a few thousand work-groups.
+static void split(int64_t n_src, int64_t tgt_blocks, int64_t *chunk, int64_t *nchunk) {
+    const int64_t tiles = (n_src + kTile - 1) / kTile;
+    int64_t want = (4096 + tgt_blocks - 1) / tgt_blocks;
+    if (want < 1) want = 1;
+    if (want > tiles) want = tiles > 0 ? tiles : 1;
+    const int64_t tiles_per = (tiles + want - 1) / want;
+    *chunk = (tiles_per > 0 ? tiles_per : 1) * kTile;
+    *nchunk = n_src > 0 ? (n_src + *chunk - 1) / *chunk : 1;
+}
+
+int launch_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, double vinf_y) {
+    if (n_tgt <= 0) return 0;
+    const double eps2 = c->prm.ve_size * c->prm.ve_size;
+    if (mixed(c)) {
+        constexpr int TPT = 4;
+        if (launch_pack_f32(c, n_src)) return 1;
+        const unsigned nb = (unsigned)((n_tgt + kTile * TPT - 1) / (kTile * TPT));
+        hipLaunchKernelGGL((k_biot_savart<float, TPT>), dim3(nb), dim3(kTile), 0, c->stream, c->xf.p,
+                           c->yf.p, c->gf.p, n_src, n_tgt, (float)eps2, vinf_x, vinf_y, c->u.p, c->v.p,
+                           c->d.p);
+    } else {
+        constexpr int TPT = 2;
+        const unsigned nb = (unsigned)((n_tgt + kTile * TPT - 1) / (kTile * TPT));
+        hipLaunchKernelGGL((k_biot_savart<double, TPT>), dim3(nb), dim3(kTile), 0, c->stream, c->x.p,
+                           c->y.p, c->g.p, n_src, n_tgt, eps2, vinf_x, vinf_y, c->u.p, c->v.p, c->d.p);
+    }
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src) {
+    if (n_tgt <= 0) return 0;
+    const double thr = 0.001 * c->prm.ve_size;
+    if (mixed(c)) {
+        constexpr int TPT = 4;
+        if (launch_pack_f32(c, n_src)) return 1;
+        const unsigned nb = (unsigned)((n_tgt + kTile * TPT - 1) / (kTile * TPT));
+        hipLaunchKernelGGL((k_diffusion<float, TPT>), dim3(nb), dim3(kTile), 0, c->stream, c->xf.p,
+                           c->yf.p, c->gf.p, n_src, n_tgt, c->d.p, c->prm.nu, thr, c->u.p, c->v.p);
+    } else {
+        constexpr int TPT = 2;
+        const unsigned nb = (unsigned)((n_tgt + kTile * TPT - 1) / (kTile * TPT));
+        hipLaunchKernelGGL((k_diffusion<double, TPT>), dim3(nb), dim3(kTile), 0, c->stream, c->x.p,
+                           c->y.p, c->g.p, n_src, n_tgt, c->d.p, c->prm.nu, thr, c->u.p, c->v.p);
+    }
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
+    const int64_t P = c->P;
+    const int64_t tb = (P + kTile - 1) / kTile;
+    int64_t chunk, nchunk;
+    split(n_src, tb, &chunk, &nchunk);
+    if (c->partial.ensure((size_t)(2 * nchunk * P), c->stream) || c->rp.ensure((size_t)P, c->stream)) return 1;
+    if (mixed(c)) {
+        if (launch_pack_f32(c, n_src)) return 1;
+        hipLaunchKernelGGL(k_right_part_partial<float>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0,
+                           c->stream, c->xf.p, c->yf.p, c->gf.p, n_src, chunk, c->pan, c->partial.p);
+    } else {
+        hipLaunchKernelGGL(k_right_part_partial<double>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0,
+                           c->stream, c->x.p, c->y.p, c->g.p, n_src, chunk, c->pan, c->partial.p);
+    }
+    VX_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)tb), dim3(kTile), 0, c->stream, c->partial.p, nchunk,
+                       c->pan, vinf_x, vinf_y, c->rp.p);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vinf_y) {
+    if (m <= 0) return 0;
+    const int64_t tb = (m + kTile - 1) / kTile;
+    int64_t chunk, nchunk;
+    split(n_src, tb, &chunk, &nchunk);
+    if (c->partial.ensure((size_t)(2 * nchunk * m), c->stream)) return 1;
+    const double eps2 = c->prm.ve_size * c->prm.ve_size;
+    if (mixed(c)) {
+        if (launch_pack_f32(c, n_src)) return 1;
+        hipLaunchKernelGGL(k_probe_partial<float>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0,
+                           c->stream, c->xf.p, c->yf.p, c->gf.p, n_src, chunk, c->px.p, c->py.p, m,
+                           (float)eps2, c->partial.p);
+    } else {
+        hipLaunchKernelGGL(k_probe_partial<double>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0,
+                           c->stream, c->x.p, c->y.p, c->g.p, n_src, chunk, c->px.p, c->py.p, m, eps2,
+                           c->partial.p);
+    }
+    VX_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_probe_final, dim3((unsigned)tb), dim3(kTile), 0, c->stream, c->partial.p, nchunk, m,
+                       vinf_x, vinf_y, c->pu.p, c->pv.p);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace vx

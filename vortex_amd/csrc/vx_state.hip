@@ -1,0 +1,59 @@
+// Capacity tail and precision packing of the vortex state.
+#include "vx_internal.h"
+
+namespace vx {
+
+// Unused capacity slots hold g = 0 far away, as zero_Kernel does
+// (src/kernel.cu:767-777: r = 2e5 + 2e5 * rand()).  The factor is a 31-bit
+// splitmix64 hash of the slot index so that the tail is reproducible.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+constexpr uint64_t kTailSeed = 12345;
+
+__global__ void k_fill_tail(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
+                            int64_t from, int64_t to) {
+    const int64_t i = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= to) return;
+    const double h = (double)(splitmix64((uint64_t)i ^ kTailSeed) >> 33);
+    const double r = 2.0e5 * h + 2.0e5;
+    x[i] = r;
+    y[i] = r;
+    g[i] = 0.0;
+}
+
+int launch_fill_tail(vx_case *c, int64_t from, int64_t to) {
+    if (to <= from) return 0;
+    const int64_t cnt = to - from;
+    hipLaunchKernelGGL(k_fill_tail, dim3((unsigned)((cnt + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, c->x.p, c->y.p, c->g.p, from, to);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_pack_f32(const double *__restrict__ x, const double *__restrict__ y,
+                           const double *__restrict__ g, float *__restrict__ xf,
+                           float *__restrict__ yf, float *__restrict__ gf, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    xf[i] = (float)x[i];
+    yf[i] = (float)y[i];
+    gf[i] = (float)g[i];
+}
+
+int launch_pack_f32(vx_case *c, int64_t count) {
+    if (count <= 0) return 0;
+    if (c->xf.ensure((size_t)c->cap, c->stream) || c->yf.ensure((size_t)c->cap, c->stream) ||
+        c->gf.ensure((size_t)c->cap, c->stream))
+        return 1;
+    hipLaunchKernelGGL(k_pack_f32, dim3((unsigned)((count + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, c->x.p, c->y.p, c->g.p, c->xf.p, c->yf.p, c->gf.p, count);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace vx

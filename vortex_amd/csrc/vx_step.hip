@@ -1,0 +1,536 @@
+// Step stages: Euler move with absorption and deletion (A9), force and
+// circulation sums (A10), compaction (A11) and collapse (A12/A13).
+//
+// These stages decide integers -- which vortex is absorbed, deleted, kept,
+// merged and where each ends up -- so every multiply and add here is
+// separately rounded and every decision uses the reference's expression.
+// Sums use a fixed two-level order (per work-group, then over work-groups),
+// so they are run-to-run deterministic.
+#pragma clang fp contract(off)
+
+#include <hipcub/hipcub.hpp>
+
+#include "vx_internal.h"
+
+namespace vx {
+
+__device__ __forceinline__ double dist2(double a0, double a1, double b0, double b1) {
+    return (a0 - b0) * (a0 - b0) + (a1 - b1) * (a1 - b1);
+}
+
+struct StepConsts {
+    double dt, rc_x, rc_y, x_min, x_max, y_min, y_max, px_min, px_max, py_min, py_max;
+};
+
+// hitting(): src/kernel.cu:1357-1431.  Segment old -> new against every
+// panel (after the profile-box reject); the crossing nearest the old position
+// wins.  A crossing set with no panel nearer than 5 (the reference would read
+// panels[-1]) counts as no crossing.
+__device__ int hitting(const PanelsSoA &pan, const StepConsts &k, double x1, double y1, double x2, double y2) {
+    const double porog = 1e-12;
+    if (((x1 < k.px_min) && (x2 < k.px_min)) || ((x1 > k.px_max) && (x2 > k.px_max)) ||
+        ((y1 < k.py_min) && (y2 < k.py_min)) || ((y1 > k.py_max) && (y2 > k.py_max)))
+        return -1;
+    const double A = y2 - y1, B = x1 - x2, D = y1 * x2 - x1 * y2;
+    double minDist = 25.0;
+    int minN = -1;
+    for (int64_t i = 0; i < pan.P; ++i) {
+        const double lx = pan.lx[i], ly = pan.ly[i], rx = pan.rx[i], ry = pan.ry[i];
+        double r0 = A * lx + B * ly + D;
+        double r1 = A * rx + B * ry + D;
+        if (fabs(r0) < porog) r0 = 0.0;
+        if (fabs(r1) < porog) r1 = 0.0;
+        if (r0 * r1 <= 0) {
+            const double A1 = ry - ly, B1 = lx - rx, D1 = ly * rx - lx * ry;
+            double r2 = A1 * x1 + B1 * y1 + D1;
+            double r3 = A1 * x2 + B1 * y2 + D1;
+            if (fabs(r2) < porog) r2 = 0.0;
+            if (fabs(r3) < porog) r3 = 0.0;
+            if (r2 * r3 <= 0) {
+                const double den = A * B1 - B * A1;
+                const double X = (B * D1 - D * B1) / den;
+                const double Y = (A1 * D - D1 * A) / den;
+                const double dd2 = (x2 - X) * (x2 - X) + (y2 - Y) * (y2 - Y);
+                if (dd2 < minDist) {
+                    minDist = dd2;
+                    minN = (int)i;
+                }
+            }
+        }
+    }
+    return minN;
+}
+
+// ---- A9: src/kernel.cu:1090-1143 -------------------------------------------
+__global__ __launch_bounds__(256) void k_step_move(double *__restrict__ x, double *__restrict__ y,
+                                                   double *__restrict__ g, const double *__restrict__ u,
+                                                   const double *__restrict__ v, int64_t n, int64_t P,
+                                                   PanelsSoA pan, StepConsts k, double *__restrict__ part) {
+    __shared__ double red[4][256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double dg = 0.0, fx = 0.0, fy = 0.0, mm = 0.0;
+    if (i < n) {
+        double xi = x[i], yi = y[i], gi = g[i];
+        if (n >= P && i >= n - P) {
+            fx = gi * (-yi);
+            fy = gi * (xi);
+            mm = gi * dist2(xi, yi, k.rc_x, k.rc_y);
+        }
+        const double xn = xi + u[i] * k.dt;
+        const double yn = yi + v[i] * k.dt;
+        if (gi != 0) {
+            const int hp = hitting(pan, k, xn, yn, xi, yi);
+            if (hp >= 0) {
+                const double cx = pan.cx[hp], cy = pan.cy[hp];
+                fx -= gi * (-cy);
+                fy -= gi * (cx);
+                mm -= gi * dist2(cx, cy, k.rc_x, k.rc_y);
+                dg = gi;
+                gi = 0;
+            }
+        }
+        xi = xn;
+        yi = yn;
+        if ((gi != 0) && ((xi > k.x_max) || (xi < k.x_min) || (yi > k.y_max) || (yi < k.y_min))) {
+            xi = -2.0e+5;
+            yi = -2.0e+5;
+            gi = 0;
+        }
+        x[i] = xi;
+        y[i] = yi;
+        g[i] = gi;
+    }
+    red[0][threadIdx.x] = dg;
+    red[1][threadIdx.x] = fx;
+    red[2][threadIdx.x] = fy;
+    red[3][threadIdx.x] = mm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[(int64_t)blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// A10: src/kernel.cu:1144-1157 -- d_g = sum, F = rho/dt sum, M = rho/(2 dt) sum.
+__global__ __launch_bounds__(256) void k_sum4(const double *__restrict__ part, int64_t nb, double rho, double dt,
+                                              double *__restrict__ out) {
+    __shared__ double red[4][256];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t b = threadIdx.x; b < nb; b += 256)
+        for (int q = 0; q < 4; ++q) a[q] += part[b * 4 + q];
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0] * (rho / dt);
+        out[2] = red[2][0] * (rho / dt);
+        out[3] = red[3][0] * (rho / (2 * dt));
+    }
+}
+
+int launch_step_move(vx_case *c, double out4[4]) {
+    const int64_t n = c->n;
+    const int64_t nb = n > 0 ? (n + 255) / 256 : 1;
+    if (c->partial.ensure((size_t)(4 * nb), c->stream) || c->red.ensure(8, c->stream)) return 1;
+    const vx_params &p = c->prm;
+    StepConsts k{p.dt, p.rc_x, p.rc_y, p.x_min, p.x_max, p.y_min, p.y_max, p.px_min, p.px_max, p.py_min, p.py_max};
+    hipLaunchKernelGGL(k_step_move, dim3((unsigned)nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->g.p, c->u.p,
+                       c->v.p, n, c->P, c->pan, k, c->partial.p);
+    hipLaunchKernelGGL(k_sum4, dim3(1), dim3(256), 0, c->stream, c->partial.p, nb, p.rho, p.dt, c->red.p);
+    VX_CHECK(hipGetLastError());
+    VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 4; ++q) out4[q] = c->h_red[q];
+    return 0;
+}
+
+// ---- A11: src/kernel.cu:1159-1178 ------------------------------------------
+// Swap-with-last removal of |g| < 1e-12 in closed form.  With n1 live
+// entries, the j-th hole of [0, n1) (ascending) takes the j-th live entry of
+// [n1, n) (descending).  Every slot e of [n1, n) ends dead (g = 0) holding
+// the position of src(e): e+1 when e+1 < n is dead, else the start of e's
+// chain -- hole h_j where j-1 live entries of [n1, n) lie above e, or n1
+// itself below the last live entry.  Gather to staging, then scatter.
+__global__ void k_live_flags(const double *__restrict__ g, int64_t n, int32_t *__restrict__ f) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = fabs(g[i]) < kDelt ? 0 : 1;
+}
+
+// s = exclusive scan of flags; n1 = total live (s[n-1] + f[n-1]).
+__global__ void k_compact_lists(const int32_t *__restrict__ f, const int32_t *__restrict__ s, int64_t n,
+                                int32_t *__restrict__ holes, int32_t *__restrict__ donors) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t n1 = (int64_t)s[n - 1] + f[n - 1];
+    if (i < n1) {
+        if (!f[i]) holes[i - s[i]] = (int32_t)i;  // rank among the holes of [0, n1)
+    } else if (f[i]) {
+        donors[n1 - s[i] - 1] = (int32_t)i;       // rank from the top among live entries of [n1, n)
+    }
+}
+
+__global__ void k_compact_gather(const double *__restrict__ x, const double *__restrict__ y,
+                                 const double *__restrict__ g, const int32_t *__restrict__ f,
+                                 const int32_t *__restrict__ s, int64_t n, const int32_t *__restrict__ holes,
+                                 const int32_t *__restrict__ donors, double *__restrict__ tx,
+                                 double *__restrict__ ty, double *__restrict__ tg) {
+    const int64_t n1 = (int64_t)s[n - 1] + f[n - 1];
+    const int64_t K = n1 > 0 ? (int64_t)(n1 - (n1 < n ? s[n1] : n1)) : 0;  // holes in [0, n1)
+    const int64_t T = n - n1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < K) {
+        const int64_t src = donors[t];
+        tx[t] = x[src];
+        ty[t] = y[src];
+        tg[t] = g[src];
+    } else if (t < K + T) {
+        const int64_t e = n1 + (t - K);
+        const int64_t above = n1 - ((int64_t)s[e] + f[e]);  // live entries of [n1, n) above e
+        const int64_t start = above < K ? (int64_t)holes[above] : n1;
+        const int64_t src = (e + 1 < n && !f[e + 1]) ? e + 1 : start;
+        tx[t] = x[src];
+        ty[t] = y[src];
+        tg[t] = 0.0;
+    }
+}
+
+__global__ void k_compact_scatter(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
+                                  const int32_t *__restrict__ f, const int32_t *__restrict__ s, int64_t n,
+                                  const int32_t *__restrict__ holes, const double *__restrict__ tx,
+                                  const double *__restrict__ ty, const double *__restrict__ tg) {
+    const int64_t n1 = (int64_t)s[n - 1] + f[n - 1];
+    const int64_t K = n1 > 0 ? (int64_t)(n1 - (n1 < n ? s[n1] : n1)) : 0;
+    const int64_t T = n - n1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t dst;
+    if (t < K)
+        dst = holes[t];
+    else if (t < K + T)
+        dst = n1 + (t - K);
+    else
+        return;
+    x[dst] = tx[t];
+    y[dst] = ty[t];
+    g[dst] = tg[t];
+}
+
+__global__ void k_compact_count(const int32_t *__restrict__ f, const int32_t *__restrict__ s, int64_t n,
+                                int64_t *__restrict__ out) {
+    out[0] = (int64_t)s[n - 1] + f[n - 1];
+}
+
+int launch_compact(vx_case *c, int64_t *n_inout) {
+    const int64_t n = *n_inout;
+    if (n <= 0) return 0;
+    hipStream_t st = c->stream;
+    if (c->flags.ensure((size_t)n, st) || c->scan.ensure((size_t)n, st) || c->lists.ensure((size_t)(2 * n), st) ||
+        c->tx.ensure((size_t)(2 * n), st) || c->ty.ensure((size_t)(2 * n), st) || c->tg.ensure((size_t)(2 * n), st) ||
+        c->dcount.ensure(4, st))
+        return 1;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_live_flags, dim3(nb), dim3(256), 0, st, c->g.p, n, c->flags.p);
+    size_t bytes = 0;
+    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->flags.p, c->scan.p, (int)n, st));
+    if (c->cub_tmp.ensure(bytes, st)) return 1;
+    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, c->flags.p, c->scan.p, (int)n, st));
+    int32_t *holes = c->lists.p, *donors = c->lists.p + n;
+    hipLaunchKernelGGL(k_compact_lists, dim3(nb), dim3(256), 0, st, c->flags.p, c->scan.p, n, holes, donors);
+    const unsigned nb2 = (unsigned)((2 * n + 255) / 256);
+    hipLaunchKernelGGL(k_compact_gather, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->flags.p,
+                       c->scan.p, n, holes, donors, c->tx.p, c->ty.p, c->tg.p);
+    hipLaunchKernelGGL(k_compact_scatter, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->flags.p,
+                       c->scan.p, n, holes, c->tx.p, c->ty.p, c->tg.p);
+    hipLaunchKernelGGL(k_compact_count, dim3(1), dim3(1), 0, st, c->flags.p, c->scan.p, n, c->dcount.p);
+    VX_CHECK(hipGetLastError());
+    VX_CHECK(hipMemcpyAsync(c->h_count, c->dcount.p, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    VX_CHECK(hipStreamSynchronize(st));
+    *n_inout = *c->h_count;
+    return 0;
+}
+
+// ---- A12/A13 search: src/kernel.cu:1180-1235 -------------------------------
+// COL_i = smallest j > i with
+//   kind 1: g_i g_j < 0 and r_ij^2 < r2
+//   kind 2: g_i g_j > 0 and r_ij^2 < r2 and |g_i + g_j| < max_ve_g
+// Candidates come from a uniform grid of cell size slightly above sqrt(r2)
+// (so every pair inside the radius lies in neighbouring cells), built by
+// sorting (cell key, index) pairs; the 3x3 cell rows are found by binary
+// search in the sorted keys.  The minimum j does not depend on the order in
+// which candidates are visited.
+struct Grid {
+    double x0, y0, inv_h;
+    int64_t nx, ny;
+};
+
+__device__ __forceinline__ void cell_of(const Grid &G, double x, double y, int64_t &cx, int64_t &cy) {
+    double fx = (x - G.x0) * G.inv_h, fy = (y - G.y0) * G.inv_h;
+    if (!(fx >= 0.0)) fx = 0.0;  // also catches NaN
+    if (!(fy >= 0.0)) fy = 0.0;
+    cx = fx >= (double)(G.nx - 1) ? G.nx - 1 : (int64_t)fx;
+    cy = fy >= (double)(G.ny - 1) ? G.ny - 1 : (int64_t)fy;
+}
+
+__global__ void k_cell_keys(const double *__restrict__ x, const double *__restrict__ y, int64_t n, Grid G,
+                            uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t cx, cy;
+    cell_of(G, x[i], y[i], cx, cy);
+    keys[i] = (uint64_t)cy * (uint64_t)G.nx + (uint64_t)cx;
+    vals[i] = (int32_t)i;
+}
+
+__device__ __forceinline__ int64_t lower_bound_u64(const uint64_t *__restrict__ a, int64_t n, uint64_t key) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_setka(const double *__restrict__ x, const double *__restrict__ y,
+                                               const double *__restrict__ g, int64_t n, Grid G, double r2,
+                                               double max_ve_g, const uint64_t *__restrict__ skeys,
+                                               const int32_t *__restrict__ svals, int32_t *__restrict__ col) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double r0 = x[i], r1 = y[i], gi = g[i];
+    int64_t cx, cy;
+    cell_of(G, r0, r1, cx, cy);
+    int64_t best = INT64_MAX;
+    for (int64_t yy = cy - 1; yy <= cy + 1; ++yy) {
+        if (yy < 0 || yy >= G.ny) continue;
+        const int64_t xa = cx > 0 ? cx - 1 : 0, xb = cx + 1 < G.nx ? cx + 1 : G.nx - 1;
+        const uint64_t ka = (uint64_t)yy * (uint64_t)G.nx + (uint64_t)xa;
+        const uint64_t kb = (uint64_t)yy * (uint64_t)G.nx + (uint64_t)xb;
+        for (int64_t p = lower_bound_u64(skeys, n, ka); p < n && skeys[p] <= kb; ++p) {
+            const int64_t j = svals[p];
+            if (j <= i || j >= best) continue;
+            const double gj = g[j];
+            bool hit;
+            if (KIND == 1)
+                hit = (gi * gj < 0) && (dist2(r0, r1, x[j], y[j]) < r2);
+            else
+                hit = (gi * gj > 0) && (dist2(r0, r1, x[j], y[j]) < r2) && (fabs(gi + gj) < max_ve_g);
+            if (hit) best = j;
+        }
+    }
+    col[i] = best == INT64_MAX ? -1 : (int32_t)best;
+}
+
+int launch_setka(vx_case *c, int kind) {
+    const int64_t n = c->n;
+    hipStream_t st = c->stream;
+    if (c->col.ensure((size_t)(n > 0 ? n : 1), st)) return 1;
+    if (n <= 0) return 0;
+    const double ve2 = c->prm.ve_size * c->prm.ve_size;
+    const double rr = kind == 1 ? c->prm.r_col_diff_sign : c->prm.r_col_same_sign;
+    const double r2 = rr * rr * ve2;  // src/unit_1.cu:1228-1229
+    if (!(r2 > 0.0)) {
+        VX_CHECK(hipMemsetAsync(c->col.p, 0xff, sizeof(int32_t) * n, st));
+        return 0;
+    }
+    const double h = sqrt(r2) * 1.001;
+    Grid G;
+    G.x0 = c->prm.x_min - 2 * h;
+    G.y0 = c->prm.y_min - 2 * h;
+    G.inv_h = 1.0 / h;
+    const double spanx = (c->prm.x_max - G.x0) / h + 3, spany = (c->prm.y_max - G.y0) / h + 3;
+    G.nx = (int64_t)fmin(spanx, 1e9);
+    G.ny = (int64_t)fmin(spany, 1e9);
+    if (G.nx < 1) G.nx = 1;
+    if (G.ny < 1) G.ny = 1;
+    int bits = 1;
+    while (bits < 63 && (((uint64_t)1 << bits) < (uint64_t)G.nx * (uint64_t)G.ny)) ++bits;
+    if (c->keys.ensure((size_t)n, st) || c->keys_alt.ensure((size_t)n, st) || c->vals.ensure((size_t)n, st) ||
+        c->vals_alt.ensure((size_t)n, st))
+        return 1;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, n, G, c->keys.p, c->vals.p);
+    size_t bytes = 0;
+    VX_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->keys.p, c->keys_alt.p, c->vals.p, c->vals_alt.p,
+                                                (int)n, 0, bits, st));
+    if (c->cub_tmp.ensure(bytes, st)) return 1;
+    VX_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, bytes, c->keys.p, c->keys_alt.p, c->vals.p,
+                                                c->vals_alt.p, (int)n, 0, bits, st));
+    if (kind == 1)
+        hipLaunchKernelGGL(k_setka<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, r2,
+                           c->prm.max_ve_g, c->keys_alt.p, c->vals_alt.p, c->col.p);
+    else
+        hipLaunchKernelGGL(k_setka<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, r2,
+                           c->prm.max_ve_g, c->keys_alt.p, c->vals_alt.p, c->col.p);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ---- A12/A13 merge: src/kernel.cu:1237-1271 --------------------------------
+// The sequential sweep (for i ascending: if COL_i >= 0 merge j = COL_i into i
+// and kill j) is resolved without a sweep.  Candidate i is active when no
+// active candidate i' < i has COL_i' = i; this is decided leaf-first over the
+// forest of edges i -> COL_i (COL_i > i) in one work-group.  Among active
+// candidates sharing a target j the smallest merges with j as it was, the
+// others with the killed j (g = 0 at 1e10), exactly as the sweep does.
+__global__ void k_col_prepare(int64_t n, int32_t *__restrict__ pc, int32_t *__restrict__ state,
+                              int32_t *__restrict__ first) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    state[i] = 0;
+    first[i] = INT32_MAX;
+    pc[i] = 0;
+}
+
+__global__ void k_col_count(const int32_t *__restrict__ col, int64_t n, int32_t *__restrict__ pc,
+                            int32_t *__restrict__ cand, unsigned long long *__restrict__ ncand) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t j = col[i];
+    if (j >= 0) {
+        atomicAdd(&pc[j], 1);
+        const unsigned long long slot = atomicAdd(ncand, 1ull);
+        cand[slot] = (int32_t)i;
+    }
+}
+
+// state bits: 1 active, 2 inactive, 4 killed by an active child, 16 published.
+// Every access goes through L2 atomics (no stale per-CU L1 lines).  Each round
+// decides at least the lowest undecided candidate (its children have lower
+// indices and were published in earlier rounds), so ncand + 1 rounds always
+// suffice; the loop bound is a guard, reported through *err.
+__global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
+                                                      const unsigned long long *__restrict__ ncand_p,
+                                                      int32_t *__restrict__ pc, int32_t *__restrict__ state,
+                                                      int32_t *__restrict__ err) {
+    __shared__ int undecided;
+    const int64_t ncand = (int64_t)*ncand_p;
+    int64_t round = 0;
+    for (;; ++round) {
+        if (threadIdx.x == 0) undecided = 0;
+        __syncthreads();
+        for (int64_t t = threadIdx.x; t < ncand; t += blockDim.x) {
+            const int32_t i = cand[t];
+            const int32_t s = atomicOr(&state[i], 0);
+            if (s & 3) continue;
+            if (atomicAdd(&pc[i], 0) == 0)
+                atomicOr(&state[i], (s & 4) ? 2 : 1);
+            else
+                undecided = 1;
+        }
+        __syncthreads();
+        for (int64_t t = threadIdx.x; t < ncand; t += blockDim.x) {
+            const int32_t i = cand[t];
+            const int32_t s = atomicOr(&state[i], 0);
+            if ((s & 3) && !(s & 16)) {
+                const int32_t j = col[i];
+                if (s & 1) atomicOr(&state[j], 4);
+                atomicOr(&state[i], 16);
+                atomicSub(&pc[j], 1);
+            }
+        }
+        __syncthreads();
+        const int more = undecided;
+        __syncthreads();
+        if (!more) break;
+        if (round > ncand + 1) {
+            if (threadIdx.x == 0) *err = 1;
+            break;
+        }
+    }
+}
+
+__global__ void k_col_first(const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
+                            const unsigned long long *__restrict__ ncand_p, const int32_t *__restrict__ state,
+                            int32_t *__restrict__ first) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)*ncand_p) return;
+    const int32_t i = cand[t];
+    if (state[i] & 1) atomicMin(&first[col[i]], i);
+}
+
+template <int KIND>
+__global__ void k_col_merge(const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ g, const int32_t *__restrict__ col,
+                            const int32_t *__restrict__ cand, const unsigned long long *__restrict__ ncand_p,
+                            const int32_t *__restrict__ state, const int32_t *__restrict__ first,
+                            double *__restrict__ tx, double *__restrict__ ty, double *__restrict__ tg) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)*ncand_p) return;
+    const int32_t i = cand[t];
+    if (!(state[i] & 1)) return;
+    const int32_t j = col[i];
+    double xj = 1e+10, yj = 1e+10, gj = 0.0;
+    if (first[j] == i) {
+        xj = x[j];
+        yj = y[j];
+        gj = g[j];
+    }
+    const double xi = x[i], yi = y[i], gi = g[i];
+    if (KIND == 1) {
+        const double mn = fabs(gi) + fabs(gj);
+        tx[t] = (xi * fabs(gi) + xj * fabs(gj)) / fabs(mn);
+        ty[t] = (yi * fabs(gi) + yj * fabs(gj)) / fabs(mn);
+        tg[t] = gi + gj;
+    } else {
+        const double ng = gi + gj;
+        tx[t] = (xi * gi + xj * gj) / ng;
+        ty[t] = (yi * gi + yj * gj) / ng;
+        tg[t] = ng;
+    }
+}
+
+__global__ void k_col_write(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
+                            const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
+                            const unsigned long long *__restrict__ ncand_p, const int32_t *__restrict__ state,
+                            const double *__restrict__ tx, const double *__restrict__ ty,
+                            const double *__restrict__ tg) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)*ncand_p) return;
+    const int32_t i = cand[t];
+    if (!(state[i] & 1)) return;
+    const int32_t j = col[i];
+    x[i] = tx[t];
+    y[i] = ty[t];
+    g[i] = tg[t];
+    x[j] = 1e+10;
+    y[j] = 1e+10;
+    g[j] = 0.0;
+}
+
+int launch_collapse(vx_case *c, int kind) {
+    const int64_t n = c->n;
+    if (n <= 0) return 0;
+    hipStream_t st = c->stream;
+    if (c->flags.ensure((size_t)n, st) || c->scan.ensure((size_t)n, st) || c->lists.ensure((size_t)(2 * n), st) ||
+        c->misc.ensure((size_t)n, st) || c->tx.ensure((size_t)n, st) || c->ty.ensure((size_t)n, st) ||
+        c->tg.ensure((size_t)n, st) || c->dcount.ensure(4, st))
+        return 1;
+    int32_t *pc = c->flags.p, *state = c->scan.p, *cand = c->lists.p, *first = c->misc.p;
+    unsigned long long *ncand = (unsigned long long *)(c->dcount.p + 2);
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    VX_CHECK(hipMemsetAsync(ncand, 0, 2 * sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n, pc, state, first);
+    hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, n, pc, cand, ncand);
+    hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, cand, ncand, pc, state,
+                       (int32_t *)(c->dcount.p + 3));
+    hipLaunchKernelGGL(k_col_first, dim3(nb), dim3(256), 0, st, c->col.p, cand, ncand, state, first);
+    if (kind == 1)
+        hipLaunchKernelGGL(k_col_merge<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cand, ncand,
+                           state, first, c->tx.p, c->ty.p, c->tg.p);
+    else
+        hipLaunchKernelGGL(k_col_merge<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cand, ncand,
+                           state, first, c->tx.p, c->ty.p, c->tg.p);
+    hipLaunchKernelGGL(k_col_write, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cand, ncand, state,
+                       c->tx.p, c->ty.p, c->tg.p);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace vx
