@@ -74,7 +74,9 @@ class Case:
 
     def set_matrix(self, M):
         M = _f64(M)
-        check(lib().vx_case_set_matrix(self._h, ptr(M), self.P + 1))
+        if M.ndim != 2 or M.shape[0] != M.shape[1]:
+            raise ValueError("matrix must be square")
+        check(lib().vx_case_set_matrix(self._h, ptr(M), M.shape[0]))
 
     def matrix(self):
         M = np.zeros((self.P + 1, self.P + 1))
