@@ -32,13 +32,18 @@ sys.path.insert(0, ROOT)
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X vector FP64
 BS_FLOP_PER_PAIR = 12
+DIFF_FLOP_PER_PAIR = 17
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_case(prec, n, P=1000, dt=0.005):
+TREE_MIN_N = 65536   # direct sums below 64k vortices, Morton tree above
+ORDER, THETA = 16, 0.5
+
+
+def make_case(prec, n, P=1000, dt=0.005, tree_min_n=TREE_MIN_N):
     from vortex_amd import geometry as geo
     from vortex_amd.solver import Case, make_params
     from vortex_amd.state import uniform_wake
@@ -46,15 +51,16 @@ def make_case(prec, n, P=1000, dt=0.005):
     eps = float(pan["length"][0])
     prm = make_params(pan, dt=dt, ve_size=eps, nu=1e-3, rho=1.0, vinf=(1.0, 0.0),
                       box=(-2.0, 10.0, -10.0, 10.0), r_col_diff_sign=0.3, r_col_same_sign=0.3,
-                      max_ve_g=2 * eps, n_col=2, precision=prec)
+                      max_ve_g=2 * eps, n_col=2, precision=prec, tree_min_n=tree_min_n,
+                      order=ORDER, theta=THETA)
     x, y, g = uniform_wake(n, seed=1, max_ve_g=2 * eps)
     c = Case(prm, pan)
     c.reserve(n + 8 * P + 64)
     return c, (x, y, g), prm
 
 
-def run_path(prec, n, steps, warmup, M=None):
-    c, (x, y, g), prm = make_case(prec, n)
+def run_path(prec, n, steps, warmup, M=None, tree_min_n=TREE_MIN_N):
+    c, (x, y, g), prm = make_case(prec, n, tree_min_n=tree_min_n)
     if M is None:
         c.build_matrix()
         M = c.matrix()
@@ -77,7 +83,11 @@ def run_path(prec, n, steps, warmup, M=None):
     c.sync()
     wall = time.perf_counter() - t
     t1 = c.timings()
+    tree = c.tree_stats() if tree_min_n else None
+    per = lambda k: (t1[k] - t0[k]) / max(1, steps)
     res = {
+        "tree": tree, "bs_kernel_ms": per("bs_kernel_ms"), "tree_build_ms": per("tree_build_ms"),
+        "wall_ms": per("wall_ms"),
         "wall": wall, "n_start": n_start, "n_end": c.n,
         "bs_ms": (t1["bs_ms"] - t0["bs_ms"]) / max(1, t1["bs_launches"] - t0["bs_launches"]),
         "diff_ms": (t1["diff_ms"] - t0["diff_ms"]) / max(1, steps),
@@ -167,18 +177,29 @@ def main():
     sps = args.steps / mixed["wall"]
     n_avg = 0.5 * (mixed["n_start"] + mixed["n_end"])
     s_bs = math.ceil(n_avg / 32) * 32
-    pairs = s_bs * (s_bs - 1)
-    bs_s = mixed["bs_ms"] / 1e3
-    achieved = BS_FLOP_PER_PAIR * pairs / bs_s / 1e12
-    log(f"[bench] mixed: {sps:.4f} steps/s, BS {mixed['bs_ms']:.1f} ms, diffusion {mixed['diff_ms']:.1f} ms, "
-        f"speed {mixed['speed_ms']:.1f} ms, creation {mixed['creation_ms']:.2f} ms, step {mixed['step_ms']:.2f} ms")
+    pairs_eff = s_bs * (s_bs - 1)
+    bs_stage_s = mixed["bs_ms"] / 1e3
+    log(f"[bench] mixed: {sps:.4f} steps/s; speed {mixed['speed_ms']:.2f} ms (tree build "
+        f"{mixed['tree_build_ms']:.2f}, BS kernel {mixed['bs_kernel_ms']:.2f}, diffusion {mixed['diff_ms']:.2f}, "
+        f"wall {mixed['wall_ms']:.2f}), creation {mixed['creation_ms']:.2f} ms, step {mixed['step_ms']:.2f} ms")
+    tr = mixed["tree"] or {}
+    log(f"[bench] tree: {tr}")
+    # dominant kernel of the step and its algorithmic FLOPs per launch
+    bs_flop = BS_FLOP_PER_PAIR * tr.get("bs_pairs", 0) + (8 * ORDER + 8) * tr.get("m2p_evals", 0)
+    diff_flop = DIFF_FLOP_PER_PAIR * tr.get("diff_pairs", 0)
+    if mixed["bs_kernel_ms"] >= mixed["diff_ms"]:
+        kname, kms, kflop = "k_tree_bs<float,16>", mixed["bs_kernel_ms"], bs_flop
+    else:
+        kname, kms, kflop = "k_tree_diffusion<float>", mixed["diff_ms"], diff_flop
+    achieved = kflop / (kms / 1e3) / 1e12 if kms > 0 else 0.0
 
     f64 = None
     dcd = None
     if not args.no_f64 and args.f64_steps > 0:
         log(f"[bench] FP64 path: steps={args.f64_steps}")
-        f64, _ = run_path(VX_PREC_F64, n, args.f64_steps, 0, M=M)
-        log(f"[bench] f64: {args.f64_steps / f64['wall']:.4f} steps/s, BS {f64['bs_ms']:.1f} ms")
+        f64, _ = run_path(VX_PREC_F64, n, args.f64_steps, 1, M=M)
+        log(f"[bench] f64: {args.f64_steps / f64['wall']:.4f} steps/s; speed {f64['speed_ms']:.2f} ms, "
+            f"BS kernel {f64['bs_kernel_ms']:.2f} ms, diffusion {f64['diff_ms']:.2f} ms")
         cd_m = one_step_cd(VX_PREC_MIXED, n, M)
         cd_f = one_step_cd(VX_PREC_F64, n, M)
         dcd = abs(cd_m - cd_f) / max(abs(cd_f), 1e-300)
@@ -192,7 +213,7 @@ def main():
         log(f"[bench] cpu: {cpu['value']:.3g} steps/s ({cpu['pairs_per_s']:.3g} BS pairs/s)")
 
     out = {
-        "metric": "time steps/s at N=1M (mixed precision, direct sums)",
+        "metric": "time steps/s at N=1M (mixed precision; Morton tree above 64k vortices)",
         "value": sps,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -204,17 +225,22 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+fp64",
         "data": "synthetic (uniform wake, seed 1)",
-        "config": {"workload": "cylinder P=1000, Re=1000, dt=0.005, eps=pi/1000, N=2^20 wake",
+        "config": {"workload": "cylinder P=1000, Re=1000, dt=0.005, eps=pi/1000, N=2^20 wake; "
+                               "tree leaf 64, order 16, theta 0.5",
                    "n_vortices": int(n_avg), "panels": 1000, "parallelism": "single GPU"},
-        "pair_interactions_per_s": pairs / bs_s,
+        "pair_interactions_per_s": pairs_eff / bs_stage_s,
+        "pair_interactions_kind": "effective N(N-1) / Biot-Savart stage time (tree build included)",
+        "evaluated": {"bs_pairs": tr.get("bs_pairs"), "m2p_evals": tr.get("m2p_evals"),
+                      "diff_pairs": tr.get("diff_pairs"), "nearest3_fallbacks": tr.get("fallback")},
         "f64_steps_per_s": (args.f64_steps / f64["wall"]) if f64 else None,
         "mixed_over_f64": (sps / (args.f64_steps / f64["wall"])) if f64 else None,
         "delta_cd_rel_one_step": dcd,
-        "stage_ms": {k: mixed[k] for k in ("creation_ms", "speed_ms", "bs_ms", "diff_ms", "step_ms")},
-        "roofline": {"kernel": "k_biot_savart<float,4>", "bound": "valu", "achieved": achieved,
+        "stage_ms": {k: mixed[k] for k in ("creation_ms", "speed_ms", "tree_build_ms", "bs_kernel_ms", "diff_ms",
+                                           "wall_ms", "step_ms")},
+        "roofline": {"kernel": kname, "bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": traffic_from_profiles(),
-                     "flop_per_pair": BS_FLOP_PER_PAIR, "pairs_per_launch": pairs},
+                     "traffic": traffic_from_profiles(), "flop_per_launch": kflop,
+                     "avg_launch_ms": kms},
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }
     print(json.dumps(out), flush=True)

@@ -79,6 +79,10 @@ typedef struct vx_params {
     int64_t inc_step;                           /* capacity growth quantum       */
     int32_t precision;                          /* VX_PREC_*                     */
     int32_t tree_min_n;                         /* Morton-tree velocities at N >= this (0: never) */
+    double  theta;                              /* tree opening angle (0: 0.5)   */
+    double  diff_cut;                           /* diffusion cut-off / d_i (0: 36) */
+    int32_t order;                              /* multipole terms, 16 or 32 (0: 16) */
+    int32_t reserved;
 } vx_params;
 
 typedef struct vx_case vx_case;
@@ -122,6 +126,20 @@ int vx_birth(vx_case *c, const double *R_p, double d_g, double *gsheet);   /* wr
 int vx_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, double vinf_y);
 int vx_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src);
 int vx_diffusion_wall(vx_case *c, int64_t n_tgt);
+/* Morton-tree forms of vx_biot_savart (targets = sources = [0, s)) and
+ * vx_diffusion (on the tree of the preceding vx_tree_biot_savart); vx_speed
+ * uses them when s >= tree_min_n.  vx_tree_stats reports the work of the
+ * last tree launches. */
+int vx_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y);
+int vx_tree_diffusion(vx_case *c);
+typedef struct vx_tree_info {
+    int64_t m, leaves, nodes;        /* particles in the tree, leaves, all nodes      */
+    int64_t fallback;                /* targets whose nearest-3 used the exact search */
+    int64_t bs_pairs, m2p_evals;     /* pairwise and multipole evaluations, last BS   */
+    int64_t diff_pairs;              /* pairwise evaluations, last diffusion          */
+    int32_t order, reserved;
+} vx_tree_info;
+int vx_tree_stats(vx_case *c, vx_tree_info *info);
 int vx_step_move(vx_case *c, double out4[4]);
 int vx_compact(vx_case *c);
 int vx_setka(vx_case *c, int kind, int32_t *COL);                           /* COL[n] out */
@@ -132,6 +150,7 @@ typedef struct vx_timings {
     double creation_ms, speed_ms, step_ms;      /* cumulative, as the timings line */
     double bs_ms, diff_ms, wall_ms;             /* cumulative, per Speed kernel    */
     int64_t bs_launches;
+    double tree_build_ms, bs_kernel_ms;         /* cumulative: tree build, BS kernel alone */
 } vx_timings;
 int vx_case_timings(vx_case *c, vx_timings *t);
 int vx_case_enable_timing(vx_case *c, int on);

@@ -39,5 +39,5 @@ def test_binding_covers_exports():
 
 def test_struct_layouts_match_header():
     from vortex_amd import _lib, geometry
-    assert ctypes.sizeof(_lib.Params) == 20 * 8 + 3 * 8 + 2 * 4
+    assert ctypes.sizeof(_lib.Params) == 20 * 8 + 3 * 8 + 2 * 4 + 2 * 8 + 2 * 4
     assert geometry.PANEL_DTYPE.itemsize == 120

@@ -1,0 +1,140 @@
+"""GPU parity of the Morton-tree velocity stages against the FP64 oracle's
+direct sums (the tree has no counterpart in the reference: its own tree is
+inactive in every shipped run file and is not the parity target).
+
+Stated tolerances (relative to the largest magnitude of the compared field):
+  * Biot-Savart, FP64, 32 multipole terms, theta 0.5: 1e-8
+  * Biot-Savart, FP64, 16 terms: 2e-4;  mixed, 16 terms: 3e-4
+  * nearest-3 distance: FP64 1e-13 (exact up to rounding), mixed 2e-6
+  * diffusion (cut-off 36 d): FP64 1e-9, mixed 2e-4
+"""
+import math
+
+import numpy as np
+import pytest
+
+from vortex_amd import geometry as geo
+from vortex_amd.solver import Case, make_params, VX_PREC_F64, VX_PREC_MIXED
+from vortex_amd.state import far_tail, uniform_wake
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def wake_cloud(n, seed):
+    """A clustered wake: dense shear layers near a cylinder, sparse far field,
+    a few isolated vortices (exercise the exact nearest-3 fallback)."""
+    rng = np.random.default_rng(seed)
+    k = n // 2
+    th = rng.uniform(0, 2 * np.pi, k)
+    r = 0.5 + rng.exponential(0.02, k)
+    x1, y1 = r * np.cos(th), r * np.sin(th)
+    x2 = rng.uniform(0.5, 4.0, n - k - 8)
+    y2 = 0.3 * np.sin(2 * x2) + rng.normal(0, 0.05, n - k - 8)
+    x3 = np.array([6.0, 7.5, -1.5, 3.0, 8.0, 9.5, 5.0, -1.9])
+    y3 = np.array([3.0, -4.0, 1.0, 6.0, 0.0, 2.0, -8.0, -1.0])
+    x = np.concatenate([x1, x2, x3]); y = np.concatenate([y1, y2, y3])
+    g = rng.uniform(-0.004, 0.004, n)
+    return x, y, g
+
+
+def tail(x, y, g, s):
+    n = x.size
+    tx, ty = far_tail(n, s - n)
+    return np.concatenate([x, tx]), np.concatenate([y, ty]), np.concatenate([g, np.zeros(s - n)])
+
+
+@pytest.fixture(scope="module")
+def cloud40k(ref):
+    n = 40000
+    x, y, g = wake_cloud(n, 1)
+    s = (n + 31) // 32 * 32
+    X, Y, G = tail(x, y, g, s)
+    eps = 0.004
+    u, v, d = ref.biot_savart(X, Y, G, eps * eps, (1.0, 0.0))
+    return X, Y, G, s, eps, u, v, d
+
+
+@pytest.mark.parametrize("prec,order,tol", [(VX_PREC_F64, 32, 1e-8), (VX_PREC_F64, 16, 2e-4),
+                                            (VX_PREC_MIXED, 16, 3e-4), (VX_PREC_MIXED, 32, 3e-5)])
+def test_tree_biot_savart_vs_direct_oracle(cloud40k, prec, order, tol):
+    X, Y, G, s, eps, ur, vr, dr = cloud40k
+    pan = geo.circle(200, 0.5)
+    c = Case(make_params(pan, precision=prec, ve_size=eps, order=order), pan)
+    c.set_state(X, Y, G)
+    c.tree_biot_savart(s, (1.0, 0.0))
+    u, v, d = c.velocity(s)
+    info = c.tree_stats()
+    assert info["m"] == s  # every slot is near the box here
+    assert info["fallback"] > 0 and info["m2p_evals"] > 0
+    assert rel_err(u - 1.0, ur - 1.0) < tol and rel_err(v, vr) < tol
+    dtol = 1e-13 if prec == VX_PREC_F64 else 2e-6
+    assert np.max(np.abs(d - dr) / dr) < dtol
+
+
+@pytest.mark.parametrize("prec,tol", [(VX_PREC_F64, 1e-9), (VX_PREC_MIXED, 2e-4)])
+def test_tree_diffusion_vs_direct_oracle(ref, prec, tol):
+    n = 20000
+    x, y, g = wake_cloud(n, 2)
+    s = (n + 31) // 32 * 32
+    X, Y, G = tail(x, y, g, s)
+    eps, nu = 0.004, 1e-3
+    pan = geo.circle(200, 0.5)
+    c = Case(make_params(pan, precision=prec, ve_size=eps, nu=nu, order=32), pan)
+    c.set_state(X, Y, G)
+    c.tree_biot_savart(s, (0.0, 0.0))
+    u0, v0, d = c.velocity(s)
+    c.tree_diffusion()
+    u, v, _ = c.velocity(s)
+    ur, vr = ref.diffusion(X, Y, G, d, nu, eps, u0, v0)
+    assert rel_err(u - u0, ur - u0) < tol and rel_err(v - v0, vr - v0) < tol
+    info = c.tree_stats()
+    assert 0 < info["diff_pairs"] < s * s / 4  # the cut-off prunes
+
+
+def test_tree_and_direct_speed_agree_inside_a_time_step(ref):
+    """Whole time steps with the tree switched on from 2000 vortices against
+    the direct path on the same start state (FP64)."""
+    pan = geo.circle(200, 0.5)
+    out = {}
+    for tmin in (0, 2000):
+        prm = make_params(pan, dt=0.01, r_col_same_sign=0.3, r_col_diff_sign=0.3, birth_offset_rel=0.5,
+                          precision=VX_PREC_F64, tree_min_n=tmin, order=32)
+        c = Case(prm, pan)
+        c.build_matrix()
+        x, y, g = wake_cloud(6000, 3)
+        c.set_state(x, y, g)
+        fs = [c.time_step((1.0, 0.0)) for _ in range(2)]
+        out[tmin] = (c.n, c.state(), fs)
+    (n0, (x0, y0, g0), f0), (n1, (x1, y1, g1), f1) = out[0], out[2000]
+    assert abs(n0 - n1) <= 2
+    k = min(n0, n1)
+    assert np.median(np.abs(x0[:k] - x1[:k])) < 1e-9
+    assert abs(f0[-1][1] - f1[-1][1]) < 1e-5 * max(1.0, abs(f0[-1][1]))
+
+
+def test_tree_one_million(ref):
+    """N = 2^20 uniform wake: mixed tree against the FP64 tree with 32 terms,
+    and the FP64 tree against the oracle's direct sum on a target sample."""
+    n = 1 << 20
+    x, y, g = uniform_wake(n, seed=1)
+    eps = math.pi / 1000
+    pan = geo.circle(1000, 0.5)
+    res = {}
+    for prec, order in ((VX_PREC_F64, 32), (VX_PREC_MIXED, 16)):
+        c = Case(make_params(pan, precision=prec, ve_size=eps, order=order), pan)
+        c.set_state(x, y, g)
+        c.tree_biot_savart(n, (0.0, 0.0))
+        res[prec] = c.velocity(n)
+        c.close()
+    u, v, d = res[VX_PREC_F64]
+    um, vm, dm = res[VX_PREC_MIXED]
+    assert rel_err(um, u) < 3e-4 and rel_err(vm, v) < 3e-4
+    assert np.max(np.abs(dm - d) / d) < 2e-6
+    sample = 512
+    ur, vr, dr = ref.biot_savart(x, y, g, eps * eps, (0.0, 0.0), n_tgt=sample)
+    assert rel_err(u[:sample], ur) < 1e-8 and rel_err(v[:sample], vr) < 1e-8
+    assert np.max(np.abs(d[:sample] - dr) / dr) < 1e-13

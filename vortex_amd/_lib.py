@@ -27,7 +27,8 @@ EXPORTS = [
     "vx_vort_creation", "vx_speed", "vx_step", "vx_time_step", "vx_run_steps",
     "vx_velocity_probe", "vx_right_part", "vx_birth", "vx_biot_savart", "vx_diffusion",
     "vx_diffusion_wall", "vx_step_move", "vx_compact", "vx_setka", "vx_collapse",
-    "vx_case_timings", "vx_case_enable_timing",
+    "vx_case_timings", "vx_case_enable_timing", "vx_tree_biot_savart", "vx_tree_diffusion",
+    "vx_tree_stats",
 ]
 
 
@@ -40,12 +41,20 @@ class Params(C.Structure):
         "r_col_diff_sign", "r_col_same_sign", "max_ve_g", "rc_x", "rc_y",
         "birth_offset_rel")] + [
         ("n_of_points", C.c_int64), ("n_col", C.c_int64), ("inc_step", C.c_int64),
-        ("precision", C.c_int32), ("tree_min_n", C.c_int32)]
+        ("precision", C.c_int32), ("tree_min_n", C.c_int32),
+        ("theta", C.c_double), ("diff_cut", C.c_double), ("order", C.c_int32), ("reserved", C.c_int32)]
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("m", "leaves", "nodes", "fallback", "bs_pairs", "m2p_evals",
+                                          "diff_pairs")] + [("order", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Timings(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("creation_ms", "speed_ms", "step_ms", "bs_ms",
-                                           "diff_ms", "wall_ms")] + [("bs_launches", C.c_int64)]
+                                           "diff_ms", "wall_ms")] + [("bs_launches", C.c_int64),
+                                                                      ("tree_build_ms", C.c_double),
+                                                                      ("bs_kernel_ms", C.c_double)]
 
 
 _lib = None
@@ -98,6 +107,9 @@ def lib():
         "vx_collapse": ([vp, C.c_int, ip], C.c_int),
         "vx_case_timings": ([vp, C.POINTER(Timings)], C.c_int),
         "vx_case_enable_timing": ([vp, C.c_int], C.c_int),
+        "vx_tree_biot_savart": ([vp, i64, d, d], C.c_int),
+        "vx_tree_diffusion": ([vp], C.c_int),
+        "vx_tree_stats": ([vp, C.POINTER(TreeInfo)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -44,27 +44,6 @@ static int reserve(vx_case *c, int64_t want) {
     return 0;
 }
 
-struct Timer {
-    vx_case *c;
-    double *acc;
-    bool on;
-    Timer(vx_case *cc, double *a, int slot) : c(cc), acc(a), on(cc->timing) {
-        if (on) (void)hipEventRecord(c->ev[slot * 2], c->stream);
-        this->slot = slot;
-    }
-    int slot;
-    void stop() {
-        if (!on) return;
-        (void)hipEventRecord(c->ev[slot * 2 + 1], c->stream);
-        (void)hipEventSynchronize(c->ev[slot * 2 + 1]);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, c->ev[slot * 2], c->ev[slot * 2 + 1]);
-        *acc += ms;
-        on = false;
-    }
-    ~Timer() { stop(); }
-};
-
 }  // namespace vx
 
 using namespace vx;
@@ -149,6 +128,7 @@ int vx_case_destroy(vx_case *c) {
     c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();
     c->keys.release(); c->keys_alt.release(); c->vals.release(); c->vals_alt.release();
     c->cub_tmp.release(); c->dcount.release();
+    tree_release(c);
     if (c->pan_block) (void)hipFree(c->pan_block);
     if (c->h_count) (void)hipHostFree(c->h_count);
     if (c->h_red) (void)hipHostFree(c->h_red);
@@ -305,6 +285,25 @@ int vx_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, doub
     return 0;
 }
 
+int vx_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y) {
+    VX_ARG(c && s >= 0 && s <= c->cap, "count beyond capacity");
+    if (launch_tree_biot_savart(c, s, vinf_x, vinf_y)) return 1;
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_tree_diffusion(vx_case *c) {
+    VX_ARG(c, "case");
+    if (launch_tree_diffusion(c)) return 1;
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int vx_tree_stats(vx_case *c, vx_tree_info *info) {
+    VX_ARG(c && info, "null pointer");
+    return tree_stats(c, info);
+}
+
 int vx_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src) {
     VX_ARG(c && n_tgt >= 0 && n_tgt <= c->cap && n_src <= c->cap, "counts beyond capacity");
     if (launch_diffusion(c, n_tgt, n_src)) return 1;
@@ -376,16 +375,21 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
     const int64_t s = ceil32(c->n);
     if (reserve(c, s)) return 1;
     Timer t(c, &c->tm.speed_ms, 1);
+    const bool tree = c->prm.tree_min_n > 0 && s >= c->prm.tree_min_n;
     {
         Timer tb(c, &c->tm.bs_ms, 2);
-        if (launch_biot_savart(c, s, s, vinf_x, vinf_y)) return 1;
+        if (tree ? launch_tree_biot_savart(c, s, vinf_x, vinf_y) : launch_biot_savart(c, s, s, vinf_x, vinf_y))
+            return 1;
         c->tm.bs_launches++;
     }
     {
         Timer td(c, &c->tm.diff_ms, 3);
-        if (launch_diffusion(c, s, s)) return 1;
+        if (tree ? launch_tree_diffusion(c) : launch_diffusion(c, s, s)) return 1;
     }
-    if (launch_diffusion_wall(c, s)) return 1;
+    {
+        Timer tw(c, &c->tm.wall_ms, 7);
+        if (launch_diffusion_wall(c, s)) return 1;
+    }
     return 0;
 }
 

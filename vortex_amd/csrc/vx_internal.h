@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "vortex.h"
 
@@ -33,6 +34,8 @@ void set_error(const std::string &msg);
             return 2;                                                                    \
         }                                                                                \
     } while (0)
+
+struct TreeBufs;
 
 // Panels on the device as structure-of-arrays (one allocation).
 struct PanelsSoA {
@@ -110,13 +113,39 @@ struct vx_case {
     double *h_red = nullptr;                          // pinned reduction read-back
     // probes
     vx::DBuf<double> px, py, pu, pv;
+    // Morton tree of the last velocity stage
+    vx::TreeBufs *tree = nullptr;
     // timing
     bool timing = false;
-    hipEvent_t ev[10] = {};
+    hipEvent_t ev[16] = {};
     vx_timings tm{};
 };
 
 namespace vx {
+
+// Event pair on the case stream accumulating elapsed milliseconds (slot k
+// uses events 2k, 2k+1); active only when timing is enabled on the case.
+struct Timer {
+    vx_case *c;
+    double *acc;
+    bool on;
+    Timer(vx_case *cc, double *a, int slot) : c(cc), acc(a), on(cc->timing) {
+        if (on) (void)hipEventRecord(c->ev[slot * 2], c->stream);
+        this->slot = slot;
+    }
+    int slot;
+    void stop() {
+        if (!on) return;
+        (void)hipEventRecord(c->ev[slot * 2 + 1], c->stream);
+        (void)hipEventSynchronize(c->ev[slot * 2 + 1]);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev[slot * 2], c->ev[slot * 2 + 1]);
+        *acc += ms;
+        on = false;
+    }
+    ~Timer() { stop(); }
+};
+
 
 // ---- launchers (kernels in the .hip units) --------------------------------
 // vx_state.hip
@@ -133,6 +162,12 @@ int launch_diffusion_wall(vx_case *c, int64_t n_tgt);
 int launch_panel_matrix(vx_case *c);                   // -> c->L
 int launch_invert(vx_case *c);                         // c->L -> c->M (L destroyed copy)
 int launch_birth(vx_case *c, double d_g);              // c->rp -> c->gsheet, newborns at [n, n+P)
+// vx_tree.hip
+int tree_build(vx_case *c, int64_t s);
+int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y);
+int launch_tree_diffusion(vx_case *c);  // reuses the tree of the last launch_tree_biot_savart
+int tree_stats(vx_case *c, vx_tree_info *info);
+void tree_release(vx_case *c);
 // vx_step.hip
 int launch_step_move(vx_case *c, double out4[4]);      // synchronous read-back of the sums
 int launch_compact(vx_case *c, int64_t *n_inout);      // synchronous read-back of n

@@ -1,0 +1,771 @@
+// Morton-ordered tree for the two O(N^2) velocity stages at large N.
+//
+// Layout.  Particles that can influence anything (g != 0, or within 400 of
+// the deletion box -- farther dead slots can never be among the three
+// nearest, whose sentinel is r^2 = 1e5) are sorted by a 32-bit Morton key
+// (16 bits per axis over their bounding box).  Leaves are runs of 64
+// consecutive sorted particles; level l+1 pairs the nodes of level l, so every
+// node is a contiguous range of the sorted order.  Each node keeps its
+// bounding box, centre, radius (half diagonal) and the normalised complex
+// multipole b_k = sum_j g_j ((z_j - c) / R)^k, k < ORDER (P2M at the leaves,
+// binomial M2M above).
+//
+// Biot-Savart (src/kernel.cu:875-983 semantics).  One wave64 per target leaf
+// walks the tree with a stack held one entry per lane.  A node is far when
+// theta |c_node - c_leaf| > R_node + R_leaf and the closest possible pair is
+// beyond the smoothing radius (|c_node - c_leaf| - R_node - R_leaf > eps, so
+// max(r^2, eps^2) = r^2 there); far nodes are evaluated from their
+// multipoles (M2P, Horner in R/(z - c)), near leaves pairwise.  The nearest-3
+// distance comes from the pairwise part; a target whose third distance is not
+// below its distance to every far node is recomputed exactly by a pruned
+// search over the whole tree.
+//
+// Diffusion (src/kernel.cu:985-1029).  The weights e^{-r/d_i} are summed over
+// the leaves whose boxes come within diff_cut * max(d) of the target leaf's
+// box; every omitted source has e^{-r/d_i} < e^{-diff_cut}.
+//
+// Mixed precision: pair and M2P arithmetic in FP32 on coordinates relative to
+// the leaf centres (so close pairs keep FP32 relative accuracy), partial sums
+// per source leaf / far node accumulated in FP64.  FP64: absolute FP64
+// coordinates throughout.
+#include <hipcub/hipcub.hpp>
+
+#include <vector>
+
+#include "vx_internal.h"
+
+namespace vx {
+
+constexpr int kLeaf = 64;
+constexpr double kTiny = 1e-300;
+
+struct TreeBufs {
+    DBuf<uint32_t> keys, keys_alt;
+    DBuf<int32_t> idx, perm, incl, child0, nchild, fb;
+    DBuf<double> sx, sy, sg, ncx, ncy, nr;
+    DBuf<float> fx, fy, fg;
+    DBuf<double4> nbb, part;
+    DBuf<double2> cd;
+    DBuf<float2> cf;
+    DBuf<unsigned long long> cnt;
+    DBuf<unsigned char> tmp;
+    int64_t m = 0, nleaf = 0, nnode = 0;
+    int32_t root = 0;
+    int order = 16;
+    std::vector<int64_t> lvl_off, lvl_n;
+};
+
+struct TreeDev {
+    int64_t m, nleaf;
+    int32_t root;
+    const int32_t *perm;
+    const double *sx, *sy, *sg;
+    const float *fx, *fy, *fg;
+    const double *ncx, *ncy, *nr;
+    const double4 *nbb;
+    const int32_t *child0, *nchild;
+    const double2 *cd;
+    const float2 *cf;
+};
+
+static TreeDev view(const TreeBufs &t) {
+    return TreeDev{t.m,      t.nleaf,  t.root,   t.perm.p,  t.sx.p,   t.sy.p,     t.sg.p,     t.fx.p, t.fy.p,
+                   t.fg.p,   t.ncx.p,  t.ncy.p,  t.nr.p,    t.nbb.p,  t.child0.p, t.nchild.p, t.cd.p, t.cf.p};
+}
+
+// ---- wave helpers ----------------------------------------------------------
+__device__ __forceinline__ double wave_min(double a) {
+    for (int o = 32; o > 0; o >>= 1) a = fmin(a, __shfl_xor(a, o, 64));
+    return a;
+}
+__device__ __forceinline__ double wave_max(double a) {
+    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
+    return a;
+}
+__device__ __forceinline__ double wave_sum(double a) {
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    return a;
+}
+
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// ---- build -----------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_incl_box(const double *__restrict__ x, const double *__restrict__ y,
+                                                  const double *__restrict__ g, int64_t s, double4 dbox,
+                                                  int32_t *__restrict__ incl, double4 *__restrict__ part,
+                                                  unsigned long long *__restrict__ cnt) {
+    __shared__ double r[4][256];
+    __shared__ int c[256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+    int in = 0;
+    if (i < s) {
+        const double xi = x[i], yi = y[i], gi = g[i];
+        if (isfinite(xi) && isfinite(yi)) {
+            const double ex = fmax(0.0, fmax(dbox.x - xi, xi - dbox.y));
+            const double ey = fmax(0.0, fmax(dbox.z - yi, yi - dbox.w));
+            in = (gi != 0.0) || (ex * ex + ey * ey <= 400.0 * 400.0);
+        }
+        incl[i] = in;
+        if (in) { x0 = x1 = xi; y0 = y1 = yi; }
+    }
+    r[0][threadIdx.x] = x0; r[1][threadIdx.x] = -x1; r[2][threadIdx.x] = y0; r[3][threadIdx.x] = -y1;
+    c[threadIdx.x] = in;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            for (int q = 0; q < 4; ++q) r[q][threadIdx.x] = fmin(r[q][threadIdx.x], r[q][threadIdx.x + st]);
+            c[threadIdx.x] += c[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = make_double4(r[0][0], -r[1][0], r[2][0], -r[3][0]);
+        atomicAdd(cnt, (unsigned long long)c[0]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_box_final(const double4 *__restrict__ part, int64_t nb,
+                                                   double4 *__restrict__ box) {
+    __shared__ double r[4][256];
+    double a[4] = {1e300, 1e300, 1e300, 1e300};
+    for (int64_t b = threadIdx.x; b < nb; b += 256) {
+        const double4 p = part[b];
+        a[0] = fmin(a[0], p.x); a[1] = fmin(a[1], -p.y); a[2] = fmin(a[2], p.z); a[3] = fmin(a[3], -p.w);
+    }
+    for (int q = 0; q < 4; ++q) r[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int q = 0; q < 4; ++q) r[q][threadIdx.x] = fmin(r[q][threadIdx.x], r[q][threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double4 b = make_double4(r[0][0], -r[1][0], r[2][0], -r[3][0]);
+        if (!(b.x <= b.y)) b = make_double4(0.0, 1.0, 0.0, 1.0);
+        box[0] = b;
+    }
+}
+
+__global__ void k_tree_keys(const double *__restrict__ x, const double *__restrict__ y,
+                            const int32_t *__restrict__ incl, int64_t s, const double4 *__restrict__ box,
+                            uint32_t *__restrict__ keys, int32_t *__restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s) return;
+    vals[i] = (int32_t)i;
+    if (!incl[i]) {
+        keys[i] = 0xFFFFFFFFu;
+        return;
+    }
+    const double4 b = box[0];
+    const double wx = b.y - b.x, wy = b.w - b.z;
+    double fx = wx > 0 ? (x[i] - b.x) / wx * 65534.0 : 0.0;
+    double fy = wy > 0 ? (y[i] - b.z) / wy * 65534.0 : 0.0;
+    fx = fmin(fmax(fx, 0.0), 65534.0);
+    fy = fmin(fmax(fy, 0.0), 65534.0);
+    keys[i] = spread16((uint32_t)fx) | (spread16((uint32_t)fy) << 1);  // at most 0xFFFFFFFC
+}
+
+__global__ void k_tree_gather(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ g, const int32_t *__restrict__ perm, int64_t m,
+                              double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sg) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int32_t i = perm[t];
+    sx[t] = x[i];
+    sy[t] = y[i];
+    sg[t] = g[i];
+}
+
+// One wave per leaf: box, centre, radius, leaf-relative FP32 copies, P2M.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_leaf_build(const double *__restrict__ sx, const double *__restrict__ sy,
+                                                    const double *__restrict__ sg, int64_t m, int64_t nleaf,
+                                                    float *__restrict__ fx, float *__restrict__ fy,
+                                                    float *__restrict__ fg, double *__restrict__ ncx,
+                                                    double *__restrict__ ncy, double *__restrict__ nr,
+                                                    double4 *__restrict__ nbb, int32_t *__restrict__ child0,
+                                                    int32_t *__restrict__ nchild, double2 *__restrict__ cd,
+                                                    float2 *__restrict__ cf) {
+    const int lane = threadIdx.x & 63;
+    const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (leaf >= nleaf) return;
+    const int64_t t = leaf * kLeaf + lane;
+    const bool ok = t < m;
+    const double xi = ok ? sx[t] : 0.0, yi = ok ? sy[t] : 0.0, gi = ok ? sg[t] : 0.0;
+    const double x0 = wave_min(ok ? xi : 1e300), x1 = wave_max(ok ? xi : -1e300);
+    const double y0 = wave_min(ok ? yi : 1e300), y1 = wave_max(ok ? yi : -1e300);
+    const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
+    const double R = 0.5 * sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+    if (ok) {
+        fx[t] = (float)(xi - cx);
+        fy[t] = (float)(yi - cy);
+        fg[t] = (float)gi;
+    }
+    const double Rs = fmax(R, kTiny);
+    const double zr = (xi - cx) / Rs, zi = (yi - cy) / Rs;
+    double pr = 1.0, pi = 0.0;  // ((z - c)/R)^k
+    for (int k = 0; k < ORDER; ++k) {
+        const double br = wave_sum(gi * pr), bi = wave_sum(gi * pi);
+        if (lane == 0) {
+            cd[leaf * ORDER + k] = make_double2(br, bi);
+            cf[leaf * ORDER + k] = make_float2((float)br, (float)bi);
+        }
+        const double npr = pr * zr - pi * zi;
+        pi = pr * zi + pi * zr;
+        pr = npr;
+    }
+    if (lane == 0) {
+        ncx[leaf] = cx;
+        ncy[leaf] = cy;
+        nr[leaf] = R;
+        nbb[leaf] = make_double4(x0, x1, y0, y1);
+        child0[leaf] = -1;
+        nchild[leaf] = 0;
+    }
+}
+
+// One thread per node of a level above the leaves: box union and M2M.
+//   b^P_k = sum_{m<=k} C(k,m) delta^{k-m} rho^m b^C_m,
+//   delta = (c_C - c_P)/R_P, rho = R_C/R_P  (|delta|, rho <= 1).
+template <int ORDER>
+__global__ void k_level_build(int64_t off_c, int64_t n_c, int64_t off_p, int64_t n_p, double *__restrict__ ncx,
+                              double *__restrict__ ncy, double *__restrict__ nr, double4 *__restrict__ nbb,
+                              int32_t *__restrict__ child0, int32_t *__restrict__ nchild, double2 *__restrict__ cd,
+                              float2 *__restrict__ cf) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_p) return;
+    const int64_t p = off_p + k;
+    const int64_t c0 = off_c + 2 * k;
+    const int nc = (2 * k + 1 < n_c) ? 2 : 1;
+    double4 b = nbb[c0];
+    if (nc == 2) {
+        const double4 b1 = nbb[c0 + 1];
+        b = make_double4(fmin(b.x, b1.x), fmax(b.y, b1.y), fmin(b.z, b1.z), fmax(b.w, b1.w));
+    }
+    const double cx = 0.5 * (b.x + b.y), cy = 0.5 * (b.z + b.w);
+    const double R = 0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z));
+    const double Rs = fmax(R, kTiny);
+    double ar[ORDER], ai[ORDER];
+    for (int q = 0; q < ORDER; ++q) ar[q] = ai[q] = 0.0;
+    for (int ch = 0; ch < nc; ++ch) {
+        const int64_t c = c0 + ch;
+        const double dr = (ncx[c] - cx) / Rs, di = (ncy[c] - cy) / Rs;
+        const double rho = nr[c] / Rs;
+        double dpr[ORDER], dpi[ORDER], rp[ORDER];
+        dpr[0] = 1.0; dpi[0] = 0.0; rp[0] = 1.0;
+        for (int q = 1; q < ORDER; ++q) {
+            dpr[q] = dpr[q - 1] * dr - dpi[q - 1] * di;
+            dpi[q] = dpr[q - 1] * di + dpi[q - 1] * dr;
+            rp[q] = rp[q - 1] * rho;
+        }
+        const double2 *bc = cd + c * ORDER;
+        for (int kk = 0; kk < ORDER; ++kk) {
+            double binom = 1.0, sr = 0.0, si = 0.0;
+            for (int mm = 0; mm <= kk; ++mm) {
+                const double2 bm = bc[mm];
+                const double wr = binom * rp[mm], tr = dpr[kk - mm] * wr, ti = dpi[kk - mm] * wr;
+                sr += tr * bm.x - ti * bm.y;
+                si += tr * bm.y + ti * bm.x;
+                binom = binom * (double)(kk - mm) / (double)(mm + 1);
+            }
+            ar[kk] += sr;
+            ai[kk] += si;
+        }
+    }
+    for (int q = 0; q < ORDER; ++q) {
+        cd[p * ORDER + q] = make_double2(ar[q], ai[q]);
+        cf[p * ORDER + q] = make_float2((float)ar[q], (float)ai[q]);
+    }
+    ncx[p] = cx;
+    ncy[p] = cy;
+    nr[p] = R;
+    nbb[p] = b;
+    child0[p] = (int32_t)c0;
+    nchild[p] = nc;
+}
+
+// ---- traversal helpers ------------------------------------------------------
+struct LaneStack {
+    int slot = 0;  // this lane's stack entry
+    int sp = 0;    // wave-uniform depth
+    __device__ __forceinline__ void push(int lane, int node) {
+        if (lane == sp) slot = node;
+        ++sp;
+    }
+    __device__ __forceinline__ int pop() {
+        --sp;
+        return __builtin_amdgcn_readlane(slot, sp);
+    }
+};
+
+template <typename T>
+struct PairMath;
+template <>
+struct PairMath<double> {
+    __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+    __device__ static __forceinline__ double med3(double a, double b, double c) {
+        return fmax(fmin(a, b), fmin(fmax(a, b), c));
+    }
+};
+template <>
+struct PairMath<float> {
+    __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static __forceinline__ float med3(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
+};
+
+// ---- Biot-Savart + nearest-3 --------------------------------------------------
+template <typename T, int ORDER>
+__global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
+                                                 double *__restrict__ u, double *__restrict__ v,
+                                                 double *__restrict__ d, int32_t *__restrict__ fb,
+                                                 unsigned long long *__restrict__ nfb) {
+    using PM = PairMath<T>;
+    constexpr bool F32 = sizeof(T) == 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (leaf >= t.nleaf) return;
+    const int64_t ti = leaf * kLeaf + lane;
+    const bool ok = ti < t.m;
+    const double cAx = t.ncx[leaf], cAy = t.ncy[leaf], rA = t.nr[leaf];
+    T ax, ay;
+    if constexpr (F32) {
+        ax = ok ? t.fx[ti] : 0.f;
+        ay = ok ? t.fy[ti] : 0.f;
+    } else {
+        ax = ok ? t.sx[ti] : cAx;
+        ay = ok ? t.sy[ti] : cAy;
+    }
+    const T eps2 = (T)(eps * eps);
+    const int64_t nvalid = min((int64_t)kLeaf, t.m - leaf * kLeaf);
+    int64_t npairs = 0, nfar = 0;
+    double su = 0.0, sv = 0.0;
+    T m0 = T(1e5), m1 = T(1e5), m2 = T(1e5), m3 = T(1e5);
+    double farmin = 1e300;
+    LaneStack st;
+    st.push(lane, t.root);
+    while (st.sp > 0) {
+        const int node = st.pop();
+        const double ncx = t.ncx[node], ncy = t.ncy[node], nr = t.nr[node];
+        const double ddx = ncx - cAx, ddy = ncy - cAy;
+        const double dist = sqrt(ddx * ddx + ddy * ddy);
+        if (dist * theta > nr + rA && dist - nr - rA > eps) {
+            // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
+            T Dx, Dy;
+            if constexpr (F32) {
+                Dx = ax + (float)(cAx - ncx);
+                Dy = ay + (float)(cAy - ncy);
+            } else {
+                Dx = ax - ncx;
+                Dy = ay - ncy;
+            }
+            const T den = Dx * Dx + Dy * Dy;
+            const T inv = T(1) / den;
+            const T qx = Dx * inv, qy = -Dy * inv;
+            const T R = (T)fmax(nr, kTiny);
+            const T tx = R * qx, ty = R * qy;
+            T sr, si;
+            if constexpr (F32) {
+                const float2 *b = t.cf + (int64_t)node * ORDER;
+                sr = b[ORDER - 1].x;
+                si = b[ORDER - 1].y;
+#pragma unroll
+                for (int k = ORDER - 2; k >= 0; --k) {
+                    const T nr_ = sr * tx - si * ty + b[k].x;
+                    si = sr * ty + si * tx + b[k].y;
+                    sr = nr_;
+                }
+            } else {
+                const double2 *b = t.cd + (int64_t)node * ORDER;
+                sr = b[ORDER - 1].x;
+                si = b[ORDER - 1].y;
+#pragma unroll
+                for (int k = ORDER - 2; k >= 0; --k) {
+                    const T nr_ = sr * tx - si * ty + b[k].x;
+                    si = sr * ty + si * tx + b[k].y;
+                    sr = nr_;
+                }
+            }
+            const T Sx = sr * qx - si * qy, Sy = sr * qy + si * qx;
+            su += (double)Sy;
+            sv += (double)Sx;
+            farmin = fmin(farmin, sqrt((double)den) - nr);
+            ++nfar;
+        } else if (t.child0[node] < 0) {
+            const int64_t j0 = (int64_t)node * kLeaf;
+            const int64_t j1 = min(j0 + kLeaf, t.m);
+            T ox = 0, oy = 0;
+            if constexpr (F32) {
+                ox = ax + (float)(cAx - ncx);  // target relative to the source leaf centre
+                oy = ay + (float)(cAy - ncy);
+            }
+            T pu = 0, pv = 0;
+            npairs += j1 - j0;
+            for (int64_t j = j0; j < j1; ++j) {
+                T dx, dy, gj;
+                if constexpr (F32) {
+                    dx = ox - t.fx[j];
+                    dy = oy - t.fy[j];
+                    gj = t.fg[j];
+                } else {
+                    dx = ax - t.sx[j];
+                    dy = ay - t.sy[j];
+                    gj = t.sg[j];
+                }
+                const T r2 = dx * dx + dy * dy;
+                m3 = PM::med3(m2, m3, r2);
+                m2 = PM::med3(m1, m2, r2);
+                m1 = PM::med3(m0, m1, r2);
+                m0 = fmin(m0, r2);
+                const T w = PM::div(gj, fmax(r2, eps2));
+                pv += w * dx;
+                pu -= w * dy;
+            }
+            su += (double)pu;
+            sv += (double)pv;
+        } else {
+            const int c0 = t.child0[node];
+            if (t.nchild[node] == 2) st.push(lane, c0 + 1);
+            st.push(lane, c0);
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(nfb + 1, (unsigned long long)(npairs * nvalid));
+        atomicAdd(nfb + 2, (unsigned long long)(nfar * nvalid));
+    }
+    if (!ok) return;
+    const int32_t i = t.perm[ti];
+    u[i] = su / (2 * kPi) + vinf_x;
+    v[i] = sv / (2 * kPi) + vinf_y;
+    d[i] = (sqrt((double)m1) + sqrt((double)m2) + sqrt((double)m3)) / 3;
+    // exact only if no far node can hold a closer particle than the third
+    if ((double)m3 >= farmin * farmin * (1.0 - 1e-6)) {
+        const unsigned long long k = atomicAdd(nfb, 1ull);
+        fb[k] = (int32_t)ti;
+    }
+}
+
+// Exact nearest-3 for the flagged targets: fresh search over every leaf whose
+// box lies within the radius already found (an upper bound), FP64.
+__global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ fb,
+                                        const unsigned long long *__restrict__ nfb, double *__restrict__ d) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)*nfb) return;
+    const int64_t ti = fb[q];
+    const int32_t i = t.perm[ti];
+    const double ax = t.sx[ti], ay = t.sy[ti];
+    const double di = d[i];
+    // the pairwise part found three distances with mean sqrt <= ... ; bound r^2 by
+    // (3 d)^2, which covers the largest of the three found
+    const double rad2 = fmin(9.0 * di * di * (1.0 + 1e-6), 1e5);
+    double m0 = 1e5, m1 = 1e5, m2 = 1e5, m3 = 1e5;
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = t.root;
+    while (sp > 0) {
+        const int node = stack[--sp];
+        const double4 b = t.nbb[node];
+        const double ex = fmax(0.0, fmax(b.x - ax, ax - b.y));
+        const double ey = fmax(0.0, fmax(b.z - ay, ay - b.w));
+        if (ex * ex + ey * ey > rad2) continue;
+        if (t.child0[node] < 0) {
+            const int64_t j0 = (int64_t)node * kLeaf, j1 = min(j0 + kLeaf, t.m);
+            for (int64_t j = j0; j < j1; ++j) {
+                const double dx = ax - t.sx[j], dy = ay - t.sy[j];
+                const double r2 = dx * dx + dy * dy;
+                m3 = PairMath<double>::med3(m2, m3, r2);
+                m2 = PairMath<double>::med3(m1, m2, r2);
+                m1 = PairMath<double>::med3(m0, m1, r2);
+                m0 = fmin(m0, r2);
+            }
+        } else if (sp < 62) {
+            const int c0 = t.child0[node];
+            if (t.nchild[node] == 2) stack[sp++] = c0 + 1;
+            stack[sp++] = c0;
+        }
+    }
+    d[i] = (sqrt(m1) + sqrt(m2) + sqrt(m3)) / 3;
+}
+
+// ---- diffusion --------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
+                                                        const double *__restrict__ dd, double *__restrict__ u,
+                                                        double *__restrict__ v, unsigned long long *__restrict__ npair) {
+    constexpr bool F32 = sizeof(T) == 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (leaf >= t.nleaf) return;
+    const int64_t ti = leaf * kLeaf + lane;
+    const bool ok = ti < t.m;
+    const int32_t i = ok ? t.perm[ti] : 0;
+    const double di = ok ? dd[i] : 0.0;
+    const double R = cut * wave_max(di);
+    const double R2 = R * R;
+    const double cAx = t.ncx[leaf], cAy = t.ncy[leaf];
+    const double4 bA = t.nbb[leaf];
+    T ax, ay, kd;
+    if constexpr (F32) {
+        ax = ok ? t.fx[ti] : 0.f;
+        ay = ok ? t.fy[ti] : 0.f;
+        kd = (float)(-1.4426950408889634 / (ok ? di : 1.0));
+    } else {
+        ax = ok ? t.sx[ti] : cAx;
+        ay = ok ? t.sy[ti] : cAy;
+        kd = 1.0 / (ok ? di : 1.0);
+    }
+    const T thr2 = (T)(thr * thr);
+    const int64_t nvalid = min((int64_t)kLeaf, t.m - leaf * kLeaf);
+    int64_t npairs = 0;
+    double a2x = 0.0, a2y = 0.0, a1 = 0.0;
+    LaneStack st;
+    st.push(lane, t.root);
+    while (st.sp > 0) {
+        const int node = st.pop();
+        const double4 b = t.nbb[node];
+        const double ex = fmax(0.0, fmax(b.x - bA.y, bA.x - b.y));
+        const double ey = fmax(0.0, fmax(b.z - bA.w, bA.z - b.w));
+        if (ex * ex + ey * ey >= R2) continue;
+        if (t.child0[node] < 0) {
+            const int64_t j0 = (int64_t)node * kLeaf, j1 = min(j0 + kLeaf, t.m);
+            T ox = 0, oy = 0;
+            if constexpr (F32) {
+                ox = ax + (float)(cAx - t.ncx[node]);
+                oy = ay + (float)(cAy - t.ncy[node]);
+            }
+            T s2x = 0, s2y = 0, s1 = 0;
+            npairs += j1 - j0;
+            for (int64_t j = j0; j < j1; ++j) {
+                if constexpr (F32) {
+                    const float dx = ox - t.fx[j], dy = oy - t.fy[j];
+                    const float r2 = dx * dx + dy * dy;
+                    const float r2c = r2 > thr2 ? r2 : 1e30f;
+                    const float rs = __builtin_amdgcn_rsqf(r2c);
+                    const float dist = r2c * rs;
+                    const float e = __builtin_amdgcn_exp2f(dist * kd);
+                    const float ge = t.fg[j] * e;
+                    const float w = ge * rs;
+                    s2x += w * dx;
+                    s2y += w * dy;
+                    s1 += ge;
+                } else {
+                    const double dx = ax - t.sx[j], dy = ay - t.sy[j];
+                    const double dist = sqrt(dx * dx + dy * dy);
+                    if (dist > thr) {
+                        const double gj = t.sg[j];
+                        const double w = gj / dist;
+                        const double e = exp(-dist * kd);
+                        s2x += w * dx * e;
+                        s2y += w * dy * e;
+                        s1 += gj * e;
+                    }
+                }
+            }
+            a2x += (double)s2x;
+            a2y += (double)s2y;
+            a1 += (double)s1;
+        } else {
+            const int c0 = t.child0[node];
+            if (t.nchild[node] == 2) st.push(lane, c0 + 1);
+            st.push(lane, c0);
+        }
+    }
+    if (lane == 0) atomicAdd(npair, (unsigned long long)(npairs * nvalid));
+    if (!ok) return;
+    if (fabs(a1) > 1e-5) {
+        u[i] += nu * a2x / (a1 * di);
+        v[i] += nu * a2y / (a1 * di);
+    }
+}
+
+// Slots left out of the tree (dead and far from the box) see no neighbour.
+__global__ void k_tree_outside(const int32_t *__restrict__ perm, int64_t m, int64_t s, double vinf_x,
+                               double vinf_y, double *__restrict__ u, double *__restrict__ v,
+                               double *__restrict__ d) {
+    const int64_t t = m + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= s) return;
+    const int32_t i = perm[t];
+    u[i] = vinf_x;
+    v[i] = vinf_y;
+    d[i] = sqrt(1e5);
+}
+
+// ---- host side ----------------------------------------------------------------
+static TreeBufs &tree_of(vx_case *c) {
+    if (!c->tree) c->tree = new TreeBufs();
+    return *c->tree;
+}
+
+void tree_release(vx_case *c) {
+    if (!c->tree) return;
+    TreeBufs &t = *c->tree;
+    t.keys.release(); t.keys_alt.release(); t.idx.release(); t.perm.release(); t.incl.release();
+    t.child0.release(); t.nchild.release(); t.fb.release();
+    t.sx.release(); t.sy.release(); t.sg.release(); t.ncx.release(); t.ncy.release(); t.nr.release();
+    t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release();
+    t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release();
+    delete c->tree;
+    c->tree = nullptr;
+}
+
+static int tree_order(const vx_case *c) { return c->prm.order >= 32 ? 32 : 16; }
+
+int tree_build(vx_case *c, int64_t s) {
+    TreeBufs &t = tree_of(c);
+    hipStream_t st = c->stream;
+    const int order = tree_order(c);
+    t.order = order;
+    const int64_t nb = (s + 255) / 256;
+    if (t.keys.ensure(s, st) || t.keys_alt.ensure(s, st) || t.idx.ensure(s, st) || t.perm.ensure(s, st) ||
+        t.incl.ensure(s, st) || t.part.ensure(nb + 1, st) || t.cnt.ensure(8, st))
+        return 1;
+    VX_CHECK(hipMemsetAsync(t.cnt.p, 0, sizeof(unsigned long long) * 8, st));
+    const double4 dbox = make_double4(c->prm.x_min, c->prm.x_max, c->prm.y_min, c->prm.y_max);
+    hipLaunchKernelGGL(k_incl_box, dim3((unsigned)nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, s, dbox, t.incl.p,
+                       t.part.p, t.cnt.p);
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, st, t.part.p, nb, t.part.p + nb);
+    hipLaunchKernelGGL(k_tree_keys, dim3((unsigned)nb), dim3(256), 0, st, c->x.p, c->y.p, t.incl.p, s,
+                       t.part.p + nb, t.keys.p, t.idx.p);
+    size_t bytes = 0;
+    VX_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p, (int)s,
+                                                0, 32, st));
+    if (t.tmp.ensure(bytes, st)) return 1;
+    VX_CHECK(hipcub::DeviceRadixSort::SortPairs(t.tmp.p, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p, (int)s, 0,
+                                                32, st));
+    VX_CHECK(hipMemcpyAsync(c->h_count, t.cnt.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    VX_CHECK(hipStreamSynchronize(st));
+    const int64_t m = (int64_t)*c->h_count;
+    t.m = m;
+    t.nleaf = (m + kLeaf - 1) / kLeaf;
+    t.lvl_off.clear();
+    t.lvl_n.clear();
+    int64_t off = 0, cnt = t.nleaf;
+    while (true) {
+        t.lvl_off.push_back(off);
+        t.lvl_n.push_back(cnt);
+        off += cnt;
+        if (cnt <= 1) break;
+        cnt = (cnt + 1) / 2;
+    }
+    if (t.lvl_off.size() > 60) {
+        set_error("tree too deep");
+        return 1;
+    }
+    t.nnode = off;
+    t.root = (int32_t)(t.nnode - 1);
+    if (m == 0) return 0;
+    const int64_t nn = t.nnode;
+    if (t.sx.ensure(m, st) || t.sy.ensure(m, st) || t.sg.ensure(m, st) || t.fx.ensure(m, st) || t.fy.ensure(m, st) ||
+        t.fg.ensure(m, st) || t.ncx.ensure(nn, st) || t.ncy.ensure(nn, st) || t.nr.ensure(nn, st) ||
+        t.nbb.ensure(nn, st) || t.child0.ensure(nn, st) || t.nchild.ensure(nn, st) ||
+        t.cd.ensure(nn * order, st) || t.cf.ensure(nn * order, st) || t.fb.ensure(m, st))
+        return 1;
+    hipLaunchKernelGGL(k_tree_gather, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, c->x.p, c->y.p, c->g.p,
+                       t.perm.p, m, t.sx.p, t.sy.p, t.sg.p);
+    const unsigned lb = (unsigned)((t.nleaf + 3) / 4);
+    if (order == 32)
+        hipLaunchKernelGGL(k_leaf_build<32>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, m, t.nleaf, t.fx.p,
+                           t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p, t.nchild.p, t.cd.p, t.cf.p);
+    else
+        hipLaunchKernelGGL(k_leaf_build<16>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, m, t.nleaf, t.fx.p,
+                           t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p, t.nchild.p, t.cd.p, t.cf.p);
+    for (size_t l = 1; l < t.lvl_off.size(); ++l) {
+        const int64_t np = t.lvl_n[l];
+        const unsigned g = (unsigned)((np + 63) / 64);
+        if (order == 32)
+            hipLaunchKernelGGL(k_level_build<32>, dim3(g), dim3(64), 0, st, t.lvl_off[l - 1], t.lvl_n[l - 1],
+                               t.lvl_off[l], np, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p, t.nchild.p, t.cd.p,
+                               t.cf.p);
+        else
+            hipLaunchKernelGGL(k_level_build<16>, dim3(g), dim3(64), 0, st, t.lvl_off[l - 1], t.lvl_n[l - 1],
+                               t.lvl_off[l], np, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p, t.nchild.p, t.cd.p,
+                               t.cf.p);
+    }
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+static double theta_of(const vx_case *c) { return c->prm.theta > 0 ? c->prm.theta : 0.5; }
+
+int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y) {
+    {
+        Timer tb(c, &c->tm.tree_build_ms, 5);
+        if (tree_build(c, s)) return 1;
+    }
+    TreeBufs &t = *c->tree;
+    hipStream_t st = c->stream;
+    if (s > t.m)
+        hipLaunchKernelGGL(k_tree_outside, dim3((unsigned)((s - t.m + 255) / 256)), dim3(256), 0, st, t.perm.p, t.m, s,
+                           vinf_x, vinf_y, c->u.p, c->v.p, c->d.p);
+    if (t.m == 0) return 0;
+    const TreeDev tv = view(t);
+    const unsigned lb = (unsigned)((t.nleaf + 3) / 4);
+    const double th = theta_of(c), eps = c->prm.ve_size;
+    unsigned long long *nfb = t.cnt.p + 1;
+    const bool mixed = c->prm.precision == VX_PREC_MIXED;
+    Timer tk(c, &c->tm.bs_kernel_ms, 6);
+    if (mixed) {
+        if (t.order == 32)
+            hipLaunchKernelGGL((k_tree_bs<float, 32>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,
+                               c->v.p, c->d.p, t.fb.p, nfb);
+        else
+            hipLaunchKernelGGL((k_tree_bs<float, 16>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,
+                               c->v.p, c->d.p, t.fb.p, nfb);
+    } else {
+        if (t.order == 32)
+            hipLaunchKernelGGL((k_tree_bs<double, 32>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y,
+                               c->u.p, c->v.p, c->d.p, t.fb.p, nfb);
+        else
+            hipLaunchKernelGGL((k_tree_bs<double, 16>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y,
+                               c->u.p, c->v.p, c->d.p, t.fb.p, nfb);
+    }
+    tk.stop();
+    hipLaunchKernelGGL(k_tree_nearest_fallback, dim3((unsigned)((t.m + 255) / 256)), dim3(256), 0, st, tv, t.fb.p, nfb,
+                       c->d.p);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Uses the tree of the preceding launch_tree_biot_savart (same positions).
+int launch_tree_diffusion(vx_case *c) {
+    if (!c->tree || c->tree->m == 0) return 0;
+    TreeBufs &t = *c->tree;
+    const TreeDev tv = view(t);
+    const unsigned lb = (unsigned)((t.nleaf + 3) / 4);
+    const double cut = c->prm.diff_cut > 0 ? c->prm.diff_cut : 36.0;
+    const double thr = 0.001 * c->prm.ve_size;
+    if (c->prm.precision == VX_PREC_MIXED)
+        hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb), dim3(256), 0, c->stream, tv, cut, c->prm.nu, thr, c->d.p,
+                           c->u.p, c->v.p, t.cnt.p + 4);
+    else
+        hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb), dim3(256), 0, c->stream, tv, cut, c->prm.nu, thr,
+                           c->d.p, c->u.p, c->v.p, t.cnt.p + 4);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+int tree_stats(vx_case *c, vx_tree_info *info) {
+    *info = vx_tree_info{};
+    if (!c->tree) return 0;
+    unsigned long long h[8];
+    VX_CHECK(hipMemcpyAsync(h, c->tree->cnt.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    info->m = c->tree->m;
+    info->leaves = c->tree->nleaf;
+    info->nodes = c->tree->nnode;
+    info->fallback = (int64_t)h[1];
+    info->bs_pairs = (int64_t)h[2];
+    info->m2p_evals = (int64_t)h[3];
+    info->diff_pairs = (int64_t)h[4];
+    info->order = c->tree->order;
+    return 0;
+}
+
+}  // namespace vx

@@ -21,7 +21,8 @@ __all__ = ["Case", "make_params", "VX_PREC_F64", "VX_PREC_MIXED"]
 def make_params(panels: np.ndarray, *, dt=0.01, ve_size=None, nu=1e-3, rho=1.0, vinf=(1.0, 0.0),
                 box=(-2.0, 10.0, -10.0, 10.0), r_col_diff_sign=1.0, r_col_same_sign=0.3,
                 max_ve_g=None, rc=(0.0, 0.0), birth_offset_rel=0.0, n_of_points=20, n_col=2,
-                inc_step=8192, precision=VX_PREC_F64, tree_min_n=0) -> Params:
+                inc_step=8192, precision=VX_PREC_F64, tree_min_n=0, theta=0.5, diff_cut=36.0,
+                order=16) -> Params:
     """Case parameters with the profile box taken from the panels' left end points."""
     bx = geo.profile_bbox(panels)
     ve = float(panels["length"].mean()) if ve_size is None else ve_size
@@ -33,7 +34,8 @@ def make_params(panels: np.ndarray, *, dt=0.01, ve_size=None, nu=1e-3, rho=1.0, 
                   r_col_diff_sign=r_col_diff_sign, r_col_same_sign=r_col_same_sign,
                   max_ve_g=max_ve_g, rc_x=rc[0], rc_y=rc[1], birth_offset_rel=birth_offset_rel,
                   n_of_points=n_of_points, n_col=n_col, inc_step=inc_step,
-                  precision=precision, tree_min_n=tree_min_n)
+                  precision=precision, tree_min_n=tree_min_n, theta=theta, diff_cut=diff_cut,
+                  order=order)
 
 
 def _f64(a):
@@ -181,6 +183,18 @@ class Case:
 
     def diffusion(self, n_tgt, n_src):
         check(lib().vx_diffusion(self._h, int(n_tgt), int(n_src)))
+
+    def tree_biot_savart(self, s, vinf=(0.0, 0.0)):
+        check(lib().vx_tree_biot_savart(self._h, int(s), vinf[0], vinf[1]))
+
+    def tree_diffusion(self):
+        check(lib().vx_tree_diffusion(self._h))
+
+    def tree_stats(self) -> dict:
+        from ._lib import TreeInfo
+        t = TreeInfo()
+        check(lib().vx_tree_stats(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in TreeInfo._fields_}
 
     def diffusion_wall(self, n_tgt):
         check(lib().vx_diffusion_wall(self._h, int(n_tgt)))
