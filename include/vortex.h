@@ -80,10 +80,14 @@ typedef struct vx_params {
     int32_t precision;                          /* VX_PREC_*                     */
     int32_t tree_min_n;                         /* Morton-tree velocities at N >= this (0: never) */
     double  theta;                              /* tree opening angle (0: 0.5)   */
-    double  diff_cut;                           /* diffusion cut-off / d_i (0: 36) */
+    double  diff_cut;                           /* diffusion cut-off / d_i (0: 20 mixed, 30 FP64) */
     int32_t order;                              /* multipole terms, 16 or 32 (0: 16) */
-    int32_t reserved;
+    int32_t flags;                              /* VX_FLAG_*                     */
 } vx_params;
+
+/* Recompute the nearest-3 distance of every target by the exact search
+ * instead of only where the tree's near field cannot guarantee it. */
+#define VX_FLAG_EXACT_NEAREST 1
 
 typedef struct vx_case vx_case;
 

@@ -42,12 +42,12 @@ class Params(C.Structure):
         "birth_offset_rel")] + [
         ("n_of_points", C.c_int64), ("n_col", C.c_int64), ("inc_step", C.c_int64),
         ("precision", C.c_int32), ("tree_min_n", C.c_int32),
-        ("theta", C.c_double), ("diff_cut", C.c_double), ("order", C.c_int32), ("reserved", C.c_int32)]
+        ("theta", C.c_double), ("diff_cut", C.c_double), ("order", C.c_int32), ("flags", C.c_int32)]
 
 
 class TreeInfo(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("m", "leaves", "nodes", "fallback", "bs_pairs", "m2p_evals",
-                                          "diff_pairs")] + [("order", C.c_int32), ("reserved", C.c_int32)]
+                                          "diff_pairs")] + [("order", C.c_int32), ("flags", C.c_int32)]
 
 
 class Timings(C.Structure):

@@ -22,7 +22,8 @@
 //
 // Diffusion (src/kernel.cu:985-1029).  The weights e^{-r/d_i} are summed over
 // the leaves whose boxes come within diff_cut * max(d) of the target leaf's
-// box; every omitted source has e^{-r/d_i} < e^{-diff_cut}.
+// box; every omitted source has e^{-r/d_i} < e^{-diff_cut} (diff_cut defaults
+// to 20 in mixed precision and 30 in FP64).
 //
 // Mixed precision: pair and M2P arithmetic in FP32 on coordinates relative to
 // the leaf centres (so close pairs keep FP32 relative accuracy), partial sums
@@ -321,15 +322,50 @@ struct PairMath<float> {
     __device__ static __forceinline__ float med3(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
 };
 
+// One source leaf staged into this wave's LDS slot: one coalesced load per
+// lane, then broadcast reads in the pair loop.  The slot is private to the
+// wave, whose LDS operations complete in order, so a scheduling barrier (no
+// work-group barrier) separates the write from the reads and the reads from
+// the next write.
+template <typename T>
+struct Src4T {
+    T x, y, g, pad;
+};
+
+template <typename T>
+__device__ __forceinline__ void stage_leaf(Src4T<T> *lds, int lane, int64_t j0, int cnt, const TreeDev &t) {
+    __builtin_amdgcn_wave_barrier();
+    Src4T<T> q;
+    if (lane < cnt) {
+        if constexpr (sizeof(T) == 4) {
+            q.x = t.fx[j0 + lane];
+            q.y = t.fy[j0 + lane];
+            q.g = t.fg[j0 + lane];
+        } else {
+            q.x = t.sx[j0 + lane];
+            q.y = t.sy[j0 + lane];
+            q.g = t.sg[j0 + lane];
+        }
+    } else {
+        q.x = q.y = T(0);
+        q.g = T(0);
+    }
+    q.pad = T(0);
+    lds[lane] = q;
+    __builtin_amdgcn_wave_barrier();
+}
+
 // ---- Biot-Savart + nearest-3 --------------------------------------------------
 template <typename T, int ORDER>
 __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
                                                  double *__restrict__ u, double *__restrict__ v,
                                                  double *__restrict__ d, int32_t *__restrict__ fb,
-                                                 unsigned long long *__restrict__ nfb) {
+                                                 unsigned long long *__restrict__ nfb, int exact_all) {
     using PM = PairMath<T>;
     constexpr bool F32 = sizeof(T) == 4;
+    __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
+    Src4T<T> *lds = lds_all[threadIdx.x >> 6];
     const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (leaf >= t.nleaf) return;
     const int64_t ti = leaf * kLeaf + lane;
@@ -400,34 +436,31 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
             ++nfar;
         } else if (t.child0[node] < 0) {
             const int64_t j0 = (int64_t)node * kLeaf;
-            const int64_t j1 = min(j0 + kLeaf, t.m);
-            T ox = 0, oy = 0;
+            const int cnt = (int)(min(j0 + kLeaf, t.m) - j0);
+            T ox, oy;
             if constexpr (F32) {
                 ox = ax + (float)(cAx - ncx);  // target relative to the source leaf centre
                 oy = ay + (float)(cAy - ncy);
+            } else {
+                ox = ax;
+                oy = ay;
             }
+            stage_leaf<T>(lds, lane, j0, cnt, t);
             T pu = 0, pv = 0;
-            npairs += j1 - j0;
-            for (int64_t j = j0; j < j1; ++j) {
-                T dx, dy, gj;
-                if constexpr (F32) {
-                    dx = ox - t.fx[j];
-                    dy = oy - t.fy[j];
-                    gj = t.fg[j];
-                } else {
-                    dx = ax - t.sx[j];
-                    dy = ay - t.sy[j];
-                    gj = t.sg[j];
-                }
+            npairs += cnt;
+            for (int jj = 0; jj < cnt; ++jj) {
+                const Src4T<T> q = lds[jj];
+                const T dx = ox - q.x, dy = oy - q.y;
                 const T r2 = dx * dx + dy * dy;
                 m3 = PM::med3(m2, m3, r2);
                 m2 = PM::med3(m1, m2, r2);
                 m1 = PM::med3(m0, m1, r2);
                 m0 = fmin(m0, r2);
-                const T w = PM::div(gj, fmax(r2, eps2));
+                const T w = PM::div(q.g, fmax(r2, eps2));
                 pv += w * dx;
                 pu -= w * dy;
             }
+            __builtin_amdgcn_wave_barrier();
             su += (double)pu;
             sv += (double)pv;
         } else {
@@ -446,7 +479,7 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     v[i] = sv / (2 * kPi) + vinf_y;
     d[i] = (sqrt((double)m1) + sqrt((double)m2) + sqrt((double)m3)) / 3;
     // exact only if no far node can hold a closer particle than the third
-    if ((double)m3 >= farmin * farmin * (1.0 - 1e-6)) {
+    if (exact_all || (double)m3 >= farmin * farmin * (1.0 - 1e-6)) {
         const unsigned long long k = atomicAdd(nfb, 1ull);
         fb[k] = (int32_t)ti;
     }
@@ -500,7 +533,9 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
                                                         const double *__restrict__ dd, double *__restrict__ u,
                                                         double *__restrict__ v, unsigned long long *__restrict__ npair) {
     constexpr bool F32 = sizeof(T) == 4;
+    __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
+    Src4T<T> *lds = lds_all[threadIdx.x >> 6];
     const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (leaf >= t.nleaf) return;
     const int64_t ti = leaf * kLeaf + lane;
@@ -534,40 +569,45 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
         const double ey = fmax(0.0, fmax(b.z - bA.w, bA.z - b.w));
         if (ex * ex + ey * ey >= R2) continue;
         if (t.child0[node] < 0) {
-            const int64_t j0 = (int64_t)node * kLeaf, j1 = min(j0 + kLeaf, t.m);
-            T ox = 0, oy = 0;
+            const int64_t j0 = (int64_t)node * kLeaf;
+            const int cnt = (int)(min(j0 + kLeaf, t.m) - j0);
+            T ox, oy;
             if constexpr (F32) {
                 ox = ax + (float)(cAx - t.ncx[node]);
                 oy = ay + (float)(cAy - t.ncy[node]);
+            } else {
+                ox = ax;
+                oy = ay;
             }
+            stage_leaf<T>(lds, lane, j0, cnt, t);
             T s2x = 0, s2y = 0, s1 = 0;
-            npairs += j1 - j0;
-            for (int64_t j = j0; j < j1; ++j) {
+            npairs += cnt;
+            for (int jj = 0; jj < cnt; ++jj) {
+                const Src4T<T> q = lds[jj];
+                const T dx = ox - q.x, dy = oy - q.y;
+                const T r2 = dx * dx + dy * dy;
                 if constexpr (F32) {
-                    const float dx = ox - t.fx[j], dy = oy - t.fy[j];
-                    const float r2 = dx * dx + dy * dy;
                     const float r2c = r2 > thr2 ? r2 : 1e30f;
                     const float rs = __builtin_amdgcn_rsqf(r2c);
                     const float dist = r2c * rs;
                     const float e = __builtin_amdgcn_exp2f(dist * kd);
-                    const float ge = t.fg[j] * e;
+                    const float ge = q.g * e;
                     const float w = ge * rs;
                     s2x += w * dx;
                     s2y += w * dy;
                     s1 += ge;
                 } else {
-                    const double dx = ax - t.sx[j], dy = ay - t.sy[j];
-                    const double dist = sqrt(dx * dx + dy * dy);
+                    const double dist = sqrt(r2);
                     if (dist > thr) {
-                        const double gj = t.sg[j];
-                        const double w = gj / dist;
+                        const double w = q.g / dist;
                         const double e = exp(-dist * kd);
                         s2x += w * dx * e;
                         s2y += w * dy * e;
-                        s1 += gj * e;
+                        s1 += q.g * e;
                     }
                 }
             }
+            __builtin_amdgcn_wave_barrier();
             a2x += (double)s2x;
             a2y += (double)s2y;
             a1 += (double)s1;
@@ -710,21 +750,22 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
     const double th = theta_of(c), eps = c->prm.ve_size;
     unsigned long long *nfb = t.cnt.p + 1;
     const bool mixed = c->prm.precision == VX_PREC_MIXED;
+    const int exact_all = (c->prm.flags & VX_FLAG_EXACT_NEAREST) ? 1 : 0;
     Timer tk(c, &c->tm.bs_kernel_ms, 6);
     if (mixed) {
         if (t.order == 32)
             hipLaunchKernelGGL((k_tree_bs<float, 32>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,
-                               c->v.p, c->d.p, t.fb.p, nfb);
+                               c->v.p, c->d.p, t.fb.p, nfb, exact_all);
         else
             hipLaunchKernelGGL((k_tree_bs<float, 16>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,
-                               c->v.p, c->d.p, t.fb.p, nfb);
+                               c->v.p, c->d.p, t.fb.p, nfb, exact_all);
     } else {
         if (t.order == 32)
             hipLaunchKernelGGL((k_tree_bs<double, 32>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y,
-                               c->u.p, c->v.p, c->d.p, t.fb.p, nfb);
+                               c->u.p, c->v.p, c->d.p, t.fb.p, nfb, exact_all);
         else
             hipLaunchKernelGGL((k_tree_bs<double, 16>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y,
-                               c->u.p, c->v.p, c->d.p, t.fb.p, nfb);
+                               c->u.p, c->v.p, c->d.p, t.fb.p, nfb, exact_all);
     }
     tk.stop();
     hipLaunchKernelGGL(k_tree_nearest_fallback, dim3((unsigned)((t.m + 255) / 256)), dim3(256), 0, st, tv, t.fb.p, nfb,
@@ -739,7 +780,9 @@ int launch_tree_diffusion(vx_case *c) {
     TreeBufs &t = *c->tree;
     const TreeDev tv = view(t);
     const unsigned lb = (unsigned)((t.nleaf + 3) / 4);
-    const double cut = c->prm.diff_cut > 0 ? c->prm.diff_cut : 36.0;
+    // omitted weights total at most (1 + c) e^{-c} of the near sum's scale:
+    // 4e-8 at c = 20 (below FP32 term error), 3e-12 at c = 30
+    const double cut = c->prm.diff_cut > 0 ? c->prm.diff_cut : (c->prm.precision == VX_PREC_MIXED ? 20.0 : 30.0);
     const double thr = 0.001 * c->prm.ve_size;
     if (c->prm.precision == VX_PREC_MIXED)
         hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb), dim3(256), 0, c->stream, tv, cut, c->prm.nu, thr, c->d.p,

@@ -21,8 +21,8 @@ __all__ = ["Case", "make_params", "VX_PREC_F64", "VX_PREC_MIXED"]
 def make_params(panels: np.ndarray, *, dt=0.01, ve_size=None, nu=1e-3, rho=1.0, vinf=(1.0, 0.0),
                 box=(-2.0, 10.0, -10.0, 10.0), r_col_diff_sign=1.0, r_col_same_sign=0.3,
                 max_ve_g=None, rc=(0.0, 0.0), birth_offset_rel=0.0, n_of_points=20, n_col=2,
-                inc_step=8192, precision=VX_PREC_F64, tree_min_n=0, theta=0.5, diff_cut=36.0,
-                order=16) -> Params:
+                inc_step=8192, precision=VX_PREC_F64, tree_min_n=0, theta=0.5, diff_cut=0.0,
+                order=16, flags=0) -> Params:
     """Case parameters with the profile box taken from the panels' left end points."""
     bx = geo.profile_bbox(panels)
     ve = float(panels["length"].mean()) if ve_size is None else ve_size
@@ -35,7 +35,7 @@ def make_params(panels: np.ndarray, *, dt=0.01, ve_size=None, nu=1e-3, rho=1.0, 
                   max_ve_g=max_ve_g, rc_x=rc[0], rc_y=rc[1], birth_offset_rel=birth_offset_rel,
                   n_of_points=n_of_points, n_col=n_col, inc_step=inc_step,
                   precision=precision, tree_min_n=tree_min_n, theta=theta, diff_cut=diff_cut,
-                  order=order)
+                  order=order, flags=flags)
 
 
 def _f64(a):
