@@ -42,7 +42,8 @@ constexpr double kTiny = 1e-300;
 
 struct TreeBufs {
     DBuf<uint32_t> keys, keys_alt;
-    DBuf<int32_t> idx, perm, incl, child0, nchild, fb;
+    DBuf<int32_t> idx, perm, incl, child0, child1, first, count, parent, fb;
+    DBuf<int32_t> rlo, rhi, rleft, rright, bstart, bscan, upper, uscan, visits;
     DBuf<double> sx, sy, sg, ncx, ncy, nr;
     DBuf<float> fx, fy, fg;
     DBuf<double4> nbb, part;
@@ -50,10 +51,9 @@ struct TreeBufs {
     DBuf<float2> cf;
     DBuf<unsigned long long> cnt;
     DBuf<unsigned char> tmp;
-    int64_t m = 0, nleaf = 0, nnode = 0;
+    int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
     int32_t root = 0;
     int order = 16;
-    std::vector<int64_t> lvl_off, lvl_n;
 };
 
 struct TreeDev {
@@ -64,14 +64,15 @@ struct TreeDev {
     const float *fx, *fy, *fg;
     const double *ncx, *ncy, *nr;
     const double4 *nbb;
-    const int32_t *child0, *nchild;
+    const int32_t *child0, *child1, *first, *count;
     const double2 *cd;
     const float2 *cf;
 };
 
 static TreeDev view(const TreeBufs &t) {
-    return TreeDev{t.m,      t.nleaf,  t.root,   t.perm.p,  t.sx.p,   t.sy.p,     t.sg.p,     t.fx.p, t.fy.p,
-                   t.fg.p,   t.ncx.p,  t.ncy.p,  t.nr.p,    t.nbb.p,  t.child0.p, t.nchild.p, t.cd.p, t.cf.p};
+    return TreeDev{t.m,     t.nleaf, t.root,     t.perm.p,   t.sx.p,    t.sy.p,    t.sg.p, t.fx.p, t.fy.p, t.fg.p,
+                   t.ncx.p, t.ncy.p, t.nr.p,     t.nbb.p,    t.child0.p, t.child1.p, t.first.p, t.count.p,
+                   t.cd.p,  t.cf.p};
 }
 
 // ---- wave helpers ----------------------------------------------------------
@@ -185,21 +186,115 @@ __global__ void k_tree_gather(const double *__restrict__ x, const double *__rest
     sg[t] = g[i];
 }
 
-// One wave per leaf: box, centre, radius, leaf-relative FP32 copies, P2M.
+// ---- binary radix tree over the sorted keys (Karras 2012) ---------------------
+// delta(i, j): length of the common prefix of keys i and j, with the index as
+// tie-breaker for equal keys; -1 outside [0, m).
+__device__ __forceinline__ int rdelta(const uint32_t *__restrict__ k, int64_t m, int64_t i, int64_t j) {
+    if (j < 0 || j >= m) return -1;
+    const uint32_t a = k[i], b = k[j];
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+// Internal node i covers [lo_i, hi_i]; children >= 0 are internal nodes,
+// -(t+1) the single particle t.
+__global__ void k_radix(const uint32_t *__restrict__ key, int64_t m, int32_t *__restrict__ lo,
+                        int32_t *__restrict__ hi, int32_t *__restrict__ left, int32_t *__restrict__ right) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m - 1) return;
+    const int d = (rdelta(key, m, i, i + 1) - rdelta(key, m, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = rdelta(key, m, i, i - d);
+    int64_t lmax = 2;
+    while (rdelta(key, m, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (rdelta(key, m, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = rdelta(key, m, i, j);
+    int64_t sp = 0;
+    for (int64_t t = (l + 1) / 2;; t = (t + 1) / 2) {
+        if (rdelta(key, m, i, i + (sp + t) * d) > dnode) sp += t;
+        if (t == 1) break;
+    }
+    const int64_t gam = i + sp * d + (d < 0 ? -1 : 0);
+    const int64_t a = i < j ? i : j, b = i < j ? j : i;
+    lo[i] = (int32_t)a;
+    hi[i] = (int32_t)b;
+    left[i] = (a == gam) ? -(int32_t)(gam + 1) : (int32_t)gam;
+    right[i] = (b == gam + 1) ? -(int32_t)(gam + 2) : (int32_t)(gam + 1);
+}
+
+__device__ __forceinline__ int64_t rsize(const int32_t *lo, const int32_t *hi, int32_t c) {
+    return c >= 0 ? (int64_t)hi[c] - lo[c] + 1 : 1;
+}
+__device__ __forceinline__ int32_t rfirst(const int32_t *lo, int32_t c) { return c >= 0 ? lo[c] : -c - 1; }
+
+// Buckets: nodes of at most kLeaf particles whose parent has more.  Marks the
+// first particle of each bucket and the internal nodes kept above them.
+__global__ void k_bucket_mark(const int32_t *__restrict__ lo, const int32_t *__restrict__ hi,
+                              const int32_t *__restrict__ left, const int32_t *__restrict__ right, int64_t m,
+                              int32_t *__restrict__ bstart, int32_t *__restrict__ upper) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m - 1) return;
+    const int64_t sz = (int64_t)hi[i] - lo[i] + 1;
+    upper[i] = sz > kLeaf ? 1 : 0;
+    if (sz <= kLeaf) return;
+    const int32_t ch[2] = {left[i], right[i]};
+    for (int q = 0; q < 2; ++q)
+        if (rsize(lo, hi, ch[q]) <= kLeaf) bstart[rfirst(lo, ch[q])] = 1;
+}
+
+// Compact node array: buckets [0, nb) in particle order, then the kept
+// internal nodes.  Writes ranges, children and parents.
+__global__ void k_bucket_ranges(const int32_t *__restrict__ bstart, const int32_t *__restrict__ bscan, int64_t m,
+                                int64_t nb, int32_t *__restrict__ first, int32_t *__restrict__ count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m || !bstart[t]) return;
+    const int32_t b = bscan[t];
+    first[b] = (int32_t)t;
+    int64_t e = t + 1;
+    while (e < m && !bstart[e]) ++e;  // buckets hold at most kLeaf particles
+    count[b] = (int32_t)(e - t);
+}
+
+__global__ void k_upper_nodes(const int32_t *__restrict__ lo, const int32_t *__restrict__ hi,
+                              const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                              const int32_t *__restrict__ upper, const int32_t *__restrict__ uscan,
+                              const int32_t *__restrict__ bscan, int64_t m, int64_t nb, int32_t *__restrict__ first,
+                              int32_t *__restrict__ count, int32_t *__restrict__ child0,
+                              int32_t *__restrict__ child1, int32_t *__restrict__ parent) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m - 1 || !upper[i]) return;
+    const int32_t u = (int32_t)(nb + uscan[i]);
+    const int32_t ch[2] = {left[i], right[i]};
+    int32_t id[2];
+    for (int q = 0; q < 2; ++q) {
+        const int32_t c = ch[q];
+        id[q] = (c >= 0 && upper[c]) ? (int32_t)(nb + uscan[c]) : bscan[rfirst(lo, c)];
+        parent[id[q]] = u;
+    }
+    child0[u] = id[0];
+    child1[u] = id[1];
+    first[u] = lo[i];
+    count[u] = hi[i] - lo[i] + 1;
+}
+
+// One wave per bucket: box, centre, radius, bucket-relative FP32 copies, P2M.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_leaf_build(const double *__restrict__ sx, const double *__restrict__ sy,
-                                                    const double *__restrict__ sg, int64_t m, int64_t nleaf,
-                                                    float *__restrict__ fx, float *__restrict__ fy,
-                                                    float *__restrict__ fg, double *__restrict__ ncx,
-                                                    double *__restrict__ ncy, double *__restrict__ nr,
-                                                    double4 *__restrict__ nbb, int32_t *__restrict__ child0,
-                                                    int32_t *__restrict__ nchild, double2 *__restrict__ cd,
-                                                    float2 *__restrict__ cf) {
+__global__ __launch_bounds__(256) void k_bucket_build(const double *__restrict__ sx, const double *__restrict__ sy,
+                                                      const double *__restrict__ sg, int64_t nb,
+                                                      const int32_t *__restrict__ first,
+                                                      const int32_t *__restrict__ count, float *__restrict__ fx,
+                                                      float *__restrict__ fy, float *__restrict__ fg,
+                                                      double *__restrict__ ncx, double *__restrict__ ncy,
+                                                      double *__restrict__ nr, double4 *__restrict__ nbb,
+                                                      int32_t *__restrict__ child0, int32_t *__restrict__ child1,
+                                                      double2 *__restrict__ cd, float2 *__restrict__ cf) {
     const int lane = threadIdx.x & 63;
-    const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (leaf >= nleaf) return;
-    const int64_t t = leaf * kLeaf + lane;
-    const bool ok = t < m;
+    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= nb) return;
+    const int64_t t = (int64_t)first[b] + lane;
+    const bool ok = lane < count[b];
     const double xi = ok ? sx[t] : 0.0, yi = ok ? sy[t] : 0.0, gi = ok ? sg[t] : 0.0;
     const double x0 = wave_min(ok ? xi : 1e300), x1 = wave_max(ok ? xi : -1e300);
     const double y0 = wave_min(ok ? yi : 1e300), y1 = wave_max(ok ? yi : -1e300);
@@ -216,50 +311,42 @@ __global__ __launch_bounds__(256) void k_leaf_build(const double *__restrict__ s
     for (int k = 0; k < ORDER; ++k) {
         const double br = wave_sum(gi * pr), bi = wave_sum(gi * pi);
         if (lane == 0) {
-            cd[leaf * ORDER + k] = make_double2(br, bi);
-            cf[leaf * ORDER + k] = make_float2((float)br, (float)bi);
+            cd[b * ORDER + k] = make_double2(br, bi);
+            cf[b * ORDER + k] = make_float2((float)br, (float)bi);
         }
         const double npr = pr * zr - pi * zi;
         pi = pr * zi + pi * zr;
         pr = npr;
     }
     if (lane == 0) {
-        ncx[leaf] = cx;
-        ncy[leaf] = cy;
-        nr[leaf] = R;
-        nbb[leaf] = make_double4(x0, x1, y0, y1);
-        child0[leaf] = -1;
-        nchild[leaf] = 0;
+        ncx[b] = cx;
+        ncy[b] = cy;
+        nr[b] = R;
+        nbb[b] = make_double4(x0, x1, y0, y1);
+        child0[b] = -1;
+        child1[b] = -1;
     }
 }
 
-// One thread per node of a level above the leaves: box union and M2M.
+// Box union and M2M of node p from its two children (both complete):
 //   b^P_k = sum_{m<=k} C(k,m) delta^{k-m} rho^m b^C_m,
 //   delta = (c_C - c_P)/R_P, rho = R_C/R_P  (|delta|, rho <= 1).
 template <int ORDER>
-__global__ void k_level_build(int64_t off_c, int64_t n_c, int64_t off_p, int64_t n_p, double *__restrict__ ncx,
-                              double *__restrict__ ncy, double *__restrict__ nr, double4 *__restrict__ nbb,
-                              int32_t *__restrict__ child0, int32_t *__restrict__ nchild, double2 *__restrict__ cd,
-                              float2 *__restrict__ cf) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_p) return;
-    const int64_t p = off_p + k;
-    const int64_t c0 = off_c + 2 * k;
-    const int nc = (2 * k + 1 < n_c) ? 2 : 1;
-    double4 b = nbb[c0];
-    if (nc == 2) {
-        const double4 b1 = nbb[c0 + 1];
-        b = make_double4(fmin(b.x, b1.x), fmax(b.y, b1.y), fmin(b.z, b1.z), fmax(b.w, b1.w));
-    }
+__device__ void node_from_children(int32_t p, const int32_t *__restrict__ child0, const int32_t *__restrict__ child1,
+                                   double *__restrict__ ncx, double *__restrict__ ncy, double *__restrict__ nr,
+                                   double4 *__restrict__ nbb, double2 *__restrict__ cd, float2 *__restrict__ cf) {
+    const int32_t c[2] = {child0[p], child1[p]};
+    const double4 b0 = nbb[c[0]], b1 = nbb[c[1]];
+    const double4 b = make_double4(fmin(b0.x, b1.x), fmax(b0.y, b1.y), fmin(b0.z, b1.z), fmax(b0.w, b1.w));
     const double cx = 0.5 * (b.x + b.y), cy = 0.5 * (b.z + b.w);
     const double R = 0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z));
     const double Rs = fmax(R, kTiny);
     double ar[ORDER], ai[ORDER];
     for (int q = 0; q < ORDER; ++q) ar[q] = ai[q] = 0.0;
-    for (int ch = 0; ch < nc; ++ch) {
-        const int64_t c = c0 + ch;
-        const double dr = (ncx[c] - cx) / Rs, di = (ncy[c] - cy) / Rs;
-        const double rho = nr[c] / Rs;
+    for (int ch = 0; ch < 2; ++ch) {
+        const int32_t cc = c[ch];
+        const double dr = (ncx[cc] - cx) / Rs, di = (ncy[cc] - cy) / Rs;
+        const double rho = nr[cc] / Rs;
         double dpr[ORDER], dpi[ORDER], rp[ORDER];
         dpr[0] = 1.0; dpi[0] = 0.0; rp[0] = 1.0;
         for (int q = 1; q < ORDER; ++q) {
@@ -267,7 +354,7 @@ __global__ void k_level_build(int64_t off_c, int64_t n_c, int64_t off_p, int64_t
             dpi[q] = dpr[q - 1] * di + dpi[q - 1] * dr;
             rp[q] = rp[q - 1] * rho;
         }
-        const double2 *bc = cd + c * ORDER;
+        const double2 *bc = cd + (int64_t)cc * ORDER;
         for (int kk = 0; kk < ORDER; ++kk) {
             double binom = 1.0, sr = 0.0, si = 0.0;
             for (int mm = 0; mm <= kk; ++mm) {
@@ -282,22 +369,48 @@ __global__ void k_level_build(int64_t off_c, int64_t n_c, int64_t off_p, int64_t
         }
     }
     for (int q = 0; q < ORDER; ++q) {
-        cd[p * ORDER + q] = make_double2(ar[q], ai[q]);
-        cf[p * ORDER + q] = make_float2((float)ar[q], (float)ai[q]);
+        cd[(int64_t)p * ORDER + q] = make_double2(ar[q], ai[q]);
+        cf[(int64_t)p * ORDER + q] = make_float2((float)ar[q], (float)ai[q]);
     }
     ncx[p] = cx;
     ncy[p] = cy;
     nr[p] = R;
     nbb[p] = b;
-    child0[p] = (int32_t)c0;
-    nchild[p] = nc;
+}
+
+// Bottom-up pass, one thread per bucket: the second thread to reach a node
+// builds it and continues upward; the first one stops.  No thread waits on
+// another, so the pass always completes.
+template <int ORDER>
+__global__ void k_upper_build(int64_t nb, const int32_t *__restrict__ parent, int32_t *__restrict__ visits,
+                              const int32_t *__restrict__ child0, const int32_t *__restrict__ child1,
+                              double *__restrict__ ncx, double *__restrict__ ncy, double *__restrict__ nr,
+                              double4 *__restrict__ nbb, double2 *__restrict__ cd, float2 *__restrict__ cf) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int32_t p = parent[b];
+    while (p >= 0) {
+        __threadfence();
+        if (atomicAdd(&visits[p], 1) == 0) return;
+        __threadfence();
+        node_from_children<ORDER>(p, child0, child1, ncx, ncy, nr, nbb, cd, cf);
+        p = parent[p];
+    }
 }
 
 // ---- traversal helpers ------------------------------------------------------
+// The tree above the buckets is at most 53 levels deep (32 key bits plus
+// the index bits that split equal keys), so a depth-first walk holds at most
+// 54 entries; a push beyond 64 is dropped and counted as an error.
 struct LaneStack {
     int slot = 0;  // this lane's stack entry
     int sp = 0;    // wave-uniform depth
+    bool overflow = false;
     __device__ __forceinline__ void push(int lane, int node) {
+        if (sp >= 64) {
+            overflow = true;
+            return;
+        }
         if (lane == sp) slot = node;
         ++sp;
     }
@@ -368,8 +481,9 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
     const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (leaf >= t.nleaf) return;
-    const int64_t ti = leaf * kLeaf + lane;
-    const bool ok = ti < t.m;
+    const int64_t ti = (int64_t)t.first[leaf] + lane;
+    const int nvalid = t.count[leaf];
+    const bool ok = lane < nvalid;
     const double cAx = t.ncx[leaf], cAy = t.ncy[leaf], rA = t.nr[leaf];
     T ax, ay;
     if constexpr (F32) {
@@ -380,7 +494,6 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
         ay = ok ? t.sy[ti] : cAy;
     }
     const T eps2 = (T)(eps * eps);
-    const int64_t nvalid = min((int64_t)kLeaf, t.m - leaf * kLeaf);
     int64_t npairs = 0, nfar = 0;
     double su = 0.0, sv = 0.0;
     T m0 = T(1e5), m1 = T(1e5), m2 = T(1e5), m3 = T(1e5);
@@ -435,8 +548,8 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
             farmin = fmin(farmin, sqrt((double)den) - nr);
             ++nfar;
         } else if (t.child0[node] < 0) {
-            const int64_t j0 = (int64_t)node * kLeaf;
-            const int cnt = (int)(min(j0 + kLeaf, t.m) - j0);
+            const int64_t j0 = t.first[node];
+            const int cnt = t.count[node];
             T ox, oy;
             if constexpr (F32) {
                 ox = ax + (float)(cAx - ncx);  // target relative to the source leaf centre
@@ -464,14 +577,14 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
             su += (double)pu;
             sv += (double)pv;
         } else {
-            const int c0 = t.child0[node];
-            if (t.nchild[node] == 2) st.push(lane, c0 + 1);
-            st.push(lane, c0);
+            st.push(lane, t.child1[node]);
+            st.push(lane, t.child0[node]);
         }
     }
     if (lane == 0) {
         atomicAdd(nfb + 1, (unsigned long long)(npairs * nvalid));
         atomicAdd(nfb + 2, (unsigned long long)(nfar * nvalid));
+        if (st.overflow) atomicAdd(nfb + 4, 1ull);
     }
     if (!ok) return;
     const int32_t i = t.perm[ti];
@@ -509,7 +622,7 @@ __global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ f
         const double ey = fmax(0.0, fmax(b.z - ay, ay - b.w));
         if (ex * ex + ey * ey > rad2) continue;
         if (t.child0[node] < 0) {
-            const int64_t j0 = (int64_t)node * kLeaf, j1 = min(j0 + kLeaf, t.m);
+            const int64_t j0 = t.first[node], j1 = j0 + t.count[node];
             for (int64_t j = j0; j < j1; ++j) {
                 const double dx = ax - t.sx[j], dy = ay - t.sy[j];
                 const double r2 = dx * dx + dy * dy;
@@ -519,9 +632,8 @@ __global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ f
                 m0 = fmin(m0, r2);
             }
         } else if (sp < 62) {
-            const int c0 = t.child0[node];
-            if (t.nchild[node] == 2) stack[sp++] = c0 + 1;
-            stack[sp++] = c0;
+            stack[sp++] = t.child1[node];
+            stack[sp++] = t.child0[node];
         }
     }
     d[i] = (sqrt(m1) + sqrt(m2) + sqrt(m3)) / 3;
@@ -538,8 +650,9 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
     const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (leaf >= t.nleaf) return;
-    const int64_t ti = leaf * kLeaf + lane;
-    const bool ok = ti < t.m;
+    const int64_t ti = (int64_t)t.first[leaf] + lane;
+    const int nvalid = t.count[leaf];
+    const bool ok = lane < nvalid;
     const int32_t i = ok ? t.perm[ti] : 0;
     const double di = ok ? dd[i] : 0.0;
     const double R = cut * wave_max(di);
@@ -557,7 +670,6 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
         kd = 1.0 / (ok ? di : 1.0);
     }
     const T thr2 = (T)(thr * thr);
-    const int64_t nvalid = min((int64_t)kLeaf, t.m - leaf * kLeaf);
     int64_t npairs = 0;
     double a2x = 0.0, a2y = 0.0, a1 = 0.0;
     LaneStack st;
@@ -569,8 +681,8 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
         const double ey = fmax(0.0, fmax(b.z - bA.w, bA.z - b.w));
         if (ex * ex + ey * ey >= R2) continue;
         if (t.child0[node] < 0) {
-            const int64_t j0 = (int64_t)node * kLeaf;
-            const int cnt = (int)(min(j0 + kLeaf, t.m) - j0);
+            const int64_t j0 = t.first[node];
+            const int cnt = t.count[node];
             T ox, oy;
             if constexpr (F32) {
                 ox = ax + (float)(cAx - t.ncx[node]);
@@ -612,12 +724,14 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
             a2y += (double)s2y;
             a1 += (double)s1;
         } else {
-            const int c0 = t.child0[node];
-            if (t.nchild[node] == 2) st.push(lane, c0 + 1);
-            st.push(lane, c0);
+            st.push(lane, t.child1[node]);
+            st.push(lane, t.child0[node]);
         }
     }
-    if (lane == 0) atomicAdd(npair, (unsigned long long)(npairs * nvalid));
+    if (lane == 0) {
+        atomicAdd(npair, (unsigned long long)(npairs * nvalid));
+        if (st.overflow) atomicAdd(npair + 1, 1ull);
+    }
     if (!ok) return;
     if (fabs(a1) > 1e-5) {
         u[i] += nu * a2x / (a1 * di);
@@ -647,7 +761,9 @@ void tree_release(vx_case *c) {
     if (!c->tree) return;
     TreeBufs &t = *c->tree;
     t.keys.release(); t.keys_alt.release(); t.idx.release(); t.perm.release(); t.incl.release();
-    t.child0.release(); t.nchild.release(); t.fb.release();
+    t.child0.release(); t.child1.release(); t.first.release(); t.count.release(); t.parent.release();
+    t.fb.release(); t.rlo.release(); t.rhi.release(); t.rleft.release(); t.rright.release();
+    t.bstart.release(); t.bscan.release(); t.upper.release(); t.uscan.release(); t.visits.release();
     t.sx.release(); t.sy.release(); t.sg.release(); t.ncx.release(); t.ncy.release(); t.nr.release();
     t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release();
     t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release();
@@ -662,17 +778,17 @@ int tree_build(vx_case *c, int64_t s) {
     hipStream_t st = c->stream;
     const int order = tree_order(c);
     t.order = order;
-    const int64_t nb = (s + 255) / 256;
+    const int64_t nbs = (s + 255) / 256;
     if (t.keys.ensure(s, st) || t.keys_alt.ensure(s, st) || t.idx.ensure(s, st) || t.perm.ensure(s, st) ||
-        t.incl.ensure(s, st) || t.part.ensure(nb + 1, st) || t.cnt.ensure(8, st))
+        t.incl.ensure(s, st) || t.part.ensure(nbs + 1, st) || t.cnt.ensure(8, st))
         return 1;
     VX_CHECK(hipMemsetAsync(t.cnt.p, 0, sizeof(unsigned long long) * 8, st));
     const double4 dbox = make_double4(c->prm.x_min, c->prm.x_max, c->prm.y_min, c->prm.y_max);
-    hipLaunchKernelGGL(k_incl_box, dim3((unsigned)nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, s, dbox, t.incl.p,
+    hipLaunchKernelGGL(k_incl_box, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, s, dbox, t.incl.p,
                        t.part.p, t.cnt.p);
-    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, st, t.part.p, nb, t.part.p + nb);
-    hipLaunchKernelGGL(k_tree_keys, dim3((unsigned)nb), dim3(256), 0, st, c->x.p, c->y.p, t.incl.p, s,
-                       t.part.p + nb, t.keys.p, t.idx.p);
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, st, t.part.p, nbs, t.part.p + nbs);
+    hipLaunchKernelGGL(k_tree_keys, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, t.incl.p, s,
+                       t.part.p + nbs, t.keys.p, t.idx.p);
     size_t bytes = 0;
     VX_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p, (int)s,
                                                 0, 32, st));
@@ -683,50 +799,75 @@ int tree_build(vx_case *c, int64_t s) {
     VX_CHECK(hipStreamSynchronize(st));
     const int64_t m = (int64_t)*c->h_count;
     t.m = m;
-    t.nleaf = (m + kLeaf - 1) / kLeaf;
-    t.lvl_off.clear();
-    t.lvl_n.clear();
-    int64_t off = 0, cnt = t.nleaf;
-    while (true) {
-        t.lvl_off.push_back(off);
-        t.lvl_n.push_back(cnt);
-        off += cnt;
-        if (cnt <= 1) break;
-        cnt = (cnt + 1) / 2;
-    }
-    if (t.lvl_off.size() > 60) {
-        set_error("tree too deep");
-        return 1;
-    }
-    t.nnode = off;
-    t.root = (int32_t)(t.nnode - 1);
+    t.nleaf = 0;
+    t.nnode = 0;
     if (m == 0) return 0;
-    const int64_t nn = t.nnode;
     if (t.sx.ensure(m, st) || t.sy.ensure(m, st) || t.sg.ensure(m, st) || t.fx.ensure(m, st) || t.fy.ensure(m, st) ||
-        t.fg.ensure(m, st) || t.ncx.ensure(nn, st) || t.ncy.ensure(nn, st) || t.nr.ensure(nn, st) ||
-        t.nbb.ensure(nn, st) || t.child0.ensure(nn, st) || t.nchild.ensure(nn, st) ||
-        t.cd.ensure(nn * order, st) || t.cf.ensure(nn * order, st) || t.fb.ensure(m, st))
+        t.fg.ensure(m, st) || t.fb.ensure(m, st) || t.rlo.ensure(m, st) || t.rhi.ensure(m, st) ||
+        t.rleft.ensure(m, st) || t.rright.ensure(m, st) || t.bstart.ensure(m, st) || t.bscan.ensure(m, st) ||
+        t.upper.ensure(m, st) || t.uscan.ensure(m, st))
         return 1;
-    hipLaunchKernelGGL(k_tree_gather, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, c->x.p, c->y.p, c->g.p,
-                       t.perm.p, m, t.sx.p, t.sy.p, t.sg.p);
-    const unsigned lb = (unsigned)((t.nleaf + 3) / 4);
-    if (order == 32)
-        hipLaunchKernelGGL(k_leaf_build<32>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, m, t.nleaf, t.fx.p,
-                           t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p, t.nchild.p, t.cd.p, t.cf.p);
-    else
-        hipLaunchKernelGGL(k_leaf_build<16>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, m, t.nleaf, t.fx.p,
-                           t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p, t.nchild.p, t.cd.p, t.cf.p);
-    for (size_t l = 1; l < t.lvl_off.size(); ++l) {
-        const int64_t np = t.lvl_n[l];
-        const unsigned g = (unsigned)((np + 63) / 64);
-        if (order == 32)
-            hipLaunchKernelGGL(k_level_build<32>, dim3(g), dim3(64), 0, st, t.lvl_off[l - 1], t.lvl_n[l - 1],
-                               t.lvl_off[l], np, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p, t.nchild.p, t.cd.p,
-                               t.cf.p);
-        else
-            hipLaunchKernelGGL(k_level_build<16>, dim3(g), dim3(64), 0, st, t.lvl_off[l - 1], t.lvl_n[l - 1],
-                               t.lvl_off[l], np, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p, t.nchild.p, t.cd.p,
-                               t.cf.p);
+    const unsigned gm = (unsigned)((m + 255) / 256);
+    hipLaunchKernelGGL(k_tree_gather, dim3(gm), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, t.perm.p, m, t.sx.p, t.sy.p,
+                       t.sg.p);
+    int64_t nb = 1, nu = 0;
+    if (m > kLeaf) {
+        VX_CHECK(hipMemsetAsync(t.bstart.p, 0, sizeof(int32_t) * m, st));
+        hipLaunchKernelGGL(k_radix, dim3(gm), dim3(256), 0, st, t.keys_alt.p, m, t.rlo.p, t.rhi.p, t.rleft.p,
+                           t.rright.p);
+        hipLaunchKernelGGL(k_bucket_mark, dim3(gm), dim3(256), 0, st, t.rlo.p, t.rhi.p, t.rleft.p, t.rright.p, m,
+                           t.bstart.p, t.upper.p);
+        size_t b1 = 0, b2 = 0;
+        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, t.bstart.p, t.bscan.p, (int)m, st));
+        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, t.upper.p, t.uscan.p, (int)(m - 1), st));
+        if (t.tmp.ensure(std::max(b1, b2), st)) return 1;
+        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(t.tmp.p, b1, t.bstart.p, t.bscan.p, (int)m, st));
+        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(t.tmp.p, b2, t.upper.p, t.uscan.p, (int)(m - 1), st));
+        int32_t h[4];
+        VX_CHECK(hipMemcpyAsync(&h[0], t.bscan.p + (m - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        VX_CHECK(hipMemcpyAsync(&h[1], t.bstart.p + (m - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        VX_CHECK(hipMemcpyAsync(&h[2], t.uscan.p + (m - 2), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        VX_CHECK(hipMemcpyAsync(&h[3], t.upper.p + (m - 2), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        VX_CHECK(hipStreamSynchronize(st));
+        nb = (int64_t)h[0] + h[1];
+        nu = (int64_t)h[2] + h[3];
+    }
+    t.nleaf = nb;
+    t.nnode = nb + nu;
+    t.root = (int32_t)(nu > 0 ? nb : 0);
+    const int64_t nn = t.nnode;
+    if (t.ncx.ensure(nn, st) || t.ncy.ensure(nn, st) || t.nr.ensure(nn, st) || t.nbb.ensure(nn, st) ||
+        t.child0.ensure(nn, st) || t.child1.ensure(nn, st) || t.first.ensure(nn, st) || t.count.ensure(nn, st) ||
+        t.parent.ensure(nn, st) || t.visits.ensure(nn, st) || t.cd.ensure(nn * order, st) ||
+        t.cf.ensure(nn * order, st))
+        return 1;
+    VX_CHECK(hipMemsetAsync(t.parent.p, 0xff, sizeof(int32_t) * nn, st));
+    VX_CHECK(hipMemsetAsync(t.visits.p, 0, sizeof(int32_t) * nn, st));
+    if (m > kLeaf) {
+        hipLaunchKernelGGL(k_bucket_ranges, dim3(gm), dim3(256), 0, st, t.bstart.p, t.bscan.p, m, nb, t.first.p,
+                           t.count.p);
+        hipLaunchKernelGGL(k_upper_nodes, dim3(gm), dim3(256), 0, st, t.rlo.p, t.rhi.p, t.rleft.p, t.rright.p,
+                           t.upper.p, t.uscan.p, t.bscan.p, m, nb, t.first.p, t.count.p, t.child0.p, t.child1.p,
+                           t.parent.p);
+    } else {
+        const int32_t one[2] = {0, (int32_t)m};
+        VX_CHECK(hipMemcpyAsync(t.first.p, &one[0], sizeof(int32_t), hipMemcpyHostToDevice, st));
+        VX_CHECK(hipMemcpyAsync(t.count.p, &one[1], sizeof(int32_t), hipMemcpyHostToDevice, st));
+    }
+    const unsigned lb = (unsigned)((nb + 3) / 4);
+    const unsigned gb = (unsigned)((nb + 255) / 256);
+    if (order == 32) {
+        hipLaunchKernelGGL(k_bucket_build<32>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, nb, t.first.p,
+                           t.count.p, t.fx.p, t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p,
+                           t.child1.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_build<32>, dim3(gb), dim3(256), 0, st, nb, t.parent.p, t.visits.p, t.child0.p,
+                           t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
+    } else {
+        hipLaunchKernelGGL(k_bucket_build<16>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, nb, t.first.p,
+                           t.count.p, t.fx.p, t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p,
+                           t.child1.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_build<16>, dim3(gb), dim3(256), 0, st, nb, t.parent.p, t.visits.p, t.child0.p,
+                           t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
     }
     VX_CHECK(hipGetLastError());
     return 0;
@@ -807,6 +948,10 @@ int tree_stats(vx_case *c, vx_tree_info *info) {
     info->bs_pairs = (int64_t)h[2];
     info->m2p_evals = (int64_t)h[3];
     info->diff_pairs = (int64_t)h[4];
+    if (h[5]) {
+        set_error("tree walk stack overflow");
+        return 1;
+    }
     info->order = c->tree->order;
     return 0;
 }
