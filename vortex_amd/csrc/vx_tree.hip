@@ -306,7 +306,9 @@ __global__ __launch_bounds__(256) void k_bucket_build(const double *__restrict__
         fg[t] = (float)gi;
     }
     const double Rs = fmax(R, kTiny);
-    const double zr = (xi - cx) / Rs, zi = (yi - cy) / Rs;
+    // idle lanes sit at the centre: their zero weight must not meet an
+    // overflowing power ((0 - c)/R)^k
+    const double zr = ok ? (xi - cx) / Rs : 0.0, zi = ok ? (yi - cy) / Rs : 0.0;
     double pr = 1.0, pi = 0.0;  // ((z - c)/R)^k
     for (int k = 0; k < ORDER; ++k) {
         const double br = wave_sum(gi * pr), bi = wave_sum(gi * pi);
