@@ -141,7 +141,8 @@ typedef struct vx_tree_info {
     int64_t fallback;                /* targets whose nearest-3 used the exact search */
     int64_t bs_pairs, m2p_evals;     /* pairwise and multipole evaluations, last BS   */
     int64_t diff_pairs;              /* pairwise evaluations, last diffusion          */
-    int32_t order, reserved;
+    int32_t order;
+    int32_t heavy;                   /* diffusion targets summed by the split kernel  */
 } vx_tree_info;
 int vx_tree_stats(vx_case *c, vx_tree_info *info);
 int vx_step_move(vx_case *c, double out4[4]);

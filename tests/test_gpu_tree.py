@@ -112,6 +112,7 @@ def test_tree_diffusion_vs_direct_oracle(ref, prec, tol):
     assert rel_err(u - u0, ur - u0) < tol and rel_err(v - v0, vr - v0) < tol
     info = c.tree_stats()
     assert 0 < info["diff_pairs"] < s * s / 4  # the cut-off prunes
+    assert info["heavy"] > 0  # isolated vortices go through the split direct kernel
 
 
 def test_tree_and_direct_speed_agree_inside_a_time_step(ref):

@@ -44,7 +44,7 @@ struct TreeBufs {
     DBuf<uint32_t> keys, keys_alt;
     DBuf<int32_t> idx, perm, incl, child0, child1, first, count, parent, fb;
     DBuf<int32_t> rlo, rhi, rleft, rright, bstart, bscan, upper, uscan, visits;
-    DBuf<double> sx, sy, sg, ncx, ncy, nr;
+    DBuf<double> sx, sy, sg, ncx, ncy, nr, fbr2, hpart;
     DBuf<float> fx, fy, fg;
     DBuf<double4> nbb, part;
     DBuf<double2> cd;
@@ -52,6 +52,7 @@ struct TreeBufs {
     DBuf<unsigned long long> cnt;
     DBuf<unsigned char> tmp;
     int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
+    int64_t heavy = 0;                     // diffusion targets of the split direct kernel
     int32_t root = 0;
     int order = 16;
 };
@@ -475,7 +476,8 @@ template <typename T, int ORDER>
 __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
                                                  double *__restrict__ u, double *__restrict__ v,
                                                  double *__restrict__ d, int32_t *__restrict__ fb,
-                                                 unsigned long long *__restrict__ nfb, int exact_all) {
+                                                 double *__restrict__ fbr2, unsigned long long *__restrict__ nfb,
+                                                 int exact_all) {
     using PM = PairMath<T>;
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
@@ -597,32 +599,36 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     if (exact_all || (double)m3 >= farmin * farmin * (1.0 - 1e-6)) {
         const unsigned long long k = atomicAdd(nfb, 1ull);
         fb[k] = (int32_t)ti;
+        fbr2[k] = (double)m3;
     }
 }
 
-// Exact nearest-3 for the flagged targets: fresh search over every leaf whose
-// box lies within the radius already found (an upper bound), FP64.
-__global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ fb,
+// Exact nearest-3 for the flagged targets, FP64: a depth-first search from
+// the root that visits the nearer child first and prunes every node farther
+// than the current third distance (initially the fast pass's, an upper bound
+// since its three candidates are real particles).
+__device__ __forceinline__ double box_d2(const double4 &b, double ax, double ay) {
+    const double ex = fmax(0.0, fmax(b.x - ax, ax - b.y));
+    const double ey = fmax(0.0, fmax(b.z - ay, ay - b.w));
+    return ex * ex + ey * ey;
+}
+
+__global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ fb, const double *__restrict__ fbr2,
                                         const unsigned long long *__restrict__ nfb, double *__restrict__ d) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (int64_t)*nfb) return;
     const int64_t ti = fb[q];
     const int32_t i = t.perm[ti];
     const double ax = t.sx[ti], ay = t.sy[ti];
-    const double di = d[i];
-    // the pairwise part found three distances with mean sqrt <= ... ; bound r^2 by
-    // (3 d)^2, which covers the largest of the three found
-    const double rad2 = fmin(9.0 * di * di * (1.0 + 1e-6), 1e5);
+    const double bound0 = fmin(fbr2[q] * (1.0 + 1e-6), 1e5);
     double m0 = 1e5, m1 = 1e5, m2 = 1e5, m3 = 1e5;
     int stack[64];
     int sp = 0;
     stack[sp++] = t.root;
     while (sp > 0) {
         const int node = stack[--sp];
-        const double4 b = t.nbb[node];
-        const double ex = fmax(0.0, fmax(b.x - ax, ax - b.y));
-        const double ey = fmax(0.0, fmax(b.z - ay, ay - b.w));
-        if (ex * ex + ey * ey > rad2) continue;
+        const double lim = fmin(bound0, m3);
+        if (box_d2(t.nbb[node], ax, ay) > lim) continue;
         if (t.child0[node] < 0) {
             const int64_t j0 = t.first[node], j1 = j0 + t.count[node];
             for (int64_t j = j0; j < j1; ++j) {
@@ -634,8 +640,15 @@ __global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ f
                 m0 = fmin(m0, r2);
             }
         } else if (sp < 62) {
-            stack[sp++] = t.child1[node];
-            stack[sp++] = t.child0[node];
+            const int c0 = t.child0[node], c1 = t.child1[node];
+            const double d0 = box_d2(t.nbb[c0], ax, ay), d1 = box_d2(t.nbb[c1], ax, ay);
+            if (d0 <= d1) {
+                stack[sp++] = c1;
+                stack[sp++] = c0;
+            } else {
+                stack[sp++] = c0;
+                stack[sp++] = c1;
+            }
         }
     }
     d[i] = (sqrt(m1) + sqrt(m2) + sqrt(m3)) / 3;
@@ -643,9 +656,10 @@ __global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ f
 
 // ---- diffusion --------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
+__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double rheavy, double nu, double thr,
                                                         const double *__restrict__ dd, double *__restrict__ u,
-                                                        double *__restrict__ v, unsigned long long *__restrict__ npair) {
+                                                        double *__restrict__ v, unsigned long long *__restrict__ npair,
+                                                        int32_t *__restrict__ heavy) {
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
@@ -657,7 +671,11 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
     const bool ok = lane < nvalid;
     const int32_t i = ok ? t.perm[ti] : 0;
     const double di = ok ? dd[i] : 0.0;
-    const double R = cut * wave_max(di);
+    // a target whose cut-off radius spans a large part of the domain is summed
+    // by the split direct kernel instead of holding its whole wave
+    const bool hv = ok && cut * di > rheavy;
+    if (hv) heavy[atomicAdd(npair + 2, 1ull)] = (int32_t)ti;
+    const double R = cut * wave_max(hv ? 0.0 : di);
     const double R2 = R * R;
     const double cAx = t.ncx[leaf], cAy = t.ncy[leaf];
     const double4 bA = t.nbb[leaf];
@@ -734,8 +752,75 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
         atomicAdd(npair, (unsigned long long)(npairs * nvalid));
         if (st.overflow) atomicAdd(npair + 1, 1ull);
     }
-    if (!ok) return;
+    if (!ok || hv) return;
     if (fabs(a1) > 1e-5) {
+        u[i] += nu * a2x / (a1 * di);
+        v[i] += nu * a2y / (a1 * di);
+    }
+}
+
+// Diffusion of the heavy targets: every target against all tree particles,
+// split over source chunks (grid y), pair arithmetic on target-relative
+// differences taken in FP64; chunk partials summed in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diff_heavy_partial(TreeDev t, const int32_t *__restrict__ heavy, int64_t H,
+                                                            int64_t chunk, const double *__restrict__ dd,
+                                                            double thr, double *__restrict__ part) {
+    __shared__ double sx[256], sy[256], sg[256];
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    const int64_t lo = c * chunk, hi = min(t.m, lo + chunk);
+    const bool ok = k < H;
+    const int64_t ti = ok ? heavy[k] : 0;
+    const double ax = ok ? t.sx[ti] : 0.0, ay = ok ? t.sy[ti] : 0.0;
+    const double di = ok ? dd[t.perm[ti]] : 1.0;
+    double a2x = 0.0, a2y = 0.0, a1 = 0.0;
+    for (int64_t base = lo; base < hi; base += 256) {
+        __syncthreads();
+        const int64_t j = base + threadIdx.x;
+        sx[threadIdx.x] = j < hi ? t.sx[j] : 0.0;
+        sy[threadIdx.x] = j < hi ? t.sy[j] : 0.0;
+        sg[threadIdx.x] = j < hi ? t.sg[j] : 0.0;
+        __syncthreads();
+        const int cnt = (int)min((int64_t)256, hi - base);
+        T s2x = 0, s2y = 0, s1 = 0;
+        for (int jj = 0; jj < cnt; ++jj) {
+            const T dx = (T)(ax - sx[jj]), dy = (T)(ay - sy[jj]);
+            const T dist = sqrt(dx * dx + dy * dy);
+            if (dist > (T)thr) {
+                const T g = (T)sg[jj];
+                const T e = exp(-dist / (T)di);
+                const T w = g / dist;
+                s2x += w * dx * e;
+                s2y += w * dy * e;
+                s1 += g * e;
+            }
+        }
+        a2x += (double)s2x;
+        a2y += (double)s2y;
+        a1 += (double)s1;
+    }
+    if (ok) {
+        part[(c * 3 + 0) * H + k] = a2x;
+        part[(c * 3 + 1) * H + k] = a2y;
+        part[(c * 3 + 2) * H + k] = a1;
+    }
+}
+
+__global__ void k_diff_heavy_final(TreeDev t, const int32_t *__restrict__ heavy, int64_t H, int64_t nchunk,
+                                   const double *__restrict__ part, const double *__restrict__ dd, double nu,
+                                   double *__restrict__ u, double *__restrict__ v) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= H) return;
+    double a2x = 0.0, a2y = 0.0, a1 = 0.0;
+    for (int64_t c = 0; c < nchunk; ++c) {
+        a2x += part[(c * 3 + 0) * H + k];
+        a2y += part[(c * 3 + 1) * H + k];
+        a1 += part[(c * 3 + 2) * H + k];
+    }
+    const int32_t i = t.perm[heavy[k]];
+    if (fabs(a1) > 1e-5) {
+        const double di = dd[i];
         u[i] += nu * a2x / (a1 * di);
         v[i] += nu * a2y / (a1 * di);
     }
@@ -768,7 +853,7 @@ void tree_release(vx_case *c) {
     t.bstart.release(); t.bscan.release(); t.upper.release(); t.uscan.release(); t.visits.release();
     t.sx.release(); t.sy.release(); t.sg.release(); t.ncx.release(); t.ncy.release(); t.nr.release();
     t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release();
-    t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release();
+    t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.fbr2.release(); t.hpart.release();
     delete c->tree;
     c->tree = nullptr;
 }
@@ -805,7 +890,7 @@ int tree_build(vx_case *c, int64_t s) {
     t.nnode = 0;
     if (m == 0) return 0;
     if (t.sx.ensure(m, st) || t.sy.ensure(m, st) || t.sg.ensure(m, st) || t.fx.ensure(m, st) || t.fy.ensure(m, st) ||
-        t.fg.ensure(m, st) || t.fb.ensure(m, st) || t.rlo.ensure(m, st) || t.rhi.ensure(m, st) ||
+        t.fg.ensure(m, st) || t.fb.ensure(m, st) || t.fbr2.ensure(m, st) || t.rlo.ensure(m, st) || t.rhi.ensure(m, st) ||
         t.rleft.ensure(m, st) || t.rright.ensure(m, st) || t.bstart.ensure(m, st) || t.bscan.ensure(m, st) ||
         t.upper.ensure(m, st) || t.uscan.ensure(m, st))
         return 1;
@@ -898,21 +983,21 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
     if (mixed) {
         if (t.order == 32)
             hipLaunchKernelGGL((k_tree_bs<float, 32>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,
-                               c->v.p, c->d.p, t.fb.p, nfb, exact_all);
+                               c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
         else
             hipLaunchKernelGGL((k_tree_bs<float, 16>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,
-                               c->v.p, c->d.p, t.fb.p, nfb, exact_all);
+                               c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
     } else {
         if (t.order == 32)
             hipLaunchKernelGGL((k_tree_bs<double, 32>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y,
-                               c->u.p, c->v.p, c->d.p, t.fb.p, nfb, exact_all);
+                               c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
         else
             hipLaunchKernelGGL((k_tree_bs<double, 16>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y,
-                               c->u.p, c->v.p, c->d.p, t.fb.p, nfb, exact_all);
+                               c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
     }
     tk.stop();
-    hipLaunchKernelGGL(k_tree_nearest_fallback, dim3((unsigned)((t.m + 255) / 256)), dim3(256), 0, st, tv, t.fb.p, nfb,
-                       c->d.p);
+    hipLaunchKernelGGL(k_tree_nearest_fallback, dim3((unsigned)((t.m + 255) / 256)), dim3(256), 0, st, tv, t.fb.p,
+                       t.fbr2.p, nfb, c->d.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
@@ -921,18 +1006,48 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
 int launch_tree_diffusion(vx_case *c) {
     if (!c->tree || c->tree->m == 0) return 0;
     TreeBufs &t = *c->tree;
+    hipStream_t st = c->stream;
     const TreeDev tv = view(t);
     const unsigned lb = (unsigned)((t.nleaf + 3) / 4);
     // omitted weights total at most (1 + c) e^{-c} of the near sum's scale:
     // 4e-8 at c = 20 (below FP32 term error), 3e-12 at c = 30
-    const double cut = c->prm.diff_cut > 0 ? c->prm.diff_cut : (c->prm.precision == VX_PREC_MIXED ? 20.0 : 30.0);
+    const bool mixed = c->prm.precision == VX_PREC_MIXED;
+    const double cut = c->prm.diff_cut > 0 ? c->prm.diff_cut : (mixed ? 20.0 : 30.0);
     const double thr = 0.001 * c->prm.ve_size;
-    if (c->prm.precision == VX_PREC_MIXED)
-        hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb), dim3(256), 0, c->stream, tv, cut, c->prm.nu, thr, c->d.p,
-                           c->u.p, c->v.p, t.cnt.p + 4);
+    double4 rootbox;
+    VX_CHECK(hipMemcpyAsync(&rootbox, t.nbb.p + t.root, sizeof(double4), hipMemcpyDeviceToHost, st));
+    VX_CHECK(hipStreamSynchronize(st));
+    const double rroot = 0.5 * sqrt((rootbox.y - rootbox.x) * (rootbox.y - rootbox.x) +
+                                    (rootbox.w - rootbox.z) * (rootbox.w - rootbox.z));
+    const double rheavy = fmax(0.05 * rroot, 1e-300);
+    if (mixed)
+        hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb), dim3(256), 0, st, tv, cut, rheavy, c->prm.nu, thr, c->d.p,
+                           c->u.p, c->v.p, t.cnt.p + 4, t.fb.p);
     else
-        hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb), dim3(256), 0, c->stream, tv, cut, c->prm.nu, thr,
-                           c->d.p, c->u.p, c->v.p, t.cnt.p + 4);
+        hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb), dim3(256), 0, st, tv, cut, rheavy, c->prm.nu, thr,
+                           c->d.p, c->u.p, c->v.p, t.cnt.p + 4, t.fb.p);
+    VX_CHECK(hipGetLastError());
+    unsigned long long hH = 0;
+    VX_CHECK(hipMemcpyAsync(&hH, t.cnt.p + 6, sizeof(hH), hipMemcpyDeviceToHost, st));
+    VX_CHECK(hipStreamSynchronize(st));
+    const int64_t H = (int64_t)hH;
+    t.heavy = H;
+    if (H == 0) return 0;
+    const int64_t tb = (H + 255) / 256;
+    int64_t want = (4096 + tb - 1) / tb;
+    const int64_t tiles = (t.m + 255) / 256;
+    if (want > tiles) want = tiles;
+    const int64_t chunk = ((tiles + want - 1) / want) * 256;
+    const int64_t nchunk = (t.m + chunk - 1) / chunk;
+    if (t.hpart.ensure(3 * nchunk * H, st)) return 1;
+    if (mixed)
+        hipLaunchKernelGGL(k_diff_heavy_partial<float>, dim3((unsigned)tb, (unsigned)nchunk), dim3(256), 0, st, tv,
+                           t.fb.p, H, chunk, c->d.p, thr, t.hpart.p);
+    else
+        hipLaunchKernelGGL(k_diff_heavy_partial<double>, dim3((unsigned)tb, (unsigned)nchunk), dim3(256), 0, st, tv,
+                           t.fb.p, H, chunk, c->d.p, thr, t.hpart.p);
+    hipLaunchKernelGGL(k_diff_heavy_final, dim3((unsigned)tb), dim3(256), 0, st, tv, t.fb.p, H, nchunk, t.hpart.p,
+                       c->d.p, c->prm.nu, c->u.p, c->v.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
@@ -955,6 +1070,7 @@ int tree_stats(vx_case *c, vx_tree_info *info) {
         return 1;
     }
     info->order = c->tree->order;
+    info->heavy = (int32_t)c->tree->heavy;
     return 0;
 }
 
