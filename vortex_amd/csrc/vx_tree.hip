@@ -596,7 +596,9 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     v[i] = sv / (2 * kPi) + vinf_y;
     d[i] = (sqrt((double)m1) + sqrt((double)m2) + sqrt((double)m3)) / 3;
     // exact only if no far node can hold a closer particle than the third
-    if (exact_all || (double)m3 >= farmin * farmin * (1.0 - 1e-6)) {
+    // FP32 squared distances of close pairs carry up to ~1e-5 relative error,
+    // hence the 1e-3 margins here and in the search bound
+    if (exact_all || (double)m3 >= farmin * farmin * (1.0 - 1e-3)) {
         const unsigned long long k = atomicAdd(nfb, 1ull);
         fb[k] = (int32_t)ti;
         fbr2[k] = (double)m3;
@@ -620,7 +622,7 @@ __global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ f
     const int64_t ti = fb[q];
     const int32_t i = t.perm[ti];
     const double ax = t.sx[ti], ay = t.sy[ti];
-    const double bound0 = fmin(fbr2[q] * (1.0 + 1e-6), 1e5);
+    const double bound0 = fmin(fbr2[q] * (1.0 + 1e-3) + 1e-300, 1e5);
     double m0 = 1e5, m1 = 1e5, m2 = 1e5, m3 = 1e5;
     int stack[64];
     int sp = 0;
