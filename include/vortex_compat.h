@@ -1,0 +1,127 @@
+/*
+ * vortex_compat.h -- the reference's host API, served by libvortex_amd.so.
+ *
+ * A caller written against grechkin-pogrebnyakov/vortex include/unit_1.cuh
+ * (the L4 boundary between src/main.c and the device code) links against
+ * libvortex_amd.so instead and keeps its call sequence.  Types and layouts
+ * are the reference's (include/definitions.h:35-142, default build with the
+ * tree field, so sizeof(Vortex) == 32): vortex arrays are caller-owned
+ * device arrays of Vortex / PVortex, the panel-system inverse is the padded
+ * (Pb+1)^2 row-major matrix.  The hidden inputs are the reference's globals:
+ * `struct conf_t conf` is read from the caller (this library only provides a
+ * weak default definition, which a caller's own definition replaces).
+ *
+ * Each entry point stages the caller's AoS arrays into the library's
+ * structure-of-arrays state, runs the same kernels as include/vortex.h, and
+ * writes the results back, with the reference's return convention (0 ok,
+ * 1 runtime error, 2 bad arguments; NULL for the matrix routines).
+ *
+ * Entry point            replaces (path:line in the reference)
+ *   matr_creation          include/unit_1.cuh:21, src/unit_1.cu:256-402
+ *   load_matrix            include/unit_1.cuh:24, src/unit_1.cu:404-456
+ *   allocate_arrays        include/unit_1.cuh:27, src/unit_1.cu:458-486
+ *   randomize_tail         include/unit_1.cuh:29, src/unit_1.cu:488-509
+ *   incr_vort_quant        include/unit_1.cuh:31, src/unit_1.cu:511-558
+ *   vort_creation          include/unit_1.cuh:34, src/unit_1.cu:560-595
+ *   start_timer            include/unit_1.cuh:38, src/unit_1.cu:597-603
+ *   stop_timer             include/unit_1.cuh:40, src/unit_1.cu:605-613
+ *   Speed                  include/unit_1.cuh:43, src/unit_1.cu:963-1075
+ *   Step                   include/unit_1.cuh:46, src/unit_1.cu:1077-1218
+ *   init_device_conf_values include/unit_1.cuh:48, src/unit_1.cu:1220-1252
+ *   velocity_control       include/unit_1.cuh:51, src/unit_1.cu:1254-1269
+ * Not provided: second_speed / first_speed (the passive and inertial second
+ * component, include/unit_1.cuh:52-53), which is outside this library.
+ */
+#ifndef VORTEX_COMPAT_H_
+#define VORTEX_COMPAT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double TVars;
+typedef TVars TVctr[2];
+
+typedef struct Vortex {
+    TVars r[2];
+    TVars g;
+    unsigned int tree_id;
+} Vortex;
+
+typedef struct PVortex {
+    TVars v[2];
+} PVortex;
+
+typedef struct tPanel {
+    unsigned int n;
+    TVctr left, right, contr, birth, norm, tang;
+    TVars length;
+    unsigned int n_of_lpanel, n_of_rpanel;
+} tPanel;
+
+/* include/definitions.h:115-142 (tree fields present) */
+struct conf_t {
+    size_t steps, saving_step;
+    TVars dt;
+    size_t inc_step;
+    TVars viscosity, rho;
+    TVars v_inf_x;
+    TVars v_inf_y;
+    size_t n_of_points;
+    TVars x_max, x_min, y_max, y_min;
+    TVars profile_x_min;
+    TVars profile_x_max;
+    TVars profile_y_min;
+    TVars profile_y_max;
+    TVars ve_size;
+    size_t n_col;
+    TVars h_col_x, h_col_y;
+    TVars rc_x;
+    TVars rc_y;
+    char pr_file[256];
+    size_t birth_quant;
+    TVars r_col_diff_sign, r_col_same_sign;
+    uint8_t matrix_load;
+    size_t v_inf_incr_steps;
+    TVars max_ve_g;
+    char log_file[256];
+    char config_file[256];
+    uint8_t log_level;
+    char timings_file[256];
+    char kadr_file[256];
+    char second_points_file[256];
+    TVars rel_t;
+    uint8_t steady_flow;
+    size_t tree_depth;
+    TVars theta;
+    uint8_t no_log_buf;
+};
+
+extern struct conf_t conf;
+
+TVars *matr_creation(tPanel *panels, size_t s, size_t birth);
+TVars *load_matrix(size_t *p);
+int allocate_arrays(Vortex **p_host, Vortex **p_dev, PVortex **v_host, PVortex **v_dev, TVars **d_dev, size_t size);
+int randomize_tail(Vortex **p_dev, size_t new_size, size_t increased);
+int incr_vort_quant(Vortex **p_host, Vortex **p_dev, PVortex **v_host, PVortex **v_dev, TVars **d_dev, size_t *size);
+int vort_creation(Vortex *pos, TVctr *V_infDev, size_t n_of_birth, size_t n_of_birth_BLOCK_S, size_t n, TVars *M_Dev,
+                  TVars *d_g, tPanel *panels);
+void start_timer(hipEvent_t *start, hipEvent_t *stop);
+float stop_timer(hipEvent_t start, hipEvent_t stop);
+int Speed(Vortex *pos, TVctr *v_inf, size_t s, PVortex *v, TVars *d, TVars nu, tPanel *panels);
+int Step(Vortex *pos, PVortex *V, size_t *n, size_t s, TVars *d_g, PVortex *F_p, TVars *M, tPanel *panels);
+int init_device_conf_values(void);
+int velocity_control(Vortex *pos, TVctr *V_inf, size_t n, Vortex *Contr_points, PVortex *V, size_t n_contr);
+
+/* Select the arithmetic of the compat entry points (VX_PREC_F64 by default). */
+int vx_compat_set_precision(int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VORTEX_COMPAT_H_ */

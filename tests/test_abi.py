@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^\s*(?:const\s+)?(?:int|int64_t|double|void|char|const char)\s*\**\s*(\w+)\s*\(",
+    names = re.findall(r"^\s*(?:const\s+)?(?:int|int64_t|double|void|char|float|TVars|const char)\s*\**\s*(\w+)\s*\(",
                        text, flags=re.M)
     return sorted(set(names))
 
