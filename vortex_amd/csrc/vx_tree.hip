@@ -230,19 +230,32 @@ __device__ __forceinline__ int64_t rsize(const int32_t *lo, const int32_t *hi, i
 }
 __device__ __forceinline__ int32_t rfirst(const int32_t *lo, int32_t c) { return c >= 0 ? lo[c] : -c - 1; }
 
-// Buckets: nodes of at most kLeaf particles whose parent has more.  Marks the
+// Buckets: nodes of at most kLeaf particles whose cell (common key prefix of
+// at least dmin bits) is no coarser than about twice the typical bucket, and
+// whose parent is not.  The depth condition keeps sparse regions from forming
+// large buckets whose near field would cover much of the domain.  Marks the
 // first particle of each bucket and the internal nodes kept above them.
-__global__ void k_bucket_mark(const int32_t *__restrict__ lo, const int32_t *__restrict__ hi,
-                              const int32_t *__restrict__ left, const int32_t *__restrict__ right, int64_t m,
-                              int32_t *__restrict__ bstart, int32_t *__restrict__ upper) {
+__device__ __forceinline__ bool is_upper(const uint32_t *__restrict__ key, const int32_t *lo, const int32_t *hi,
+                                         int32_t i, int dmin) {
+    const int64_t sz = (int64_t)hi[i] - lo[i] + 1;
+    if (sz > kLeaf) return true;
+    const uint32_t a = key[lo[i]], b = key[hi[i]];
+    const int pref = a == b ? 32 : __clz(a ^ b);
+    return pref < dmin;
+}
+
+__global__ void k_bucket_mark(const uint32_t *__restrict__ key, const int32_t *__restrict__ lo,
+                              const int32_t *__restrict__ hi, const int32_t *__restrict__ left,
+                              const int32_t *__restrict__ right, int64_t m, int dmin, int32_t *__restrict__ bstart,
+                              int32_t *__restrict__ upper) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m - 1) return;
-    const int64_t sz = (int64_t)hi[i] - lo[i] + 1;
-    upper[i] = sz > kLeaf ? 1 : 0;
-    if (sz <= kLeaf) return;
+    const bool up = is_upper(key, lo, hi, (int32_t)i, dmin);
+    upper[i] = up ? 1 : 0;
+    if (!up) return;
     const int32_t ch[2] = {left[i], right[i]};
     for (int q = 0; q < 2; ++q)
-        if (rsize(lo, hi, ch[q]) <= kLeaf) bstart[rfirst(lo, ch[q])] = 1;
+        if (ch[q] < 0 || !is_upper(key, lo, hi, ch[q], dmin)) bstart[rfirst(lo, ch[q])] = 1;
 }
 
 // Compact node array: buckets [0, nb) in particle order, then the kept
@@ -271,7 +284,7 @@ __global__ void k_upper_nodes(const int32_t *__restrict__ lo, const int32_t *__r
     int32_t id[2];
     for (int q = 0; q < 2; ++q) {
         const int32_t c = ch[q];
-        id[q] = (c >= 0 && upper[c]) ? (int32_t)(nb + uscan[c]) : bscan[rfirst(lo, c)];
+        id[q] = (c >= 0 && upper[c]) ? (int32_t)(nb + uscan[c]) : bscan[rfirst(lo, c)];  // bucket: its first particle
         parent[id[q]] = u;
     }
     child0[u] = id[0];
@@ -900,12 +913,18 @@ int tree_build(vx_case *c, int64_t s) {
     hipLaunchKernelGGL(k_tree_gather, dim3(gm), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, t.perm.p, m, t.sx.p, t.sy.p,
                        t.sg.p);
     int64_t nb = 1, nu = 0;
+    // Key prefix depth of a bucket: at most one quadtree level coarser than a
+    // cell holding kLeaf particles at the mean density.  Keys normalise each
+    // axis of the bounding box separately, so a level-L cell holds m / 4^L
+    // particles on average whatever the box's aspect ratio.
+    const double lv = floor(0.5 * log2((double)m / kLeaf)) - 1.0;
+    const int dmin = 2 * (int)fmin(fmax(lv, 0.0), 15.0);
     if (m > kLeaf) {
         VX_CHECK(hipMemsetAsync(t.bstart.p, 0, sizeof(int32_t) * m, st));
         hipLaunchKernelGGL(k_radix, dim3(gm), dim3(256), 0, st, t.keys_alt.p, m, t.rlo.p, t.rhi.p, t.rleft.p,
                            t.rright.p);
-        hipLaunchKernelGGL(k_bucket_mark, dim3(gm), dim3(256), 0, st, t.rlo.p, t.rhi.p, t.rleft.p, t.rright.p, m,
-                           t.bstart.p, t.upper.p);
+        hipLaunchKernelGGL(k_bucket_mark, dim3(gm), dim3(256), 0, st, t.keys_alt.p, t.rlo.p, t.rhi.p, t.rleft.p,
+                           t.rright.p, m, dmin, t.bstart.p, t.upper.p);
         size_t b1 = 0, b2 = 0;
         VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, t.bstart.p, t.bscan.p, (int)m, st));
         VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, t.upper.p, t.uscan.p, (int)(m - 1), st));
