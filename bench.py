@@ -59,7 +59,7 @@ def make_case(prec, n, P=1000, dt=0.005, tree_min_n=TREE_MIN_N):
     return c, (x, y, g), prm
 
 
-def run_path(prec, n, steps, warmup, M=None, tree_min_n=TREE_MIN_N):
+def run_path(prec, n, steps, warmup, M=None, tree_min_n=TREE_MIN_N, calibrate=False):
     c, (x, y, g), prm = make_case(prec, n, tree_min_n=tree_min_n)
     if M is None:
         c.build_matrix()
@@ -67,6 +67,12 @@ def run_path(prec, n, steps, warmup, M=None, tree_min_n=TREE_MIN_N):
     else:
         c.set_matrix(M)
     c.set_state(x, y, g)
+    if calibrate:
+        # a kernel of known traffic for HBM counter calibration: the mixed
+        # right-hand side packs the n sources to FP32 (k_pack_f32 reads
+        # 3 x 8 B and writes 3 x 4 B per vortex, all coalesced)
+        log(f"[bench] calibration: k_pack_f32 over {n} vortices ({24 * n} B read, {12 * n} B written)")
+        c.right_part(n, (1.0, 0.0))
     c.enable_timing(True)
     vinf = (1.0, 0.0)
     first = None
@@ -87,6 +93,7 @@ def run_path(prec, n, steps, warmup, M=None, tree_min_n=TREE_MIN_N):
     per = lambda k: (t1[k] - t0[k]) / max(1, steps)
     res = {
         "tree": tree, "bs_kernel_ms": per("bs_kernel_ms"), "tree_build_ms": per("tree_build_ms"),
+        "diff_kernel_ms": per("diff_kernel_ms"),
         "wall_ms": per("wall_ms"),
         "wall": wall, "n_start": n_start, "n_end": c.n,
         "bs_ms": (t1["bs_ms"] - t0["bs_ms"]) / max(1, t1["bs_launches"] - t0["bs_launches"]),
@@ -139,17 +146,22 @@ def cpu_baseline(n, sample_targets, threads):
     }
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    counter summary, if present (FETCH_SIZE + WRITE_SIZE, KiB, copy-calibrated)."""
-    p = os.path.join(ROOT, "profiles", "biot_savart_mixed_1M_counters.txt")
-    if not os.path.exists(p):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tree_step_mixed_1M_counters.txt")
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same
+    counter's reading on a kernel of known traffic), or None.  Lines read
+    `traffic_bytes_per_launch <kernel> <bytes>`."""
+    if not os.path.exists(TRAFFIC_FILE):
         return None
-    for line in open(p):
-        if line.startswith("traffic_bytes_per_launch"):
+    for line in open(TRAFFIC_FILE):
+        tok = line.split()
+        if len(tok) == 3 and tok[0] == "traffic_bytes_per_launch" and tok[1] == kernel:
             try:
-                return float(line.split()[1])
-            except (IndexError, ValueError):
+                return float(tok[2])
+            except ValueError:
                 return None
     return None
 
@@ -164,6 +176,8 @@ def main():
     ap.add_argument("--cpu-targets", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run one kernel of known HBM traffic before the steps (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -173,24 +187,25 @@ def main():
     from vortex_amd.solver import VX_PREC_F64, VX_PREC_MIXED
     n = args.n
     log(f"[bench] mixed path: N={n}, warmup={args.warmup}, steps={args.steps}")
-    mixed, M = run_path(VX_PREC_MIXED, n, args.steps, args.warmup)
+    mixed, M = run_path(VX_PREC_MIXED, n, args.steps, args.warmup, calibrate=args.calibrate)
     sps = args.steps / mixed["wall"]
     n_avg = 0.5 * (mixed["n_start"] + mixed["n_end"])
     s_bs = math.ceil(n_avg / 32) * 32
     pairs_eff = s_bs * (s_bs - 1)
     bs_stage_s = mixed["bs_ms"] / 1e3
     log(f"[bench] mixed: {sps:.4f} steps/s; speed {mixed['speed_ms']:.2f} ms (tree build "
-        f"{mixed['tree_build_ms']:.2f}, BS kernel {mixed['bs_kernel_ms']:.2f}, diffusion {mixed['diff_ms']:.2f}, "
+        f"{mixed['tree_build_ms']:.2f}, BS kernel {mixed['bs_kernel_ms']:.2f}, diffusion {mixed['diff_ms']:.2f} "
+        f"(kernel {mixed['diff_kernel_ms']:.2f}), "
         f"wall {mixed['wall_ms']:.2f}), creation {mixed['creation_ms']:.2f} ms, step {mixed['step_ms']:.2f} ms")
     tr = mixed["tree"] or {}
     log(f"[bench] tree: {tr}")
     # dominant kernel of the step and its algorithmic FLOPs per launch
     bs_flop = BS_FLOP_PER_PAIR * tr.get("bs_pairs", 0) + (8 * ORDER + 8) * tr.get("m2p_evals", 0)
     diff_flop = DIFF_FLOP_PER_PAIR * tr.get("diff_pairs", 0)
-    if mixed["bs_kernel_ms"] >= mixed["diff_ms"]:
-        kname, kms, kflop = "k_tree_bs<float,16>", mixed["bs_kernel_ms"], bs_flop
+    if mixed["bs_kernel_ms"] >= mixed["diff_kernel_ms"]:
+        kname, kms, kflop = "k_tree_bs", mixed["bs_kernel_ms"], bs_flop
     else:
-        kname, kms, kflop = "k_tree_diffusion<float>", mixed["diff_ms"], diff_flop
+        kname, kms, kflop = "k_tree_diffusion", mixed["diff_kernel_ms"], diff_flop
     achieved = kflop / (kms / 1e3) / 1e12 if kms > 0 else 0.0
 
     f64 = None
@@ -236,10 +251,14 @@ def main():
         "mixed_over_f64": (sps / (args.f64_steps / f64["wall"])) if f64 else None,
         "delta_cd_rel_one_step": dcd,
         "stage_ms": {k: mixed[k] for k in ("creation_ms", "speed_ms", "tree_build_ms", "bs_kernel_ms", "diff_ms",
-                                           "wall_ms", "step_ms")},
-        "roofline": {"kernel": kname, "bound": "valu", "achieved": achieved,
-                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": traffic_from_profiles(), "flop_per_launch": kflop,
+                                           "diff_kernel_ms", "wall_ms", "step_ms")},
+        # compute-bound FP32 work on the vector ALUs, whose peak equals the
+        # dense FP32 matrix peak (157.3 TFLOP/s); "mfma" is the contract's
+        # name for the compute roof
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32, "traffic": traffic_from_profiles(kname),
+                     "kernel": kname + (f"<float,{ORDER}>" if kname == "k_tree_bs" else "<float>"),
+                     "pipe": "VALU FP32", "flop_per_launch": kflop,
                      "avg_launch_ms": kms},
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }

@@ -156,6 +156,7 @@ typedef struct vx_timings {
     double bs_ms, diff_ms, wall_ms;             /* cumulative, per Speed kernel    */
     int64_t bs_launches;
     double tree_build_ms, bs_kernel_ms;         /* cumulative: tree build, BS kernel alone */
+    double diff_kernel_ms;                      /* cumulative: tree diffusion kernel alone */
 } vx_timings;
 int vx_case_timings(vx_case *c, vx_timings *t);
 int vx_case_enable_timing(vx_case *c, int on);

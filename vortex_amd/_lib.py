@@ -54,7 +54,8 @@ class Timings(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("creation_ms", "speed_ms", "step_ms", "bs_ms",
                                            "diff_ms", "wall_ms")] + [("bs_launches", C.c_int64),
                                                                       ("tree_build_ms", C.c_double),
-                                                                      ("bs_kernel_ms", C.c_double)]
+                                                                      ("bs_kernel_ms", C.c_double),
+                                                                      ("diff_kernel_ms", C.c_double)]
 
 
 _lib = None

@@ -117,7 +117,7 @@ struct vx_case {
     vx::TreeBufs *tree = nullptr;
     // timing
     bool timing = false;
-    hipEvent_t ev[16] = {};
+    hipEvent_t ev[18] = {};
     vx_timings tm{};
 };
 

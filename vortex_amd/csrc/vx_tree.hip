@@ -1041,12 +1041,14 @@ int launch_tree_diffusion(vx_case *c) {
     const double rroot = 0.5 * sqrt((rootbox.y - rootbox.x) * (rootbox.y - rootbox.x) +
                                     (rootbox.w - rootbox.z) * (rootbox.w - rootbox.z));
     const double rheavy = fmax(0.05 * rroot, 1e-300);
+    Timer tk(c, &c->tm.diff_kernel_ms, 8);
     if (mixed)
         hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb), dim3(256), 0, st, tv, cut, rheavy, c->prm.nu, thr, c->d.p,
                            c->u.p, c->v.p, t.cnt.p + 4, t.fb.p);
     else
         hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb), dim3(256), 0, st, tv, cut, rheavy, c->prm.nu, thr,
                            c->d.p, c->u.p, c->v.p, t.cnt.p + 4, t.fb.p);
+    tk.stop();
     VX_CHECK(hipGetLastError());
     unsigned long long hH = 0;
     VX_CHECK(hipMemcpyAsync(&hH, t.cnt.p + 6, sizeof(hH), hipMemcpyDeviceToHost, st));
