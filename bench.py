@@ -107,9 +107,9 @@ def run_path(prec, n, steps, warmup, M=None, tree_min_n=TREE_MIN_N, calibrate=Fa
     return res, M
 
 
-def one_step_cd(prec, n, M):
+def one_step_cd(prec, n, M, tree_min_n=TREE_MIN_N):
     """Drag of one full step from the synthetic state (Cd = Fx / (0.5 rho U^2 D), D = 1)."""
-    c, (x, y, g), prm = make_case(prec, n)
+    c, (x, y, g), prm = make_case(prec, n, tree_min_n=tree_min_n)
     c.set_matrix(M)
     c.set_state(x, y, g)
     f = c.time_step((1.0, 0.0))
@@ -176,6 +176,8 @@ def main():
     ap.add_argument("--cpu-targets", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--direct", action="store_true",
+                    help="direct O(N^2) sums at every N (the reference's algorithm) instead of the tree")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one kernel of known HBM traffic before the steps (for counter runs)")
     args = ap.parse_args()
@@ -187,7 +189,8 @@ def main():
     from vortex_amd.solver import VX_PREC_F64, VX_PREC_MIXED
     n = args.n
     log(f"[bench] mixed path: N={n}, warmup={args.warmup}, steps={args.steps}")
-    mixed, M = run_path(VX_PREC_MIXED, n, args.steps, args.warmup, calibrate=args.calibrate)
+    tmin = 0 if args.direct else TREE_MIN_N
+    mixed, M = run_path(VX_PREC_MIXED, n, args.steps, args.warmup, tree_min_n=tmin, calibrate=args.calibrate)
     sps = args.steps / mixed["wall"]
     n_avg = 0.5 * (mixed["n_start"] + mixed["n_end"])
     s_bs = math.ceil(n_avg / 32) * 32
@@ -202,7 +205,14 @@ def main():
     # dominant kernel of the step and its algorithmic FLOPs per launch
     bs_flop = BS_FLOP_PER_PAIR * tr.get("bs_pairs", 0) + (8 * ORDER + 8) * tr.get("m2p_evals", 0)
     diff_flop = DIFF_FLOP_PER_PAIR * tr.get("diff_pairs", 0)
-    if mixed["bs_kernel_ms"] >= mixed["diff_kernel_ms"]:
+    if args.direct:
+        # every target against every source of the padded count s
+        bs_flop = diff_flop = None
+        kname, kms = "k_biot_savart", mixed["bs_ms"]
+        kflop = BS_FLOP_PER_PAIR * s_bs * s_bs
+        if mixed["diff_ms"] > kms:
+            kname, kms, kflop = "k_diffusion", mixed["diff_ms"], DIFF_FLOP_PER_PAIR * s_bs * s_bs
+    elif mixed["bs_kernel_ms"] >= mixed["diff_kernel_ms"]:
         kname, kms, kflop = "k_tree_bs", mixed["bs_kernel_ms"], bs_flop
     else:
         kname, kms, kflop = "k_tree_diffusion", mixed["diff_kernel_ms"], diff_flop
@@ -212,11 +222,11 @@ def main():
     dcd = None
     if not args.no_f64 and args.f64_steps > 0:
         log(f"[bench] FP64 path: steps={args.f64_steps}")
-        f64, _ = run_path(VX_PREC_F64, n, args.f64_steps, 1, M=M)
+        f64, _ = run_path(VX_PREC_F64, n, args.f64_steps, 1, M=M, tree_min_n=tmin)
         log(f"[bench] f64: {args.f64_steps / f64['wall']:.4f} steps/s; speed {f64['speed_ms']:.2f} ms, "
             f"BS kernel {f64['bs_kernel_ms']:.2f} ms, diffusion {f64['diff_ms']:.2f} ms")
-        cd_m = one_step_cd(VX_PREC_MIXED, n, M)
-        cd_f = one_step_cd(VX_PREC_F64, n, M)
+        cd_m = one_step_cd(VX_PREC_MIXED, n, M, tmin)
+        cd_f = one_step_cd(VX_PREC_F64, n, M, tmin)
         dcd = abs(cd_m - cd_f) / max(abs(cd_f), 1e-300)
         log(f"[bench] one-step Cd mixed {cd_m:.9g} f64 {cd_f:.9g} rel diff {dcd:.3g}")
 
@@ -228,7 +238,8 @@ def main():
         log(f"[bench] cpu: {cpu['value']:.3g} steps/s ({cpu['pairs_per_s']:.3g} BS pairs/s)")
 
     out = {
-        "metric": "time steps/s at N=1M (mixed precision; Morton tree above 64k vortices)",
+        "metric": ("time steps/s at N=1M (mixed precision; direct sums)" if args.direct else
+                   "time steps/s at N=1M (mixed precision; Morton tree above 64k vortices)"),
         "value": sps,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -240,13 +251,15 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+fp64",
         "data": "synthetic (uniform wake, seed 1)",
-        "config": {"workload": "cylinder P=1000, Re=1000, dt=0.005, eps=pi/1000, N=2^20 wake; "
-                               "tree leaf 64, order 16, theta 0.5",
+        "config": {"workload": "cylinder P=1000, Re=1000, dt=0.005, eps=pi/1000, N=2^20 wake; " +
+                               ("direct O(N^2) sums" if args.direct else "tree leaf 64, order 16, theta 0.5"),
                    "n_vortices": int(n_avg), "panels": 1000, "parallelism": "single GPU"},
         "pair_interactions_per_s": pairs_eff / bs_stage_s,
-        "pair_interactions_kind": "effective N(N-1) / Biot-Savart stage time (tree build included)",
-        "evaluated": {"bs_pairs": tr.get("bs_pairs"), "m2p_evals": tr.get("m2p_evals"),
-                      "diff_pairs": tr.get("diff_pairs"), "nearest3_fallbacks": tr.get("fallback")},
+        "pair_interactions_kind": ("N(N-1) / Biot-Savart stage time" if args.direct else
+                                   "effective N(N-1) / Biot-Savart stage time (tree build included)"),
+        "evaluated": ({"bs_pairs": s_bs * s_bs, "diff_pairs": s_bs * s_bs} if args.direct else
+                      {"bs_pairs": tr.get("bs_pairs"), "m2p_evals": tr.get("m2p_evals"),
+                       "diff_pairs": tr.get("diff_pairs"), "nearest3_fallbacks": tr.get("fallback")}),
         "f64_steps_per_s": (args.f64_steps / f64["wall"]) if f64 else None,
         "mixed_over_f64": (sps / (args.f64_steps / f64["wall"])) if f64 else None,
         "delta_cd_rel_one_step": dcd,
@@ -257,7 +270,8 @@ def main():
         # name for the compute roof
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": traffic_from_profiles(kname),
-                     "kernel": kname + (f"<float,{ORDER}>" if kname == "k_tree_bs" else "<float>"),
+                     "kernel": kname + {"k_tree_bs": f"<float,{ORDER}>", "k_tree_diffusion": "<float>"}.get(
+                         kname, "<float,4>"),
                      "pipe": "VALU FP32", "flop_per_launch": kflop,
                      "avg_launch_ms": kms},
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
