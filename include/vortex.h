@@ -80,7 +80,7 @@ typedef struct vx_params {
     int32_t precision;                          /* VX_PREC_*                     */
     int32_t tree_min_n;                         /* Morton-tree velocities at N >= this (0: never) */
     double  theta;                              /* tree opening angle (0: 0.5)   */
-    double  diff_cut;                           /* diffusion cut-off / d_i (0: 20 mixed, 30 FP64) */
+    double  diff_cut;                           /* diffusion cut-off / d_i (0: 18 mixed, 30 FP64) */
     int32_t order;                              /* multipole terms, 16 or 32 (0: 16) */
     int32_t flags;                              /* VX_FLAG_*                     */
 } vx_params;
@@ -142,7 +142,7 @@ typedef struct vx_tree_info {
     int64_t bs_pairs, m2p_evals;     /* pairwise and multipole evaluations, last BS   */
     int64_t diff_pairs;              /* pairwise evaluations, last diffusion          */
     int32_t order;
-    int32_t heavy;                   /* diffusion targets summed by the split kernel  */
+    int32_t heavy;                   /* diffusion targets with a domain-scale radius  */
 } vx_tree_info;
 int vx_tree_stats(vx_case *c, vx_tree_info *info);
 int vx_step_move(vx_case *c, double out4[4]);

@@ -52,7 +52,7 @@ struct TreeBufs {
     DBuf<unsigned long long> cnt;
     DBuf<unsigned char> tmp;
     int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
-    int64_t heavy = 0;                     // diffusion targets of the split direct kernel
+    int64_t heavy = 0;                     // diffusion targets summed by k_diff_far
     int32_t root = 0;
     int order = 16;
 };
@@ -485,6 +485,15 @@ __device__ __forceinline__ void stage_leaf(Src4T<T> *lds, int lane, int64_t j0, 
 }
 
 // ---- Biot-Savart + nearest-3 --------------------------------------------------
+// Work-groups are dispatched to the 8 XCDs in turn (work-group b to XCD b % 8).
+// Each XCD gets a contiguous eighth of the Morton-ordered buckets instead, so
+// the source buckets its waves re-read stay within one region and its L2.  The
+// grid is padded to a multiple of 8 work-groups.
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t q = (int64_t)(gridDim.x >> 3);
+    return (int64_t)(blockIdx.x & 7u) * q + (int64_t)(blockIdx.x >> 3);
+}
+
 template <typename T, int ORDER>
 __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
                                                  double *__restrict__ u, double *__restrict__ v,
@@ -496,7 +505,7 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
-    const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t leaf = xcd_block() * 4 + (threadIdx.x >> 6);
     if (leaf >= t.nleaf) return;
     const int64_t ti = (int64_t)t.first[leaf] + lane;
     const int nvalid = t.count[leaf];
@@ -670,54 +679,111 @@ __global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ f
 }
 
 // ---- diffusion --------------------------------------------------------------
+// Diffusion pair terms (src/kernel.cu:1011-1019): s1 += g e, (s2x, s2y) +=
+// g e (dx, dy) / r with e = exp(-r / d), for r above the threshold.  FP32:
+// kd = -log2(e) / d and a below-threshold r^2 is pushed to 1e30, whose
+// weight underflows to zero; FP64: kd = 1 / d.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double rheavy, double nu, double thr,
-                                                        const double *__restrict__ dd, double *__restrict__ u,
-                                                        double *__restrict__ v, unsigned long long *__restrict__ npair,
-                                                        int32_t *__restrict__ heavy) {
+struct DiffPair;
+template <>
+struct DiffPair<float> {
+    __device__ static __forceinline__ void add(float dx, float dy, float g, float kd, float thr2, float &s2x,
+                                               float &s2y, float &s1) {
+        const float r2 = dx * dx + dy * dy;
+        const float r2c = r2 > thr2 ? r2 : 1e30f;
+        const float rs = __builtin_amdgcn_rsqf(r2c);
+        const float e = __builtin_amdgcn_exp2f(r2c * rs * kd);
+        const float ge = g * e;
+        const float w = ge * rs;
+        s2x += w * dx;
+        s2y += w * dy;
+        s1 += ge;
+    }
+};
+template <>
+struct DiffPair<double> {
+    __device__ static __forceinline__ void add(double dx, double dy, double g, double kd, double thr,
+                                               double &s2x, double &s2y, double &s1) {
+        const double dist = sqrt(dx * dx + dy * dy);
+        if (dist > thr) {
+            const double w = g / dist;
+            const double e = exp(-dist * kd);
+            s2x += w * dx * e;
+            s2y += w * dy * e;
+            s1 += g * e;
+        }
+    }
+};
+
+// Targets are summed over the sources within cut * d_i of them (the omitted
+// weight is bounded by (1 + cut) e^{-cut} of the near sum).  One wave per
+// bucket: a node is visited when the box around the bucket within the
+// largest radius of its lanes reaches it and, lane by lane, when some lane's
+// own radius reaches it.  A lane whose radius exceeds rbig is listed for
+// k_diff_far instead and does not widen the wave's radius.  cnt[0] +=
+// evaluated pairs, cnt[1] stack overflows, cnt[2] the list length.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double rbig, double nu,
+                                                        double thr, const double *__restrict__ dd,
+                                                        double *__restrict__ u, double *__restrict__ v,
+                                                        unsigned long long *__restrict__ cnt,
+                                                        int32_t *__restrict__ lists) {
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
-    const int64_t leaf = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t leaf = xcd_block() * 4 + (threadIdx.x >> 6);
     if (leaf >= t.nleaf) return;
     const int64_t ti = (int64_t)t.first[leaf] + lane;
     const int nvalid = t.count[leaf];
     const bool ok = lane < nvalid;
     const int32_t i = ok ? t.perm[ti] : 0;
     const double di = ok ? dd[i] : 0.0;
-    // a target whose cut-off radius spans a large part of the domain is summed
-    // by the split direct kernel instead of holding its whole wave
-    const bool hv = ok && cut * di > rheavy;
-    if (hv) heavy[atomicAdd(npair + 2, 1ull)] = (int32_t)ti;
-    const double R = cut * wave_max(hv ? 0.0 : di);
+    const bool far = ok && cut * di > rbig;
+    if (far) lists[atomicAdd(cnt + 2, 1ull)] = (int32_t)ti;
+    const bool mine = ok && !far;
+    const double R = cut * wave_max(mine ? di : 0.0);
     const double R2 = R * R;
     const double cAx = t.ncx[leaf], cAy = t.ncy[leaf];
     const double4 bA = t.nbb[leaf];
-    T ax, ay, kd;
+    T ax, ay, kd, ri2;
     if constexpr (F32) {
         ax = ok ? t.fx[ti] : 0.f;
         ay = ok ? t.fy[ti] : 0.f;
         kd = (float)(-1.4426950408889634 / (ok ? di : 1.0));
+        ri2 = mine ? (float)(cut * di * cut * di * (1.0 + 1e-4)) : -1.f;
     } else {
         ax = ok ? t.sx[ti] : cAx;
         ay = ok ? t.sy[ti] : cAy;
         kd = 1.0 / (ok ? di : 1.0);
+        ri2 = mine ? cut * di * cut * di : -1.0;
     }
-    const T thr2 = (T)(thr * thr);
+    const T thr2 = F32 ? (T)(thr * thr) : (T)thr;
     int64_t npairs = 0;
     double a2x = 0.0, a2y = 0.0, a1 = 0.0;
     LaneStack st;
-    st.push(lane, t.root);
+    if (R > 0.0) st.push(lane, t.root);
     while (st.sp > 0) {
         const int node = st.pop();
         const double4 b = t.nbb[node];
         const double ex = fmax(0.0, fmax(b.x - bA.y, bA.x - b.y));
         const double ey = fmax(0.0, fmax(b.z - bA.w, bA.z - b.w));
         if (ex * ex + ey * ey >= R2) continue;
+        // lane test: distance from this lane's target to the node box
+        T lx, ly;
+        if constexpr (F32) {
+            const float x0 = (float)(b.x - cAx), x1 = (float)(b.y - cAx);
+            const float y0 = (float)(b.z - cAy), y1 = (float)(b.w - cAy);
+            lx = fmaxf(0.f, fmaxf(x0 - ax, ax - x1));
+            ly = fmaxf(0.f, fmaxf(y0 - ay, ay - y1));
+        } else {
+            lx = fmax(0.0, fmax(b.x - ax, ax - b.y));
+            ly = fmax(0.0, fmax(b.z - ay, ay - b.w));
+        }
+        if (__ballot(lx * lx + ly * ly < ri2) == 0) continue;
         if (t.child0[node] < 0) {
             const int64_t j0 = t.first[node];
-            const int cnt = t.count[node];
+            const int cnt_j = t.count[node];
             T ox, oy;
             if constexpr (F32) {
                 ox = ax + (float)(cAx - t.ncx[node]);
@@ -726,33 +792,12 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
                 ox = ax;
                 oy = ay;
             }
-            stage_leaf<T>(lds, lane, j0, cnt, t);
+            stage_leaf<T>(lds, lane, j0, cnt_j, t);
             T s2x = 0, s2y = 0, s1 = 0;
-            npairs += cnt;
-            for (int jj = 0; jj < cnt; ++jj) {
+            npairs += cnt_j;
+            for (int jj = 0; jj < cnt_j; ++jj) {
                 const Src4T<T> q = lds[jj];
-                const T dx = ox - q.x, dy = oy - q.y;
-                const T r2 = dx * dx + dy * dy;
-                if constexpr (F32) {
-                    const float r2c = r2 > thr2 ? r2 : 1e30f;
-                    const float rs = __builtin_amdgcn_rsqf(r2c);
-                    const float dist = r2c * rs;
-                    const float e = __builtin_amdgcn_exp2f(dist * kd);
-                    const float ge = q.g * e;
-                    const float w = ge * rs;
-                    s2x += w * dx;
-                    s2y += w * dy;
-                    s1 += ge;
-                } else {
-                    const double dist = sqrt(r2);
-                    if (dist > thr) {
-                        const double w = q.g / dist;
-                        const double e = exp(-dist * kd);
-                        s2x += w * dx * e;
-                        s2y += w * dy * e;
-                        s1 += q.g * e;
-                    }
-                }
+                DiffPair<T>::add(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
             }
             __builtin_amdgcn_wave_barrier();
             a2x += (double)s2x;
@@ -763,81 +808,131 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
             st.push(lane, t.child0[node]);
         }
     }
+    const int nmine = __popcll(__ballot(mine));
     if (lane == 0) {
-        atomicAdd(npair, (unsigned long long)(npairs * nvalid));
-        if (st.overflow) atomicAdd(npair + 1, 1ull);
+        atomicAdd(cnt, (unsigned long long)(npairs * nmine));
+        if (st.overflow) atomicAdd(cnt + 1, 1ull);
     }
-    if (!ok || hv) return;
+    if (!mine) return;
     if (fabs(a1) > 1e-5) {
         u[i] += nu * a2x / (a1 * di);
         v[i] += nu * a2y / (a1 * di);
     }
 }
 
-// Diffusion of the heavy targets: every target against all tree particles,
-// split over source chunks (grid y), pair arithmetic on target-relative
-// differences taken in FP64; chunk partials summed in a fixed order.
+// Targets whose radius exceeds rbig, listed by k_tree_diffusion: each wave
+// takes one target and one chunk of 64 consecutive buckets (Morton order),
+// tests the 64 bucket boxes at once, then sums the buckets in range with one
+// source per lane, loading the next bucket while summing the current one.
+// Partial sums go to part[(chunk * 3 + q) * H + k].
 template <typename T>
-__global__ __launch_bounds__(256) void k_diff_heavy_partial(TreeDev t, const int32_t *__restrict__ heavy, int64_t H,
-                                                            int64_t chunk, const double *__restrict__ dd,
-                                                            double thr, double *__restrict__ part) {
-    __shared__ double sx[256], sy[256], sg[256];
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t c = blockIdx.y;
-    const int64_t lo = c * chunk, hi = min(t.m, lo + chunk);
-    const bool ok = k < H;
-    const int64_t ti = ok ? heavy[k] : 0;
-    const double ax = ok ? t.sx[ti] : 0.0, ay = ok ? t.sy[ti] : 0.0;
-    const double di = ok ? dd[t.perm[ti]] : 1.0;
-    double a2x = 0.0, a2y = 0.0, a1 = 0.0;
-    for (int64_t base = lo; base < hi; base += 256) {
-        __syncthreads();
-        const int64_t j = base + threadIdx.x;
-        sx[threadIdx.x] = j < hi ? t.sx[j] : 0.0;
-        sy[threadIdx.x] = j < hi ? t.sy[j] : 0.0;
-        sg[threadIdx.x] = j < hi ? t.sg[j] : 0.0;
-        __syncthreads();
-        const int cnt = (int)min((int64_t)256, hi - base);
-        T s2x = 0, s2y = 0, s1 = 0;
-        for (int jj = 0; jj < cnt; ++jj) {
-            const T dx = (T)(ax - sx[jj]), dy = (T)(ay - sy[jj]);
-            const T dist = sqrt(dx * dx + dy * dy);
-            if (dist > (T)thr) {
-                const T g = (T)sg[jj];
-                const T e = exp(-dist / (T)di);
-                const T w = g / dist;
-                s2x += w * dx * e;
-                s2y += w * dy * e;
-                s1 += g * e;
-            }
+__device__ __forceinline__ void far_load(const TreeDev &t, int b, int lane, double ax, double ay, T &dx, T &dy,
+                                         T &g) {
+    const int64_t j0 = t.first[b];
+    const int cnt = t.count[b];
+    if (lane < cnt) {
+        if constexpr (sizeof(T) == 4) {
+            dx = (float)(ax - t.ncx[b]) - t.fx[j0 + lane];
+            dy = (float)(ay - t.ncy[b]) - t.fy[j0 + lane];
+            g = t.fg[j0 + lane];
+        } else {
+            dx = ax - t.sx[j0 + lane];
+            dy = ay - t.sy[j0 + lane];
+            g = t.sg[j0 + lane];
         }
-        a2x += (double)s2x;
-        a2y += (double)s2y;
-        a1 += (double)s1;
-    }
-    if (ok) {
-        part[(c * 3 + 0) * H + k] = a2x;
-        part[(c * 3 + 1) * H + k] = a2y;
-        part[(c * 3 + 2) * H + k] = a1;
+    } else {
+        dx = T(1);  // outside the bucket: zero circulation
+        dy = T(1);
+        g = T(0);
     }
 }
 
-__global__ void k_diff_heavy_final(TreeDev t, const int32_t *__restrict__ heavy, int64_t H, int64_t nchunk,
-                                   const double *__restrict__ part, const double *__restrict__ dd, double nu,
-                                   double *__restrict__ u, double *__restrict__ v) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= H) return;
+template <typename T>
+__global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const int32_t *__restrict__ list, int64_t H,
+                                                  int64_t nchunk, double cut, double thr,
+                                                  const double *__restrict__ dd, double *__restrict__ part,
+                                                  unsigned long long *__restrict__ cnt) {
+    constexpr bool F32 = sizeof(T) == 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t k = blockIdx.x;
+    const int64_t ch = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (ch >= nchunk) return;
+    const int64_t ti = list[k];
+    const double di = dd[t.perm[ti]];
+    const double R2 = (cut * di) * (cut * di);
+    const double ax = t.sx[ti], ay = t.sy[ti];
+    const T kd = F32 ? (T)(-1.4426950408889634 / di) : (T)(1.0 / di);
+    const T thr2 = F32 ? (T)(thr * thr) : (T)thr;
+    const int64_t b = ch * 64 + lane;
+    const bool need = b < t.nleaf && box_d2(t.nbb[b], ax, ay) < R2;
+    uint64_t mask = __ballot(need);
     double a2x = 0.0, a2y = 0.0, a1 = 0.0;
-    for (int64_t c = 0; c < nchunk; ++c) {
-        a2x += part[(c * 3 + 0) * H + k];
-        a2y += part[(c * 3 + 1) * H + k];
-        a1 += part[(c * 3 + 2) * H + k];
+    int64_t npairs = 0;
+    if (mask) {
+        int bc = (int)(ch * 64) + __builtin_ctzll(mask);
+        mask &= mask - 1;
+        T dx, dy, g;
+        far_load<T>(t, bc, lane, ax, ay, dx, dy, g);
+        npairs += t.count[bc];
+        for (;;) {
+            const bool more = mask != 0;
+            T nx = T(1), ny = T(1), ng = T(0);
+            if (more) {
+                const int bn = (int)(ch * 64) + __builtin_ctzll(mask);
+                mask &= mask - 1;
+                far_load<T>(t, bn, lane, ax, ay, nx, ny, ng);
+                npairs += t.count[bn];
+            }
+            T s2x = 0, s2y = 0, s1 = 0;
+            DiffPair<T>::add(dx, dy, g, kd, thr2, s2x, s2y, s1);
+            a2x += (double)s2x;
+            a2y += (double)s2y;
+            a1 += (double)s1;
+            if (!more) break;
+            dx = nx;
+            dy = ny;
+            g = ng;
+        }
     }
-    const int32_t i = t.perm[heavy[k]];
-    if (fabs(a1) > 1e-5) {
+    a2x = wave_sum(a2x);
+    a2y = wave_sum(a2y);
+    a1 = wave_sum(a1);
+    if (lane != 0) return;
+    if (npairs) atomicAdd(cnt, (unsigned long long)npairs);
+    part[(ch * 3 + 0) * H + k] = a2x;
+    part[(ch * 3 + 1) * H + k] = a2y;
+    part[(ch * 3 + 2) * H + k] = a1;
+}
+
+// One work-group per listed target: the chunk partials summed in a fixed
+// order (strided per thread, then a tree in LDS), then the velocity update.
+__global__ __launch_bounds__(256) void k_diff_far_final(TreeDev t, const int32_t *__restrict__ list, int64_t H,
+                                                        int64_t nchunk, const double *__restrict__ part,
+                                                        const double *__restrict__ dd, double nu,
+                                                        double *__restrict__ u, double *__restrict__ v) {
+    __shared__ double r[3][256];
+    const int64_t k = blockIdx.x;
+    double a2x = 0.0, a2y = 0.0, a1 = 0.0;
+    for (int64_t ch = threadIdx.x; ch < nchunk; ch += 256) {
+        a2x += part[(ch * 3 + 0) * H + k];
+        a2y += part[(ch * 3 + 1) * H + k];
+        a1 += part[(ch * 3 + 2) * H + k];
+    }
+    r[0][threadIdx.x] = a2x;
+    r[1][threadIdx.x] = a2y;
+    r[2][threadIdx.x] = a1;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int q = 0; q < 3; ++q) r[q][threadIdx.x] += r[q][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const int32_t i = t.perm[list[k]];
+    if (fabs(r[2][0]) > 1e-5) {
         const double di = dd[i];
-        u[i] += nu * a2x / (a1 * di);
-        v[i] += nu * a2y / (a1 * di);
+        u[i] += nu * r[0][0] / (r[2][0] * di);
+        v[i] += nu * r[1][0] / (r[2][0] * di);
     }
 }
 
@@ -995,7 +1090,7 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
                            vinf_x, vinf_y, c->u.p, c->v.p, c->d.p);
     if (t.m == 0) return 0;
     const TreeDev tv = view(t);
-    const unsigned lb = (unsigned)((t.nleaf + 3) / 4);
+    const unsigned lb = (unsigned)((t.nleaf + 3) / 4 + 7) / 8 * 8;  // padded for xcd_block()
     const double th = theta_of(c), eps = c->prm.ve_size;
     unsigned long long *nfb = t.cnt.p + 1;
     const bool mixed = c->prm.precision == VX_PREC_MIXED;
@@ -1030,46 +1125,45 @@ int launch_tree_diffusion(vx_case *c) {
     hipStream_t st = c->stream;
     const TreeDev tv = view(t);
     const unsigned lb = (unsigned)((t.nleaf + 3) / 4);
+    const unsigned lb8 = (lb + 7) / 8 * 8;
     // omitted weights total at most (1 + c) e^{-c} of the near sum's scale:
-    // 4e-8 at c = 20 (below FP32 term error), 3e-12 at c = 30
+    // 3e-7 at c = 18 (the FP32 rounding level of the sums), 3e-12 at c = 30
     const bool mixed = c->prm.precision == VX_PREC_MIXED;
-    const double cut = c->prm.diff_cut > 0 ? c->prm.diff_cut : (mixed ? 20.0 : 30.0);
+    const double cut = c->prm.diff_cut > 0 ? c->prm.diff_cut : (mixed ? 18.0 : 30.0);
     const double thr = 0.001 * c->prm.ve_size;
     double4 rootbox;
     VX_CHECK(hipMemcpyAsync(&rootbox, t.nbb.p + t.root, sizeof(double4), hipMemcpyDeviceToHost, st));
     VX_CHECK(hipStreamSynchronize(st));
     const double rroot = 0.5 * sqrt((rootbox.y - rootbox.x) * (rootbox.y - rootbox.x) +
                                     (rootbox.w - rootbox.z) * (rootbox.w - rootbox.z));
-    const double rheavy = fmax(0.05 * rroot, 1e-300);
-    Timer tk(c, &c->tm.diff_kernel_ms, 8);
-    if (mixed)
-        hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb), dim3(256), 0, st, tv, cut, rheavy, c->prm.nu, thr, c->d.p,
-                           c->u.p, c->v.p, t.cnt.p + 4, t.fb.p);
-    else
-        hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb), dim3(256), 0, st, tv, cut, rheavy, c->prm.nu, thr,
-                           c->d.p, c->u.p, c->v.p, t.cnt.p + 4, t.fb.p);
-    tk.stop();
+    const double rbig = fmax(0.05 * rroot, 1e-300);
+    unsigned long long *cnt = t.cnt.p + 4;  // pairs, overflow, far targets
+    {
+        Timer tk(c, &c->tm.diff_kernel_ms, 8);
+        if (mixed)
+            hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb8), dim3(256), 0, st, tv, cut, rbig, c->prm.nu, thr,
+                               c->d.p, c->u.p, c->v.p, cnt, t.fb.p);
+        else
+            hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb8), dim3(256), 0, st, tv, cut, rbig, c->prm.nu, thr,
+                               c->d.p, c->u.p, c->v.p, cnt, t.fb.p);
+    }
     VX_CHECK(hipGetLastError());
     unsigned long long hH = 0;
-    VX_CHECK(hipMemcpyAsync(&hH, t.cnt.p + 6, sizeof(hH), hipMemcpyDeviceToHost, st));
+    VX_CHECK(hipMemcpyAsync(&hH, cnt + 2, sizeof(hH), hipMemcpyDeviceToHost, st));
     VX_CHECK(hipStreamSynchronize(st));
     const int64_t H = (int64_t)hH;
     t.heavy = H;
     if (H == 0) return 0;
-    const int64_t tb = (H + 255) / 256;
-    int64_t want = (4096 + tb - 1) / tb;
-    const int64_t tiles = (t.m + 255) / 256;
-    if (want > tiles) want = tiles;
-    const int64_t chunk = ((tiles + want - 1) / want) * 256;
-    const int64_t nchunk = (t.m + chunk - 1) / chunk;
-    if (t.hpart.ensure(3 * nchunk * H, st)) return 1;
+    const int64_t nchunk = (t.nleaf + 63) / 64;
+    if (t.hpart.ensure((size_t)(3 * nchunk * H), st)) return 1;
+    const dim3 g((unsigned)H, (unsigned)((nchunk + 3) / 4));
     if (mixed)
-        hipLaunchKernelGGL(k_diff_heavy_partial<float>, dim3((unsigned)tb, (unsigned)nchunk), dim3(256), 0, st, tv,
-                           t.fb.p, H, chunk, c->d.p, thr, t.hpart.p);
+        hipLaunchKernelGGL(k_diff_far<float>, g, dim3(256), 0, st, tv, t.fb.p, H, nchunk, cut, thr, c->d.p,
+                           t.hpart.p, cnt);
     else
-        hipLaunchKernelGGL(k_diff_heavy_partial<double>, dim3((unsigned)tb, (unsigned)nchunk), dim3(256), 0, st, tv,
-                           t.fb.p, H, chunk, c->d.p, thr, t.hpart.p);
-    hipLaunchKernelGGL(k_diff_heavy_final, dim3((unsigned)tb), dim3(256), 0, st, tv, t.fb.p, H, nchunk, t.hpart.p,
+        hipLaunchKernelGGL(k_diff_far<double>, g, dim3(256), 0, st, tv, t.fb.p, H, nchunk, cut, thr, c->d.p,
+                           t.hpart.p, cnt);
+    hipLaunchKernelGGL(k_diff_far_final, dim3((unsigned)H), dim3(256), 0, st, tv, t.fb.p, H, nchunk, t.hpart.p,
                        c->d.p, c->prm.nu, c->u.p, c->v.p);
     VX_CHECK(hipGetLastError());
     return 0;
