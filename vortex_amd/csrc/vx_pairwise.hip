@@ -281,15 +281,38 @@ __global__ __launch_bounds__(kTile) void k_right_part_partial(
     }
 }
 
-__global__ void k_right_part_final(const double *__restrict__ part, int64_t nchunk, PanelsSoA pan,
-                                   double vinf_x, double vinf_y, double *__restrict__ R) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= pan.P) return;
-    double y0 = 0.0, y1 = 0.0;
-    for (int64_t c = 0; c < nchunk; ++c) {
-        y0 += part[(c * 2 + 0) * pan.P + i];
-        y1 += part[(c * 2 + 1) * pan.P + i];
+// Sum of the chunk partials part[(c * 2 + q) * m + i] for one output i per
+// work-group: a strided pass per thread, then a tree in LDS (fixed order).
+__device__ __forceinline__ void chunk_sum2(const double *__restrict__ part, int64_t nchunk, int64_t m, int64_t i,
+                                           double &a, double &b) {
+    __shared__ double r[2][kTile];
+    double x0 = 0.0, x1 = 0.0;
+    for (int64_t c = threadIdx.x; c < nchunk; c += kTile) {
+        x0 += part[(c * 2 + 0) * m + i];
+        x1 += part[(c * 2 + 1) * m + i];
     }
+    r[0][threadIdx.x] = x0;
+    r[1][threadIdx.x] = x1;
+    __syncthreads();
+    for (int st = kTile / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            r[0][threadIdx.x] += r[0][threadIdx.x + st];
+            r[1][threadIdx.x] += r[1][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    a = r[0][0];
+    b = r[1][0];
+}
+
+// One work-group per panel.
+__global__ __launch_bounds__(kTile) void k_right_part_final(const double *__restrict__ part, int64_t nchunk,
+                                                            PanelsSoA pan, double vinf_x, double vinf_y,
+                                                            double *__restrict__ R) {
+    const int64_t i = blockIdx.x;
+    double y0, y1;
+    chunk_sum2(part, nchunk, pan.P, i, y0, y1);
+    if (threadIdx.x != 0) return;
     const double d0 = pan.rx[i] - pan.lx[i], d1 = pan.ry[i] - pan.ly[i];
     const double mnog = 1 / (2 * kPi * (d0 * d0 + d1 * d1));
     R[i] = -((mnog * y0 + vinf_x) * pan.tx[i] + (mnog * y1 + vinf_y) * pan.ty[i]);
@@ -337,16 +360,14 @@ __global__ __launch_bounds__(kTile) void k_probe_partial(
     }
 }
 
-__global__ void k_probe_final(const double *__restrict__ part, int64_t nchunk, int64_t m,
-                              double vinf_x, double vinf_y, double *__restrict__ u,
-                              double *__restrict__ v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    double a = 0.0, b = 0.0;
-    for (int64_t c = 0; c < nchunk; ++c) {
-        a += part[(c * 2 + 0) * m + i];
-        b += part[(c * 2 + 1) * m + i];
-    }
+// One work-group per probe.
+__global__ __launch_bounds__(kTile) void k_probe_final(const double *__restrict__ part, int64_t nchunk, int64_t m,
+                                                       double vinf_x, double vinf_y, double *__restrict__ u,
+                                                       double *__restrict__ v) {
+    const int64_t i = blockIdx.x;
+    double a, b;
+    chunk_sum2(part, nchunk, m, i, a, b);
+    if (threadIdx.x != 0) return;
     u[i] = a / (2 * kPi) + vinf_x;
     v[i] = b / (2 * kPi) + vinf_y;
 }
@@ -420,7 +441,7 @@ int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
                            c->stream, c->x.p, c->y.p, c->g.p, n_src, chunk, c->pan, c->partial.p);
     }
     VX_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)tb), dim3(kTile), 0, c->stream, c->partial.p, nchunk,
+    hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)P), dim3(kTile), 0, c->stream, c->partial.p, nchunk,
                        c->pan, vinf_x, vinf_y, c->rp.p);
     VX_CHECK(hipGetLastError());
     return 0;
@@ -444,7 +465,7 @@ int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vin
                            c->partial.p);
     }
     VX_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_probe_final, dim3((unsigned)tb), dim3(kTile), 0, c->stream, c->partial.p, nchunk, m,
+    hipLaunchKernelGGL(k_probe_final, dim3((unsigned)m), dim3(kTile), 0, c->stream, c->partial.p, nchunk, m,
                        vinf_x, vinf_y, c->pu.p, c->pv.p);
     VX_CHECK(hipGetLastError());
     return 0;

@@ -24,48 +24,108 @@ struct StepConsts {
 
 // hitting(): src/kernel.cu:1357-1431.  Segment old -> new against every
 // panel (after the profile-box reject); the crossing nearest the old position
-// wins.  A crossing set with no panel nearer than 5 (the reference would read
-// panels[-1]) counts as no crossing.
-__device__ int hitting(const PanelsSoA &pan, const StepConsts &k, double x1, double y1, double x2, double y2) {
+// wins, the lowest panel index among equal distances.  A crossing set with no
+// panel nearer than 5 (the reference would read panels[-1]) counts as no
+// crossing.  Split in two: k_hit_candidates lists the moving vortices whose
+// segment box meets the profile box, then one wave per candidate tests the
+// panels lane-parallel and reduces (distance, index) to the same winner as
+// the sequential scan.
+__device__ __forceinline__ bool hit_box_reject(const StepConsts &k, double x1, double y1, double x2, double y2) {
+    return ((x1 < k.px_min) && (x2 < k.px_min)) || ((x1 > k.px_max) && (x2 > k.px_max)) ||
+           ((y1 < k.py_min) && (y2 < k.py_min)) || ((y1 > k.py_max) && (y2 > k.py_max));
+}
+
+// Squared distance from (x2, y2) to the crossing of the segment with panel
+// i, or -1 if they do not cross.
+__device__ __forceinline__ double panel_cross(const PanelsSoA &pan, int64_t i, double A, double B, double D,
+                                              double x1, double y1, double x2, double y2) {
     const double porog = 1e-12;
-    if (((x1 < k.px_min) && (x2 < k.px_min)) || ((x1 > k.px_max) && (x2 > k.px_max)) ||
-        ((y1 < k.py_min) && (y2 < k.py_min)) || ((y1 > k.py_max) && (y2 > k.py_max)))
-        return -1;
-    const double A = y2 - y1, B = x1 - x2, D = y1 * x2 - x1 * y2;
-    double minDist = 25.0;
-    int minN = -1;
-    for (int64_t i = 0; i < pan.P; ++i) {
-        const double lx = pan.lx[i], ly = pan.ly[i], rx = pan.rx[i], ry = pan.ry[i];
-        double r0 = A * lx + B * ly + D;
-        double r1 = A * rx + B * ry + D;
-        if (fabs(r0) < porog) r0 = 0.0;
-        if (fabs(r1) < porog) r1 = 0.0;
-        if (r0 * r1 <= 0) {
-            const double A1 = ry - ly, B1 = lx - rx, D1 = ly * rx - lx * ry;
-            double r2 = A1 * x1 + B1 * y1 + D1;
-            double r3 = A1 * x2 + B1 * y2 + D1;
-            if (fabs(r2) < porog) r2 = 0.0;
-            if (fabs(r3) < porog) r3 = 0.0;
-            if (r2 * r3 <= 0) {
-                const double den = A * B1 - B * A1;
-                const double X = (B * D1 - D * B1) / den;
-                const double Y = (A1 * D - D1 * A) / den;
-                const double dd2 = (x2 - X) * (x2 - X) + (y2 - Y) * (y2 - Y);
-                if (dd2 < minDist) {
-                    minDist = dd2;
-                    minN = (int)i;
-                }
-            }
+    const double lx = pan.lx[i], ly = pan.ly[i], rx = pan.rx[i], ry = pan.ry[i];
+    double r0 = A * lx + B * ly + D;
+    double r1 = A * rx + B * ry + D;
+    if (fabs(r0) < porog) r0 = 0.0;
+    if (fabs(r1) < porog) r1 = 0.0;
+    if (!(r0 * r1 <= 0)) return -1.0;
+    const double A1 = ry - ly, B1 = lx - rx, D1 = ly * rx - lx * ry;
+    double r2 = A1 * x1 + B1 * y1 + D1;
+    double r3 = A1 * x2 + B1 * y2 + D1;
+    if (fabs(r2) < porog) r2 = 0.0;
+    if (fabs(r3) < porog) r3 = 0.0;
+    if (!(r2 * r3 <= 0)) return -1.0;
+    const double den = A * B1 - B * A1;
+    const double X = (B * D1 - D * B1) / den;
+    const double Y = (A1 * D - D1 * A) / den;
+    return (x2 - X) * (x2 - X) + (y2 - Y) * (y2 - Y);
+}
+
+__global__ void k_set_u64(unsigned long long *__restrict__ p, unsigned long long v) { *p = v; }
+
+__global__ __launch_bounds__(256) void k_hit_candidates(const double *__restrict__ x, const double *__restrict__ y,
+                                                        const double *__restrict__ g, const double *__restrict__ u,
+                                                        const double *__restrict__ v, int64_t n, StepConsts k,
+                                                        int32_t *__restrict__ hitp, int32_t *__restrict__ cand,
+                                                        unsigned long long *__restrict__ ncand) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool c = false;
+    if (i < n) {
+        hitp[i] = -1;
+        const double xi = x[i], yi = y[i];
+        if (g[i] != 0) {
+            const double xn = xi + u[i] * k.dt;
+            const double yn = yi + v[i] * k.dt;
+            c = !hit_box_reject(k, xn, yn, xi, yi);
         }
     }
-    return minN;
+    const uint64_t m = __ballot(c);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == __builtin_ctzll(m)) base = atomicAdd(ncand, (unsigned long long)__popcll(m));
+    base = __shfl(base, __builtin_ctzll(m), 64);
+    if (c) cand[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)i;
+}
+
+// Grid-stride over the candidates, one wave each; the count stays on the device.
+__global__ __launch_bounds__(256) void k_hit_panels(const double *__restrict__ x, const double *__restrict__ y,
+                                                    const double *__restrict__ u, const double *__restrict__ v,
+                                                    PanelsSoA pan, StepConsts k, const int32_t *__restrict__ cand,
+                                                    const unsigned long long *__restrict__ ncand,
+                                                    int32_t *__restrict__ hitp) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    const int64_t total = (int64_t)*ncand;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total; t += nw) {
+        const int64_t i = cand[t];
+        const double x2 = x[i], y2 = y[i];
+        const double x1 = x2 + u[i] * k.dt, y1 = y2 + v[i] * k.dt;
+        const double A = y2 - y1, B = x1 - x2, D = y1 * x2 - x1 * y2;
+        double best = 25.0;
+        int bi = -1;
+        for (int64_t p = lane; p < pan.P; p += 64) {
+            const double dd2 = panel_cross(pan, p, A, B, D, x1, y1, x2, y2);
+            if (dd2 >= 0.0 && dd2 < best) {
+                best = dd2;
+                bi = (int)p;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o, 64);
+            const int oi = __shfl_xor(bi, o, 64);
+            if (oi >= 0 && (bi < 0 || ob < best || (ob == best && oi < bi))) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        if (lane == 0) hitp[i] = bi;
+    }
 }
 
 // ---- A9: src/kernel.cu:1090-1143 -------------------------------------------
 __global__ __launch_bounds__(256) void k_step_move(double *__restrict__ x, double *__restrict__ y,
                                                    double *__restrict__ g, const double *__restrict__ u,
                                                    const double *__restrict__ v, int64_t n, int64_t P,
-                                                   PanelsSoA pan, StepConsts k, double *__restrict__ part) {
+                                                   PanelsSoA pan, StepConsts k, const int32_t *__restrict__ hitp,
+                                                   double *__restrict__ part) {
     __shared__ double red[4][256];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double dg = 0.0, fx = 0.0, fy = 0.0, mm = 0.0;
@@ -79,7 +139,7 @@ __global__ __launch_bounds__(256) void k_step_move(double *__restrict__ x, doubl
         const double xn = xi + u[i] * k.dt;
         const double yn = yi + v[i] * k.dt;
         if (gi != 0) {
-            const int hp = hitting(pan, k, xn, yn, xi, yi);
+            const int hp = hitp[i];
             if (hp >= 0) {
                 const double cx = pan.cx[hp], cy = pan.cy[hp];
                 fx -= gi * (-cy);
@@ -139,10 +199,20 @@ int launch_step_move(vx_case *c, double out4[4]) {
     const int64_t n = c->n;
     const int64_t nb = n > 0 ? (n + 255) / 256 : 1;
     if (c->partial.ensure((size_t)(4 * nb), c->stream) || c->red.ensure(8, c->stream)) return 1;
+    if (c->misc.ensure((size_t)(n > 0 ? n : 1), c->stream) || c->lists.ensure((size_t)(n > 0 ? n : 1), c->stream) ||
+        c->dcount.ensure(6, c->stream))
+        return 1;
     const vx_params &p = c->prm;
     StepConsts k{p.dt, p.rc_x, p.rc_y, p.x_min, p.x_max, p.y_min, p.y_max, p.px_min, p.px_max, p.py_min, p.py_max};
+    int32_t *hitp = c->misc.p, *cand = c->lists.p;
+    unsigned long long *ncand = (unsigned long long *)(c->dcount.p + 4);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, c->stream, ncand, 0ull);
+    hipLaunchKernelGGL(k_hit_candidates, dim3((unsigned)nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->g.p, c->u.p,
+                       c->v.p, n, k, hitp, cand, ncand);
+    hipLaunchKernelGGL(k_hit_panels, dim3(1024), dim3(256), 0, c->stream, c->x.p, c->y.p, c->u.p, c->v.p, c->pan, k,
+                       cand, ncand, hitp);
     hipLaunchKernelGGL(k_step_move, dim3((unsigned)nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->g.p, c->u.p,
-                       c->v.p, n, c->P, c->pan, k, c->partial.p);
+                       c->v.p, n, c->P, c->pan, k, hitp, c->partial.p);
     hipLaunchKernelGGL(k_sum4, dim3(1), dim3(256), 0, c->stream, c->partial.p, nb, p.rho, p.dt, c->red.p);
     VX_CHECK(hipGetLastError());
     VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -260,10 +330,10 @@ int launch_compact(vx_case *c, int64_t *n_inout) {
 //   kind 1: g_i g_j < 0 and r_ij^2 < r2
 //   kind 2: g_i g_j > 0 and r_ij^2 < r2 and |g_i + g_j| < max_ve_g
 // Candidates come from a uniform grid of cell size slightly above sqrt(r2)
-// (so every pair inside the radius lies in neighbouring cells), built by
-// sorting (cell key, index) pairs; the 3x3 cell rows are found by binary
-// search in the sorted keys.  The minimum j does not depend on the order in
-// which candidates are visited.
+// (so every pair inside the radius lies in neighbouring cells).  Occupied
+// cells live in an open-addressing hash table (cell + 1 as key, 0 empty,
+// load factor at most 1/2) whose slots head a list of the cell's vortices.
+// The minimum j does not depend on the order of the lists.
 struct Grid {
     double x0, y0, inv_h;
     int64_t nx, ny;
@@ -277,30 +347,45 @@ __device__ __forceinline__ void cell_of(const Grid &G, double x, double y, int64
     cy = fy >= (double)(G.ny - 1) ? G.ny - 1 : (int64_t)fy;
 }
 
-__global__ void k_cell_keys(const double *__restrict__ x, const double *__restrict__ y, int64_t n, Grid G,
-                            uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+__device__ __forceinline__ uint64_t cell_hash(uint64_t c) {
+    c ^= c >> 33;
+    c *= 0xff51afd7ed558ccdull;
+    c ^= c >> 33;
+    c *= 0xc4ceb9fe1a85ec53ull;
+    c ^= c >> 33;
+    return c;
+}
+
+__global__ void k_grid_clear(unsigned long long *__restrict__ key, int32_t *__restrict__ head, int64_t T) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    key[t] = 0ull;
+    head[t] = -1;
+}
+
+__global__ void k_grid_insert(const double *__restrict__ x, const double *__restrict__ y, int64_t n, Grid G,
+                              unsigned long long *__restrict__ key, int32_t *__restrict__ head,
+                              int32_t *__restrict__ next, uint64_t mask) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int64_t cx, cy;
     cell_of(G, x[i], y[i], cx, cy);
-    keys[i] = (uint64_t)cy * (uint64_t)G.nx + (uint64_t)cx;
-    vals[i] = (int32_t)i;
-}
-
-__device__ __forceinline__ int64_t lower_bound_u64(const uint64_t *__restrict__ a, int64_t n, uint64_t key) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    const unsigned long long k1 = (unsigned long long)cy * (unsigned long long)G.nx + (unsigned long long)cx + 1ull;
+    uint64_t h = cell_hash(k1) & mask;
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+        const unsigned long long prev = atomicCAS(&key[h], 0ull, k1);
+        if (prev == 0ull || prev == k1) break;
+        h = (h + 1) & mask;
     }
-    return lo;
+    next[i] = atomicExch(&head[h], (int32_t)i);
 }
 
 template <int KIND>
 __global__ __launch_bounds__(256) void k_setka(const double *__restrict__ x, const double *__restrict__ y,
                                                const double *__restrict__ g, int64_t n, Grid G, double r2,
-                                               double max_ve_g, const uint64_t *__restrict__ skeys,
-                                               const int32_t *__restrict__ svals, int32_t *__restrict__ col) {
+                                               double max_ve_g, const unsigned long long *__restrict__ key,
+                                               const int32_t *__restrict__ head, const int32_t *__restrict__ next,
+                                               uint64_t mask, int32_t *__restrict__ col) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double r0 = x[i], r1 = y[i], gi = g[i];
@@ -309,19 +394,31 @@ __global__ __launch_bounds__(256) void k_setka(const double *__restrict__ x, con
     int64_t best = INT64_MAX;
     for (int64_t yy = cy - 1; yy <= cy + 1; ++yy) {
         if (yy < 0 || yy >= G.ny) continue;
-        const int64_t xa = cx > 0 ? cx - 1 : 0, xb = cx + 1 < G.nx ? cx + 1 : G.nx - 1;
-        const uint64_t ka = (uint64_t)yy * (uint64_t)G.nx + (uint64_t)xa;
-        const uint64_t kb = (uint64_t)yy * (uint64_t)G.nx + (uint64_t)xb;
-        for (int64_t p = lower_bound_u64(skeys, n, ka); p < n && skeys[p] <= kb; ++p) {
-            const int64_t j = svals[p];
-            if (j <= i || j >= best) continue;
-            const double gj = g[j];
-            bool hit;
-            if (KIND == 1)
-                hit = (gi * gj < 0) && (dist2(r0, r1, x[j], y[j]) < r2);
-            else
-                hit = (gi * gj > 0) && (dist2(r0, r1, x[j], y[j]) < r2) && (fabs(gi + gj) < max_ve_g);
-            if (hit) best = j;
+        for (int64_t xx = cx - 1; xx <= cx + 1; ++xx) {
+            if (xx < 0 || xx >= G.nx) continue;
+            const unsigned long long k1 =
+                (unsigned long long)yy * (unsigned long long)G.nx + (unsigned long long)xx + 1ull;
+            uint64_t h = cell_hash(k1) & mask;
+            int32_t j = -1;
+            for (uint64_t probe = 0; probe <= mask; ++probe) {
+                const unsigned long long kk = key[h];
+                if (kk == 0ull) break;
+                if (kk == k1) {
+                    j = head[h];
+                    break;
+                }
+                h = (h + 1) & mask;
+            }
+            for (; j >= 0; j = next[j]) {
+                if (j <= i || j >= best) continue;
+                const double gj = g[j];
+                bool hit;
+                if (KIND == 1)
+                    hit = (gi * gj < 0) && (dist2(r0, r1, x[j], y[j]) < r2);
+                else
+                    hit = (gi * gj > 0) && (dist2(r0, r1, x[j], y[j]) < r2) && (fabs(gi + gj) < max_ve_g);
+                if (hit) best = j;
+            }
         }
     }
     col[i] = best == INT64_MAX ? -1 : (int32_t)best;
@@ -349,25 +446,22 @@ int launch_setka(vx_case *c, int kind) {
     G.ny = (int64_t)fmin(spany, 1e9);
     if (G.nx < 1) G.nx = 1;
     if (G.ny < 1) G.ny = 1;
-    int bits = 1;
-    while (bits < 63 && (((uint64_t)1 << bits) < (uint64_t)G.nx * (uint64_t)G.ny)) ++bits;
-    if (c->keys.ensure((size_t)n, st) || c->keys_alt.ensure((size_t)n, st) || c->vals.ensure((size_t)n, st) ||
-        c->vals_alt.ensure((size_t)n, st))
+    int64_t T = 1024;
+    while (T < 2 * n) T <<= 1;
+    if (c->keys.ensure((size_t)T, st) || c->vals.ensure((size_t)T, st) || c->vals_alt.ensure((size_t)n, st))
         return 1;
+    unsigned long long *key = (unsigned long long *)c->keys.p;
+    const uint64_t mask = (uint64_t)T - 1;
+    hipLaunchKernelGGL(k_grid_clear, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, st, key, c->vals.p, T);
     const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, n, G, c->keys.p, c->vals.p);
-    size_t bytes = 0;
-    VX_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->keys.p, c->keys_alt.p, c->vals.p, c->vals_alt.p,
-                                                (int)n, 0, bits, st));
-    if (c->cub_tmp.ensure(bytes, st)) return 1;
-    VX_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, bytes, c->keys.p, c->keys_alt.p, c->vals.p,
-                                                c->vals_alt.p, (int)n, 0, bits, st));
+    hipLaunchKernelGGL(k_grid_insert, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, n, G, key, c->vals.p,
+                       c->vals_alt.p, mask);
     if (kind == 1)
         hipLaunchKernelGGL(k_setka<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, r2,
-                           c->prm.max_ve_g, c->keys_alt.p, c->vals_alt.p, c->col.p);
+                           c->prm.max_ve_g, key, c->vals.p, c->vals_alt.p, mask, c->col.p);
     else
         hipLaunchKernelGGL(k_setka<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, r2,
-                           c->prm.max_ve_g, c->keys_alt.p, c->vals_alt.p, c->col.p);
+                           c->prm.max_ve_g, key, c->vals.p, c->vals_alt.p, mask, c->col.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
@@ -388,16 +482,19 @@ __global__ void k_col_prepare(int64_t n, int32_t *__restrict__ pc, int32_t *__re
     pc[i] = 0;
 }
 
-__global__ void k_col_count(const int32_t *__restrict__ col, int64_t n, int32_t *__restrict__ pc,
-                            int32_t *__restrict__ cand, unsigned long long *__restrict__ ncand) {
+__global__ __launch_bounds__(256) void k_col_count(const int32_t *__restrict__ col, int64_t n,
+                                                   int32_t *__restrict__ pc, int32_t *__restrict__ cand,
+                                                   unsigned long long *__restrict__ ncand) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t j = col[i];
-    if (j >= 0) {
-        atomicAdd(&pc[j], 1);
-        const unsigned long long slot = atomicAdd(ncand, 1ull);
-        cand[slot] = (int32_t)i;
-    }
+    const int32_t j = i < n ? col[i] : -1;
+    if (j >= 0) atomicAdd(&pc[j], 1);
+    const uint64_t m = __ballot(j >= 0);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == __builtin_ctzll(m)) base = atomicAdd(ncand, (unsigned long long)__popcll(m));
+    base = __shfl(base, __builtin_ctzll(m), 64);
+    if (j >= 0) cand[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)i;
 }
 
 // state bits: 1 active, 2 inactive, 4 killed by an active child, 16 published.

@@ -346,7 +346,9 @@ __global__ __launch_bounds__(256) void k_bucket_build(const double *__restrict__
 
 // Box union and M2M of node p from its two children (both complete):
 //   b^P_k = sum_{m<=k} C(k,m) delta^{k-m} rho^m b^C_m,
-//   delta = (c_C - c_P)/R_P, rho = R_C/R_P  (|delta|, rho <= 1).
+//   delta = (c_C - c_P)/R_P, rho = R_C/R_P  (|delta|, rho <= 1),
+// evaluated without binomials or divisions as ORDER-1 passes of
+// w_k += delta w_{k-1} (k descending within a pass) on w_m = rho^m b^C_m.
 template <int ORDER>
 __device__ void node_from_children(int32_t p, const int32_t *__restrict__ child0, const int32_t *__restrict__ child1,
                                    double *__restrict__ ncx, double *__restrict__ ncy, double *__restrict__ nr,
@@ -358,32 +360,38 @@ __device__ void node_from_children(int32_t p, const int32_t *__restrict__ child0
     const double R = 0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z));
     const double Rs = fmax(R, kTiny);
     double ar[ORDER], ai[ORDER];
+#pragma unroll
     for (int q = 0; q < ORDER; ++q) ar[q] = ai[q] = 0.0;
     for (int ch = 0; ch < 2; ++ch) {
         const int32_t cc = c[ch];
         const double dr = (ncx[cc] - cx) / Rs, di = (ncy[cc] - cy) / Rs;
         const double rho = nr[cc] / Rs;
-        double dpr[ORDER], dpi[ORDER], rp[ORDER];
-        dpr[0] = 1.0; dpi[0] = 0.0; rp[0] = 1.0;
-        for (int q = 1; q < ORDER; ++q) {
-            dpr[q] = dpr[q - 1] * dr - dpi[q - 1] * di;
-            dpi[q] = dpr[q - 1] * di + dpi[q - 1] * dr;
-            rp[q] = rp[q - 1] * rho;
-        }
         const double2 *bc = cd + (int64_t)cc * ORDER;
-        for (int kk = 0; kk < ORDER; ++kk) {
-            double binom = 1.0, sr = 0.0, si = 0.0;
-            for (int mm = 0; mm <= kk; ++mm) {
-                const double2 bm = bc[mm];
-                const double wr = binom * rp[mm], tr = dpr[kk - mm] * wr, ti = dpi[kk - mm] * wr;
-                sr += tr * bm.x - ti * bm.y;
-                si += tr * bm.y + ti * bm.x;
-                binom = binom * (double)(kk - mm) / (double)(mm + 1);
+        double wr[ORDER], wi[ORDER];
+        double rk = 1.0;
+#pragma unroll
+        for (int q = 0; q < ORDER; ++q) {
+            const double2 bm = bc[q];
+            wr[q] = bm.x * rk;
+            wi[q] = bm.y * rk;
+            rk *= rho;
+        }
+#pragma unroll
+        for (int pass = 0; pass < ORDER - 1; ++pass) {
+#pragma unroll
+            for (int k = ORDER - 1; k > pass; --k) {
+                const double nwr = wr[k] + (dr * wr[k - 1] - di * wi[k - 1]);
+                wi[k] = wi[k] + (dr * wi[k - 1] + di * wr[k - 1]);
+                wr[k] = nwr;
             }
-            ar[kk] += sr;
-            ai[kk] += si;
+        }
+#pragma unroll
+        for (int q = 0; q < ORDER; ++q) {
+            ar[q] += wr[q];
+            ai[q] += wi[q];
         }
     }
+#pragma unroll
     for (int q = 0; q < ORDER; ++q) {
         cd[(int64_t)p * ORDER + q] = make_double2(ar[q], ai[q]);
         cf[(int64_t)p * ORDER + q] = make_float2((float)ar[q], (float)ai[q]);
@@ -396,7 +404,8 @@ __device__ void node_from_children(int32_t p, const int32_t *__restrict__ child0
 
 // Bottom-up pass, one thread per bucket: the second thread to reach a node
 // builds it and continues upward; the first one stops.  No thread waits on
-// another, so the pass always completes.
+// another, so the pass always completes.  The fences publish a node before
+// its parent's counter moves and order the reads of the sibling after it.
 template <int ORDER>
 __global__ void k_upper_build(int64_t nb, const int32_t *__restrict__ parent, int32_t *__restrict__ visits,
                               const int32_t *__restrict__ child0, const int32_t *__restrict__ child1,

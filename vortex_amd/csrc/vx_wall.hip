@@ -10,7 +10,8 @@
 //                           and the panel loop stops there.
 //   V_i += nu II3 / II0.
 // Targets outside the box of birth points grown by 5 max(l) meet no panel and
-// are skipped without touching the panel list.
+// are skipped without touching the panel list; the others are listed first
+// and then summed one wave per target.
 #include "vx_internal.h"
 
 namespace vx {
@@ -37,52 +38,107 @@ __device__ __forceinline__ void quad_i0_i3(T ra0, T ra1, T rb0, T rb1, T n0, T n
     res31 = s31;
 }
 
+// Targets inside the box, listed with one append per wave.
+__global__ __launch_bounds__(256) void k_wall_candidates(const double *__restrict__ x, const double *__restrict__ y,
+                                                         int64_t n_tgt, double4 box, int32_t *__restrict__ list,
+                                                         unsigned long long *__restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = false;
+    if (i < n_tgt) {
+        const double xi = x[i], yi = y[i];
+        in = xi >= box.x && xi <= box.y && yi >= box.z && yi <= box.w;
+    }
+    const uint64_t m = __ballot(in);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == __builtin_ctzll(m)) base = atomicAdd(cnt, (unsigned long long)__popcll(m));
+    base = __shfl(base, __builtin_ctzll(m), 64);
+    if (in) list[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)i;
+}
+
+__global__ void k_wall_zero(unsigned long long *__restrict__ cnt) { *cnt = 0ull; }
+
+// One wave per listed target (grid-stride, the count stays on the device);
+// lanes take panels lane, lane + 64, ...  The first panel in index order with
+// r <= 0.1 l decides the result alone (the sequential loop overwrites II0 and
+// II3 there and stops); without one, the mid-range terms are summed, lane
+// partials in FP64 and then across the wave.
 template <typename T>
 __global__ __launch_bounds__(256) void k_diffusion_wall(const double *__restrict__ x,
                                                         const double *__restrict__ y,
-                                                        const double *__restrict__ d, int64_t n_tgt,
+                                                        const double *__restrict__ d,
+                                                        const int32_t *__restrict__ list,
+                                                        const unsigned long long *__restrict__ cnt,
                                                         PanelsSoA pan, double nu, int64_t npts,
-                                                        double4 box, double *__restrict__ u,
-                                                        double *__restrict__ v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_tgt) return;
-    const double xi = x[i], yi = y[i];
-    if (!(xi >= box.x && xi <= box.y && yi >= box.z && yi <= box.w)) return;
-    const T a0 = (T)xi, a1 = (T)yi, dd = (T)d[i];
-    T II0 = T(2 * kPi) * dd * dd;
-    T II30 = 0, II31 = 0;
-    for (int64_t f = 0; f < pan.P; ++f) {
-        const T dL = (T)pan.len[f];
-        const T q0 = a0 - (T)pan.bx[f], q1 = a1 - (T)pan.by[f];
-        const T r = sqrt(q0 * q0 + q1 * q1);
-        if ((r < T(5) * dL) && (r > T(0.1) * dL)) {
-            T R0, R30, R31;
-            quad_i0_i3<T>((T)pan.lx[f], (T)pan.ly[f], (T)pan.rx[f], (T)pan.ry[f], -(T)pan.nx[f], -(T)pan.ny[f], a0,
-                          a1, dL, dd, npts, R0, R30, R31);
-            II0 += (-dd) * R0;
-            II30 -= R30;
-            II31 -= R31;
-        } else if (r <= T(0.1) * dL) {
+                                                        double *__restrict__ u, double *__restrict__ v) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    const int64_t total = (int64_t)*cnt;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total; t += nw) {
+        const int64_t i = list[t];
+        const T a0 = (T)x[i], a1 = (T)y[i], dd = (T)d[i];
+        double c0 = 0.0, c30 = 0.0, c31 = 0.0;
+        int64_t fb = pan.P;
+        for (int64_t f = lane; f < pan.P; f += 64) {
+            const T dL = (T)pan.len[f];
+            const T q0 = a0 - (T)pan.bx[f], q1 = a1 - (T)pan.by[f];
+            const T r = sqrt(q0 * q0 + q1 * q1);
+            if ((r < T(5) * dL) && (r > T(0.1) * dL)) {
+                T R0, R30, R31;
+                quad_i0_i3<T>((T)pan.lx[f], (T)pan.ly[f], (T)pan.rx[f], (T)pan.ry[f], -(T)pan.nx[f],
+                              -(T)pan.ny[f], a0, a1, dL, dd, npts, R0, R30, R31);
+                c0 += (double)((-dd) * R0);
+                c30 += (double)R30;
+                c31 += (double)R31;
+            } else if (r <= T(0.1) * dL) {
+                fb = f;
+                break;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const int64_t ofb = __shfl_xor(fb, o, 64);
+            fb = ofb < fb ? ofb : fb;
+        }
+        double II0, II30, II31;
+        if (fb < pan.P) {
+            const T dL = (T)pan.len[fb];
             const T ex = T(1) - exp(-dL / (T(2) * dd));
-            II0 = T(kPi) * dd * dd;
-            II30 = T(2) * (T)pan.nx[f] * dd * ex;
-            II31 = T(2) * (T)pan.ny[f] * dd * ex;
-            break;
+            II0 = (double)(T(kPi) * dd * dd);
+            II30 = (double)(T(2) * (T)pan.nx[fb] * dd * ex);
+            II31 = (double)(T(2) * (T)pan.ny[fb] * dd * ex);
+        } else {
+            for (int o = 32; o > 0; o >>= 1) {
+                c0 += __shfl_xor(c0, o, 64);
+                c30 += __shfl_xor(c30, o, 64);
+                c31 += __shfl_xor(c31, o, 64);
+            }
+            II0 = (double)(T(2 * kPi) * dd * dd) + c0;
+            II30 = -c30;
+            II31 = -c31;
+        }
+        if (lane == 0) {
+            u[i] += nu * II30 / II0;
+            v[i] += nu * II31 / II0;
         }
     }
-    u[i] += nu * (double)II30 / (double)II0;
-    v[i] += nu * (double)II31 / (double)II0;
 }
 
 int launch_diffusion_wall(vx_case *c, int64_t n_tgt) {
     if (n_tgt <= 0 || c->P <= 0) return 0;
+    hipStream_t st = c->stream;
+    if (c->lists.ensure((size_t)n_tgt, st) || c->dcount.ensure(6, st)) return 1;
+    unsigned long long *cnt = (unsigned long long *)(c->dcount.p + 5);
     const unsigned nb = (unsigned)((n_tgt + 255) / 256);
+    hipLaunchKernelGGL(k_wall_zero, dim3(1), dim3(1), 0, st, cnt);
+    hipLaunchKernelGGL(k_wall_candidates, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, n_tgt, c->wall_box,
+                       c->lists.p, cnt);
     if (c->prm.precision == VX_PREC_MIXED)
-        hipLaunchKernelGGL(k_diffusion_wall<float>, dim3(nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->d.p,
-                           n_tgt, c->pan, c->prm.nu, c->prm.n_of_points, c->wall_box, c->u.p, c->v.p);
+        hipLaunchKernelGGL(k_diffusion_wall<float>, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->d.p,
+                           c->lists.p, cnt, c->pan, c->prm.nu, c->prm.n_of_points, c->u.p, c->v.p);
     else
-        hipLaunchKernelGGL(k_diffusion_wall<double>, dim3(nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->d.p,
-                           n_tgt, c->pan, c->prm.nu, c->prm.n_of_points, c->wall_box, c->u.p, c->v.p);
+        hipLaunchKernelGGL(k_diffusion_wall<double>, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->d.p,
+                           c->lists.p, cnt, c->pan, c->prm.nu, c->prm.n_of_points, c->u.p, c->v.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
