@@ -182,6 +182,7 @@ int vx_case_reserve(vx_case *c, int64_t capacity) {
 
 int vx_case_set_state(vx_case *c, const double *x, const double *y, const double *g, int64_t n) {
     VX_ARG(c && n >= 0 && (n == 0 || (x && y && g)), "state arrays");
+    c->live_box_ok = false;
     if (reserve(c, ceil32(n + c->P))) return 1;
     if (n) {
         VX_CHECK(hipMemcpyAsync(c->x.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
@@ -226,6 +227,7 @@ double vx_case_d_g(const vx_case *c) { return c ? c->d_g : 0.0; }
 
 int vx_case_set_n(vx_case *c, int64_t n) {
     VX_ARG(c && n >= 0 && n <= c->cap, "n beyond capacity");
+    c->live_box_ok = false;
     c->n = n;
     return 0;
 }
@@ -265,6 +267,7 @@ int vx_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y, doubl
 
 int vx_birth(vx_case *c, const double *R_p, double d_g, double *gsheet) {
     VX_ARG(c && c->have_matrix, "no panel matrix");
+    c->live_box_ok = false;
     if (reserve(c, ceil32(c->n + c->P))) return 1;
     if (R_p) {
         if (c->rp.ensure((size_t)c->P, c->stream)) return 1;
@@ -362,6 +365,7 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL) {
 // source count, then the newborns at [n, n+P).
 int vx_vort_creation(vx_case *c, double vinf_x, double vinf_y) {
     VX_ARG(c && c->have_matrix, "no panel matrix");
+    c->live_box_ok = false;
     Timer t(c, &c->tm.creation_ms, 0);
     if (reserve(c, ceil32(c->n + c->P))) return 1;
     if (launch_right_part(c, ceil32(c->n), vinf_x, vinf_y) || launch_birth(c, c->d_g)) return 1;
@@ -411,6 +415,7 @@ int vx_step(vx_case *c, double out4[4]) {
         }
     }
     c->d_g = out4[0];
+    c->live_box_ok = false;
     return 0;
 }
 

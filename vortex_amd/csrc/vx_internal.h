@@ -92,6 +92,8 @@ struct vx_case {
     void *pan_block = nullptr;
     int64_t P = 0;
     double4 wall_box{};             // birth-point box grown by 5 max(length)
+    double4 live_box{};             // box of g != 0 after the last move (collapse grid)
+    bool live_box_ok = false;       // live_box holds for the current positions
     // panel system
     vx::DBuf<double> L, M;          // (P+1)^2
     bool have_matrix = false;

@@ -26,14 +26,26 @@ struct StepConsts {
 // panel (after the profile-box reject); the crossing nearest the old position
 // wins, the lowest panel index among equal distances.  A crossing set with no
 // panel nearer than 5 (the reference would read panels[-1]) counts as no
-// crossing.  Split in two: k_hit_candidates lists the moving vortices whose
-// segment box meets the profile box, then one wave per candidate tests the
-// panels lane-parallel and reduces (distance, index) to the same winner as
-// the sequential scan.
+// crossing.  The moving vortices whose segment box meets the profile box are
+// selected in index order; one wave per candidate tests the panels lane,
+// lane + 64, ... and reduces (distance, index) to the winner of the
+// sequential scan.
 __device__ __forceinline__ bool hit_box_reject(const StepConsts &k, double x1, double y1, double x2, double y2) {
     return ((x1 < k.px_min) && (x2 < k.px_min)) || ((x1 > k.px_max) && (x2 > k.px_max)) ||
            ((y1 < k.py_min) && (y2 < k.py_min)) || ((y1 > k.py_max) && (y2 > k.py_max));
 }
+
+struct HitCandidate {
+    const double *x, *y, *g, *u, *v;
+    StepConsts k;
+    __device__ bool operator()(int32_t i) const {
+        const double xi = x[i], yi = y[i];
+        if (g[i] == 0) return false;
+        const double xn = xi + u[i] * k.dt;
+        const double yn = yi + v[i] * k.dt;
+        return !hit_box_reject(k, xn, yn, xi, yi);
+    }
+};
 
 // Squared distance from (x2, y2) to the crossing of the segment with panel
 // i, or -1 if they do not cross.
@@ -58,42 +70,14 @@ __device__ __forceinline__ double panel_cross(const PanelsSoA &pan, int64_t i, d
     return (x2 - X) * (x2 - X) + (y2 - Y) * (y2 - Y);
 }
 
-__global__ void k_set_u64(unsigned long long *__restrict__ p, unsigned long long v) { *p = v; }
-
-__global__ __launch_bounds__(256) void k_hit_candidates(const double *__restrict__ x, const double *__restrict__ y,
-                                                        const double *__restrict__ g, const double *__restrict__ u,
-                                                        const double *__restrict__ v, int64_t n, StepConsts k,
-                                                        int32_t *__restrict__ hitp, int32_t *__restrict__ cand,
-                                                        unsigned long long *__restrict__ ncand) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool c = false;
-    if (i < n) {
-        hitp[i] = -1;
-        const double xi = x[i], yi = y[i];
-        if (g[i] != 0) {
-            const double xn = xi + u[i] * k.dt;
-            const double yn = yi + v[i] * k.dt;
-            c = !hit_box_reject(k, xn, yn, xi, yi);
-        }
-    }
-    const uint64_t m = __ballot(c);
-    if (m == 0) return;
-    const int lane = threadIdx.x & 63;
-    unsigned long long base = 0;
-    if (lane == __builtin_ctzll(m)) base = atomicAdd(ncand, (unsigned long long)__popcll(m));
-    base = __shfl(base, __builtin_ctzll(m), 64);
-    if (c) cand[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)i;
-}
-
 // Grid-stride over the candidates, one wave each; the count stays on the device.
 __global__ __launch_bounds__(256) void k_hit_panels(const double *__restrict__ x, const double *__restrict__ y,
                                                     const double *__restrict__ u, const double *__restrict__ v,
                                                     PanelsSoA pan, StepConsts k, const int32_t *__restrict__ cand,
-                                                    const unsigned long long *__restrict__ ncand,
-                                                    int32_t *__restrict__ hitp) {
+                                                    const int32_t *__restrict__ ncand, int32_t *__restrict__ hitp) {
     const int lane = threadIdx.x & 63;
     const int64_t nw = (int64_t)gridDim.x * 4;
-    const int64_t total = (int64_t)*ncand;
+    const int64_t total = *ncand;
     for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total; t += nw) {
         const int64_t i = cand[t];
         const double x2 = x[i], y2 = y[i];
@@ -121,14 +105,19 @@ __global__ __launch_bounds__(256) void k_hit_panels(const double *__restrict__ x
 }
 
 // ---- A9: src/kernel.cu:1090-1143 -------------------------------------------
+// hitp[i] holds the hit panel of every selected candidate (others are not
+// read).  Per work-group partials: d_g, Fx, Fy, M sums, then min x, -max x,
+// min y, -max y over the vortices alive after the move (the box the collapse
+// grid covers).
 __global__ __launch_bounds__(256) void k_step_move(double *__restrict__ x, double *__restrict__ y,
                                                    double *__restrict__ g, const double *__restrict__ u,
                                                    const double *__restrict__ v, int64_t n, int64_t P,
                                                    PanelsSoA pan, StepConsts k, const int32_t *__restrict__ hitp,
                                                    double *__restrict__ part) {
-    __shared__ double red[4][256];
+    __shared__ double red[8][256];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double dg = 0.0, fx = 0.0, fy = 0.0, mm = 0.0;
+    double bx0 = 1e300, bx1 = 1e300, by0 = 1e300, by1 = 1e300;
     if (i < n) {
         double xi = x[i], yi = y[i], gi = g[i];
         if (n >= P && i >= n - P) {
@@ -138,7 +127,7 @@ __global__ __launch_bounds__(256) void k_step_move(double *__restrict__ x, doubl
         }
         const double xn = xi + u[i] * k.dt;
         const double yn = yi + v[i] * k.dt;
-        if (gi != 0) {
+        if ((gi != 0) && !hit_box_reject(k, xn, yn, xi, yi)) {
             const int hp = hitp[i];
             if (hp >= 0) {
                 const double cx = pan.cx[hp], cy = pan.cy[hp];
@@ -159,32 +148,49 @@ __global__ __launch_bounds__(256) void k_step_move(double *__restrict__ x, doubl
         x[i] = xi;
         y[i] = yi;
         g[i] = gi;
+        if (gi != 0) {
+            bx0 = xi;
+            bx1 = -xi;
+            by0 = yi;
+            by1 = -yi;
+        }
     }
     red[0][threadIdx.x] = dg;
     red[1][threadIdx.x] = fx;
     red[2][threadIdx.x] = fy;
     red[3][threadIdx.x] = mm;
+    red[4][threadIdx.x] = bx0;
+    red[5][threadIdx.x] = bx1;
+    red[6][threadIdx.x] = by0;
+    red[7][threadIdx.x] = by1;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+        if (threadIdx.x < s) {
             for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+            for (int q = 4; q < 8; ++q) red[q][threadIdx.x] = fmin(red[q][threadIdx.x], red[q][threadIdx.x + s]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x < 4) part[(int64_t)blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 8) part[(int64_t)blockIdx.x * 8 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// A10: src/kernel.cu:1144-1157 -- d_g = sum, F = rho/dt sum, M = rho/(2 dt) sum.
+// A10: src/kernel.cu:1144-1157 -- d_g = sum, F = rho/dt sum, M = rho/(2 dt)
+// sum; out[4..7] the live box (min x, max x, min y, max y).
 __global__ __launch_bounds__(256) void k_sum4(const double *__restrict__ part, int64_t nb, double rho, double dt,
                                               double *__restrict__ out) {
-    __shared__ double red[4][256];
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t b = threadIdx.x; b < nb; b += 256)
-        for (int q = 0; q < 4; ++q) a[q] += part[b * 4 + q];
-    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = a[q];
+    __shared__ double red[8][256];
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 1e300, 1e300, 1e300, 1e300};
+    for (int64_t b = threadIdx.x; b < nb; b += 256) {
+        for (int q = 0; q < 4; ++q) a[q] += part[b * 8 + q];
+        for (int q = 4; q < 8; ++q) a[q] = fmin(a[q], part[b * 8 + q]);
+    }
+    for (int q = 0; q < 8; ++q) red[q][threadIdx.x] = a[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+        if (threadIdx.x < s) {
             for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+            for (int q = 4; q < 8; ++q) red[q][threadIdx.x] = fmin(red[q][threadIdx.x], red[q][threadIdx.x + s]);
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -192,32 +198,44 @@ __global__ __launch_bounds__(256) void k_sum4(const double *__restrict__ part, i
         out[1] = red[1][0] * (rho / dt);
         out[2] = red[2][0] * (rho / dt);
         out[3] = red[3][0] * (rho / (2 * dt));
+        out[4] = red[4][0];
+        out[5] = -red[5][0];
+        out[6] = red[6][0];
+        out[7] = -red[7][0];
     }
 }
 
 int launch_step_move(vx_case *c, double out4[4]) {
     const int64_t n = c->n;
     const int64_t nb = n > 0 ? (n + 255) / 256 : 1;
-    if (c->partial.ensure((size_t)(4 * nb), c->stream) || c->red.ensure(8, c->stream)) return 1;
-    if (c->misc.ensure((size_t)(n > 0 ? n : 1), c->stream) || c->lists.ensure((size_t)(n > 0 ? n : 1), c->stream) ||
-        c->dcount.ensure(6, c->stream))
+    hipStream_t st = c->stream;
+    if (c->partial.ensure((size_t)(8 * nb), st) || c->red.ensure(8, st) ||
+        c->misc.ensure((size_t)(n > 0 ? n : 1), st) || c->lists.ensure((size_t)(n > 0 ? n : 1), st) ||
+        c->dcount.ensure(8, st))
         return 1;
     const vx_params &p = c->prm;
     StepConsts k{p.dt, p.rc_x, p.rc_y, p.x_min, p.x_max, p.y_min, p.y_max, p.px_min, p.px_max, p.py_min, p.py_max};
     int32_t *hitp = c->misc.p, *cand = c->lists.p;
-    unsigned long long *ncand = (unsigned long long *)(c->dcount.p + 4);
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, c->stream, ncand, 0ull);
-    hipLaunchKernelGGL(k_hit_candidates, dim3((unsigned)nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->g.p, c->u.p,
-                       c->v.p, n, k, hitp, cand, ncand);
-    hipLaunchKernelGGL(k_hit_panels, dim3(1024), dim3(256), 0, c->stream, c->x.p, c->y.p, c->u.p, c->v.p, c->pan, k,
-                       cand, ncand, hitp);
-    hipLaunchKernelGGL(k_step_move, dim3((unsigned)nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->g.p, c->u.p,
-                       c->v.p, n, c->P, c->pan, k, hitp, c->partial.p);
-    hipLaunchKernelGGL(k_sum4, dim3(1), dim3(256), 0, c->stream, c->partial.p, nb, p.rho, p.dt, c->red.p);
+    int32_t *ncand = (int32_t *)(c->dcount.p + 4);
+    if (n > 0) {
+        const HitCandidate pred{c->x.p, c->y.p, c->g.p, c->u.p, c->v.p, k};
+        hipcub::CountingInputIterator<int32_t> it(0);
+        size_t bytes = 0;
+        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cand, ncand, n, pred, st));
+        if (c->cub_tmp.ensure(bytes, st)) return 1;
+        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cand, ncand, n, pred, st));
+        hipLaunchKernelGGL(k_hit_panels, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->u.p, c->v.p, c->pan, k,
+                           cand, ncand, hitp);
+    }
+    hipLaunchKernelGGL(k_step_move, dim3((unsigned)nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->u.p, c->v.p, n,
+                       c->P, c->pan, k, hitp, c->partial.p);
+    hipLaunchKernelGGL(k_sum4, dim3(1), dim3(256), 0, st, c->partial.p, nb, p.rho, p.dt, c->red.p);
     VX_CHECK(hipGetLastError());
-    VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    VX_CHECK(hipStreamSynchronize(c->stream));
+    VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    VX_CHECK(hipStreamSynchronize(st));
     for (int q = 0; q < 4; ++q) out4[q] = c->h_red[q];
+    c->live_box = make_double4(c->h_red[4], c->h_red[5], c->h_red[6], c->h_red[7]);
+    c->live_box_ok = true;
     return 0;
 }
 
@@ -329,99 +347,145 @@ int launch_compact(vx_case *c, int64_t *n_inout) {
 // COL_i = smallest j > i with
 //   kind 1: g_i g_j < 0 and r_ij^2 < r2
 //   kind 2: g_i g_j > 0 and r_ij^2 < r2 and |g_i + g_j| < max_ve_g
-// Candidates come from a uniform grid of cell size slightly above sqrt(r2)
-// (so every pair inside the radius lies in neighbouring cells).  Occupied
-// cells live in an open-addressing hash table (cell + 1 as key, 0 empty,
-// load factor at most 1/2) whose slots head a list of the cell's vortices.
-// The minimum j does not depend on the order of the lists.
+// Only vortices with g != 0 can pair.  They are linked into the cells of a
+// dense grid over their box, cell size s > 2 sqrt(r2): a partner of i lies in
+// i's cell or in the neighbour on the side of the cell's half that holds i,
+// per axis, so four cells are searched.  The minimum j does not depend on the
+// order of the cell lists.
 struct Grid {
-    double x0, y0, inv_h;
+    double x0, y0, inv_s;
     int64_t nx, ny;
 };
 
-__device__ __forceinline__ void cell_of(const Grid &G, double x, double y, int64_t &cx, int64_t &cy) {
-    double fx = (x - G.x0) * G.inv_h, fy = (y - G.y0) * G.inv_h;
+__device__ __forceinline__ void cell_of(const Grid &G, double x, double y, int64_t &cx, int64_t &cy, double &fx,
+                                        double &fy) {
+    fx = (x - G.x0) * G.inv_s;
+    fy = (y - G.y0) * G.inv_s;
     if (!(fx >= 0.0)) fx = 0.0;  // also catches NaN
     if (!(fy >= 0.0)) fy = 0.0;
     cx = fx >= (double)(G.nx - 1) ? G.nx - 1 : (int64_t)fx;
     cy = fy >= (double)(G.ny - 1) ? G.ny - 1 : (int64_t)fy;
+    fx -= (double)cx;
+    fy -= (double)cy;
 }
 
-__device__ __forceinline__ uint64_t cell_hash(uint64_t c) {
-    c ^= c >> 33;
-    c *= 0xff51afd7ed558ccdull;
-    c ^= c >> 33;
-    c *= 0xc4ceb9fe1a85ec53ull;
-    c ^= c >> 33;
-    return c;
-}
-
-__global__ void k_grid_clear(unsigned long long *__restrict__ key, int32_t *__restrict__ head, int64_t T) {
+__global__ void k_fill_i32(int32_t *__restrict__ a, int64_t count, int32_t value) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    key[t] = 0ull;
-    head[t] = -1;
+    if (t < count) a[t] = value;
 }
 
-__global__ void k_grid_insert(const double *__restrict__ x, const double *__restrict__ y, int64_t n, Grid G,
-                              unsigned long long *__restrict__ key, int32_t *__restrict__ head,
-                              int32_t *__restrict__ next, uint64_t mask) {
+__global__ void k_grid_insert(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ g, int64_t n, Grid G, int32_t *__restrict__ head,
+                              int32_t *__restrict__ next) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || g[i] == 0) return;
     int64_t cx, cy;
-    cell_of(G, x[i], y[i], cx, cy);
-    const unsigned long long k1 = (unsigned long long)cy * (unsigned long long)G.nx + (unsigned long long)cx + 1ull;
-    uint64_t h = cell_hash(k1) & mask;
-    for (uint64_t probe = 0; probe <= mask; ++probe) {
-        const unsigned long long prev = atomicCAS(&key[h], 0ull, k1);
-        if (prev == 0ull || prev == k1) break;
-        h = (h + 1) & mask;
-    }
-    next[i] = atomicExch(&head[h], (int32_t)i);
+    double fx, fy;
+    cell_of(G, x[i], y[i], cx, cy, fx, fy);
+    next[i] = atomicExch(&head[cy * G.nx + cx], (int32_t)i);
 }
 
 template <int KIND>
 __global__ __launch_bounds__(256) void k_setka(const double *__restrict__ x, const double *__restrict__ y,
                                                const double *__restrict__ g, int64_t n, Grid G, double r2,
-                                               double max_ve_g, const unsigned long long *__restrict__ key,
-                                               const int32_t *__restrict__ head, const int32_t *__restrict__ next,
-                                               uint64_t mask, int32_t *__restrict__ col) {
+                                               double max_ve_g, const int32_t *__restrict__ head,
+                                               const int32_t *__restrict__ next, int32_t *__restrict__ col) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double r0 = x[i], r1 = y[i], gi = g[i];
+    const double gi = g[i];
+    if (gi == 0) {
+        col[i] = -1;
+        return;
+    }
+    const double r0 = x[i], r1 = y[i];
     int64_t cx, cy;
-    cell_of(G, r0, r1, cx, cy);
+    double fx, fy;
+    cell_of(G, r0, r1, cx, cy, fx, fy);
+    const int64_t ox = fx < 0.5 ? cx - 1 : cx + 1, oy = fy < 0.5 ? cy - 1 : cy + 1;
+    const int64_t xs[2] = {cx, ox}, ys[2] = {cy, oy};
+    int32_t hd[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t xx = xs[q & 1], yy = ys[q >> 1];
+        hd[q] = (xx >= 0 && xx < G.nx && yy >= 0 && yy < G.ny) ? head[yy * G.nx + xx] : -1;
+    }
     int64_t best = INT64_MAX;
-    for (int64_t yy = cy - 1; yy <= cy + 1; ++yy) {
-        if (yy < 0 || yy >= G.ny) continue;
-        for (int64_t xx = cx - 1; xx <= cx + 1; ++xx) {
-            if (xx < 0 || xx >= G.nx) continue;
-            const unsigned long long k1 =
-                (unsigned long long)yy * (unsigned long long)G.nx + (unsigned long long)xx + 1ull;
-            uint64_t h = cell_hash(k1) & mask;
-            int32_t j = -1;
-            for (uint64_t probe = 0; probe <= mask; ++probe) {
-                const unsigned long long kk = key[h];
-                if (kk == 0ull) break;
-                if (kk == k1) {
-                    j = head[h];
-                    break;
-                }
-                h = (h + 1) & mask;
-            }
-            for (; j >= 0; j = next[j]) {
-                if (j <= i || j >= best) continue;
-                const double gj = g[j];
-                bool hit;
-                if (KIND == 1)
-                    hit = (gi * gj < 0) && (dist2(r0, r1, x[j], y[j]) < r2);
-                else
-                    hit = (gi * gj > 0) && (dist2(r0, r1, x[j], y[j]) < r2) && (fabs(gi + gj) < max_ve_g);
-                if (hit) best = j;
-            }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        for (int32_t j = hd[q]; j >= 0; j = next[j]) {
+            if (j <= i || j >= best) continue;
+            const double gj = g[j];
+            bool hit;
+            if (KIND == 1)
+                hit = (gi * gj < 0) && (dist2(r0, r1, x[j], y[j]) < r2);
+            else
+                hit = (gi * gj > 0) && (dist2(r0, r1, x[j], y[j]) < r2) && (fabs(gi + gj) < max_ve_g);
+            if (hit) best = j;
         }
     }
     col[i] = best == INT64_MAX ? -1 : (int32_t)best;
+}
+
+// Box of the vortices with g != 0 (min x, -max x, min y, -max y partials).
+__global__ __launch_bounds__(256) void k_live_box(const double *__restrict__ x, const double *__restrict__ y,
+                                                  const double *__restrict__ g, int64_t n,
+                                                  double *__restrict__ part) {
+    __shared__ double red[4][256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double b[4] = {1e300, 1e300, 1e300, 1e300};
+    if (i < n && g[i] != 0) {
+        b[0] = x[i];
+        b[1] = -x[i];
+        b[2] = y[i];
+        b[3] = -y[i];
+    }
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = b[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = fmin(red[q][threadIdx.x], red[q][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[(int64_t)blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_live_box_final(const double *__restrict__ part, int64_t nb,
+                                                        double *__restrict__ out) {
+    __shared__ double red[4][256];
+    double b[4] = {1e300, 1e300, 1e300, 1e300};
+    for (int64_t k = threadIdx.x; k < nb; k += 256)
+        for (int q = 0; q < 4; ++q) b[q] = fmin(b[q], part[k * 4 + q]);
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = b[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = fmin(red[q][threadIdx.x], red[q][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = -red[1][0];
+        out[2] = red[2][0];
+        out[3] = -red[3][0];
+    }
+}
+
+static int live_box(vx_case *c, double4 *box) {
+    if (!c->live_box_ok) {
+        const int64_t n = c->n;
+        const int64_t nb = (n + 255) / 256;
+        if (c->partial.ensure((size_t)(4 * nb), c->stream) || c->red.ensure(8, c->stream)) return 1;
+        hipLaunchKernelGGL(k_live_box, dim3((unsigned)nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->g.p, n,
+                           c->partial.p);
+        hipLaunchKernelGGL(k_live_box_final, dim3(1), dim3(256), 0, c->stream, c->partial.p, nb, c->red.p);
+        VX_CHECK(hipGetLastError());
+        VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        VX_CHECK(hipStreamSynchronize(c->stream));
+        c->live_box = make_double4(c->h_red[0], c->h_red[1], c->h_red[2], c->h_red[3]);
+        c->live_box_ok = true;
+    }
+    *box = c->live_box;
+    return 0;
 }
 
 int launch_setka(vx_case *c, int kind) {
@@ -429,39 +493,39 @@ int launch_setka(vx_case *c, int kind) {
     hipStream_t st = c->stream;
     if (c->col.ensure((size_t)(n > 0 ? n : 1), st)) return 1;
     if (n <= 0) return 0;
+    const unsigned nb = (unsigned)((n + 255) / 256);
     const double ve2 = c->prm.ve_size * c->prm.ve_size;
     const double rr = kind == 1 ? c->prm.r_col_diff_sign : c->prm.r_col_same_sign;
     const double r2 = rr * rr * ve2;  // src/unit_1.cu:1228-1229
-    if (!(r2 > 0.0)) {
-        VX_CHECK(hipMemsetAsync(c->col.p, 0xff, sizeof(int32_t) * n, st));
+    double4 bx;
+    if (live_box(c, &bx)) return 1;
+    if (!(r2 > 0.0) || !(bx.x <= bx.y && bx.z <= bx.w)) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, c->col.p, n, -1);
+        VX_CHECK(hipGetLastError());
         return 0;
     }
-    const double h = sqrt(r2) * 1.001;
+    // at most 2^25 cells (128 MB of heads): a wider cell stays correct
+    constexpr double kMaxCells = 33554432.0;
+    double s = 2.0 * sqrt(r2) * 1.001;
+    const double w = bx.y - bx.x, h = bx.w - bx.z;
+    while ((floor(w / s) + 1) * (floor(h / s) + 1) > kMaxCells) s *= 1.25;
     Grid G;
-    G.x0 = c->prm.x_min - 2 * h;
-    G.y0 = c->prm.y_min - 2 * h;
-    G.inv_h = 1.0 / h;
-    const double spanx = (c->prm.x_max - G.x0) / h + 3, spany = (c->prm.y_max - G.y0) / h + 3;
-    G.nx = (int64_t)fmin(spanx, 1e9);
-    G.ny = (int64_t)fmin(spany, 1e9);
-    if (G.nx < 1) G.nx = 1;
-    if (G.ny < 1) G.ny = 1;
-    int64_t T = 1024;
-    while (T < 2 * n) T <<= 1;
-    if (c->keys.ensure((size_t)T, st) || c->vals.ensure((size_t)T, st) || c->vals_alt.ensure((size_t)n, st))
-        return 1;
-    unsigned long long *key = (unsigned long long *)c->keys.p;
-    const uint64_t mask = (uint64_t)T - 1;
-    hipLaunchKernelGGL(k_grid_clear, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, st, key, c->vals.p, T);
-    const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_grid_insert, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, n, G, key, c->vals.p,
-                       c->vals_alt.p, mask);
+    G.x0 = bx.x;
+    G.y0 = bx.z;
+    G.inv_s = 1.0 / s;
+    G.nx = (int64_t)floor(w / s) + 1;
+    G.ny = (int64_t)floor(h / s) + 1;
+    const int64_t cells = G.nx * G.ny;
+    if (c->vals.ensure((size_t)cells, st) || c->vals_alt.ensure((size_t)n, st)) return 1;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, c->vals.p, cells, -1);
+    hipLaunchKernelGGL(k_grid_insert, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, c->vals.p,
+                       c->vals_alt.p);
     if (kind == 1)
         hipLaunchKernelGGL(k_setka<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, r2,
-                           c->prm.max_ve_g, key, c->vals.p, c->vals_alt.p, mask, c->col.p);
+                           c->prm.max_ve_g, c->vals.p, c->vals_alt.p, c->col.p);
     else
         hipLaunchKernelGGL(k_setka<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, r2,
-                           c->prm.max_ve_g, key, c->vals.p, c->vals_alt.p, mask, c->col.p);
+                           c->prm.max_ve_g, c->vals.p, c->vals_alt.p, c->col.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }

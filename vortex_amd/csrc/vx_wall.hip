@@ -10,8 +10,9 @@
 //                           and the panel loop stops there.
 //   V_i += nu II3 / II0.
 // Targets outside the box of birth points grown by 5 max(l) meet no panel and
-// are skipped without touching the panel list; the others are listed first
-// and then summed one wave per target.
+// are skipped without touching the panel list.
+#include <hipcub/hipcub.hpp>
+
 #include "vx_internal.h"
 
 namespace vx {
@@ -38,85 +39,79 @@ __device__ __forceinline__ void quad_i0_i3(T ra0, T ra1, T rb0, T rb1, T n0, T n
     res31 = s31;
 }
 
-// Targets inside the box, listed with one append per wave.
-__global__ __launch_bounds__(256) void k_wall_candidates(const double *__restrict__ x, const double *__restrict__ y,
-                                                         int64_t n_tgt, double4 box, int32_t *__restrict__ list,
-                                                         unsigned long long *__restrict__ cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool in = false;
-    if (i < n_tgt) {
-        const double xi = x[i], yi = y[i];
-        in = xi >= box.x && xi <= box.y && yi >= box.z && yi <= box.w;
+// One target per wave: the lanes take panels lane, lane + 64, ...  The first panel in
+// index order with r <= 0.1 l decides the result alone (the sequential loop
+// overwrites II0 and II3 there and stops); without one, the mid-range terms
+// are summed, lane partials in FP64 and then across the wave.
+template <typename T>
+__device__ void wave_wall(const PanelsSoA &pan, int lane, T a0, T a1, T dd, int64_t npts, double &II0,
+                          double &II30, double &II31) {
+    double c0 = 0.0, c30 = 0.0, c31 = 0.0;
+    int64_t fb = pan.P;
+    for (int64_t f = lane; f < pan.P; f += 64) {
+        const T dL = (T)pan.len[f];
+        const T q0 = a0 - (T)pan.bx[f], q1 = a1 - (T)pan.by[f];
+        const T r = sqrt(q0 * q0 + q1 * q1);
+        if ((r < T(5) * dL) && (r > T(0.1) * dL)) {
+            T R0, R30, R31;
+            quad_i0_i3<T>((T)pan.lx[f], (T)pan.ly[f], (T)pan.rx[f], (T)pan.ry[f], -(T)pan.nx[f], -(T)pan.ny[f], a0,
+                          a1, dL, dd, npts, R0, R30, R31);
+            c0 += (double)((-dd) * R0);
+            c30 += (double)R30;
+            c31 += (double)R31;
+        } else if (r <= T(0.1) * dL) {
+            fb = f;
+            break;
+        }
     }
-    const uint64_t m = __ballot(in);
-    if (m == 0) return;
-    const int lane = threadIdx.x & 63;
-    unsigned long long base = 0;
-    if (lane == __builtin_ctzll(m)) base = atomicAdd(cnt, (unsigned long long)__popcll(m));
-    base = __shfl(base, __builtin_ctzll(m), 64);
-    if (in) list[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)i;
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t ofb = __shfl_xor(fb, o, 64);
+        fb = ofb < fb ? ofb : fb;
+    }
+    if (fb < pan.P) {
+        const T dL = (T)pan.len[fb];
+        const T ex = T(1) - exp(-dL / (T(2) * dd));
+        II0 = (double)(T(kPi) * dd * dd);
+        II30 = (double)(T(2) * (T)pan.nx[fb] * dd * ex);
+        II31 = (double)(T(2) * (T)pan.ny[fb] * dd * ex);
+        return;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        c0 += __shfl_xor(c0, o, 64);
+        c30 += __shfl_xor(c30, o, 64);
+        c31 += __shfl_xor(c31, o, 64);
+    }
+    II0 = (double)(T(2 * kPi) * dd * dd) + c0;
+    II30 = -c30;
+    II31 = -c31;
 }
 
-__global__ void k_wall_zero(unsigned long long *__restrict__ cnt) { *cnt = 0ull; }
+struct InWallBox {
+    const double *x, *y;
+    double4 box;
+    __device__ bool operator()(int32_t i) const {
+        const double xi = x[i], yi = y[i];
+        return xi >= box.x && xi <= box.y && yi >= box.z && yi <= box.w;
+    }
+};
 
-// One wave per listed target (grid-stride, the count stays on the device);
-// lanes take panels lane, lane + 64, ...  The first panel in index order with
-// r <= 0.1 l decides the result alone (the sequential loop overwrites II0 and
-// II3 there and stops); without one, the mid-range terms are summed, lane
-// partials in FP64 and then across the wave.
+// One wave per target inside the box, selected in index order (grid-stride,
+// the count stays on the device).
 template <typename T>
 __global__ __launch_bounds__(256) void k_diffusion_wall(const double *__restrict__ x,
                                                         const double *__restrict__ y,
                                                         const double *__restrict__ d,
                                                         const int32_t *__restrict__ list,
-                                                        const unsigned long long *__restrict__ cnt,
-                                                        PanelsSoA pan, double nu, int64_t npts,
-                                                        double *__restrict__ u, double *__restrict__ v) {
+                                                        const int32_t *__restrict__ cnt, PanelsSoA pan,
+                                                        double nu, int64_t npts, double *__restrict__ u,
+                                                        double *__restrict__ v) {
     const int lane = threadIdx.x & 63;
     const int64_t nw = (int64_t)gridDim.x * 4;
-    const int64_t total = (int64_t)*cnt;
+    const int64_t total = *cnt;
     for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total; t += nw) {
         const int64_t i = list[t];
-        const T a0 = (T)x[i], a1 = (T)y[i], dd = (T)d[i];
-        double c0 = 0.0, c30 = 0.0, c31 = 0.0;
-        int64_t fb = pan.P;
-        for (int64_t f = lane; f < pan.P; f += 64) {
-            const T dL = (T)pan.len[f];
-            const T q0 = a0 - (T)pan.bx[f], q1 = a1 - (T)pan.by[f];
-            const T r = sqrt(q0 * q0 + q1 * q1);
-            if ((r < T(5) * dL) && (r > T(0.1) * dL)) {
-                T R0, R30, R31;
-                quad_i0_i3<T>((T)pan.lx[f], (T)pan.ly[f], (T)pan.rx[f], (T)pan.ry[f], -(T)pan.nx[f],
-                              -(T)pan.ny[f], a0, a1, dL, dd, npts, R0, R30, R31);
-                c0 += (double)((-dd) * R0);
-                c30 += (double)R30;
-                c31 += (double)R31;
-            } else if (r <= T(0.1) * dL) {
-                fb = f;
-                break;
-            }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const int64_t ofb = __shfl_xor(fb, o, 64);
-            fb = ofb < fb ? ofb : fb;
-        }
         double II0, II30, II31;
-        if (fb < pan.P) {
-            const T dL = (T)pan.len[fb];
-            const T ex = T(1) - exp(-dL / (T(2) * dd));
-            II0 = (double)(T(kPi) * dd * dd);
-            II30 = (double)(T(2) * (T)pan.nx[fb] * dd * ex);
-            II31 = (double)(T(2) * (T)pan.ny[fb] * dd * ex);
-        } else {
-            for (int o = 32; o > 0; o >>= 1) {
-                c0 += __shfl_xor(c0, o, 64);
-                c30 += __shfl_xor(c30, o, 64);
-                c31 += __shfl_xor(c31, o, 64);
-            }
-            II0 = (double)(T(2 * kPi) * dd * dd) + c0;
-            II30 = -c30;
-            II31 = -c31;
-        }
+        wave_wall<T>(pan, lane, (T)x[i], (T)y[i], (T)d[i], npts, II0, II30, II31);
         if (lane == 0) {
             u[i] += nu * II30 / II0;
             v[i] += nu * II31 / II0;
@@ -127,12 +122,14 @@ __global__ __launch_bounds__(256) void k_diffusion_wall(const double *__restrict
 int launch_diffusion_wall(vx_case *c, int64_t n_tgt) {
     if (n_tgt <= 0 || c->P <= 0) return 0;
     hipStream_t st = c->stream;
-    if (c->lists.ensure((size_t)n_tgt, st) || c->dcount.ensure(6, st)) return 1;
-    unsigned long long *cnt = (unsigned long long *)(c->dcount.p + 5);
-    const unsigned nb = (unsigned)((n_tgt + 255) / 256);
-    hipLaunchKernelGGL(k_wall_zero, dim3(1), dim3(1), 0, st, cnt);
-    hipLaunchKernelGGL(k_wall_candidates, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, n_tgt, c->wall_box,
-                       c->lists.p, cnt);
+    if (c->lists.ensure((size_t)n_tgt, st) || c->dcount.ensure(8, st)) return 1;
+    int32_t *cnt = (int32_t *)(c->dcount.p + 5);
+    const InWallBox pred{c->x.p, c->y.p, c->wall_box};
+    hipcub::CountingInputIterator<int32_t> it(0);
+    size_t bytes = 0;
+    VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, c->lists.p, cnt, n_tgt, pred, st));
+    if (c->cub_tmp.ensure(bytes, st)) return 1;
+    VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, c->lists.p, cnt, n_tgt, pred, st));
     if (c->prm.precision == VX_PREC_MIXED)
         hipLaunchKernelGGL(k_diffusion_wall<float>, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->d.p,
                            c->lists.p, cnt, c->pan, c->prm.nu, c->prm.n_of_points, c->u.p, c->v.p);
