@@ -40,6 +40,14 @@ namespace vx {
 constexpr int kLeaf = 64;
 constexpr double kTiny = 1e-300;
 
+// Traversal record of a node, one 32-byte load: centre, radius and either
+// the children (a = child0 >= 0, b = child1) or, for a bucket, its particles
+// (a = -1 - first, b = count).
+struct NodeRec {
+    double cx, cy, r;
+    int32_t a, b;
+};
+
 struct TreeBufs {
     DBuf<uint32_t> keys, keys_alt;
     DBuf<int32_t> idx, perm, incl, child0, child1, first, count, parent, fb;
@@ -49,6 +57,7 @@ struct TreeBufs {
     DBuf<double4> nbb, part;
     DBuf<double2> cd;
     DBuf<float2> cf;
+    DBuf<NodeRec> rec;
     DBuf<unsigned long long> cnt;
     DBuf<unsigned char> tmp;
     int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
@@ -68,12 +77,13 @@ struct TreeDev {
     const int32_t *child0, *child1, *first, *count;
     const double2 *cd;
     const float2 *cf;
+    const NodeRec *rec;
 };
 
 static TreeDev view(const TreeBufs &t) {
     return TreeDev{t.m,     t.nleaf, t.root,     t.perm.p,   t.sx.p,    t.sy.p,    t.sg.p, t.fx.p, t.fy.p, t.fg.p,
                    t.ncx.p, t.ncy.p, t.nr.p,     t.nbb.p,    t.child0.p, t.child1.p, t.first.p, t.count.p,
-                   t.cd.p,  t.cf.p};
+                   t.cd.p,  t.cf.p,  t.rec.p};
 }
 
 // ---- wave helpers ----------------------------------------------------------
@@ -423,6 +433,22 @@ __global__ void k_upper_build(int64_t nb, const int32_t *__restrict__ parent, in
     }
 }
 
+__global__ void k_pack_nodes(int64_t nn, const double *__restrict__ ncx, const double *__restrict__ ncy,
+                             const double *__restrict__ nr, const int32_t *__restrict__ child0,
+                             const int32_t *__restrict__ child1, const int32_t *__restrict__ first,
+                             const int32_t *__restrict__ count, NodeRec *__restrict__ rec) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nn) return;
+    NodeRec q;
+    q.cx = ncx[u];
+    q.cy = ncy[u];
+    q.r = nr[u];
+    const int32_t c0 = child0[u];
+    q.a = c0 >= 0 ? c0 : -1 - first[u];
+    q.b = c0 >= 0 ? child1[u] : count[u];
+    rec[u] = q;
+}
+
 // ---- traversal helpers ------------------------------------------------------
 // The tree above the buckets is at most 53 levels deep (32 key bits plus
 // the index bits that split equal keys), so a depth-first walk holds at most
@@ -514,12 +540,13 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
-    const int64_t leaf = xcd_block() * 4 + (threadIdx.x >> 6);
+    const int64_t leaf = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
     if (leaf >= t.nleaf) return;
-    const int64_t ti = (int64_t)t.first[leaf] + lane;
-    const int nvalid = t.count[leaf];
+    const NodeRec L = t.rec[leaf];
+    const int64_t ti = (int64_t)(-1 - L.a) + lane;
+    const int nvalid = L.b;
     const bool ok = lane < nvalid;
-    const double cAx = t.ncx[leaf], cAy = t.ncy[leaf], rA = t.nr[leaf];
+    const double cAx = L.cx, cAy = L.cy, rA = L.r;
     T ax, ay;
     if constexpr (F32) {
         ax = ok ? t.fx[ti] : 0.f;
@@ -529,31 +556,35 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
         ay = ok ? t.sy[ti] : cAy;
     }
     const T eps2 = (T)(eps * eps);
+    const double th2 = theta * theta;
     int64_t npairs = 0, nfar = 0;
     double su = 0.0, sv = 0.0;
+    T fu = 0, fv = 0;  // far-field sums, folded into su, sv at each near bucket
     T m0 = T(1e5), m1 = T(1e5), m2 = T(1e5), m3 = T(1e5);
-    double farmin = 1e300;
+    T farmin = T(1e30);
     LaneStack st;
     st.push(lane, t.root);
     while (st.sp > 0) {
         const int node = st.pop();
-        const double ncx = t.ncx[node], ncy = t.ncy[node], nr = t.nr[node];
-        const double ddx = ncx - cAx, ddy = ncy - cAy;
-        const double dist = sqrt(ddx * ddx + ddy * ddy);
-        if (dist * theta > nr + rA && dist - nr - rA > eps) {
+        const NodeRec nd = t.rec[node];
+        const double ddx = nd.cx - cAx, ddy = nd.cy - cAy;
+        const double d2 = ddx * ddx + ddy * ddy;
+        const double rs = nd.r + rA;
+        // dist * theta > r_node + r_leaf and dist - r_node - r_leaf > eps
+        if (d2 * th2 > rs * rs && d2 > (rs + eps) * (rs + eps)) {
             // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
             T Dx, Dy;
             if constexpr (F32) {
-                Dx = ax + (float)(cAx - ncx);
-                Dy = ay + (float)(cAy - ncy);
+                Dx = ax - (float)ddx;
+                Dy = ay - (float)ddy;
             } else {
-                Dx = ax - ncx;
-                Dy = ay - ncy;
+                Dx = ax - nd.cx;
+                Dy = ay - nd.cy;
             }
             const T den = Dx * Dx + Dy * Dy;
-            const T inv = T(1) / den;
+            const T inv = PM::div(T(1), den);
             const T qx = Dx * inv, qy = -Dy * inv;
-            const T R = (T)fmax(nr, kTiny);
+            const T R = (T)fmax(nd.r, kTiny);
             const T tx = R * qx, ty = R * qy;
             T sr, si;
             if constexpr (F32) {
@@ -577,18 +608,17 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
                     sr = nr_;
                 }
             }
-            const T Sx = sr * qx - si * qy, Sy = sr * qy + si * qx;
-            su += (double)Sy;
-            sv += (double)Sx;
-            farmin = fmin(farmin, sqrt((double)den) - nr);
+            fu += sr * qy + si * qx;
+            fv += sr * qx - si * qy;
+            farmin = fmin(farmin, sqrt(den) - (T)nd.r);
             ++nfar;
-        } else if (t.child0[node] < 0) {
-            const int64_t j0 = t.first[node];
-            const int cnt = t.count[node];
+        } else if (nd.a < 0) {
+            const int64_t j0 = -1 - nd.a;
+            const int cnt = nd.b;
             T ox, oy;
             if constexpr (F32) {
-                ox = ax + (float)(cAx - ncx);  // target relative to the source leaf centre
-                oy = ay + (float)(cAy - ncy);
+                ox = ax - (float)ddx;  // target relative to the source bucket centre
+                oy = ay - (float)ddy;
             } else {
                 ox = ax;
                 oy = ay;
@@ -609,13 +639,16 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
                 pu -= w * dy;
             }
             __builtin_amdgcn_wave_barrier();
-            su += (double)pu;
-            sv += (double)pv;
+            su += (double)pu + (double)fu;
+            sv += (double)pv + (double)fv;
+            fu = fv = 0;
         } else {
-            st.push(lane, t.child1[node]);
-            st.push(lane, t.child0[node]);
+            st.push(lane, nd.b);
+            st.push(lane, nd.a);
         }
     }
+    su += (double)fu;
+    sv += (double)fv;
     if (lane == 0) {
         atomicAdd(nfb + 1, (unsigned long long)(npairs * nvalid));
         atomicAdd(nfb + 2, (unsigned long long)(nfar * nvalid));
@@ -629,7 +662,8 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     // exact only if no far node can hold a closer particle than the third
     // FP32 squared distances of close pairs carry up to ~1e-5 relative error,
     // hence the 1e-3 margins here and in the search bound
-    if (exact_all || (double)m3 >= farmin * farmin * (1.0 - 1e-3)) {
+    const double fm = (double)farmin;
+    if (exact_all || (double)m3 >= fm * fm * (1.0 - 1e-3)) {
         const unsigned long long k = atomicAdd(nfb, 1ull);
         fb[k] = (int32_t)ti;
         fbr2[k] = (double)m3;
@@ -971,7 +1005,7 @@ void tree_release(vx_case *c) {
     t.fb.release(); t.rlo.release(); t.rhi.release(); t.rleft.release(); t.rright.release();
     t.bstart.release(); t.bscan.release(); t.upper.release(); t.uscan.release(); t.visits.release();
     t.sx.release(); t.sy.release(); t.sg.release(); t.ncx.release(); t.ncy.release(); t.nr.release();
-    t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release();
+    t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release(); t.rec.release();
     t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.fbr2.release(); t.hpart.release();
     delete c->tree;
     c->tree = nullptr;
@@ -1050,7 +1084,7 @@ int tree_build(vx_case *c, int64_t s) {
     const int64_t nn = t.nnode;
     if (t.ncx.ensure(nn, st) || t.ncy.ensure(nn, st) || t.nr.ensure(nn, st) || t.nbb.ensure(nn, st) ||
         t.child0.ensure(nn, st) || t.child1.ensure(nn, st) || t.first.ensure(nn, st) || t.count.ensure(nn, st) ||
-        t.parent.ensure(nn, st) || t.visits.ensure(nn, st) || t.cd.ensure(nn * order, st) ||
+        t.parent.ensure(nn, st) || t.visits.ensure(nn, st) || t.rec.ensure(nn, st) || t.cd.ensure(nn * order, st) ||
         t.cf.ensure(nn * order, st))
         return 1;
     VX_CHECK(hipMemsetAsync(t.parent.p, 0xff, sizeof(int32_t) * nn, st));
@@ -1081,6 +1115,8 @@ int tree_build(vx_case *c, int64_t s) {
         hipLaunchKernelGGL(k_upper_build<16>, dim3(gb), dim3(256), 0, st, nb, t.parent.p, t.visits.p, t.child0.p,
                            t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
     }
+    hipLaunchKernelGGL(k_pack_nodes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, nn, t.ncx.p, t.ncy.p,
+                       t.nr.p, t.child0.p, t.child1.p, t.first.p, t.count.p, t.rec.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
