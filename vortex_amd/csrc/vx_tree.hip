@@ -52,6 +52,7 @@ struct TreeBufs {
     DBuf<uint32_t> keys, keys_alt;
     DBuf<int32_t> idx, perm, incl, child0, child1, first, count, parent, fb;
     DBuf<int32_t> rlo, rhi, rleft, rright, bstart, bscan, upper, uscan, visits;
+    DBuf<int32_t> gcs, gcs2, gflag, gscan, gfirst;  // target groups
     DBuf<double> sx, sy, sg, ncx, ncy, nr, fbr2, hpart;
     DBuf<float> fx, fy, fg;
     DBuf<double4> nbb, part;
@@ -61,6 +62,7 @@ struct TreeBufs {
     DBuf<unsigned long long> cnt;
     DBuf<unsigned char> tmp;
     int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
+    int64_t ngroup = 0;                    // target groups (gfirst)
     int64_t heavy = 0;                     // diffusion targets summed by k_diff_far
     int32_t root = 0;
     int order = 16;
@@ -78,12 +80,14 @@ struct TreeDev {
     const double2 *cd;
     const float2 *cf;
     const NodeRec *rec;
+    const int32_t *gfirst;
+    int64_t ngroup;
 };
 
 static TreeDev view(const TreeBufs &t) {
     return TreeDev{t.m,     t.nleaf, t.root,     t.perm.p,   t.sx.p,    t.sy.p,    t.sg.p, t.fx.p, t.fy.p, t.fg.p,
                    t.ncx.p, t.ncy.p, t.nr.p,     t.nbb.p,    t.child0.p, t.child1.p, t.first.p, t.count.p,
-                   t.cd.p,  t.cf.p,  t.rec.p};
+                   t.cd.p,  t.cf.p,  t.rec.p, t.gfirst.p, t.ngroup};
 }
 
 // ---- wave helpers ----------------------------------------------------------
@@ -519,6 +523,57 @@ __device__ __forceinline__ void stage_leaf(Src4T<T> *lds, int lane, int64_t j0, 
     __builtin_amdgcn_wave_barrier();
 }
 
+// ---- target groups ------------------------------------------------------------
+// The targets of one wave: within each key-prefix cell of the bucket depth
+// (a cell holds about four times 64 particles at the mean density), runs of
+// 64 consecutive particles of the Morton order, so that nearly every lane has
+// a target and a group never straddles a jump of the curve between cells.
+// Box, centre and radius by wave reductions.
+__global__ void k_group_cells(const uint32_t *__restrict__ key, int64_t m, int shift, int32_t *__restrict__ cs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t c = shift >= 32 ? 0u : key[i] >> shift;
+    const bool start = i == 0 || (shift >= 32 ? 0u : key[i - 1] >> shift) != c;
+    cs[i] = start ? (int32_t)i : 0;
+}
+
+// cs2 = running maximum of cs: the start of each particle's cell.
+__global__ void k_group_flags(const int32_t *__restrict__ cs2, int64_t m, int32_t *__restrict__ gflag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) gflag[i] = ((i - cs2[i]) & 63) == 0 ? 1 : 0;
+}
+
+__global__ void k_group_first(const int32_t *__restrict__ gflag, const int32_t *__restrict__ gscan, int64_t m,
+                              int32_t *__restrict__ gfirst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m && gflag[i]) gfirst[gscan[i]] = (int32_t)i;
+}
+
+struct TargetGroup {
+    int64_t ti;
+    bool ok;
+    int nvalid;
+    double cx, cy, r;
+    double4 box;
+};
+
+__device__ __forceinline__ TargetGroup target_group(const TreeDev &t, int64_t grp, int lane) {
+    TargetGroup G;
+    const int64_t f = t.gfirst[grp];
+    const int64_t e = grp + 1 < t.ngroup ? (int64_t)t.gfirst[grp + 1] : t.m;
+    G.ti = f + lane;
+    G.nvalid = (int)(e - f);
+    G.ok = lane < G.nvalid;
+    const double xi = G.ok ? t.sx[G.ti] : 0.0, yi = G.ok ? t.sy[G.ti] : 0.0;
+    const double x0 = wave_min(G.ok ? xi : 1e300), x1 = wave_max(G.ok ? xi : -1e300);
+    const double y0 = wave_min(G.ok ? yi : 1e300), y1 = wave_max(G.ok ? yi : -1e300);
+    G.box = make_double4(x0, x1, y0, y1);
+    G.cx = 0.5 * (x0 + x1);
+    G.cy = 0.5 * (y0 + y1);
+    G.r = 0.5 * sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+    return G;
+}
+
 // ---- Biot-Savart + nearest-3 --------------------------------------------------
 // Work-groups are dispatched to the 8 XCDs in turn (work-group b to XCD b % 8).
 // Each XCD gets a contiguous eighth of the Morton-ordered buckets instead, so
@@ -540,6 +595,8 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
+    // targets: one bucket per wave (a compact node, which keeps the MAC's
+    // opening radius small)
     const int64_t leaf = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
     if (leaf >= t.nleaf) return;
     const NodeRec L = t.rec[leaf];
@@ -760,7 +817,7 @@ struct DiffPair<double> {
 
 // Targets are summed over the sources within cut * d_i of them (the omitted
 // weight is bounded by (1 + cut) e^{-cut} of the near sum).  One wave per
-// bucket: a node is visited when the box around the bucket within the
+// target group: a node is visited when the box around the group within the
 // largest radius of its lanes reaches it and, lane by lane, when some lane's
 // own radius reaches it.  A lane whose radius exceeds rbig is listed for
 // k_diff_far instead and does not widen the wave's radius.  cnt[0] +=
@@ -775,11 +832,11 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
     __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
-    const int64_t leaf = xcd_block() * 4 + (threadIdx.x >> 6);
-    if (leaf >= t.nleaf) return;
-    const int64_t ti = (int64_t)t.first[leaf] + lane;
-    const int nvalid = t.count[leaf];
-    const bool ok = lane < nvalid;
+    const int64_t grp = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
+    if (grp >= t.ngroup) return;
+    const TargetGroup G = target_group(t, grp, lane);
+    const int64_t ti = G.ti;
+    const bool ok = G.ok;
     const int32_t i = ok ? t.perm[ti] : 0;
     const double di = ok ? dd[i] : 0.0;
     const bool far = ok && cut * di > rbig;
@@ -787,12 +844,12 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
     const bool mine = ok && !far;
     const double R = cut * wave_max(mine ? di : 0.0);
     const double R2 = R * R;
-    const double cAx = t.ncx[leaf], cAy = t.ncy[leaf];
-    const double4 bA = t.nbb[leaf];
+    const double cAx = G.cx, cAy = G.cy;
+    const double4 bA = G.box;
     T ax, ay, kd, ri2;
     if constexpr (F32) {
-        ax = ok ? t.fx[ti] : 0.f;
-        ay = ok ? t.fy[ti] : 0.f;
+        ax = ok ? (float)(t.sx[ti] - cAx) : 0.f;  // target relative to the group centre
+        ay = ok ? (float)(t.sy[ti] - cAy) : 0.f;
         kd = (float)(-1.4426950408889634 / (ok ? di : 1.0));
         ri2 = mine ? (float)(cut * di * cut * di * (1.0 + 1e-4)) : -1.f;
     } else {
@@ -1006,6 +1063,7 @@ void tree_release(vx_case *c) {
     t.bstart.release(); t.bscan.release(); t.upper.release(); t.uscan.release(); t.visits.release();
     t.sx.release(); t.sy.release(); t.sg.release(); t.ncx.release(); t.ncy.release(); t.nr.release();
     t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release(); t.rec.release();
+    t.gcs.release(); t.gcs2.release(); t.gflag.release(); t.gscan.release(); t.gfirst.release();
     t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.fbr2.release(); t.hpart.release();
     delete c->tree;
     c->tree = nullptr;
@@ -1040,12 +1098,14 @@ int tree_build(vx_case *c, int64_t s) {
     const int64_t m = (int64_t)*c->h_count;
     t.m = m;
     t.nleaf = 0;
+    t.ngroup = 0;
     t.nnode = 0;
     if (m == 0) return 0;
     if (t.sx.ensure(m, st) || t.sy.ensure(m, st) || t.sg.ensure(m, st) || t.fx.ensure(m, st) || t.fy.ensure(m, st) ||
         t.fg.ensure(m, st) || t.fb.ensure(m, st) || t.fbr2.ensure(m, st) || t.rlo.ensure(m, st) || t.rhi.ensure(m, st) ||
         t.rleft.ensure(m, st) || t.rright.ensure(m, st) || t.bstart.ensure(m, st) || t.bscan.ensure(m, st) ||
-        t.upper.ensure(m, st) || t.uscan.ensure(m, st))
+        t.upper.ensure(m, st) || t.uscan.ensure(m, st) || t.gcs.ensure(m, st) || t.gcs2.ensure(m, st) ||
+        t.gflag.ensure(m, st) || t.gscan.ensure(m, st) || t.gfirst.ensure(m, st))
         return 1;
     const unsigned gm = (unsigned)((m + 255) / 256);
     hipLaunchKernelGGL(k_tree_gather, dim3(gm), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, t.perm.p, m, t.sx.p, t.sy.p,
@@ -1057,6 +1117,20 @@ int tree_build(vx_case *c, int64_t s) {
     // particles on average whatever the box's aspect ratio.
     const double lv = floor(0.5 * log2((double)m / kLeaf)) - 1.0;
     const int dmin = 2 * (int)fmin(fmax(lv, 0.0), 15.0);
+    {
+        hipLaunchKernelGGL(k_group_cells, dim3(gm), dim3(256), 0, st, t.keys_alt.p, m, 32 - dmin, t.gcs.p);
+        size_t b1 = 0, b2 = 0;
+        VX_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, b1, t.gcs.p, t.gcs2.p, hipcub::Max(), (int)m, st));
+        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, t.gflag.p, t.gscan.p, (int)m, st));
+        if (t.tmp.ensure(std::max(b1, b2), st)) return 1;
+        VX_CHECK(hipcub::DeviceScan::InclusiveScan(t.tmp.p, b1, t.gcs.p, t.gcs2.p, hipcub::Max(), (int)m, st));
+        hipLaunchKernelGGL(k_group_flags, dim3(gm), dim3(256), 0, st, t.gcs2.p, m, t.gflag.p);
+        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(t.tmp.p, b2, t.gflag.p, t.gscan.p, (int)m, st));
+        hipLaunchKernelGGL(k_group_first, dim3(gm), dim3(256), 0, st, t.gflag.p, t.gscan.p, m, t.gfirst.p);
+    }
+    int32_t hg[2];
+    VX_CHECK(hipMemcpyAsync(&hg[0], t.gscan.p + (m - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VX_CHECK(hipMemcpyAsync(&hg[1], t.gflag.p + (m - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
     if (m > kLeaf) {
         VX_CHECK(hipMemsetAsync(t.bstart.p, 0, sizeof(int32_t) * m, st));
         hipLaunchKernelGGL(k_radix, dim3(gm), dim3(256), 0, st, t.keys_alt.p, m, t.rlo.p, t.rhi.p, t.rleft.p,
@@ -1077,7 +1151,10 @@ int tree_build(vx_case *c, int64_t s) {
         VX_CHECK(hipStreamSynchronize(st));
         nb = (int64_t)h[0] + h[1];
         nu = (int64_t)h[2] + h[3];
+    } else {
+        VX_CHECK(hipStreamSynchronize(st));
     }
+    t.ngroup = (int64_t)hg[0] + hg[1];
     t.nleaf = nb;
     t.nnode = nb + nu;
     t.root = (int32_t)(nu > 0 ? nb : 0);
@@ -1169,8 +1246,7 @@ int launch_tree_diffusion(vx_case *c) {
     TreeBufs &t = *c->tree;
     hipStream_t st = c->stream;
     const TreeDev tv = view(t);
-    const unsigned lb = (unsigned)((t.nleaf + 3) / 4);
-    const unsigned lb8 = (lb + 7) / 8 * 8;
+    const unsigned lb8 = (unsigned)((t.ngroup + 3) / 4 + 7) / 8 * 8;  // target groups, padded for xcd_block()
     // omitted weights total at most (1 + c) e^{-c} of the near sum's scale:
     // 3e-7 at c = 18 (the FP32 rounding level of the sums), 3e-12 at c = 30
     const bool mixed = c->prm.precision == VX_PREC_MIXED;
