@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--f64-steps", type=int, default=2)
-    ap.add_argument("--cpu-targets", type=int, default=2048)
+    ap.add_argument("--cpu-targets", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--direct", action="store_true",
