@@ -124,6 +124,22 @@ int vx_run_steps(vx_case *c, int64_t steps, double vinf_x, double vinf_y, double
 int vx_velocity_probe(vx_case *c, const double *px, const double *py, int64_t m,
                       double vinf_x, double vinf_y, double *u, double *v);
 
+/* ---- second component (tracer points) ------------------------------------
+ * The reference advects two copies of the points of second_points_file each
+ * step (src/main.c:729-741): passive ones with the flow velocity
+ * (first_speed, src/unit_1.cu:1271-1298) and inertial ones whose velocity
+ * relaxes towards it with time constant rel_t (second_speed,
+ * src/unit_1.cu:1300-1340).  The flow velocity at the points is
+ * vx_velocity_probe.  Host arrays of m points.
+ *   vx_points_move   step_Kernel on points (src/kernel.cu:1090-1143 with null
+ *                    force pointers): x += u dt; absorbed by the profile -> g = 0;
+ *                    out of the box -> (-2e5, -2e5), g = 0
+ *   vx_points_relax  second_speed_Kernel (src/kernel.cu:1478-1489), host:
+ *                    u = (u_env - u_prev) / rel_t * dt + u_prev, u_prev in FP32 */
+int vx_points_move(vx_case *c, double *x, double *y, double *g, const double *u, const double *v, int64_t m);
+int vx_points_relax(const double *g, const double *uenv, const double *venv, double *u, double *v, int64_t m,
+                    double rel_t, double dt);
+
 /* ---- stage-level entry points (parity tests, profiling) ------------------ */
 int vx_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y, double *R_p);
 int vx_birth(vx_case *c, const double *R_p, double d_g, double *gsheet);   /* writes [n, n+P) */

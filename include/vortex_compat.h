@@ -29,8 +29,10 @@
  *   Step                   include/unit_1.cuh:46, src/unit_1.cu:1077-1218
  *   init_device_conf_values include/unit_1.cuh:48, src/unit_1.cu:1220-1252
  *   velocity_control       include/unit_1.cuh:51, src/unit_1.cu:1254-1269
- * Not provided: second_speed / first_speed (the passive and inertial second
- * component, include/unit_1.cuh:52-53), which is outside this library.
+ *   second_speed           include/unit_1.cuh:52, src/unit_1.cu:1300-1340
+ *   first_speed            include/unit_1.cuh:53, src/unit_1.cu:1271-1298
+ * The second-component calls write output/vels/VelNNNNNN_S.txt for the
+ * caller's current_step as the reference's do (the directory must exist).
  */
 #ifndef VORTEX_COMPAT_H_
 #define VORTEX_COMPAT_H_
@@ -103,6 +105,10 @@ struct conf_t {
 };
 
 extern struct conf_t conf;
+/* the reference's step counters (src/main.c:18-19), read by second_speed and
+ * first_speed; the caller defines them as the reference's main.c does */
+extern int current_step;
+extern int first_step;
 
 TVars *matr_creation(tPanel *panels, size_t s, size_t birth);
 TVars *load_matrix(size_t *p);
@@ -117,6 +123,10 @@ int Speed(Vortex *pos, TVctr *v_inf, size_t s, PVortex *v, TVars *d, TVars nu, t
 int Step(Vortex *pos, PVortex *V, size_t *n, size_t s, TVars *d_g, PVortex *F_p, TVars *M, tPanel *panels);
 int init_device_conf_values(void);
 int velocity_control(Vortex *pos, TVctr *V_inf, size_t n, Vortex *Contr_points, PVortex *V, size_t n_contr);
+int second_speed(Vortex *pos, TVctr *V_inf, size_t n, Vortex *second_pos, PVortex *v_second, PVortex *v_env,
+                 size_t *n_second, tPanel *panels);
+int first_speed(Vortex *pos, TVctr *V_inf, size_t n, Vortex *second_pos, PVortex *v_second, size_t *n_second,
+                tPanel *panels);
 
 /* Select the arithmetic of the compat entry points (VX_PREC_F64 by default). */
 int vx_compat_set_precision(int precision);

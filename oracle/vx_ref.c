@@ -475,6 +475,40 @@ void vxr_step_move(double *x, double *y, double *g, const double *u, const doubl
     if (!m_i) free(mv);
 }
 
+/* ---- second component: src/unit_1.cu:1271-1340, src/kernel.cu:1478-1489 -- */
+void vxr_points_move(double *x, double *y, double *g, const double *u, const double *v, int64_t m,
+                     int64_t P, const vxr_panel *pan, const vxr_params *prm) {
+    const double dt = prm->dt;
+    for (int64_t i = 0; i < m; ++i) {
+        const double xn = x[i] + u[i] * dt;
+        const double yn = y[i] + v[i] * dt;
+        int hp = -1;
+        if ((g[i] != 0) && vxr_hitting(pan, P, prm, xn, yn, x[i], y[i], &hp)) g[i] = 0;
+        x[i] = xn;
+        y[i] = yn;
+        if ((g[i] != 0) && ((x[i] > prm->x_max) || (x[i] < prm->x_min) ||
+                            (y[i] > prm->y_max) || (y[i] < prm->y_min))) {
+            x[i] = -2.0e+5;
+            y[i] = -2.0e+5;
+            g[i] = 0;
+        }
+    }
+}
+
+void vxr_points_relax(const double *g, const double *uenv, const double *venv, double *u, double *v,
+                      int64_t m, double rel_t, double dt) {
+    for (int64_t i = 0; i < m; ++i) {
+        if (!g[i]) {
+            u[i] = 0.0;
+            v[i] = 0.0;
+            continue;
+        }
+        const float pu = (float)u[i], pv = (float)v[i];
+        u[i] = (uenv[i] - pu) / rel_t * dt + pu;
+        v[i] = (venv[i] - pv) / rel_t * dt + pv;
+    }
+}
+
 /* ---- A11: src/kernel.cu:1159-1178 --------------------------------------- */
 int64_t vxr_compact(double *x, double *y, double *g, int64_t n) {
     for (int64_t i = 0; i < n; ++i) {

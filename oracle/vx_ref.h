@@ -128,6 +128,16 @@ void vxr_step_move(double *x, double *y, double *g, const double *u, const doubl
 int  vxr_hitting(const vxr_panel *pan, int64_t P, const vxr_params *prm,
                  double new_x, double new_y, double old_x, double old_y, int *pan_out);
 
+/* Second component: step_Kernel on m tracer points with null force pointers
+ * (src/unit_1.cu:1295,1334; src/kernel.cu:1090-1143): move, absorption
+ * (g = 0), deletion; no impulse terms. */
+void vxr_points_move(double *x, double *y, double *g, const double *u, const double *v, int64_t m,
+                     int64_t P, const vxr_panel *pan, const vxr_params *prm);
+/* second_speed_Kernel (src/kernel.cu:1478-1489): u = (u_env - u_prev)/rel_t*dt
+ * + u_prev with u_prev read as float; 0 where g = 0. */
+void vxr_points_relax(const double *g, const double *uenv, const double *venv, double *u, double *v,
+                      int64_t m, double rel_t, double dt);
+
 /* A11: swap-with-last removal of |g| < 1e-12 (src/kernel.cu:1159-1178). */
 int64_t vxr_compact(double *x, double *y, double *g, int64_t n);
 

@@ -60,6 +60,8 @@ def lib():
                                     _vp, _vp, _vp, _vp, _dp]
         L.vxr_hitting.argtypes = [_vp, _i64, C.POINTER(Params), _d, _d, _d, _d, C.POINTER(C.c_int)]
         L.vxr_hitting.restype = C.c_int
+        L.vxr_points_move.argtypes = [_dp, _dp, _dp, _dp, _dp, _i64, _i64, _vp, C.POINTER(Params)]
+        L.vxr_points_relax.argtypes = [_dp, _dp, _dp, _dp, _dp, _i64, _d, _d]
         L.vxr_compact.argtypes = [_dp, _dp, _dp, _i64]
         L.vxr_compact.restype = _i64
         L.vxr_setka.argtypes = [_dp, _dp, _dp, _i64, C.c_int, _d, _d, _ip]
@@ -175,6 +177,18 @@ def hitting(pan, prm: Params, new, old):
     r = lib().vxr_hitting(_pp(pan), pan.size, C.byref(prm), new[0], new[1], old[0], old[1],
                           C.byref(hp))
     return bool(r), hp.value
+
+
+def points_move(x, y, g, u, v, pan, prm: Params):
+    x, y, g = _f(x).copy(), _f(y).copy(), _f(g).copy()
+    lib().vxr_points_move(x, y, g, _f(u), _f(v), x.size, pan.size, _pp(pan), C.byref(prm))
+    return x, y, g
+
+
+def points_relax(g, uenv, venv, u, v, rel_t, dt):
+    u, v = _f(u).copy(), _f(v).copy()
+    lib().vxr_points_relax(_f(g), _f(uenv), _f(venv), u, v, u.size, rel_t, dt)
+    return u, v
 
 
 def compact(x, y, g, n=None):

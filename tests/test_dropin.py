@@ -105,7 +105,37 @@ def test_cli_parses_reference_run_files(name):
         assert js["P"] == 1000
 
 
-# ---------------------------------------------------------------- GPU
+def write_points(path, x, y, g):
+    with open(path, "w") as f:
+        f.write(f"{len(x)}\n")
+        for i in range(len(x)):
+            f.write(f"{i} 0.008 {float(x[i])!r} {float(y[i])!r} 0 0 0 {float(g[i])!r}\n")
+
+
+def tracer_points(n=40):
+    th = np.linspace(0.3, 2 * np.pi - 0.3, n)
+    r = np.linspace(0.51, 0.9, n)
+    return -0.6 + 0.0 * th + r * np.cos(th) * 0.2, r * np.sin(th), np.ones(n)
+
+
+def test_cli_second_component_requires_rel_t(tmp_path):
+    """src/main.c:480-484: a second component without rel_t stops the run;
+    steady_flow without one too (:506-509).  Both before any GPU work."""
+    tmp = str(tmp_path)
+    write_case(tmp)
+    write_points(os.path.join(tmp, "pts.txt"), *tracer_points())
+    with open(os.path.join(tmp, "run.conf"), "a") as f:
+        f.write("second_points_file pts.txt\nrel_t 0\n")
+    out = subprocess.run([CLI, "-c", "run.conf"], cwd=tmp, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "rel_t is nessesary" in out.stdout
+    write_case(tmp)
+    with open(os.path.join(tmp, "run.conf"), "a") as f:
+        f.write("steady_flow 1\n")
+    out = subprocess.run([CLI, "-c", "run.conf"], cwd=tmp, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "steady_flow and no second component" in out.stdout
+
+
+# ---------------------------------------------------------------- GPU# ---------------------------------------------------------------- GPU
 @pytest.mark.gpu
 def test_cli_run_writes_reference_outputs(tmp_path):
     from vortex_amd.solver import Case, make_params
@@ -157,7 +187,8 @@ def test_compat_driver_matches_native_api(tmp_path, prec):
     args = [DRIVER, "profile.txt", str(steps)] + (["mixed"] if prec == "mixed" else [])
     out = subprocess.run(args, cwd=tmp, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
-    rows = np.array([[float(v) for v in line.split()] for line in out.stdout.strip().splitlines()])
+    rows = np.array([[float(v) for v in line.split()] for line in out.stdout.strip().splitlines()
+                     if not line.startswith(("first", "second"))])
     eps = float(pan["length"][0])
     c = Case(make_params(pan, dt=0.01, ve_size=eps, max_ve_g=2 * eps, r_col_diff_sign=0.3, r_col_same_sign=0.3,
                          precision=VX_PREC_MIXED if prec == "mixed" else VX_PREC_F64), pan)
@@ -167,3 +198,94 @@ def test_compat_driver_matches_native_api(tmp_path, prec):
         assert rows[k, 1] == c.n, k
         assert np.allclose(rows[k, 2:], f[1:], rtol=1e-12, atol=1e-12), (k, rows[k], f)
     assert os.path.exists(os.path.join(tmp, "M.txt"))  # matr_creation writes it, as the reference does
+
+
+@pytest.mark.gpu
+def test_cli_second_component_matches_native_api(tmp_path):
+    """Inertial and passive tracer points (src/main.c:729-741): the CLI's point
+    snapshots and velocity files against the same loop on the native API."""
+    from vortex_amd.solver import Case, make_params, points_relax
+    tmp = str(tmp_path)
+    pan, eps = write_case(tmp, steps=4, save=2)
+    px, py, pg = tracer_points()
+    write_points(os.path.join(tmp, "pts.txt"), px, py, pg)
+    with open(os.path.join(tmp, "run.conf"), "a") as f:
+        f.write("second_points_file pts.txt\nrel_t 0.1\n")
+    out = subprocess.run([CLI, "-c", "run.conf"], cwd=tmp, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    for sub in ("first", "second"):
+        assert sorted(os.listdir(os.path.join(tmp, "output", sub))) == [
+            "Kadr000000.txt", "Kadr000002.txt", "Kadr000004.txt"]
+    assert os.path.exists(os.path.join(tmp, "output", "vels", "Vel000003_4.txt"))
+    # the same loop on the native API
+    c = Case(make_params(pan, dt=0.01, ve_size=eps, max_ve_g=2 * eps, r_col_diff_sign=0.3, r_col_same_sign=0.3),
+             pan)
+    c.build_matrix()
+    fx, fy, fgv = px.copy(), py.copy(), pg.copy()
+    sx, sy, sg = px.copy(), py.copy(), pg.copy()
+    su = sv = None
+    for k in range(4):
+        c.vort_creation((1.0, 0.0))
+        ue, ve = c.velocity_probe(sx, sy, (1.0, 0.0))
+        if su is None:
+            su, sv = ue.copy(), ve.copy()
+        if k == 3:
+            v4 = np.loadtxt(os.path.join(tmp, "output", "vels", "Vel000003_4.txt"), skiprows=1)
+            assert np.allclose(v4[:, 3], ue, atol=2e-6) and np.allclose(v4[:, 1], sx, atol=2e-6)
+        su, sv = points_relax(sg, ue, ve, su, sv, 0.1, 0.01)
+        sx, sy, sg = c.points_move(sx, sy, sg, su, sv)
+        uf, vf = c.velocity_probe(fx, fy, (1.0, 0.0))
+        fx, fy, fgv = c.points_move(fx, fy, fgv, uf, vf)
+        c.speed((1.0, 0.0))
+        c.step()
+    for sub, (x, y, g) in (("first", (fx, fy, fgv)), ("second", (sx, sy, sg))):
+        snap = np.loadtxt(os.path.join(tmp, "output", sub, "Kadr000004.txt"), skiprows=1)
+        assert np.allclose(snap[:, 2], x, atol=2e-6) and np.allclose(snap[:, 3], y, atol=2e-6)
+        assert np.array_equal(snap[:, 7], g)
+    assert not np.allclose(fx, sx)  # inertia separates the two copies
+
+
+@pytest.mark.gpu
+def test_compat_second_component_matches_native_api(tmp_path):
+    """second_speed / first_speed through the reference-named entry points
+    against the same loop on the native API (FP64)."""
+    from vortex_amd.solver import Case, make_params, points_relax
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "output", "vels"))
+    pan = geo.circle(64, 0.5)
+    geo.write_profile(os.path.join(tmp, "profile.txt"), pan)
+    px, py, pg = tracer_points(24)
+    write_points(os.path.join(tmp, "pts.txt"), px, py, pg)
+    steps = 5
+    out = subprocess.run([DRIVER, "profile.txt", str(steps), "f64", "pts.txt"], cwd=tmp, capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    pts = {"first": [], "second": []}
+    for line in out.stdout.splitlines():
+        tok = line.split()
+        if tok[0] in pts:
+            pts[tok[0]].append([float(v) for v in tok[1:]])
+    eps = float(pan["length"][0])
+    c = Case(make_params(pan, dt=0.01, ve_size=eps, max_ve_g=2 * eps, r_col_diff_sign=0.3, r_col_same_sign=0.3),
+             pan)
+    c.build_matrix()
+    fx, fy, fgv = px.copy(), py.copy(), pg.copy()
+    sx, sy, sg = px.copy(), py.copy(), pg.copy()
+    su = sv = None
+    for k in range(steps):
+        c.vort_creation((1.0, 0.0))
+        ue, ve = c.velocity_probe(sx, sy, (1.0, 0.0))
+        if su is None:
+            su, sv = ue.copy(), ve.copy()
+        su, sv = points_relax(sg, ue, ve, su, sv, 0.1, 0.01)
+        sx, sy, sg = c.points_move(sx, sy, sg, su, sv)
+        uf, vf = c.velocity_probe(fx, fy, (1.0, 0.0))
+        fx, fy, fgv = c.points_move(fx, fy, fgv, uf, vf)
+        c.speed((1.0, 0.0))
+        c.step()
+    for name, (x, y, g) in (("first", (fx, fy, fgv)), ("second", (sx, sy, sg))):
+        a = np.array(pts[name])
+        assert a.shape == (len(px), 3)
+        assert np.allclose(a[:, 0], x, rtol=0, atol=1e-12) and np.allclose(a[:, 1], y, rtol=0, atol=1e-12)
+        assert np.array_equal(a[:, 2], g)
+    assert os.path.exists(os.path.join(tmp, "output", "vels", "Vel000004_4.txt"))

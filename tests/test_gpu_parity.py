@@ -226,6 +226,27 @@ def test_step_move_bitexact(ref, seed):
         assert abs(out[q] - outr[q]) <= 1e-12 * max(1.0, abs(outr[q]))
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_points_move_and_relax_bitexact(ref, seed):
+    """Second component: tracer points move, are absorbed and deleted as the
+    oracle does; the inertial relaxation matches it bit for bit."""
+    from vortex_amd.solver import points_relax
+    pan = geo.circle(200, 0.5)
+    c, prm = make_case(pan, VX_PREC_F64, dt=0.02)
+    x, y, g, u, v = crossing_state(pan, 3000, seed)
+    g = np.where(g != 0, 1.0, 0.0)
+    X, Y, G = c.points_move(x, y, g, u, v)
+    xr, yr, gr = ref.points_move(x, y, g, u, v, pan, ref_params(prm))
+    assert np.array_equal(X, xr) and np.array_equal(Y, yr) and np.array_equal(G, gr)
+    assert np.count_nonzero((g != 0) & (gr == 0) & (xr != -2e5)) > 20   # some absorbed
+    assert np.count_nonzero(xr == -2e5) > 5                              # some deleted
+    rng = np.random.default_rng(seed)
+    ue, ve = rng.normal(0, 1, x.size), rng.normal(0, 1, x.size)
+    U, V = points_relax(G, ue, ve, u, v, 0.1, 0.02)
+    ur, vr = ref.points_relax(gr, ue, ve, u, v, 0.1, 0.02)
+    assert np.array_equal(U, ur) and np.array_equal(V, vr)
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_compact_bitexact(ref, cyl, seed):
     rng = np.random.default_rng(seed)

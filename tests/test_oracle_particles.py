@@ -218,6 +218,40 @@ def test_step_move_absorb_delete_and_impulse(ref):
     assert out[3] == pytest.approx(m.sum() / (2 * prm.dt), rel=1e-15)
 
 
+# ---------------------------------------------------------------- second component
+def test_points_move_absorb_delete_no_impulse(ref):
+    """Tracer points move like vortices but carry no impulse terms; the same
+    point set as the vortex case gives the same positions and flags."""
+    pan = square_body()
+    prm = params(pan, dt=1.0)
+    x = np.array([0.0, 4.5, 3.0, 2.0, 0.0]); y = np.array([2.0, 0.0, 3.0, 2.0, -3.0])
+    g = np.array([1.0, 1.0, 0.0, 1.0, 1.0])
+    u = np.array([0.0, 1.0, 0.0, 0.1, 0.0]); v = np.array([-1.5, 0.0, 0.0, 0.0, 0.1])
+    xo, yo, go = ref.points_move(x, y, g, u, v, pan, prm)
+    assert go[0] == 0 and (xo[0], yo[0]) == (0.0, 0.5)       # absorbed at the top panel
+    assert go[1] == 0 and (xo[1], yo[1]) == (-2e5, -2e5)      # left the box
+    assert go[2] == 0 and (xo[2], yo[2]) == (3.0, 3.0)        # dead, moved only
+    assert go[3] == 1.0 and xo[3] == 2.1                      # free
+    assert go[4] == 1.0 and (xo[4], yo[4]) == (0.0, -2.9)
+    vx_, vy_, vg, _, _ = ref.step_move(x, y, g, u, v, 5, pan, prm)
+    assert np.array_equal(xo, vx_) and np.array_equal(yo, vy_) and np.array_equal(go, vg)
+
+
+def test_points_relax_known_answers(ref):
+    """v = (v_env - v_prev)/rel_t*dt + v_prev with v_prev read as FP32; 0 where
+    g = 0 (src/kernel.cu:1478-1489)."""
+    g = np.array([1.0, 1.0, 0.0, 2.0])
+    ue = np.array([1.5, 0.25, 9.0, 0.1]); ve = np.array([-0.5, 0.25, 9.0, 0.3])
+    u = np.array([0.5, 0.25, 4.0, 0.1]); v = np.array([0.5, 0.25, 4.0, 0.3])
+    uo, vo = ref.points_relax(g, ue, ve, u, v, 0.5, 0.25)
+    assert (uo[0], vo[0]) == (1.0, 0.0)                 # exact binary fractions
+    assert (uo[1], vo[1]) == (0.25, 0.25)               # at rest in the flow
+    assert (uo[2], vo[2]) == (0.0, 0.0)                 # absorbed point stops
+    # a previous velocity that FP32 cannot hold is rounded before use
+    p = float(np.float32(0.1))
+    assert p != 0.1 and uo[3] == (0.1 - p) / 0.5 * 0.25 + p
+
+
 # ---------------------------------------------------------------- A11
 @pytest.mark.parametrize("seed", range(12))
 def test_compact_matches_closed_form(ref, seed):

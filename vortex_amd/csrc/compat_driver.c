@@ -1,10 +1,12 @@
-/* compat_driver PROFILE STEPS [mixed]
+/* compat_driver PROFILE STEPS [mixed|f64] [POINTS]
  *
  * A run loop written only against the reference's host API
  * (include/vortex_compat.h), in the order of src/main.c:652-785: grow the
  * arrays when needed, vort_creation, Speed, Step; prints one line per step:
- * "step n Fx Fy M".  Used to check that the compat entry points reproduce
- * the native API on the same case. */
+ * "step n Fx Fy M".  With a POINTS file (Kadr format) the second component
+ * runs too (second_speed, first_speed after birth, rel_t = 0.1) and the final
+ * points are printed as "first|second x y g".  Used to check that the compat
+ * entry points reproduce the native API on the same case. */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -50,6 +52,32 @@ int main(int argc, char **argv) {
     conf.max_ve_g = 2 * conf.ve_size;
     const int steps = atoi(argv[2]);
     if (argc > 3 && !strcmp(argv[3], "mixed")) vx_compat_set_precision(1);
+    /* second component: passive (first) and inertial (second) copies */
+    size_t m = 0;
+    Vortex *p1_h = NULL, *p1_d = NULL, *p2_d = NULL;
+    PVortex *v1_d = NULL, *v2_d = NULL, *venv_d = NULL;
+    if (argc > 4) {
+        FILE *f = fopen(argv[4], "r");
+        if (!f || fscanf(f, "%zu", &m) != 1) { fprintf(stderr, "points\n"); return 1; }
+        p1_h = (Vortex *)calloc(m, sizeof(Vortex));
+        for (size_t i = 0; i < m; ++i)
+            if (fscanf(f, " %*u %*f %lf %lf %*f %*f %*f %lf", &p1_h[i].r[0], &p1_h[i].r[1], &p1_h[i].g) != 3) {
+                fprintf(stderr, "points\n");
+                return 1;
+            }
+        fclose(f);
+        conf.rel_t = 0.1;
+        hipMalloc((void **)&p1_d, m * sizeof(Vortex));
+        hipMalloc((void **)&p2_d, m * sizeof(Vortex));
+        hipMemcpy(p1_d, p1_h, m * sizeof(Vortex), hipMemcpyHostToDevice);
+        hipMemcpy(p2_d, p1_h, m * sizeof(Vortex), hipMemcpyHostToDevice);
+        hipMalloc((void **)&v1_d, m * sizeof(PVortex));
+        hipMalloc((void **)&v2_d, m * sizeof(PVortex));
+        hipMalloc((void **)&venv_d, m * sizeof(PVortex));
+        hipMemset(v1_d, 0, m * sizeof(PVortex));
+        hipMemset(v2_d, 0, m * sizeof(PVortex));
+        hipMemset(venv_d, 0, m * sizeof(PVortex));
+    }
     if (init_device_conf_values()) return 1;
     TVars *M = matr_creation(pan_h, conf.birth_quant, birth);
     if (!M) { fprintf(stderr, "matrix\n"); return 1; }
@@ -73,6 +101,11 @@ int main(int argc, char **argv) {
         if (s > size && incr_vort_quant(&pos_h, &pos_d, &V_h, &V_d, &d_d, &size)) return 1;
         if (vort_creation(pos_d, vinf_d, conf.birth_quant, birth, n, M_d, dg_d, pan_d)) return 1;
         n += conf.birth_quant;
+        if (m) {
+            current_step = step;
+            if (second_speed(pos_d, vinf_d, n, p2_d, v2_d, venv_d, &m, pan_d)) return 1;
+            if (first_speed(pos_d, vinf_d, n, p1_d, v1_d, &m, pan_d)) return 1;
+        }
         if (Speed(pos_d, vinf_d, s, V_d, d_d, conf.viscosity, pan_d)) return 1;
         hipMemcpy(F_d, zero, sizeof(PVortex), hipMemcpyHostToDevice);
         hipMemcpy(dg_d, zero, sizeof(TVars), hipMemcpyHostToDevice);
@@ -83,6 +116,11 @@ int main(int argc, char **argv) {
         hipMemcpy(&Mo, Mom_d, sizeof Mo, hipMemcpyDeviceToHost);
         printf("%d %zu %.17g %.17g %.17g\n", step, n, F.v[0], F.v[1], Mo);
     }
-    free(M); free(pan_h); free(pos_h); free(V_h);
+    for (int which = 0; which < 2 && m; ++which) {
+        hipMemcpy(p1_h, which ? p2_d : p1_d, m * sizeof(Vortex), hipMemcpyDeviceToHost);
+        for (size_t i = 0; i < m; ++i)
+            printf("%s %.17g %.17g %.17g\n", which ? "second" : "first", p1_h[i].r[0], p1_h[i].r[1], p1_h[i].g);
+    }
+    free(M); free(pan_h); free(pos_h); free(V_h); free(p1_h);
     return 0;
 }

@@ -5,11 +5,14 @@
 // load the panel-system inverse, start from an empty wake or a Kadr snapshot
 // (the start step is taken from the file name), then per step:
 //   ramp V_inf, birth, timings line, every saving_step the Kadr snapshot and
-//   the probe velocities, probe velocities of this step, the force line of
-//   the previous step, Speed, Step
+//   the probe velocities, the second component (inertial, then passive
+//   tracer points), probe velocities of this step, the force line of the
+//   previous step, Speed, Step
 // and finally the last snapshot and force line.  Outputs go to
-// output/kadrs/KadrNNNNNN.txt, output/vels/contr_VelNNNNNN.txt and
-// output/F_p.txt in the reference's text formats.
+// output/kadrs/KadrNNNNNN.txt, output/vels/contr_VelNNNNNN.txt,
+// output/vels/VelNNNNNN_S.txt, output/first|second/KadrNNNNNN.txt and
+// output/F_p.txt in the reference's text formats.  With steady_flow the
+// wake is frozen (no birth, Speed or Step) and only the points move.
 #include <sys/stat.h>
 #include <sys/types.h>
 
@@ -61,6 +64,32 @@ int fail(vx_case *c, const char *what) {
     if (c) vx_case_destroy(c);
     return 1;
 }
+
+// save_vel_to_file (src/unit_1.cu:616-631): output/vels/VelNNNNNN_S.txt.
+void write_vels(int64_t step, int stage, const std::vector<double> &x, const std::vector<double> &y,
+                const std::vector<double> &u, const std::vector<double> &v) {
+    char fn[256];
+    snprintf(fn, sizeof(fn), "output/vels/Vel%06" PRId64 "_%d.txt", step, stage);
+    FILE *f = fopen(fn, "w");
+    if (!f) {
+        logf(1, "error file opening %s : %s", fn, strerror(errno));
+        return;
+    }
+    fprintf(f, "%zu\n", x.size());
+    for (size_t i = 0; i < x.size(); ++i) fprintf(f, "%zu %lf %lf %lf %lf\n", i, x[i], y[i], u[i], v[i]);
+    fclose(f);
+}
+
+// One component of tracer points (src/main.c:480-509): positions, circulation
+// flag (0 once absorbed or deleted) and velocity.
+struct Points {
+    std::vector<double> x, y, g, u, v;
+    size_t size() const { return x.size(); }
+    void resize_vel() {
+        u.assign(x.size(), 0.0);
+        v.assign(x.size(), 0.0);
+    }
+};
 
 }  // namespace
 
@@ -115,8 +144,15 @@ int main(int argc, char **argv) {
     }
     if (!log_path.empty()) g_log = fopen(log_path.c_str(), "w");
     FILE *timings = conf.timings_file.empty() ? nullptr : fopen(conf.timings_file.c_str(), "w");
-    if (!conf.second_points_file.empty() || conf.steady_flow)
-        logf(2, "second component (second_points_file / steady_flow) is not supported; running the vortex wake only");
+    const bool have_points = !conf.second_points_file.empty();
+    if (have_points && conf.rel_t == 0.0) {
+        logf(1, "rel_t is nessesary if second component set");
+        return 1;
+    }
+    if (!have_points && conf.steady_flow) {
+        logf(1, "steady_flow and no second component");
+        return 1;
+    }
 
     std::vector<vx_panel> pan;
     if (vxio::read_profile(conf.pr_file, pan, &err)) {
@@ -153,6 +189,25 @@ int main(int argc, char **argv) {
         logf(3, "Matrix created!");
     }
 
+    // second component: inertial (second) and passive (first) copies of the
+    // points (src/main.c:480-509)
+    Points first, second;
+    if (have_points) {
+        if (vxio::read_kadr(conf.second_points_file, second.x, second.y, second.g, &err)) {
+            logf(1, "%s", err.c_str());
+            vx_case_destroy(c);
+            return 1;
+        }
+        first = second;
+        first.resize_vel();
+        second.resize_vel();
+        if (make_dir("output/first") || make_dir("output/second")) {
+            logf(1, "cannot create output directories: %s", strerror(errno));
+            vx_case_destroy(c);
+            return 1;
+        }
+    }
+
     int64_t step = 0;
     if (!conf.kadr_file.empty()) {
         std::vector<double> x, y, g;
@@ -166,6 +221,28 @@ int main(int argc, char **argv) {
         if (k >= 0) {
             step = k;
             conf.steps += k;
+        }
+    }
+    const int64_t first_step = step;
+    if (step && have_points) {
+        // restart of the points from their own snapshots (src/main.c:606-648)
+        char fs[256], ff[256];
+        snprintf(fs, sizeof(fs), "output/second/Kadr%06" PRId64 ".txt", step);
+        snprintf(ff, sizeof(ff), "output/first/Kadr%06" PRId64 ".txt", step);
+        struct stat st;
+        if (stat(fs, &st) == 0 && !S_ISDIR(st.st_mode) && stat(ff, &st) == 0 && !S_ISDIR(st.st_mode)) {
+            Points s2, f2;
+            if (vxio::read_kadr(fs, s2.x, s2.y, s2.g, &err) || vxio::read_kadr(ff, f2.x, f2.y, f2.g, &err) ||
+                s2.size() != f2.size()) {
+                logf(1, "bad point snapshots at step %" PRId64 ": %s", step, err.c_str());
+                vx_case_destroy(c);
+                return 1;
+            }
+            second = s2;
+            first = f2;
+            first.resize_vel();
+            second.resize_vel();
+            logf(3, "new n_of_second = %zu", second.size());
         }
     }
 
@@ -197,13 +274,13 @@ int main(int argc, char **argv) {
             vinf[0] += dv[0];
             vinf[1] += dv[1];
         }
-        if (vx_vort_creation(c, vinf[0], vinf[1])) return fail(c, "Creation ERROR!");
+        if (!conf.steady_flow && vx_vort_creation(c, vinf[0], vinf[1])) return fail(c, "Creation ERROR!");
         vx_timings tm;
         vx_case_timings(c, &tm);
         FILE *tf = timings ? timings : stdout;
         fprintf(tf, "%" PRId64 "\tN = %" PRId64 "\tCreation time = %f\tSpeed time = %f\tStep time = %f\n", step,
                 vx_case_n(c), tm.creation_ms / 1e3, tm.speed_ms / 1e3, tm.step_ms / 1e3);
-        if (save > 0 && step % save == 0) {
+        if (save > 0 && step % save == 0 && !conf.steady_flow) {
             logf(2, "Output %" PRId64, step);
             const int64_t n = vx_case_n(c);
             hx.resize(n); hy.resize(n); hg.resize(n);
@@ -220,6 +297,42 @@ int main(int argc, char **argv) {
             }
             vslot = 0;
         }
+        if (save > 0 && step % save == 0 && have_points) {
+            char fn[256];
+            snprintf(fn, sizeof(fn), "output/second/Kadr%06" PRId64 ".txt", step);
+            vxio::write_kadr(fn, second.x.data(), second.y.data(), second.g.data(), (int64_t)second.size(), eps_out);
+            snprintf(fn, sizeof(fn), "output/first/Kadr%06" PRId64 ".txt", step);
+            vxio::write_kadr(fn, first.x.data(), first.y.data(), first.g.data(), (int64_t)first.size(), eps_out);
+        }
+        if (have_points && second.size()) {
+            // second_speed (src/unit_1.cu:1300-1340): flow velocity at the
+            // inertial points, relaxation of their velocity, move
+            const int64_t m = (int64_t)second.size();
+            std::vector<double> ue(m), ve(m);
+            if (vx_velocity_probe(c, second.x.data(), second.y.data(), m, vinf[0], vinf[1], ue.data(), ve.data()))
+                return fail(c, "second Speed evaluation ERROR!");
+            if (step == first_step) {
+                second.u = ue;
+                second.v = ve;
+            }
+            write_vels(step, 4, second.x, second.y, ue, ve);
+            if (vx_points_relax(second.g.data(), ue.data(), ve.data(), second.u.data(), second.v.data(), m,
+                                conf.rel_t, conf.dt))
+                return fail(c, "second Speed evaluation ERROR!");
+            write_vels(step, 5, second.x, second.y, second.u, second.v);
+            if (vx_points_move(c, second.x.data(), second.y.data(), second.g.data(), second.u.data(),
+                               second.v.data(), m))
+                return fail(c, "second Speed evaluation ERROR!");
+            // first_speed (src/unit_1.cu:1271-1298): passive points move with the flow
+            if (vx_velocity_probe(c, first.x.data(), first.y.data(), m, vinf[0], vinf[1], first.u.data(),
+                                  first.v.data()))
+                return fail(c, "second Speed evaluation ERROR!");
+            write_vels(step, 5, first.x, first.y, first.u, first.v);
+            if (vx_points_move(c, first.x.data(), first.y.data(), first.g.data(), first.u.data(), first.v.data(),
+                               m))
+                return fail(c, "second Speed evaluation ERROR!");
+        }
+        if (conf.steady_flow) continue;
         if (vx_velocity_probe(c, px.data(), py.data(), NPROBE, vinf[0], vinf[1], pu.data(), pv.data()))
             return fail(c, "velocity control");
         if (save > 0 && vslot < save) {
@@ -246,8 +359,16 @@ int main(int argc, char **argv) {
         if (vx_case_get_state(c, hx.data(), hy.data(), hg.data(), n)) return fail(c, "Saving ERROR at POS copy");
         char fn[256];
         snprintf(fn, sizeof(fn), "output/kadrs/Kadr%06" PRId64 ".txt", conf.steps);
-        vxio::write_kadr(fn, hx.data(), hy.data(), hg.data(), n, eps_out);
-        if (fp) fprintf(fp, "%" PRId64 " %lf %lf %lf\n", conf.steps, forces[1], forces[2], forces[3]);
+        if (!conf.steady_flow) {
+            vxio::write_kadr(fn, hx.data(), hy.data(), hg.data(), n, eps_out);
+            if (fp) fprintf(fp, "%" PRId64 " %lf %lf %lf\n", conf.steps, forces[1], forces[2], forces[3]);
+        }
+        if (have_points) {
+            snprintf(fn, sizeof(fn), "output/second/Kadr%06" PRId64 ".txt", conf.steps);
+            vxio::write_kadr(fn, second.x.data(), second.y.data(), second.g.data(), (int64_t)second.size(), eps_out);
+            snprintf(fn, sizeof(fn), "output/first/Kadr%06" PRId64 ".txt", conf.steps);
+            vxio::write_kadr(fn, first.x.data(), first.y.data(), first.g.data(), (int64_t)first.size(), eps_out);
+        }
     }
     if (fp) fclose(fp);
     if (timings) fclose(timings);

@@ -122,7 +122,7 @@ int vx_case_destroy(vx_case *c) {
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     vx::DBuf<double> *db[] = {&c->L, &c->M, &c->x, &c->y, &c->g, &c->u, &c->v, &c->d, &c->rp, &c->gsheet,
-                              &c->partial, &c->red, &c->tx, &c->ty, &c->tg, &c->px, &c->py, &c->pu, &c->pv};
+                              &c->partial, &c->red, &c->tx, &c->ty, &c->tg, &c->px, &c->py, &c->pu, &c->pv, &c->pg};
     for (auto *p : db) p->release();
     c->xf.release(); c->yf.release(); c->gf.release();
     c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();
@@ -430,6 +430,50 @@ int vx_run_steps(vx_case *c, int64_t steps, double vinf_x, double vinf_y, double
     double tmp[4];
     for (int64_t k = 0; k < steps; ++k)
         if (vx_time_step(c, vinf_x, vinf_y, f4 ? f4 + 4 * k : tmp)) return 1;
+    return 0;
+}
+
+// Second component move (src/unit_1.cu:1295,1334): step_Kernel on the points
+// with null force pointers.
+int vx_points_move(vx_case *c, double *x, double *y, double *g, const double *u, const double *v, int64_t m) {
+    VX_ARG(c && m >= 0 && (m == 0 || (x && y && g && u && v)), "null pointer");
+    if (!m) return 0;
+    hipStream_t s = c->stream;
+    if (c->px.ensure((size_t)m, s) || c->py.ensure((size_t)m, s) || c->pg.ensure((size_t)m, s) ||
+        c->pu.ensure((size_t)m, s) || c->pv.ensure((size_t)m, s))
+        return 1;
+    const size_t b = sizeof(double) * (size_t)m;
+    VX_CHECK(hipMemcpyAsync(c->px.p, x, b, hipMemcpyHostToDevice, s));
+    VX_CHECK(hipMemcpyAsync(c->py.p, y, b, hipMemcpyHostToDevice, s));
+    VX_CHECK(hipMemcpyAsync(c->pg.p, g, b, hipMemcpyHostToDevice, s));
+    VX_CHECK(hipMemcpyAsync(c->pu.p, u, b, hipMemcpyHostToDevice, s));
+    VX_CHECK(hipMemcpyAsync(c->pv.p, v, b, hipMemcpyHostToDevice, s));
+    if (launch_points_move(c, m)) return 1;
+    VX_CHECK(hipMemcpyAsync(x, c->px.p, b, hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipMemcpyAsync(y, c->py.p, b, hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipMemcpyAsync(g, c->pg.p, b, hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+// second_speed_Kernel (src/kernel.cu:1478-1489): the inertial points' velocity
+// relaxes towards the carrier flow, v = (v_env - v_prev) / rel_t * dt + v_prev,
+// with v_prev read as FP32 (float2 in the reference); 0 where g = 0.
+int vx_points_relax(const double *g, const double *uenv, const double *venv, double *u, double *v, int64_t m,
+                    double rel_t, double dt) {
+#pragma clang fp contract(off)
+    VX_ARG(m >= 0 && (m == 0 || (g && uenv && venv && u && v)), "null pointer");
+    VX_ARG(rel_t != 0.0, "rel_t must be non-zero");
+    for (int64_t i = 0; i < m; ++i) {
+        if (!g[i]) {
+            u[i] = 0.0;
+            v[i] = 0.0;
+            continue;
+        }
+        const double pu = (double)(float)u[i], pv = (double)(float)v[i];
+        u[i] = (uenv[i] - pu) / rel_t * dt + pu;
+        v[i] = (venv[i] - pv) / rel_t * dt + pv;
+    }
     return 0;
 }
 

@@ -13,9 +13,11 @@
 #include "vortex_compat.h"
 #include "vx_internal.h"
 
-// The caller (the run driver) owns the configuration; this weak definition
-// only keeps the library loadable on its own.
+// The caller (the run driver) owns the configuration and the step counters;
+// these weak definitions only keep the library loadable on its own.
 __attribute__((weak)) struct conf_t conf;
+__attribute__((weak)) int current_step = 0;
+__attribute__((weak)) int first_step = 0;
 
 namespace vx {
 namespace {
@@ -417,6 +419,95 @@ int velocity_control(Vortex *pos, TVctr *V_inf, size_t n, Vortex *Contr_points, 
     }
     if (n_contr) VX_CHECK(hipMemcpy(V, out.data(), sizeof(PVortex) * n_contr, hipMemcpyHostToDevice));
     return 0;
+}
+
+// ---- second component (src/unit_1.cu:1271-1340) ------------------------------
+namespace {
+
+// save_vel_to_file (src/unit_1.cu:616-631)
+void compat_write_vels(const std::vector<Vortex> &pos, const std::vector<PVortex> &vel, int stage) {
+    char fn[256];
+    snprintf(fn, sizeof(fn), "output/vels/Vel%06d_%d.txt", current_step, stage);
+    FILE *f = fopen(fn, "w");
+    if (!f) return;
+    fprintf(f, "%zu\n", pos.size());
+    for (size_t i = 0; i < pos.size(); ++i)
+        fprintf(f, "%zu %lf %lf %lf %lf\n", i, pos[i].r[0], pos[i].r[1], vel[i].v[0], vel[i].v[1]);
+    fclose(f);
+}
+
+// step_Kernel on the points with null force pointers: positions and flags of
+// the caller's device AoS points moved by the device velocities.
+int compat_points_move(vx_case *c, Vortex *pts_dev, std::vector<Vortex> &pts, const std::vector<PVortex> &vel) {
+    const size_t m = pts.size();
+    std::vector<double> x(m), y(m), g(m), u(m), v(m);
+    for (size_t i = 0; i < m; ++i) {
+        x[i] = pts[i].r[0];
+        y[i] = pts[i].r[1];
+        g[i] = pts[i].g;
+        u[i] = vel[i].v[0];
+        v[i] = vel[i].v[1];
+    }
+    if (vx_points_move(c, x.data(), y.data(), g.data(), u.data(), v.data(), (int64_t)m)) return 1;
+    for (size_t i = 0; i < m; ++i) {
+        pts[i].r[0] = x[i];
+        pts[i].r[1] = y[i];
+        pts[i].g = g[i];
+    }
+    VX_CHECK(hipMemcpy(pts_dev, pts.data(), sizeof(Vortex) * m, hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // namespace
+
+int first_speed(Vortex *pos, TVctr *V_inf, size_t n, Vortex *second_pos, PVortex *v_second, size_t *n_second,
+                tPanel *panels) {
+    VX_ARG(pos && V_inf && second_pos && v_second && n_second && panels, "null pointer");
+    if (velocity_control(pos, V_inf, n, second_pos, v_second, *n_second)) return 1;
+    vx_case *c = compat_case(panels);
+    if (!c) return 1;
+    const size_t m = *n_second;
+    std::vector<Vortex> pts(m);
+    std::vector<PVortex> vel(m);
+    VX_CHECK(hipMemcpy(pts.data(), second_pos, sizeof(Vortex) * m, hipMemcpyDeviceToHost));
+    VX_CHECK(hipMemcpy(vel.data(), v_second, sizeof(PVortex) * m, hipMemcpyDeviceToHost));
+    compat_write_vels(pts, vel, 5);
+    return compat_points_move(c, second_pos, pts, vel);
+}
+
+int second_speed(Vortex *pos, TVctr *V_inf, size_t n, Vortex *second_pos, PVortex *v_second, PVortex *v_env,
+                 size_t *n_second, tPanel *panels) {
+    VX_ARG(pos && V_inf && second_pos && v_second && v_env && n_second && panels, "null pointer");
+    VX_ARG(conf.rel_t != 0.0, "rel_t is nessesary if second component set");
+    if (velocity_control(pos, V_inf, n, second_pos, v_env, *n_second)) return 1;
+    vx_case *c = compat_case(panels);
+    if (!c) return 1;
+    const size_t m = *n_second;
+    if (current_step == first_step)
+        VX_CHECK(hipMemcpy(v_second, v_env, sizeof(PVortex) * m, hipMemcpyDeviceToDevice));
+    std::vector<Vortex> pts(m);
+    std::vector<PVortex> env(m), vel(m);
+    VX_CHECK(hipMemcpy(pts.data(), second_pos, sizeof(Vortex) * m, hipMemcpyDeviceToHost));
+    VX_CHECK(hipMemcpy(env.data(), v_env, sizeof(PVortex) * m, hipMemcpyDeviceToHost));
+    VX_CHECK(hipMemcpy(vel.data(), v_second, sizeof(PVortex) * m, hipMemcpyDeviceToHost));
+    compat_write_vels(pts, env, 4);
+    std::vector<double> g(m), ue(m), ve(m), u(m), v(m);
+    for (size_t i = 0; i < m; ++i) {
+        g[i] = pts[i].g;
+        ue[i] = env[i].v[0];
+        ve[i] = env[i].v[1];
+        u[i] = vel[i].v[0];
+        v[i] = vel[i].v[1];
+    }
+    if (vx_points_relax(g.data(), ue.data(), ve.data(), u.data(), v.data(), (int64_t)m, conf.rel_t, conf.dt))
+        return 2;
+    for (size_t i = 0; i < m; ++i) {
+        vel[i].v[0] = u[i];
+        vel[i].v[1] = v[i];
+    }
+    VX_CHECK(hipMemcpy(v_second, vel.data(), sizeof(PVortex) * m, hipMemcpyHostToDevice));
+    compat_write_vels(pts, vel, 5);
+    return compat_points_move(c, second_pos, pts, vel);
 }
 
 }  // extern "C"

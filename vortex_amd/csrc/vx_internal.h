@@ -114,7 +114,7 @@ struct vx_case {
     int64_t *h_count = nullptr;                       // pinned read-back slot
     double *h_red = nullptr;                          // pinned reduction read-back
     // probes
-    vx::DBuf<double> px, py, pu, pv;
+    vx::DBuf<double> px, py, pu, pv, pg;
     // Morton tree of the last velocity stage
     vx::TreeBufs *tree = nullptr;
     // timing
@@ -172,6 +172,7 @@ int tree_stats(vx_case *c, vx_tree_info *info);
 void tree_release(vx_case *c);
 // vx_step.hip
 int launch_step_move(vx_case *c, double out4[4]);      // synchronous read-back of the sums
+int launch_points_move(vx_case *c, int64_t m);         // c->px, py, pg moved by c->pu, pv
 int launch_compact(vx_case *c, int64_t *n_inout);      // synchronous read-back of n
 int launch_setka(vx_case *c, int kind);                // -> c->col
 int launch_collapse(vx_case *c, int kind);             // uses c->col

@@ -239,6 +239,54 @@ int launch_step_move(vx_case *c, double out4[4]) {
     return 0;
 }
 
+// ---- tracer points: step_Kernel for the second component ---------------------
+// src/unit_1.cu:1271-1340 move the passive and inertial points with
+// step_Kernel and null force pointers: the same move, absorption (g = 0) and
+// deletion as for vortices, without impulse terms or sums.
+__global__ __launch_bounds__(256) void k_points_move(double *__restrict__ x, double *__restrict__ y,
+                                                     double *__restrict__ g, const double *__restrict__ u,
+                                                     const double *__restrict__ v, int64_t m, StepConsts k,
+                                                     const int32_t *__restrict__ hitp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double xi = x[i], yi = y[i], gi = g[i];
+    const double xn = xi + u[i] * k.dt;
+    const double yn = yi + v[i] * k.dt;
+    if ((gi != 0) && !hit_box_reject(k, xn, yn, xi, yi) && hitp[i] >= 0) gi = 0;
+    xi = xn;
+    yi = yn;
+    if ((gi != 0) && ((xi > k.x_max) || (xi < k.x_min) || (yi > k.y_max) || (yi < k.y_min))) {
+        xi = -2.0e+5;
+        yi = -2.0e+5;
+        gi = 0;
+    }
+    x[i] = xi;
+    y[i] = yi;
+    g[i] = gi;
+}
+
+int launch_points_move(vx_case *c, int64_t m) {
+    if (m <= 0) return 0;
+    hipStream_t st = c->stream;
+    if (c->misc.ensure((size_t)m, st) || c->lists.ensure((size_t)m, st) || c->dcount.ensure(8, st)) return 1;
+    const vx_params &p = c->prm;
+    StepConsts k{p.dt, p.rc_x, p.rc_y, p.x_min, p.x_max, p.y_min, p.y_max, p.px_min, p.px_max, p.py_min, p.py_max};
+    int32_t *hitp = c->misc.p, *cand = c->lists.p;
+    int32_t *ncand = (int32_t *)(c->dcount.p + 6);
+    const HitCandidate pred{c->px.p, c->py.p, c->pg.p, c->pu.p, c->pv.p, k};
+    hipcub::CountingInputIterator<int32_t> it(0);
+    size_t bytes = 0;
+    VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cand, ncand, m, pred, st));
+    if (c->cub_tmp.ensure(bytes, st)) return 1;
+    VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cand, ncand, m, pred, st));
+    hipLaunchKernelGGL(k_hit_panels, dim3(256), dim3(256), 0, st, c->px.p, c->py.p, c->pu.p, c->pv.p, c->pan, k,
+                       cand, ncand, hitp);
+    hipLaunchKernelGGL(k_points_move, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, c->px.p, c->py.p, c->pg.p,
+                       c->pu.p, c->pv.p, m, k, hitp);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
 // ---- A11: src/kernel.cu:1159-1178 ------------------------------------------
 // Swap-with-last removal of |g| < 1e-12 in closed form.  With n1 live
 // entries, the j-th hole of [0, n1) (ascending) takes the j-th live entry of

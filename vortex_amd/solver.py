@@ -42,6 +42,15 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def points_relax(g, uenv, venv, u, v, rel_t, dt):
+    """Inertial points' velocity relaxed towards the flow (host arithmetic);
+    returns new (u, v)."""
+    g, uenv, venv = _f64(g), _f64(uenv), _f64(venv)
+    u, v = _f64(u).copy(), _f64(v).copy()
+    check(lib().vx_points_relax(ptr(g), ptr(uenv), ptr(venv), ptr(u), ptr(v), g.size, float(rel_t), float(dt)))
+    return u, v
+
+
 class Case:
     def __init__(self, params: Params, panels: np.ndarray):
         self.panels = np.ascontiguousarray(panels)
@@ -166,6 +175,14 @@ class Case:
         check(lib().vx_velocity_probe(self._h, ptr(px), ptr(py), px.size, vinf[0], vinf[1],
                                       ptr(u), ptr(v)))
         return u, v
+
+    def points_move(self, x, y, g, u, v):
+        """Move tracer points over dt (absorbed: g = 0; out of the box: deleted).
+        Returns new (x, y, g)."""
+        x, y, g = _f64(x).copy(), _f64(y).copy(), _f64(g).copy()
+        u, v = _f64(u), _f64(v)
+        check(lib().vx_points_move(self._h, ptr(x), ptr(y), ptr(g), ptr(u), ptr(v), x.size))
+        return x, y, g
 
     def right_part(self, n_src, vinf):
         R = np.zeros(self.P)
