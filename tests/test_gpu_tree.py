@@ -6,10 +6,10 @@ Stated tolerances (relative to the largest magnitude of the compared field):
   * Biot-Savart, FP64, 32 multipole terms, theta 0.5: 1e-8
   * Biot-Savart, FP64, 16 terms: 2e-4;  mixed, 16 terms: 3e-4
   * nearest-3 distance: FP64 1e-13 (exact up to rounding), mixed 1e-4
-    (leaf-relative FP32 coordinates; a few Morton-run leaves straddle large
-    cells, which bounds this)
-  * diffusion (cut-off 30 d FP64, 20 d mixed): FP64 1e-9, mixed 1e-3 (FP32 terms, amplified
-    where I1 = sum g e^{-r/d} cancels)
+    (bucket-relative FP32 coordinates)
+  * diffusion (cut-off 30 d FP64, 18 d mixed; the isolated vortices take the
+    domain-scale path): FP64 1e-9, mixed 1e-3 (FP32 terms, amplified where
+    I1 = sum g e^{-r/d} cancels)
 """
 import math
 
