@@ -44,7 +44,8 @@ struct PanelsSoA {
     int64_t P;
 };
 
-// Growable device buffer.
+// Growable device buffer.  Growth is geometric (at least 1.25x), so sizes
+// that creep up from step to step reallocate O(log) times, not every step.
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -52,6 +53,7 @@ struct DBuf {
     int ensure(size_t want, hipStream_t s, bool keep = false) {
         if (want <= n) return 0;
         size_t cap = want < 1024 ? 1024 : want;
+        if (n && cap < n + n / 4) cap = n + n / 4;
         T *q = nullptr;
         hipError_t e = hipMalloc((void **)&q, cap * sizeof(T));
         if (e != hipSuccess) {

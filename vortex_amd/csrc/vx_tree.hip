@@ -55,7 +55,8 @@ struct TreeBufs {
     DBuf<int32_t> gcs, gcs2, gflag, gscan, gfirst;  // target groups
     DBuf<double> sx, sy, sg, ncx, ncy, nr, fbr2, hpart;
     DBuf<float> fx, fy, fg;
-    DBuf<double4> nbb, part;
+    DBuf<double4> nbb, part, cbox;
+    DBuf<int2> fpairs;
     DBuf<double2> cd;
     DBuf<float2> cf;
     DBuf<NodeRec> rec;
@@ -63,7 +64,6 @@ struct TreeBufs {
     DBuf<unsigned char> tmp;
     int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
     int64_t ngroup = 0;                    // target groups (gfirst)
-    int64_t heavy = 0;                     // diffusion targets summed by k_diff_far
     int32_t root = 0;
     int order = 16;
 };
@@ -815,16 +815,31 @@ struct DiffPair<double> {
     }
 };
 
+// Radius beyond which a diffusion target is summed by k_diff_far: 5 % of
+// the root's radius.  At most kFarCap targets are listed; further ones stay
+// in their group's wave.
+constexpr int64_t kFarCap = 4096;
+__device__ __forceinline__ double far_radius(const TreeDev &t) {
+    const double4 b = t.nbb[t.root];
+    const double rr = 0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z));
+    return fmax(0.05 * rr, 1e-300);
+}
+
+__device__ __forceinline__ int64_t far_count(const unsigned long long *cnt) {
+    const unsigned long long h = cnt[2];
+    return (int64_t)(h < (unsigned long long)kFarCap ? h : (unsigned long long)kFarCap);
+}
+
 // Targets are summed over the sources within cut * d_i of them (the omitted
 // weight is bounded by (1 + cut) e^{-cut} of the near sum).  One wave per
 // target group: a node is visited when the box around the group within the
 // largest radius of its lanes reaches it and, lane by lane, when some lane's
-// own radius reaches it.  A lane whose radius exceeds rbig is listed for
+// own radius reaches it.  A lane whose radius exceeds far_radius() is listed for
 // k_diff_far instead and does not widen the wave's radius.  cnt[0] +=
 // evaluated pairs, cnt[1] stack overflows, cnt[2] the list length.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double rbig, double nu,
-                                                        double thr, const double *__restrict__ dd,
+__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
+                                                        const double *__restrict__ dd,
                                                         double *__restrict__ u, double *__restrict__ v,
                                                         unsigned long long *__restrict__ cnt,
                                                         int32_t *__restrict__ lists) {
@@ -839,8 +854,15 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
     const bool ok = G.ok;
     const int32_t i = ok ? t.perm[ti] : 0;
     const double di = ok ? dd[i] : 0.0;
-    const bool far = ok && cut * di > rbig;
-    if (far) lists[atomicAdd(cnt + 2, 1ull)] = (int32_t)ti;
+    const double rbig = far_radius(t);
+    bool far = ok && cut * di > rbig;
+    if (far) {
+        const unsigned long long k = atomicAdd(cnt + 2, 1ull);
+        if (k < (unsigned long long)kFarCap)
+            lists[k] = (int32_t)ti;
+        else
+            far = false;  // beyond the list's capacity: summed here
+    }
     const bool mine = ok && !far;
     const double R = cut * wave_max(mine ? di : 0.0);
     const double R2 = R * R;
@@ -920,20 +942,23 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
     }
 }
 
-// Targets whose radius exceeds rbig, listed by k_tree_diffusion: each wave
-// takes one target and one chunk of 64 consecutive buckets (Morton order),
-// tests the 64 bucket boxes at once, then sums the buckets in range with one
-// source per lane, loading the next bucket while summing the current one.
-// Partial sums go to part[(chunk * 3 + q) * H + k].
+// Targets whose radius exceeds far_radius(), listed by k_tree_diffusion.  The
+// buckets are cut into chunks of 64 consecutive ones (Morton order) with a
+// box each; k_far_pairs lists the (target, chunk) pairs whose boxes meet, and
+// one wave per pair tests the chunk's 64 bucket boxes at once, then sums the
+// buckets in range with one source per lane, loading the next bucket while
+// summing the current one.  Partial sums go to part[(chunk * 3 + q) * H + k]
+// (zero for pairs not listed).
 template <typename T>
 __device__ __forceinline__ void far_load(const TreeDev &t, int b, int lane, double ax, double ay, T &dx, T &dy,
                                          T &g) {
-    const int64_t j0 = t.first[b];
-    const int cnt = t.count[b];
+    const NodeRec nd = t.rec[b];
+    const int64_t j0 = -1 - nd.a;
+    const int cnt = nd.b;
     if (lane < cnt) {
         if constexpr (sizeof(T) == 4) {
-            dx = (float)(ax - t.ncx[b]) - t.fx[j0 + lane];
-            dy = (float)(ay - t.ncy[b]) - t.fy[j0 + lane];
+            dx = (float)(ax - nd.cx) - t.fx[j0 + lane];
+            dy = (float)(ay - nd.cy) - t.fy[j0 + lane];
             g = t.fg[j0 + lane];
         } else {
             dx = ax - t.sx[j0 + lane];
@@ -947,16 +972,68 @@ __device__ __forceinline__ void far_load(const TreeDev &t, int b, int lane, doub
     }
 }
 
+// One wave per chunk: the box of its 64 buckets.
+__global__ __launch_bounds__(256) void k_chunk_box(TreeDev t, int64_t nchunk, double4 *__restrict__ cbox) {
+    const int lane = threadIdx.x & 63;
+    const int64_t ch = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ch >= nchunk) return;
+    const int64_t b = ch * 64 + lane;
+    const bool ok = b < t.nleaf;
+    const double4 q = ok ? t.nbb[b] : make_double4(1e300, -1e300, 1e300, -1e300);
+    const double4 r = make_double4(wave_min(q.x), wave_max(q.y), wave_min(q.z), wave_max(q.w));
+    if (lane == 0) cbox[ch] = r;
+}
+
+// One wave per listed target: the chunks whose box lies within its radius,
+// appended as (target, chunk) pairs, one atomic per wave and 64 chunks.
+__device__ void far_pairs_of(const TreeDev &t, const int32_t *__restrict__ list, int64_t k, int64_t nchunk,
+                             const double4 *__restrict__ cbox, double cut, const double *__restrict__ dd,
+                             int2 *__restrict__ pairs, unsigned long long *__restrict__ npairs, int lane) {
+    const int64_t ti = list[k];
+    const double di = dd[t.perm[ti]];
+    const double R2 = (cut * di) * (cut * di);
+    const double ax = t.sx[ti], ay = t.sy[ti];
+    for (int64_t c0 = 0; c0 < nchunk; c0 += 64) {
+        const int64_t ch = c0 + lane;
+        const bool need = ch < nchunk && box_d2(cbox[ch], ax, ay) < R2;
+        const uint64_t m = __ballot(need);
+        if (m == 0) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(npairs, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (need) pairs[base + __popcll(m & ((1ull << lane) - 1))] = make_int2((int)k, (int)ch);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_far_pairs(TreeDev t, const int32_t *__restrict__ list,
+                                                   const unsigned long long *__restrict__ cnt, int64_t nchunk,
+                                                   const double4 *__restrict__ cbox, double cut,
+                                                   const double *__restrict__ dd, int2 *__restrict__ pairs,
+                                                   unsigned long long *__restrict__ npairs) {
+    const int lane = threadIdx.x & 63;
+    const int64_t H = far_count(cnt);
+    for (int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); k < H; k += (int64_t)gridDim.x * 4)
+        far_pairs_of(t, list, k, nchunk, cbox, cut, dd, pairs, npairs, lane);
+}
+
+// Zero the counters a[q] whose bit q is set in mask.
+__global__ void k_zero_u64(unsigned long long *__restrict__ a, unsigned mask) {
+    if (threadIdx.x < 32 && ((mask >> threadIdx.x) & 1u)) a[threadIdx.x] = 0ull;
+}
+
+// Zero the partials of the listed targets (grid-stride; H on the device).
+__global__ void k_far_zero(double *__restrict__ a, int64_t nchunk, const unsigned long long *__restrict__ cnt) {
+    const int64_t count = 3 * nchunk * far_count(cnt);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
+        a[t] = 0.0;
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const int32_t *__restrict__ list, int64_t H,
-                                                  int64_t nchunk, double cut, double thr,
-                                                  const double *__restrict__ dd, double *__restrict__ part,
-                                                  unsigned long long *__restrict__ cnt) {
+__device__ void diff_far_pair(const TreeDev &t, const int32_t *__restrict__ list, int64_t H, int64_t k, int64_t ch,
+                              double cut, double thr, const double *__restrict__ dd, double *__restrict__ part,
+                              unsigned long long *__restrict__ cnt) {
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x & 63;
-    const int64_t k = blockIdx.x;
-    const int64_t ch = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (ch >= nchunk) return;
     const int64_t ti = list[k];
     const double di = dd[t.perm[ti]];
     const double R2 = (cut * di) * (cut * di);
@@ -973,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const int32_t *__re
         mask &= mask - 1;
         T dx, dy, g;
         far_load<T>(t, bc, lane, ax, ay, dx, dy, g);
-        npairs += t.count[bc];
+        npairs += t.rec[bc].b;
         for (;;) {
             const bool more = mask != 0;
             T nx = T(1), ny = T(1), ng = T(0);
@@ -981,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const int32_t *__re
                 const int bn = (int)(ch * 64) + __builtin_ctzll(mask);
                 mask &= mask - 1;
                 far_load<T>(t, bn, lane, ax, ay, nx, ny, ng);
-                npairs += t.count[bn];
+                npairs += t.rec[bn].b;
             }
             T s2x = 0, s2y = 0, s1 = 0;
             DiffPair<T>::add(dx, dy, g, kd, thr2, s2x, s2y, s1);
@@ -1004,14 +1081,26 @@ __global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const int32_t *__re
     part[(ch * 3 + 2) * H + k] = a1;
 }
 
+// Grid-stride over the listed pairs, one wave each; the count stays on the device.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const int32_t *__restrict__ list,
+                                                  const int2 *__restrict__ pairs,
+                                                  const unsigned long long *__restrict__ npair_list, double cut,
+                                                  double thr, const double *__restrict__ dd,
+                                                  double *__restrict__ part, unsigned long long *__restrict__ cnt) {
+    const int64_t H = far_count(cnt);
+    const int64_t total = (int64_t)*npair_list;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < total; w += nw)
+        diff_far_pair<T>(t, list, H, pairs[w].x, pairs[w].y, cut, thr, dd, part, cnt);
+}
+
 // One work-group per listed target: the chunk partials summed in a fixed
 // order (strided per thread, then a tree in LDS), then the velocity update.
-__global__ __launch_bounds__(256) void k_diff_far_final(TreeDev t, const int32_t *__restrict__ list, int64_t H,
-                                                        int64_t nchunk, const double *__restrict__ part,
-                                                        const double *__restrict__ dd, double nu,
-                                                        double *__restrict__ u, double *__restrict__ v) {
+__device__ void far_final_of(const TreeDev &t, const int32_t *__restrict__ list, int64_t H, int64_t k,
+                             int64_t nchunk, const double *__restrict__ part, const double *__restrict__ dd,
+                             double nu, double *__restrict__ u, double *__restrict__ v) {
     __shared__ double r[3][256];
-    const int64_t k = blockIdx.x;
     double a2x = 0.0, a2y = 0.0, a1 = 0.0;
     for (int64_t ch = threadIdx.x; ch < nchunk; ch += 256) {
         a2x += part[(ch * 3 + 0) * H + k];
@@ -1034,6 +1123,15 @@ __global__ __launch_bounds__(256) void k_diff_far_final(TreeDev t, const int32_t
         u[i] += nu * r[0][0] / (r[2][0] * di);
         v[i] += nu * r[1][0] / (r[2][0] * di);
     }
+}
+
+__global__ __launch_bounds__(256) void k_diff_far_final(TreeDev t, const int32_t *__restrict__ list,
+                                                        const unsigned long long *__restrict__ cnt,
+                                                        int64_t nchunk, const double *__restrict__ part,
+                                                        const double *__restrict__ dd, double nu,
+                                                        double *__restrict__ u, double *__restrict__ v) {
+    const int64_t H = far_count(cnt);
+    for (int64_t k = blockIdx.x; k < H; k += gridDim.x) far_final_of(t, list, H, k, nchunk, part, dd, nu, u, v);
 }
 
 // Slots left out of the tree (dead and far from the box) see no neighbour.
@@ -1063,6 +1161,7 @@ void tree_release(vx_case *c) {
     t.bstart.release(); t.bscan.release(); t.upper.release(); t.uscan.release(); t.visits.release();
     t.sx.release(); t.sy.release(); t.sg.release(); t.ncx.release(); t.ncy.release(); t.nr.release();
     t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release(); t.rec.release();
+    t.cbox.release(); t.fpairs.release();
     t.gcs.release(); t.gcs2.release(); t.gflag.release(); t.gscan.release(); t.gfirst.release();
     t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.fbr2.release(); t.hpart.release();
     delete c->tree;
@@ -1252,40 +1351,36 @@ int launch_tree_diffusion(vx_case *c) {
     const bool mixed = c->prm.precision == VX_PREC_MIXED;
     const double cut = c->prm.diff_cut > 0 ? c->prm.diff_cut : (mixed ? 18.0 : 30.0);
     const double thr = 0.001 * c->prm.ve_size;
-    double4 rootbox;
-    VX_CHECK(hipMemcpyAsync(&rootbox, t.nbb.p + t.root, sizeof(double4), hipMemcpyDeviceToHost, st));
-    VX_CHECK(hipStreamSynchronize(st));
-    const double rroot = 0.5 * sqrt((rootbox.y - rootbox.x) * (rootbox.y - rootbox.x) +
-                                    (rootbox.w - rootbox.z) * (rootbox.w - rootbox.z));
-    const double rbig = fmax(0.05 * rroot, 1e-300);
-    unsigned long long *cnt = t.cnt.p + 4;  // pairs, overflow, far targets
+    unsigned long long *cnt = t.cnt.p + 4;  // pairs, overflow, far targets, far (target, chunk) pairs
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 0xdu);  // not the shared overflow flag
     {
         Timer tk(c, &c->tm.diff_kernel_ms, 8);
         if (mixed)
-            hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb8), dim3(256), 0, st, tv, cut, rbig, c->prm.nu, thr,
-                               c->d.p, c->u.p, c->v.p, cnt, t.fb.p);
+            hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb8), dim3(256), 0, st, tv, cut, c->prm.nu, thr, c->d.p,
+                               c->u.p, c->v.p, cnt, t.fb.p);
         else
-            hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb8), dim3(256), 0, st, tv, cut, rbig, c->prm.nu, thr,
+            hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb8), dim3(256), 0, st, tv, cut, c->prm.nu, thr,
                                c->d.p, c->u.p, c->v.p, cnt, t.fb.p);
     }
     VX_CHECK(hipGetLastError());
-    unsigned long long hH = 0;
-    VX_CHECK(hipMemcpyAsync(&hH, cnt + 2, sizeof(hH), hipMemcpyDeviceToHost, st));
-    VX_CHECK(hipStreamSynchronize(st));
-    const int64_t H = (int64_t)hH;
-    t.heavy = H;
-    if (H == 0) return 0;
+    // the listed far targets: all counts stay on the device (grid-stride
+    // kernels sized for the list's capacity)
     const int64_t nchunk = (t.nleaf + 63) / 64;
-    if (t.hpart.ensure((size_t)(3 * nchunk * H), st)) return 1;
-    const dim3 g((unsigned)H, (unsigned)((nchunk + 3) / 4));
+    if (t.hpart.ensure((size_t)(3 * nchunk * kFarCap), st) || t.cbox.ensure((size_t)nchunk, st) ||
+        t.fpairs.ensure((size_t)(nchunk * kFarCap), st))
+        return 1;
+    hipLaunchKernelGGL(k_far_zero, dim3(512), dim3(256), 0, st, t.hpart.p, nchunk, cnt);
+    hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)((nchunk + 3) / 4)), dim3(256), 0, st, tv, nchunk, t.cbox.p);
+    hipLaunchKernelGGL(k_far_pairs, dim3(256), dim3(256), 0, st, tv, t.fb.p, cnt, nchunk, t.cbox.p, cut, c->d.p,
+                       t.fpairs.p, cnt + 3);
     if (mixed)
-        hipLaunchKernelGGL(k_diff_far<float>, g, dim3(256), 0, st, tv, t.fb.p, H, nchunk, cut, thr, c->d.p,
-                           t.hpart.p, cnt);
+        hipLaunchKernelGGL(k_diff_far<float>, dim3(4096), dim3(256), 0, st, tv, t.fb.p, t.fpairs.p, cnt + 3, cut, thr,
+                           c->d.p, t.hpart.p, cnt);
     else
-        hipLaunchKernelGGL(k_diff_far<double>, g, dim3(256), 0, st, tv, t.fb.p, H, nchunk, cut, thr, c->d.p,
-                           t.hpart.p, cnt);
-    hipLaunchKernelGGL(k_diff_far_final, dim3((unsigned)H), dim3(256), 0, st, tv, t.fb.p, H, nchunk, t.hpart.p,
-                       c->d.p, c->prm.nu, c->u.p, c->v.p);
+        hipLaunchKernelGGL(k_diff_far<double>, dim3(4096), dim3(256), 0, st, tv, t.fb.p, t.fpairs.p, cnt + 3, cut,
+                           thr, c->d.p, t.hpart.p, cnt);
+    hipLaunchKernelGGL(k_diff_far_final, dim3(1024), dim3(256), 0, st, tv, t.fb.p, cnt, nchunk, t.hpart.p, c->d.p,
+                       c->prm.nu, c->u.p, c->v.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
@@ -1308,7 +1403,7 @@ int tree_stats(vx_case *c, vx_tree_info *info) {
         return 1;
     }
     info->order = c->tree->order;
-    info->heavy = (int32_t)c->tree->heavy;
+    info->heavy = (int32_t)(h[6] < (unsigned long long)kFarCap ? h[6] : (unsigned long long)kFarCap);
     return 0;
 }
 
