@@ -127,7 +127,7 @@ int vx_case_destroy(vx_case *c) {
     c->xf.release(); c->yf.release(); c->gf.release();
     c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();
     c->keys.release(); c->keys_alt.release(); c->vals.release(); c->vals_alt.release();
-    c->cub_tmp.release(); c->dcount.release();
+    c->cub_tmp.release(); c->gridp.release(); c->dcount.release();
     tree_release(c);
     if (c->pan_block) (void)hipFree(c->pan_block);
     if (c->h_count) (void)hipHostFree(c->h_count);
@@ -182,7 +182,6 @@ int vx_case_reserve(vx_case *c, int64_t capacity) {
 
 int vx_case_set_state(vx_case *c, const double *x, const double *y, const double *g, int64_t n) {
     VX_ARG(c && n >= 0 && (n == 0 || (x && y && g)), "state arrays");
-    c->live_box_ok = false;
     if (reserve(c, ceil32(n + c->P))) return 1;
     if (n) {
         VX_CHECK(hipMemcpyAsync(c->x.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
@@ -227,7 +226,6 @@ double vx_case_d_g(const vx_case *c) { return c ? c->d_g : 0.0; }
 
 int vx_case_set_n(vx_case *c, int64_t n) {
     VX_ARG(c && n >= 0 && n <= c->cap, "n beyond capacity");
-    c->live_box_ok = false;
     c->n = n;
     return 0;
 }
@@ -267,7 +265,6 @@ int vx_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y, doubl
 
 int vx_birth(vx_case *c, const double *R_p, double d_g, double *gsheet) {
     VX_ARG(c && c->have_matrix, "no panel matrix");
-    c->live_box_ok = false;
     if (reserve(c, ceil32(c->n + c->P))) return 1;
     if (R_p) {
         if (c->rp.ensure((size_t)c->P, c->stream)) return 1;
@@ -321,22 +318,34 @@ int vx_diffusion_wall(vx_case *c, int64_t n_tgt) {
     return 0;
 }
 
+// Read back the step sums (c->red[0..3]), the device count and the collapse
+// error flag; one synchronisation.
+static int step_readback(vx_case *c, double out4[4], bool *err) {
+    if (out4) VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipMemcpyAsync(c->h_count, c->dcount.p, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    if (out4)
+        for (int q = 0; q < 4; ++q) out4[q] = c->h_red[q];
+    c->n = c->h_count[1];
+    if (err) *err = (int32_t)c->h_count[3] != 0;
+    return 0;
+}
+
 int vx_step_move(vx_case *c, double out4[4]) {
     VX_ARG(c && out4, "null pointer");
-    return launch_step_move(c, out4);
+    if (launch_set_dn(c, c->n) || launch_step_move(c)) return 1;
+    return step_readback(c, out4, nullptr);
 }
 
 int vx_compact(vx_case *c) {
     VX_ARG(c, "case");
-    int64_t n = c->n;
-    if (launch_compact(c, &n)) return 1;
-    c->n = n;
-    return 0;
+    if (launch_set_dn(c, c->n) || launch_compact(c, c->n)) return 1;
+    return step_readback(c, nullptr, nullptr);
 }
 
 int vx_setka(vx_case *c, int kind, int32_t *COL) {
     VX_ARG(c && (kind == 1 || kind == 2), "kind must be 1 or 2");
-    if (launch_setka(c, kind)) return 1;
+    if (launch_set_dn(c, c->n) || launch_live_box(c) || launch_setka(c, kind, c->n)) return 1;
     if (COL && c->n) VX_CHECK(hipMemcpyAsync(COL, c->col.p, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
     VX_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -350,10 +359,9 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL) {
         if (c->col.ensure((size_t)(c->n > 0 ? c->n : 1), c->stream)) return 1;
         if (c->n) VX_CHECK(hipMemcpyAsync(c->col.p, COL, sizeof(int32_t) * c->n, hipMemcpyHostToDevice, c->stream));
     }
-    if (launch_collapse(c, kind)) return 1;
-    int32_t err = 0;
-    VX_CHECK(hipMemcpyAsync(&err, (int32_t *)(c->dcount.p + 3), sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    VX_CHECK(hipStreamSynchronize(c->stream));
+    if (launch_set_dn(c, c->n) || launch_collapse(c, kind, c->n)) return 1;
+    bool err = false;
+    if (step_readback(c, nullptr, &err)) return 1;
     if (err) {
         set_error("collapse resolution did not converge");
         return 1;
@@ -365,7 +373,6 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL) {
 // source count, then the newborns at [n, n+P).
 int vx_vort_creation(vx_case *c, double vinf_x, double vinf_y) {
     VX_ARG(c && c->have_matrix, "no panel matrix");
-    c->live_box_ok = false;
     Timer t(c, &c->tm.creation_ms, 0);
     if (reserve(c, ceil32(c->n + c->P))) return 1;
     if (launch_right_part(c, ceil32(c->n), vinf_x, vinf_y) || launch_birth(c, c->d_g)) return 1;
@@ -397,25 +404,25 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
     return 0;
 }
 
-// Step (src/unit_1.cu:1077-1218).
+// Step (src/unit_1.cu:1077-1218).  The count stays on the device until the
+// end; every collapse pass runs: a pass after one that merged nothing finds
+// the same (empty) pairs again, so the result equals the reference's early
+// exit when n stops changing.
 int vx_step(vx_case *c, double out4[4]) {
     VX_ARG(c && out4, "null pointer");
     Timer t(c, &c->tm.step_ms, 4);
-    if (launch_step_move(c, out4)) return 1;
-    int64_t n = c->n;
-    if (launch_compact(c, &n)) return 1;
-    c->n = n;
-    for (int kind = 1; kind <= 2; ++kind) {
-        for (int64_t cc = 0; cc < c->prm.n_col; ++cc) {
-            if (launch_setka(c, kind) || launch_collapse(c, kind)) return 1;
-            int64_t nt = c->n;
-            if (launch_compact(c, &nt)) return 1;
-            if (nt == c->n) break;
-            c->n = nt;
-        }
+    const int64_t n_ub = c->n;
+    if (launch_set_dn(c, n_ub) || launch_step_move(c) || launch_compact(c, n_ub)) return 1;
+    for (int kind = 1; kind <= 2; ++kind)
+        for (int64_t cc = 0; cc < c->prm.n_col; ++cc)
+            if (launch_setka(c, kind, n_ub) || launch_collapse(c, kind, n_ub) || launch_compact(c, n_ub)) return 1;
+    bool err = false;
+    if (step_readback(c, out4, &err)) return 1;
+    if (err) {
+        set_error("collapse resolution did not converge");
+        return 1;
     }
     c->d_g = out4[0];
-    c->live_box_ok = false;
     return 0;
 }
 

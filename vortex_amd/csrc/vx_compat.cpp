@@ -148,7 +148,6 @@ vx_case *compat_case(const tPanel *panels_dev) {
 }
 
 int stage_in(vx_case *c, const Vortex *pos, int64_t cnt) {
-    c->live_box_ok = false;
     if (cnt <= 0) return 0;
     hipLaunchKernelGGL(k_aos_in, dim3(blocks(cnt)), dim3(256), 0, c->stream, pos, cnt, c->x.p, c->y.p, c->g.p);
     VX_CHECK(hipGetLastError());

@@ -94,8 +94,6 @@ struct vx_case {
     void *pan_block = nullptr;
     int64_t P = 0;
     double4 wall_box{};             // birth-point box grown by 5 max(length)
-    double4 live_box{};             // box of g != 0 after the last move (collapse grid)
-    bool live_box_ok = false;       // live_box holds for the current positions
     // panel system
     vx::DBuf<double> L, M;          // (P+1)^2
     bool have_matrix = false;
@@ -112,6 +110,7 @@ struct vx_case {
     vx::DBuf<uint64_t> keys, keys_alt;
     vx::DBuf<int32_t> vals, vals_alt;
     vx::DBuf<unsigned char> cub_tmp;
+    vx::DBuf<unsigned char> gridp;                     // collapse grid parameters (device)
     vx::DBuf<int64_t> dcount;                         // device counters
     int64_t *h_count = nullptr;                       // pinned read-back slot
     double *h_red = nullptr;                          // pinned reduction read-back
@@ -173,10 +172,15 @@ int launch_tree_diffusion(vx_case *c);  // reuses the tree of the last launch_tr
 int tree_stats(vx_case *c, vx_tree_info *info);
 void tree_release(vx_case *c);
 // vx_step.hip
-int launch_step_move(vx_case *c, double out4[4]);      // synchronous read-back of the sums
+// The step stages run on a device-resident vortex count *dn (dcount[1]):
+// n_ub is an upper bound of it (n never grows inside Step), used for grids.
+int64_t *step_dn(vx_case *c);                          // the device count
+int launch_set_dn(vx_case *c, int64_t n);              // *dn = n (stream-ordered)
+int launch_step_move(vx_case *c);                      // sums and live box -> c->red[0..7]
 int launch_points_move(vx_case *c, int64_t m);         // c->px, py, pg moved by c->pu, pv
-int launch_compact(vx_case *c, int64_t *n_inout);      // synchronous read-back of n
-int launch_setka(vx_case *c, int kind);                // -> c->col
-int launch_collapse(vx_case *c, int kind);             // uses c->col
+int launch_compact(vx_case *c, int64_t n_ub);          // [0, *dn) compacted, new count -> *dn
+int launch_live_box(vx_case *c);                       // box of g != 0 in [0, *dn) -> c->red[4..7]
+int launch_setka(vx_case *c, int kind, int64_t n_ub);  // -> c->col, box from c->red[4..7]
+int launch_collapse(vx_case *c, int kind, int64_t n_ub);  // uses c->col; error flag in dcount[3]
 
 }  // namespace vx

@@ -205,7 +205,23 @@ __global__ __launch_bounds__(256) void k_sum4(const double *__restrict__ part, i
     }
 }
 
-int launch_step_move(vx_case *c, double out4[4]) {
+int64_t *step_dn(vx_case *c) { return c->dcount.p + 1; }
+
+// dcount[1] = n (the device count), dcount[3] = 0 (the collapse error flag,
+// sticky over the passes of one step).
+__global__ void k_step_init(int64_t *__restrict__ dcount, int64_t n) {
+    dcount[1] = n;
+    dcount[3] = 0;
+}
+
+int launch_set_dn(vx_case *c, int64_t n) {
+    if (c->dcount.ensure(8, c->stream)) return 1;
+    hipLaunchKernelGGL(k_step_init, dim3(1), dim3(1), 0, c->stream, c->dcount.p, n);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_step_move(vx_case *c) {
     const int64_t n = c->n;
     const int64_t nb = n > 0 ? (n + 255) / 256 : 1;
     hipStream_t st = c->stream;
@@ -231,11 +247,6 @@ int launch_step_move(vx_case *c, double out4[4]) {
                        c->P, c->pan, k, hitp, c->partial.p);
     hipLaunchKernelGGL(k_sum4, dim3(1), dim3(256), 0, st, c->partial.p, nb, p.rho, p.dt, c->red.p);
     VX_CHECK(hipGetLastError());
-    VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
-    VX_CHECK(hipStreamSynchronize(st));
-    for (int q = 0; q < 4; ++q) out4[q] = c->h_red[q];
-    c->live_box = make_double4(c->h_red[4], c->h_red[5], c->h_red[6], c->h_red[7]);
-    c->live_box_ok = true;
     return 0;
 }
 
@@ -294,14 +305,18 @@ int launch_points_move(vx_case *c, int64_t m) {
 // the position of src(e): e+1 when e+1 < n is dead, else the start of e's
 // chain -- hole h_j where j-1 live entries of [n1, n) lie above e, or n1
 // itself below the last live entry.  Gather to staging, then scatter.
-__global__ void k_live_flags(const double *__restrict__ g, int64_t n, int32_t *__restrict__ f) {
+// Grids cover the upper bound n_ub; the count n = *dn is read on the device.
+__global__ void k_live_flags(const double *__restrict__ g, int64_t n_ub, const int64_t *__restrict__ dn,
+                             int32_t *__restrict__ f) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) f[i] = fabs(g[i]) < kDelt ? 0 : 1;
+    if (i < n_ub) f[i] = (i < *dn && !(fabs(g[i]) < kDelt)) ? 1 : 0;
 }
 
 // s = exclusive scan of flags; n1 = total live (s[n-1] + f[n-1]).
-__global__ void k_compact_lists(const int32_t *__restrict__ f, const int32_t *__restrict__ s, int64_t n,
-                                int32_t *__restrict__ holes, int32_t *__restrict__ donors) {
+__global__ void k_compact_lists(const int32_t *__restrict__ f, const int32_t *__restrict__ s,
+                                const int64_t *__restrict__ dn, int32_t *__restrict__ holes,
+                                int32_t *__restrict__ donors) {
+    const int64_t n = *dn;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t n1 = (int64_t)s[n - 1] + f[n - 1];
@@ -314,9 +329,11 @@ __global__ void k_compact_lists(const int32_t *__restrict__ f, const int32_t *__
 
 __global__ void k_compact_gather(const double *__restrict__ x, const double *__restrict__ y,
                                  const double *__restrict__ g, const int32_t *__restrict__ f,
-                                 const int32_t *__restrict__ s, int64_t n, const int32_t *__restrict__ holes,
-                                 const int32_t *__restrict__ donors, double *__restrict__ tx,
-                                 double *__restrict__ ty, double *__restrict__ tg) {
+                                 const int32_t *__restrict__ s, const int64_t *__restrict__ dn,
+                                 const int32_t *__restrict__ holes, const int32_t *__restrict__ donors,
+                                 double *__restrict__ tx, double *__restrict__ ty, double *__restrict__ tg) {
+    const int64_t n = *dn;
+    if (n <= 0) return;
     const int64_t n1 = (int64_t)s[n - 1] + f[n - 1];
     const int64_t K = n1 > 0 ? (int64_t)(n1 - (n1 < n ? s[n1] : n1)) : 0;  // holes in [0, n1)
     const int64_t T = n - n1;
@@ -338,9 +355,12 @@ __global__ void k_compact_gather(const double *__restrict__ x, const double *__r
 }
 
 __global__ void k_compact_scatter(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
-                                  const int32_t *__restrict__ f, const int32_t *__restrict__ s, int64_t n,
-                                  const int32_t *__restrict__ holes, const double *__restrict__ tx,
-                                  const double *__restrict__ ty, const double *__restrict__ tg) {
+                                  const int32_t *__restrict__ f, const int32_t *__restrict__ s,
+                                  const int64_t *__restrict__ dn, const int32_t *__restrict__ holes,
+                                  const double *__restrict__ tx, const double *__restrict__ ty,
+                                  const double *__restrict__ tg) {
+    const int64_t n = *dn;
+    if (n <= 0) return;
     const int64_t n1 = (int64_t)s[n - 1] + f[n - 1];
     const int64_t K = n1 > 0 ? (int64_t)(n1 - (n1 < n ? s[n1] : n1)) : 0;
     const int64_t T = n - n1;
@@ -357,37 +377,35 @@ __global__ void k_compact_scatter(double *__restrict__ x, double *__restrict__ y
     g[dst] = tg[t];
 }
 
-__global__ void k_compact_count(const int32_t *__restrict__ f, const int32_t *__restrict__ s, int64_t n,
-                                int64_t *__restrict__ out) {
-    out[0] = (int64_t)s[n - 1] + f[n - 1];
+__global__ void k_compact_count(const int32_t *__restrict__ f, const int32_t *__restrict__ s,
+                                int64_t *__restrict__ dn) {
+    const int64_t n = *dn;
+    if (n > 0) *dn = (int64_t)s[n - 1] + f[n - 1];
 }
 
-int launch_compact(vx_case *c, int64_t *n_inout) {
-    const int64_t n = *n_inout;
-    if (n <= 0) return 0;
+int launch_compact(vx_case *c, int64_t n_ub) {
+    if (n_ub <= 0) return 0;
     hipStream_t st = c->stream;
-    if (c->flags.ensure((size_t)n, st) || c->scan.ensure((size_t)n, st) || c->lists.ensure((size_t)(2 * n), st) ||
-        c->tx.ensure((size_t)(2 * n), st) || c->ty.ensure((size_t)(2 * n), st) || c->tg.ensure((size_t)(2 * n), st) ||
-        c->dcount.ensure(4, st))
+    if (c->flags.ensure((size_t)n_ub, st) || c->scan.ensure((size_t)n_ub, st) ||
+        c->lists.ensure((size_t)(2 * n_ub), st) || c->tx.ensure((size_t)(2 * n_ub), st) ||
+        c->ty.ensure((size_t)(2 * n_ub), st) || c->tg.ensure((size_t)(2 * n_ub), st) || c->dcount.ensure(8, st))
         return 1;
-    const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_live_flags, dim3(nb), dim3(256), 0, st, c->g.p, n, c->flags.p);
+    int64_t *dn = step_dn(c);
+    const unsigned nb = (unsigned)((n_ub + 255) / 256);
+    hipLaunchKernelGGL(k_live_flags, dim3(nb), dim3(256), 0, st, c->g.p, n_ub, dn, c->flags.p);
     size_t bytes = 0;
-    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->flags.p, c->scan.p, (int)n, st));
+    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->flags.p, c->scan.p, (int)n_ub, st));
     if (c->cub_tmp.ensure(bytes, st)) return 1;
-    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, c->flags.p, c->scan.p, (int)n, st));
-    int32_t *holes = c->lists.p, *donors = c->lists.p + n;
-    hipLaunchKernelGGL(k_compact_lists, dim3(nb), dim3(256), 0, st, c->flags.p, c->scan.p, n, holes, donors);
-    const unsigned nb2 = (unsigned)((2 * n + 255) / 256);
-    hipLaunchKernelGGL(k_compact_gather, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->flags.p,
-                       c->scan.p, n, holes, donors, c->tx.p, c->ty.p, c->tg.p);
+    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, c->flags.p, c->scan.p, (int)n_ub, st));
+    int32_t *holes = c->lists.p, *donors = c->lists.p + n_ub;
+    hipLaunchKernelGGL(k_compact_lists, dim3(nb), dim3(256), 0, st, c->flags.p, c->scan.p, dn, holes, donors);
+    const unsigned nb2 = (unsigned)((2 * n_ub + 255) / 256);
+    hipLaunchKernelGGL(k_compact_gather, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->flags.p, c->scan.p,
+                       dn, holes, donors, c->tx.p, c->ty.p, c->tg.p);
     hipLaunchKernelGGL(k_compact_scatter, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->flags.p,
-                       c->scan.p, n, holes, c->tx.p, c->ty.p, c->tg.p);
-    hipLaunchKernelGGL(k_compact_count, dim3(1), dim3(1), 0, st, c->flags.p, c->scan.p, n, c->dcount.p);
+                       c->scan.p, dn, holes, c->tx.p, c->ty.p, c->tg.p);
+    hipLaunchKernelGGL(k_compact_count, dim3(1), dim3(1), 0, st, c->flags.p, c->scan.p, dn);
     VX_CHECK(hipGetLastError());
-    VX_CHECK(hipMemcpyAsync(c->h_count, c->dcount.p, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    VX_CHECK(hipStreamSynchronize(st));
-    *n_inout = *c->h_count;
     return 0;
 }
 
@@ -399,10 +417,11 @@ int launch_compact(vx_case *c, int64_t *n_inout) {
 // dense grid over their box, cell size s > 2 sqrt(r2): a partner of i lies in
 // i's cell or in the neighbour on the side of the cell's half that holds i,
 // per axis, so four cells are searched.  The minimum j does not depend on the
-// order of the cell lists.
+// order of the cell lists.  The grid is set up on the device from the box.
 struct Grid {
     double x0, y0, inv_s;
-    int64_t nx, ny;
+    int64_t nx, ny, cells;
+    int valid;
 };
 
 __device__ __forceinline__ void cell_of(const Grid &G, double x, double y, int64_t &cx, int64_t &cy, double &fx,
@@ -417,16 +436,38 @@ __device__ __forceinline__ void cell_of(const Grid &G, double x, double y, int64
     fy -= (double)cy;
 }
 
-__global__ void k_fill_i32(int32_t *__restrict__ a, int64_t count, int32_t value) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < count) a[t] = value;
+// box = (min x, max x, min y, max y) of the g != 0 vortices; at most
+// max_cells cells (a wider cell stays correct).
+__global__ void k_grid_setup(const double *__restrict__ box, double r2, int64_t max_cells, Grid *__restrict__ G) {
+    Grid q{};
+    const double x0 = box[0], x1 = box[1], y0 = box[2], y1 = box[3];
+    q.valid = (r2 > 0.0) && (x0 <= x1) && (y0 <= y1);
+    if (q.valid) {
+        double s = 2.0 * sqrt(r2) * 1.001;
+        const double w = x1 - x0, h = y1 - y0;
+        while ((floor(w / s) + 1) * (floor(h / s) + 1) > (double)max_cells) s *= 1.25;
+        q.x0 = x0;
+        q.y0 = y0;
+        q.inv_s = 1.0 / s;
+        q.nx = (int64_t)floor(w / s) + 1;
+        q.ny = (int64_t)floor(h / s) + 1;
+        q.cells = q.nx * q.ny;
+    }
+    *G = q;
+}
+
+__global__ void k_grid_clear(int32_t *__restrict__ head, const Grid *__restrict__ G) {
+    const int64_t cells = G->valid ? G->cells : 0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cells; t += (int64_t)gridDim.x * blockDim.x)
+        head[t] = -1;
 }
 
 __global__ void k_grid_insert(const double *__restrict__ x, const double *__restrict__ y,
-                              const double *__restrict__ g, int64_t n, Grid G, int32_t *__restrict__ head,
-                              int32_t *__restrict__ next) {
+                              const double *__restrict__ g, const int64_t *__restrict__ dn,
+                              const Grid *__restrict__ Gp, int32_t *__restrict__ head, int32_t *__restrict__ next) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || g[i] == 0) return;
+    const Grid G = *Gp;
+    if (!G.valid || i >= *dn || g[i] == 0) return;
     int64_t cx, cy;
     double fx, fy;
     cell_of(G, x[i], y[i], cx, cy, fx, fy);
@@ -435,13 +476,15 @@ __global__ void k_grid_insert(const double *__restrict__ x, const double *__rest
 
 template <int KIND>
 __global__ __launch_bounds__(256) void k_setka(const double *__restrict__ x, const double *__restrict__ y,
-                                               const double *__restrict__ g, int64_t n, Grid G, double r2,
-                                               double max_ve_g, const int32_t *__restrict__ head,
-                                               const int32_t *__restrict__ next, int32_t *__restrict__ col) {
+                                               const double *__restrict__ g, const int64_t *__restrict__ dn,
+                                               const Grid *__restrict__ Gp, double r2, double max_ve_g,
+                                               const int32_t *__restrict__ head, const int32_t *__restrict__ next,
+                                               int32_t *__restrict__ col) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= *dn) return;
+    const Grid G = *Gp;
     const double gi = g[i];
-    if (gi == 0) {
+    if (!G.valid || gi == 0) {
         col[i] = -1;
         return;
     }
@@ -474,14 +517,15 @@ __global__ __launch_bounds__(256) void k_setka(const double *__restrict__ x, con
     col[i] = best == INT64_MAX ? -1 : (int32_t)best;
 }
 
-// Box of the vortices with g != 0 (min x, -max x, min y, -max y partials).
+// Box of the vortices with g != 0 in [0, *dn) (min x, -max x, min y, -max y
+// partials, then the box into out[0..3]).
 __global__ __launch_bounds__(256) void k_live_box(const double *__restrict__ x, const double *__restrict__ y,
-                                                  const double *__restrict__ g, int64_t n,
+                                                  const double *__restrict__ g, const int64_t *__restrict__ dn,
                                                   double *__restrict__ part) {
     __shared__ double red[4][256];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double b[4] = {1e300, 1e300, 1e300, 1e300};
-    if (i < n && g[i] != 0) {
+    if (i < *dn && g[i] != 0) {
         b[0] = x[i];
         b[1] = -x[i];
         b[2] = y[i];
@@ -518,61 +562,41 @@ __global__ __launch_bounds__(256) void k_live_box_final(const double *__restrict
     }
 }
 
-static int live_box(vx_case *c, double4 *box) {
-    if (!c->live_box_ok) {
-        const int64_t n = c->n;
-        const int64_t nb = (n + 255) / 256;
-        if (c->partial.ensure((size_t)(4 * nb), c->stream) || c->red.ensure(8, c->stream)) return 1;
-        hipLaunchKernelGGL(k_live_box, dim3((unsigned)nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->g.p, n,
-                           c->partial.p);
-        hipLaunchKernelGGL(k_live_box_final, dim3(1), dim3(256), 0, c->stream, c->partial.p, nb, c->red.p);
-        VX_CHECK(hipGetLastError());
-        VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        VX_CHECK(hipStreamSynchronize(c->stream));
-        c->live_box = make_double4(c->h_red[0], c->h_red[1], c->h_red[2], c->h_red[3]);
-        c->live_box_ok = true;
-    }
-    *box = c->live_box;
+int launch_live_box(vx_case *c) {
+    const int64_t n = c->n > 0 ? c->n : 1;
+    const int64_t nb = (n + 255) / 256;
+    if (c->partial.ensure((size_t)(4 * nb), c->stream) || c->red.ensure(8, c->stream)) return 1;
+    hipLaunchKernelGGL(k_live_box, dim3((unsigned)nb), dim3(256), 0, c->stream, c->x.p, c->y.p, c->g.p, step_dn(c),
+                       c->partial.p);
+    hipLaunchKernelGGL(k_live_box_final, dim3(1), dim3(256), 0, c->stream, c->partial.p, nb, c->red.p + 4);
+    VX_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_setka(vx_case *c, int kind) {
-    const int64_t n = c->n;
+int launch_setka(vx_case *c, int kind, int64_t n_ub) {
     hipStream_t st = c->stream;
-    if (c->col.ensure((size_t)(n > 0 ? n : 1), st)) return 1;
-    if (n <= 0) return 0;
-    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (c->col.ensure((size_t)(n_ub > 0 ? n_ub : 1), st)) return 1;
+    if (n_ub <= 0) return 0;
+    const unsigned nb = (unsigned)((n_ub + 255) / 256);
     const double ve2 = c->prm.ve_size * c->prm.ve_size;
     const double rr = kind == 1 ? c->prm.r_col_diff_sign : c->prm.r_col_same_sign;
     const double r2 = rr * rr * ve2;  // src/unit_1.cu:1228-1229
-    double4 bx;
-    if (live_box(c, &bx)) return 1;
-    if (!(r2 > 0.0) || !(bx.x <= bx.y && bx.z <= bx.w)) {
-        hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, c->col.p, n, -1);
-        VX_CHECK(hipGetLastError());
-        return 0;
-    }
-    // at most 2^25 cells (128 MB of heads): a wider cell stays correct
-    constexpr double kMaxCells = 33554432.0;
-    double s = 2.0 * sqrt(r2) * 1.001;
-    const double w = bx.y - bx.x, h = bx.w - bx.z;
-    while ((floor(w / s) + 1) * (floor(h / s) + 1) > kMaxCells) s *= 1.25;
-    Grid G;
-    G.x0 = bx.x;
-    G.y0 = bx.z;
-    G.inv_s = 1.0 / s;
-    G.nx = (int64_t)floor(w / s) + 1;
-    G.ny = (int64_t)floor(h / s) + 1;
-    const int64_t cells = G.nx * G.ny;
-    if (c->vals.ensure((size_t)cells, st) || c->vals_alt.ensure((size_t)n, st)) return 1;
-    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, c->vals.p, cells, -1);
-    hipLaunchKernelGGL(k_grid_insert, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, c->vals.p,
+    // at most max(2^20, 8 n) cells of heads
+    const int64_t max_cells = std::max<int64_t>(1 << 20, 8 * n_ub);
+    if (c->vals.ensure((size_t)max_cells, st) || c->vals_alt.ensure((size_t)n_ub, st) ||
+        c->gridp.ensure(sizeof(Grid), st))
+        return 1;
+    Grid *G = (Grid *)c->gridp.p;
+    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1), 0, st, c->red.p + 4, r2, max_cells, G);
+    hipLaunchKernelGGL(k_grid_clear, dim3(1024), dim3(256), 0, st, c->vals.p, G);
+    int64_t *dn = step_dn(c);
+    hipLaunchKernelGGL(k_grid_insert, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn, G, c->vals.p,
                        c->vals_alt.p);
     if (kind == 1)
-        hipLaunchKernelGGL(k_setka<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, r2,
+        hipLaunchKernelGGL(k_setka<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn, G, r2,
                            c->prm.max_ve_g, c->vals.p, c->vals_alt.p, c->col.p);
     else
-        hipLaunchKernelGGL(k_setka<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n, G, r2,
+        hipLaunchKernelGGL(k_setka<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn, G, r2,
                            c->prm.max_ve_g, c->vals.p, c->vals_alt.p, c->col.p);
     VX_CHECK(hipGetLastError());
     return 0;
@@ -585,20 +609,21 @@ int launch_setka(vx_case *c, int kind) {
 // forest of edges i -> COL_i (COL_i > i) in one work-group.  Among active
 // candidates sharing a target j the smallest merges with j as it was, the
 // others with the killed j (g = 0 at 1e10), exactly as the sweep does.
-__global__ void k_col_prepare(int64_t n, int32_t *__restrict__ pc, int32_t *__restrict__ state,
-                              int32_t *__restrict__ first) {
+__global__ void k_col_prepare(int64_t n_ub, int32_t *__restrict__ pc, int32_t *__restrict__ state,
+                              int32_t *__restrict__ first, unsigned long long *__restrict__ ncand_err) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i == 0) ncand_err[0] = 0ull;  // candidate count
+    if (i >= n_ub) return;
     state[i] = 0;
     first[i] = INT32_MAX;
     pc[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_col_count(const int32_t *__restrict__ col, int64_t n,
+__global__ __launch_bounds__(256) void k_col_count(const int32_t *__restrict__ col, const int64_t *__restrict__ dn,
                                                    int32_t *__restrict__ pc, int32_t *__restrict__ cand,
                                                    unsigned long long *__restrict__ ncand) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t j = i < n ? col[i] : -1;
+    const int32_t j = i < *dn ? col[i] : -1;
     if (j >= 0) atomicAdd(&pc[j], 1);
     const uint64_t m = __ballot(j >= 0);
     if (m == 0) return;
@@ -713,20 +738,19 @@ __global__ void k_col_write(double *__restrict__ x, double *__restrict__ y, doub
     g[j] = 0.0;
 }
 
-int launch_collapse(vx_case *c, int kind) {
-    const int64_t n = c->n;
-    if (n <= 0) return 0;
+int launch_collapse(vx_case *c, int kind, int64_t n_ub) {
+    if (n_ub <= 0) return 0;
     hipStream_t st = c->stream;
-    if (c->flags.ensure((size_t)n, st) || c->scan.ensure((size_t)n, st) || c->lists.ensure((size_t)(2 * n), st) ||
-        c->misc.ensure((size_t)n, st) || c->tx.ensure((size_t)n, st) || c->ty.ensure((size_t)n, st) ||
-        c->tg.ensure((size_t)n, st) || c->dcount.ensure(4, st))
+    if (c->flags.ensure((size_t)n_ub, st) || c->scan.ensure((size_t)n_ub, st) ||
+        c->lists.ensure((size_t)(2 * n_ub), st) || c->misc.ensure((size_t)n_ub, st) ||
+        c->tx.ensure((size_t)n_ub, st) || c->ty.ensure((size_t)n_ub, st) || c->tg.ensure((size_t)n_ub, st) ||
+        c->dcount.ensure(8, st))
         return 1;
     int32_t *pc = c->flags.p, *state = c->scan.p, *cand = c->lists.p, *first = c->misc.p;
     unsigned long long *ncand = (unsigned long long *)(c->dcount.p + 2);
-    const unsigned nb = (unsigned)((n + 255) / 256);
-    VX_CHECK(hipMemsetAsync(ncand, 0, 2 * sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n, pc, state, first);
-    hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, n, pc, cand, ncand);
+    const unsigned nb = (unsigned)((n_ub + 255) / 256);
+    hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, pc, state, first, ncand);
+    hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, step_dn(c), pc, cand, ncand);
     hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, cand, ncand, pc, state,
                        (int32_t *)(c->dcount.p + 3));
     hipLaunchKernelGGL(k_col_first, dim3(nb), dim3(256), 0, st, c->col.p, cand, ncand, state, first);
