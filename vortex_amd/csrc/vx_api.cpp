@@ -122,7 +122,8 @@ int vx_case_destroy(vx_case *c) {
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     vx::DBuf<double> *db[] = {&c->L, &c->M, &c->x, &c->y, &c->g, &c->u, &c->v, &c->d, &c->rp, &c->gsheet,
-                              &c->partial, &c->red, &c->tx, &c->ty, &c->tg, &c->px, &c->py, &c->pu, &c->pv, &c->pg};
+                              &c->partial, &c->red, &c->tx, &c->ty, &c->tg, &c->px, &c->py, &c->pu, &c->pv, &c->pg,
+                              &c->rhs_s};
     for (auto *p : db) p->release();
     c->xf.release(); c->yf.release(); c->gf.release();
     c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();

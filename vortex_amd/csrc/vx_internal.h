@@ -105,6 +105,7 @@ struct vx_case {
     vx::DBuf<float> xf, yf, gf;
     // scratch
     vx::DBuf<double> rp, gsheet, partial, red;       // RHS, sheet, split-K and reductions
+    vx::DBuf<double> rhs_s;                           // RHS far-field expansion sums
     vx::DBuf<double> tx, ty, tg;                      // compaction/collapse staging
     vx::DBuf<int32_t> flags, scan, lists, col, misc;
     vx::DBuf<uint64_t> keys, keys_alt;

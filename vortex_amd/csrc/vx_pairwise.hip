@@ -9,6 +9,8 @@
 // targets spaced one block apart so that target loads and stores coalesce.
 #include <cfloat>
 
+#include <hipcub/hipcub.hpp>
+
 #include "vx_internal.h"
 
 namespace vx {
@@ -244,20 +246,46 @@ __device__ __forceinline__ float vx_atan2f(float y, float x) {
     return __builtin_copysignf(r, y);
 }
 
-// ---- A3: src/kernel.cu:779-847, split over source chunks -----------------
+// ---- A3: src/kernel.cu:779-847 --------------------------------------------
 // For panel i with end points l, r, d = r - l:
 //   y += g_j (alpha d + beta k x d), alpha = atan(s0.d/z) - atan(s1.d/z),
 //   beta = 0.5 ln(|s1|^2/|s0|^2), s0 = l - r_j, s1 = r - r_j, z = d x s0.
 // In FP32 alpha is evaluated as one atan2(-|d|^2 z, z^2 + (s0.d)(s1.d)),
-// the same angle difference.
+// the same angle difference.  This is the panel average of the point
+// vortices' conjugate velocity g/(2 pi i (z - z_j)), times 2 pi |d|^2.
+//
+// Sources far from the profile are summed as one local expansion instead.
+// The panels lie in the profile box, centre c, half-diagonal R_b.  For
+// |z_j - c| >= 2 R_b and z on a panel,
+//   (u - i v)(z) = sum_k L_k (z - c)^k,  L_k = (i / 2 pi) S_k,
+//   S_k = sum_j g_j w_j^{k+1},  w_j = 1 / (z_j - c),
+// truncated after kLoc terms, which leaves at most 2^{1-kLoc} of the far sum.
+// Its panel average is exact:
+//   mean over [z_l, z_r] of (z - c)^k = Q_k / (k + 1),
+//   Q_k = (a^{k+1} - b^{k+1}) / (a - b) = a Q_{k-1} + b^k,  a = z_r - c, b = z_l - c.
+// The sources closer than 2 R_b (with g != 0) are listed in index order and
+// summed pair by pair, split over chunks of kRhsChunk list entries.
+constexpr int kLoc = 48;
+constexpr int kRhsChunk = 512;
+
+struct RhsNear {
+    const double *x, *y, *g;
+    double cx, cy, r2;
+    __device__ bool operator()(int32_t j) const {
+        const double dx = x[j] - cx, dy = y[j] - cy;
+        return g[j] != 0 && dx * dx + dy * dy < r2;
+    }
+};
+
 template <typename T>
 __global__ __launch_bounds__(kTile) void k_right_part_partial(
-    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, int64_t n_src,
-    int64_t chunk, PanelsSoA pan, double *__restrict__ part) {
+    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, const int32_t *__restrict__ list,
+    const int32_t *__restrict__ nlist, PanelsSoA pan, double *__restrict__ part) {
     __shared__ Src4<T> ls[kTile];
     const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
     const int64_t c = blockIdx.y;
-    const int64_t lo = c * chunk, hi = min(n_src, lo + chunk);
+    const int64_t n_src = *nlist;
+    const int64_t lo = c * kRhsChunk, hi = min(n_src, lo + kRhsChunk);
     const bool live = i < pan.P;
     const T l0 = live ? (T)pan.lx[i] : T(0), l1 = live ? (T)pan.ly[i] : T(0);
     const T r0 = live ? (T)pan.rx[i] : T(1), r1 = live ? (T)pan.ry[i] : T(0);
@@ -267,7 +295,22 @@ __global__ __launch_bounds__(kTile) void k_right_part_partial(
     double a0 = 0.0, a1 = 0.0;
     for (int64_t base = lo; base < hi; base += kTile) {
         __syncthreads();
-        stage_tile<T>(ls, sx, sy, sg, base, hi);
+        {
+            const int64_t j = base + threadIdx.x;
+            Src4<T> q;
+            if (j < hi) {
+                const int32_t k = list[j];
+                q.x = sx[k];
+                q.y = sy[k];
+                q.g = sg[k];
+            } else {
+                q.x = Arith<T>::kFar;
+                q.y = Arith<T>::kFar;
+                q.g = T(0);
+            }
+            q.pad = T(0);
+            ls[threadIdx.x] = q;
+        }
         __syncthreads();
         const int cnt = (int)min((int64_t)kTile, hi - base);
         for (int jj = 0; jj < cnt; ++jj) {
@@ -305,6 +348,59 @@ __global__ __launch_bounds__(kTile) void k_right_part_partial(
     }
 }
 
+// Far sources: per work-group partial S_k (FP64), k < kLoc, grid-stride over
+// the sources; spart[(blk * kLoc + k) * 2 + {0,1}].
+__global__ __launch_bounds__(256) void k_rhs_far_partial(const double *__restrict__ x, const double *__restrict__ y,
+                                                         const double *__restrict__ g, int64_t n_src, double cx,
+                                                         double cy, double r2, double *__restrict__ spart) {
+    __shared__ double red[16][256];
+    double sr[kLoc], si[kLoc];
+#pragma unroll
+    for (int k = 0; k < kLoc; ++k) sr[k] = si[k] = 0.0;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_src; j += (int64_t)gridDim.x * blockDim.x) {
+        const double gj = g[j];
+        const double dx = x[j] - cx, dy = y[j] - cy;
+        const double q2 = dx * dx + dy * dy;
+        if (gj == 0 || !(q2 >= r2)) continue;
+        const double wr = dx / q2, wi = -dy / q2;  // w = 1 / (z_j - c)
+        double pr = gj * wr, pi = gj * wi;           // g w^{k+1}
+#pragma unroll
+        for (int k = 0; k < kLoc; ++k) {
+            sr[k] += pr;
+            si[k] += pi;
+            const double npr = pr * wr - pi * wi;
+            pi = pr * wi + pi * wr;
+            pr = npr;
+        }
+    }
+    // block sums, eight terms at a time, in a fixed order
+#pragma unroll
+    for (int k0 = 0; k0 < kLoc; k0 += 8) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            red[2 * q][threadIdx.x] = sr[k0 + q];
+            red[2 * q + 1][threadIdx.x] = si[k0 + q];
+        }
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st)
+                for (int q = 0; q < 16; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x < 16) spart[((int64_t)blockIdx.x * kLoc + k0) * 2 + threadIdx.x] = red[threadIdx.x][0];
+    }
+}
+
+// S_k over the work-group partials (one thread per real/imaginary part).
+__global__ void k_rhs_far_final(const double *__restrict__ spart, int64_t nblk, double *__restrict__ S) {
+    const int t = threadIdx.x;
+    if (t >= 2 * kLoc) return;
+    double a = 0.0;
+    for (int64_t b = 0; b < nblk; ++b) a += spart[b * 2 * kLoc + t];
+    S[t] = a;
+}
+
 // Sum of the chunk partials part[(c * 2 + q) * m + i] for one output i per
 // work-group: a strided pass per thread, then a tree in LDS (fixed order).
 __device__ __forceinline__ void chunk_sum2(const double *__restrict__ part, int64_t nchunk, int64_t m, int64_t i,
@@ -329,9 +425,11 @@ __device__ __forceinline__ void chunk_sum2(const double *__restrict__ part, int6
     b = r[1][0];
 }
 
-// One work-group per panel.
+// One work-group per panel: the near chunk partials, plus the panel average
+// of the far-field expansion (thread 0).
 __global__ __launch_bounds__(kTile) void k_right_part_final(const double *__restrict__ part, int64_t nchunk,
-                                                            PanelsSoA pan, double vinf_x, double vinf_y,
+                                                            PanelsSoA pan, const double *__restrict__ S, double cx,
+                                                            double cy, double vinf_x, double vinf_y,
                                                             double *__restrict__ R) {
     const int64_t i = blockIdx.x;
     double y0, y1;
@@ -339,7 +437,29 @@ __global__ __launch_bounds__(kTile) void k_right_part_final(const double *__rest
     if (threadIdx.x != 0) return;
     const double d0 = pan.rx[i] - pan.lx[i], d1 = pan.ry[i] - pan.ly[i];
     const double mnog = 1 / (2 * kPi * (d0 * d0 + d1 * d1));
-    R[i] = -((mnog * y0 + vinf_x) * pan.tx[i] + (mnog * y1 + vinf_y) * pan.ty[i]);
+    // W = sum_k (i S_k / 2 pi) Q_k / (k + 1), u = Re W, v = -Im W
+    const double ar = pan.rx[i] - cx, ai = pan.ry[i] - cy;  // a = z_r - c
+    const double br = pan.lx[i] - cx, bi = pan.ly[i] - cy;  // b = z_l - c
+    double qr = 1.0, qi = 0.0;                               // Q_0
+    double bkr = 1.0, bki = 0.0;                             // b^k
+    double wr = 0.0, wi = 0.0;
+    for (int k = 0; k < kLoc; ++k) {
+        if (k > 0) {
+            const double nbr = bkr * br - bki * bi;
+            bki = bkr * bi + bki * br;
+            bkr = nbr;
+            const double nqr = ar * qr - ai * qi + bkr;
+            qi = ar * qi + ai * qr + bki;
+            qr = nqr;
+        }
+        const double sr = S[2 * k], si = S[2 * k + 1];
+        const double inv = 1.0 / (double)(k + 1);
+        wr += (sr * qr - si * qi) * inv;
+        wi += (sr * qi + si * qr) * inv;
+    }
+    // times i / (2 pi)
+    const double uf = -wi / (2 * kPi), vf = -(wr / (2 * kPi));
+    R[i] = -((mnog * y0 + uf + vinf_x) * pan.tx[i] + (mnog * y1 + vf + vinf_y) * pan.ty[i]);
 }
 
 // ---- A19: src/kernel.cu:1434-1475, split over source chunks ---------------
@@ -452,21 +572,44 @@ int launch_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src) {
 
 int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
     const int64_t P = c->P;
+    hipStream_t st = c->stream;
     const int64_t tb = (P + kTile - 1) / kTile;
-    int64_t chunk, nchunk;
-    split(n_src, tb, &chunk, &nchunk);
-    if (c->partial.ensure((size_t)(2 * nchunk * P), c->stream) || c->rp.ensure((size_t)P, c->stream)) return 1;
+    const int64_t nchunk = n_src > 0 ? (n_src + kRhsChunk - 1) / kRhsChunk : 1;
+    constexpr int64_t kFarBlocks = 512;
+    if (c->partial.ensure((size_t)(2 * nchunk * P), st) || c->rp.ensure((size_t)P, st) ||
+        c->lists.ensure((size_t)(n_src > 0 ? n_src : 1), st) || c->dcount.ensure(8, st) ||
+        c->rhs_s.ensure((size_t)(2 * kLoc * (kFarBlocks + 1)), st))
+        return 1;
+    const vx_params &p = c->prm;
+    const double cx = 0.5 * (p.px_min + p.px_max), cy = 0.5 * (p.py_min + p.py_max);
+    const double wx = p.px_max - p.px_min, wy = p.py_max - p.py_min;
+    const double rnear2 = wx * wx + wy * wy;  // (2 R_b)^2
+    int32_t *nlist = (int32_t *)(c->dcount.p + 7);
+    double *S = c->rhs_s.p + 2 * kLoc * kFarBlocks;
+    // near sources, in index order
+    {
+        const RhsNear pred{c->x.p, c->y.p, c->g.p, cx, cy, rnear2};
+        hipcub::CountingInputIterator<int32_t> it(0);
+        size_t bytes = 0;
+        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, c->lists.p, nlist, n_src, pred, st));
+        if (c->cub_tmp.ensure(bytes, st)) return 1;
+        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, c->lists.p, nlist, n_src, pred, st));
+    }
     if (mixed(c)) {
         if (launch_pack_f32(c, n_src)) return 1;
-        hipLaunchKernelGGL(k_right_part_partial<float>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0,
-                           c->stream, c->xf.p, c->yf.p, c->gf.p, n_src, chunk, c->pan, c->partial.p);
+        hipLaunchKernelGGL(k_right_part_partial<float>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0, st,
+                           c->xf.p, c->yf.p, c->gf.p, c->lists.p, nlist, c->pan, c->partial.p);
     } else {
-        hipLaunchKernelGGL(k_right_part_partial<double>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0,
-                           c->stream, c->x.p, c->y.p, c->g.p, n_src, chunk, c->pan, c->partial.p);
+        hipLaunchKernelGGL(k_right_part_partial<double>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0, st,
+                           c->x.p, c->y.p, c->g.p, c->lists.p, nlist, c->pan, c->partial.p);
     }
+    // far sources: the local expansion about the profile box centre
+    hipLaunchKernelGGL(k_rhs_far_partial, dim3((unsigned)kFarBlocks), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n_src,
+                       cx, cy, rnear2, c->rhs_s.p);
+    hipLaunchKernelGGL(k_rhs_far_final, dim3(1), dim3(2 * kLoc), 0, st, c->rhs_s.p, kFarBlocks, S);
     VX_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)P), dim3(kTile), 0, c->stream, c->partial.p, nchunk,
-                       c->pan, vinf_x, vinf_y, c->rp.p);
+    hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)P), dim3(kTile), 0, st, c->partial.p, nchunk, c->pan, S, cx,
+                       cy, vinf_x, vinf_y, c->rp.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
