@@ -392,13 +392,21 @@ __global__ __launch_bounds__(256) void k_rhs_far_partial(const double *__restric
     }
 }
 
-// S_k over the work-group partials (one thread per real/imaginary part).
-__global__ void k_rhs_far_final(const double *__restrict__ spart, int64_t nblk, double *__restrict__ S) {
-    const int t = threadIdx.x;
-    if (t >= 2 * kLoc) return;
+// S_k over the work-group partials: one work-group per real/imaginary part of
+// each term, a strided pass per thread and a tree in LDS (fixed order).
+__global__ __launch_bounds__(256) void k_rhs_far_final(const double *__restrict__ spart, int64_t nblk,
+                                                       double *__restrict__ S) {
+    __shared__ double r[256];
+    const int t = blockIdx.x;
     double a = 0.0;
-    for (int64_t b = 0; b < nblk; ++b) a += spart[b * 2 * kLoc + t];
-    S[t] = a;
+    for (int64_t b = threadIdx.x; b < nblk; b += 256) a += spart[b * 2 * kLoc + t];
+    r[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) r[threadIdx.x] += r[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S[t] = r[0];
 }
 
 // Sum of the chunk partials part[(c * 2 + q) * m + i] for one output i per
@@ -443,6 +451,7 @@ __global__ __launch_bounds__(kTile) void k_right_part_final(const double *__rest
     double qr = 1.0, qi = 0.0;                               // Q_0
     double bkr = 1.0, bki = 0.0;                             // b^k
     double wr = 0.0, wi = 0.0;
+#pragma unroll
     for (int k = 0; k < kLoc; ++k) {
         if (k > 0) {
             const double nbr = bkr * br - bki * bi;
@@ -606,7 +615,7 @@ int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
     // far sources: the local expansion about the profile box centre
     hipLaunchKernelGGL(k_rhs_far_partial, dim3((unsigned)kFarBlocks), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n_src,
                        cx, cy, rnear2, c->rhs_s.p);
-    hipLaunchKernelGGL(k_rhs_far_final, dim3(1), dim3(2 * kLoc), 0, st, c->rhs_s.p, kFarBlocks, S);
+    hipLaunchKernelGGL(k_rhs_far_final, dim3(2 * kLoc), dim3(256), 0, st, c->rhs_s.p, kFarBlocks, S);
     VX_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)P), dim3(kTile), 0, st, c->partial.p, nchunk, c->pan, S, cx,
                        cy, vinf_x, vinf_y, c->rp.p);

@@ -610,9 +610,9 @@ int launch_setka(vx_case *c, int kind, int64_t n_ub) {
 // candidates sharing a target j the smallest merges with j as it was, the
 // others with the killed j (g = 0 at 1e10), exactly as the sweep does.
 __global__ void k_col_prepare(int64_t n_ub, int32_t *__restrict__ pc, int32_t *__restrict__ state,
-                              int32_t *__restrict__ first, unsigned long long *__restrict__ ncand_err) {
+                              int32_t *__restrict__ first, int32_t *__restrict__ nrest) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) ncand_err[0] = 0ull;  // candidate count
+    if (i == 0) nrest[0] = 0;  // non-leaf candidate count
     if (i >= n_ub) return;
     state[i] = 0;
     first[i] = INT32_MAX;
@@ -620,27 +620,65 @@ __global__ void k_col_prepare(int64_t n_ub, int32_t *__restrict__ pc, int32_t *_
 }
 
 __global__ __launch_bounds__(256) void k_col_count(const int32_t *__restrict__ col, const int64_t *__restrict__ dn,
-                                                   int32_t *__restrict__ pc, int32_t *__restrict__ cand,
-                                                   unsigned long long *__restrict__ ncand) {
+                                                   int32_t *__restrict__ pc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t j = i < *dn ? col[i] : -1;
     if (j >= 0) atomicAdd(&pc[j], 1);
-    const uint64_t m = __ballot(j >= 0);
+}
+
+// Candidates in index order: i < *dn with COL_i >= 0.
+struct ColCandidate {
+    const int32_t *col;
+    const int64_t *dn;
+    __device__ bool operator()(int32_t i) const { return i < *dn && col[i] >= 0; }
+};
+
+// Leaves of the forest (no candidate points at them) are active: nothing
+// precedes them.  The first kernel decides them (pc is only read) and lists the
+// other candidates; the second publishes them (kills COL_i, counts the edge off).
+__global__ __launch_bounds__(256) void k_col_leaves(const int32_t *__restrict__ cand,
+                                                    const unsigned long long *__restrict__ ncand_p,
+                                                    const int32_t *__restrict__ pc, int32_t *__restrict__ state,
+                                                    int32_t *__restrict__ rest, int32_t *__restrict__ nrest) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ncand = (int64_t)*ncand_p;
+    const int32_t i = t < ncand ? cand[t] : -1;
+    const bool leaf = i >= 0 && pc[i] == 0;
+    if (leaf) state[i] = 1;
+    const uint64_t m = __ballot(i >= 0 && !leaf);
     if (m == 0) return;
     const int lane = threadIdx.x & 63;
-    unsigned long long base = 0;
-    if (lane == __builtin_ctzll(m)) base = atomicAdd(ncand, (unsigned long long)__popcll(m));
-    base = __shfl(base, __builtin_ctzll(m), 64);
-    if (j >= 0) cand[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)i;
+    const int first = __builtin_ctzll(m);
+    int base = 0;
+    if (lane == first) base = atomicAdd(nrest, __popcll(m));
+    base = __shfl(base, first, 64);
+    if (i >= 0 && !leaf) rest[base + __popcll(m & ((1ull << lane) - 1))] = i;
+}
+
+__global__ __launch_bounds__(256) void k_col_publish_leaves(const int32_t *__restrict__ col,
+                                                            const int32_t *__restrict__ cand,
+                                                            const unsigned long long *__restrict__ ncand_p,
+                                                            int32_t *__restrict__ pc, int32_t *__restrict__ state) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)*ncand_p) return;
+    const int32_t i = cand[t];
+    // a leaf's slot is written by this thread only; other candidates' slots
+    // may be receiving bit 4 concurrently, which never carries bit 1
+    if (!(atomicOr(&state[i], 0) & 1)) return;
+    const int32_t j = col[i];
+    state[i] = 1 | 16;
+    atomicOr(&state[j], 4);
+    atomicSub(&pc[j], 1);
 }
 
 // state bits: 1 active, 2 inactive, 4 killed by an active child, 16 published.
+// The remaining (non-leaf) candidates are decided here in one work-group.
 // Every access goes through L2 atomics (no stale per-CU L1 lines).  Each round
 // decides at least the lowest undecided candidate (its children have lower
 // indices and were published in earlier rounds), so ncand + 1 rounds always
 // suffice; the loop bound is a guard, reported through *err.
 __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
-                                                      const unsigned long long *__restrict__ ncand_p,
+                                                      const int32_t *__restrict__ ncand_p,
                                                       int32_t *__restrict__ pc, int32_t *__restrict__ state,
                                                       int32_t *__restrict__ err) {
     __shared__ int undecided;
@@ -749,9 +787,20 @@ int launch_collapse(vx_case *c, int kind, int64_t n_ub) {
     int32_t *pc = c->flags.p, *state = c->scan.p, *cand = c->lists.p, *first = c->misc.p;
     unsigned long long *ncand = (unsigned long long *)(c->dcount.p + 2);
     const unsigned nb = (unsigned)((n_ub + 255) / 256);
-    hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, pc, state, first, ncand);
-    hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, step_dn(c), pc, cand, ncand);
-    hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, cand, ncand, pc, state,
+    int32_t *rest = c->lists.p + n_ub, *nrest = (int32_t *)(c->dcount.p + 4);
+    hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, pc, state, first, nrest);
+    hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, step_dn(c), pc);
+    {
+        const ColCandidate pred{c->col.p, step_dn(c)};
+        hipcub::CountingInputIterator<int32_t> it(0);
+        size_t bytes = 0;
+        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cand, ncand, (int)n_ub, pred, st));
+        if (c->cub_tmp.ensure(bytes, st)) return 1;
+        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cand, ncand, (int)n_ub, pred, st));
+    }
+    hipLaunchKernelGGL(k_col_leaves, dim3(nb), dim3(256), 0, st, cand, ncand, pc, state, rest, nrest);
+    hipLaunchKernelGGL(k_col_publish_leaves, dim3(nb), dim3(256), 0, st, c->col.p, cand, ncand, pc, state);
+    hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, rest, nrest, pc, state,
                        (int32_t *)(c->dcount.p + 3));
     hipLaunchKernelGGL(k_col_first, dim3(nb), dim3(256), 0, st, c->col.p, cand, ncand, state, first);
     if (kind == 1)

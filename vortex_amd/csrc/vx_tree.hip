@@ -53,6 +53,9 @@ struct TreeBufs {
     DBuf<int32_t> idx, perm, incl, child0, child1, first, count, parent, fb;
     DBuf<int32_t> rlo, rhi, rleft, rright, bstart, bscan, upper, uscan, visits;
     DBuf<int32_t> gcs, gcs2, gflag, gscan, gfirst;  // target groups
+    DBuf<int32_t> pcount;                          // included particles per block
+    DBuf<int32_t> arr;                             // cut nodes, count at arr[0]
+    DBuf<int64_t> sizes;                           // group, bucket and upper-node counts
     DBuf<double> sx, sy, sg, ncx, ncy, nr, fbr2, hpart;
     DBuf<float> fx, fy, fg;
     DBuf<double4> nbb, part, cbox;
@@ -66,6 +69,7 @@ struct TreeBufs {
     int64_t ngroup = 0;                    // target groups (gfirst)
     int32_t root = 0;
     int order = 16;
+    hipEvent_t ev = nullptr;                       // read-back completion
 };
 
 struct TreeDev {
@@ -117,7 +121,7 @@ __device__ __forceinline__ uint32_t spread16(uint32_t v) {
 __global__ __launch_bounds__(256) void k_incl_box(const double *__restrict__ x, const double *__restrict__ y,
                                                   const double *__restrict__ g, int64_t s, double4 dbox,
                                                   int32_t *__restrict__ incl, double4 *__restrict__ part,
-                                                  unsigned long long *__restrict__ cnt) {
+                                                  int32_t *__restrict__ pcount) {
     __shared__ double r[4][256];
     __shared__ int c[256];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -145,26 +149,35 @@ __global__ __launch_bounds__(256) void k_incl_box(const double *__restrict__ x, 
     }
     if (threadIdx.x == 0) {
         part[blockIdx.x] = make_double4(r[0][0], -r[1][0], r[2][0], -r[3][0]);
-        atomicAdd(cnt, (unsigned long long)c[0]);
+        pcount[blockIdx.x] = c[0];
     }
 }
 
-__global__ __launch_bounds__(256) void k_box_final(const double4 *__restrict__ part, int64_t nb,
-                                                   double4 *__restrict__ box) {
+// Box of the included particles and their count (cnt[0]) from the block partials.
+__global__ __launch_bounds__(256) void k_box_final(const double4 *__restrict__ part, const int32_t *__restrict__ pcount,
+                                                   int64_t nb, double4 *__restrict__ box,
+                                                   unsigned long long *__restrict__ cnt) {
     __shared__ double r[4][256];
+    __shared__ long long rc[256];
     double a[4] = {1e300, 1e300, 1e300, 1e300};
+    long long k = 0;
     for (int64_t b = threadIdx.x; b < nb; b += 256) {
         const double4 p = part[b];
         a[0] = fmin(a[0], p.x); a[1] = fmin(a[1], -p.y); a[2] = fmin(a[2], p.z); a[3] = fmin(a[3], -p.w);
+        k += pcount[b];
     }
     for (int q = 0; q < 4; ++q) r[q][threadIdx.x] = a[q];
+    rc[threadIdx.x] = k;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st)
+        if (threadIdx.x < st) {
             for (int q = 0; q < 4; ++q) r[q][threadIdx.x] = fmin(r[q][threadIdx.x], r[q][threadIdx.x + st]);
+            rc[threadIdx.x] += rc[threadIdx.x + st];
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        cnt[0] = (unsigned long long)rc[0];
         double4 b = make_double4(r[0][0], -r[1][0], r[2][0], -r[3][0]);
         if (!(b.x <= b.y)) b = make_double4(0.0, 1.0, 0.0, 1.0);
         box[0] = b;
@@ -273,16 +286,17 @@ __global__ void k_bucket_mark(const uint32_t *__restrict__ key, const int32_t *_
 }
 
 // Compact node array: buckets [0, nb) in particle order, then the kept
-// internal nodes.  Writes ranges, children and parents.
-__global__ void k_bucket_ranges(const int32_t *__restrict__ bstart, const int32_t *__restrict__ bscan, int64_t m,
-                                int64_t nb, int32_t *__restrict__ first, int32_t *__restrict__ count) {
+// internal nodes.  The buckets' first particles, then their counts.
+__global__ void k_bucket_first(const int32_t *__restrict__ bstart, const int32_t *__restrict__ bscan, int64_t m,
+                               int32_t *__restrict__ first) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m || !bstart[t]) return;
-    const int32_t b = bscan[t];
-    first[b] = (int32_t)t;
-    int64_t e = t + 1;
-    while (e < m && !bstart[e]) ++e;  // buckets hold at most kLeaf particles
-    count[b] = (int32_t)(e - t);
+    if (t < m && bstart[t]) first[bscan[t]] = (int32_t)t;
+}
+
+__global__ void k_bucket_count(const int32_t *__restrict__ first, int64_t nb, int64_t m,
+                               int32_t *__restrict__ count) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) count[b] = (int32_t)((b + 1 < nb ? (int64_t)first[b + 1] : m) - first[b]);
 }
 
 __global__ void k_upper_nodes(const int32_t *__restrict__ lo, const int32_t *__restrict__ hi,
@@ -376,6 +390,7 @@ __device__ void node_from_children(int32_t p, const int32_t *__restrict__ child0
     double ar[ORDER], ai[ORDER];
 #pragma unroll
     for (int q = 0; q < ORDER; ++q) ar[q] = ai[q] = 0.0;
+#pragma unroll 1
     for (int ch = 0; ch < 2; ++ch) {
         const int32_t cc = c[ch];
         const double dr = (ncx[cc] - cx) / Rs, di = (ncy[cc] - cy) / Rs;
@@ -416,25 +431,101 @@ __device__ void node_from_children(int32_t p, const int32_t *__restrict__ child0
     nbb[p] = b;
 }
 
-// Bottom-up pass, one thread per bucket: the second thread to reach a node
-// builds it and continues upward; the first one stops.  No thread waits on
-// another, so the pass always completes.  The fences publish a node before
-// its parent's counter moves and order the reads of the sibling after it.
-template <int ORDER>
-__global__ void k_upper_build(int64_t nb, const int32_t *__restrict__ parent, int32_t *__restrict__ visits,
-                              const int32_t *__restrict__ child0, const int32_t *__restrict__ child1,
-                              double *__restrict__ ncx, double *__restrict__ ncy, double *__restrict__ nr,
-                              double4 *__restrict__ nbb, double2 *__restrict__ cd, float2 *__restrict__ cf) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    int32_t p = parent[b];
-    while (p >= 0) {
-        __threadfence();
-        if (atomicAdd(&visits[p], 1) == 0) return;
-        __threadfence();
-        node_from_children<ORDER>(p, child0, child1, ncx, ncy, nr, nbb, cd, cf);
-        p = parent[p];
+// Bottom-up pass in two kernels: the second thread to reach a node builds it
+// and continues upward; the first one stops.  No thread waits on another, so
+// each pass always completes.  All the arrivals at a node come from one
+// work-group, so work-group-scope fences order them: they publish a node
+// before its parent's counter moves and order the reads of the sibling after
+// it (the work-group shares one CU and its L1).
+//   Cut nodes hold at most kCut buckets and have a parent (or are the root)
+//   with more; their subtrees partition the buckets.
+//   k_upper_local: a work-group per cut node, a thread per bucket, builds the
+//     cut node's subtree.
+//   k_upper_top: one work-group, a thread per cut node, builds the nodes above.
+constexpr int kCut = 256;
+
+__device__ __forceinline__ int64_t bucket_lo(int32_t u, int64_t nb, const int32_t *__restrict__ first,
+                                             const int32_t *__restrict__ bscan) {
+    return u < nb ? (int64_t)u : (int64_t)bscan[first[u]];
+}
+
+__device__ __forceinline__ int64_t bucket_n(int32_t u, int64_t nb, int64_t m, const int32_t *__restrict__ first,
+                                            const int32_t *__restrict__ count, const int32_t *__restrict__ bscan) {
+    if (u < nb) return 1;
+    const int32_t f = first[u];
+    const int64_t e = (int64_t)f + count[u];
+    return (e < m ? (int64_t)bscan[e] : nb) - bscan[f];
+}
+
+struct CutNode {
+    const int32_t *first, *count, *bscan, *parent;
+    int64_t nb, m;
+    __device__ bool operator()(int32_t u) const {
+        if (bucket_n(u, nb, m, first, count, bscan) > kCut) return false;
+        const int32_t p = parent[u];
+        return p < 0 || bucket_n(p, nb, m, first, count, bscan) > kCut;
     }
+};
+
+__device__ __forceinline__ bool arrive_wg(int32_t *visits, int32_t p) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (__hip_atomic_fetch_add(&visits[p], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return true;
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(kCut) void k_upper_local(const int32_t *__restrict__ cut, const int32_t *__restrict__ ncut,
+                                                      int64_t nb, int64_t m, const int32_t *__restrict__ bscan,
+                                                      const int32_t *__restrict__ first,
+                                                      const int32_t *__restrict__ count,
+                                                      const int32_t *__restrict__ parent, int32_t *__restrict__ visits,
+                                                      const int32_t *__restrict__ child0,
+                                                      const int32_t *__restrict__ child1, double *__restrict__ ncx,
+                                                      double *__restrict__ ncy, double *__restrict__ nr,
+                                                      double4 *__restrict__ nbb, double2 *__restrict__ cd,
+                                                      float2 *__restrict__ cf) {
+    const int n = *ncut;
+    for (int w = blockIdx.x; w < n; w += gridDim.x) {
+        const int32_t r = cut[w];
+        if ((int64_t)threadIdx.x >= bucket_n(r, nb, m, first, count, bscan)) continue;
+        int32_t cur = (int32_t)(bucket_lo(r, nb, first, bscan) + threadIdx.x);
+        while (cur != r) {
+            const int32_t p = parent[cur];
+            if (!arrive_wg(visits, p)) break;
+            node_from_children<ORDER>(p, child0, child1, ncx, ncy, nr, nbb, cd, cf);
+            cur = p;
+        }
+    }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_upper_top(const int32_t *__restrict__ cut, const int32_t *__restrict__ ncut,
+                                                   const int32_t *__restrict__ parent, int32_t *__restrict__ visits,
+                                                   const int32_t *__restrict__ child0,
+                                                   const int32_t *__restrict__ child1, double *__restrict__ ncx,
+                                                   double *__restrict__ ncy, double *__restrict__ nr,
+                                                   double4 *__restrict__ nbb, double2 *__restrict__ cd,
+                                                   float2 *__restrict__ cf) {
+    const int n = *ncut;
+    for (int k = threadIdx.x; k < n; k += 256) {
+        int32_t p = parent[cut[k]];
+        while (p >= 0) {
+            if (!arrive_wg(visits, p)) break;
+            node_from_children<ORDER>(p, child0, child1, ncx, ncy, nr, nbb, cd, cf);
+            p = parent[p];
+        }
+    }
+}
+
+// Group count, bucket count and upper-node count from the ends of their scans.
+__global__ void k_tree_counts(const int32_t *__restrict__ gscan, const int32_t *__restrict__ gflag,
+                              const int32_t *__restrict__ bscan, const int32_t *__restrict__ bstart,
+                              const int32_t *__restrict__ uscan, const int32_t *__restrict__ upper, int64_t m, int big,
+                              int64_t *__restrict__ out) {
+    out[0] = (int64_t)gscan[m - 1] + gflag[m - 1];
+    out[1] = big ? (int64_t)bscan[m - 1] + bstart[m - 1] : 1;
+    out[2] = big ? (int64_t)uscan[m - 2] + upper[m - 2] : 0;
 }
 
 __global__ void k_pack_nodes(int64_t nn, const double *__restrict__ ncx, const double *__restrict__ ncy,
@@ -727,55 +818,102 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     }
 }
 
-// Exact nearest-3 for the flagged targets, FP64: a depth-first search from
-// the root that visits the nearer child first and prunes every node farther
-// than the current third distance (initially the fast pass's, an upper bound
-// since its three candidates are real particles).
+// Exact nearest-3 for the flagged targets, FP64, one wave per target.  The
+// four smallest squared distances (the target itself included) are searched
+// below a bound: the fast pass's third distance, an upper bound since its
+// three candidates are real particles, tightened by the smallest of the
+// lanes' own fourth distances.  Nodes are popped up to 64 at a time from a
+// stack in LDS; a lane that pops a leaf scans its particles into its private
+// four smallest, which merge over the wave at the end.  Should the stack fill,
+// the target restarts as a scan of every leaf, which needs no stack.
 __device__ __forceinline__ double box_d2(const double4 &b, double ax, double ay) {
     const double ex = fmax(0.0, fmax(b.x - ax, ax - b.y));
     const double ey = fmax(0.0, fmax(b.z - ay, ay - b.w));
     return ex * ex + ey * ey;
 }
 
-__global__ void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ fb, const double *__restrict__ fbr2,
-                                        const unsigned long long *__restrict__ nfb, double *__restrict__ d) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)*nfb) return;
-    const int64_t ti = fb[q];
-    const int32_t i = t.perm[ti];
-    const double ax = t.sx[ti], ay = t.sy[ti];
-    const double bound0 = fmin(fbr2[q] * (1.0 + 1e-3) + 1e-300, 1e5);
-    double m0 = 1e5, m1 = 1e5, m2 = 1e5, m3 = 1e5;
-    int stack[64];
-    int sp = 0;
-    stack[sp++] = t.root;
-    while (sp > 0) {
-        const int node = stack[--sp];
-        const double lim = fmin(bound0, m3);
-        if (box_d2(t.nbb[node], ax, ay) > lim) continue;
-        if (t.child0[node] < 0) {
-            const int64_t j0 = t.first[node], j1 = j0 + t.count[node];
-            for (int64_t j = j0; j < j1; ++j) {
-                const double dx = ax - t.sx[j], dy = ay - t.sy[j];
-                const double r2 = dx * dx + dy * dy;
-                m3 = PairMath<double>::med3(m2, m3, r2);
-                m2 = PairMath<double>::med3(m1, m2, r2);
-                m1 = PairMath<double>::med3(m0, m1, r2);
-                m0 = fmin(m0, r2);
+constexpr int kFbStack = 4096;
+
+__device__ __forceinline__ void near4_leaf(const TreeDev &t, int node, double ax, double ay, double &m0, double &m1,
+                                           double &m2, double &m3) {
+    const int64_t j0 = t.first[node], j1 = j0 + t.count[node];
+    for (int64_t j = j0; j < j1; ++j) {
+        const double dx = ax - t.sx[j], dy = ay - t.sy[j];
+        const double r2 = dx * dx + dy * dy;
+        m3 = PairMath<double>::med3(m2, m3, r2);
+        m2 = PairMath<double>::med3(m1, m2, r2);
+        m1 = PairMath<double>::med3(m0, m1, r2);
+        m0 = fmin(m0, r2);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ fb,
+                                                              const double *__restrict__ fbr2,
+                                                              const unsigned long long *__restrict__ nfb,
+                                                              double *__restrict__ d) {
+    __shared__ int32_t stk[kFbStack];
+    const int lane = threadIdx.x;
+    const int64_t nq = (int64_t)*nfb;
+    for (int64_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const int64_t ti = fb[q];
+        const double ax = t.sx[ti], ay = t.sy[ti];
+        const double bound0 = fmin(fbr2[q] * (1.0 + 1e-3) + 1e-300, 1e5);
+        double m0 = 1e5, m1 = 1e5, m2 = 1e5, m3 = 1e5;
+        bool full = false;  // stack filled: scan every leaf instead
+        __syncthreads();
+        if (lane == 0) stk[0] = t.root;
+        __syncthreads();
+        int sp = 1;
+        while (sp > 0) {
+            const int k = sp < 64 ? sp : 64;
+            const int node = lane < k ? stk[sp - 1 - lane] : -1;
+            sp -= k;
+            const double lim = fmin(bound0, wave_min(m3));
+            bool leaf = false, expand = false;
+            if (node >= 0 && box_d2(t.nbb[node], ax, ay) <= lim) {
+                leaf = t.child0[node] < 0;
+                expand = !leaf;
             }
-        } else if (sp < 62) {
-            const int c0 = t.child0[node], c1 = t.child1[node];
-            const double d0 = box_d2(t.nbb[c0], ax, ay), d1 = box_d2(t.nbb[c1], ax, ay);
-            if (d0 <= d1) {
-                stack[sp++] = c1;
-                stack[sp++] = c0;
-            } else {
-                stack[sp++] = c0;
-                stack[sp++] = c1;
+            const uint64_t em = __ballot(expand);
+            const int ne = __popcll(em);
+            if (sp + 2 * ne > kFbStack) {
+                full = true;
+                break;
+            }
+            __syncthreads();  // every pop is read before the pushes reuse the slots
+            if (expand) {
+                const int pos = sp + 2 * __popcll(em & ((1ull << lane) - 1));
+                stk[pos] = t.child1[node];
+                stk[pos + 1] = t.child0[node];
+            }
+            sp += 2 * ne;
+            if (leaf) near4_leaf(t, node, ax, ay, m0, m1, m2, m3);
+            __syncthreads();
+        }
+        if (full) {
+            m0 = m1 = m2 = m3 = 1e5;
+            for (int64_t b = lane; b < t.nleaf; b += 64) {
+                const double lim = fmin(bound0, m3);
+                if (box_d2(t.nbb[b], ax, ay) <= lim) near4_leaf(t, (int)b, ax, ay, m0, m1, m2, m3);
             }
         }
+        // the four smallest of the union: four rounds of a wave minimum, the
+        // lowest lane holding it dropping its smallest each time
+        double r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double w = wave_min(m0);
+            r[k] = w;
+            const uint64_t hm = __ballot(m0 == w);
+            if (lane == __builtin_ctzll(hm)) {
+                m0 = m1;
+                m1 = m2;
+                m2 = m3;
+                m3 = 1e5;
+            }
+        }
+        if (lane == 0) d[t.perm[ti]] = (sqrt(r[1]) + sqrt(r[2]) + sqrt(r[3])) / 3;
     }
-    d[i] = (sqrt(m1) + sqrt(m2) + sqrt(m3)) / 3;
 }
 
 // ---- diffusion --------------------------------------------------------------
@@ -1162,11 +1300,18 @@ void tree_release(vx_case *c) {
     t.sx.release(); t.sy.release(); t.sg.release(); t.ncx.release(); t.ncy.release(); t.nr.release();
     t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release(); t.rec.release();
     t.cbox.release(); t.fpairs.release();
-    t.gcs.release(); t.gcs2.release(); t.gflag.release(); t.gscan.release(); t.gfirst.release();
+    t.gcs.release(); t.gcs2.release(); t.gflag.release(); t.gscan.release(); t.gfirst.release(); t.pcount.release(); t.arr.release(); t.sizes.release();
     t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.fbr2.release(); t.hpart.release();
+    if (t.ev) (void)hipEventDestroy(t.ev);
     delete c->tree;
     c->tree = nullptr;
 }
+
+// Key sort: rocPRIM's onesweep radix sort (four 8-bit passes) at every size;
+// its default switches to a merge sort up to 2^20 items, several times slower
+// here.
+using KeySortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                 rocprim::default_config, 0>;
 
 static int tree_order(const vx_case *c) { return c->prm.order >= 32 ? 32 : 16; }
 
@@ -1177,23 +1322,26 @@ int tree_build(vx_case *c, int64_t s) {
     t.order = order;
     const int64_t nbs = (s + 255) / 256;
     if (t.keys.ensure(s, st) || t.keys_alt.ensure(s, st) || t.idx.ensure(s, st) || t.perm.ensure(s, st) ||
-        t.incl.ensure(s, st) || t.part.ensure(nbs + 1, st) || t.cnt.ensure(8, st))
+        t.incl.ensure(s, st) || t.part.ensure(nbs + 1, st) || t.pcount.ensure(nbs, st) || t.cnt.ensure(8, st))
         return 1;
     VX_CHECK(hipMemsetAsync(t.cnt.p, 0, sizeof(unsigned long long) * 8, st));
     const double4 dbox = make_double4(c->prm.x_min, c->prm.x_max, c->prm.y_min, c->prm.y_max);
     hipLaunchKernelGGL(k_incl_box, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, s, dbox, t.incl.p,
-                       t.part.p, t.cnt.p);
-    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, st, t.part.p, nbs, t.part.p + nbs);
+                       t.part.p, t.pcount.p);
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, st, t.part.p, t.pcount.p, nbs, t.part.p + nbs, t.cnt.p);
+    // the included count is read back while the keys are sorted
+    if (!t.ev) VX_CHECK(hipEventCreateWithFlags(&t.ev, hipEventDisableTiming));
+    VX_CHECK(hipMemcpyAsync(c->h_count, t.cnt.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    VX_CHECK(hipEventRecord(t.ev, st));
     hipLaunchKernelGGL(k_tree_keys, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, t.incl.p, s,
                        t.part.p + nbs, t.keys.p, t.idx.p);
     size_t bytes = 0;
-    VX_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p, (int)s,
-                                                0, 32, st));
+    VX_CHECK(rocprim::radix_sort_pairs<KeySortConfig>(nullptr, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p,
+                                                      (unsigned)s, 0, 32, st));
     if (t.tmp.ensure(bytes, st)) return 1;
-    VX_CHECK(hipcub::DeviceRadixSort::SortPairs(t.tmp.p, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p, (int)s, 0,
-                                                32, st));
-    VX_CHECK(hipMemcpyAsync(c->h_count, t.cnt.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    VX_CHECK(hipStreamSynchronize(st));
+    VX_CHECK(rocprim::radix_sort_pairs<KeySortConfig>(t.tmp.p, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p,
+                                                      (unsigned)s, 0, 32, st));
+    VX_CHECK(hipEventSynchronize(t.ev));
     const int64_t m = (int64_t)*c->h_count;
     t.m = m;
     t.nleaf = 0;
@@ -1207,8 +1355,6 @@ int tree_build(vx_case *c, int64_t s) {
         t.gflag.ensure(m, st) || t.gscan.ensure(m, st) || t.gfirst.ensure(m, st))
         return 1;
     const unsigned gm = (unsigned)((m + 255) / 256);
-    hipLaunchKernelGGL(k_tree_gather, dim3(gm), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, t.perm.p, m, t.sx.p, t.sy.p,
-                       t.sg.p);
     int64_t nb = 1, nu = 0;
     // Key prefix depth of a bucket: at most one quadtree level coarser than a
     // cell holding kLeaf particles at the mean density.  Keys normalise each
@@ -1227,9 +1373,6 @@ int tree_build(vx_case *c, int64_t s) {
         VX_CHECK(hipcub::DeviceScan::ExclusiveSum(t.tmp.p, b2, t.gflag.p, t.gscan.p, (int)m, st));
         hipLaunchKernelGGL(k_group_first, dim3(gm), dim3(256), 0, st, t.gflag.p, t.gscan.p, m, t.gfirst.p);
     }
-    int32_t hg[2];
-    VX_CHECK(hipMemcpyAsync(&hg[0], t.gscan.p + (m - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    VX_CHECK(hipMemcpyAsync(&hg[1], t.gflag.p + (m - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
     if (m > kLeaf) {
         VX_CHECK(hipMemsetAsync(t.bstart.p, 0, sizeof(int32_t) * m, st));
         hipLaunchKernelGGL(k_radix, dim3(gm), dim3(256), 0, st, t.keys_alt.p, m, t.rlo.p, t.rhi.p, t.rleft.p,
@@ -1242,18 +1385,20 @@ int tree_build(vx_case *c, int64_t s) {
         if (t.tmp.ensure(std::max(b1, b2), st)) return 1;
         VX_CHECK(hipcub::DeviceScan::ExclusiveSum(t.tmp.p, b1, t.bstart.p, t.bscan.p, (int)m, st));
         VX_CHECK(hipcub::DeviceScan::ExclusiveSum(t.tmp.p, b2, t.upper.p, t.uscan.p, (int)(m - 1), st));
-        int32_t h[4];
-        VX_CHECK(hipMemcpyAsync(&h[0], t.bscan.p + (m - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        VX_CHECK(hipMemcpyAsync(&h[1], t.bstart.p + (m - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        VX_CHECK(hipMemcpyAsync(&h[2], t.uscan.p + (m - 2), sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        VX_CHECK(hipMemcpyAsync(&h[3], t.upper.p + (m - 2), sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        VX_CHECK(hipStreamSynchronize(st));
-        nb = (int64_t)h[0] + h[1];
-        nu = (int64_t)h[2] + h[3];
-    } else {
-        VX_CHECK(hipStreamSynchronize(st));
     }
-    t.ngroup = (int64_t)hg[0] + hg[1];
+    // the group, bucket and upper-node counts in one read-back
+    if (t.sizes.ensure(4, st)) return 1;
+    hipLaunchKernelGGL(k_tree_counts, dim3(1), dim3(1), 0, st, t.gscan.p, t.gflag.p, t.bscan.p, t.bstart.p, t.uscan.p,
+                       t.upper.p, m, (int)(m > kLeaf), t.sizes.p);
+    VX_CHECK(hipMemcpyAsync(c->h_count, t.sizes.p, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    VX_CHECK(hipEventRecord(t.ev, st));
+    // the sorted particles are gathered while the counts come back
+    hipLaunchKernelGGL(k_tree_gather, dim3(gm), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, t.perm.p, m, t.sx.p, t.sy.p,
+                       t.sg.p);
+    VX_CHECK(hipEventSynchronize(t.ev));
+    t.ngroup = c->h_count[0];
+    nb = c->h_count[1];
+    nu = c->h_count[2];
     t.nleaf = nb;
     t.nnode = nb + nu;
     t.root = (int32_t)(nu > 0 ? nb : 0);
@@ -1266,7 +1411,8 @@ int tree_build(vx_case *c, int64_t s) {
     VX_CHECK(hipMemsetAsync(t.parent.p, 0xff, sizeof(int32_t) * nn, st));
     VX_CHECK(hipMemsetAsync(t.visits.p, 0, sizeof(int32_t) * nn, st));
     if (m > kLeaf) {
-        hipLaunchKernelGGL(k_bucket_ranges, dim3(gm), dim3(256), 0, st, t.bstart.p, t.bscan.p, m, nb, t.first.p,
+        hipLaunchKernelGGL(k_bucket_first, dim3(gm), dim3(256), 0, st, t.bstart.p, t.bscan.p, m, t.first.p);
+        hipLaunchKernelGGL(k_bucket_count, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, t.first.p, nb, m,
                            t.count.p);
         hipLaunchKernelGGL(k_upper_nodes, dim3(gm), dim3(256), 0, st, t.rlo.p, t.rhi.p, t.rleft.p, t.rright.p,
                            t.upper.p, t.uscan.p, t.bscan.p, m, nb, t.first.p, t.count.p, t.child0.p, t.child1.p,
@@ -1277,19 +1423,36 @@ int tree_build(vx_case *c, int64_t s) {
         VX_CHECK(hipMemcpyAsync(t.count.p, &one[1], sizeof(int32_t), hipMemcpyHostToDevice, st));
     }
     const unsigned lb = (unsigned)((nb + 3) / 4);
-    const unsigned gb = (unsigned)((nb + 255) / 256);
+    // cut nodes, in node order
+    if (t.arr.ensure((size_t)nn + 1, st)) return 1;
+    int32_t *ncut = t.arr.p, *cut = t.arr.p + 1;
+    {
+        const CutNode pred{t.first.p, t.count.p, t.bscan.p, t.parent.p, nb, m};
+        hipcub::CountingInputIterator<int32_t> it(0);
+        size_t bytes = 0;
+        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cut, ncut, (int)nn, pred, st));
+        if (t.tmp.ensure(bytes, st)) return 1;
+        VX_CHECK(hipcub::DeviceSelect::If(t.tmp.p, bytes, it, cut, ncut, (int)nn, pred, st));
+    }
+    const unsigned gc = (unsigned)std::min<int64_t>(nb, 2048);
     if (order == 32) {
         hipLaunchKernelGGL(k_bucket_build<32>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, nb, t.first.p,
                            t.count.p, t.fx.p, t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p,
                            t.child1.p, t.cd.p, t.cf.p);
-        hipLaunchKernelGGL(k_upper_build<32>, dim3(gb), dim3(256), 0, st, nb, t.parent.p, t.visits.p, t.child0.p,
-                           t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_local<32>, dim3(gc), dim3(kCut), 0, st, cut, ncut, nb, m, t.bscan.p, t.first.p,
+                           t.count.p, t.parent.p, t.visits.p, t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p,
+                           t.nbb.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_top<32>, dim3(1), dim3(256), 0, st, cut, ncut, t.parent.p, t.visits.p,
+                           t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
     } else {
         hipLaunchKernelGGL(k_bucket_build<16>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, nb, t.first.p,
                            t.count.p, t.fx.p, t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p,
                            t.child1.p, t.cd.p, t.cf.p);
-        hipLaunchKernelGGL(k_upper_build<16>, dim3(gb), dim3(256), 0, st, nb, t.parent.p, t.visits.p, t.child0.p,
-                           t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_local<16>, dim3(gc), dim3(kCut), 0, st, cut, ncut, nb, m, t.bscan.p, t.first.p,
+                           t.count.p, t.parent.p, t.visits.p, t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p,
+                           t.nbb.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_top<16>, dim3(1), dim3(256), 0, st, cut, ncut, t.parent.p, t.visits.p,
+                           t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
     }
     hipLaunchKernelGGL(k_pack_nodes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, nn, t.ncx.p, t.ncy.p,
                        t.nr.p, t.child0.p, t.child1.p, t.first.p, t.count.p, t.rec.p);
@@ -1333,8 +1496,7 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
                                c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
     }
     tk.stop();
-    hipLaunchKernelGGL(k_tree_nearest_fallback, dim3((unsigned)((t.m + 255) / 256)), dim3(256), 0, st, tv, t.fb.p,
-                       t.fbr2.p, nfb, c->d.p);
+    hipLaunchKernelGGL(k_tree_nearest_fallback, dim3(1024), dim3(64), 0, st, tv, t.fb.p, t.fbr2.p, nfb, c->d.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
