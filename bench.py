@@ -59,7 +59,7 @@ def make_case(prec, n, P=1000, dt=0.005, tree_min_n=TREE_MIN_N):
     return c, (x, y, g), prm
 
 
-def run_path(prec, n, steps, warmup, M=None, tree_min_n=TREE_MIN_N, calibrate=False):
+def run_path(prec, n, steps, warmup, M=None, tree_min_n=TREE_MIN_N, calibrate=False, prof_steps=3):
     c, (x, y, g), prm = make_case(prec, n, tree_min_n=tree_min_n)
     if M is None:
         c.build_matrix()
@@ -73,34 +73,46 @@ def run_path(prec, n, steps, warmup, M=None, tree_min_n=TREE_MIN_N, calibrate=Fa
         # 3 x 8 B and writes 3 x 4 B per vortex, all coalesced)
         log(f"[bench] calibration: k_pack_f32 over {n} vortices ({24 * n} B read, {12 * n} B written)")
         c.right_part(n, (1.0, 0.0))
-    c.enable_timing(True)
     vinf = (1.0, 0.0)
     first = None
     for _ in range(warmup):
         f = c.time_step(vinf)
         first = f if first is None else first
     c.sync()
-    t0 = c.timings()
     n_start = c.n
+    # the timed steps run uninstrumented: stage timing synchronises the host
+    # at every stage boundary
     t = time.perf_counter()
     forces = []
+    marks = [t]
     for _ in range(steps):
-        forces.append(c.time_step(vinf))
+        forces.append(c.time_step(vinf))  # ends with the step's one read-back
+        marks.append(time.perf_counter())
     c.sync()
     wall = time.perf_counter() - t
+    n_end = c.n
+    # stage breakdown and kernel durations (HIP events on the case stream)
+    # from further steps with timing on
+    c.enable_timing(True)
+    t0 = c.timings()
+    for _ in range(prof_steps):
+        c.time_step(vinf)
+    c.sync()
     t1 = c.timings()
+    c.enable_timing(False)
     tree = c.tree_stats() if tree_min_n else None
-    per = lambda k: (t1[k] - t0[k]) / max(1, steps)
+    per = lambda k: (t1[k] - t0[k]) / max(1, prof_steps)
     res = {
         "tree": tree, "bs_kernel_ms": per("bs_kernel_ms"), "tree_build_ms": per("tree_build_ms"),
         "diff_kernel_ms": per("diff_kernel_ms"),
         "wall_ms": per("wall_ms"),
-        "wall": wall, "n_start": n_start, "n_end": c.n,
+        "wall": wall, "n_start": n_start, "n_end": n_end, "prof_steps": prof_steps,
+        "step_wall_ms": [1e3 * (b - a) for a, b in zip(marks, marks[1:])],
         "bs_ms": (t1["bs_ms"] - t0["bs_ms"]) / max(1, t1["bs_launches"] - t0["bs_launches"]),
-        "diff_ms": (t1["diff_ms"] - t0["diff_ms"]) / max(1, steps),
-        "speed_ms": (t1["speed_ms"] - t0["speed_ms"]) / max(1, steps),
-        "creation_ms": (t1["creation_ms"] - t0["creation_ms"]) / max(1, steps),
-        "step_ms": (t1["step_ms"] - t0["step_ms"]) / max(1, steps),
+        "diff_ms": per("diff_ms"),
+        "speed_ms": per("speed_ms"),
+        "creation_ms": per("creation_ms"),
+        "step_ms": per("step_ms"),
         "first_forces": first, "forces": forces,
     }
     c.close()
@@ -265,6 +277,8 @@ def main():
         "delta_cd_rel_one_step": dcd,
         "stage_ms": {k: mixed[k] for k in ("creation_ms", "speed_ms", "tree_build_ms", "bs_kernel_ms", "diff_ms",
                                            "diff_kernel_ms", "wall_ms", "step_ms")},
+        "stage_steps": mixed["prof_steps"],  # stage_ms: further steps with stage timing on
+        "step_wall_ms": mixed["step_wall_ms"],
         # compute-bound FP32 work on the vector ALUs, whose peak equals the
         # dense FP32 matrix peak (157.3 TFLOP/s); "mfma" is the contract's
         # name for the compute roof

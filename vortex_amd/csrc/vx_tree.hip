@@ -86,12 +86,13 @@ struct TreeDev {
     const NodeRec *rec;
     const int32_t *gfirst;
     int64_t ngroup;
+    const int32_t *parent, *bscan, *bstart;
 };
 
 static TreeDev view(const TreeBufs &t) {
     return TreeDev{t.m,     t.nleaf, t.root,     t.perm.p,   t.sx.p,    t.sy.p,    t.sg.p, t.fx.p, t.fy.p, t.fg.p,
                    t.ncx.p, t.ncy.p, t.nr.p,     t.nbb.p,    t.child0.p, t.child1.p, t.first.p, t.count.p,
-                   t.cd.p,  t.cf.p,  t.rec.p, t.gfirst.p, t.ngroup};
+                   t.cd.p,  t.cf.p,  t.rec.p, t.gfirst.p, t.ngroup, t.parent.p, t.bscan.p, t.bstart.p};
 }
 
 // ---- wave helpers ----------------------------------------------------------
@@ -322,6 +323,7 @@ __global__ void k_upper_nodes(const int32_t *__restrict__ lo, const int32_t *__r
 }
 
 // One wave per bucket: box, centre, radius, bucket-relative FP32 copies, P2M.
+// ORDER is a multiple of 8.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_bucket_build(const double *__restrict__ sx, const double *__restrict__ sy,
                                                       const double *__restrict__ sg, int64_t nb,
@@ -332,6 +334,7 @@ __global__ __launch_bounds__(256) void k_bucket_build(const double *__restrict__
                                                       double *__restrict__ nr, double4 *__restrict__ nbb,
                                                       int32_t *__restrict__ child0, int32_t *__restrict__ child1,
                                                       double2 *__restrict__ cd, float2 *__restrict__ cf) {
+    __shared__ double2 tr_all[4][8][65];
     const int lane = threadIdx.x & 63;
     const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (b >= nb) return;
@@ -351,16 +354,39 @@ __global__ __launch_bounds__(256) void k_bucket_build(const double *__restrict__
     // idle lanes sit at the centre: their zero weight must not meet an
     // overflowing power ((0 - c)/R)^k
     const double zr = ok ? (xi - cx) / Rs : 0.0, zi = ok ? (yi - cy) / Rs : 0.0;
+    // b_k = sum_j g_j ((z_j - c)/R)^k, eight terms at a time: each lane writes
+    // its particle's terms to LDS (term-major, rows padded against bank
+    // conflicts), lane (k, e) sums term k over the e-th eighth of the
+    // particles, and three shuffles add the eighths.
+    double2(*tr)[65] = tr_all[threadIdx.x >> 6];
+    const int kk = lane & 7, e8 = lane >> 3;
     double pr = 1.0, pi = 0.0;  // ((z - c)/R)^k
-    for (int k = 0; k < ORDER; ++k) {
-        const double br = wave_sum(gi * pr), bi = wave_sum(gi * pi);
-        if (lane == 0) {
-            cd[b * ORDER + k] = make_double2(br, bi);
-            cf[b * ORDER + k] = make_float2((float)br, (float)bi);
+    for (int k0 = 0; k0 < ORDER; k0 += 8) {
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            tr[k][lane] = make_double2(gi * pr, gi * pi);
+            const double npr = pr * zr - pi * zi;
+            pi = pr * zi + pi * zr;
+            pr = npr;
         }
-        const double npr = pr * zr - pi * zi;
-        pi = pr * zi + pi * zr;
-        pr = npr;
+        __builtin_amdgcn_wave_barrier();
+        double br = 0.0, bi = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const double2 q = tr[kk][8 * e8 + j];
+            br += q.x;
+            bi += q.y;
+        }
+#pragma unroll
+        for (int o = 8; o < 64; o <<= 1) {
+            br += __shfl_xor(br, o, 64);
+            bi += __shfl_xor(bi, o, 64);
+        }
+        if (e8 == 0) {
+            cd[b * ORDER + k0 + kk] = make_double2(br, bi);
+            cf[b * ORDER + k0 + kk] = make_float2((float)br, (float)bi);
+        }
     }
     if (lane == 0) {
         ncx[b] = cx;
@@ -571,6 +597,7 @@ struct PairMath;
 template <>
 struct PairMath<double> {
     __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+    __device__ static __forceinline__ double sqrt_bound(double a) { return sqrt(a); }
     __device__ static __forceinline__ double med3(double a, double b, double c) {
         return fmax(fmin(a, b), fmin(fmax(a, b), c));
     }
@@ -578,6 +605,8 @@ struct PairMath<double> {
 template <>
 struct PairMath<float> {
     __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    // the hardware square root (1 ulp), for bounds that carry a margin
+    __device__ static __forceinline__ float sqrt_bound(float a) { return __builtin_amdgcn_sqrtf(a); }
     __device__ static __forceinline__ float med3(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
 };
 
@@ -612,6 +641,28 @@ __device__ __forceinline__ void stage_leaf(Src4T<T> *lds, int lane, int64_t j0, 
     q.pad = T(0);
     lds[lane] = q;
     __builtin_amdgcn_wave_barrier();
+}
+
+// Pair loop over one staged leaf: Biot-Savart sums and, with TRACK, the four
+// smallest squared distances (the target itself included).
+template <typename T, bool TRACK>
+__device__ __forceinline__ void p2p_leaf(const Src4T<T> *lds, int cnt, T ox, T oy, T eps2, T &pu, T &pv, T &m0,
+                                         T &m1, T &m2, T &m3) {
+    using PM = PairMath<T>;
+    for (int jj = 0; jj < cnt; ++jj) {
+        const Src4T<T> q = lds[jj];
+        const T dx = ox - q.x, dy = oy - q.y;
+        const T r2 = dx * dx + dy * dy;
+        if constexpr (TRACK) {
+            m3 = PM::med3(m2, m3, r2);
+            m2 = PM::med3(m1, m2, r2);
+            m1 = PM::med3(m0, m1, r2);
+            m0 = fmin(m0, r2);
+        }
+        const T w = PM::div(q.g, fmax(r2, eps2));
+        pv += w * dx;
+        pu -= w * dy;
+    }
 }
 
 // ---- target groups ------------------------------------------------------------
@@ -665,6 +716,26 @@ __device__ __forceinline__ TargetGroup target_group(const TreeDev &t, int64_t gr
     return G;
 }
 
+// Per-wave counts (the values held by lane 0 of each wave) summed over the
+// work-group in LDS, then added once to the global counters: a counter that
+// every wave of a launch updates serialises at one L2 address, and the
+// launch does not end before the queue drains.  Called by every thread.
+__device__ __forceinline__ void block_count_add(unsigned long long a, unsigned long long b,
+                                                unsigned long long *ga, unsigned long long *gb) {
+    __shared__ unsigned long long s[2];
+    if (threadIdx.x == 0) s[0] = s[1] = 0ull;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        if (a) atomicAdd(&s[0], a);
+        if (b) atomicAdd(&s[1], b);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s[0]) atomicAdd(ga, s[0]);
+        if (gb && s[1]) atomicAdd(gb, s[1]);
+    }
+}
+
 // ---- Biot-Savart + nearest-3 --------------------------------------------------
 // Work-groups are dispatched to the 8 XCDs in turn (work-group b to XCD b % 8).
 // Each XCD gets a contiguous eighth of the Morton-ordered buckets instead, so
@@ -676,11 +747,11 @@ __device__ __forceinline__ int64_t xcd_block() {
 }
 
 template <typename T, int ORDER>
-__global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
-                                                 double *__restrict__ u, double *__restrict__ v,
-                                                 double *__restrict__ d, int32_t *__restrict__ fb,
-                                                 double *__restrict__ fbr2, unsigned long long *__restrict__ nfb,
-                                                 int exact_all) {
+__device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double eps, double vinf_x, double vinf_y,
+                                        double *__restrict__ u, double *__restrict__ v, double *__restrict__ d,
+                                        int32_t *__restrict__ fb, double *__restrict__ fbr2,
+                                        unsigned long long *__restrict__ nfb, int exact_all,
+                                        unsigned long long &cpairs, unsigned long long &cfar) {
     using PM = PairMath<T>;
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
@@ -710,10 +781,22 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     T fu = 0, fv = 0;  // far-field sums, folded into su, sv at each near bucket
     T m0 = T(1e5), m1 = T(1e5), m2 = T(1e5), m3 = T(1e5);
     T farmin = T(1e30);
+    // the own bucket first: its particles are usually the nearest, which
+    // tightens the bound that lets most other leaves skip the nearest-3 updates
+    {
+        stage_leaf<T>(lds, lane, -1 - L.a, nvalid, t);
+        T pu = 0, pv = 0;
+        p2p_leaf<T, true>(lds, nvalid, ax, ay, eps2, pu, pv, m0, m1, m2, m3);
+        __builtin_amdgcn_wave_barrier();
+        npairs += nvalid;
+        su += (double)pu;
+        sv += (double)pv;
+    }
     LaneStack st;
     st.push(lane, t.root);
     while (st.sp > 0) {
         const int node = st.pop();
+        if (node == (int)leaf) continue;
         const NodeRec nd = t.rec[node];
         const double ddx = nd.cx - cAx, ddy = nd.cy - cAy;
         const double d2 = ddx * ddx + ddy * ddy;
@@ -758,7 +841,7 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
             }
             fu += sr * qy + si * qx;
             fv += sr * qx - si * qy;
-            farmin = fmin(farmin, sqrt(den) - (T)nd.r);
+            farmin = fmin(farmin, PM::sqrt_bound(den) - (T)nd.r);
             ++nfar;
         } else if (nd.a < 0) {
             const int64_t j0 = -1 - nd.a;
@@ -774,18 +857,16 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
             stage_leaf<T>(lds, lane, j0, cnt, t);
             T pu = 0, pv = 0;
             npairs += cnt;
-            for (int jj = 0; jj < cnt; ++jj) {
-                const Src4T<T> q = lds[jj];
-                const T dx = ox - q.x, dy = oy - q.y;
-                const T r2 = dx * dx + dy * dy;
-                m3 = PM::med3(m2, m3, r2);
-                m2 = PM::med3(m1, m2, r2);
-                m1 = PM::med3(m0, m1, r2);
-                m0 = fmin(m0, r2);
-                const T w = PM::div(q.g, fmax(r2, eps2));
-                pv += w * dx;
-                pu -= w * dy;
-            }
+            // a pair at r2 >= m3 leaves the four smallest unchanged: skip the
+            // updates when the leaf's circle lies beyond sqrt(m3) of every
+            // target (a 1e-3 margin covers the FP32 distances)
+            const T cx_ = F32 ? ox : ax - (T)nd.cx, cy_ = F32 ? oy : ay - (T)nd.cy;
+            const T reach = (T)nd.r + PM::sqrt_bound(m3);
+            const bool near3 = ok && !(cx_ * cx_ + cy_ * cy_ > reach * reach * T(1.002));
+            if (__ballot(near3))
+                p2p_leaf<T, true>(lds, cnt, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
+            else
+                p2p_leaf<T, false>(lds, cnt, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
             __builtin_amdgcn_wave_barrier();
             su += (double)pu + (double)fu;
             sv += (double)pv + (double)fv;
@@ -797,11 +878,9 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     }
     su += (double)fu;
     sv += (double)fv;
-    if (lane == 0) {
-        atomicAdd(nfb + 1, (unsigned long long)(npairs * nvalid));
-        atomicAdd(nfb + 2, (unsigned long long)(nfar * nvalid));
-        if (st.overflow) atomicAdd(nfb + 4, 1ull);
-    }
+    cpairs = (unsigned long long)(npairs * nvalid);
+    cfar = (unsigned long long)(nfar * nvalid);
+    if (lane == 0 && st.overflow) atomicAdd(nfb + 4, 1ull);
     if (!ok) return;
     const int32_t i = t.perm[ti];
     u[i] = su / (2 * kPi) + vinf_x;
@@ -818,14 +897,28 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
     }
 }
 
-// Exact nearest-3 for the flagged targets, FP64, one wave per target.  The
-// four smallest squared distances (the target itself included) are searched
-// below a bound: the fast pass's third distance, an upper bound since its
-// three candidates are real particles, tightened by the smallest of the
-// lanes' own fourth distances.  Nodes are popped up to 64 at a time from a
-// stack in LDS; a lane that pops a leaf scans its particles into its private
-// four smallest, which merge over the wave at the end.  Should the stack fill,
-// the target restarts as a scan of every leaf, which needs no stack.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
+                                                 double *__restrict__ u, double *__restrict__ v,
+                                                 double *__restrict__ d, int32_t *__restrict__ fb,
+                                                 double *__restrict__ fbr2, unsigned long long *__restrict__ nfb,
+                                                 int exact_all) {
+    unsigned long long cp = 0, cf = 0;
+    bs_wave<T, ORDER>(t, theta, eps, vinf_x, vinf_y, u, v, d, fb, fbr2, nfb, exact_all, cp, cf);
+    block_count_add(cp, cf, nfb + 1, nfb + 2);
+}
+
+// Exact nearest-3 for the flagged targets, FP64, one wave per target: the
+// four smallest squared distances (the target itself included), searched
+// from the target's own bucket upward.  The bucket is scanned first; then,
+// for each ancestor, the subtree on the other side is searched below the
+// bound and the bound tightened to the exact fourth smallest found so far.
+// The bound starts at the fast pass's third distance (its three candidates
+// are real particles).  A subtree is walked with a stack in LDS, up to 64
+// nodes popped at a time; the popped buckets are scanned by the wave, one
+// particle per lane, into each lane's private four smallest, merged over the
+// wave for each bound and at the end.  Should the stack fill, the target
+// restarts as a scan of every bucket below the bound, which needs no stack.
 __device__ __forceinline__ double box_d2(const double4 &b, double ax, double ay) {
     const double ex = fmax(0.0, fmax(b.x - ax, ax - b.y));
     const double ey = fmax(0.0, fmax(b.z - ay, ay - b.w));
@@ -847,6 +940,95 @@ __device__ __forceinline__ void near4_leaf(const TreeDev &t, int node, double ax
     }
 }
 
+__device__ __forceinline__ void near4_insert(double r2, double &m0, double &m1, double &m2, double &m3) {
+    m3 = PairMath<double>::med3(m2, m3, r2);
+    m2 = PairMath<double>::med3(m1, m2, r2);
+    m1 = PairMath<double>::med3(m0, m1, r2);
+    m0 = fmin(m0, r2);
+}
+
+// The four smallest of the lanes' sorted lists: four rounds of a wave
+// minimum, the lowest lane holding it dropping its smallest each time.
+__device__ __forceinline__ void near4_merge(int lane, double m0, double m1, double m2, double m3, double r[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double w = wave_min(m0);
+        r[k] = w;
+        const uint64_t hm = __ballot(m0 == w);
+        if (lane == __builtin_ctzll(hm)) {
+            m0 = m1;
+            m1 = m2;
+            m2 = m3;
+            m3 = 1e5;
+        }
+    }
+}
+
+// The buckets in the wave's ballot `lm` (node ids in the lanes' `node`), four
+// at a time, one particle per lane (a bucket holds at most 64): loads first,
+// then the updates.
+__device__ __forceinline__ void near4_buckets(const TreeDev &t, uint64_t lm, int node, int lane, double ax,
+                                              double ay, double &m0, double &m1, double &m2, double &m3) {
+    while (lm) {
+        double px[4], py[4];
+        bool pk[4];
+#pragma unroll
+        for (int u4 = 0; u4 < 4; ++u4) {
+            pk[u4] = false;
+            px[u4] = py[u4] = 0.0;
+            if (lm) {
+                const int nl = __builtin_amdgcn_readlane(node, __builtin_ctzll(lm));
+                lm &= lm - 1;
+                const int64_t j0 = t.first[nl];
+                pk[u4] = lane < t.count[nl];
+                if (pk[u4]) {
+                    px[u4] = t.sx[j0 + lane];
+                    py[u4] = t.sy[j0 + lane];
+                }
+            }
+        }
+#pragma unroll
+        for (int u4 = 0; u4 < 4; ++u4) {
+            if (pk[u4]) {
+                const double dx = ax - px[u4], dy = ay - py[u4];
+                near4_insert(dx * dx + dy * dy, m0, m1, m2, m3);
+            }
+        }
+    }
+}
+
+// Walks the subtree at `top` below the bound lim; false if the stack filled.
+__device__ bool near4_subtree(const TreeDev &t, int32_t *stk, int top, int lane, double ax, double ay, double lim,
+                              double &m0, double &m1, double &m2, double &m3) {
+    __syncthreads();
+    if (lane == 0) stk[0] = top;
+    __syncthreads();
+    int sp = 1;
+    while (sp > 0) {
+        const int k = sp < 64 ? sp : 64;
+        const int node = lane < k ? stk[sp - 1 - lane] : -1;
+        sp -= k;
+        bool leaf = false, expand = false;
+        if (node >= 0 && box_d2(t.nbb[node], ax, ay) <= lim) {
+            leaf = t.child0[node] < 0;
+            expand = !leaf;
+        }
+        const uint64_t em = __ballot(expand);
+        const int ne = __popcll(em);
+        if (sp + 2 * ne > kFbStack) return false;
+        __syncthreads();  // every pop is read before the pushes reuse the slots
+        if (expand) {
+            const int pos = sp + 2 * __popcll(em & ((1ull << lane) - 1));
+            stk[pos] = t.child1[node];
+            stk[pos + 1] = t.child0[node];
+        }
+        sp += 2 * ne;
+        near4_buckets(t, __ballot(leaf), node, lane, ax, ay, m0, m1, m2, m3);
+        __syncthreads();
+    }
+    return true;
+}
+
 __global__ __launch_bounds__(64) void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ fb,
                                                               const double *__restrict__ fbr2,
                                                               const unsigned long long *__restrict__ nfb,
@@ -857,61 +1039,31 @@ __global__ __launch_bounds__(64) void k_tree_nearest_fallback(TreeDev t, const i
     for (int64_t q = blockIdx.x; q < nq; q += gridDim.x) {
         const int64_t ti = fb[q];
         const double ax = t.sx[ti], ay = t.sy[ti];
-        const double bound0 = fmin(fbr2[q] * (1.0 + 1e-3) + 1e-300, 1e5);
-        double m0 = 1e5, m1 = 1e5, m2 = 1e5, m3 = 1e5;
-        bool full = false;  // stack filled: scan every leaf instead
-        __syncthreads();
-        if (lane == 0) stk[0] = t.root;
-        __syncthreads();
-        int sp = 1;
-        while (sp > 0) {
-            const int k = sp < 64 ? sp : 64;
-            const int node = lane < k ? stk[sp - 1 - lane] : -1;
-            sp -= k;
-            const double lim = fmin(bound0, wave_min(m3));
-            bool leaf = false, expand = false;
-            if (node >= 0 && box_d2(t.nbb[node], ax, ay) <= lim) {
-                leaf = t.child0[node] < 0;
-                expand = !leaf;
-            }
-            const uint64_t em = __ballot(expand);
-            const int ne = __popcll(em);
-            if (sp + 2 * ne > kFbStack) {
+        double lim = fmin(fbr2[q] * (1.0 + 1e-3) + 1e-300, 1e5);
+        double m0 = 1e5, m1 = 1e5, m2 = 1e5, m3 = 1e5, r[4];
+        // the target's own bucket: the one holding sorted position ti
+        int cur = t.nleaf == 1 ? 0 : t.bscan[ti] + t.bstart[ti] - 1;
+        near4_buckets(t, 1ull, lane == 0 ? cur : -1, lane, ax, ay, m0, m1, m2, m3);
+        near4_merge(lane, m0, m1, m2, m3, r);
+        lim = fmin(lim, r[3]);
+        bool full = false;
+        while (cur != t.root) {
+            const int p = t.parent[cur];
+            const int sib = t.child0[p] == cur ? t.child1[p] : t.child0[p];
+            if (!near4_subtree(t, stk, sib, lane, ax, ay, lim, m0, m1, m2, m3)) {
                 full = true;
                 break;
             }
-            __syncthreads();  // every pop is read before the pushes reuse the slots
-            if (expand) {
-                const int pos = sp + 2 * __popcll(em & ((1ull << lane) - 1));
-                stk[pos] = t.child1[node];
-                stk[pos + 1] = t.child0[node];
-            }
-            sp += 2 * ne;
-            if (leaf) near4_leaf(t, node, ax, ay, m0, m1, m2, m3);
-            __syncthreads();
+            near4_merge(lane, m0, m1, m2, m3, r);
+            lim = fmin(lim, r[3]);
+            cur = p;
         }
-        if (full) {
+        if (full) {  // stack filled: every bucket below the bound
             m0 = m1 = m2 = m3 = 1e5;
-            for (int64_t b = lane; b < t.nleaf; b += 64) {
-                const double lim = fmin(bound0, m3);
+            for (int64_t b = lane; b < t.nleaf; b += 64)
                 if (box_d2(t.nbb[b], ax, ay) <= lim) near4_leaf(t, (int)b, ax, ay, m0, m1, m2, m3);
-            }
         }
-        // the four smallest of the union: four rounds of a wave minimum, the
-        // lowest lane holding it dropping its smallest each time
-        double r[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double w = wave_min(m0);
-            r[k] = w;
-            const uint64_t hm = __ballot(m0 == w);
-            if (lane == __builtin_ctzll(hm)) {
-                m0 = m1;
-                m1 = m2;
-                m2 = m3;
-                m3 = 1e5;
-            }
-        }
+        near4_merge(lane, m0, m1, m2, m3, r);
         if (lane == 0) d[t.perm[ti]] = (sqrt(r[1]) + sqrt(r[2]) + sqrt(r[3])) / 3;
     }
 }
@@ -925,10 +1077,11 @@ template <typename T>
 struct DiffPair;
 template <>
 struct DiffPair<float> {
+    template <bool GUARD>
     __device__ static __forceinline__ void add(float dx, float dy, float g, float kd, float thr2, float &s2x,
                                                float &s2y, float &s1) {
         const float r2 = dx * dx + dy * dy;
-        const float r2c = r2 > thr2 ? r2 : 1e30f;
+        const float r2c = (!GUARD || r2 > thr2) ? r2 : 1e30f;
         const float rs = __builtin_amdgcn_rsqf(r2c);
         const float e = __builtin_amdgcn_exp2f(r2c * rs * kd);
         const float ge = g * e;
@@ -940,10 +1093,11 @@ struct DiffPair<float> {
 };
 template <>
 struct DiffPair<double> {
+    template <bool GUARD>
     __device__ static __forceinline__ void add(double dx, double dy, double g, double kd, double thr,
                                                double &s2x, double &s2y, double &s1) {
         const double dist = sqrt(dx * dx + dy * dy);
-        if (dist > thr) {
+        if (!GUARD || dist > thr) {
             const double w = g / dist;
             const double e = exp(-dist * kd);
             s2x += w * dx * e;
@@ -976,11 +1130,10 @@ __device__ __forceinline__ int64_t far_count(const unsigned long long *cnt) {
 // k_diff_far instead and does not widen the wave's radius.  cnt[0] +=
 // evaluated pairs, cnt[1] stack overflows, cnt[2] the list length.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
-                                                        const double *__restrict__ dd,
-                                                        double *__restrict__ u, double *__restrict__ v,
-                                                        unsigned long long *__restrict__ cnt,
-                                                        int32_t *__restrict__ lists) {
+__device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, double nu, double thr,
+                                               const double *__restrict__ dd, double *__restrict__ u,
+                                               double *__restrict__ v, unsigned long long *__restrict__ cnt,
+                                               int32_t *__restrict__ lists, unsigned long long &cpairs) {
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
@@ -1019,6 +1172,7 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
         ri2 = mine ? cut * di * cut * di : -1.0;
     }
     const T thr2 = F32 ? (T)(thr * thr) : (T)thr;
+    const T gap2 = (T)(4.0 * thr * thr);
     int64_t npairs = 0;
     double a2x = 0.0, a2y = 0.0, a1 = 0.0;
     LaneStack st;
@@ -1055,9 +1209,18 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
             stage_leaf<T>(lds, lane, j0, cnt_j, t);
             T s2x = 0, s2y = 0, s1 = 0;
             npairs += cnt_j;
-            for (int jj = 0; jj < cnt_j; ++jj) {
-                const Src4T<T> q = lds[jj];
-                DiffPair<T>::add(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
+            // the threshold test is needed only where a pair can come within
+            // it: a leaf box nearer than twice the threshold to some target
+            if (__ballot(mine && !(lx * lx + ly * ly > gap2))) {
+                for (int jj = 0; jj < cnt_j; ++jj) {
+                    const Src4T<T> q = lds[jj];
+                    DiffPair<T>::template add<true>(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
+                }
+            } else {
+                for (int jj = 0; jj < cnt_j; ++jj) {
+                    const Src4T<T> q = lds[jj];
+                    DiffPair<T>::template add<false>(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
+                }
             }
             __builtin_amdgcn_wave_barrier();
             a2x += (double)s2x;
@@ -1069,10 +1232,8 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
         }
     }
     const int nmine = __popcll(__ballot(mine));
-    if (lane == 0) {
-        atomicAdd(cnt, (unsigned long long)(npairs * nmine));
-        if (st.overflow) atomicAdd(cnt + 1, 1ull);
-    }
+    cpairs = (unsigned long long)(npairs * nmine);
+    if (lane == 0 && st.overflow) atomicAdd(cnt + 1, 1ull);
     if (!mine) return;
     if (fabs(a1) > 1e-5) {
         u[i] += nu * a2x / (a1 * di);
@@ -1080,13 +1241,24 @@ __global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, d
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
+                                                        const double *__restrict__ dd,
+                                                        double *__restrict__ u, double *__restrict__ v,
+                                                        unsigned long long *__restrict__ cnt,
+                                                        int32_t *__restrict__ lists) {
+    unsigned long long cp = 0;
+    diffusion_wave<T>(t, cut, nu, thr, dd, u, v, cnt, lists, cp);
+    block_count_add(cp, 0ull, cnt, nullptr);
+}
+
 // Targets whose radius exceeds far_radius(), listed by k_tree_diffusion.  The
 // buckets are cut into chunks of 64 consecutive ones (Morton order) with a
 // box each; k_far_pairs lists the (target, chunk) pairs whose boxes meet, and
 // one wave per pair tests the chunk's 64 bucket boxes at once, then sums the
-// buckets in range with one source per lane, loading the next bucket while
-// summing the current one.  Partial sums go to part[(chunk * 3 + q) * H + k]
-// (zero for pairs not listed).
+// buckets in range with one source per lane, eight buckets' loads in flight.
+// Partial sums go to part[(chunk * 3 + q) * H + k] (zero for pairs not
+// listed).
 template <typename T>
 __device__ __forceinline__ void far_load(const TreeDev &t, int b, int lane, double ax, double ay, T &dx, T &dy,
                                          T &g) {
@@ -1169,7 +1341,7 @@ __global__ void k_far_zero(double *__restrict__ a, int64_t nchunk, const unsigne
 template <typename T>
 __device__ void diff_far_pair(const TreeDev &t, const int32_t *__restrict__ list, int64_t H, int64_t k, int64_t ch,
                               double cut, double thr, const double *__restrict__ dd, double *__restrict__ part,
-                              unsigned long long *__restrict__ cnt) {
+                              unsigned long long &cpairs) {
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x & 63;
     const int64_t ti = list[k];
@@ -1183,37 +1355,40 @@ __device__ void diff_far_pair(const TreeDev &t, const int32_t *__restrict__ list
     uint64_t mask = __ballot(need);
     double a2x = 0.0, a2y = 0.0, a1 = 0.0;
     int64_t npairs = 0;
-    if (mask) {
-        int bc = (int)(ch * 64) + __builtin_ctzll(mask);
-        mask &= mask - 1;
-        T dx, dy, g;
-        far_load<T>(t, bc, lane, ax, ay, dx, dy, g);
-        npairs += t.rec[bc].b;
-        for (;;) {
-            const bool more = mask != 0;
-            T nx = T(1), ny = T(1), ng = T(0);
-            if (more) {
+    // the buckets in range, eight at a time: all their loads first, then the
+    // sums in bucket order
+    constexpr int kB = 8;
+    while (mask) {
+        T dx[kB], dy[kB], g[kB];
+        int nbk = 0;
+#pragma unroll
+        for (int q = 0; q < kB; ++q) {
+            dx[q] = dy[q] = T(1);
+            g[q] = T(0);
+            if (mask) {
                 const int bn = (int)(ch * 64) + __builtin_ctzll(mask);
                 mask &= mask - 1;
-                far_load<T>(t, bn, lane, ax, ay, nx, ny, ng);
+                far_load<T>(t, bn, lane, ax, ay, dx[q], dy[q], g[q]);
                 npairs += t.rec[bn].b;
+                ++nbk;
             }
-            T s2x = 0, s2y = 0, s1 = 0;
-            DiffPair<T>::add(dx, dy, g, kd, thr2, s2x, s2y, s1);
-            a2x += (double)s2x;
-            a2y += (double)s2y;
-            a1 += (double)s1;
-            if (!more) break;
-            dx = nx;
-            dy = ny;
-            g = ng;
+        }
+#pragma unroll
+        for (int q = 0; q < kB; ++q) {
+            if (q < nbk) {
+                T s2x = 0, s2y = 0, s1 = 0;
+                DiffPair<T>::template add<true>(dx[q], dy[q], g[q], kd, thr2, s2x, s2y, s1);
+                a2x += (double)s2x;
+                a2y += (double)s2y;
+                a1 += (double)s1;
+            }
         }
     }
     a2x = wave_sum(a2x);
     a2y = wave_sum(a2y);
     a1 = wave_sum(a1);
+    cpairs += (unsigned long long)npairs;
     if (lane != 0) return;
-    if (npairs) atomicAdd(cnt, (unsigned long long)npairs);
     part[(ch * 3 + 0) * H + k] = a2x;
     part[(ch * 3 + 1) * H + k] = a2y;
     part[(ch * 3 + 2) * H + k] = a1;
@@ -1229,8 +1404,10 @@ __global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const int32_t *__re
     const int64_t H = far_count(cnt);
     const int64_t total = (int64_t)*npair_list;
     const int64_t nw = (int64_t)gridDim.x * 4;
+    unsigned long long cp = 0;
     for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < total; w += nw)
-        diff_far_pair<T>(t, list, H, pairs[w].x, pairs[w].y, cut, thr, dd, part, cnt);
+        diff_far_pair<T>(t, list, H, pairs[w].x, pairs[w].y, cut, thr, dd, part, cp);
+    block_count_add(cp, 0ull, cnt, nullptr);
 }
 
 // One work-group per listed target: the chunk partials summed in a fixed
