@@ -159,6 +159,9 @@ typedef struct vx_tree_info {
     int64_t diff_pairs;              /* pairwise evaluations, last diffusion          */
     int32_t order;
     int32_t heavy;                   /* diffusion targets with a domain-scale radius  */
+    int64_t m2l;                     /* far nodes translated into bucket local
+                                        expansions (FP32 path; the FP64 path
+                                        evaluates m2p_evals directly), last BS */
 } vx_tree_info;
 int vx_tree_stats(vx_case *c, vx_tree_info *info);
 int vx_step_move(vx_case *c, double out4[4]);

@@ -47,7 +47,8 @@ class Params(C.Structure):
 
 class TreeInfo(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("m", "leaves", "nodes", "fallback", "bs_pairs", "m2p_evals",
-                                          "diff_pairs")] + [("order", C.c_int32), ("heavy", C.c_int32)]
+                                          "diff_pairs")] + [("order", C.c_int32), ("heavy", C.c_int32),
+                                                            ("m2l", C.c_int64)]
 
 
 class Timings(C.Structure):

@@ -736,6 +736,84 @@ __device__ __forceinline__ void block_count_add(unsigned long long a, unsigned l
     }
 }
 
+// Binomial coefficient C(n, k); folded to a constant wherever n and k are.
+__host__ __device__ constexpr double binom(int n, int k) {
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (double)(n - k + i) / (double)i;
+    return r;
+}
+
+__device__ __forceinline__ float wave_sum_f(float a) {
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    return a;
+}
+
+__device__ __forceinline__ float readlane_f(float a, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), l));
+}
+
+// M2L (FP32): far source nodes translated into the target bucket's local
+// expansion about its centre c_T (radius R_T), one node per lane:
+//   F(z) = sum_j g_j / (z - z_j) = sum_l L_l u^l,  u = (z - c_T) / R_T,
+//   L_l += q (-beta)^l sum_k C(k + l, l) alpha^k b_k,
+//   q = 1 / (c_T - c_S),  alpha = R_S q,  beta = R_T q,
+// with b_k the node's multipole (about c_S, scaled by R_S).  It follows from
+// 1/(D + w)^(k+1) = sum_l C(k + l, l) (-w)^l / D^(k+l+1), D = c_T - c_S,
+// w = z - c_T.  The opening test keeps (R_S + R_T) / |D| below theta, so the
+// truncation at ORDER terms on both sides leaves about theta^ORDER of the
+// node's field.  Lane l < ORDER accumulates L_l over the batches; nodes
+// fq[0, count) of this wave's queue.
+template <int ORDER>
+__device__ __forceinline__ void m2l_batch(const TreeDev &t, const int32_t *fq, int count, int lane, double cTx,
+                                          double cTy, float rT, float &Lr, float &Li) {
+    const int node = lane < count ? fq[lane] : -1;
+    float akr[ORDER], aki[ORDER];  // alpha^k b_k
+    float qbr = 0.f, qbi = 0.f;    // q (-beta)^l
+    float mbr = 0.f, mbi = 0.f;    // -beta
+#pragma unroll
+    for (int k = 0; k < ORDER; ++k) akr[k] = aki[k] = 0.f;
+    if (node >= 0) {
+        const NodeRec S = t.rec[node];
+        const float dx = (float)(cTx - S.cx), dy = (float)(cTy - S.cy);
+        const float inv = __builtin_amdgcn_rcpf(dx * dx + dy * dy);
+        qbr = dx * inv;
+        qbi = -dy * inv;
+        const float rs = fmaxf((float)S.r, 1e-30f);
+        const float alr = rs * qbr, ali = rs * qbi;
+        mbr = -rT * qbr;
+        mbi = -rT * qbi;
+        const float2 *b = t.cf + (int64_t)node * ORDER;
+        float pr = 1.f, pi = 0.f;
+#pragma unroll
+        for (int k = 0; k < ORDER; ++k) {
+            const float2 bk = b[k];
+            akr[k] = pr * bk.x - pi * bk.y;
+            aki[k] = pr * bk.y + pi * bk.x;
+            const float npr = pr * alr - pi * ali;
+            pi = pr * ali + pi * alr;
+            pr = npr;
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < ORDER; ++l) {
+        float sr = 0.f, si = 0.f;
+#pragma unroll
+        for (int k = 0; k < ORDER; ++k) {
+            const float c = (float)binom(k + l, l);
+            sr += c * akr[k];
+            si += c * aki[k];
+        }
+        const float wr = wave_sum_f(qbr * sr - qbi * si), wi = wave_sum_f(qbr * si + qbi * sr);
+        if (lane == l) {
+            Lr += wr;
+            Li += wi;
+        }
+        const float nqr = qbr * mbr - qbi * mbi;
+        qbi = qbr * mbi + qbi * mbr;
+        qbr = nqr;
+    }
+}
+
 // ---- Biot-Savart + nearest-3 --------------------------------------------------
 // Work-groups are dispatched to the 8 XCDs in turn (work-group b to XCD b % 8).
 // Each XCD gets a contiguous eighth of the Morton-ordered buckets instead, so
@@ -751,12 +829,15 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
                                         double *__restrict__ u, double *__restrict__ v, double *__restrict__ d,
                                         int32_t *__restrict__ fb, double *__restrict__ fbr2,
                                         unsigned long long *__restrict__ nfb, int exact_all,
-                                        unsigned long long &cpairs, unsigned long long &cfar) {
+                                        unsigned long long &cpairs, unsigned long long &cfar,
+                                        unsigned long long &cm2l) {
     using PM = PairMath<T>;
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
+    __shared__ int32_t fq_all[4][64];  // FP32: far nodes waiting for M2L
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
+    int32_t *fq = fq_all[threadIdx.x >> 6];
     // targets: one bucket per wave (a compact node, which keeps the MAC's
     // opening radius small)
     const int64_t leaf = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
@@ -780,7 +861,11 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     double su = 0.0, sv = 0.0;
     T fu = 0, fv = 0;  // far-field sums, folded into su, sv at each near bucket
     T m0 = T(1e5), m1 = T(1e5), m2 = T(1e5), m3 = T(1e5);
-    T farmin = T(1e30);
+    T farmin = T(1e30);         // FP64: per target, nearest point of a far node
+    double farmin_w = 1e30;     // FP32: per bucket, |c_T - c_S| - R_S - R_T
+    int nq = 0;                 // FP32: far nodes queued
+    float Lr = 0.f, Li = 0.f;   // FP32: lane l < ORDER holds local coefficient l
+    const float rT = fmaxf((float)rA, 1e-30f);  // FP32: kTiny would round to zero
     // the own bucket first: its particles are usually the nearest, which
     // tightens the bound that lets most other leaves skip the nearest-3 updates
     {
@@ -803,46 +888,39 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
         const double rs = nd.r + rA;
         // dist * theta > r_node + r_leaf and dist - r_node - r_leaf > eps
         if (d2 * th2 > rs * rs && d2 > (rs + eps) * (rs + eps)) {
-            // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
-            T Dx, Dy;
-            if constexpr (F32) {
-                Dx = ax - (float)ddx;
-                Dy = ay - (float)ddy;
-            } else {
-                Dx = ax - nd.cx;
-                Dy = ay - nd.cy;
-            }
-            const T den = Dx * Dx + Dy * Dy;
-            const T inv = PM::div(T(1), den);
-            const T qx = Dx * inv, qy = -Dy * inv;
-            const T R = (T)fmax(nd.r, kTiny);
-            const T tx = R * qx, ty = R * qy;
-            T sr, si;
-            if constexpr (F32) {
-                const float2 *b = t.cf + (int64_t)node * ORDER;
-                sr = b[ORDER - 1].x;
-                si = b[ORDER - 1].y;
-#pragma unroll
-                for (int k = ORDER - 2; k >= 0; --k) {
-                    const T nr_ = sr * tx - si * ty + b[k].x;
-                    si = sr * ty + si * tx + b[k].y;
-                    sr = nr_;
-                }
-            } else {
-                const double2 *b = t.cd + (int64_t)node * ORDER;
-                sr = b[ORDER - 1].x;
-                si = b[ORDER - 1].y;
-#pragma unroll
-                for (int k = ORDER - 2; k >= 0; --k) {
-                    const T nr_ = sr * tx - si * ty + b[k].x;
-                    si = sr * ty + si * tx + b[k].y;
-                    sr = nr_;
-                }
-            }
-            fu += sr * qy + si * qx;
-            fv += sr * qx - si * qy;
-            farmin = fmin(farmin, PM::sqrt_bound(den) - (T)nd.r);
             ++nfar;
+            if constexpr (F32) {
+                // queued for M2L into the bucket's local expansion, 64 at a time
+                farmin_w = fmin(farmin_w, sqrt(d2) - rs);
+                if (lane == 0) fq[nq] = node;
+                if (++nq == 64) {
+                    __builtin_amdgcn_wave_barrier();
+                    m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, Lr, Li);
+                    __builtin_amdgcn_wave_barrier();
+                    cm2l += (unsigned long long)nq;
+                    nq = 0;
+                }
+                continue;
+            } else {
+                // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
+                const T Dx = ax - nd.cx, Dy = ay - nd.cy;
+                const T den = Dx * Dx + Dy * Dy;
+                const T inv = PM::div(T(1), den);
+                const T qx = Dx * inv, qy = -Dy * inv;
+                const T R = (T)fmax(nd.r, kTiny);
+                const T tx = R * qx, ty = R * qy;
+                const double2 *b = t.cd + (int64_t)node * ORDER;
+                T sr = b[ORDER - 1].x, si = b[ORDER - 1].y;
+#pragma unroll
+                for (int k = ORDER - 2; k >= 0; --k) {
+                    const T nr_ = sr * tx - si * ty + b[k].x;
+                    si = sr * ty + si * tx + b[k].y;
+                    sr = nr_;
+                }
+                fu += sr * qy + si * qx;
+                fv += sr * qx - si * qy;
+                farmin = fmin(farmin, PM::sqrt_bound(den) - (T)nd.r);
+            }
         } else if (nd.a < 0) {
             const int64_t j0 = -1 - nd.a;
             const int cnt = nd.b;
@@ -878,6 +956,25 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     }
     su += (double)fu;
     sv += (double)fv;
+    if constexpr (F32) {
+        if (nq > 0) {
+            __builtin_amdgcn_wave_barrier();
+            m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, Lr, Li);
+            cm2l += (unsigned long long)nq;
+        }
+        // L2P: F = sum_l L_l u^l at u = (z - c_T) / R_T (ax, ay are relative
+        // to the bucket centre); u += Im F, v += Re F
+        const float ur = ax / rT, ui = ay / rT;
+        float Fr = readlane_f(Lr, ORDER - 1), Fi = readlane_f(Li, ORDER - 1);
+#pragma unroll
+        for (int l = ORDER - 2; l >= 0; --l) {
+            const float nr_ = Fr * ur - Fi * ui + readlane_f(Lr, l);
+            Fi = Fr * ui + Fi * ur + readlane_f(Li, l);
+            Fr = nr_;
+        }
+        su += (double)Fi;
+        sv += (double)Fr;
+    }
     cpairs = (unsigned long long)(npairs * nvalid);
     cfar = (unsigned long long)(nfar * nvalid);
     if (lane == 0 && st.overflow) atomicAdd(nfb + 4, 1ull);
@@ -889,7 +986,7 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     // exact only if no far node can hold a closer particle than the third
     // FP32 squared distances of close pairs carry up to ~1e-5 relative error,
     // hence the 1e-3 margins here and in the search bound
-    const double fm = (double)farmin;
+    const double fm = F32 ? farmin_w : (double)farmin;
     if (exact_all || (double)m3 >= fm * fm * (1.0 - 1e-3)) {
         const unsigned long long k = atomicAdd(nfb, 1ull);
         fb[k] = (int32_t)ti;
@@ -903,9 +1000,10 @@ __global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double
                                                  double *__restrict__ d, int32_t *__restrict__ fb,
                                                  double *__restrict__ fbr2, unsigned long long *__restrict__ nfb,
                                                  int exact_all) {
-    unsigned long long cp = 0, cf = 0;
-    bs_wave<T, ORDER>(t, theta, eps, vinf_x, vinf_y, u, v, d, fb, fbr2, nfb, exact_all, cp, cf);
+    unsigned long long cp = 0, cf = 0, cl = 0;
+    bs_wave<T, ORDER>(t, theta, eps, vinf_x, vinf_y, u, v, d, fb, fbr2, nfb, exact_all, cp, cf, cl);
     block_count_add(cp, cf, nfb + 1, nfb + 2);
+    block_count_add(cl, 0ull, nfb + 7, nullptr);
 }
 
 // Exact nearest-3 for the flagged targets, FP64, one wave per target: the
@@ -1499,9 +1597,9 @@ int tree_build(vx_case *c, int64_t s) {
     t.order = order;
     const int64_t nbs = (s + 255) / 256;
     if (t.keys.ensure(s, st) || t.keys_alt.ensure(s, st) || t.idx.ensure(s, st) || t.perm.ensure(s, st) ||
-        t.incl.ensure(s, st) || t.part.ensure(nbs + 1, st) || t.pcount.ensure(nbs, st) || t.cnt.ensure(8, st))
+        t.incl.ensure(s, st) || t.part.ensure(nbs + 1, st) || t.pcount.ensure(nbs, st) || t.cnt.ensure(16, st))
         return 1;
-    VX_CHECK(hipMemsetAsync(t.cnt.p, 0, sizeof(unsigned long long) * 8, st));
+    VX_CHECK(hipMemsetAsync(t.cnt.p, 0, sizeof(unsigned long long) * 16, st));
     const double4 dbox = make_double4(c->prm.x_min, c->prm.x_max, c->prm.y_min, c->prm.y_max);
     hipLaunchKernelGGL(k_incl_box, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, s, dbox, t.incl.p,
                        t.part.p, t.pcount.p);
@@ -1727,7 +1825,7 @@ int launch_tree_diffusion(vx_case *c) {
 int tree_stats(vx_case *c, vx_tree_info *info) {
     *info = vx_tree_info{};
     if (!c->tree) return 0;
-    unsigned long long h[8];
+    unsigned long long h[16];
     VX_CHECK(hipMemcpyAsync(h, c->tree->cnt.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     VX_CHECK(hipStreamSynchronize(c->stream));
     info->m = c->tree->m;
@@ -1743,6 +1841,7 @@ int tree_stats(vx_case *c, vx_tree_info *info) {
     }
     info->order = c->tree->order;
     info->heavy = (int32_t)(h[6] < (unsigned long long)kFarCap ? h[6] : (unsigned long long)kFarCap);
+    info->m2l = (int64_t)h[8];
     return 0;
 }
 
