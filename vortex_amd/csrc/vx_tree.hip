@@ -761,11 +761,14 @@ __device__ __forceinline__ float readlane_f(float a, int l) {
 // 1/(D + w)^(k+1) = sum_l C(k + l, l) (-w)^l / D^(k+l+1), D = c_T - c_S,
 // w = z - c_T.  The opening test keeps (R_S + R_T) / |D| below theta, so the
 // truncation at ORDER terms on both sides leaves about theta^ORDER of the
-// node's field.  Lane l < ORDER accumulates L_l over the batches; nodes
-// fq[0, count) of this wave's queue.
+// node's field.  The nodes are fq[0, count) of this wave's queue.  The
+// lanes' terms are summed over the wave through LDS four coefficients at a
+// time (tr: this wave's 4 x 64 slots): lane (c, e), c = lane & 3, adds the
+// terms of lanes 4e .. 4e + 3 for coefficient 4 j + c into acc[j]; the
+// partial sums over e are combined once, by l2p_coeffs().
 template <int ORDER>
 __device__ __forceinline__ void m2l_batch(const TreeDev &t, const int32_t *fq, int count, int lane, double cTx,
-                                          double cTy, float rT, float &Lr, float &Li) {
+                                          double cTy, float rT, float2 (*tr)[64], float2 (&acc)[ORDER / 4]) {
     const int node = lane < count ? fq[lane] : -1;
     float akr[ORDER], aki[ORDER];  // alpha^k b_k
     float qbr = 0.f, qbi = 0.f;    // q (-beta)^l
@@ -794,6 +797,7 @@ __device__ __forceinline__ void m2l_batch(const TreeDev &t, const int32_t *fq, i
             pr = npr;
         }
     }
+    const int c4 = lane & 3, e4 = lane >> 2;
 #pragma unroll
     for (int l = 0; l < ORDER; ++l) {
         float sr = 0.f, si = 0.f;
@@ -803,14 +807,36 @@ __device__ __forceinline__ void m2l_batch(const TreeDev &t, const int32_t *fq, i
             sr += c * akr[k];
             si += c * aki[k];
         }
-        const float wr = wave_sum_f(qbr * sr - qbi * si), wi = wave_sum_f(qbr * si + qbi * sr);
-        if (lane == l) {
-            Lr += wr;
-            Li += wi;
-        }
+        tr[l & 3][lane] = make_float2(qbr * sr - qbi * si, qbr * si + qbi * sr);
         const float nqr = qbr * mbr - qbi * mbi;
         qbi = qbr * mbi + qbi * mbr;
         qbr = nqr;
+        if ((l & 3) == 3) {
+            __builtin_amdgcn_wave_barrier();
+            float2 a = acc[l >> 2];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float2 q = tr[c4][4 * e4 + j];
+                a.x += q.x;
+                a.y += q.y;
+            }
+            acc[l >> 2] = a;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+// The local coefficients: acc[j] summed over the lanes sharing lane & 3, so
+// that every lane with lane & 3 == c holds L_{4 j + c}.
+template <int ORDER>
+__device__ __forceinline__ void l2p_coeffs(float2 (&acc)[ORDER / 4]) {
+#pragma unroll
+    for (int j = 0; j < ORDER / 4; ++j) {
+#pragma unroll
+        for (int o = 4; o < 64; o <<= 1) {
+            acc[j].x += __shfl_xor(acc[j].x, o, 64);
+            acc[j].y += __shfl_xor(acc[j].y, o, 64);
+        }
     }
 }
 
@@ -834,10 +860,12 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     using PM = PairMath<T>;
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
-    __shared__ int32_t fq_all[4][64];  // FP32: far nodes waiting for M2L
+    __shared__ int32_t fq_all[4][64];     // FP32: far nodes waiting for M2L
+    __shared__ float2 tr_all[4][4][64];   // FP32: M2L sums over the wave
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
     int32_t *fq = fq_all[threadIdx.x >> 6];
+    float2(*tr)[64] = tr_all[threadIdx.x >> 6];
     // targets: one bucket per wave (a compact node, which keeps the MAC's
     // opening radius small)
     const int64_t leaf = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
@@ -864,7 +892,9 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     T farmin = T(1e30);         // FP64: per target, nearest point of a far node
     double farmin_w = 1e30;     // FP32: per bucket, |c_T - c_S| - R_S - R_T
     int nq = 0;                 // FP32: far nodes queued
-    float Lr = 0.f, Li = 0.f;   // FP32: lane l < ORDER holds local coefficient l
+    float2 lacc[ORDER / 4];     // FP32: local coefficients, partial (m2l_batch)
+#pragma unroll
+    for (int j = 0; j < ORDER / 4; ++j) lacc[j] = make_float2(0.f, 0.f);
     const float rT = fmaxf((float)rA, 1e-30f);  // FP32: kTiny would round to zero
     // the own bucket first: its particles are usually the nearest, which
     // tightens the bound that lets most other leaves skip the nearest-3 updates
@@ -895,7 +925,7 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
                 if (lane == 0) fq[nq] = node;
                 if (++nq == 64) {
                     __builtin_amdgcn_wave_barrier();
-                    m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, Lr, Li);
+                    m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, tr, lacc);
                     __builtin_amdgcn_wave_barrier();
                     cm2l += (unsigned long long)nq;
                     nq = 0;
@@ -959,17 +989,19 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     if constexpr (F32) {
         if (nq > 0) {
             __builtin_amdgcn_wave_barrier();
-            m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, Lr, Li);
+            m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, tr, lacc);
             cm2l += (unsigned long long)nq;
         }
         // L2P: F = sum_l L_l u^l at u = (z - c_T) / R_T (ax, ay are relative
         // to the bucket centre); u += Im F, v += Re F
+        l2p_coeffs<ORDER>(lacc);
         const float ur = ax / rT, ui = ay / rT;
-        float Fr = readlane_f(Lr, ORDER - 1), Fi = readlane_f(Li, ORDER - 1);
+        float Fr = readlane_f(lacc[(ORDER - 1) >> 2].x, (ORDER - 1) & 3);
+        float Fi = readlane_f(lacc[(ORDER - 1) >> 2].y, (ORDER - 1) & 3);
 #pragma unroll
         for (int l = ORDER - 2; l >= 0; --l) {
-            const float nr_ = Fr * ur - Fi * ui + readlane_f(Lr, l);
-            Fi = Fr * ui + Fi * ur + readlane_f(Li, l);
+            const float nr_ = Fr * ur - Fi * ui + readlane_f(lacc[l >> 2].x, l & 3);
+            Fi = Fr * ui + Fi * ur + readlane_f(lacc[l >> 2].y, l & 3);
             Fr = nr_;
         }
         su += (double)Fi;
@@ -995,7 +1027,8 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
 }
 
 template <typename T, int ORDER>
-__global__ __launch_bounds__(256) void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && ORDER == 16 ? 6 : 1)))
+void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
                                                  double *__restrict__ u, double *__restrict__ v,
                                                  double *__restrict__ d, int32_t *__restrict__ fb,
                                                  double *__restrict__ fbr2, unsigned long long *__restrict__ nfb,
@@ -1340,7 +1373,8 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
+void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
                                                         const double *__restrict__ dd,
                                                         double *__restrict__ u, double *__restrict__ v,
                                                         unsigned long long *__restrict__ cnt,
