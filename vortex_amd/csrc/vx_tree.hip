@@ -58,7 +58,7 @@ struct TreeBufs {
     DBuf<int64_t> sizes;                           // group, bucket and upper-node counts
     DBuf<double> sx, sy, sg, ncx, ncy, nr, fbr2, hpart;
     DBuf<float> fx, fy, fg;
-    DBuf<double4> nbb, part, cbox;
+    DBuf<double4> nbb, part, cbox, ftgt;
     DBuf<int2> fpairs;
     DBuf<double2> cd;
     DBuf<float2> cf;
@@ -1181,13 +1181,14 @@ __global__ __launch_bounds__(64) void k_tree_nearest_fallback(TreeDev t, const i
         while (cur != t.root) {
             const int p = t.parent[cur];
             const int sib = t.child0[p] == cur ? t.child1[p] : t.child0[p];
+            cur = p;
+            if (box_d2(t.nbb[sib], ax, ay) > lim) continue;  // nothing below the bound
             if (!near4_subtree(t, stk, sib, lane, ax, ay, lim, m0, m1, m2, m3)) {
                 full = true;
                 break;
             }
             near4_merge(lane, m0, m1, m2, m3, r);
             lim = fmin(lim, r[3]);
-            cur = p;
         }
         if (full) {  // stack filled: every bucket below the bound
             m0 = m1 = m2 = m3 = 1e5;
@@ -1392,15 +1393,12 @@ void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
 // Partial sums go to part[(chunk * 3 + q) * H + k] (zero for pairs not
 // listed).
 template <typename T>
-__device__ __forceinline__ void far_load(const TreeDev &t, int b, int lane, double ax, double ay, T &dx, T &dy,
-                                         T &g) {
-    const NodeRec nd = t.rec[b];
-    const int64_t j0 = -1 - nd.a;
-    const int cnt = nd.b;
+__device__ __forceinline__ void far_load(const TreeDev &t, int64_t j0, int cnt, double bcx, double bcy, int lane,
+                                         double ax, double ay, T &dx, T &dy, T &g) {
     if (lane < cnt) {
         if constexpr (sizeof(T) == 4) {
-            dx = (float)(ax - nd.cx) - t.fx[j0 + lane];
-            dy = (float)(ay - nd.cy) - t.fy[j0 + lane];
+            dx = (float)(ax - bcx) - t.fx[j0 + lane];
+            dy = (float)(ay - bcy) - t.fy[j0 + lane];
             g = t.fg[j0 + lane];
         } else {
             dx = ax - t.sx[j0 + lane];
@@ -1412,6 +1410,14 @@ __device__ __forceinline__ void far_load(const TreeDev &t, int b, int lane, doub
         dy = T(1);
         g = T(0);
     }
+}
+
+__device__ __forceinline__ double readlane_d(double a, int l) {
+    const int2 v = *reinterpret_cast<int2 *>(&a);
+    int2 r;
+    r.x = __builtin_amdgcn_readlane(v.x, l);
+    r.y = __builtin_amdgcn_readlane(v.y, l);
+    return *reinterpret_cast<double *>(&r);
 }
 
 // One wave per chunk: the box of its 64 buckets.
@@ -1430,11 +1436,13 @@ __global__ __launch_bounds__(256) void k_chunk_box(TreeDev t, int64_t nchunk, do
 // appended as (target, chunk) pairs, one atomic per wave and 64 chunks.
 __device__ void far_pairs_of(const TreeDev &t, const int32_t *__restrict__ list, int64_t k, int64_t nchunk,
                              const double4 *__restrict__ cbox, double cut, const double *__restrict__ dd,
-                             int2 *__restrict__ pairs, unsigned long long *__restrict__ npairs, int lane) {
+                             int2 *__restrict__ pairs, unsigned long long *__restrict__ npairs,
+                             double4 *__restrict__ ftgt, int lane) {
     const int64_t ti = list[k];
     const double di = dd[t.perm[ti]];
     const double R2 = (cut * di) * (cut * di);
     const double ax = t.sx[ti], ay = t.sy[ti];
+    if (lane == 0) ftgt[k] = make_double4(ax, ay, di, 0.0);  // for the pair kernel
     for (int64_t c0 = 0; c0 < nchunk; c0 += 64) {
         const int64_t ch = c0 + lane;
         const bool need = ch < nchunk && box_d2(cbox[ch], ax, ay) < R2;
@@ -1451,11 +1459,12 @@ __global__ __launch_bounds__(256) void k_far_pairs(TreeDev t, const int32_t *__r
                                                    const unsigned long long *__restrict__ cnt, int64_t nchunk,
                                                    const double4 *__restrict__ cbox, double cut,
                                                    const double *__restrict__ dd, int2 *__restrict__ pairs,
-                                                   unsigned long long *__restrict__ npairs) {
+                                                   unsigned long long *__restrict__ npairs,
+                                                   double4 *__restrict__ ftgt) {
     const int lane = threadIdx.x & 63;
     const int64_t H = far_count(cnt);
     for (int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); k < H; k += (int64_t)gridDim.x * 4)
-        far_pairs_of(t, list, k, nchunk, cbox, cut, dd, pairs, npairs, lane);
+        far_pairs_of(t, list, k, nchunk, cbox, cut, dd, pairs, npairs, ftgt, lane);
 }
 
 // Zero the counters a[q] whose bit q is set in mask.
@@ -1471,19 +1480,20 @@ __global__ void k_far_zero(double *__restrict__ a, int64_t nchunk, const unsigne
 }
 
 template <typename T>
-__device__ void diff_far_pair(const TreeDev &t, const int32_t *__restrict__ list, int64_t H, int64_t k, int64_t ch,
-                              double cut, double thr, const double *__restrict__ dd, double *__restrict__ part,
-                              unsigned long long &cpairs) {
+__device__ void diff_far_pair(const TreeDev &t, const double4 *__restrict__ ftgt, int64_t H, int64_t k, int64_t ch,
+                              double cut, double thr, double *__restrict__ part, unsigned long long &cpairs) {
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x & 63;
-    const int64_t ti = list[k];
-    const double di = dd[t.perm[ti]];
+    const double4 tg = ftgt[k];
+    const double ax = tg.x, ay = tg.y, di = tg.z;
     const double R2 = (cut * di) * (cut * di);
-    const double ax = t.sx[ti], ay = t.sy[ti];
     const T kd = F32 ? (T)(-1.4426950408889634 / di) : (T)(1.0 / di);
     const T thr2 = F32 ? (T)(thr * thr) : (T)thr;
+    // the chunk's buckets, one per lane: box test and the record in one load
     const int64_t b = ch * 64 + lane;
-    const bool need = b < t.nleaf && box_d2(t.nbb[b], ax, ay) < R2;
+    const bool inb = b < t.nleaf;
+    const NodeRec rb = inb ? t.rec[b] : NodeRec{0.0, 0.0, 0.0, -1, 0};
+    const bool need = inb && box_d2(t.nbb[b], ax, ay) < R2;
     uint64_t mask = __ballot(need);
     double a2x = 0.0, a2y = 0.0, a1 = 0.0;
     int64_t npairs = 0;
@@ -1498,10 +1508,13 @@ __device__ void diff_far_pair(const TreeDev &t, const int32_t *__restrict__ list
             dx[q] = dy[q] = T(1);
             g[q] = T(0);
             if (mask) {
-                const int bn = (int)(ch * 64) + __builtin_ctzll(mask);
+                const int bl = __builtin_ctzll(mask);
                 mask &= mask - 1;
-                far_load<T>(t, bn, lane, ax, ay, dx[q], dy[q], g[q]);
-                npairs += t.rec[bn].b;
+                const int64_t j0 = -1 - __builtin_amdgcn_readlane(rb.a, bl);
+                const int cnt = __builtin_amdgcn_readlane(rb.b, bl);
+                far_load<T>(t, j0, cnt, readlane_d(rb.cx, bl), readlane_d(rb.cy, bl), lane, ax, ay, dx[q], dy[q],
+                            g[q]);
+                npairs += cnt;
                 ++nbk;
             }
         }
@@ -1528,17 +1541,17 @@ __device__ void diff_far_pair(const TreeDev &t, const int32_t *__restrict__ list
 
 // Grid-stride over the listed pairs, one wave each; the count stays on the device.
 template <typename T>
-__global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const int32_t *__restrict__ list,
+__global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const double4 *__restrict__ ftgt,
                                                   const int2 *__restrict__ pairs,
                                                   const unsigned long long *__restrict__ npair_list, double cut,
-                                                  double thr, const double *__restrict__ dd,
-                                                  double *__restrict__ part, unsigned long long *__restrict__ cnt) {
+                                                  double thr, double *__restrict__ part,
+                                                  unsigned long long *__restrict__ cnt) {
     const int64_t H = far_count(cnt);
     const int64_t total = (int64_t)*npair_list;
     const int64_t nw = (int64_t)gridDim.x * 4;
     unsigned long long cp = 0;
     for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < total; w += nw)
-        diff_far_pair<T>(t, list, H, pairs[w].x, pairs[w].y, cut, thr, dd, part, cp);
+        diff_far_pair<T>(t, ftgt, H, pairs[w].x, pairs[w].y, cut, thr, part, cp);
     block_count_add(cp, 0ull, cnt, nullptr);
 }
 
@@ -1608,7 +1621,7 @@ void tree_release(vx_case *c) {
     t.bstart.release(); t.bscan.release(); t.upper.release(); t.uscan.release(); t.visits.release();
     t.sx.release(); t.sy.release(); t.sg.release(); t.ncx.release(); t.ncy.release(); t.nr.release();
     t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release(); t.rec.release();
-    t.cbox.release(); t.fpairs.release();
+    t.cbox.release(); t.fpairs.release(); t.ftgt.release();
     t.gcs.release(); t.gcs2.release(); t.gflag.release(); t.gscan.release(); t.gfirst.release(); t.pcount.release(); t.arr.release(); t.sizes.release();
     t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.fbr2.release(); t.hpart.release();
     if (t.ev) (void)hipEventDestroy(t.ev);
@@ -1838,18 +1851,18 @@ int launch_tree_diffusion(vx_case *c) {
     // kernels sized for the list's capacity)
     const int64_t nchunk = (t.nleaf + 63) / 64;
     if (t.hpart.ensure((size_t)(3 * nchunk * kFarCap), st) || t.cbox.ensure((size_t)nchunk, st) ||
-        t.fpairs.ensure((size_t)(nchunk * kFarCap), st))
+        t.fpairs.ensure((size_t)(nchunk * kFarCap), st) || t.ftgt.ensure((size_t)kFarCap, st))
         return 1;
     hipLaunchKernelGGL(k_far_zero, dim3(512), dim3(256), 0, st, t.hpart.p, nchunk, cnt);
     hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)((nchunk + 3) / 4)), dim3(256), 0, st, tv, nchunk, t.cbox.p);
     hipLaunchKernelGGL(k_far_pairs, dim3(256), dim3(256), 0, st, tv, t.fb.p, cnt, nchunk, t.cbox.p, cut, c->d.p,
-                       t.fpairs.p, cnt + 3);
+                       t.fpairs.p, cnt + 3, t.ftgt.p);
     if (mixed)
-        hipLaunchKernelGGL(k_diff_far<float>, dim3(4096), dim3(256), 0, st, tv, t.fb.p, t.fpairs.p, cnt + 3, cut, thr,
-                           c->d.p, t.hpart.p, cnt);
+        hipLaunchKernelGGL(k_diff_far<float>, dim3(4096), dim3(256), 0, st, tv, t.ftgt.p, t.fpairs.p, cnt + 3, cut,
+                           thr, t.hpart.p, cnt);
     else
-        hipLaunchKernelGGL(k_diff_far<double>, dim3(4096), dim3(256), 0, st, tv, t.fb.p, t.fpairs.p, cnt + 3, cut,
-                           thr, c->d.p, t.hpart.p, cnt);
+        hipLaunchKernelGGL(k_diff_far<double>, dim3(4096), dim3(256), 0, st, tv, t.ftgt.p, t.fpairs.p, cnt + 3, cut,
+                           thr, t.hpart.p, cnt);
     hipLaunchKernelGGL(k_diff_far_final, dim3(1024), dim3(256), 0, st, tv, t.fb.p, cnt, nchunk, t.hpart.p, c->d.p,
                        c->prm.nu, c->u.p, c->v.p);
     VX_CHECK(hipGetLastError());
