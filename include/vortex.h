@@ -155,7 +155,9 @@ int vx_tree_diffusion(vx_case *c);
 typedef struct vx_tree_info {
     int64_t m, leaves, nodes;        /* particles in the tree, leaves, all nodes      */
     int64_t fallback;                /* targets whose nearest-3 used the exact search */
-    int64_t bs_pairs, m2p_evals;     /* pairwise and multipole evaluations, last BS   */
+    int64_t bs_pairs, m2p_evals;     /* pairwise pairs and (target, far node)
+                                        interactions, last BS; the FP64 path
+                                        evaluates each by M2P                 */
     int64_t diff_pairs;              /* pairwise evaluations, last diffusion          */
     int32_t order;
     int32_t heavy;                   /* diffusion targets with a domain-scale radius  */
