@@ -644,8 +644,9 @@ __device__ __forceinline__ void stage_leaf(Src4T<T> *lds, int lane, int64_t j0, 
 }
 
 // Pair loop over one staged leaf: Biot-Savart sums and, with TRACK, the four
-// smallest squared distances (the target itself included).
-template <typename T, bool TRACK>
+// smallest squared distances (the target itself included).  Without CLAMP
+// the weight is g / r2: the caller has shown r2 >= eps2 for every pair.
+template <typename T, bool TRACK, bool CLAMP = true>
 __device__ __forceinline__ void p2p_leaf(const Src4T<T> *lds, int cnt, T ox, T oy, T eps2, T &pu, T &pv, T &m0,
                                          T &m1, T &m2, T &m3) {
     using PM = PairMath<T>;
@@ -659,7 +660,7 @@ __device__ __forceinline__ void p2p_leaf(const Src4T<T> *lds, int cnt, T ox, T o
             m1 = PM::med3(m0, m1, r2);
             m0 = fmin(m0, r2);
         }
-        const T w = PM::div(q.g, fmax(r2, eps2));
+        const T w = PM::div(q.g, CLAMP ? fmax(r2, eps2) : r2);
         pv += w * dx;
         pu -= w * dy;
     }
@@ -761,14 +762,16 @@ __device__ __forceinline__ float readlane_f(float a, int l) {
 // 1/(D + w)^(k+1) = sum_l C(k + l, l) (-w)^l / D^(k+l+1), D = c_T - c_S,
 // w = z - c_T.  The opening test keeps (R_S + R_T) / |D| below theta, so the
 // truncation at ORDER terms on both sides leaves about theta^ORDER of the
-// node's field.  The nodes are fq[0, count) of this wave's queue.  The
+// node's field.  The nodes are fq[0, count) of this wave's queue; gap
+// takes the lane's least |c_T - c_S| - R_S - R_T.  The
 // lanes' terms are summed over the wave through LDS four coefficients at a
 // time (tr: this wave's 4 x 64 slots): lane (c, e), c = lane & 3, adds the
 // terms of lanes 4e .. 4e + 3 for coefficient 4 j + c into acc[j]; the
 // partial sums over e are combined once, by l2p_coeffs().
 template <int ORDER>
 __device__ __forceinline__ void m2l_batch(const TreeDev &t, const int32_t *fq, int count, int lane, double cTx,
-                                          double cTy, float rT, float2 (*tr)[64], float2 (&acc)[ORDER / 4]) {
+                                          double cTy, float rT, float2 (*tr)[64], float2 (&acc)[ORDER / 4],
+                                          float &gap) {
     const int node = lane < count ? fq[lane] : -1;
     float akr[ORDER], aki[ORDER];  // alpha^k b_k
     float qbr = 0.f, qbi = 0.f;    // q (-beta)^l
@@ -778,10 +781,13 @@ __device__ __forceinline__ void m2l_batch(const TreeDev &t, const int32_t *fq, i
     if (node >= 0) {
         const NodeRec S = t.rec[node];
         const float dx = (float)(cTx - S.cx), dy = (float)(cTy - S.cy);
-        const float inv = __builtin_amdgcn_rcpf(dx * dx + dy * dy);
+        const float d2 = dx * dx + dy * dy;
+        const float inv = __builtin_amdgcn_rcpf(d2);
         qbr = dx * inv;
         qbi = -dy * inv;
         const float rs = fmaxf((float)S.r, 1e-30f);
+        // nearest possible particle of the node: |c_T - c_S| - R_S - R_T
+        gap = fminf(gap, __builtin_amdgcn_sqrtf(d2) - rs - rT);
         const float alr = rs * qbr, ali = rs * qbi;
         mbr = -rT * qbr;
         mbi = -rT * qbi;
@@ -891,6 +897,7 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     T m0 = T(1e5), m1 = T(1e5), m2 = T(1e5), m3 = T(1e5);
     T farmin = T(1e30);         // FP64: per target, nearest point of a far node
     double farmin_w = 1e30;     // FP32: per bucket, |c_T - c_S| - R_S - R_T
+    float gap = 1e30f;          // FP32: this lane's share of it (m2l_batch)
     int nq = 0;                 // FP32: far nodes queued
     float2 lacc[ORDER / 4];     // FP32: local coefficients, partial (m2l_batch)
 #pragma unroll
@@ -921,11 +928,10 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
             ++nfar;
             if constexpr (F32) {
                 // queued for M2L into the bucket's local expansion, 64 at a time
-                farmin_w = fmin(farmin_w, sqrt(d2) - rs);
                 if (lane == 0) fq[nq] = node;
                 if (++nq == 64) {
                     __builtin_amdgcn_wave_barrier();
-                    m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, tr, lacc);
+                    m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, tr, lacc, gap);
                     __builtin_amdgcn_wave_barrier();
                     cm2l += (unsigned long long)nq;
                     nq = 0;
@@ -968,13 +974,19 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
             // a pair at r2 >= m3 leaves the four smallest unchanged: skip the
             // updates when the leaf's circle lies beyond sqrt(m3) of every
             // target (a 1e-3 margin covers the FP32 distances)
+            // likewise the cap max(r2, eps2) only matters for a leaf within
+            // eps of some target
             const T cx_ = F32 ? ox : ax - (T)nd.cx, cy_ = F32 ? oy : ay - (T)nd.cy;
-            const T reach = (T)nd.r + PM::sqrt_bound(m3);
-            const bool near3 = ok && !(cx_ * cx_ + cy_ * cy_ > reach * reach * T(1.002));
+            const T dc2 = cx_ * cx_ + cy_ * cy_;
+            const T reach = (T)nd.r + PM::sqrt_bound(m3), reache = (T)nd.r + (T)eps;
+            const bool near3 = ok && !(dc2 > reach * reach * T(1.002));
+            const bool neare = ok && !(dc2 > reache * reache * T(1.002));
             if (__ballot(near3))
                 p2p_leaf<T, true>(lds, cnt, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
-            else
+            else if (__ballot(neare))
                 p2p_leaf<T, false>(lds, cnt, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
+            else
+                p2p_leaf<T, false, false>(lds, cnt, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
             __builtin_amdgcn_wave_barrier();
             su += (double)pu + (double)fu;
             sv += (double)pv + (double)fv;
@@ -989,9 +1001,11 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     if constexpr (F32) {
         if (nq > 0) {
             __builtin_amdgcn_wave_barrier();
-            m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, tr, lacc);
+            m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, tr, lacc, gap);
             cm2l += (unsigned long long)nq;
         }
+        for (int o = 32; o > 0; o >>= 1) gap = fminf(gap, __shfl_xor(gap, o, 64));
+        farmin_w = (double)gap;
         // L2P: F = sum_l L_l u^l at u = (z - c_T) / R_T (ax, ay are relative
         // to the bucket centre); u += Im F, v += Re F
         l2p_coeffs<ORDER>(lacc);
