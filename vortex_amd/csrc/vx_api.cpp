@@ -80,7 +80,7 @@ int vx_case_create(const vx_params *prm, const vx_panel *panels, int64_t P, vx_c
     c->P = P;
     VX_CHECK(hipGetDevice(&c->device));
     VX_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    VX_CHECK(hipHostMalloc((void **)&c->h_count, 64, hipHostMallocDefault));
+    VX_CHECK(hipHostMalloc((void **)&c->h_count, 128, hipHostMallocDefault));
     VX_CHECK(hipHostMalloc((void **)&c->h_red, 64, hipHostMallocDefault));
     for (auto &e : c->ev) VX_CHECK(hipEventCreate(&e));
     // panels -> SoA, one allocation: 13 double columns + rp
@@ -319,15 +319,15 @@ int vx_diffusion_wall(vx_case *c, int64_t n_tgt) {
     return 0;
 }
 
-// Read back the step sums (c->red[0..3]), the device count and the collapse
-// error flag; one synchronisation.
+// Read back the step sums (c->red[0..3]), the device count (slot dn_slot)
+// and the collapse error flag; one synchronisation.
 static int step_readback(vx_case *c, double out4[4], bool *err) {
     if (out4) VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    VX_CHECK(hipMemcpyAsync(c->h_count, c->dcount.p, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipMemcpyAsync(c->h_count, c->dcount.p, 9 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     VX_CHECK(hipStreamSynchronize(c->stream));
     if (out4)
         for (int q = 0; q < 4; ++q) out4[q] = c->h_red[q];
-    c->n = c->h_count[1];
+    c->n = c->h_count[c->dn_slot];
     if (err) *err = (int32_t)c->h_count[3] != 0;
     return 0;
 }

@@ -113,7 +113,8 @@ struct vx_case {
     vx::DBuf<unsigned char> cub_tmp;
     vx::DBuf<unsigned char> gridp;                     // collapse grid parameters (device)
     vx::DBuf<int64_t> dcount;                         // device counters
-    int64_t *h_count = nullptr;                       // pinned read-back slot
+    int dn_slot = 1;                                  // dcount slot of the step's current count (1 or 8)
+    int64_t *h_count = nullptr;                       // pinned read-back slots (16)
     double *h_red = nullptr;                          // pinned reduction read-back
     // probes
     vx::DBuf<double> px, py, pu, pv, pg;

@@ -586,7 +586,7 @@ int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
     const int64_t nchunk = n_src > 0 ? (n_src + kRhsChunk - 1) / kRhsChunk : 1;
     constexpr int64_t kFarBlocks = 512;
     if (c->partial.ensure((size_t)(2 * nchunk * P), st) || c->rp.ensure((size_t)P, st) ||
-        c->lists.ensure((size_t)(n_src > 0 ? n_src : 1), st) || c->dcount.ensure(8, st) ||
+        c->lists.ensure((size_t)(n_src > 0 ? n_src : 1), st) || c->dcount.ensure(16, st) ||
         c->rhs_s.ensure((size_t)(2 * kLoc * (kFarBlocks + 1)), st))
         return 1;
     const vx_params &p = c->prm;

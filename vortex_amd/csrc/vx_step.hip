@@ -205,7 +205,10 @@ __global__ __launch_bounds__(256) void k_sum4(const double *__restrict__ part, i
     }
 }
 
-int64_t *step_dn(vx_case *c) { return c->dcount.p + 1; }
+// The count lives in dcount[1] or dcount[8]: each compaction reads the
+// current slot and writes the new count to the other one, so no kernel reads
+// a count that another thread of the same launch is replacing.
+int64_t *step_dn(vx_case *c) { return c->dcount.p + c->dn_slot; }
 
 // dcount[1] = n (the device count), dcount[3] = 0 (the collapse error flag,
 // sticky over the passes of one step).
@@ -215,9 +218,10 @@ __global__ void k_step_init(int64_t *__restrict__ dcount, int64_t n) {
 }
 
 int launch_set_dn(vx_case *c, int64_t n) {
-    if (c->dcount.ensure(8, c->stream)) return 1;
+    if (c->dcount.ensure(16, c->stream)) return 1;
     hipLaunchKernelGGL(k_step_init, dim3(1), dim3(1), 0, c->stream, c->dcount.p, n);
     VX_CHECK(hipGetLastError());
+    c->dn_slot = 1;
     return 0;
 }
 
@@ -227,7 +231,7 @@ int launch_step_move(vx_case *c) {
     hipStream_t st = c->stream;
     if (c->partial.ensure((size_t)(8 * nb), st) || c->red.ensure(8, st) ||
         c->misc.ensure((size_t)(n > 0 ? n : 1), st) || c->lists.ensure((size_t)(n > 0 ? n : 1), st) ||
-        c->dcount.ensure(8, st))
+        c->dcount.ensure(16, st))
         return 1;
     const vx_params &p = c->prm;
     StepConsts k{p.dt, p.rc_x, p.rc_y, p.x_min, p.x_max, p.y_min, p.y_max, p.px_min, p.px_max, p.py_min, p.py_max};
@@ -279,7 +283,7 @@ __global__ __launch_bounds__(256) void k_points_move(double *__restrict__ x, dou
 int launch_points_move(vx_case *c, int64_t m) {
     if (m <= 0) return 0;
     hipStream_t st = c->stream;
-    if (c->misc.ensure((size_t)m, st) || c->lists.ensure((size_t)m, st) || c->dcount.ensure(8, st)) return 1;
+    if (c->misc.ensure((size_t)m, st) || c->lists.ensure((size_t)m, st) || c->dcount.ensure(16, st)) return 1;
     const vx_params &p = c->prm;
     StepConsts k{p.dt, p.rc_x, p.rc_y, p.x_min, p.x_max, p.y_min, p.y_max, p.px_min, p.px_max, p.py_min, p.py_max};
     int32_t *hitp = c->misc.p, *cand = c->lists.p;
@@ -305,36 +309,45 @@ int launch_points_move(vx_case *c, int64_t m) {
 // the position of src(e): e+1 when e+1 < n is dead, else the start of e's
 // chain -- hole h_j where j-1 live entries of [n1, n) lie above e, or n1
 // itself below the last live entry.  Gather to staging, then scatter.
-// Grids cover the upper bound n_ub; the count n = *dn is read on the device.
-__global__ void k_live_flags(const double *__restrict__ g, int64_t n_ub, const int64_t *__restrict__ dn,
-                             int32_t *__restrict__ f) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_ub) f[i] = (i < *dn && !(fabs(g[i]) < kDelt)) ? 1 : 0;
+// Grids cover the upper bound n_ub; the count n = *dn is read on the device
+// and n1 goes to the other count slot.
+__device__ __forceinline__ int32_t live_at(const double *__restrict__ g, int64_t i, int64_t n) {
+    return (i < n && !(fabs(g[i]) < kDelt)) ? 1 : 0;
 }
 
-// s = exclusive scan of flags; n1 = total live (s[n-1] + f[n-1]).
-__global__ void k_compact_lists(const int32_t *__restrict__ f, const int32_t *__restrict__ s,
-                                const int64_t *__restrict__ dn, int32_t *__restrict__ holes,
-                                int32_t *__restrict__ donors) {
+// The scan's input: live flags made on the fly.
+struct LiveFlag {
+    const double *g;
+    const int64_t *dn;
+    __device__ int32_t operator()(int64_t i) const { return live_at(g, i, *dn); }
+};
+
+// s = exclusive scan of the flags; n1 = total live (s[n-1] + f[n-1]), written
+// to the other count slot *dn1 by thread 0.
+__global__ void k_compact_lists(const double *__restrict__ g, const int32_t *__restrict__ s,
+                                const int64_t *__restrict__ dn, int64_t *__restrict__ dn1,
+                                int32_t *__restrict__ holes, int32_t *__restrict__ donors) {
     const int64_t n = *dn;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n1 = n > 0 ? (int64_t)s[n - 1] + live_at(g, n - 1, n) : 0;
+    if (i == 0) *dn1 = n1;
     if (i >= n) return;
-    const int64_t n1 = (int64_t)s[n - 1] + f[n - 1];
+    const int32_t f = live_at(g, i, n);
     if (i < n1) {
-        if (!f[i]) holes[i - s[i]] = (int32_t)i;  // rank among the holes of [0, n1)
-    } else if (f[i]) {
-        donors[n1 - s[i] - 1] = (int32_t)i;       // rank from the top among live entries of [n1, n)
+        if (!f) holes[i - s[i]] = (int32_t)i;  // rank among the holes of [0, n1)
+    } else if (f) {
+        donors[n1 - s[i] - 1] = (int32_t)i;    // rank from the top among live entries of [n1, n)
     }
 }
 
 __global__ void k_compact_gather(const double *__restrict__ x, const double *__restrict__ y,
-                                 const double *__restrict__ g, const int32_t *__restrict__ f,
-                                 const int32_t *__restrict__ s, const int64_t *__restrict__ dn,
+                                 const double *__restrict__ g, const int32_t *__restrict__ s,
+                                 const int64_t *__restrict__ dn, const int64_t *__restrict__ dn1,
                                  const int32_t *__restrict__ holes, const int32_t *__restrict__ donors,
                                  double *__restrict__ tx, double *__restrict__ ty, double *__restrict__ tg) {
     const int64_t n = *dn;
     if (n <= 0) return;
-    const int64_t n1 = (int64_t)s[n - 1] + f[n - 1];
+    const int64_t n1 = *dn1;
     const int64_t K = n1 > 0 ? (int64_t)(n1 - (n1 < n ? s[n1] : n1)) : 0;  // holes in [0, n1)
     const int64_t T = n - n1;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -345,9 +358,9 @@ __global__ void k_compact_gather(const double *__restrict__ x, const double *__r
         tg[t] = g[src];
     } else if (t < K + T) {
         const int64_t e = n1 + (t - K);
-        const int64_t above = n1 - ((int64_t)s[e] + f[e]);  // live entries of [n1, n) above e
+        const int64_t above = n1 - ((int64_t)s[e] + live_at(g, e, n));  // live entries of [n1, n) above e
         const int64_t start = above < K ? (int64_t)holes[above] : n1;
-        const int64_t src = (e + 1 < n && !f[e + 1]) ? e + 1 : start;
+        const int64_t src = (e + 1 < n && !live_at(g, e + 1, n)) ? e + 1 : start;
         tx[t] = x[src];
         ty[t] = y[src];
         tg[t] = 0.0;
@@ -355,13 +368,13 @@ __global__ void k_compact_gather(const double *__restrict__ x, const double *__r
 }
 
 __global__ void k_compact_scatter(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
-                                  const int32_t *__restrict__ f, const int32_t *__restrict__ s,
-                                  const int64_t *__restrict__ dn, const int32_t *__restrict__ holes,
+                                  const int32_t *__restrict__ s, const int64_t *__restrict__ dn,
+                                  const int64_t *__restrict__ dn1, const int32_t *__restrict__ holes,
                                   const double *__restrict__ tx, const double *__restrict__ ty,
                                   const double *__restrict__ tg) {
     const int64_t n = *dn;
     if (n <= 0) return;
-    const int64_t n1 = (int64_t)s[n - 1] + f[n - 1];
+    const int64_t n1 = *dn1;
     const int64_t K = n1 > 0 ? (int64_t)(n1 - (n1 < n ? s[n1] : n1)) : 0;
     const int64_t T = n - n1;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -377,35 +390,31 @@ __global__ void k_compact_scatter(double *__restrict__ x, double *__restrict__ y
     g[dst] = tg[t];
 }
 
-__global__ void k_compact_count(const int32_t *__restrict__ f, const int32_t *__restrict__ s,
-                                int64_t *__restrict__ dn) {
-    const int64_t n = *dn;
-    if (n > 0) *dn = (int64_t)s[n - 1] + f[n - 1];
-}
-
 int launch_compact(vx_case *c, int64_t n_ub) {
     if (n_ub <= 0) return 0;
     hipStream_t st = c->stream;
-    if (c->flags.ensure((size_t)n_ub, st) || c->scan.ensure((size_t)n_ub, st) ||
-        c->lists.ensure((size_t)(2 * n_ub), st) || c->tx.ensure((size_t)(2 * n_ub), st) ||
-        c->ty.ensure((size_t)(2 * n_ub), st) || c->tg.ensure((size_t)(2 * n_ub), st) || c->dcount.ensure(8, st))
+    if (c->scan.ensure((size_t)n_ub, st) || c->lists.ensure((size_t)(2 * n_ub), st) ||
+        c->tx.ensure((size_t)(2 * n_ub), st) || c->ty.ensure((size_t)(2 * n_ub), st) ||
+        c->tg.ensure((size_t)(2 * n_ub), st) || c->dcount.ensure(16, st))
         return 1;
-    int64_t *dn = step_dn(c);
+    int64_t *dn = step_dn(c), *dn1 = c->dcount.p + (c->dn_slot == 1 ? 8 : 1);
     const unsigned nb = (unsigned)((n_ub + 255) / 256);
-    hipLaunchKernelGGL(k_live_flags, dim3(nb), dim3(256), 0, st, c->g.p, n_ub, dn, c->flags.p);
+    hipcub::CountingInputIterator<int64_t> it(0);
+    hipcub::TransformInputIterator<int32_t, LiveFlag, hipcub::CountingInputIterator<int64_t>> fl(it,
+                                                                                               LiveFlag{c->g.p, dn});
     size_t bytes = 0;
-    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->flags.p, c->scan.p, (int)n_ub, st));
+    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, fl, c->scan.p, (int)n_ub, st));
     if (c->cub_tmp.ensure(bytes, st)) return 1;
-    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, c->flags.p, c->scan.p, (int)n_ub, st));
+    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, fl, c->scan.p, (int)n_ub, st));
     int32_t *holes = c->lists.p, *donors = c->lists.p + n_ub;
-    hipLaunchKernelGGL(k_compact_lists, dim3(nb), dim3(256), 0, st, c->flags.p, c->scan.p, dn, holes, donors);
+    hipLaunchKernelGGL(k_compact_lists, dim3(nb), dim3(256), 0, st, c->g.p, c->scan.p, dn, dn1, holes, donors);
     const unsigned nb2 = (unsigned)((2 * n_ub + 255) / 256);
-    hipLaunchKernelGGL(k_compact_gather, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->flags.p, c->scan.p,
-                       dn, holes, donors, c->tx.p, c->ty.p, c->tg.p);
-    hipLaunchKernelGGL(k_compact_scatter, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->flags.p,
-                       c->scan.p, dn, holes, c->tx.p, c->ty.p, c->tg.p);
-    hipLaunchKernelGGL(k_compact_count, dim3(1), dim3(1), 0, st, c->flags.p, c->scan.p, dn);
+    hipLaunchKernelGGL(k_compact_gather, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->scan.p, dn, dn1,
+                       holes, donors, c->tx.p, c->ty.p, c->tg.p);
+    hipLaunchKernelGGL(k_compact_scatter, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->scan.p, dn, dn1,
+                       holes, c->tx.p, c->ty.p, c->tg.p);
     VX_CHECK(hipGetLastError());
+    c->dn_slot = c->dn_slot == 1 ? 8 : 1;
     return 0;
 }
 
@@ -438,7 +447,7 @@ __device__ __forceinline__ void cell_of(const Grid &G, double x, double y, int64
 
 // box = (min x, max x, min y, max y) of the g != 0 vortices; at most
 // max_cells cells (a wider cell stays correct).
-__global__ void k_grid_setup(const double *__restrict__ box, double r2, int64_t max_cells, Grid *__restrict__ G) {
+__device__ Grid grid_params(const double *__restrict__ box, double r2, int64_t max_cells) {
     Grid q{};
     const double x0 = box[0], x1 = box[1], y0 = box[2], y1 = box[3];
     q.valid = (r2 > 0.0) && (x0 <= x1) && (y0 <= y1);
@@ -453,11 +462,20 @@ __global__ void k_grid_setup(const double *__restrict__ box, double r2, int64_t 
         q.ny = (int64_t)floor(h / s) + 1;
         q.cells = q.nx * q.ny;
     }
-    *G = q;
+    return q;
 }
 
-__global__ void k_grid_clear(int32_t *__restrict__ head, const Grid *__restrict__ G) {
-    const int64_t cells = G->valid ? G->cells : 0;
+// Each work-group derives the grid from the box (the same result in every
+// one), work-group 0 stores it, and the cell heads are cleared to -1.
+__global__ void k_grid_clear(int32_t *__restrict__ head, const double *__restrict__ box, double r2,
+                             int64_t max_cells, Grid *__restrict__ G) {
+    __shared__ Grid sq;
+    if (threadIdx.x == 0) {
+        sq = grid_params(box, r2, max_cells);
+        if (blockIdx.x == 0) *G = sq;
+    }
+    __syncthreads();
+    const int64_t cells = sq.valid ? sq.cells : 0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cells; t += (int64_t)gridDim.x * blockDim.x)
         head[t] = -1;
 }
@@ -587,8 +605,7 @@ int launch_setka(vx_case *c, int kind, int64_t n_ub) {
         c->gridp.ensure(sizeof(Grid), st))
         return 1;
     Grid *G = (Grid *)c->gridp.p;
-    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1), 0, st, c->red.p + 4, r2, max_cells, G);
-    hipLaunchKernelGGL(k_grid_clear, dim3(1024), dim3(256), 0, st, c->vals.p, G);
+    hipLaunchKernelGGL(k_grid_clear, dim3(1024), dim3(256), 0, st, c->vals.p, c->red.p + 4, r2, max_cells, G);
     int64_t *dn = step_dn(c);
     hipLaunchKernelGGL(k_grid_insert, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn, G, c->vals.p,
                        c->vals_alt.p);
@@ -658,7 +675,8 @@ __global__ __launch_bounds__(256) void k_col_leaves(const int32_t *__restrict__ 
 __global__ __launch_bounds__(256) void k_col_publish_leaves(const int32_t *__restrict__ col,
                                                             const int32_t *__restrict__ cand,
                                                             const unsigned long long *__restrict__ ncand_p,
-                                                            int32_t *__restrict__ pc, int32_t *__restrict__ state) {
+                                                            int32_t *__restrict__ pc, int32_t *__restrict__ state,
+                                                            int32_t *__restrict__ first) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)*ncand_p) return;
     const int32_t i = cand[t];
@@ -668,6 +686,7 @@ __global__ __launch_bounds__(256) void k_col_publish_leaves(const int32_t *__res
     const int32_t j = col[i];
     state[i] = 1 | 16;
     atomicOr(&state[j], 4);
+    atomicMin(&first[j], i);
     atomicSub(&pc[j], 1);
 }
 
@@ -680,7 +699,7 @@ __global__ __launch_bounds__(256) void k_col_publish_leaves(const int32_t *__res
 __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
                                                       const int32_t *__restrict__ ncand_p,
                                                       int32_t *__restrict__ pc, int32_t *__restrict__ state,
-                                                      int32_t *__restrict__ err) {
+                                                      int32_t *__restrict__ first, int32_t *__restrict__ err) {
     __shared__ int undecided;
     const int64_t ncand = (int64_t)*ncand_p;
     int64_t round = 0;
@@ -702,7 +721,10 @@ __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict_
             const int32_t s = atomicOr(&state[i], 0);
             if ((s & 3) && !(s & 16)) {
                 const int32_t j = col[i];
-                if (s & 1) atomicOr(&state[j], 4);
+                if (s & 1) {
+                    atomicOr(&state[j], 4);
+                    atomicMin(&first[j], i);
+                }
                 atomicOr(&state[i], 16);
                 atomicSub(&pc[j], 1);
             }
@@ -716,15 +738,6 @@ __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict_
             break;
         }
     }
-}
-
-__global__ void k_col_first(const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
-                            const unsigned long long *__restrict__ ncand_p, const int32_t *__restrict__ state,
-                            int32_t *__restrict__ first) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)*ncand_p) return;
-    const int32_t i = cand[t];
-    if (state[i] & 1) atomicMin(&first[col[i]], i);
 }
 
 template <int KIND>
@@ -782,7 +795,7 @@ int launch_collapse(vx_case *c, int kind, int64_t n_ub) {
     if (c->flags.ensure((size_t)n_ub, st) || c->scan.ensure((size_t)n_ub, st) ||
         c->lists.ensure((size_t)(2 * n_ub), st) || c->misc.ensure((size_t)n_ub, st) ||
         c->tx.ensure((size_t)n_ub, st) || c->ty.ensure((size_t)n_ub, st) || c->tg.ensure((size_t)n_ub, st) ||
-        c->dcount.ensure(8, st))
+        c->dcount.ensure(16, st))
         return 1;
     int32_t *pc = c->flags.p, *state = c->scan.p, *cand = c->lists.p, *first = c->misc.p;
     unsigned long long *ncand = (unsigned long long *)(c->dcount.p + 2);
@@ -799,10 +812,9 @@ int launch_collapse(vx_case *c, int kind, int64_t n_ub) {
         VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cand, ncand, (int)n_ub, pred, st));
     }
     hipLaunchKernelGGL(k_col_leaves, dim3(nb), dim3(256), 0, st, cand, ncand, pc, state, rest, nrest);
-    hipLaunchKernelGGL(k_col_publish_leaves, dim3(nb), dim3(256), 0, st, c->col.p, cand, ncand, pc, state);
-    hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, rest, nrest, pc, state,
+    hipLaunchKernelGGL(k_col_publish_leaves, dim3(nb), dim3(256), 0, st, c->col.p, cand, ncand, pc, state, first);
+    hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, rest, nrest, pc, state, first,
                        (int32_t *)(c->dcount.p + 3));
-    hipLaunchKernelGGL(k_col_first, dim3(nb), dim3(256), 0, st, c->col.p, cand, ncand, state, first);
     if (kind == 1)
         hipLaunchKernelGGL(k_col_merge<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cand, ncand,
                            state, first, c->tx.p, c->ty.p, c->tg.p);

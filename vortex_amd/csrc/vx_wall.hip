@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_diffusion_wall(const double *__restrict
 int launch_diffusion_wall(vx_case *c, int64_t n_tgt) {
     if (n_tgt <= 0 || c->P <= 0) return 0;
     hipStream_t st = c->stream;
-    if (c->lists.ensure((size_t)n_tgt, st) || c->dcount.ensure(8, st)) return 1;
+    if (c->lists.ensure((size_t)n_tgt, st) || c->dcount.ensure(16, st)) return 1;
     int32_t *cnt = (int32_t *)(c->dcount.p + 5);
     const InWallBox pred{c->x.p, c->y.p, c->wall_box};
     hipcub::CountingInputIterator<int32_t> it(0);
