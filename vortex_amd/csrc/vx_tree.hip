@@ -344,7 +344,9 @@ __global__ __launch_bounds__(256) void k_bucket_build(const double *__restrict__
     const double x0 = wave_min(ok ? xi : 1e300), x1 = wave_max(ok ? xi : -1e300);
     const double y0 = wave_min(ok ? yi : 1e300), y1 = wave_max(ok ? yi : -1e300);
     const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
-    const double R = 0.5 * sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+    // radius: the farthest particle from the box centre (at most the half
+    // diagonal; a tighter circle opens fewer nodes)
+    const double R = sqrt(wave_max(ok ? (xi - cx) * (xi - cx) + (yi - cy) * (yi - cy) : 0.0));
     if (ok) {
         fx[t] = (float)(xi - cx);
         fy[t] = (float)(yi - cy);
@@ -411,7 +413,16 @@ __device__ void node_from_children(int32_t p, const int32_t *__restrict__ child0
     const double4 b0 = nbb[c[0]], b1 = nbb[c[1]];
     const double4 b = make_double4(fmin(b0.x, b1.x), fmax(b0.y, b1.y), fmin(b0.z, b1.z), fmax(b0.w, b1.w));
     const double cx = 0.5 * (b.x + b.y), cy = 0.5 * (b.z + b.w);
-    const double R = 0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z));
+    // radius: the smaller of two bounds on the particles' distance from the
+    // centre, the half diagonal and max over the children of |c - c_C| + R_C;
+    // both keep |delta|, rho <= 1
+    double Rc = 0.0;
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        const double ex = ncx[c[ch]] - cx, ey = ncy[c[ch]] - cy;
+        Rc = fmax(Rc, sqrt(ex * ex + ey * ey) + nr[c[ch]]);
+    }
+    const double R = fmin(0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z)), Rc);
     const double Rs = fmax(R, kTiny);
     double ar[ORDER], ai[ORDER];
 #pragma unroll
