@@ -78,6 +78,38 @@ def test_tree_biot_savart_vs_direct_oracle(cloud40k, prec, order, tol):
     assert np.max(np.abs(d - dr) / dr) < dtol
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 200, 1000, 5000])
+@pytest.mark.parametrize("prec", [VX_PREC_F64, VX_PREC_MIXED])
+@pytest.mark.parametrize("exact", [0, 1])
+def test_tree_small_sizes_vs_direct_oracle(ref, n, prec, exact):
+    """Trees of one bucket, of a few buckets with no node above the cut
+    nodes, and with partial batches of far nodes: velocities and nearest-3
+    distances against the FP64 direct sums, with and without the exact
+    nearest-3 search forced for every target."""
+    rng = np.random.default_rng(100 + n)
+    if n >= 1000:
+        x, y, g = wake_cloud(n, 5)
+    else:
+        x, y = rng.uniform(-1.0, 1.0, n), rng.uniform(-1.0, 1.0, n)
+        g = rng.uniform(-0.004, 0.004, n)
+    s = (n + 31) // 32 * 32
+    X, Y, G = tail(x, y, g, s)
+    eps = 0.004
+    ur, vr, dr = ref.biot_savart(X, Y, G, eps * eps, (1.0, 0.0))
+    pan = geo.circle(200, 0.5)
+    c = Case(make_params(pan, precision=prec, ve_size=eps, flags=exact), pan)
+    c.set_state(X, Y, G)
+    c.tree_biot_savart(s, (1.0, 0.0))
+    u, v, d = c.velocity(s)
+    tol = 2e-4 if prec == VX_PREC_F64 else 3e-4
+    scale = max(np.max(np.abs(ur[:n] - 1.0)), np.max(np.abs(vr[:n])), 1e-12)
+    assert np.max(np.abs(u[:n] - ur[:n])) <= tol * scale + 1e-12
+    assert np.max(np.abs(v[:n] - vr[:n])) <= tol * scale + 1e-12
+    if n >= 4:  # the three nearest lie among the vortices, not the far tail
+        dtol = 1e-13 if (prec == VX_PREC_F64 or exact) else 1e-4
+        assert np.max(np.abs(d[:n] - dr[:n]) / dr[:n]) < dtol
+
+
 @pytest.mark.parametrize("prec", [VX_PREC_F64, VX_PREC_MIXED])
 def test_exact_nearest_search_for_every_target(cloud40k, prec):
     """The exact nearest-3 search (otherwise run only where the near field
