@@ -109,6 +109,34 @@ int vx_case_create(const vx_params *prm, const vx_panel *panels, int64_t P, vx_c
     for (int k = 0; k < 13; ++k) *f[k] = b + (size_t)k * P;
     c->pan.rp = (int32_t *)((char *)c->pan_block + sizeof(double) * 13 * P);
     c->pan.P = P;
+    // birth RHS segments: runs of at most 128 consecutive panels
+    {
+        const int64_t K = (P + 127) / 128;
+        std::vector<vx::RhsSeg> segs((size_t)K);
+        std::vector<int32_t> seg_of((size_t)P);
+        for (int64_t sg = 0; sg < K; ++sg) {
+            const int64_t f = sg * P / K, e = (sg + 1) * P / K;
+            double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+            for (int64_t i = f; i < e; ++i) {
+                const vx_panel &p = panels[i];
+                x0 = std::min({x0, p.left[0], p.right[0]}); x1 = std::max({x1, p.left[0], p.right[0]});
+                y0 = std::min({y0, p.left[1], p.right[1]}); y1 = std::max({y1, p.left[1], p.right[1]});
+            }
+            const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
+            double r = 0.0;
+            for (int64_t i = f; i < e; ++i) {
+                const vx_panel &p = panels[i];
+                r = std::max({r, std::hypot(p.left[0] - cx, p.left[1] - cy), std::hypot(p.right[0] - cx, p.right[1] - cy)});
+                seg_of[(size_t)i] = (int32_t)sg;
+            }
+            segs[(size_t)sg] = vx::RhsSeg{cx, cy, r, (int32_t)f, (int32_t)(e - f)};
+        }
+        VX_CHECK(hipMalloc((void **)&c->segs, sizeof(vx::RhsSeg) * K + sizeof(int32_t) * P));
+        c->seg_of = (int32_t *)(c->segs + K);
+        VX_CHECK(hipMemcpy(c->segs, segs.data(), sizeof(vx::RhsSeg) * K, hipMemcpyHostToDevice));
+        VX_CHECK(hipMemcpy(c->seg_of, seg_of.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice));
+        c->nseg = K;
+    }
     if (reserve(c, c->prm.inc_step > 0 ? c->prm.inc_step : 8192)) {
         vx_case_destroy(c);
         return 1;
@@ -131,6 +159,7 @@ int vx_case_destroy(vx_case *c) {
     c->cub_tmp.release(); c->gridp.release(); c->dcount.release();
     tree_release(c);
     if (c->pan_block) (void)hipFree(c->pan_block);
+    if (c->segs) (void)hipFree(c->segs);
     if (c->h_count) (void)hipHostFree(c->h_count);
     if (c->h_red) (void)hipHostFree(c->h_red);
     for (auto &e : c->ev)

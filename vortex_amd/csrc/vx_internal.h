@@ -44,6 +44,14 @@ struct PanelsSoA {
     int64_t P;
 };
 
+// A run of consecutive panels for the birth right-hand side: centre and
+// radius of a circle holding its panels (the box centre, the farthest
+// endpoint), first panel and count.
+struct RhsSeg {
+    double cx, cy, r;
+    int32_t first, count;
+};
+
 // Growable device buffer.  Growth is geometric (at least 1.25x), so sizes
 // that creep up from step to step reallocate O(log) times, not every step.
 template <typename T>
@@ -92,6 +100,9 @@ struct vx_case {
     hipStream_t stream = nullptr;
     vx::PanelsSoA pan{};
     void *pan_block = nullptr;
+    vx::RhsSeg *segs = nullptr;                       // birth RHS segments (device)
+    int32_t *seg_of = nullptr;                        // segment of each panel (device)
+    int64_t nseg = 0;
     int64_t P = 0;
     double4 wall_box{};             // birth-point box grown by 5 max(length)
     // panel system

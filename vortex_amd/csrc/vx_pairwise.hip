@@ -254,17 +254,23 @@ __device__ __forceinline__ float vx_atan2f(float y, float x) {
 // the same angle difference.  This is the panel average of the point
 // vortices' conjugate velocity g/(2 pi i (z - z_j)), times 2 pi |d|^2.
 //
-// Sources far from the profile are summed as one local expansion instead.
-// The panels lie in the profile box, centre c, half-diagonal R_b.  For
-// |z_j - c| >= 2 R_b and z on a panel,
-//   (u - i v)(z) = sum_k L_k (z - c)^k,  L_k = (i / 2 pi) S_k,
+// Sources far from the profile are summed as local expansions instead.
+// The panels lie in the profile box, centre c, half-diagonal R_b, and in
+// segments of at most 128 consecutive panels, each inside a circle of centre
+// c_s and radius R_s.  For sources with |z_j - c| >= 2 R_b (global) and for
+// z on a panel,
+//   (u - i v)(z) = (i / 2 pi) sum_k S_k (z - c)^k,
 //   S_k = sum_j g_j w_j^{k+1},  w_j = 1 / (z_j - c),
 // truncated after kLoc terms, which leaves at most 2^{1-kLoc} of the far sum.
-// Its panel average is exact:
-//   mean over [z_l, z_r] of (z - c)^k = Q_k / (k + 1),
-//   Q_k = (a^{k+1} - b^{k+1}) / (a - b) = a Q_{k-1} + b^k,  a = z_r - c, b = z_l - c.
-// The sources closer than 2 R_b (with g != 0) are listed in index order and
-// summed pair by pair, split over chunks of kRhsChunk list entries.
+// It is shifted exactly to each segment centre (a polynomial of degree
+// kLoc - 1 re-expanded: sum_{k>=m} S_k C(k, m) (c_s - c)^{k-m}).  Sources
+// nearer than 2 R_b but at least 2 R_s from c_s add their own terms
+// g_j w_{s,j}^{m+1} about c_s (the same bound).  The rest, nearer than 2 R_s,
+// are summed pair by pair for that segment's panels.  The panel average of
+// the segment's expansion is exact:
+//   mean over [z_l, z_r] of (z - c_s)^k = Q_k / (k + 1),
+//   Q_k = (a^{k+1} - b^{k+1}) / (a - b) = a Q_{k-1} + b^k,  a = z_r - c_s, b = z_l - c_s.
+// The sources nearer than 2 R_b (with g != 0) are listed in index order.
 constexpr int kLoc = 48;
 constexpr int kRhsChunk = 512;
 
@@ -277,46 +283,58 @@ struct RhsNear {
     }
 };
 
+constexpr int kSegTile = 128;  // panels per segment, threads per pair work-group
+
+// One work-group per (segment, chunk of kRhsChunk listed sources), a thread
+// per panel of the segment.  Each tile of sources is staged with a flag for
+// |z_j - c_s| < 2 R_s (FP64 positions, as the expansion kernel tests), the
+// flagged ones compacted in LDS and summed pair by pair.
 template <typename T>
-__global__ __launch_bounds__(kTile) void k_right_part_partial(
-    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, const int32_t *__restrict__ list,
-    const int32_t *__restrict__ nlist, PanelsSoA pan, double *__restrict__ part) {
-    __shared__ Src4<T> ls[kTile];
-    const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+__global__ __launch_bounds__(kSegTile) void k_right_part_partial(
+    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, const double *__restrict__ x,
+    const double *__restrict__ y, const int32_t *__restrict__ list, const int32_t *__restrict__ nlist,
+    PanelsSoA pan, const RhsSeg *__restrict__ segs, double *__restrict__ part) {
+    __shared__ Src4<T> ls[kSegTile];
+    __shared__ int wcount[2];
+    const RhsSeg S = segs[blockIdx.x];
     const int64_t c = blockIdx.y;
     const int64_t n_src = *nlist;
     const int64_t lo = c * kRhsChunk, hi = min(n_src, lo + kRhsChunk);
-    const bool live = i < pan.P;
-    const T l0 = live ? (T)pan.lx[i] : T(0), l1 = live ? (T)pan.ly[i] : T(0);
-    const T r0 = live ? (T)pan.rx[i] : T(1), r1 = live ? (T)pan.ry[i] : T(0);
+    const bool live = (int)threadIdx.x < S.count;
+    const int64_t i = S.first + (live ? threadIdx.x : 0);
+    const T l0 = (T)pan.lx[i], l1 = (T)pan.ly[i];
+    const T r0 = (T)pan.rx[i], r1 = (T)pan.ry[i];
     const T d0 = r0 - l0, d1 = r1 - l1;
     const T dd2 = d0 * d0 + d1 * d1;
+    const double rn2 = 4.0 * S.r * S.r;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     T y0 = 0, y1 = 0;
     double a0 = 0.0, a1 = 0.0;
-    for (int64_t base = lo; base < hi; base += kTile) {
-        __syncthreads();
-        {
-            const int64_t j = base + threadIdx.x;
-            Src4<T> q;
-            if (j < hi) {
-                const int32_t k = list[j];
-                q.x = sx[k];
-                q.y = sy[k];
-                q.g = sg[k];
-            } else {
-                q.x = Arith<T>::kFar;
-                q.y = Arith<T>::kFar;
-                q.g = T(0);
-            }
+    for (int64_t base = lo; base < hi; base += kSegTile) {
+        const int64_t j = base + threadIdx.x;
+        bool near = false;
+        Src4<T> q;
+        if (j < hi) {
+            const int32_t k = list[j];
+            const double ex = x[k] - S.cx, ey = y[k] - S.cy;
+            near = ex * ex + ey * ey < rn2;
+            q.x = sx[k];
+            q.y = sy[k];
+            q.g = sg[k];
             q.pad = T(0);
-            ls[threadIdx.x] = q;
         }
+        const uint64_t m = __ballot(near);
+        __syncthreads();  // the previous tile is read
+        if (lane == 0) wcount[wv] = __popcll(m);
         __syncthreads();
-        const int cnt = (int)min((int64_t)kTile, hi - base);
+        const int pos = (wv ? wcount[0] : 0) + __popcll(m & ((1ull << lane) - 1));
+        if (near) ls[pos] = q;
+        __syncthreads();
+        const int cnt = wcount[0] + wcount[1];
         for (int jj = 0; jj < cnt; ++jj) {
-            const Src4<T> s = ls[jj];
-            const T s00 = l0 - s.x, s01 = l1 - s.y;
-            const T s10 = r0 - s.x, s11 = r1 - s.y;
+            const Src4<T> s_ = ls[jj];
+            const T s00 = l0 - s_.x, s01 = l1 - s_.y;
+            const T s10 = r0 - s_.x, s11 = r1 - s_.y;
             const T z = d0 * s01 - d1 * s00;
             const T A = s00 * d0 + s01 * d1;
             const T B = s10 * d0 + s11 * d1;
@@ -329,8 +347,8 @@ __global__ __launch_bounds__(kTile) void k_right_part_partial(
                 alpha = atan(A / z) - atan(B / z);
                 beta = 0.5 * log((s10 * s10 + s11 * s11) / (s00 * s00 + s01 * s01));
             }
-            y0 += s.g * (alpha * d0 - beta * d1);
-            y1 += s.g * (alpha * d1 + beta * d0);
+            y0 += s_.g * (alpha * d0 - beta * d1);
+            y1 += s_.g * (alpha * d1 + beta * d0);
         }
         if constexpr (sizeof(T) == 4) {
             a0 += (double)y0;
@@ -409,6 +427,90 @@ __global__ __launch_bounds__(256) void k_rhs_far_final(const double *__restrict_
     if (threadIdx.x == 0) S[t] = r[0];
 }
 
+// Listed sources at least 2 R_s from a segment's centre: their terms
+// g w^{m+1}, w = 1 / (z_j - c_s), for m < kLoc, per work-group (blockIdx.y
+// of kSegBlocks for segment blockIdx.x, grid-stride over the list), summed in
+// a fixed order; spart[((s * kSegBlocks + blk) * kLoc + m) * 2 + {0,1}].
+constexpr int kSegBlocks = 32;
+
+__global__ __launch_bounds__(256) void k_rhs_seg_partial(const double *__restrict__ x, const double *__restrict__ y,
+                                                         const double *__restrict__ g,
+                                                         const int32_t *__restrict__ list,
+                                                         const int32_t *__restrict__ nlist,
+                                                         const RhsSeg *__restrict__ segs,
+                                                         double *__restrict__ spart) {
+    __shared__ double red[16][256];
+    const RhsSeg S = segs[blockIdx.x];
+    const double rn2 = 4.0 * S.r * S.r;
+    const int64_t n = *nlist;
+    double sr[kLoc], si[kLoc];
+#pragma unroll
+    for (int k = 0; k < kLoc; ++k) sr[k] = si[k] = 0.0;
+    for (int64_t t = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.y * blockDim.x) {
+        const int32_t j = list[t];
+        const double dx = x[j] - S.cx, dy = y[j] - S.cy;
+        const double q2 = dx * dx + dy * dy;
+        if (!(q2 >= rn2)) continue;
+        const double wr = dx / q2, wi = -dy / q2;
+        double pr = g[j] * wr, pi = g[j] * wi;
+#pragma unroll
+        for (int k = 0; k < kLoc; ++k) {
+            sr[k] += pr;
+            si[k] += pi;
+            const double npr = pr * wr - pi * wi;
+            pi = pr * wi + pi * wr;
+            pr = npr;
+        }
+    }
+    const int64_t blk = (int64_t)blockIdx.x * gridDim.y + blockIdx.y;
+#pragma unroll
+    for (int k0 = 0; k0 < kLoc; k0 += 8) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            red[2 * q][threadIdx.x] = sr[k0 + q];
+            red[2 * q + 1][threadIdx.x] = si[k0 + q];
+        }
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st)
+                for (int q = 0; q < 16; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x < 16) spart[(blk * kLoc + k0) * 2 + threadIdx.x] = red[threadIdx.x][0];
+    }
+}
+
+// Each segment's coefficients about c_s: the global S shifted exactly,
+//   sum_{k >= m} S_k C(k, m) (c_s - c)^{k - m},
+// plus its own sources' partials (fixed order).  Thread m < kLoc.
+__global__ void k_rhs_seg_final(const double *__restrict__ spart, const double *__restrict__ S, double cx,
+                                double cy, const RhsSeg *__restrict__ segs, double *__restrict__ Ss) {
+    const int m = threadIdx.x;
+    if (m >= kLoc) return;
+    const RhsSeg G = segs[blockIdx.x];
+    const double er = G.cx - cx, ei = G.cy - cy;  // c_s - c
+    double ar = 0.0, ai = 0.0;
+    double bin = 1.0;              // C(k, m)
+    double pr = 1.0, pi = 0.0;     // (c_s - c)^{k - m}
+    for (int k = m; k < kLoc; ++k) {
+        const double tr = pr * bin, ti = pi * bin;
+        ar += S[2 * k] * tr - S[2 * k + 1] * ti;
+        ai += S[2 * k] * ti + S[2 * k + 1] * tr;
+        bin = bin * (double)(k + 1) / (double)(k + 1 - m);
+        const double npr = pr * er - pi * ei;
+        pi = pr * ei + pi * er;
+        pr = npr;
+    }
+    for (int b = 0; b < kSegBlocks; ++b) {
+        const double *q = spart + (((int64_t)blockIdx.x * kSegBlocks + b) * kLoc + m) * 2;
+        ar += q[0];
+        ai += q[1];
+    }
+    Ss[((int64_t)blockIdx.x * kLoc + m) * 2] = ar;
+    Ss[((int64_t)blockIdx.x * kLoc + m) * 2 + 1] = ai;
+}
+
 // Sum of the chunk partials part[(c * 2 + q) * m + i] for one output i per
 // work-group: a strided pass per thread, then a tree in LDS (fixed order).
 __device__ __forceinline__ void chunk_sum2(const double *__restrict__ part, int64_t nchunk, int64_t m, int64_t i,
@@ -434,20 +536,24 @@ __device__ __forceinline__ void chunk_sum2(const double *__restrict__ part, int6
 }
 
 // One work-group per panel: the near chunk partials, plus the panel average
-// of the far-field expansion (thread 0).
+// of its segment's expansion (thread 0).
 __global__ __launch_bounds__(kTile) void k_right_part_final(const double *__restrict__ part, int64_t nchunk,
-                                                            PanelsSoA pan, const double *__restrict__ S, double cx,
-                                                            double cy, double vinf_x, double vinf_y,
-                                                            double *__restrict__ R) {
+                                                            PanelsSoA pan, const double *__restrict__ Ss,
+                                                            const RhsSeg *__restrict__ segs,
+                                                            const int32_t *__restrict__ seg_of, double vinf_x,
+                                                            double vinf_y, double *__restrict__ R) {
     const int64_t i = blockIdx.x;
     double y0, y1;
     chunk_sum2(part, nchunk, pan.P, i, y0, y1);
     if (threadIdx.x != 0) return;
+    const int32_t sg = seg_of[i];
+    const double cx = segs[sg].cx, cy = segs[sg].cy;
+    const double *S = Ss + (int64_t)sg * 2 * kLoc;
     const double d0 = pan.rx[i] - pan.lx[i], d1 = pan.ry[i] - pan.ly[i];
     const double mnog = 1 / (2 * kPi * (d0 * d0 + d1 * d1));
     // W = sum_k (i S_k / 2 pi) Q_k / (k + 1), u = Re W, v = -Im W
-    const double ar = pan.rx[i] - cx, ai = pan.ry[i] - cy;  // a = z_r - c
-    const double br = pan.lx[i] - cx, bi = pan.ly[i] - cy;  // b = z_l - c
+    const double ar = pan.rx[i] - cx, ai = pan.ry[i] - cy;  // a = z_r - c_s
+    const double br = pan.lx[i] - cx, bi = pan.ly[i] - cy;  // b = z_l - c_s
     double qr = 1.0, qi = 0.0;                               // Q_0
     double bkr = 1.0, bki = 0.0;                             // b^k
     double wr = 0.0, wi = 0.0;
@@ -580,22 +686,23 @@ int launch_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src) {
 }
 
 int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
-    const int64_t P = c->P;
+    const int64_t P = c->P, K = c->nseg;
     hipStream_t st = c->stream;
-    const int64_t tb = (P + kTile - 1) / kTile;
     const int64_t nchunk = n_src > 0 ? (n_src + kRhsChunk - 1) / kRhsChunk : 1;
     constexpr int64_t kFarBlocks = 512;
+    const size_t ns_far = (size_t)(2 * kLoc * kFarBlocks), ns_glob = 2 * kLoc,
+                 ns_seg = (size_t)(2 * kLoc * kSegBlocks * K), ns_segS = (size_t)(2 * kLoc * K);
     if (c->partial.ensure((size_t)(2 * nchunk * P), st) || c->rp.ensure((size_t)P, st) ||
         c->lists.ensure((size_t)(n_src > 0 ? n_src : 1), st) || c->dcount.ensure(16, st) ||
-        c->rhs_s.ensure((size_t)(2 * kLoc * (kFarBlocks + 1)), st))
+        c->rhs_s.ensure(ns_far + ns_glob + ns_seg + ns_segS, st))
         return 1;
     const vx_params &p = c->prm;
     const double cx = 0.5 * (p.px_min + p.px_max), cy = 0.5 * (p.py_min + p.py_max);
     const double wx = p.px_max - p.px_min, wy = p.py_max - p.py_min;
     const double rnear2 = wx * wx + wy * wy;  // (2 R_b)^2
     int32_t *nlist = (int32_t *)(c->dcount.p + 7);
-    double *S = c->rhs_s.p + 2 * kLoc * kFarBlocks;
-    // near sources, in index order
+    double *S = c->rhs_s.p + ns_far, *spart = S + ns_glob, *Ss = spart + ns_seg;
+    // sources nearer than 2 R_b, in index order
     {
         const RhsNear pred{c->x.p, c->y.p, c->g.p, cx, cy, rnear2};
         hipcub::CountingInputIterator<int32_t> it(0);
@@ -604,21 +711,28 @@ int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
         if (c->cub_tmp.ensure(bytes, st)) return 1;
         VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, c->lists.p, nlist, n_src, pred, st));
     }
+    // pairs within 2 R_s of each segment
     if (mixed(c)) {
         if (launch_pack_f32(c, n_src)) return 1;
-        hipLaunchKernelGGL(k_right_part_partial<float>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0, st,
-                           c->xf.p, c->yf.p, c->gf.p, c->lists.p, nlist, c->pan, c->partial.p);
+        hipLaunchKernelGGL(k_right_part_partial<float>, dim3((unsigned)K, (unsigned)nchunk), dim3(kSegTile), 0, st,
+                           c->xf.p, c->yf.p, c->gf.p, c->x.p, c->y.p, c->lists.p, nlist, c->pan, c->segs,
+                           c->partial.p);
     } else {
-        hipLaunchKernelGGL(k_right_part_partial<double>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0, st,
-                           c->x.p, c->y.p, c->g.p, c->lists.p, nlist, c->pan, c->partial.p);
+        hipLaunchKernelGGL(k_right_part_partial<double>, dim3((unsigned)K, (unsigned)nchunk), dim3(kSegTile), 0, st,
+                           c->x.p, c->y.p, c->g.p, c->x.p, c->y.p, c->lists.p, nlist, c->pan, c->segs,
+                           c->partial.p);
     }
-    // far sources: the local expansion about the profile box centre
+    // the global expansion (sources beyond 2 R_b), the segments' own
+    // sources, and each segment's total about c_s
     hipLaunchKernelGGL(k_rhs_far_partial, dim3((unsigned)kFarBlocks), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n_src,
                        cx, cy, rnear2, c->rhs_s.p);
     hipLaunchKernelGGL(k_rhs_far_final, dim3(2 * kLoc), dim3(256), 0, st, c->rhs_s.p, kFarBlocks, S);
+    hipLaunchKernelGGL(k_rhs_seg_partial, dim3((unsigned)K, (unsigned)kSegBlocks), dim3(256), 0, st, c->x.p, c->y.p,
+                       c->g.p, c->lists.p, nlist, c->segs, spart);
+    hipLaunchKernelGGL(k_rhs_seg_final, dim3((unsigned)K), dim3(64), 0, st, spart, S, cx, cy, c->segs, Ss);
     VX_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)P), dim3(kTile), 0, st, c->partial.p, nchunk, c->pan, S, cx,
-                       cy, vinf_x, vinf_y, c->rp.p);
+    hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)P), dim3(kTile), 0, st, c->partial.p, nchunk, c->pan, Ss,
+                       c->segs, c->seg_of, vinf_x, vinf_y, c->rp.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
