@@ -40,6 +40,14 @@ namespace vx {
 constexpr int kLeaf = 64;
 constexpr double kTiny = 1e-300;
 
+// Binomial coefficient C(n, k); folded to a constant wherever n and k are.
+__host__ __device__ constexpr double binom(int n, int k) {
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (double)(n - k + i) / (double)i;
+    return r;
+}
+
+
 // Traversal record of a node, one 32-byte load: centre, radius and either
 // the children (a = child0 >= 0, b = child1) or, for a bucket, its particles
 // (a = -1 - first, b = count).
@@ -51,7 +59,7 @@ struct NodeRec {
 struct TreeBufs {
     DBuf<uint32_t> keys, keys_alt;
     DBuf<int32_t> idx, perm, incl, child0, child1, first, count, parent, fb;
-    DBuf<int32_t> rlo, rhi, rleft, rright, bstart, bscan, upper, uscan, visits;
+    DBuf<int32_t> rlo, rhi, rleft, rright, bstart, bscan, upper, uscan, toplist;
     DBuf<int32_t> gcs, gcs2, gflag, gscan, gfirst;  // target groups
     DBuf<int32_t> pcount;                          // included particles per block
     DBuf<int32_t> arr;                             // cut nodes, count at arr[0]
@@ -400,85 +408,12 @@ __global__ __launch_bounds__(256) void k_bucket_build(const double *__restrict__
     }
 }
 
-// Box union and M2M of node p from its two children (both complete):
-//   b^P_k = sum_{m<=k} C(k,m) delta^{k-m} rho^m b^C_m,
-//   delta = (c_C - c_P)/R_P, rho = R_C/R_P  (|delta|, rho <= 1),
-// evaluated without binomials or divisions as ORDER-1 passes of
-// w_k += delta w_{k-1} (k descending within a pass) on w_m = rho^m b^C_m.
-template <int ORDER>
-__device__ void node_from_children(int32_t p, const int32_t *__restrict__ child0, const int32_t *__restrict__ child1,
-                                   double *__restrict__ ncx, double *__restrict__ ncy, double *__restrict__ nr,
-                                   double4 *__restrict__ nbb, double2 *__restrict__ cd, float2 *__restrict__ cf) {
-    const int32_t c[2] = {child0[p], child1[p]};
-    const double4 b0 = nbb[c[0]], b1 = nbb[c[1]];
-    const double4 b = make_double4(fmin(b0.x, b1.x), fmax(b0.y, b1.y), fmin(b0.z, b1.z), fmax(b0.w, b1.w));
-    const double cx = 0.5 * (b.x + b.y), cy = 0.5 * (b.z + b.w);
-    // radius: the smaller of two bounds on the particles' distance from the
-    // centre, the half diagonal and max over the children of |c - c_C| + R_C;
-    // both keep |delta|, rho <= 1
-    double Rc = 0.0;
-#pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
-        const double ex = ncx[c[ch]] - cx, ey = ncy[c[ch]] - cy;
-        Rc = fmax(Rc, sqrt(ex * ex + ey * ey) + nr[c[ch]]);
-    }
-    const double R = fmin(0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z)), Rc);
-    const double Rs = fmax(R, kTiny);
-    double ar[ORDER], ai[ORDER];
-#pragma unroll
-    for (int q = 0; q < ORDER; ++q) ar[q] = ai[q] = 0.0;
-#pragma unroll 1
-    for (int ch = 0; ch < 2; ++ch) {
-        const int32_t cc = c[ch];
-        const double dr = (ncx[cc] - cx) / Rs, di = (ncy[cc] - cy) / Rs;
-        const double rho = nr[cc] / Rs;
-        const double2 *bc = cd + (int64_t)cc * ORDER;
-        double wr[ORDER], wi[ORDER];
-        double rk = 1.0;
-#pragma unroll
-        for (int q = 0; q < ORDER; ++q) {
-            const double2 bm = bc[q];
-            wr[q] = bm.x * rk;
-            wi[q] = bm.y * rk;
-            rk *= rho;
-        }
-#pragma unroll
-        for (int pass = 0; pass < ORDER - 1; ++pass) {
-#pragma unroll
-            for (int k = ORDER - 1; k > pass; --k) {
-                const double nwr = wr[k] + (dr * wr[k - 1] - di * wi[k - 1]);
-                wi[k] = wi[k] + (dr * wi[k - 1] + di * wr[k - 1]);
-                wr[k] = nwr;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < ORDER; ++q) {
-            ar[q] += wr[q];
-            ai[q] += wi[q];
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < ORDER; ++q) {
-        cd[(int64_t)p * ORDER + q] = make_double2(ar[q], ai[q]);
-        cf[(int64_t)p * ORDER + q] = make_float2((float)ar[q], (float)ai[q]);
-    }
-    ncx[p] = cx;
-    ncy[p] = cy;
-    nr[p] = R;
-    nbb[p] = b;
-}
-
-// Bottom-up pass in two kernels: the second thread to reach a node builds it
-// and continues upward; the first one stops.  No thread waits on another, so
-// each pass always completes.  All the arrivals at a node come from one
-// work-group, so work-group-scope fences order them: they publish a node
-// before its parent's counter moves and order the reads of the sibling after
-// it (the work-group shares one CU and its L1).
+// Bottom-up pass (above the buckets) in two kernels, each building levels
+// deepest first with a work-group barrier between them (build_levels below).
 //   Cut nodes hold at most kCut buckets and have a parent (or are the root)
 //   with more; their subtrees partition the buckets.
-//   k_upper_local: a work-group per cut node, a thread per bucket, builds the
-//     cut node's subtree.
-//   k_upper_top: one work-group, a thread per cut node, builds the nodes above.
+//   k_upper_local: a work-group per cut node builds the cut node's subtree.
+//   k_upper_top: one work-group builds the nodes above the cut nodes.
 constexpr int kCut = 256;
 
 __device__ __forceinline__ int64_t bucket_lo(int32_t u, int64_t nb, const int32_t *__restrict__ first,
@@ -504,55 +439,200 @@ struct CutNode {
     }
 };
 
-__device__ __forceinline__ bool arrive_wg(int32_t *visits, int32_t p) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if (__hip_atomic_fetch_add(&visits[p], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    return true;
+// Box union and M2M of node p from its two children (both complete), by a
+// group of ORDER lanes (lane k writes coefficient k).  Radius: the smaller of
+// two bounds on the particles' distance from the centre, the half diagonal
+// and max over the children of |c - c_C| + R_C; both keep |delta|, rho <= 1.
+//   b^P_k = sum over the two children of sum_{m<=k} C(k,m) delta^{k-m} w_m,
+//   w_m = rho^m b^C_m,  delta = (c_C - c_P)/R_P,  rho = R_C/R_P,
+// as a Horner sum in delta over m.  w_m is broadcast through the group's LDS
+// slots ws[0, ORDER); bin[m][k] = C(k, m) (0 for m > k).  p < 0: no node.
+template <int ORDER>
+__device__ __forceinline__ void m2m_coop(int32_t p, int k, const int32_t *__restrict__ child0,
+                                         const int32_t *__restrict__ child1, double *__restrict__ ncx,
+                                         double *__restrict__ ncy, double *__restrict__ nr,
+                                         double4 *__restrict__ nbb, double2 *__restrict__ cd,
+                                         float2 *__restrict__ cf, double2 *ws, const double (*bin)[ORDER]) {
+    const bool on = p >= 0;
+    const int32_t c[2] = {on ? child0[p] : 0, on ? child1[p] : 0};
+    double4 b = make_double4(0.0, 0.0, 0.0, 0.0);
+    double cx = 0.0, cy = 0.0, R = 0.0, Rs = 1.0;
+    if (on) {
+        const double4 b0 = nbb[c[0]], b1 = nbb[c[1]];
+        b = make_double4(fmin(b0.x, b1.x), fmax(b0.y, b1.y), fmin(b0.z, b1.z), fmax(b0.w, b1.w));
+        cx = 0.5 * (b.x + b.y);
+        cy = 0.5 * (b.z + b.w);
+        double Rc = 0.0;
+        for (int ch = 0; ch < 2; ++ch) {
+            const double ex = ncx[c[ch]] - cx, ey = ncy[c[ch]] - cy;
+            Rc = fmax(Rc, sqrt(ex * ex + ey * ey) + nr[c[ch]]);
+        }
+        R = fmin(0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z)), Rc);
+        Rs = fmax(R, kTiny);
+    }
+    double accr = 0.0, acci = 0.0;
+#pragma unroll 1
+    for (int ch = 0; ch < 2; ++ch) {
+        double dr = 0.0, di = 0.0, wr = 0.0, wi = 0.0;
+        if (on) {
+            const int32_t cc = c[ch];
+            dr = (ncx[cc] - cx) / Rs;
+            di = (ncy[cc] - cy) / Rs;
+            const double rho = nr[cc] / Rs;
+            double rk = 1.0, rb = rho;  // rho^k by binary powers
+#pragma unroll
+            for (int e = 0; (1 << e) < ORDER; ++e) {
+                if ((k >> e) & 1) rk *= rb;
+                rb *= rb;
+            }
+            const double2 bm = cd[(int64_t)cc * ORDER + k];
+            wr = bm.x * rk;
+            wi = bm.y * rk;
+        }
+        __builtin_amdgcn_wave_barrier();
+        ws[k] = make_double2(wr, wi);
+        __builtin_amdgcn_wave_barrier();
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int m = 0; m < ORDER; ++m) {
+            const double2 w = ws[m];
+            const double cb = bin[m][k];
+            const double nar = ar * dr - ai * di + cb * w.x;
+            const double nai = ar * di + ai * dr + cb * w.y;
+            ar = m <= k ? nar : ar;
+            ai = m <= k ? nai : ai;
+        }
+        accr += ar;
+        acci += ai;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (!on) return;
+    cd[(int64_t)p * ORDER + k] = make_double2(accr, acci);
+    cf[(int64_t)p * ORDER + k] = make_float2((float)accr, (float)acci);
+    if (k == 0) {
+        ncx[p] = cx;
+        ncy[p] = cy;
+        nr[p] = R;
+        nbb[p] = b;
+    }
+}
+
+// Builds the internal nodes below root r (r included) that desc(u) accepts,
+// with NW waves: the nodes are listed by depth (a breadth-first walk into
+// accepted children, into lst), then built deepest level first, a node per
+// group of ORDER lanes, with a work-group barrier between levels.  Node
+// depths in the radix tree are below 64 (each level lengthens the common
+// prefix of the 64-bit index-extended keys), so kLevCap level starts suffice.
+constexpr int kLevCap = 72;
+
+template <int ORDER, int NW, typename Desc>
+__device__ void build_levels(int32_t r, Desc desc, int32_t *lst, int32_t *lev, int *ntot,
+                             const int32_t *__restrict__ child0, const int32_t *__restrict__ child1,
+                             double *__restrict__ ncx, double *__restrict__ ncy, double *__restrict__ nr,
+                             double4 *__restrict__ nbb, double2 *__restrict__ cd, float2 *__restrict__ cf,
+                             double2 *ws, const double (*bin)[ORDER]) {
+    constexpr int G = 64 / ORDER;  // nodes per wave
+    const int wave = threadIdx.x >> 6, gi = (threadIdx.x & 63) / ORDER, k = threadIdx.x % ORDER;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        lst[0] = r;
+        lev[0] = 0;
+        lev[1] = 1;
+        *ntot = 1;
+    }
+    __syncthreads();
+    int nlev = 1;
+    for (int d = 0; d + 2 < kLevCap; ++d) {
+        const int s0 = lev[d], s1 = lev[d + 1];
+        for (int t = s0 + threadIdx.x; t < s1; t += NW * 64) {
+            const int32_t u = lst[t];
+            const int32_t ch[2] = {child0[u], child1[u]};
+            for (int q = 0; q < 2; ++q)
+                if (desc(ch[q])) lst[atomicAdd(ntot, 1)] = ch[q];
+        }
+        __syncthreads();
+        const int e = *ntot;
+        if (e == s1) break;
+        if (threadIdx.x == 0) lev[d + 2] = e;
+        nlev = d + 2;
+        __syncthreads();
+    }
+    for (int d = nlev - 1; d >= 0; --d) {
+        const int s0 = lev[d], s1 = lev[d + 1];
+        for (int base = s0 + wave * G; base < s1; base += NW * G) {
+            const int idx = base + gi;
+            m2m_coop<ORDER>(idx < s1 ? lst[idx] : -1, k, child0, child1, ncx, ncy, nr, nbb, cd, cf, ws, bin);
+        }
+        __syncthreads();
+    }
 }
 
 template <int ORDER>
-__global__ __launch_bounds__(kCut) void k_upper_local(const int32_t *__restrict__ cut, const int32_t *__restrict__ ncut,
-                                                      int64_t nb, int64_t m, const int32_t *__restrict__ bscan,
-                                                      const int32_t *__restrict__ first,
-                                                      const int32_t *__restrict__ count,
-                                                      const int32_t *__restrict__ parent, int32_t *__restrict__ visits,
-                                                      const int32_t *__restrict__ child0,
-                                                      const int32_t *__restrict__ child1, double *__restrict__ ncx,
-                                                      double *__restrict__ ncy, double *__restrict__ nr,
-                                                      double4 *__restrict__ nbb, double2 *__restrict__ cd,
-                                                      float2 *__restrict__ cf) {
+__device__ void fill_binomials(double (*bin)[ORDER]) {
+    for (int t = threadIdx.x; t < ORDER * ORDER; t += blockDim.x) {
+        const int m = t / ORDER, k = t % ORDER;
+        bin[m][k] = m <= k ? binom(k, m) : 0.0;
+    }
+}
+
+struct IsInternal {
+    int64_t nb;
+    __device__ bool operator()(int32_t u) const { return u >= nb; }
+};
+
+// k_upper_local: a work-group per cut node builds its subtree (at most
+// kCut - 1 internal nodes, listed in LDS).
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_upper_local(const int32_t *__restrict__ cut, const int32_t *__restrict__ ncut,
+                                                     int64_t nb, const int32_t *__restrict__ child0,
+                                                     const int32_t *__restrict__ child1, double *__restrict__ ncx,
+                                                     double *__restrict__ ncy, double *__restrict__ nr,
+                                                     double4 *__restrict__ nbb, double2 *__restrict__ cd,
+                                                     float2 *__restrict__ cf) {
+    __shared__ int32_t lst[kCut];
+    __shared__ int32_t lev[kLevCap];
+    __shared__ int ntot;
+    __shared__ double2 ws_all[4][64];
+    __shared__ double bin[ORDER][ORDER];
+    fill_binomials<ORDER>(bin);
+    double2 *ws = ws_all[threadIdx.x >> 6] + ((threadIdx.x & 63) / ORDER) * ORDER;
     const int n = *ncut;
     for (int w = blockIdx.x; w < n; w += gridDim.x) {
         const int32_t r = cut[w];
-        if ((int64_t)threadIdx.x >= bucket_n(r, nb, m, first, count, bscan)) continue;
-        int32_t cur = (int32_t)(bucket_lo(r, nb, first, bscan) + threadIdx.x);
-        while (cur != r) {
-            const int32_t p = parent[cur];
-            if (!arrive_wg(visits, p)) break;
-            node_from_children<ORDER>(p, child0, child1, ncx, ncy, nr, nbb, cd, cf);
-            cur = p;
-        }
+        if (r < nb) continue;  // a bucket: nothing to build (the same for every thread)
+        build_levels<ORDER, 4>(r, IsInternal{nb}, lst, lev, &ntot, child0, child1, ncx, ncy, nr, nbb, cd, cf, ws,
+                               bin);
     }
 }
 
+// The nodes above the cut nodes: internal nodes holding more than kCut buckets.
+struct IsTop {
+    const int32_t *first, *count, *bscan;
+    int64_t nb, m;
+    __device__ bool operator()(int32_t u) const { return u >= nb && bucket_n(u, nb, m, first, count, bscan) > kCut; }
+};
+
+// k_upper_top: one work-group builds the nodes above the cut nodes (fewer
+// than nb, listed in lst).
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_upper_top(const int32_t *__restrict__ cut, const int32_t *__restrict__ ncut,
-                                                   const int32_t *__restrict__ parent, int32_t *__restrict__ visits,
-                                                   const int32_t *__restrict__ child0,
-                                                   const int32_t *__restrict__ child1, double *__restrict__ ncx,
-                                                   double *__restrict__ ncy, double *__restrict__ nr,
-                                                   double4 *__restrict__ nbb, double2 *__restrict__ cd,
-                                                   float2 *__restrict__ cf) {
-    const int n = *ncut;
-    for (int k = threadIdx.x; k < n; k += 256) {
-        int32_t p = parent[cut[k]];
-        while (p >= 0) {
-            if (!arrive_wg(visits, p)) break;
-            node_from_children<ORDER>(p, child0, child1, ncx, ncy, nr, nbb, cd, cf);
-            p = parent[p];
-        }
-    }
+__global__ __launch_bounds__(1024) void k_upper_top(int32_t root, int64_t nb, int64_t m,
+                                                    const int32_t *__restrict__ first,
+                                                    const int32_t *__restrict__ count,
+                                                    const int32_t *__restrict__ bscan, int32_t *__restrict__ lst,
+                                                    const int32_t *__restrict__ child0,
+                                                    const int32_t *__restrict__ child1, double *__restrict__ ncx,
+                                                    double *__restrict__ ncy, double *__restrict__ nr,
+                                                    double4 *__restrict__ nbb, double2 *__restrict__ cd,
+                                                    float2 *__restrict__ cf) {
+    __shared__ int32_t lev[kLevCap];
+    __shared__ int ntot;
+    __shared__ double2 ws_all[16][64];
+    __shared__ double bin[ORDER][ORDER];
+    const IsTop top{first, count, bscan, nb, m};
+    if (!top(root)) return;  // the root is a bucket or a cut node (the same for every thread)
+    fill_binomials<ORDER>(bin);
+    double2 *ws = ws_all[threadIdx.x >> 6] + ((threadIdx.x & 63) / ORDER) * ORDER;
+    build_levels<ORDER, 16>(root, top, lst, lev, &ntot, child0, child1, ncx, ncy, nr, nbb, cd, cf, ws, bin);
 }
 
 // Group count, bucket count and upper-node count from the ends of their scans.
@@ -746,13 +826,6 @@ __device__ __forceinline__ void block_count_add(unsigned long long a, unsigned l
         if (s[0]) atomicAdd(ga, s[0]);
         if (gb && s[1]) atomicAdd(gb, s[1]);
     }
-}
-
-// Binomial coefficient C(n, k); folded to a constant wherever n and k are.
-__host__ __device__ constexpr double binom(int n, int k) {
-    double r = 1.0;
-    for (int i = 1; i <= k; ++i) r = r * (double)(n - k + i) / (double)i;
-    return r;
 }
 
 __device__ __forceinline__ float wave_sum_f(float a) {
@@ -1643,7 +1716,7 @@ void tree_release(vx_case *c) {
     t.keys.release(); t.keys_alt.release(); t.idx.release(); t.perm.release(); t.incl.release();
     t.child0.release(); t.child1.release(); t.first.release(); t.count.release(); t.parent.release();
     t.fb.release(); t.rlo.release(); t.rhi.release(); t.rleft.release(); t.rright.release();
-    t.bstart.release(); t.bscan.release(); t.upper.release(); t.uscan.release(); t.visits.release();
+    t.bstart.release(); t.bscan.release(); t.upper.release(); t.uscan.release(); t.toplist.release();
     t.sx.release(); t.sy.release(); t.sg.release(); t.ncx.release(); t.ncy.release(); t.nr.release();
     t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release(); t.rec.release();
     t.cbox.release(); t.fpairs.release(); t.ftgt.release();
@@ -1752,11 +1825,10 @@ int tree_build(vx_case *c, int64_t s) {
     const int64_t nn = t.nnode;
     if (t.ncx.ensure(nn, st) || t.ncy.ensure(nn, st) || t.nr.ensure(nn, st) || t.nbb.ensure(nn, st) ||
         t.child0.ensure(nn, st) || t.child1.ensure(nn, st) || t.first.ensure(nn, st) || t.count.ensure(nn, st) ||
-        t.parent.ensure(nn, st) || t.visits.ensure(nn, st) || t.rec.ensure(nn, st) || t.cd.ensure(nn * order, st) ||
+        t.parent.ensure(nn, st) || t.toplist.ensure(nn, st) || t.rec.ensure(nn, st) || t.cd.ensure(nn * order, st) ||
         t.cf.ensure(nn * order, st))
         return 1;
     VX_CHECK(hipMemsetAsync(t.parent.p, 0xff, sizeof(int32_t) * nn, st));
-    VX_CHECK(hipMemsetAsync(t.visits.p, 0, sizeof(int32_t) * nn, st));
     if (m > kLeaf) {
         hipLaunchKernelGGL(k_bucket_first, dim3(gm), dim3(256), 0, st, t.bstart.p, t.bscan.p, m, t.first.p);
         hipLaunchKernelGGL(k_bucket_count, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, t.first.p, nb, m,
@@ -1786,20 +1858,20 @@ int tree_build(vx_case *c, int64_t s) {
         hipLaunchKernelGGL(k_bucket_build<32>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, nb, t.first.p,
                            t.count.p, t.fx.p, t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p,
                            t.child1.p, t.cd.p, t.cf.p);
-        hipLaunchKernelGGL(k_upper_local<32>, dim3(gc), dim3(kCut), 0, st, cut, ncut, nb, m, t.bscan.p, t.first.p,
-                           t.count.p, t.parent.p, t.visits.p, t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p,
-                           t.nbb.p, t.cd.p, t.cf.p);
-        hipLaunchKernelGGL(k_upper_top<32>, dim3(1), dim3(256), 0, st, cut, ncut, t.parent.p, t.visits.p,
-                           t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_local<32>, dim3(gc), dim3(256), 0, st, cut, ncut, nb, t.child0.p, t.child1.p,
+                           t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_top<32>, dim3(1), dim3(1024), 0, st, t.root, nb, m, t.first.p, t.count.p,
+                           t.bscan.p, t.toplist.p, t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p,
+                           t.cf.p);
     } else {
         hipLaunchKernelGGL(k_bucket_build<16>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, nb, t.first.p,
                            t.count.p, t.fx.p, t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p,
                            t.child1.p, t.cd.p, t.cf.p);
-        hipLaunchKernelGGL(k_upper_local<16>, dim3(gc), dim3(kCut), 0, st, cut, ncut, nb, m, t.bscan.p, t.first.p,
-                           t.count.p, t.parent.p, t.visits.p, t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p,
-                           t.nbb.p, t.cd.p, t.cf.p);
-        hipLaunchKernelGGL(k_upper_top<16>, dim3(1), dim3(256), 0, st, cut, ncut, t.parent.p, t.visits.p,
-                           t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_local<16>, dim3(gc), dim3(256), 0, st, cut, ncut, nb, t.child0.p, t.child1.p,
+                           t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p, t.cf.p);
+        hipLaunchKernelGGL(k_upper_top<16>, dim3(1), dim3(1024), 0, st, t.root, nb, m, t.first.p, t.count.p,
+                           t.bscan.p, t.toplist.p, t.child0.p, t.child1.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.cd.p,
+                           t.cf.p);
     }
     hipLaunchKernelGGL(k_pack_nodes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, nn, t.ncx.p, t.ncy.p,
                        t.nr.p, t.child0.p, t.child1.p, t.first.p, t.count.p, t.rec.p);
