@@ -375,7 +375,7 @@ int vx_compact(vx_case *c) {
 
 int vx_setka(vx_case *c, int kind, int32_t *COL) {
     VX_ARG(c && (kind == 1 || kind == 2), "kind must be 1 or 2");
-    if (launch_set_dn(c, c->n) || launch_live_box(c) || launch_setka(c, kind, c->n)) return 1;
+    if (launch_set_dn(c, c->n) || launch_live_box(c) || launch_setka(c, kind, c->n, false)) return 1;
     if (COL && c->n) VX_CHECK(hipMemcpyAsync(COL, c->col.p, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
     VX_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -389,7 +389,7 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL) {
         if (c->col.ensure((size_t)(c->n > 0 ? c->n : 1), c->stream)) return 1;
         if (c->n) VX_CHECK(hipMemcpyAsync(c->col.p, COL, sizeof(int32_t) * c->n, hipMemcpyHostToDevice, c->stream));
     }
-    if (launch_set_dn(c, c->n) || launch_collapse(c, kind, c->n)) return 1;
+    if (launch_set_dn(c, c->n) || launch_collapse(c, kind, c->n, false)) return 1;
     bool err = false;
     if (step_readback(c, nullptr, &err)) return 1;
     if (err) {
@@ -445,7 +445,8 @@ int vx_step(vx_case *c, double out4[4]) {
     if (launch_set_dn(c, n_ub) || launch_step_move(c) || launch_compact(c, n_ub)) return 1;
     for (int kind = 1; kind <= 2; ++kind)
         for (int64_t cc = 0; cc < c->prm.n_col; ++cc)
-            if (launch_setka(c, kind, n_ub) || launch_collapse(c, kind, n_ub) || launch_compact(c, n_ub)) return 1;
+            if (launch_setka(c, kind, n_ub, true) || launch_collapse(c, kind, n_ub, true) || launch_compact(c, n_ub))
+                return 1;
     bool err = false;
     if (step_readback(c, out4, &err)) return 1;
     if (err) {

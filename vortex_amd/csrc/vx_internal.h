@@ -193,7 +193,10 @@ int launch_step_move(vx_case *c);                      // sums and live box -> c
 int launch_points_move(vx_case *c, int64_t m);         // c->px, py, pg moved by c->pu, pv
 int launch_compact(vx_case *c, int64_t n_ub);          // [0, *dn) compacted, new count -> *dn
 int launch_live_box(vx_case *c);                       // box of g != 0 in [0, *dn) -> c->red[4..7]
-int launch_setka(vx_case *c, int kind, int64_t n_ub);  // -> c->col, box from c->red[4..7]
-int launch_collapse(vx_case *c, int kind, int64_t n_ub);  // uses c->col; error flag in dcount[3]
+// launch_setka -> c->col (box from c->red[4..7]); with prepare it also fills
+// the collapse counts and candidate list, and launch_collapse is then called
+// with prepared.  Collapse error flag in dcount[3].
+int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare);
+int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared);
 
 }  // namespace vx

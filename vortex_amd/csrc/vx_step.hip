@@ -427,6 +427,54 @@ int launch_compact(vx_case *c, int64_t n_ub) {
 // i's cell or in the neighbour on the side of the cell's half that holds i,
 // per axis, so four cells are searched.  The minimum j does not depend on the
 // order of the cell lists.  The grid is set up on the device from the box.
+// Collapse work arrays (per vortex slot unless noted): pc counts the
+// candidates pointing at a vortex, pcd the ones not yet published, state
+// holds the decision bits, first the smallest active candidate merging with
+// it; cand lists the candidates (count *ncand), rest the ones left to
+// resolve (count *nrest).
+struct ColBufs {
+    int32_t *pc, *pcd, *state, *first, *cand, *rest, *nrest;
+    unsigned long long *ncand;
+};
+
+static int col_bufs(vx_case *c, int64_t n_ub, ColBufs &b) {
+    hipStream_t st = c->stream;
+    if (c->flags.ensure((size_t)n_ub, st) || c->scan.ensure((size_t)n_ub, st) ||
+        c->lists.ensure((size_t)(3 * n_ub), st) || c->misc.ensure((size_t)n_ub, st) ||
+        c->tx.ensure((size_t)n_ub, st) || c->ty.ensure((size_t)n_ub, st) || c->tg.ensure((size_t)n_ub, st) ||
+        c->dcount.ensure(16, st))
+        return 1;
+    b.pc = c->flags.p;
+    b.state = c->scan.p;
+    b.cand = c->lists.p;
+    b.rest = c->lists.p + n_ub;
+    b.pcd = c->lists.p + 2 * n_ub;
+    b.first = c->misc.p;
+    b.ncand = (unsigned long long *)(c->dcount.p + 2);
+    b.nrest = (int32_t *)(c->dcount.p + 4);
+    return 0;
+}
+
+// Appends val of every flagged thread to out[*cnt, ...) with one atomic per
+// 256-thread work-group; every thread of the work-group calls it.  The order
+// within the list is not defined.
+__device__ void block_append(bool flag, int32_t val, int32_t *__restrict__ out, unsigned long long *__restrict__ cnt) {
+    __shared__ int wcnt[4];
+    __shared__ unsigned long long base;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t m = __ballot(flag);
+    if (lane == 0) wcnt[w] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        base = tot ? atomicAdd(cnt, (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    int off = 0;
+    for (int q = 0; q < w; ++q) off += wcnt[q];
+    if (flag) out[base + off + __popcll(m & ((1ull << lane) - 1))] = val;
+}
+
 struct Grid {
     double x0, y0, inv_s;
     int64_t nx, ny, cells;
@@ -466,9 +514,11 @@ __device__ Grid grid_params(const double *__restrict__ box, double r2, int64_t m
 }
 
 // Each work-group derives the grid from the box (the same result in every
-// one), work-group 0 stores it, and the cell heads are cleared to -1.
+// one), work-group 0 stores it, and the cell heads are cleared to -1.  With
+// cb.pc set, the collapse arrays of [0, n_ub) are prepared as well (counts
+// and states 0, first INT32_MAX, both list counts 0).
 __global__ void k_grid_clear(int32_t *__restrict__ head, const double *__restrict__ box, double r2,
-                             int64_t max_cells, Grid *__restrict__ G) {
+                             int64_t max_cells, Grid *__restrict__ G, int64_t n_ub, ColBufs cb) {
     __shared__ Grid sq;
     if (threadIdx.x == 0) {
         sq = grid_params(box, r2, max_cells);
@@ -476,8 +526,19 @@ __global__ void k_grid_clear(int32_t *__restrict__ head, const double *__restric
     }
     __syncthreads();
     const int64_t cells = sq.valid ? sq.cells : 0;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cells; t += (int64_t)gridDim.x * blockDim.x)
-        head[t] = -1;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = t0; t < cells; t += stride) head[t] = -1;
+    if (!cb.pc) return;
+    for (int64_t t = t0; t < n_ub; t += stride) {
+        cb.pc[t] = 0;
+        cb.pcd[t] = 0;
+        cb.state[t] = 0;
+        cb.first[t] = INT32_MAX;
+    }
+    if (t0 == 0) {
+        *cb.ncand = 0;
+        *cb.nrest = 0;
+    }
 }
 
 __global__ void k_grid_insert(const double *__restrict__ x, const double *__restrict__ y,
@@ -492,47 +553,55 @@ __global__ void k_grid_insert(const double *__restrict__ x, const double *__rest
     next[i] = atomicExch(&head[cy * G.nx + cx], (int32_t)i);
 }
 
+// With cb.pc set, each pair found also counts at its target (pc, pcd) and
+// lists i as a candidate.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_setka(const double *__restrict__ x, const double *__restrict__ y,
                                                const double *__restrict__ g, const int64_t *__restrict__ dn,
                                                const Grid *__restrict__ Gp, double r2, double max_ve_g,
                                                const int32_t *__restrict__ head, const int32_t *__restrict__ next,
-                                               int32_t *__restrict__ col) {
+                                               int32_t *__restrict__ col, ColBufs cb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *dn) return;
-    const Grid G = *Gp;
-    const double gi = g[i];
-    if (!G.valid || gi == 0) {
-        col[i] = -1;
-        return;
-    }
-    const double r0 = x[i], r1 = y[i];
-    int64_t cx, cy;
-    double fx, fy;
-    cell_of(G, r0, r1, cx, cy, fx, fy);
-    const int64_t ox = fx < 0.5 ? cx - 1 : cx + 1, oy = fy < 0.5 ? cy - 1 : cy + 1;
-    const int64_t xs[2] = {cx, ox}, ys[2] = {cy, oy};
-    int32_t hd[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int64_t xx = xs[q & 1], yy = ys[q >> 1];
-        hd[q] = (xx >= 0 && xx < G.nx && yy >= 0 && yy < G.ny) ? head[yy * G.nx + xx] : -1;
-    }
     int64_t best = INT64_MAX;
+    if (i < *dn) {
+        const Grid G = *Gp;
+        const double gi = g[i];
+        if (G.valid && gi != 0) {
+            const double r0 = x[i], r1 = y[i];
+            int64_t cx, cy;
+            double fx, fy;
+            cell_of(G, r0, r1, cx, cy, fx, fy);
+            const int64_t ox = fx < 0.5 ? cx - 1 : cx + 1, oy = fy < 0.5 ? cy - 1 : cy + 1;
+            const int64_t xs[2] = {cx, ox}, ys[2] = {cy, oy};
+            int32_t hd[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        for (int32_t j = hd[q]; j >= 0; j = next[j]) {
-            if (j <= i || j >= best) continue;
-            const double gj = g[j];
-            bool hit;
-            if (KIND == 1)
-                hit = (gi * gj < 0) && (dist2(r0, r1, x[j], y[j]) < r2);
-            else
-                hit = (gi * gj > 0) && (dist2(r0, r1, x[j], y[j]) < r2) && (fabs(gi + gj) < max_ve_g);
-            if (hit) best = j;
+            for (int q = 0; q < 4; ++q) {
+                const int64_t xx = xs[q & 1], yy = ys[q >> 1];
+                hd[q] = (xx >= 0 && xx < G.nx && yy >= 0 && yy < G.ny) ? head[yy * G.nx + xx] : -1;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                for (int32_t j = hd[q]; j >= 0; j = next[j]) {
+                    if (j <= i || j >= best) continue;
+                    const double gj = g[j];
+                    bool hit;
+                    if (KIND == 1)
+                        hit = (gi * gj < 0) && (dist2(r0, r1, x[j], y[j]) < r2);
+                    else
+                        hit = (gi * gj > 0) && (dist2(r0, r1, x[j], y[j]) < r2) && (fabs(gi + gj) < max_ve_g);
+                    if (hit) best = j;
+                }
+            }
         }
+        col[i] = best == INT64_MAX ? -1 : (int32_t)best;
     }
-    col[i] = best == INT64_MAX ? -1 : (int32_t)best;
+    if (!cb.pc) return;  // the same for every thread
+    const bool hit = best != INT64_MAX;
+    if (hit) {
+        atomicAdd(&cb.pc[best], 1);
+        atomicAdd(&cb.pcd[best], 1);
+    }
+    block_append(hit, (int32_t)i, cb.cand, cb.ncand);
 }
 
 // Box of the vortices with g != 0 in [0, *dn) (min x, -max x, min y, -max y
@@ -591,10 +660,12 @@ int launch_live_box(vx_case *c) {
     return 0;
 }
 
-int launch_setka(vx_case *c, int kind, int64_t n_ub) {
+int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare) {
     hipStream_t st = c->stream;
     if (c->col.ensure((size_t)(n_ub > 0 ? n_ub : 1), st)) return 1;
     if (n_ub <= 0) return 0;
+    ColBufs cb{};
+    if (prepare && col_bufs(c, n_ub, cb)) return 1;
     const unsigned nb = (unsigned)((n_ub + 255) / 256);
     const double ve2 = c->prm.ve_size * c->prm.ve_size;
     const double rr = kind == 1 ? c->prm.r_col_diff_sign : c->prm.r_col_same_sign;
@@ -605,16 +676,17 @@ int launch_setka(vx_case *c, int kind, int64_t n_ub) {
         c->gridp.ensure(sizeof(Grid), st))
         return 1;
     Grid *G = (Grid *)c->gridp.p;
-    hipLaunchKernelGGL(k_grid_clear, dim3(1024), dim3(256), 0, st, c->vals.p, c->red.p + 4, r2, max_cells, G);
+    hipLaunchKernelGGL(k_grid_clear, dim3(1024), dim3(256), 0, st, c->vals.p, c->red.p + 4, r2, max_cells, G, n_ub,
+                       cb);
     int64_t *dn = step_dn(c);
     hipLaunchKernelGGL(k_grid_insert, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn, G, c->vals.p,
                        c->vals_alt.p);
     if (kind == 1)
         hipLaunchKernelGGL(k_setka<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn, G, r2,
-                           c->prm.max_ve_g, c->vals.p, c->vals_alt.p, c->col.p);
+                           c->prm.max_ve_g, c->vals.p, c->vals_alt.p, c->col.p, cb);
     else
         hipLaunchKernelGGL(k_setka<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn, G, r2,
-                           c->prm.max_ve_g, c->vals.p, c->vals_alt.p, c->col.p);
+                           c->prm.max_ve_g, c->vals.p, c->vals_alt.p, c->col.p, cb);
     VX_CHECK(hipGetLastError());
     return 0;
 }
@@ -626,21 +698,25 @@ int launch_setka(vx_case *c, int kind, int64_t n_ub) {
 // forest of edges i -> COL_i (COL_i > i) in one work-group.  Among active
 // candidates sharing a target j the smallest merges with j as it was, the
 // others with the killed j (g = 0 at 1e10), exactly as the sweep does.
-__global__ void k_col_prepare(int64_t n_ub, int32_t *__restrict__ pc, int32_t *__restrict__ state,
-                              int32_t *__restrict__ first, int32_t *__restrict__ nrest) {
+// Preparation and counting when COL comes from elsewhere than k_setka.
+__global__ void k_col_prepare(int64_t n_ub, ColBufs cb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) nrest[0] = 0;  // non-leaf candidate count
+    if (i == 0) *cb.nrest = 0;
     if (i >= n_ub) return;
-    state[i] = 0;
-    first[i] = INT32_MAX;
-    pc[i] = 0;
+    cb.pc[i] = 0;
+    cb.pcd[i] = 0;
+    cb.state[i] = 0;
+    cb.first[i] = INT32_MAX;
 }
 
 __global__ __launch_bounds__(256) void k_col_count(const int32_t *__restrict__ col, const int64_t *__restrict__ dn,
-                                                   int32_t *__restrict__ pc) {
+                                                   ColBufs cb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t j = i < *dn ? col[i] : -1;
-    if (j >= 0) atomicAdd(&pc[j], 1);
+    if (j >= 0) {
+        atomicAdd(&cb.pc[j], 1);
+        atomicAdd(&cb.pcd[j], 1);
+    }
 }
 
 // Candidates in index order: i < *dn with COL_i >= 0.
@@ -650,55 +726,45 @@ struct ColCandidate {
     __device__ bool operator()(int32_t i) const { return i < *dn && col[i] >= 0; }
 };
 
-// Leaves of the forest (no candidate points at them) are active: nothing
-// precedes them.  The first kernel decides them (pc is only read) and lists the
-// other candidates; the second publishes them (kills COL_i, counts the edge off).
-__global__ __launch_bounds__(256) void k_col_leaves(const int32_t *__restrict__ cand,
-                                                    const unsigned long long *__restrict__ ncand_p,
-                                                    const int32_t *__restrict__ pc, int32_t *__restrict__ state,
-                                                    int32_t *__restrict__ rest, int32_t *__restrict__ nrest) {
+// Leaves of the forest (no candidate points at them: pc 0) are active, as
+// nothing precedes them, and are published at once: their target is marked
+// killed, its first merger recorded and its pending count pcd lowered.  pc
+// itself is only read, so every thread sees the counts from before any
+// publication.  No candidate points at a leaf, so its state slot is written
+// by this thread only.  The other candidates are listed in rest.
+__global__ __launch_bounds__(256) void k_col_leaves(const int32_t *__restrict__ col, ColBufs cb) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t ncand = (int64_t)*ncand_p;
-    const int32_t i = t < ncand ? cand[t] : -1;
-    const bool leaf = i >= 0 && pc[i] == 0;
-    if (leaf) state[i] = 1;
+    const int64_t ncand = (int64_t)*cb.ncand;
+    const int32_t i = t < ncand ? cb.cand[t] : -1;
+    const bool leaf = i >= 0 && cb.pc[i] == 0;
+    if (leaf) {
+        const int32_t j = col[i];
+        cb.state[i] = 1 | 16;
+        atomicOr(&cb.state[j], 4);
+        atomicMin(&cb.first[j], i);
+        atomicSub(&cb.pcd[j], 1);
+    }
     const uint64_t m = __ballot(i >= 0 && !leaf);
     if (m == 0) return;
     const int lane = threadIdx.x & 63;
-    const int first = __builtin_ctzll(m);
+    const int f = __builtin_ctzll(m);
     int base = 0;
-    if (lane == first) base = atomicAdd(nrest, __popcll(m));
-    base = __shfl(base, first, 64);
-    if (i >= 0 && !leaf) rest[base + __popcll(m & ((1ull << lane) - 1))] = i;
-}
-
-__global__ __launch_bounds__(256) void k_col_publish_leaves(const int32_t *__restrict__ col,
-                                                            const int32_t *__restrict__ cand,
-                                                            const unsigned long long *__restrict__ ncand_p,
-                                                            int32_t *__restrict__ pc, int32_t *__restrict__ state,
-                                                            int32_t *__restrict__ first) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)*ncand_p) return;
-    const int32_t i = cand[t];
-    // a leaf's slot is written by this thread only; other candidates' slots
-    // may be receiving bit 4 concurrently, which never carries bit 1
-    if (!(atomicOr(&state[i], 0) & 1)) return;
-    const int32_t j = col[i];
-    state[i] = 1 | 16;
-    atomicOr(&state[j], 4);
-    atomicMin(&first[j], i);
-    atomicSub(&pc[j], 1);
+    if (lane == f) base = atomicAdd(cb.nrest, __popcll(m));
+    base = __shfl(base, f, 64);
+    if (i >= 0 && !leaf) cb.rest[base + __popcll(m & ((1ull << lane) - 1))] = i;
 }
 
 // state bits: 1 active, 2 inactive, 4 killed by an active child, 16 published.
-// The remaining (non-leaf) candidates are decided here in one work-group.
+// The remaining (non-leaf) candidates are decided here in one work-group: a
+// candidate is decided once pcd, its count of unpublished candidates pointing
+// at it, reaches 0.
 // Every access goes through L2 atomics (no stale per-CU L1 lines).  Each round
 // decides at least the lowest undecided candidate (its children have lower
 // indices and were published in earlier rounds), so ncand + 1 rounds always
 // suffice; the loop bound is a guard, reported through *err.
 __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
                                                       const int32_t *__restrict__ ncand_p,
-                                                      int32_t *__restrict__ pc, int32_t *__restrict__ state,
+                                                      int32_t *__restrict__ pcd, int32_t *__restrict__ state,
                                                       int32_t *__restrict__ first, int32_t *__restrict__ err) {
     __shared__ int undecided;
     const int64_t ncand = (int64_t)*ncand_p;
@@ -710,7 +776,7 @@ __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict_
             const int32_t i = cand[t];
             const int32_t s = atomicOr(&state[i], 0);
             if (s & 3) continue;
-            if (atomicAdd(&pc[i], 0) == 0)
+            if (atomicAdd(&pcd[i], 0) == 0)
                 atomicOr(&state[i], (s & 4) ? 2 : 1);
             else
                 undecided = 1;
@@ -726,7 +792,7 @@ __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict_
                     atomicMin(&first[j], i);
                 }
                 atomicOr(&state[i], 16);
-                atomicSub(&pc[j], 1);
+                atomicSub(&pcd[j], 1);
             }
         }
         __syncthreads();
@@ -789,40 +855,33 @@ __global__ void k_col_write(double *__restrict__ x, double *__restrict__ y, doub
     g[j] = 0.0;
 }
 
-int launch_collapse(vx_case *c, int kind, int64_t n_ub) {
+int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared) {
     if (n_ub <= 0) return 0;
     hipStream_t st = c->stream;
-    if (c->flags.ensure((size_t)n_ub, st) || c->scan.ensure((size_t)n_ub, st) ||
-        c->lists.ensure((size_t)(2 * n_ub), st) || c->misc.ensure((size_t)n_ub, st) ||
-        c->tx.ensure((size_t)n_ub, st) || c->ty.ensure((size_t)n_ub, st) || c->tg.ensure((size_t)n_ub, st) ||
-        c->dcount.ensure(16, st))
-        return 1;
-    int32_t *pc = c->flags.p, *state = c->scan.p, *cand = c->lists.p, *first = c->misc.p;
-    unsigned long long *ncand = (unsigned long long *)(c->dcount.p + 2);
+    ColBufs cb{};
+    if (col_bufs(c, n_ub, cb)) return 1;
     const unsigned nb = (unsigned)((n_ub + 255) / 256);
-    int32_t *rest = c->lists.p + n_ub, *nrest = (int32_t *)(c->dcount.p + 4);
-    hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, pc, state, first, nrest);
-    hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, step_dn(c), pc);
-    {
+    if (!prepared) {
+        hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, cb);
+        hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, step_dn(c), cb);
         const ColCandidate pred{c->col.p, step_dn(c)};
         hipcub::CountingInputIterator<int32_t> it(0);
         size_t bytes = 0;
-        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cand, ncand, (int)n_ub, pred, st));
+        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cb.cand, cb.ncand, (int)n_ub, pred, st));
         if (c->cub_tmp.ensure(bytes, st)) return 1;
-        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cand, ncand, (int)n_ub, pred, st));
+        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cb.cand, cb.ncand, (int)n_ub, pred, st));
     }
-    hipLaunchKernelGGL(k_col_leaves, dim3(nb), dim3(256), 0, st, cand, ncand, pc, state, rest, nrest);
-    hipLaunchKernelGGL(k_col_publish_leaves, dim3(nb), dim3(256), 0, st, c->col.p, cand, ncand, pc, state, first);
-    hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, rest, nrest, pc, state, first,
-                       (int32_t *)(c->dcount.p + 3));
+    hipLaunchKernelGGL(k_col_leaves, dim3(nb), dim3(256), 0, st, c->col.p, cb);
+    hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, cb.rest, cb.nrest, cb.pcd, cb.state,
+                       cb.first, (int32_t *)(c->dcount.p + 3));
     if (kind == 1)
-        hipLaunchKernelGGL(k_col_merge<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cand, ncand,
-                           state, first, c->tx.p, c->ty.p, c->tg.p);
+        hipLaunchKernelGGL(k_col_merge<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb.cand,
+                           cb.ncand, cb.state, cb.first, c->tx.p, c->ty.p, c->tg.p);
     else
-        hipLaunchKernelGGL(k_col_merge<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cand, ncand,
-                           state, first, c->tx.p, c->ty.p, c->tg.p);
-    hipLaunchKernelGGL(k_col_write, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cand, ncand, state,
-                       c->tx.p, c->ty.p, c->tg.p);
+        hipLaunchKernelGGL(k_col_merge<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb.cand,
+                           cb.ncand, cb.state, cb.first, c->tx.p, c->ty.p, c->tg.p);
+    hipLaunchKernelGGL(k_col_write, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb.cand, cb.ncand,
+                       cb.state, c->tx.p, c->ty.p, c->tg.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
