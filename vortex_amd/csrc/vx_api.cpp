@@ -157,6 +157,8 @@ int vx_case_destroy(vx_case *c) {
     c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();
     c->keys.release(); c->keys_alt.release(); c->vals.release(); c->vals_alt.release();
     c->cub_tmp.release(); c->gridp.release(); c->dcount.release();
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    c->wlist.release(); c->wtmp.release(); c->wstage.release();
     tree_release(c);
     if (c->pan_block) (void)hipFree(c->pan_block);
     if (c->segs) (void)hipFree(c->segs);
@@ -164,6 +166,9 @@ int vx_case_destroy(vx_case *c) {
     if (c->h_red) (void)hipHostFree(c->h_red);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -423,15 +428,26 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
             return 1;
         c->tm.bs_launches++;
     }
+    // The wall term and the tree's domain-scale diffusion targets run on the
+    // side stream, overlapping the groups' diffusion on the main stream; the
+    // wall's terms are added after the join.
+    int rc = 0;
     {
         Timer td(c, &c->tm.diff_ms, 3);
-        if (tree ? launch_tree_diffusion(c) : launch_diffusion(c, s, s)) return 1;
+        if ((tree && tree_diffusion_pre(c)) || side_fork(c)) return 1;
+        if (c->timing) (void)hipEventRecord(c->ev[14], c->side);
+        rc = launch_wall_side(c, s, c->side);
+        if (c->timing) (void)hipEventRecord(c->ev[15], c->side);
+        if (!rc && tree) rc = tree_diffusion_far(c, c->side);
+        if (!rc) rc = tree ? tree_diffusion_main(c) : launch_diffusion(c, s, s);
+        const int rj = side_join(c);
+        if (!rc) rc = rj ? rj : launch_wall_add(c);
     }
-    {
-        Timer tw(c, &c->tm.wall_ms, 7);
-        if (launch_diffusion_wall(c, s)) return 1;
+    if (!rc && c->timing) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[14], c->ev[15]) == hipSuccess) c->tm.wall_ms += ms;
     }
-    return 0;
+    return rc;
 }
 
 // Step (src/unit_1.cu:1077-1218).  The count stays on the device until the

@@ -98,6 +98,11 @@ struct vx_case {
     vx_params prm;
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream (default priority) for work that overlaps the diffusion:
+    // fork_ev orders it after the main stream, join_ev the main stream after it
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool wall_pending = false;                        // a wall pass forked and not yet joined
     vx::PanelsSoA pan{};
     void *pan_block = nullptr;
     vx::RhsSeg *segs = nullptr;                       // birth RHS segments (device)
@@ -124,6 +129,9 @@ struct vx_case {
     vx::DBuf<unsigned char> cub_tmp;
     vx::DBuf<unsigned char> gridp;                     // collapse grid parameters (device)
     vx::DBuf<int64_t> dcount;                         // device counters
+    vx::DBuf<int32_t> wlist;                          // wall targets (side stream)
+    vx::DBuf<unsigned char> wtmp;                     // their selection's scratch
+    vx::DBuf<double> wstage;                          // their velocity terms, per list slot
     int dn_slot = 1;                                  // dcount slot of the step's current count (1 or 8)
     int64_t *h_count = nullptr;                       // pinned read-back slots (16)
     double *h_red = nullptr;                          // pinned reduction read-back
@@ -145,14 +153,16 @@ struct Timer {
     vx_case *c;
     double *acc;
     bool on;
-    Timer(vx_case *cc, double *a, int slot) : c(cc), acc(a), on(cc->timing) {
-        if (on) (void)hipEventRecord(c->ev[slot * 2], c->stream);
+    hipStream_t s;
+    Timer(vx_case *cc, double *a, int slot, hipStream_t st = nullptr)
+        : c(cc), acc(a), on(cc->timing), s(st ? st : cc->stream) {
+        if (on) (void)hipEventRecord(c->ev[slot * 2], s);
         this->slot = slot;
     }
     int slot;
     void stop() {
         if (!on) return;
-        (void)hipEventRecord(c->ev[slot * 2 + 1], c->stream);
+        (void)hipEventRecord(c->ev[slot * 2 + 1], s);
         (void)hipEventSynchronize(c->ev[slot * 2 + 1]);
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->ev[slot * 2], c->ev[slot * 2 + 1]);
@@ -166,6 +176,8 @@ struct Timer {
 // ---- launchers (kernels in the .hip units) --------------------------------
 // vx_state.hip
 int launch_fill_tail(vx_case *c, int64_t from, int64_t to);
+int side_fork(vx_case *c);  // the side stream waits for the work queued so far on the main stream
+int side_join(vx_case *c);  // the main stream waits for the work queued so far on the side stream
 int launch_pack_f32(vx_case *c, int64_t count);
 // vx_pairwise.hip
 int launch_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, double vinf_y);
@@ -173,6 +185,12 @@ int launch_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src);
 int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y);  // -> c->rp
 int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vinf_y); // c->px.. -> c->pu,pv
 // vx_wall.hip
+// launch_wall_side queues the wall term on stream st (the main stream, or the
+// side stream after side_fork), into per-target staging; launch_wall_add
+// (main stream, after side_join if st was the side stream) adds the terms to
+// u, v.  launch_diffusion_wall is the two in a row on the main stream.
+int launch_wall_side(vx_case *c, int64_t n_tgt, hipStream_t st);
+int launch_wall_add(vx_case *c);
 int launch_diffusion_wall(vx_case *c, int64_t n_tgt);
 // vx_panel.hip
 int launch_panel_matrix(vx_case *c);                   // -> c->L
@@ -182,6 +200,11 @@ int launch_birth(vx_case *c, double d_g);              // c->rp -> c->gsheet, ne
 int tree_build(vx_case *c, int64_t s);
 int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y);
 int launch_tree_diffusion(vx_case *c);  // reuses the tree of the last launch_tree_biot_savart
+// launch_tree_diffusion in three parts: pre (main stream), far (any stream
+// ordered after pre), main (main stream); far and main write disjoint targets
+int tree_diffusion_pre(vx_case *c);
+int tree_diffusion_far(vx_case *c, hipStream_t st);
+int tree_diffusion_main(vx_case *c);
 int tree_stats(vx_case *c, vx_tree_info *info);
 void tree_release(vx_case *c);
 // vx_step.hip

@@ -26,6 +26,23 @@ __global__ void k_fill_tail(double *__restrict__ x, double *__restrict__ y, doub
     g[i] = 0.0;
 }
 
+int side_fork(vx_case *c) {
+    if (!c->side) {
+        VX_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        VX_CHECK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+        VX_CHECK(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+    }
+    VX_CHECK(hipEventRecord(c->fork_ev, c->stream));
+    VX_CHECK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    return 0;
+}
+
+int side_join(vx_case *c) {
+    VX_CHECK(hipEventRecord(c->join_ev, c->side));
+    VX_CHECK(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+    return 0;
+}
+
 int launch_fill_tail(vx_case *c, int64_t from, int64_t to) {
     if (to <= from) return 0;
     const int64_t cnt = to - from;

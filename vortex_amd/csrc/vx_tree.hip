@@ -64,7 +64,7 @@ struct TreeBufs {
     DBuf<int32_t> pcount;                          // included particles per block
     DBuf<int32_t> arr;                             // cut nodes, count at arr[0]
     DBuf<int64_t> sizes;                           // group, bucket and upper-node counts
-    DBuf<double> sx, sy, sg, ncx, ncy, nr, fbr2, hpart;
+    DBuf<double> sx, sy, sg, ncx, ncy, nr, fbr2, hpart, sd;
     DBuf<float> fx, fy, fg;
     DBuf<double4> nbb, part, cbox, ftgt;
     DBuf<int2> fpairs;
@@ -1361,9 +1361,9 @@ __device__ __forceinline__ int64_t far_count(const unsigned long long *cnt) {
 // evaluated pairs, cnt[1] stack overflows, cnt[2] the list length.
 template <typename T>
 __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, double nu, double thr,
-                                               const double *__restrict__ dd, double *__restrict__ u,
+                                               const double *__restrict__ sd, double *__restrict__ u,
                                                double *__restrict__ v, unsigned long long *__restrict__ cnt,
-                                               int32_t *__restrict__ lists, unsigned long long &cpairs) {
+                                               unsigned long long &cpairs) {
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
@@ -1374,17 +1374,9 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
     const int64_t ti = G.ti;
     const bool ok = G.ok;
     const int32_t i = ok ? t.perm[ti] : 0;
-    const double di = ok ? dd[i] : 0.0;
-    const double rbig = far_radius(t);
-    bool far = ok && cut * di > rbig;
-    if (far) {
-        const unsigned long long k = atomicAdd(cnt + 2, 1ull);
-        if (k < (unsigned long long)kFarCap)
-            lists[k] = (int32_t)ti;
-        else
-            far = false;  // beyond the list's capacity: summed here
-    }
-    const bool mine = ok && !far;
+    const double sdi = ok ? sd[ti] : 0.0;  // negative: listed for k_diff_far (d > 0 there)
+    const double di = fabs(sdi);
+    const bool mine = ok && !(sdi < 0.0);
     const double R = cut * wave_max(mine ? di : 0.0);
     const double R2 = R * R;
     const double cAx = G.cx, cAy = G.cy;
@@ -1474,13 +1466,48 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
 void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
-                                                        const double *__restrict__ dd,
+                                                        const double *__restrict__ sd,
                                                         double *__restrict__ u, double *__restrict__ v,
-                                                        unsigned long long *__restrict__ cnt,
-                                                        int32_t *__restrict__ lists) {
+                                                        unsigned long long *__restrict__ cnt) {
     unsigned long long cp = 0;
-    diffusion_wave<T>(t, cut, nu, thr, dd, u, v, cnt, lists, cp);
+    diffusion_wave<T>(t, cut, nu, thr, sd, u, v, cnt, cp);
     block_count_add(cp, 0ull, cnt, nullptr);
+}
+
+// The diffusion's d in sorted order, negated for the targets listed for
+// k_diff_far (radius cut * d beyond far_radius(), at most kFarCap of them,
+// appended with one atomic per work-group; cnt[2] counts every such target,
+// so targets past the capacity stay with their group's wave).
+__global__ __launch_bounds__(256) void k_far_list(TreeDev t, double cut, const double *__restrict__ dd,
+                                                  double *__restrict__ sd, int32_t *__restrict__ lists,
+                                                  unsigned long long *__restrict__ cnt) {
+    __shared__ int wcnt[4];
+    __shared__ unsigned long long base;
+    const int64_t ti = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double di = 0.0;
+    bool far = false;
+    if (ti < t.m) {
+        di = dd[t.perm[ti]];
+        far = cut * di > far_radius(t);
+    }
+    const uint64_t m = __ballot(far);
+    if (lane == 0) wcnt[w] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        base = tot ? atomicAdd(cnt + 2, (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    if (far) {
+        unsigned long long k = base + __popcll(m & ((1ull << lane) - 1));
+        for (int q = 0; q < w; ++q) k += wcnt[q];
+        if (k < (unsigned long long)kFarCap)
+            lists[k] = (int32_t)ti;
+        else
+            far = false;
+    }
+    if (ti < t.m) sd[ti] = far ? -di : di;
 }
 
 // Targets whose radius exceeds far_radius(), listed by k_tree_diffusion.  The
@@ -1722,6 +1749,7 @@ void tree_release(vx_case *c) {
     t.cbox.release(); t.fpairs.release(); t.ftgt.release();
     t.gcs.release(); t.gcs2.release(); t.gflag.release(); t.gscan.release(); t.gfirst.release(); t.pcount.release(); t.arr.release(); t.sizes.release();
     t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.fbr2.release(); t.hpart.release();
+    t.sd.release();
     if (t.ev) (void)hipEventDestroy(t.ev);
     delete c->tree;
     c->tree = nullptr;
@@ -1920,36 +1948,44 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
     return 0;
 }
 
-// Uses the tree of the preceding launch_tree_biot_savart (same positions).
-int launch_tree_diffusion(vx_case *c) {
+// The diffusion over the tree of the preceding launch_tree_biot_savart (same
+// positions) in three parts: tree_diffusion_pre (main stream: counters, far
+// list, sorted d), tree_diffusion_far (the listed targets; any stream ordered
+// after the pre part) and tree_diffusion_main (the groups' waves; main
+// stream).  The far and main parts write disjoint targets.
+static double diff_cut_of(const vx_case *c) {
+    const bool mixed = c->prm.precision == VX_PREC_MIXED;
+    // omitted weights total at most (1 + c) e^{-c} of the near sum's scale:
+    // 3e-7 at c = 18 (the FP32 rounding level of the sums), 3e-12 at c = 30
+    return c->prm.diff_cut > 0 ? c->prm.diff_cut : (mixed ? 18.0 : 30.0);
+}
+
+int tree_diffusion_pre(vx_case *c) {
     if (!c->tree || c->tree->m == 0) return 0;
     TreeBufs &t = *c->tree;
     hipStream_t st = c->stream;
-    const TreeDev tv = view(t);
-    const unsigned lb8 = (unsigned)((t.ngroup + 3) / 4 + 7) / 8 * 8;  // target groups, padded for xcd_block()
-    // omitted weights total at most (1 + c) e^{-c} of the near sum's scale:
-    // 3e-7 at c = 18 (the FP32 rounding level of the sums), 3e-12 at c = 30
-    const bool mixed = c->prm.precision == VX_PREC_MIXED;
-    const double cut = c->prm.diff_cut > 0 ? c->prm.diff_cut : (mixed ? 18.0 : 30.0);
-    const double thr = 0.001 * c->prm.ve_size;
-    unsigned long long *cnt = t.cnt.p + 4;  // pairs, overflow, far targets, far (target, chunk) pairs
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 0xdu);  // not the shared overflow flag
-    {
-        Timer tk(c, &c->tm.diff_kernel_ms, 8);
-        if (mixed)
-            hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb8), dim3(256), 0, st, tv, cut, c->prm.nu, thr, c->d.p,
-                               c->u.p, c->v.p, cnt, t.fb.p);
-        else
-            hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb8), dim3(256), 0, st, tv, cut, c->prm.nu, thr,
-                               c->d.p, c->u.p, c->v.p, cnt, t.fb.p);
-    }
-    VX_CHECK(hipGetLastError());
-    // the listed far targets: all counts stay on the device (grid-stride
-    // kernels sized for the list's capacity)
     const int64_t nchunk = (t.nleaf + 63) / 64;
     if (t.hpart.ensure((size_t)(3 * nchunk * kFarCap), st) || t.cbox.ensure((size_t)nchunk, st) ||
-        t.fpairs.ensure((size_t)(nchunk * kFarCap), st) || t.ftgt.ensure((size_t)kFarCap, st))
+        t.fpairs.ensure((size_t)(nchunk * kFarCap), st) || t.ftgt.ensure((size_t)kFarCap, st) ||
+        t.sd.ensure((size_t)t.m, st))
         return 1;
+    unsigned long long *cnt = t.cnt.p + 4;  // pairs, overflow, far targets, far (target, chunk) pairs
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 0xdu);  // not the shared overflow flag
+    hipLaunchKernelGGL(k_far_list, dim3((unsigned)((t.m + 255) / 256)), dim3(256), 0, st, view(t), diff_cut_of(c),
+                       c->d.p, t.sd.p, t.fb.p, cnt);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+int tree_diffusion_far(vx_case *c, hipStream_t st) {
+    if (!c->tree || c->tree->m == 0) return 0;
+    TreeBufs &t = *c->tree;
+    const TreeDev tv = view(t);
+    const bool mixed = c->prm.precision == VX_PREC_MIXED;
+    const double cut = diff_cut_of(c), thr = 0.001 * c->prm.ve_size;
+    unsigned long long *cnt = t.cnt.p + 4;
+    // all counts stay on the device (grid-stride kernels sized for the list's capacity)
+    const int64_t nchunk = (t.nleaf + 63) / 64;
     hipLaunchKernelGGL(k_far_zero, dim3(512), dim3(256), 0, st, t.hpart.p, nchunk, cnt);
     hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)((nchunk + 3) / 4)), dim3(256), 0, st, tv, nchunk, t.cbox.p);
     hipLaunchKernelGGL(k_far_pairs, dim3(256), dim3(256), 0, st, tv, t.fb.p, cnt, nchunk, t.cbox.p, cut, c->d.p,
@@ -1964,6 +2000,29 @@ int launch_tree_diffusion(vx_case *c) {
                        c->prm.nu, c->u.p, c->v.p);
     VX_CHECK(hipGetLastError());
     return 0;
+}
+
+int tree_diffusion_main(vx_case *c) {
+    if (!c->tree || c->tree->m == 0) return 0;
+    TreeBufs &t = *c->tree;
+    hipStream_t st = c->stream;
+    const unsigned lb8 = (unsigned)((t.ngroup + 3) / 4 + 7) / 8 * 8;  // target groups, padded for xcd_block()
+    const bool mixed = c->prm.precision == VX_PREC_MIXED;
+    const double cut = diff_cut_of(c), thr = 0.001 * c->prm.ve_size;
+    unsigned long long *cnt = t.cnt.p + 4;
+    Timer tk(c, &c->tm.diff_kernel_ms, 8);
+    if (mixed)
+        hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb8), dim3(256), 0, st, view(t), cut, c->prm.nu, thr,
+                           t.sd.p, c->u.p, c->v.p, cnt);
+    else
+        hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb8), dim3(256), 0, st, view(t), cut, c->prm.nu, thr,
+                           t.sd.p, c->u.p, c->v.p, cnt);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_tree_diffusion(vx_case *c) {
+    return tree_diffusion_pre(c) || tree_diffusion_far(c, c->stream) || tree_diffusion_main(c);
 }
 
 int tree_stats(vx_case *c, vx_tree_info *info) {

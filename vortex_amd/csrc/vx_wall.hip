@@ -96,15 +96,15 @@ struct InWallBox {
 };
 
 // One wave per target inside the box, selected in index order (grid-stride,
-// the count stays on the device).
+// the count stays on the device).  The velocity terms go to list slot t of
+// wst (u term at 2t, v term at 2t + 1); k_wall_add adds them.
 template <typename T>
 __global__ __launch_bounds__(256) void k_diffusion_wall(const double *__restrict__ x,
                                                         const double *__restrict__ y,
                                                         const double *__restrict__ d,
                                                         const int32_t *__restrict__ list,
                                                         const int32_t *__restrict__ cnt, PanelsSoA pan,
-                                                        double nu, int64_t npts, double *__restrict__ u,
-                                                        double *__restrict__ v) {
+                                                        double nu, int64_t npts, double *__restrict__ wst) {
     const int lane = threadIdx.x & 63;
     const int64_t nw = (int64_t)gridDim.x * 4;
     const int64_t total = *cnt;
@@ -113,31 +113,59 @@ __global__ __launch_bounds__(256) void k_diffusion_wall(const double *__restrict
         double II0, II30, II31;
         wave_wall<T>(pan, lane, (T)x[i], (T)y[i], (T)d[i], npts, II0, II30, II31);
         if (lane == 0) {
-            u[i] += nu * II30 / II0;
-            v[i] += nu * II31 / II0;
+            wst[2 * t] = nu * II30 / II0;
+            wst[2 * t + 1] = nu * II31 / II0;
         }
     }
 }
 
-int launch_diffusion_wall(vx_case *c, int64_t n_tgt) {
+__global__ __launch_bounds__(256) void k_wall_add(const int32_t *__restrict__ list, const int32_t *__restrict__ cnt,
+                                                  const double *__restrict__ wst, double *__restrict__ u,
+                                                  double *__restrict__ v) {
+    const int64_t total = *cnt;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = list[t];
+        u[i] += wst[2 * t];
+        v[i] += wst[2 * t + 1];
+    }
+}
+
+int launch_wall_side(vx_case *c, int64_t n_tgt, hipStream_t st) {
     if (n_tgt <= 0 || c->P <= 0) return 0;
-    hipStream_t st = c->stream;
-    if (c->lists.ensure((size_t)n_tgt, st) || c->dcount.ensure(16, st)) return 1;
+    // buffers grow on the main stream: a reallocation synchronises it, and
+    // with it every earlier side-stream pass (joined into it)
+    if (c->wlist.ensure((size_t)n_tgt, c->stream) || c->wstage.ensure((size_t)(2 * n_tgt), c->stream) ||
+        c->dcount.ensure(16, c->stream))
+        return 1;
     int32_t *cnt = (int32_t *)(c->dcount.p + 5);
     const InWallBox pred{c->x.p, c->y.p, c->wall_box};
     hipcub::CountingInputIterator<int32_t> it(0);
     size_t bytes = 0;
-    VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, c->lists.p, cnt, n_tgt, pred, st));
-    if (c->cub_tmp.ensure(bytes, st)) return 1;
-    VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, c->lists.p, cnt, n_tgt, pred, st));
+    VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, c->wlist.p, cnt, n_tgt, pred, st));
+    if (c->wtmp.ensure(bytes, c->stream)) return 1;
+    VX_CHECK(hipcub::DeviceSelect::If(c->wtmp.p, bytes, it, c->wlist.p, cnt, n_tgt, pred, st));
     if (c->prm.precision == VX_PREC_MIXED)
         hipLaunchKernelGGL(k_diffusion_wall<float>, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->d.p,
-                           c->lists.p, cnt, c->pan, c->prm.nu, c->prm.n_of_points, c->u.p, c->v.p);
+                           c->wlist.p, cnt, c->pan, c->prm.nu, c->prm.n_of_points, c->wstage.p);
     else
         hipLaunchKernelGGL(k_diffusion_wall<double>, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->d.p,
-                           c->lists.p, cnt, c->pan, c->prm.nu, c->prm.n_of_points, c->u.p, c->v.p);
+                           c->wlist.p, cnt, c->pan, c->prm.nu, c->prm.n_of_points, c->wstage.p);
+    VX_CHECK(hipGetLastError());
+    c->wall_pending = true;
+    return 0;
+}
+
+int launch_wall_add(vx_case *c) {
+    if (!c->wall_pending) return 0;
+    c->wall_pending = false;
+    hipLaunchKernelGGL(k_wall_add, dim3(64), dim3(256), 0, c->stream, c->wlist.p, (const int32_t *)(c->dcount.p + 5),
+                       c->wstage.p, c->u.p, c->v.p);
     VX_CHECK(hipGetLastError());
     return 0;
+}
+
+int launch_diffusion_wall(vx_case *c, int64_t n_tgt) {
+    return launch_wall_side(c, n_tgt, c->stream) || launch_wall_add(c);
 }
 
 }  // namespace vx
