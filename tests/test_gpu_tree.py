@@ -168,6 +168,38 @@ def test_tree_and_direct_speed_agree_inside_a_time_step(ref):
     assert abs(f0[-1][1] - f1[-1][1]) < 1e-5 * max(1.0, abs(f0[-1][1]))
 
 
+@pytest.mark.parametrize("prec,order", [(VX_PREC_MIXED, 16), (VX_PREC_F64, 32)])
+def test_fused_time_step_matches_the_stage_calls(prec, order):
+    """vx_time_step builds the tree on a second stream while the birth stage
+    runs (the newborns' positions taken from the case's birth points); the
+    three stage calls run one after the other.  Both must give the same state
+    and forces, bit for bit, over several steps."""
+    pan = geo.circle(300, 0.5)
+    prm = make_params(pan, dt=0.01, r_col_same_sign=0.3, r_col_diff_sign=0.3, birth_offset_rel=0.5,
+                      precision=prec, tree_min_n=2000, order=order)
+    x, y, g = wake_cloud(20000, 5)
+    res = []
+    for fused in (True, False):
+        c = Case(prm, pan)
+        c.build_matrix()
+        c.set_state(x, y, g)
+        fs = []
+        for _ in range(3):
+            if fused:
+                fs.append(c.time_step((1.0, 0.0)))
+            else:
+                c.vort_creation((1.0, 0.0))
+                c.speed((1.0, 0.0))
+                fs.append(c.step())
+        res.append((c.n, c.state(), np.array(fs)))
+        c.close()
+    (n0, s0, f0), (n1, s1, f1) = res
+    assert n0 == n1
+    for a, b in zip(s0, s1):
+        assert np.array_equal(a, b)
+    assert np.array_equal(f0, f1)
+
+
 def test_tree_one_million(ref):
     """N = 2^20 uniform wake: mixed tree against the FP64 tree with 32 terms,
     and the FP64 tree against the oracle's direct sum on a target sample."""

@@ -137,6 +137,24 @@ int vx_case_create(const vx_params *prm, const vx_panel *panels, int64_t P, vx_c
         VX_CHECK(hipMemcpy(c->seg_of, seg_of.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice));
         c->nseg = K;
     }
+    // newborn positions, and whether all of them lie well inside the tree's
+    // inclusion margin (400 around the deletion box), which lets the tree
+    // include them before their circulations are known
+    {
+        VX_CHECK(hipMalloc((void **)&c->bpos, sizeof(double) * 2 * P));
+        if (launch_birth_positions(c)) return 1;
+        std::vector<double> bp((size_t)(2 * P));
+        VX_CHECK(hipMemcpyAsync(bp.data(), c->bpos, sizeof(double) * 2 * P, hipMemcpyDeviceToHost, c->stream));
+        VX_CHECK(hipStreamSynchronize(c->stream));
+        bool near = true;
+        for (int64_t i = 0; i < P; ++i) {
+            const double bx = bp[(size_t)i], by = bp[(size_t)(P + i)];
+            const double ex = std::max(0.0, std::max(c->prm.x_min - bx, bx - c->prm.x_max));
+            const double ey = std::max(0.0, std::max(c->prm.y_min - by, by - c->prm.y_max));
+            near = near && std::isfinite(bx) && std::isfinite(by) && ex * ex + ey * ey <= 399.0 * 399.0;
+        }
+        c->births_near = near;
+    }
     if (reserve(c, c->prm.inc_step > 0 ? c->prm.inc_step : 8192)) {
         vx_case_destroy(c);
         return 1;
@@ -162,12 +180,13 @@ int vx_case_destroy(vx_case *c) {
     tree_release(c);
     if (c->pan_block) (void)hipFree(c->pan_block);
     if (c->segs) (void)hipFree(c->segs);
+    if (c->bpos) (void)hipFree(c->bpos);
     if (c->h_count) (void)hipHostFree(c->h_count);
     if (c->h_red) (void)hipHostFree(c->h_red);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
-    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+    for (hipEvent_t e : {c->fork_ev, c->join_ev, c->pre_ev, c->born_ev})
+        if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -410,7 +429,19 @@ int vx_vort_creation(vx_case *c, double vinf_x, double vinf_y) {
     VX_ARG(c && c->have_matrix, "no panel matrix");
     Timer t(c, &c->tm.creation_ms, 0);
     if (reserve(c, ceil32(c->n + c->P))) return 1;
+    // the speed stage's tree build may start before the newborns are written
+    // (vx_time_step only, without stage timing)
+    const int64_t s_next = ceil32(c->n + c->P);
+    const bool ov = c->overlap_births && !c->timing && c->births_near && c->prm.tree_min_n > 0 &&
+                    s_next >= c->prm.tree_min_n;
+    c->born_at = -1;
+    if (ov) {
+        if (side_stream(c)) return 1;
+        VX_CHECK(hipEventRecord(c->pre_ev, c->stream));
+        if (tree_build_early(c, s_next, c->n)) return 1;
+    }
     if (launch_right_part(c, ceil32(c->n), vinf_x, vinf_y) || launch_birth(c, c->d_g)) return 1;
+    if (ov) VX_CHECK(hipEventRecord(c->born_ev, c->stream));
     c->n += c->P;
     return 0;
 }
@@ -475,7 +506,14 @@ int vx_step(vx_case *c, double out4[4]) {
 
 int vx_time_step(vx_case *c, double vinf_x, double vinf_y, double forces4[4]) {
     VX_ARG(c && forces4, "null pointer");
-    if (vx_vort_creation(c, vinf_x, vinf_y) || vx_speed(c, vinf_x, vinf_y) || vx_step(c, forces4)) return 1;
+    c->overlap_births = true;
+    const int rc = vx_vort_creation(c, vinf_x, vinf_y) || vx_speed(c, vinf_x, vinf_y);
+    c->overlap_births = false;
+    if (c->born_at >= 0) {  // an early tree part left behind by a failed stage
+        c->born_at = -1;
+        (void)side_join(c);
+    }
+    if (rc || vx_step(c, forces4)) return 1;
     return 0;
 }
 

@@ -103,9 +103,19 @@ struct vx_case {
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     bool wall_pending = false;                        // a wall pass forked and not yet joined
+    // Within vx_time_step, the tree build of the speed stage starts on the
+    // side stream while the birth stage runs on the main stream: pre_ev marks
+    // the state before the birth, born_ev the newborns written, born_at their
+    // first slot (-1: no such overlap pending).
+    bool overlap_births = false;
+    hipEvent_t pre_ev = nullptr, born_ev = nullptr;
+    int64_t born_at = -1;
+    int64_t tree_early_s = 0;                         // the size tree_build_early started
     vx::PanelsSoA pan{};
     void *pan_block = nullptr;
     vx::RhsSeg *segs = nullptr;                       // birth RHS segments (device)
+    double *bpos = nullptr;                           // newborn positions: x [0, P), y [P, 2P) (device)
+    bool births_near = false;                         // every newborn position within the tree's margin
     int32_t *seg_of = nullptr;                        // segment of each panel (device)
     int64_t nseg = 0;
     int64_t P = 0;
@@ -176,6 +186,7 @@ struct Timer {
 // ---- launchers (kernels in the .hip units) --------------------------------
 // vx_state.hip
 int launch_fill_tail(vx_case *c, int64_t from, int64_t to);
+int side_stream(vx_case *c);  // creates the side stream and its events once
 int side_fork(vx_case *c);  // the side stream waits for the work queued so far on the main stream
 int side_join(vx_case *c);  // the main stream waits for the work queued so far on the side stream
 int launch_pack_f32(vx_case *c, int64_t count);
@@ -196,8 +207,10 @@ int launch_diffusion_wall(vx_case *c, int64_t n_tgt);
 int launch_panel_matrix(vx_case *c);                   // -> c->L
 int launch_invert(vx_case *c);                         // c->L -> c->M (L destroyed copy)
 int launch_birth(vx_case *c, double d_g);              // c->rp -> c->gsheet, newborns at [n, n+P)
+int launch_birth_positions(vx_case *c);                // -> c->bpos (once per case)
 // vx_tree.hip
 int tree_build(vx_case *c, int64_t s);
+int tree_build_early(vx_case *c, int64_t s, int64_t at);  // part A on the side stream (birth overlap)
 int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y);
 int launch_tree_diffusion(vx_case *c);  // reuses the tree of the last launch_tree_biot_savart
 // launch_tree_diffusion in three parts: pre (main stream), far (any stream

@@ -224,14 +224,30 @@ __global__ __launch_bounds__(256) void k_birth_gemv(const double *__restrict__ M
     if (lane == 0) gs[row] = acc + Mr[P] * d_g;
 }
 
-__global__ void k_birth_place(const double *__restrict__ gs, PanelsSoA pan, double off_rel, int64_t n,
+// Newborn positions, one per panel: the panel midpoint moved off along the
+// normal (computed once per case; bp holds x at [0, P), y at [P, 2P)).
+__global__ void k_birth_pos(PanelsSoA pan, double off_rel, double *__restrict__ bp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pan.P) return;
+    const double off = off_rel > 0.0 ? off_rel * pan.len[i] : 1e-6;
+    bp[i] = (pan.lx[i] + pan.rx[i]) / 2 + off * pan.nx[i];
+    bp[pan.P + i] = (pan.ly[i] + pan.ry[i]) / 2 + off * pan.ny[i];
+}
+
+int launch_birth_positions(vx_case *c) {
+    hipLaunchKernelGGL(k_birth_pos, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, c->pan,
+                       c->prm.birth_offset_rel, c->bpos);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_birth_place(const double *__restrict__ gs, PanelsSoA pan, const double *__restrict__ bp, int64_t n,
                               double *__restrict__ x, double *__restrict__ y, double *__restrict__ g) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= pan.P) return;
     const int64_t nx = pan.rp[i];
-    const double off = off_rel > 0.0 ? off_rel * pan.len[i] : 1e-6;
-    x[n + i] = (pan.lx[i] + pan.rx[i]) / 2 + off * pan.nx[i];
-    y[n + i] = (pan.ly[i] + pan.ry[i]) / 2 + off * pan.ny[i];
+    x[n + i] = bp[i];
+    y[n + i] = bp[pan.P + i];
     g[n + i] = 0.5 * (gs[i] * pan.len[i] + gs[nx] * pan.len[nx]);
 }
 
@@ -241,7 +257,7 @@ int launch_birth(vx_case *c, double d_g) {
     hipLaunchKernelGGL(k_birth_gemv, dim3((unsigned)((P + 3) / 4)), dim3(256), 0, c->stream, c->M.p, c->rp.p, d_g,
                        P, c->gsheet.p);
     hipLaunchKernelGGL(k_birth_place, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream, c->gsheet.p,
-                       c->pan, c->prm.birth_offset_rel, c->n, c->x.p, c->y.p, c->g.p);
+                       c->pan, c->bpos, c->n, c->x.p, c->y.p, c->g.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }

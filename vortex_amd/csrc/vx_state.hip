@@ -26,12 +26,19 @@ __global__ void k_fill_tail(double *__restrict__ x, double *__restrict__ y, doub
     g[i] = 0.0;
 }
 
-int side_fork(vx_case *c) {
+int side_stream(vx_case *c) {
     if (!c->side) {
         VX_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         VX_CHECK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
         VX_CHECK(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+        VX_CHECK(hipEventCreateWithFlags(&c->pre_ev, hipEventDisableTiming));
+        VX_CHECK(hipEventCreateWithFlags(&c->born_ev, hipEventDisableTiming));
     }
+    return 0;
+}
+
+int side_fork(vx_case *c) {
+    if (side_stream(c)) return 1;
     VX_CHECK(hipEventRecord(c->fork_ev, c->stream));
     VX_CHECK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
     return 0;

@@ -127,18 +127,32 @@ __device__ __forceinline__ uint32_t spread16(uint32_t v) {
 }
 
 // ---- build -----------------------------------------------------------------
+// Newborn slots [at, at + P) whose positions come from bp (x at [0, P), y at
+// [P, 2P)) instead of the state arrays, which the birth may still be writing;
+// they lie inside the inclusion margin, so they are included whatever their
+// circulation.  P = 0: none.
+struct Births {
+    int64_t at, P;
+    const double *bp;
+    __device__ bool has(int64_t i) const { return i >= at && i - at < P; }
+};
+
 __global__ __launch_bounds__(256) void k_incl_box(const double *__restrict__ x, const double *__restrict__ y,
                                                   const double *__restrict__ g, int64_t s, double4 dbox,
                                                   int32_t *__restrict__ incl, double4 *__restrict__ part,
-                                                  int32_t *__restrict__ pcount) {
+                                                  int32_t *__restrict__ pcount, Births B) {
     __shared__ double r[4][256];
     __shared__ int c[256];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
     int in = 0;
     if (i < s) {
-        const double xi = x[i], yi = y[i], gi = g[i];
-        if (isfinite(xi) && isfinite(yi)) {
+        const bool nb = B.has(i);
+        const double xi = nb ? B.bp[i - B.at] : x[i], yi = nb ? B.bp[B.P + i - B.at] : y[i];
+        if (nb) {
+            in = 1;
+        } else if (isfinite(xi) && isfinite(yi)) {
+            const double gi = g[i];
             const double ex = fmax(0.0, fmax(dbox.x - xi, xi - dbox.y));
             const double ey = fmax(0.0, fmax(dbox.z - yi, yi - dbox.w));
             in = (gi != 0.0) || (ex * ex + ey * ey <= 400.0 * 400.0);
@@ -195,7 +209,7 @@ __global__ __launch_bounds__(256) void k_box_final(const double4 *__restrict__ p
 
 __global__ void k_tree_keys(const double *__restrict__ x, const double *__restrict__ y,
                             const int32_t *__restrict__ incl, int64_t s, const double4 *__restrict__ box,
-                            uint32_t *__restrict__ keys, int32_t *__restrict__ vals) {
+                            uint32_t *__restrict__ keys, int32_t *__restrict__ vals, Births B) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s) return;
     vals[i] = (int32_t)i;
@@ -205,8 +219,10 @@ __global__ void k_tree_keys(const double *__restrict__ x, const double *__restri
     }
     const double4 b = box[0];
     const double wx = b.y - b.x, wy = b.w - b.z;
-    double fx = wx > 0 ? (x[i] - b.x) / wx * 65534.0 : 0.0;
-    double fy = wy > 0 ? (y[i] - b.z) / wy * 65534.0 : 0.0;
+    const bool nb = B.has(i);
+    const double xi = nb ? B.bp[i - B.at] : x[i], yi = nb ? B.bp[B.P + i - B.at] : y[i];
+    double fx = wx > 0 ? (xi - b.x) / wx * 65534.0 : 0.0;
+    double fy = wy > 0 ? (yi - b.z) / wy * 65534.0 : 0.0;
     fx = fmin(fmax(fx, 0.0), 65534.0);
     fy = fmin(fmax(fy, 0.0), 65534.0);
     keys[i] = spread16((uint32_t)fx) | (spread16((uint32_t)fy) << 1);  // at most 0xFFFFFFFC
@@ -1763,9 +1779,12 @@ using KeySortConfig = rocprim::radix_sort_config<rocprim::default_config, rocpri
 
 static int tree_order(const vx_case *c) { return c->prm.order >= 32 ? 32 : 16; }
 
-int tree_build(vx_case *c, int64_t s) {
+// Builds the tree on stream st in two parts: A (inclusion, box, keys, sort;
+// no host wait) and B (the rest, after the included count is back).  With
+// B.P > 0 the newborns' positions come from B and the gather of the sorted
+// particles first waits for born_ev.
+static int tree_build_a(vx_case *c, int64_t s, hipStream_t st, Births B) {
     TreeBufs &t = tree_of(c);
-    hipStream_t st = c->stream;
     const int order = tree_order(c);
     t.order = order;
     const int64_t nbs = (s + 255) / 256;
@@ -1775,20 +1794,26 @@ int tree_build(vx_case *c, int64_t s) {
     VX_CHECK(hipMemsetAsync(t.cnt.p, 0, sizeof(unsigned long long) * 16, st));
     const double4 dbox = make_double4(c->prm.x_min, c->prm.x_max, c->prm.y_min, c->prm.y_max);
     hipLaunchKernelGGL(k_incl_box, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, s, dbox, t.incl.p,
-                       t.part.p, t.pcount.p);
+                       t.part.p, t.pcount.p, B);
     hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, st, t.part.p, t.pcount.p, nbs, t.part.p + nbs, t.cnt.p);
     // the included count is read back while the keys are sorted
     if (!t.ev) VX_CHECK(hipEventCreateWithFlags(&t.ev, hipEventDisableTiming));
     VX_CHECK(hipMemcpyAsync(c->h_count, t.cnt.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     VX_CHECK(hipEventRecord(t.ev, st));
     hipLaunchKernelGGL(k_tree_keys, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, t.incl.p, s,
-                       t.part.p + nbs, t.keys.p, t.idx.p);
+                       t.part.p + nbs, t.keys.p, t.idx.p, B);
     size_t bytes = 0;
     VX_CHECK(rocprim::radix_sort_pairs<KeySortConfig>(nullptr, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p,
                                                       (unsigned)s, 0, 32, st));
     if (t.tmp.ensure(bytes, st)) return 1;
     VX_CHECK(rocprim::radix_sort_pairs<KeySortConfig>(t.tmp.p, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p,
                                                       (unsigned)s, 0, 32, st));
+    return 0;
+}
+
+static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
+    TreeBufs &t = *c->tree;
+    const int order = t.order;
     VX_CHECK(hipEventSynchronize(t.ev));
     const int64_t m = (int64_t)*c->h_count;
     t.m = m;
@@ -1841,6 +1866,7 @@ int tree_build(vx_case *c, int64_t s) {
     VX_CHECK(hipMemcpyAsync(c->h_count, t.sizes.p, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     VX_CHECK(hipEventRecord(t.ev, st));
     // the sorted particles are gathered while the counts come back
+    if (B.P > 0) VX_CHECK(hipStreamWaitEvent(st, c->born_ev, 0));
     hipLaunchKernelGGL(k_tree_gather, dim3(gm), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, t.perm.p, m, t.sx.p, t.sy.p,
                        t.sg.p);
     VX_CHECK(hipEventSynchronize(t.ev));
@@ -1905,6 +1931,38 @@ int tree_build(vx_case *c, int64_t s) {
                        t.nr.p, t.child0.p, t.child1.p, t.first.p, t.count.p, t.rec.p);
     VX_CHECK(hipGetLastError());
     return 0;
+}
+
+// Part A of the next tree build on the side stream, queued before the birth
+// stage's work: it waits for the state before the birth (pre_ev) and takes
+// the newborns' positions at [at, at + P) from the case's birth points.
+int tree_build_early(vx_case *c, int64_t s, int64_t at) {
+    if (side_stream(c)) return 1;
+    VX_CHECK(hipStreamWaitEvent(c->side, c->pre_ev, 0));
+    const int rc = tree_build_a(c, s, c->side, Births{at, c->P, c->bpos});
+    if (rc) {
+        (void)side_join(c);
+        return rc;
+    }
+    c->born_at = at;
+    c->tree_early_s = s;
+    return 0;
+}
+
+// With part A queued by tree_build_early, part B follows on the side stream
+// and the main stream continues after it; else the whole build runs on the
+// main stream.
+int tree_build(vx_case *c, int64_t s) {
+    if (c->born_at < 0) {
+        const Births none{-1, 0, nullptr};
+        return tree_build_a(c, s, c->stream, none) || tree_build_b(c, c->stream, none);
+    }
+    const Births B{c->born_at, c->P, c->bpos};
+    c->born_at = -1;
+    VX_ARG(s == c->tree_early_s, "tree size differs from the one started with the birth stage");
+    const int rc = tree_build_b(c, c->side, B);
+    const int rj = side_join(c);
+    return rc ? rc : rj;
 }
 
 static double theta_of(const vx_case *c) { return c->prm.theta > 0 ? c->prm.theta : 0.5; }
