@@ -200,6 +200,32 @@ def test_fused_time_step_matches_the_stage_calls(prec, order):
     assert np.array_equal(f0, f1)
 
 
+@pytest.mark.parametrize("prec,order", [(VX_PREC_MIXED, 16), (VX_PREC_F64, 32)])
+def test_speed_stage_matches_the_separate_kernels(prec, order):
+    """vx_speed runs the wall term and the domain-scale diffusion targets on
+    a second stream beside the groups' diffusion; the separate entry points
+    run everything in order on one stream.  Same u, v, d bit for bit."""
+    pan = geo.circle(300, 0.5)
+    prm = make_params(pan, dt=0.01, birth_offset_rel=0.5, precision=prec, tree_min_n=2000, order=order)
+    x, y, g = wake_cloud(20000, 7)
+    s = (x.size + 31) // 32 * 32
+    out = []
+    for split in (False, True):
+        c = Case(prm, pan)
+        c.set_state(x, y, g)
+        if split:
+            c.tree_biot_savart(s, (1.0, 0.0))
+            c.tree_diffusion()
+            c.diffusion_wall(s)
+        else:
+            c.speed((1.0, 0.0))
+        assert c.tree_stats()["fallback"] > 0  # the isolated vortices take the exact search
+        out.append(c.velocity(x.size))
+        c.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_tree_one_million(ref):
     """N = 2^20 uniform wake: mixed tree against the FP64 tree with 32 terms,
     and the FP64 tree against the oracle's direct sum on a target sample."""

@@ -440,9 +440,7 @@ struct ColBufs {
 static int col_bufs(vx_case *c, int64_t n_ub, ColBufs &b) {
     hipStream_t st = c->stream;
     if (c->flags.ensure((size_t)n_ub, st) || c->scan.ensure((size_t)n_ub, st) ||
-        c->lists.ensure((size_t)(3 * n_ub), st) || c->misc.ensure((size_t)n_ub, st) ||
-        c->tx.ensure((size_t)n_ub, st) || c->ty.ensure((size_t)n_ub, st) || c->tg.ensure((size_t)n_ub, st) ||
-        c->dcount.ensure(16, st))
+        c->lists.ensure((size_t)(3 * n_ub), st) || c->misc.ensure((size_t)n_ub, st) || c->dcount.ensure(16, st))
         return 1;
     b.pc = c->flags.p;
     b.state = c->scan.p;
@@ -806,53 +804,49 @@ __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict_
     }
 }
 
+// Each active candidate i merges with j = COL_i: the smallest active
+// candidate of j (first[j] == i) with j as it was, then kills j; the others
+// with the killed j (1e10, 1e10, 0).  No two threads touch one slot: an
+// active i is never the target of an active candidate (that would have made
+// it inactive), so x[i] is read and written by thread i alone, and x[j] by
+// j's first merger alone.
 template <int KIND>
-__global__ void k_col_merge(const double *__restrict__ x, const double *__restrict__ y,
-                            const double *__restrict__ g, const int32_t *__restrict__ col,
-                            const int32_t *__restrict__ cand, const unsigned long long *__restrict__ ncand_p,
-                            const int32_t *__restrict__ state, const int32_t *__restrict__ first,
-                            double *__restrict__ tx, double *__restrict__ ty, double *__restrict__ tg) {
+__global__ void k_col_merge(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
+                            const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
+                            const unsigned long long *__restrict__ ncand_p, const int32_t *__restrict__ state,
+                            const int32_t *__restrict__ first) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)*ncand_p) return;
     const int32_t i = cand[t];
     if (!(state[i] & 1)) return;
     const int32_t j = col[i];
+    const bool fst = first[j] == i;
     double xj = 1e+10, yj = 1e+10, gj = 0.0;
-    if (first[j] == i) {
+    if (fst) {
         xj = x[j];
         yj = y[j];
         gj = g[j];
     }
     const double xi = x[i], yi = y[i], gi = g[i];
+    double nx, ny, ng;
     if (KIND == 1) {
         const double mn = fabs(gi) + fabs(gj);
-        tx[t] = (xi * fabs(gi) + xj * fabs(gj)) / fabs(mn);
-        ty[t] = (yi * fabs(gi) + yj * fabs(gj)) / fabs(mn);
-        tg[t] = gi + gj;
+        nx = (xi * fabs(gi) + xj * fabs(gj)) / fabs(mn);
+        ny = (yi * fabs(gi) + yj * fabs(gj)) / fabs(mn);
+        ng = gi + gj;
     } else {
-        const double ng = gi + gj;
-        tx[t] = (xi * gi + xj * gj) / ng;
-        ty[t] = (yi * gi + yj * gj) / ng;
-        tg[t] = ng;
+        ng = gi + gj;
+        nx = (xi * gi + xj * gj) / ng;
+        ny = (yi * gi + yj * gj) / ng;
     }
-}
-
-__global__ void k_col_write(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
-                            const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
-                            const unsigned long long *__restrict__ ncand_p, const int32_t *__restrict__ state,
-                            const double *__restrict__ tx, const double *__restrict__ ty,
-                            const double *__restrict__ tg) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)*ncand_p) return;
-    const int32_t i = cand[t];
-    if (!(state[i] & 1)) return;
-    const int32_t j = col[i];
-    x[i] = tx[t];
-    y[i] = ty[t];
-    g[i] = tg[t];
-    x[j] = 1e+10;
-    y[j] = 1e+10;
-    g[j] = 0.0;
+    x[i] = nx;
+    y[i] = ny;
+    g[i] = ng;
+    if (fst) {
+        x[j] = 1e+10;
+        y[j] = 1e+10;
+        g[j] = 0.0;
+    }
 }
 
 int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared) {
@@ -876,12 +870,10 @@ int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared) {
                        cb.first, (int32_t *)(c->dcount.p + 3));
     if (kind == 1)
         hipLaunchKernelGGL(k_col_merge<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb.cand,
-                           cb.ncand, cb.state, cb.first, c->tx.p, c->ty.p, c->tg.p);
+                           cb.ncand, cb.state, cb.first);
     else
         hipLaunchKernelGGL(k_col_merge<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb.cand,
-                           cb.ncand, cb.state, cb.first, c->tx.p, c->ty.p, c->tg.p);
-    hipLaunchKernelGGL(k_col_write, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb.cand, cb.ncand,
-                       cb.state, c->tx.p, c->ty.p, c->tg.p);
+                           cb.ncand, cb.state, cb.first);
     VX_CHECK(hipGetLastError());
     return 0;
 }

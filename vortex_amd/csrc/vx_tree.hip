@@ -1279,6 +1279,8 @@ __global__ __launch_bounds__(64) void k_tree_nearest_fallback(TreeDev t, const i
                                                               const unsigned long long *__restrict__ nfb,
                                                               double *__restrict__ d) {
     __shared__ int32_t stk[kFbStack];
+    __shared__ int32_t path[kLevCap], sibs[kLevCap];
+    __shared__ double sdist[kLevCap];
     const int lane = threadIdx.x;
     const int64_t nq = (int64_t)*nfb;
     for (int64_t q = blockIdx.x; q < nq; q += gridDim.x) {
@@ -1291,13 +1293,33 @@ __global__ __launch_bounds__(64) void k_tree_nearest_fallback(TreeDev t, const i
         near4_buckets(t, 1ull, lane == 0 ? cur : -1, lane, ax, ay, m0, m1, m2, m3);
         near4_merge(lane, m0, m1, m2, m3, r);
         lim = fmin(lim, r[3]);
-        bool full = false;
-        while (cur != t.root) {
-            const int p = t.parent[cur];
-            const int sib = t.child0[p] == cur ? t.child1[p] : t.child0[p];
-            cur = p;
-            if (box_d2(t.nbb[sib], ax, ay) > lim) continue;  // nothing below the bound
-            if (!near4_subtree(t, stk, sib, lane, ax, ay, lim, m0, m1, m2, m3)) {
+        // the path to the root (path[k]: the node k levels above the bucket)
+        // by one chain of parent loads; then each level's other child and
+        // its box distance, one level per lane; then the levels bottom-up
+        int L = 0;
+        bool full;
+        {
+            const int32_t *__restrict__ par = t.parent;
+            int c0 = cur;
+            if (lane == 0) path[0] = c0;
+            while (c0 != t.root && L + 1 < kLevCap) {
+                c0 = par[c0];
+                ++L;
+                if (lane == 0) path[L] = c0;
+            }
+            full = c0 != t.root;  // deeper than kLevCap levels (never, by the key length)
+        }
+        __syncthreads();
+        for (int k = lane; k < L; k += 64) {
+            const NodeRec pr = t.rec[path[k + 1]];
+            const int sb = pr.a == path[k] ? pr.b : pr.a;
+            sibs[k] = sb;
+            sdist[k] = box_d2(t.nbb[sb], ax, ay);
+        }
+        __syncthreads();
+        for (int k = 0; k < L && !full; ++k) {
+            if (sdist[k] > lim) continue;  // nothing below the bound
+            if (!near4_subtree(t, stk, sibs[k], lane, ax, ay, lim, m0, m1, m2, m3)) {
                 full = true;
                 break;
             }
