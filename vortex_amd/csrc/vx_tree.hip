@@ -956,6 +956,8 @@ __device__ __forceinline__ int64_t xcd_block() {
     return (int64_t)(blockIdx.x & 7u) * q + (int64_t)(blockIdx.x >> 3);
 }
 
+constexpr int kBsStack = 512;  // traversal stack entries per wave
+
 template <typename T, int ORDER>
 __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double eps, double vinf_x, double vinf_y,
                                         double *__restrict__ u, double *__restrict__ v, double *__restrict__ d,
@@ -966,7 +968,9 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     using PM = PairMath<T>;
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
-    __shared__ int32_t fq_all[4][64];     // FP32: far nodes waiting for M2L
+    __shared__ int32_t fq_all[4][128];    // FP32: far nodes waiting for M2L
+    __shared__ int32_t stk_all[4][kBsStack];
+    __shared__ int32_t lq_all[4][64];     // near leaves of the popped nodes
     __shared__ float2 tr_all[4][4][64];   // FP32: M2L sums over the wave
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
@@ -1014,50 +1018,95 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
         su += (double)pu;
         sv += (double)pv;
     }
-    LaneStack st;
-    st.push(lane, t.root);
-    while (st.sp > 0) {
-        const int node = st.pop();
-        if (node == (int)leaf) continue;
-        const NodeRec nd = t.rec[node];
-        const double ddx = nd.cx - cAx, ddy = nd.cy - cAy;
-        const double d2 = ddx * ddx + ddy * ddy;
-        const double rs = nd.r + rA;
-        // dist * theta > r_node + r_leaf and dist - r_node - r_leaf > eps
-        if (d2 * th2 > rs * rs && d2 > (rs + eps) * (rs + eps)) {
-            ++nfar;
-            if constexpr (F32) {
-                // queued for M2L into the bucket's local expansion, 64 at a time
-                if (lane == 0) fq[nq] = node;
-                if (++nq == 64) {
-                    __builtin_amdgcn_wave_barrier();
-                    m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, tr, lacc, gap);
-                    __builtin_amdgcn_wave_barrier();
-                    cm2l += (unsigned long long)nq;
-                    nq = 0;
-                }
-                continue;
-            } else {
-                // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
+    // Traversal: a stack in the wave's LDS slot, popped up to 64 nodes at a
+    // time, one per lane, which tests its node's MAC; far nodes are queued
+    // (FP32: for M2L; FP64: evaluated one at a time), near leaves summed one
+    // at a time, opened nodes' children pushed.  While the stack is within
+    // 64 + 54 entries of its capacity, nodes are popped one at a time: a
+    // depth-first descent adds at most one entry per level (the tree is at
+    // most 54 levels deep), so the stack never outgrows kBsStack.
+    int32_t *stk = stk_all[threadIdx.x >> 6];
+    int32_t *lq = lq_all[threadIdx.x >> 6];
+    int sp = 1;
+    if (lane == 0) stk[0] = t.root;
+    while (sp > 0) {
+        const int k = sp + 64 + 54 <= kBsStack ? (sp < 64 ? sp : 64) : 1;
+        __builtin_amdgcn_wave_barrier();
+        const int node = lane < k ? stk[sp - 1 - lane] : -1;
+        __builtin_amdgcn_wave_barrier();
+        sp -= k;
+        bool isfar = false, isleaf = false, isopen = false;
+        int ca = 0, cb = 0;
+        if (node >= 0 && node != (int)leaf) {
+            const NodeRec nd = t.rec[node];
+            const double ddx = nd.cx - cAx, ddy = nd.cy - cAy;
+            const double d2 = ddx * ddx + ddy * ddy;
+            const double rs = nd.r + rA;
+            // dist * theta > r_node + r_leaf and dist - r_node - r_leaf > eps
+            isfar = d2 * th2 > rs * rs && d2 > (rs + eps) * (rs + eps);
+            isleaf = !isfar && nd.a < 0;
+            isopen = !isfar && !isleaf;
+            ca = nd.a;
+            cb = nd.b;
+        }
+        // children of the opened nodes, the first child on top
+        const uint64_t om = __ballot(isopen);
+        if (isopen) {
+            const int pos = sp + 2 * __popcll(om & ((1ull << lane) - 1));
+            stk[pos] = cb;
+            stk[pos + 1] = ca;
+        }
+        sp += 2 * __popcll(om);
+        // near leaves listed in LDS (summed below, one at a time)
+        const uint64_t lm = __ballot(isleaf);
+        if (isleaf) lq[__popcll(lm & ((1ull << lane) - 1))] = node;
+        const int nl = __popcll(lm);
+        const uint64_t fm = __ballot(isfar);
+        nfar += __popcll(fm);
+        if constexpr (F32) {
+            // queued for M2L into the bucket's local expansion, 64 at a time
+            if (isfar) fq[nq + __popcll(fm & ((1ull << lane) - 1))] = node;
+            nq += __popcll(fm);
+            if (nq >= 64) {
+                __builtin_amdgcn_wave_barrier();
+                m2l_batch<ORDER>(t, fq, 64, lane, cAx, cAy, rT, tr, lacc, gap);
+                __builtin_amdgcn_wave_barrier();
+                cm2l += 64ull;
+                nq -= 64;
+                const int moved = lane < nq ? fq[64 + lane] : 0;
+                __builtin_amdgcn_wave_barrier();
+                if (lane < nq) fq[lane] = moved;
+                __builtin_amdgcn_wave_barrier();
+            }
+        } else {
+            // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
+            for (uint64_t m = fm; m; m &= m - 1) {
+                const int fnode = __builtin_amdgcn_readlane(node, __builtin_ctzll(m));
+                const NodeRec nd = t.rec[fnode];
                 const T Dx = ax - nd.cx, Dy = ay - nd.cy;
                 const T den = Dx * Dx + Dy * Dy;
                 const T inv = PM::div(T(1), den);
                 const T qx = Dx * inv, qy = -Dy * inv;
                 const T R = (T)fmax(nd.r, kTiny);
                 const T tx = R * qx, ty = R * qy;
-                const double2 *b = t.cd + (int64_t)node * ORDER;
-                T sr = b[ORDER - 1].x, si = b[ORDER - 1].y;
+                const double2 *bm = t.cd + (int64_t)fnode * ORDER;
+                T sr = bm[ORDER - 1].x, si = bm[ORDER - 1].y;
 #pragma unroll
-                for (int k = ORDER - 2; k >= 0; --k) {
-                    const T nr_ = sr * tx - si * ty + b[k].x;
-                    si = sr * ty + si * tx + b[k].y;
+                for (int q = ORDER - 2; q >= 0; --q) {
+                    const T nr_ = sr * tx - si * ty + bm[q].x;
+                    si = sr * ty + si * tx + bm[q].y;
                     sr = nr_;
                 }
                 fu += sr * qy + si * qx;
                 fv += sr * qx - si * qy;
                 farmin = fmin(farmin, PM::sqrt_bound(den) - (T)nd.r);
             }
-        } else if (nd.a < 0) {
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int li = 0; li < nl; ++li) {
+            const int lnode = __builtin_amdgcn_readfirstlane(lq[li]);
+            const NodeRec nd = t.rec[lnode];
+            const double ddx = nd.cx - cAx, ddy = nd.cy - cAy;
             const int64_t j0 = -1 - nd.a;
             const int cnt = nd.b;
             T ox, oy;
@@ -1091,9 +1140,6 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
             su += (double)pu + (double)fu;
             sv += (double)pv + (double)fv;
             fu = fv = 0;
-        } else {
-            st.push(lane, nd.b);
-            st.push(lane, nd.a);
         }
     }
     su += (double)fu;
@@ -1123,7 +1169,6 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     }
     cpairs = (unsigned long long)(npairs * nvalid);
     cfar = (unsigned long long)(nfar * nvalid);
-    if (lane == 0 && st.overflow) atomicAdd(nfb + 4, 1ull);
     if (!ok) return;
     const int32_t i = t.perm[ti];
     u[i] = su / (2 * kPi) + vinf_x;
