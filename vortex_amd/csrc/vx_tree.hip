@@ -757,7 +757,37 @@ template <typename T, bool TRACK, bool CLAMP = true>
 __device__ __forceinline__ void p2p_leaf(const Src4T<T> *lds, int cnt, T ox, T oy, T eps2, T &pu, T &pv, T &m0,
                                          T &m1, T &m2, T &m3) {
     using PM = PairMath<T>;
-    for (int jj = 0; jj < cnt; ++jj) {
+    int jj = 0;
+    if constexpr (sizeof(T) == 4) {
+        // two sources at a time with one quarter-rate reciprocal:
+        // 1/a = b / (a b), 1/b = a / (a b).  For a target lane, a, b lie in
+        // [eps^2, 1e6] (the included vortices lie within 400 of the domain),
+        // so the product neither overflows nor underflows in FP32; lanes
+        // without a target may produce inf or NaN, which are never stored.
+        for (; jj + 1 < cnt; jj += 2) {
+            const Src4T<T> q = lds[jj], p = lds[jj + 1];
+            const T dx = ox - q.x, dy = oy - q.y, ex = ox - p.x, ey = oy - p.y;
+            const T r2 = dx * dx + dy * dy, s2 = ex * ex + ey * ey;
+            if constexpr (TRACK) {
+                m3 = PM::med3(m2, m3, r2);
+                m2 = PM::med3(m1, m2, r2);
+                m1 = PM::med3(m0, m1, r2);
+                m0 = fmin(m0, r2);
+                m3 = PM::med3(m2, m3, s2);
+                m2 = PM::med3(m1, m2, s2);
+                m1 = PM::med3(m0, m1, s2);
+                m0 = fmin(m0, s2);
+            }
+            const T a = CLAMP ? fmax(r2, eps2) : r2, b = CLAMP ? fmax(s2, eps2) : s2;
+            const T rab = __builtin_amdgcn_rcpf(a * b);
+            const T w = q.g * (b * rab), v = p.g * (a * rab);
+            pv += w * dx;
+            pu -= w * dy;
+            pv += v * ex;
+            pu -= v * ey;
+        }
+    }
+    for (; jj < cnt; ++jj) {
         const Src4T<T> q = lds[jj];
         const T dx = ox - q.x, dy = oy - q.y;
         const T r2 = dx * dx + dy * dy;
