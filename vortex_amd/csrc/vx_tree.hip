@@ -596,8 +596,114 @@ struct IsInternal {
     __device__ bool operator()(int32_t u) const { return u >= nb; }
 };
 
+// A cut subtree's internal nodes held in LDS while it is built (slot = list
+// index): children (an internal child's slot, or -1 - the id of a bucket,
+// read from global memory), geometry and multipole coefficients.  About
+// 86 KB at 16 coefficients.
+template <int ORDER>
+struct SubLds {
+    int32_t lst[kCut];
+    int32_t ch[kCut][2];
+    int32_t lev[kLevCap];
+    int ntot;
+    double gx[kCut], gy[kCut], gr[kCut];
+    double4 gb[kCut];
+    double2 c[kCut][ORDER];
+};
+
+// m2m_coop with the children read from LDS (internal) or global memory
+// (buckets) and the result kept in LDS slot idx; the same arithmetic.
+template <int ORDER>
+__device__ __forceinline__ void m2m_lds(SubLds<ORDER> &L, int idx, int k, const double *__restrict__ ncx,
+                                        const double *__restrict__ ncy, const double *__restrict__ nr,
+                                        const double4 *__restrict__ nbb, const double2 *__restrict__ cd,
+                                        double2 *ws, const double (*bin)[ORDER]) {
+    const bool on = idx >= 0;
+    double ccx[2] = {0.0, 0.0}, ccy[2] = {0.0, 0.0}, crr[2] = {0.0, 0.0};
+    double4 cb[2] = {make_double4(0.0, 0.0, 0.0, 0.0), make_double4(0.0, 0.0, 0.0, 0.0)};
+    int32_t cs[2] = {0, 0};
+    if (on) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            cs[q] = L.ch[idx][q];
+            if (cs[q] >= 0) {
+                ccx[q] = L.gx[cs[q]];
+                ccy[q] = L.gy[cs[q]];
+                crr[q] = L.gr[cs[q]];
+                cb[q] = L.gb[cs[q]];
+            } else {
+                const int32_t bq = -1 - cs[q];
+                ccx[q] = ncx[bq];
+                ccy[q] = ncy[bq];
+                crr[q] = nr[bq];
+                cb[q] = nbb[bq];
+            }
+        }
+    }
+    double4 b = make_double4(0.0, 0.0, 0.0, 0.0);
+    double cx = 0.0, cy = 0.0, R = 0.0, Rs = 1.0;
+    if (on) {
+        b = make_double4(fmin(cb[0].x, cb[1].x), fmax(cb[0].y, cb[1].y), fmin(cb[0].z, cb[1].z),
+                         fmax(cb[0].w, cb[1].w));
+        cx = 0.5 * (b.x + b.y);
+        cy = 0.5 * (b.z + b.w);
+        double Rc = 0.0;
+        for (int ch = 0; ch < 2; ++ch) {
+            const double ex = ccx[ch] - cx, ey = ccy[ch] - cy;
+            Rc = fmax(Rc, sqrt(ex * ex + ey * ey) + crr[ch]);
+        }
+        R = fmin(0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z)), Rc);
+        Rs = fmax(R, kTiny);
+    }
+    double accr = 0.0, acci = 0.0;
+#pragma unroll 1
+    for (int ch = 0; ch < 2; ++ch) {
+        double dr = 0.0, di = 0.0, wr = 0.0, wi = 0.0;
+        if (on) {
+            dr = (ccx[ch] - cx) / Rs;
+            di = (ccy[ch] - cy) / Rs;
+            const double rho = crr[ch] / Rs;
+            double rk = 1.0, rb = rho;  // rho^k by binary powers
+#pragma unroll
+            for (int e = 0; (1 << e) < ORDER; ++e) {
+                if ((k >> e) & 1) rk *= rb;
+                rb *= rb;
+            }
+            const double2 bm = cs[ch] >= 0 ? L.c[cs[ch]][k] : cd[(int64_t)(-1 - cs[ch]) * ORDER + k];
+            wr = bm.x * rk;
+            wi = bm.y * rk;
+        }
+        __builtin_amdgcn_wave_barrier();
+        ws[k] = make_double2(wr, wi);
+        __builtin_amdgcn_wave_barrier();
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int m = 0; m < ORDER; ++m) {
+            const double2 w = ws[m];
+            const double cbm = bin[m][k];
+            const double nar = ar * dr - ai * di + cbm * w.x;
+            const double nai = ar * di + ai * dr + cbm * w.y;
+            ar = m <= k ? nar : ar;
+            ai = m <= k ? nai : ai;
+        }
+        accr += ar;
+        acci += ai;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (!on) return;
+    L.c[idx][k] = make_double2(accr, acci);
+    if (k == 0) {
+        L.gx[idx] = cx;
+        L.gy[idx] = cy;
+        L.gr[idx] = R;
+        L.gb[idx] = b;
+    }
+}
+
 // k_upper_local: a work-group per cut node builds its subtree (at most
-// kCut - 1 internal nodes, listed in LDS).
+// kCut - 1 internal nodes).  At 16 coefficients the subtree stays in LDS
+// until it is complete (SubLds), so a level waits on LDS rather than on
+// global memory; at 32 it is built through global memory (build_levels).
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_upper_local(const int32_t *__restrict__ cut, const int32_t *__restrict__ ncut,
                                                      int64_t nb, const int32_t *__restrict__ child0,
@@ -605,19 +711,82 @@ __global__ __launch_bounds__(256) void k_upper_local(const int32_t *__restrict__
                                                      double *__restrict__ ncy, double *__restrict__ nr,
                                                      double4 *__restrict__ nbb, double2 *__restrict__ cd,
                                                      float2 *__restrict__ cf) {
-    __shared__ int32_t lst[kCut];
-    __shared__ int32_t lev[kLevCap];
-    __shared__ int ntot;
     __shared__ double2 ws_all[4][64];
     __shared__ double bin[ORDER][ORDER];
     fill_binomials<ORDER>(bin);
     double2 *ws = ws_all[threadIdx.x >> 6] + ((threadIdx.x & 63) / ORDER) * ORDER;
     const int n = *ncut;
-    for (int w = blockIdx.x; w < n; w += gridDim.x) {
-        const int32_t r = cut[w];
-        if (r < nb) continue;  // a bucket: nothing to build (the same for every thread)
-        build_levels<ORDER, 4>(r, IsInternal{nb}, lst, lev, &ntot, child0, child1, ncx, ncy, nr, nbb, cd, cf, ws,
-                               bin);
+    if constexpr (ORDER <= 16) {
+        __shared__ SubLds<ORDER> L;
+        constexpr int G = 64 / ORDER;  // nodes per wave
+        const int wave = threadIdx.x >> 6, gi = (threadIdx.x & 63) / ORDER, k = threadIdx.x % ORDER;
+        for (int w = blockIdx.x; w < n; w += gridDim.x) {
+            const int32_t r = cut[w];
+            if (r < nb) continue;  // a bucket: nothing to build (the same for every thread)
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                L.lst[0] = r;
+                L.lev[0] = 0;
+                L.lev[1] = 1;
+                L.ntot = 1;
+            }
+            __syncthreads();
+            int nlev = 1;
+            for (int d = 0; d + 2 < kLevCap; ++d) {
+                const int s0 = L.lev[d], s1 = L.lev[d + 1];
+                for (int t = s0 + threadIdx.x; t < s1; t += 256) {
+                    const int32_t u = L.lst[t];
+                    const int32_t chq[2] = {child0[u], child1[u]};
+                    for (int q = 0; q < 2; ++q) {
+                        if (chq[q] >= nb) {
+                            const int pos = atomicAdd(&L.ntot, 1);
+                            L.lst[pos] = chq[q];
+                            L.ch[t][q] = pos;
+                        } else {
+                            L.ch[t][q] = -1 - chq[q];
+                        }
+                    }
+                }
+                __syncthreads();
+                const int e = L.ntot;
+                if (e == s1) break;
+                if (threadIdx.x == 0) L.lev[d + 2] = e;
+                nlev = d + 2;
+                __syncthreads();
+            }
+            for (int d = nlev - 1; d >= 0; --d) {
+                const int s0 = L.lev[d], s1 = L.lev[d + 1];
+                for (int base = s0 + wave * G; base < s1; base += 4 * G) {
+                    const int idx = base + gi;
+                    m2m_lds<ORDER>(L, idx < s1 ? idx : -1, k, ncx, ncy, nr, nbb, cd, ws, bin);
+                }
+                __syncthreads();
+            }
+            // the subtree to global memory
+            const int tot = L.ntot;
+            for (int t = threadIdx.x / ORDER; t < tot; t += 256 / ORDER) {
+                const int32_t u = L.lst[t];
+                const double2 c = L.c[t][k];
+                cd[(int64_t)u * ORDER + k] = c;
+                cf[(int64_t)u * ORDER + k] = make_float2((float)c.x, (float)c.y);
+                if (k == 0) {
+                    ncx[u] = L.gx[t];
+                    ncy[u] = L.gy[t];
+                    nr[u] = L.gr[t];
+                    nbb[u] = L.gb[t];
+                }
+            }
+        }
+    } else {
+        __shared__ int32_t lst[kCut];
+        __shared__ int32_t lev[kLevCap];
+        __shared__ int ntot;
+        for (int w = blockIdx.x; w < n; w += gridDim.x) {
+            const int32_t r = cut[w];
+            if (r < nb) continue;  // a bucket: nothing to build (the same for every thread)
+            build_levels<ORDER, 4>(r, IsInternal{nb}, lst, lev, &ntot, child0, child1, ncx, ncy, nr, nbb, cd, cf,
+                                   ws, bin);
+        }
     }
 }
 
