@@ -700,6 +700,81 @@ __device__ __forceinline__ void m2m_lds(SubLds<ORDER> &L, int idx, int k, const 
     }
 }
 
+// Builds the nodes below root r that desc accepts (r included) in LDS with
+// NW waves: listed by depth, built deepest level first, a node per group of
+// ORDER lanes, then written out.  Children that desc does not accept come
+// from global memory.  False (and nothing built) if more than kCut - 1
+// nodes would be listed.
+template <int ORDER, int NW, typename Desc>
+__device__ bool build_sub_lds(int32_t r, Desc desc, SubLds<ORDER> &L, const int32_t *__restrict__ child0,
+                              const int32_t *__restrict__ child1, double *__restrict__ ncx,
+                              double *__restrict__ ncy, double *__restrict__ nr, double4 *__restrict__ nbb,
+                              double2 *__restrict__ cd, float2 *__restrict__ cf, double2 *ws,
+                              const double (*bin)[ORDER]) {
+    constexpr int G = 64 / ORDER;  // nodes per wave
+    const int wave = threadIdx.x >> 6, gi = (threadIdx.x & 63) / ORDER, k = threadIdx.x % ORDER;
+    __shared__ int full;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        L.lst[0] = r;
+        L.lev[0] = 0;
+        L.lev[1] = 1;
+        L.ntot = 1;
+        full = 0;
+    }
+    __syncthreads();
+    int nlev = 1;
+    for (int d = 0; d + 2 < kLevCap; ++d) {
+        const int s0 = L.lev[d], s1 = L.lev[d + 1];
+        for (int t = s0 + threadIdx.x; t < s1; t += NW * 64) {
+            const int32_t u = L.lst[t];
+            const int32_t chq[2] = {child0[u], child1[u]};
+            for (int q = 0; q < 2; ++q) {
+                if (desc(chq[q])) {
+                    const int pos = atomicAdd(&L.ntot, 1);
+                    if (pos < kCut - 1) {
+                        L.lst[pos] = chq[q];
+                        L.ch[t][q] = pos;
+                    } else {
+                        full = 1;
+                    }
+                } else {
+                    L.ch[t][q] = -1 - chq[q];
+                }
+            }
+        }
+        __syncthreads();
+        const int e = L.ntot;
+        if (full || e == s1) break;
+        if (threadIdx.x == 0) L.lev[d + 2] = e;
+        nlev = d + 2;
+        __syncthreads();
+    }
+    if (full) return false;  // the same for every thread
+    for (int d = nlev - 1; d >= 0; --d) {
+        const int s0 = L.lev[d], s1 = L.lev[d + 1];
+        for (int base = s0 + wave * G; base < s1; base += NW * G) {
+            const int idx = base + gi;
+            m2m_lds<ORDER>(L, idx < s1 ? idx : -1, k, ncx, ncy, nr, nbb, cd, ws, bin);
+        }
+        __syncthreads();
+    }
+    const int tot = L.ntot;
+    for (int t = threadIdx.x / ORDER; t < tot; t += NW * 64 / ORDER) {
+        const int32_t u = L.lst[t];
+        const double2 c = L.c[t][k];
+        cd[(int64_t)u * ORDER + k] = c;
+        cf[(int64_t)u * ORDER + k] = make_float2((float)c.x, (float)c.y);
+        if (k == 0) {
+            ncx[u] = L.gx[t];
+            ncy[u] = L.gy[t];
+            nr[u] = L.gr[t];
+            nbb[u] = L.gb[t];
+        }
+    }
+    return true;
+}
+
 // k_upper_local: a work-group per cut node builds its subtree (at most
 // kCut - 1 internal nodes).  At 16 coefficients the subtree stays in LDS
 // until it is complete (SubLds), so a level waits on LDS rather than on
@@ -718,64 +793,10 @@ __global__ __launch_bounds__(256) void k_upper_local(const int32_t *__restrict__
     const int n = *ncut;
     if constexpr (ORDER <= 16) {
         __shared__ SubLds<ORDER> L;
-        constexpr int G = 64 / ORDER;  // nodes per wave
-        const int wave = threadIdx.x >> 6, gi = (threadIdx.x & 63) / ORDER, k = threadIdx.x % ORDER;
         for (int w = blockIdx.x; w < n; w += gridDim.x) {
             const int32_t r = cut[w];
             if (r < nb) continue;  // a bucket: nothing to build (the same for every thread)
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                L.lst[0] = r;
-                L.lev[0] = 0;
-                L.lev[1] = 1;
-                L.ntot = 1;
-            }
-            __syncthreads();
-            int nlev = 1;
-            for (int d = 0; d + 2 < kLevCap; ++d) {
-                const int s0 = L.lev[d], s1 = L.lev[d + 1];
-                for (int t = s0 + threadIdx.x; t < s1; t += 256) {
-                    const int32_t u = L.lst[t];
-                    const int32_t chq[2] = {child0[u], child1[u]};
-                    for (int q = 0; q < 2; ++q) {
-                        if (chq[q] >= nb) {
-                            const int pos = atomicAdd(&L.ntot, 1);
-                            L.lst[pos] = chq[q];
-                            L.ch[t][q] = pos;
-                        } else {
-                            L.ch[t][q] = -1 - chq[q];
-                        }
-                    }
-                }
-                __syncthreads();
-                const int e = L.ntot;
-                if (e == s1) break;
-                if (threadIdx.x == 0) L.lev[d + 2] = e;
-                nlev = d + 2;
-                __syncthreads();
-            }
-            for (int d = nlev - 1; d >= 0; --d) {
-                const int s0 = L.lev[d], s1 = L.lev[d + 1];
-                for (int base = s0 + wave * G; base < s1; base += 4 * G) {
-                    const int idx = base + gi;
-                    m2m_lds<ORDER>(L, idx < s1 ? idx : -1, k, ncx, ncy, nr, nbb, cd, ws, bin);
-                }
-                __syncthreads();
-            }
-            // the subtree to global memory
-            const int tot = L.ntot;
-            for (int t = threadIdx.x / ORDER; t < tot; t += 256 / ORDER) {
-                const int32_t u = L.lst[t];
-                const double2 c = L.c[t][k];
-                cd[(int64_t)u * ORDER + k] = c;
-                cf[(int64_t)u * ORDER + k] = make_float2((float)c.x, (float)c.y);
-                if (k == 0) {
-                    ncx[u] = L.gx[t];
-                    ncy[u] = L.gy[t];
-                    nr[u] = L.gr[t];
-                    nbb[u] = L.gb[t];
-                }
-            }
+            (void)build_sub_lds<ORDER, 4>(r, IsInternal{nb}, L, child0, child1, ncx, ncy, nr, nbb, cd, cf, ws, bin);
         }
     } else {
         __shared__ int32_t lst[kCut];
@@ -817,6 +838,11 @@ __global__ __launch_bounds__(1024) void k_upper_top(int32_t root, int64_t nb, in
     if (!top(root)) return;  // the root is a bucket or a cut node (the same for every thread)
     fill_binomials<ORDER>(bin);
     double2 *ws = ws_all[threadIdx.x >> 6] + ((threadIdx.x & 63) / ORDER) * ORDER;
+    if constexpr (ORDER <= 16) {
+        // in LDS when the nodes above the cut nodes fit (a few hundred at 2^20)
+        __shared__ SubLds<ORDER> L;
+        if (build_sub_lds<ORDER, 16>(root, top, L, child0, child1, ncx, ncy, nr, nbb, cd, cf, ws, bin)) return;
+    }
     build_levels<ORDER, 16>(root, top, lst, lev, &ntot, child0, child1, ncx, ncy, nr, nbb, cd, cf, ws, bin);
 }
 
