@@ -186,7 +186,8 @@ def main():
     # (collapse); three untimed steps pass that transient
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--f64-steps", type=int, default=2)
+    # default: as many as the mixed path, so both time the same steps
+    ap.add_argument("--f64-steps", type=int, default=None)
     ap.add_argument("--cpu-targets", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
@@ -195,9 +196,11 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="run one kernel of known HBM traffic before the steps (for counter runs)")
     args = ap.parse_args()
+    if args.f64_steps is None:
+        args.f64_steps = args.steps
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by targets, see DESIGN.md)", file=sys.stderr)
+              "(the step would shard by targets, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from vortex_amd.solver import VX_PREC_F64, VX_PREC_MIXED

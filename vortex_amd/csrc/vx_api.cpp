@@ -174,7 +174,7 @@ int vx_case_destroy(vx_case *c) {
     c->xf.release(); c->yf.release(); c->gf.release();
     c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();
     c->keys.release(); c->keys_alt.release(); c->vals.release(); c->vals_alt.release();
-    c->cub_tmp.release(); c->gridp.release(); c->dcount.release();
+    c->cub_tmp.release(); c->gridp.release(); c->dcount.release(); c->colseg.release();
     if (c->side) (void)hipStreamSynchronize(c->side);
     c->wlist.release(); c->wtmp.release(); c->wstage.release();
     tree_release(c);

@@ -139,6 +139,7 @@ struct vx_case {
     vx::DBuf<unsigned char> cub_tmp;
     vx::DBuf<unsigned char> gridp;                     // collapse grid parameters (device)
     vx::DBuf<int64_t> dcount;                         // device counters
+    vx::DBuf<int32_t> colseg;                         // collapse candidate counts per list segment
     vx::DBuf<int32_t> wlist;                          // wall targets (side stream)
     vx::DBuf<unsigned char> wtmp;                     // their selection's scratch
     vx::DBuf<double> wstage;                          // their velocity terms, per list slot

@@ -430,42 +430,63 @@ int launch_compact(vx_case *c, int64_t n_ub) {
 // Collapse work arrays (per vortex slot unless noted): pc counts the
 // candidates pointing at a vortex, pcd the ones not yet published, state
 // holds the decision bits, first the smallest active candidate merging with
-// it; cand lists the candidates (count *ncand), rest the ones left to
-// resolve (count *nrest).
+// it; cand lists the candidates in nseg segments of cap slots (segment s
+// holds ncs[s] of them), rest the ones left to resolve (count *nrest).  The
+// search fills kColSeg segments, one per residue of the work-group index
+// mod kColSeg, so that no single counter takes every work-group's atomic;
+// a candidate list from a device select is one segment.
+constexpr int kColSeg = 32;
+
 struct ColBufs {
-    int32_t *pc, *pcd, *state, *first, *cand, *rest, *nrest;
-    unsigned long long *ncand;
+    int32_t *pc, *pcd, *state, *first, *cand, *rest, *nrest, *ncs;
+    int64_t cap;
+    int nseg;
 };
 
-static int col_bufs(vx_case *c, int64_t n_ub, ColBufs &b) {
+// Slots of the kColSeg segments for a search over n_ub vortices in 256-thread
+// work-groups.
+static int64_t col_seg_cap(int64_t n_ub) { return ((n_ub + 255) / 256 + kColSeg - 1) / kColSeg * 256; }
+
+static int col_bufs(vx_case *c, int64_t n_ub, bool segmented, ColBufs &b) {
     hipStream_t st = c->stream;
+    const int64_t cap = col_seg_cap(n_ub), ncap = kColSeg * cap;
     if (c->flags.ensure((size_t)n_ub, st) || c->scan.ensure((size_t)n_ub, st) ||
-        c->lists.ensure((size_t)(3 * n_ub), st) || c->misc.ensure((size_t)n_ub, st) || c->dcount.ensure(16, st))
+        c->lists.ensure((size_t)(ncap + 2 * n_ub), st) || c->misc.ensure((size_t)n_ub, st) ||
+        c->dcount.ensure(16, st) || c->colseg.ensure(kColSeg, st))
         return 1;
     b.pc = c->flags.p;
     b.state = c->scan.p;
     b.cand = c->lists.p;
-    b.rest = c->lists.p + n_ub;
-    b.pcd = c->lists.p + 2 * n_ub;
+    b.rest = c->lists.p + ncap;
+    b.pcd = c->lists.p + ncap + n_ub;
     b.first = c->misc.p;
-    b.ncand = (unsigned long long *)(c->dcount.p + 2);
+    b.ncs = c->colseg.p;
     b.nrest = (int32_t *)(c->dcount.p + 4);
+    b.cap = segmented ? cap : n_ub;
+    b.nseg = segmented ? kColSeg : 1;
     return 0;
+}
+
+// The candidate at slot t of the segmented list, or -1.
+__device__ __forceinline__ int32_t col_cand(const ColBufs &cb, int64_t t) {
+    if (t >= (int64_t)cb.nseg * cb.cap) return -1;
+    const int64_t sg = t / cb.cap, j = t - sg * cb.cap;
+    return j < cb.ncs[sg] ? cb.cand[t] : -1;
 }
 
 // Appends val of every flagged thread to out[*cnt, ...) with one atomic per
 // 256-thread work-group; every thread of the work-group calls it.  The order
 // within the list is not defined.
-__device__ void block_append(bool flag, int32_t val, int32_t *__restrict__ out, unsigned long long *__restrict__ cnt) {
+__device__ void block_append(bool flag, int32_t val, int32_t *__restrict__ out, int32_t *__restrict__ cnt) {
     __shared__ int wcnt[4];
-    __shared__ unsigned long long base;
+    __shared__ int base;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t m = __ballot(flag);
     if (lane == 0) wcnt[w] = __popcll(m);
     __syncthreads();
     if (threadIdx.x == 0) {
         const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-        base = tot ? atomicAdd(cnt, (unsigned long long)tot) : 0ull;
+        base = tot ? atomicAdd(cnt, tot) : 0;
     }
     __syncthreads();
     int off = 0;
@@ -533,10 +554,8 @@ __global__ void k_grid_clear(int32_t *__restrict__ head, const double *__restric
         cb.state[t] = 0;
         cb.first[t] = INT32_MAX;
     }
-    if (t0 == 0) {
-        *cb.ncand = 0;
-        *cb.nrest = 0;
-    }
+    if (t0 < kColSeg) cb.ncs[t0] = 0;
+    if (t0 == 0) *cb.nrest = 0;
 }
 
 __global__ void k_grid_insert(const double *__restrict__ x, const double *__restrict__ y,
@@ -599,7 +618,8 @@ __global__ __launch_bounds__(256) void k_setka(const double *__restrict__ x, con
         atomicAdd(&cb.pc[best], 1);
         atomicAdd(&cb.pcd[best], 1);
     }
-    block_append(hit, (int32_t)i, cb.cand, cb.ncand);
+    const int sg = (int)(blockIdx.x % kColSeg);
+    block_append(hit, (int32_t)i, cb.cand + sg * cb.cap, cb.ncs + sg);
 }
 
 // Box of the vortices with g != 0 in [0, *dn) (min x, -max x, min y, -max y
@@ -663,7 +683,7 @@ int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare) {
     if (c->col.ensure((size_t)(n_ub > 0 ? n_ub : 1), st)) return 1;
     if (n_ub <= 0) return 0;
     ColBufs cb{};
-    if (prepare && col_bufs(c, n_ub, cb)) return 1;
+    if (prepare && col_bufs(c, n_ub, true, cb)) return 1;
     const unsigned nb = (unsigned)((n_ub + 255) / 256);
     const double ve2 = c->prm.ve_size * c->prm.ve_size;
     const double rr = kind == 1 ? c->prm.r_col_diff_sign : c->prm.r_col_same_sign;
@@ -700,6 +720,7 @@ int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare) {
 __global__ void k_col_prepare(int64_t n_ub, ColBufs cb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *cb.nrest = 0;
+    if (i < kColSeg) cb.ncs[i] = 0;
     if (i >= n_ub) return;
     cb.pc[i] = 0;
     cb.pcd[i] = 0;
@@ -732,8 +753,7 @@ struct ColCandidate {
 // by this thread only.  The other candidates are listed in rest.
 __global__ __launch_bounds__(256) void k_col_leaves(const int32_t *__restrict__ col, ColBufs cb) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t ncand = (int64_t)*cb.ncand;
-    const int32_t i = t < ncand ? cb.cand[t] : -1;
+    const int32_t i = col_cand(cb, t);
     const bool leaf = i >= 0 && cb.pc[i] == 0;
     if (leaf) {
         const int32_t j = col[i];
@@ -812,12 +832,11 @@ __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict_
 // j's first merger alone.
 template <int KIND>
 __global__ void k_col_merge(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
-                            const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
-                            const unsigned long long *__restrict__ ncand_p, const int32_t *__restrict__ state,
-                            const int32_t *__restrict__ first) {
+                            const int32_t *__restrict__ col, ColBufs cb) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)*ncand_p) return;
-    const int32_t i = cand[t];
+    const int32_t i = col_cand(cb, t);
+    if (i < 0) return;
+    const int32_t *__restrict__ state = cb.state, *__restrict__ first = cb.first;
     if (!(state[i] & 1)) return;
     const int32_t j = col[i];
     const bool fst = first[j] == i;
@@ -853,27 +872,26 @@ int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared) {
     if (n_ub <= 0) return 0;
     hipStream_t st = c->stream;
     ColBufs cb{};
-    if (col_bufs(c, n_ub, cb)) return 1;
+    if (col_bufs(c, n_ub, prepared, cb)) return 1;
     const unsigned nb = (unsigned)((n_ub + 255) / 256);
+    const unsigned nbc = (unsigned)(((int64_t)cb.nseg * cb.cap + 255) / 256);  // over the candidate slots
     if (!prepared) {
         hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, cb);
         hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, step_dn(c), cb);
         const ColCandidate pred{c->col.p, step_dn(c)};
         hipcub::CountingInputIterator<int32_t> it(0);
         size_t bytes = 0;
-        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cb.cand, cb.ncand, (int)n_ub, pred, st));
+        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cb.cand, cb.ncs, (int)n_ub, pred, st));
         if (c->cub_tmp.ensure(bytes, st)) return 1;
-        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cb.cand, cb.ncand, (int)n_ub, pred, st));
+        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cb.cand, cb.ncs, (int)n_ub, pred, st));
     }
-    hipLaunchKernelGGL(k_col_leaves, dim3(nb), dim3(256), 0, st, c->col.p, cb);
+    hipLaunchKernelGGL(k_col_leaves, dim3(nbc), dim3(256), 0, st, c->col.p, cb);
     hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, cb.rest, cb.nrest, cb.pcd, cb.state,
                        cb.first, (int32_t *)(c->dcount.p + 3));
     if (kind == 1)
-        hipLaunchKernelGGL(k_col_merge<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb.cand,
-                           cb.ncand, cb.state, cb.first);
+        hipLaunchKernelGGL(k_col_merge<1>, dim3(nbc), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb);
     else
-        hipLaunchKernelGGL(k_col_merge<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb.cand,
-                           cb.ncand, cb.state, cb.first);
+        hipLaunchKernelGGL(k_col_merge<2>, dim3(nbc), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb);
     VX_CHECK(hipGetLastError());
     return 0;
 }

@@ -1094,7 +1094,7 @@ __device__ __forceinline__ float readlane_f(float a, int l) {
 // terms of lanes 4e .. 4e + 3 for coefficient 4 j + c into acc[j]; the
 // partial sums over e are combined once, by l2p_coeffs().
 template <int ORDER>
-__device__ __forceinline__ void m2l_batch(const TreeDev &t, const int32_t *fq, int count, int lane, double cTx,
+__device__ __noinline__ void m2l_batch(const TreeDev &t, const int32_t *fq, int count, int lane, double cTx,
                                           double cTy, float rT, float2 (*tr)[64], float2 (&acc)[ORDER / 4],
                                           float &gap) {
     const int node = lane < count ? fq[lane] : -1;
