@@ -2063,11 +2063,15 @@ void tree_release(vx_case *c) {
     c->tree = nullptr;
 }
 
-// Key sort: rocPRIM's onesweep radix sort (four 8-bit passes) at every size;
-// its default switches to a merge sort up to 2^20 items, several times slower
-// here.
-using KeySortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                                 rocprim::default_config, 0>;
+// Key sort: rocPRIM's onesweep radix sort, forced at every size (its default
+// switches to a merge sort up to 2^20 items, several times slower here), with
+// 11-bit digits: three passes over the 32-bit keys instead of four (about
+// 20 us less at 2^20).
+using KeySortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 11,
+                                        rocprim::block_radix_rank_algorithm::match>,
+    0>;
 
 static int tree_order(const vx_case *c) { return c->prm.order >= 32 ? 32 : 16; }
 
