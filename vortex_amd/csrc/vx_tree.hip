@@ -1181,7 +1181,8 @@ __device__ __forceinline__ int64_t xcd_block() {
     return (int64_t)(blockIdx.x & 7u) * q + (int64_t)(blockIdx.x >> 3);
 }
 
-constexpr int kBsStack = 512;  // traversal stack entries per wave
+constexpr int kBsStack = 256;  // traversal stack entries per wave
+constexpr int kFarQ = 512;     // FP32: far nodes queued per wave before any M2L
 
 template <typename T, int ORDER>
 __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double eps, double vinf_x, double vinf_y,
@@ -1193,7 +1194,7 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     using PM = PairMath<T>;
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
-    __shared__ int32_t fq_all[4][128];    // FP32: far nodes waiting for M2L
+    __shared__ int32_t fq_all[4][kFarQ];  // FP32: far nodes waiting for M2L
     __shared__ int32_t stk_all[4][kBsStack];
     __shared__ int32_t lq_all[4][64];     // near leaves of the popped nodes
     __shared__ float2 tr_all[4][4][64];   // FP32: M2L sums over the wave
@@ -1289,19 +1290,18 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
         const uint64_t fm = __ballot(isfar);
         nfar += __popcll(fm);
         if constexpr (F32) {
-            // queued for M2L into the bucket's local expansion, 64 at a time
+            // queued for M2L into the bucket's local expansion after the
+            // traversal, 64 at a time; only a nearly full queue (rare: a
+            // bucket meets about 140 far nodes at 2^20) is translated here,
+            // so the traversal keeps its registers
             if (isfar) fq[nq + __popcll(fm & ((1ull << lane) - 1))] = node;
             nq += __popcll(fm);
-            if (nq >= 64) {
+            if (nq > kFarQ - 64) {
                 __builtin_amdgcn_wave_barrier();
-                m2l_batch<ORDER>(t, fq, 64, lane, cAx, cAy, rT, tr, lacc, gap);
+                m2l_batch<ORDER>(t, fq + nq - 64, 64, lane, cAx, cAy, rT, tr, lacc, gap);
                 __builtin_amdgcn_wave_barrier();
                 cm2l += 64ull;
                 nq -= 64;
-                const int moved = lane < nq ? fq[64 + lane] : 0;
-                __builtin_amdgcn_wave_barrier();
-                if (lane < nq) fq[lane] = moved;
-                __builtin_amdgcn_wave_barrier();
             }
         } else {
             // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
@@ -1370,10 +1370,13 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     su += (double)fu;
     sv += (double)fv;
     if constexpr (F32) {
-        if (nq > 0) {
+        while (nq > 0) {
+            const int nb_ = nq < 64 ? nq : 64;
             __builtin_amdgcn_wave_barrier();
-            m2l_batch<ORDER>(t, fq, nq, lane, cAx, cAy, rT, tr, lacc, gap);
-            cm2l += (unsigned long long)nq;
+            m2l_batch<ORDER>(t, fq + nq - nb_, nb_, lane, cAx, cAy, rT, tr, lacc, gap);
+            __builtin_amdgcn_wave_barrier();
+            cm2l += (unsigned long long)nb_;
+            nq -= nb_;
         }
         for (int o = 32; o > 0; o >>= 1) gap = fminf(gap, __shfl_xor(gap, o, 64));
         farmin_w = (double)gap;
