@@ -873,9 +873,10 @@ __global__ void k_pack_nodes(int64_t nn, const double *__restrict__ ncx, const d
 }
 
 // ---- traversal helpers ------------------------------------------------------
-// The tree above the buckets is at most 53 levels deep (32 key bits plus
-// the index bits that split equal keys), so a depth-first walk holds at most
-// 54 entries; a push beyond 64 is dropped and counted as an error.
+// The tree above the buckets is at most 63 levels deep (each level lengthens
+// the common prefix of the 64-bit keys extended by the index, and the index
+// fits 31 bits), so a depth-first walk holds at most 64 entries; a push
+// beyond 64 is dropped and counted as an error.
 struct LaneStack {
     int slot = 0;  // this lane's stack entry
     int sp = 0;    // wave-uniform depth
@@ -1248,15 +1249,15 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     // time, one per lane, which tests its node's MAC; far nodes are queued
     // (FP32: for M2L; FP64: evaluated one at a time), near leaves summed one
     // at a time, opened nodes' children pushed.  While the stack is within
-    // 64 + 54 entries of its capacity, nodes are popped one at a time: a
+    // 64 + 64 entries of its capacity, nodes are popped one at a time: a
     // depth-first descent adds at most one entry per level (the tree is at
-    // most 54 levels deep), so the stack never outgrows kBsStack.
+    // most 63 levels deep), so the stack never outgrows kBsStack.
     int32_t *stk = stk_all[threadIdx.x >> 6];
     int32_t *lq = lq_all[threadIdx.x >> 6];
     int sp = 1;
     if (lane == 0) stk[0] = t.root;
     while (sp > 0) {
-        const int k = sp + 64 + 54 <= kBsStack ? (sp < 64 ? sp : 64) : 1;
+        const int k = sp + 64 + 64 <= kBsStack ? (sp < 64 ? sp : 64) : 1;
         __builtin_amdgcn_wave_barrier();
         const int node = lane < k ? stk[sp - 1 - lane] : -1;
         __builtin_amdgcn_wave_barrier();
