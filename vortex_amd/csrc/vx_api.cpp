@@ -438,10 +438,14 @@ int vx_vort_creation(vx_case *c, double vinf_x, double vinf_y) {
     if (ov) {
         if (side_stream(c)) return 1;
         VX_CHECK(hipEventRecord(c->pre_ev, c->stream));
-        if (tree_build_early(c, s_next, c->n)) return 1;
     }
     if (launch_right_part(c, ceil32(c->n), vinf_x, vinf_y) || launch_birth(c, c->d_g)) return 1;
-    if (ov) VX_CHECK(hipEventRecord(c->born_ev, c->stream));
+    if (ov) {
+        VX_CHECK(hipEventRecord(c->born_ev, c->stream));
+        // queued after the birth's kernels (the host issues them first, so
+        // the main stream starts at once) but ordered only after pre_ev
+        if (tree_build_early(c, s_next, c->n)) return 1;
+    }
     c->n += c->P;
     return 0;
 }
