@@ -470,11 +470,13 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
     {
         Timer td(c, &c->tm.diff_ms, 3);
         if ((tree && tree_diffusion_pre(c)) || side_fork(c)) return 1;
-        if (c->timing) (void)hipEventRecord(c->ev[14], c->side);
-        rc = launch_wall_side(c, s, c->side);
-        if (c->timing) (void)hipEventRecord(c->ev[15], c->side);
+        // the long main-stream kernel is issued first; the side stream's
+        // work is ordered after the fork point only
+        rc = tree ? tree_diffusion_main(c) : launch_diffusion(c, s, s);
+        if (!rc && c->timing) (void)hipEventRecord(c->ev[14], c->side);
+        if (!rc) rc = launch_wall_side(c, s, c->side);
+        if (!rc && c->timing) (void)hipEventRecord(c->ev[15], c->side);
         if (!rc && tree) rc = tree_diffusion_far(c, c->side);
-        if (!rc) rc = tree ? tree_diffusion_main(c) : launch_diffusion(c, s, s);
         const int rj = side_join(c);
         if (!rc) rc = rj ? rj : launch_wall_add(c);
     }
