@@ -77,6 +77,7 @@ struct TreeBufs {
     int64_t ngroup = 0;                    // target groups (gfirst)
     int32_t root = 0;
     int order = 16;
+    int64_t fcap = 4096;  // domain-scale diffusion targets listed at most (set per diffusion)
     hipEvent_t ev = nullptr;                       // read-back completion
 };
 
@@ -95,12 +96,14 @@ struct TreeDev {
     const int32_t *gfirst;
     int64_t ngroup;
     const int32_t *parent, *bscan, *bstart;
+    int64_t fcap;  // capacity of the domain-scale diffusion target list
 };
 
 static TreeDev view(const TreeBufs &t) {
     return TreeDev{t.m,     t.nleaf, t.root,     t.perm.p,   t.sx.p,    t.sy.p,    t.sg.p, t.fx.p, t.fy.p, t.fg.p,
                    t.ncx.p, t.ncy.p, t.nr.p,     t.nbb.p,    t.child0.p, t.child1.p, t.first.p, t.count.p,
-                   t.cd.p,  t.cf.p,  t.rec.p, t.gfirst.p, t.ngroup, t.parent.p, t.bscan.p, t.bstart.p};
+                   t.cd.p,  t.cf.p,  t.rec.p, t.gfirst.p, t.ngroup, t.parent.p, t.bscan.p, t.bstart.p,
+                   t.fcap};
 }
 
 // ---- wave helpers ----------------------------------------------------------
@@ -1650,8 +1653,9 @@ struct DiffPair<double> {
 };
 
 // Radius beyond which a diffusion target is summed by k_diff_far: 5 % of
-// the root's radius.  At most kFarCap targets are listed; further ones stay
-// in their group's wave.
+// the root's radius.  At most t.fcap targets are listed (kFarCap or one in
+// 16 included vortices, whichever is more: sparse wakes have many); further
+// ones stay in their group's wave.
 constexpr int64_t kFarCap = 4096;
 __device__ __forceinline__ double far_radius(const TreeDev &t) {
     const double4 b = t.nbb[t.root];
@@ -1659,9 +1663,9 @@ __device__ __forceinline__ double far_radius(const TreeDev &t) {
     return fmax(0.05 * rr, 1e-300);
 }
 
-__device__ __forceinline__ int64_t far_count(const unsigned long long *cnt) {
+__device__ __forceinline__ int64_t far_count(const unsigned long long *cnt, int64_t cap) {
     const unsigned long long h = cnt[2];
-    return (int64_t)(h < (unsigned long long)kFarCap ? h : (unsigned long long)kFarCap);
+    return (int64_t)(h < (unsigned long long)cap ? h : (unsigned long long)cap);
 }
 
 // Targets are summed over the sources within cut * d_i of them (the omitted
@@ -1787,7 +1791,7 @@ void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
 }
 
 // The diffusion's d in sorted order, negated for the targets listed for
-// k_diff_far (radius cut * d beyond far_radius(), at most kFarCap of them,
+// k_diff_far (radius cut * d beyond far_radius(), at most t.fcap of them,
 // appended with one atomic per work-group; cnt[2] counts every such target,
 // so targets past the capacity stay with their group's wave).
 __global__ __launch_bounds__(256) void k_far_list(TreeDev t, double cut, const double *__restrict__ dd,
@@ -1814,7 +1818,7 @@ __global__ __launch_bounds__(256) void k_far_list(TreeDev t, double cut, const d
     if (far) {
         unsigned long long k = base + __popcll(m & ((1ull << lane) - 1));
         for (int q = 0; q < w; ++q) k += wcnt[q];
-        if (k < (unsigned long long)kFarCap)
+        if (k < (unsigned long long)t.fcap)
             lists[k] = (int32_t)ti;
         else
             far = false;
@@ -1899,7 +1903,7 @@ __global__ __launch_bounds__(256) void k_far_pairs(TreeDev t, const int32_t *__r
                                                    unsigned long long *__restrict__ npairs,
                                                    double4 *__restrict__ ftgt) {
     const int lane = threadIdx.x & 63;
-    const int64_t H = far_count(cnt);
+    const int64_t H = far_count(cnt, t.fcap);
     for (int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); k < H; k += (int64_t)gridDim.x * 4)
         far_pairs_of(t, list, k, nchunk, cbox, cut, dd, pairs, npairs, ftgt, lane);
 }
@@ -1910,8 +1914,9 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ a, unsigned mask) {
 }
 
 // Zero the partials of the listed targets (grid-stride; H on the device).
-__global__ void k_far_zero(double *__restrict__ a, int64_t nchunk, const unsigned long long *__restrict__ cnt) {
-    const int64_t count = 3 * nchunk * far_count(cnt);
+__global__ void k_far_zero(double *__restrict__ a, int64_t nchunk, const unsigned long long *__restrict__ cnt,
+                           int64_t cap) {
+    const int64_t count = 3 * nchunk * far_count(cnt, cap);
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
         a[t] = 0.0;
 }
@@ -1983,7 +1988,7 @@ __global__ __launch_bounds__(256) void k_diff_far(TreeDev t, const double4 *__re
                                                   const unsigned long long *__restrict__ npair_list, double cut,
                                                   double thr, double *__restrict__ part,
                                                   unsigned long long *__restrict__ cnt) {
-    const int64_t H = far_count(cnt);
+    const int64_t H = far_count(cnt, t.fcap);
     const int64_t total = (int64_t)*npair_list;
     const int64_t nw = (int64_t)gridDim.x * 4;
     unsigned long long cp = 0;
@@ -2027,7 +2032,7 @@ __global__ __launch_bounds__(256) void k_diff_far_final(TreeDev t, const int32_t
                                                         int64_t nchunk, const double *__restrict__ part,
                                                         const double *__restrict__ dd, double nu,
                                                         double *__restrict__ u, double *__restrict__ v) {
-    const int64_t H = far_count(cnt);
+    const int64_t H = far_count(cnt, t.fcap);
     for (int64_t k = blockIdx.x; k < H; k += gridDim.x) far_final_of(t, list, H, k, nchunk, part, dd, nu, u, v);
 }
 
@@ -2323,8 +2328,9 @@ int tree_diffusion_pre(vx_case *c) {
     TreeBufs &t = *c->tree;
     hipStream_t st = c->stream;
     const int64_t nchunk = (t.nleaf + 63) / 64;
-    if (t.hpart.ensure((size_t)(3 * nchunk * kFarCap), st) || t.cbox.ensure((size_t)nchunk, st) ||
-        t.fpairs.ensure((size_t)(nchunk * kFarCap), st) || t.ftgt.ensure((size_t)kFarCap, st) ||
+    t.fcap = std::max<int64_t>(kFarCap, t.m / 16);
+    if (t.hpart.ensure((size_t)(3 * nchunk * t.fcap), st) || t.cbox.ensure((size_t)nchunk, st) ||
+        t.fpairs.ensure((size_t)(nchunk * t.fcap), st) || t.ftgt.ensure((size_t)t.fcap, st) ||
         t.sd.ensure((size_t)t.m, st))
         return 1;
     unsigned long long *cnt = t.cnt.p + 4;  // pairs, overflow, far targets, far (target, chunk) pairs
@@ -2344,7 +2350,7 @@ int tree_diffusion_far(vx_case *c, hipStream_t st) {
     unsigned long long *cnt = t.cnt.p + 4;
     // all counts stay on the device (grid-stride kernels sized for the list's capacity)
     const int64_t nchunk = (t.nleaf + 63) / 64;
-    hipLaunchKernelGGL(k_far_zero, dim3(512), dim3(256), 0, st, t.hpart.p, nchunk, cnt);
+    hipLaunchKernelGGL(k_far_zero, dim3(512), dim3(256), 0, st, t.hpart.p, nchunk, cnt, t.fcap);
     hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)((nchunk + 3) / 4)), dim3(256), 0, st, tv, nchunk, t.cbox.p);
     hipLaunchKernelGGL(k_far_pairs, dim3(256), dim3(256), 0, st, tv, t.fb.p, cnt, nchunk, t.cbox.p, cut, c->d.p,
                        t.fpairs.p, cnt + 3, t.ftgt.p);
@@ -2401,7 +2407,8 @@ int tree_stats(vx_case *c, vx_tree_info *info) {
         return 1;
     }
     info->order = c->tree->order;
-    info->heavy = (int32_t)(h[6] < (unsigned long long)kFarCap ? h[6] : (unsigned long long)kFarCap);
+    const unsigned long long cap = (unsigned long long)c->tree->fcap;
+    info->heavy = (int32_t)(h[6] < cap ? h[6] : cap);
     info->m2l = (int64_t)h[8];
     return 0;
 }
