@@ -62,7 +62,7 @@ struct TreeBufs {
     DBuf<int32_t> rlo, rhi, rleft, rright, bstart, bscan, upper, uscan, toplist;
     DBuf<int32_t> gcs, gcs2, gflag, gscan, gfirst;  // target groups
     DBuf<int32_t> pcount;                          // included particles per block
-    DBuf<int32_t> arr;                             // cut nodes, count at arr[0]
+    DBuf<int32_t> arr;                             // cut nodes (count in cnt[10])
     DBuf<int64_t> sizes;                           // group, bucket and upper-node counts
     DBuf<double> sx, sy, sg, ncx, ncy, nr, fbr2, hpart, sd;
     DBuf<float> fx, fy, fg;
@@ -458,6 +458,31 @@ struct CutNode {
     }
 };
 
+// The cut nodes (CutNode) among nodes [0, nn), appended to cut with one
+// atomic per work-group on *ncut (zero on entry); their order is arbitrary,
+// each one is processed on its own.
+__global__ __launch_bounds__(256) void k_cut_list(CutNode pred, int64_t nn, int32_t *__restrict__ cut,
+                                                  unsigned long long *__restrict__ ncut) {
+    __shared__ int wcnt[4];
+    __shared__ unsigned long long base;
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool hit = u < nn && pred((int32_t)u);
+    const uint64_t mk = __ballot(hit);
+    if (lane == 0) wcnt[w] = __popcll(mk);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        base = tot ? atomicAdd(ncut, (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    if (hit) {
+        unsigned long long k = base + __popcll(mk & ((1ull << lane) - 1));
+        for (int q = 0; q < w; ++q) k += wcnt[q];
+        cut[k] = (int32_t)u;
+    }
+}
+
 // Box union and M2M of node p from its two children (both complete), by a
 // group of ORDER lanes (lane k writes coefficient k).  Radius: the smaller of
 // two bounds on the particles' distance from the centre, the half diagonal
@@ -783,7 +808,8 @@ __device__ bool build_sub_lds(int32_t r, Desc desc, SubLds<ORDER> &L, const int3
 // until it is complete (SubLds), so a level waits on LDS rather than on
 // global memory; at 32 it is built through global memory (build_levels).
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_upper_local(const int32_t *__restrict__ cut, const int32_t *__restrict__ ncut,
+__global__ __launch_bounds__(256) void k_upper_local(const int32_t *__restrict__ cut,
+                                                     const unsigned long long *__restrict__ ncut,
                                                      int64_t nb, const int32_t *__restrict__ child0,
                                                      const int32_t *__restrict__ child1, double *__restrict__ ncx,
                                                      double *__restrict__ ncy, double *__restrict__ nr,
@@ -793,7 +819,7 @@ __global__ __launch_bounds__(256) void k_upper_local(const int32_t *__restrict__
     __shared__ double bin[ORDER][ORDER];
     fill_binomials<ORDER>(bin);
     double2 *ws = ws_all[threadIdx.x >> 6] + ((threadIdx.x & 63) / ORDER) * ORDER;
-    const int n = *ncut;
+    const int n = (int)*ncut;
     if constexpr (ORDER <= 16) {
         __shared__ SubLds<ORDER> L;
         for (int w = blockIdx.x; w < n; w += gridDim.x) {
@@ -2201,17 +2227,12 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
         VX_CHECK(hipMemcpyAsync(t.count.p, &one[1], sizeof(int32_t), hipMemcpyHostToDevice, st));
     }
     const unsigned lb = (unsigned)((nb + 3) / 4);
-    // cut nodes, in node order
-    if (t.arr.ensure((size_t)nn + 1, st)) return 1;
-    int32_t *ncut = t.arr.p, *cut = t.arr.p + 1;
-    {
-        const CutNode pred{t.first.p, t.count.p, t.bscan.p, t.parent.p, nb, m};
-        hipcub::CountingInputIterator<int32_t> it(0);
-        size_t bytes = 0;
-        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cut, ncut, (int)nn, pred, st));
-        if (t.tmp.ensure(bytes, st)) return 1;
-        VX_CHECK(hipcub::DeviceSelect::If(t.tmp.p, bytes, it, cut, ncut, (int)nn, pred, st));
-    }
+    // cut nodes, in any order (count in cnt[10], cleared with the counters)
+    if (t.arr.ensure((size_t)nn, st)) return 1;
+    int32_t *cut = t.arr.p;
+    unsigned long long *ncut = t.cnt.p + 10;
+    hipLaunchKernelGGL(k_cut_list, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st,
+                       CutNode{t.first.p, t.count.p, t.bscan.p, t.parent.p, nb, m}, nn, cut, ncut);
     const unsigned gc = (unsigned)std::min<int64_t>(nb, 2048);
     if (order == 32) {
         hipLaunchKernelGGL(k_bucket_build<32>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, nb, t.first.p,
