@@ -1456,36 +1456,21 @@ void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y
     block_count_add(cl, 0ull, nfb + 7, nullptr);
 }
 
-// Exact nearest-3 for the flagged targets, FP64, one wave per target: the
-// four smallest squared distances (the target itself included), searched
-// from the target's own bucket upward.  The bucket is scanned first; then,
-// for each ancestor, the subtree on the other side is searched below the
-// bound and the bound tightened to the exact fourth smallest found so far.
-// The bound starts at the fast pass's third distance (its three candidates
-// are real particles).  A subtree is walked with a stack in LDS, up to 64
-// nodes popped at a time; the popped buckets are scanned by the wave, one
-// particle per lane, into each lane's private four smallest, merged over the
-// wave for each bound and at the end.  Should the stack fill, the target
-// restarts as a scan of every bucket below the bound, which needs no stack.
+// Exact nearest-3 for the flagged targets, FP64: the four smallest squared
+// distances over all included particles (the target itself included), one
+// 1024-thread work-group per target.  The bound starts at the fast pass's
+// third distance (its three candidates are real particles), widened by 1e-3.
+// The cut nodes partition the buckets: those whose box lies within the bound
+// are listed in LDS.  The nearest one's buckets (a contiguous range) are
+// scanned first, and the bound tightened to the fourth smallest found; then
+// each wave takes listed cut nodes within the new bound.  A bucket within the
+// bound is read one particle per lane into the lane's private four smallest;
+// the lanes' lists are merged at the end.  Should the list overflow (or the
+// tree be a single bucket), every bucket is tested against the first bound.
 __device__ __forceinline__ double box_d2(const double4 &b, double ax, double ay) {
     const double ex = fmax(0.0, fmax(b.x - ax, ax - b.y));
     const double ey = fmax(0.0, fmax(b.z - ay, ay - b.w));
     return ex * ex + ey * ey;
-}
-
-constexpr int kFbStack = 4096;
-
-__device__ __forceinline__ void near4_leaf(const TreeDev &t, int node, double ax, double ay, double &m0, double &m1,
-                                           double &m2, double &m3) {
-    const int64_t j0 = t.first[node], j1 = j0 + t.count[node];
-    for (int64_t j = j0; j < j1; ++j) {
-        const double dx = ax - t.sx[j], dy = ay - t.sy[j];
-        const double r2 = dx * dx + dy * dy;
-        m3 = PairMath<double>::med3(m2, m3, r2);
-        m2 = PairMath<double>::med3(m1, m2, r2);
-        m1 = PairMath<double>::med3(m0, m1, r2);
-        m0 = fmin(m0, r2);
-    }
 }
 
 __device__ __forceinline__ void near4_insert(double r2, double &m0, double &m1, double &m2, double &m3) {
@@ -1545,97 +1530,96 @@ __device__ __forceinline__ void near4_buckets(const TreeDev &t, uint64_t lm, int
     }
 }
 
-// Walks the subtree at `top` below the bound lim; false if the stack filled.
-__device__ bool near4_subtree(const TreeDev &t, int32_t *stk, int top, int lane, double ax, double ay, double lim,
-                              double &m0, double &m1, double &m2, double &m3) {
+constexpr int kFbList = 1024;
+
+// The four smallest of the work-group's lanes (all 16 waves), in every
+// thread's r; `wr` is LDS scratch, and a barrier ends the call.
+__device__ __forceinline__ void near4_block(int lane, int w, double m0, double m1, double m2, double m3,
+                                            double (*wr)[4], double r[4]) {
+    near4_merge(lane, m0, m1, m2, m3, r);
+    if (lane < 4) wr[w][lane] = lane == 0 ? r[0] : lane == 1 ? r[1] : lane == 2 ? r[2] : r[3];
     __syncthreads();
-    if (lane == 0) stk[0] = top;
+    near4_merge(lane, wr[lane >> 2][lane & 3], 1e5, 1e5, 1e5, r);
     __syncthreads();
-    int sp = 1;
-    while (sp > 0) {
-        const int k = sp < 64 ? sp : 64;
-        const int node = lane < k ? stk[sp - 1 - lane] : -1;
-        sp -= k;
-        bool leaf = false, expand = false;
-        if (node >= 0 && box_d2(t.nbb[node], ax, ay) <= lim) {
-            leaf = t.child0[node] < 0;
-            expand = !leaf;
-        }
-        const uint64_t em = __ballot(expand);
-        const int ne = __popcll(em);
-        if (sp + 2 * ne > kFbStack) return false;
-        __syncthreads();  // every pop is read before the pushes reuse the slots
-        if (expand) {
-            const int pos = sp + 2 * __popcll(em & ((1ull << lane) - 1));
-            stk[pos] = t.child1[node];
-            stk[pos + 1] = t.child0[node];
-        }
-        sp += 2 * ne;
-        near4_buckets(t, __ballot(leaf), node, lane, ax, ay, m0, m1, m2, m3);
-        __syncthreads();
-    }
-    return true;
 }
 
-__global__ __launch_bounds__(64) void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ fb,
-                                                              const double *__restrict__ fbr2,
-                                                              const unsigned long long *__restrict__ nfb,
-                                                              double *__restrict__ d) {
-    __shared__ int32_t stk[kFbStack];
-    __shared__ int32_t path[kLevCap], sibs[kLevCap];
-    __shared__ double sdist[kLevCap];
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(1024) void k_tree_nearest_fallback(TreeDev t, const int32_t *__restrict__ cut,
+                                                                const unsigned long long *__restrict__ ncut,
+                                                                const int32_t *__restrict__ fb,
+                                                                const double *__restrict__ fbr2,
+                                                                const unsigned long long *__restrict__ nfb,
+                                                                double *__restrict__ d) {
+    __shared__ int32_t lst[kFbList];
+    __shared__ double lsd[kFbList];
+    __shared__ int nl;
+    __shared__ unsigned long long best;
+    __shared__ double wr[16][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t nq = (int64_t)*nfb;
+    const int nc = t.nleaf == 1 ? 0 : (int)*ncut;  // one bucket: no bucket scan, no cut list
     for (int64_t q = blockIdx.x; q < nq; q += gridDim.x) {
         const int64_t ti = fb[q];
         const double ax = t.sx[ti], ay = t.sy[ti];
         double lim = fmin(fbr2[q] * (1.0 + 1e-3) + 1e-300, 1e5);
+        if (threadIdx.x == 0) {
+            nl = 0;
+            best = ~0ull;
+        }
+        __syncthreads();
+        // the cut nodes within the bound, and the nearest one (box distance,
+        // then list slot: a non-negative double orders as its bit pattern)
+        for (int k = threadIdx.x; k < nc; k += 1024) {
+            const int32_t u = cut[k];
+            const double b2 = box_d2(t.nbb[u], ax, ay);
+            if (b2 <= lim) {
+                const int p = atomicAdd(&nl, 1);
+                if (p < kFbList) {
+                    lst[p] = u;
+                    lsd[p] = b2;
+                    atomicMin(&best, ((unsigned long long)__double_as_longlong(b2) & ~0x3ffull) | (unsigned)p);
+                }
+            }
+        }
+        __syncthreads();
+        const int L = nl;
         double m0 = 1e5, m1 = 1e5, m2 = 1e5, m3 = 1e5, r[4];
-        // the target's own bucket: the one holding sorted position ti
-        int cur = t.nleaf == 1 ? 0 : t.bscan[ti] + t.bstart[ti] - 1;
-        near4_buckets(t, 1ull, lane == 0 ? cur : -1, lane, ax, ay, m0, m1, m2, m3);
-        near4_merge(lane, m0, m1, m2, m3, r);
-        lim = fmin(lim, r[3]);
-        // the path to the root (path[k]: the node k levels above the bucket)
-        // by one chain of parent loads; then each level's other child and
-        // its box distance, one level per lane; then the levels bottom-up
-        int L = 0;
-        bool full;
-        {
-            const int32_t *__restrict__ par = t.parent;
-            int c0 = cur;
-            if (lane == 0) path[0] = c0;
-            while (c0 != t.root && L + 1 < kLevCap) {
-                c0 = par[c0];
-                ++L;
-                if (lane == 0) path[L] = c0;
+        if (L <= kFbList && t.nleaf > 1) {
+            // the nearest cut node first, its buckets spread over the waves
+            // (16 each); its four smallest tighten the bound
+            const int e0 = L > 0 ? (int)(best & 0x3ffull) : -1;
+            if (e0 >= 0) {
+                const int32_t u = lst[e0];
+                const int64_t f = t.first[u], e1 = f + t.count[u];
+                const int64_t b0 = t.bscan[f], b1 = e1 < t.m ? (int64_t)t.bscan[e1] : t.nleaf;
+                for (int64_t bb = b0 + w * 16; bb < b1; bb += 256) {
+                    const int64_t b = bb + lane;
+                    const bool in = lane < 16 && b < b1 && box_d2(t.nbb[b], ax, ay) <= lim;
+                    near4_buckets(t, __ballot(in), (int)b, lane, ax, ay, m0, m1, m2, m3);
+                }
+                near4_block(lane, w, m0, m1, m2, m3, wr, r);
+                lim = fmin(lim, r[3]);
             }
-            full = c0 != t.root;  // deeper than kLevCap levels (never, by the key length)
-        }
-        __syncthreads();
-        for (int k = lane; k < L; k += 64) {
-            const NodeRec pr = t.rec[path[k + 1]];
-            const int sb = pr.a == path[k] ? pr.b : pr.a;
-            sibs[k] = sb;
-            sdist[k] = box_d2(t.nbb[sb], ax, ay);
-        }
-        __syncthreads();
-        for (int k = 0; k < L && !full; ++k) {
-            if (sdist[k] > lim) continue;  // nothing below the bound
-            if (!near4_subtree(t, stk, sibs[k], lane, ax, ay, lim, m0, m1, m2, m3)) {
-                full = true;
-                break;
+            // the others within the tightened bound, one cut node per wave
+            for (int e = w; e < L; e += 16) {
+                if (e == e0 || lsd[e] > lim) continue;
+                const int32_t u = lst[e];
+                const int64_t f = t.first[u], e1 = f + t.count[u];
+                const int64_t b0 = t.bscan[f], b1 = e1 < t.m ? (int64_t)t.bscan[e1] : t.nleaf;
+                for (int64_t bb = b0; bb < b1; bb += 64) {
+                    const int64_t b = bb + lane;
+                    const bool in = b < b1 && box_d2(t.nbb[b], ax, ay) <= lim;
+                    near4_buckets(t, __ballot(in), (int)b, lane, ax, ay, m0, m1, m2, m3);
+                }
             }
-            near4_merge(lane, m0, m1, m2, m3, r);
-            lim = fmin(lim, r[3]);
+        } else {
+            for (int64_t bb = (int64_t)w * 64; bb < t.nleaf; bb += 1024) {
+                const int64_t b = bb + lane;
+                const bool in = b < t.nleaf && box_d2(t.nbb[b], ax, ay) <= lim;
+                near4_buckets(t, __ballot(in), (int)b, lane, ax, ay, m0, m1, m2, m3);
+            }
         }
-        if (full) {  // stack filled: every bucket below the bound
-            m0 = m1 = m2 = m3 = 1e5;
-            for (int64_t b = lane; b < t.nleaf; b += 64)
-                if (box_d2(t.nbb[b], ax, ay) <= lim) near4_leaf(t, (int)b, ax, ay, m0, m1, m2, m3);
-        }
-        near4_merge(lane, m0, m1, m2, m3, r);
-        if (lane == 0) d[t.perm[ti]] = (sqrt(r[1]) + sqrt(r[2]) + sqrt(r[3])) / 3;
+        near4_block(lane, w, m0, m1, m2, m3, wr, r);
+        if (threadIdx.x == 0) d[t.perm[ti]] = (sqrt(r[1]) + sqrt(r[2]) + sqrt(r[3])) / 3;
     }
 }
 
@@ -2327,7 +2311,8 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
                                c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
     }
     tk.stop();
-    hipLaunchKernelGGL(k_tree_nearest_fallback, dim3(1024), dim3(64), 0, st, tv, t.fb.p, t.fbr2.p, nfb, c->d.p);
+    hipLaunchKernelGGL(k_tree_nearest_fallback, dim3(512), dim3(1024), 0, st, tv, t.arr.p, t.cnt.p + 10, t.fb.p,
+                       t.fbr2.p, nfb, c->d.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
