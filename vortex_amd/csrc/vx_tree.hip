@@ -78,6 +78,7 @@ struct TreeBufs {
     int32_t root = 0;
     int order = 16;
     int64_t fcap = 4096;  // domain-scale diffusion targets listed at most (set per diffusion)
+    unsigned long long *h_far = nullptr;  // pinned: the last diffusion's domain-scale target count
     hipEvent_t ev = nullptr;                       // read-back completion
 };
 
@@ -1663,9 +1664,9 @@ struct DiffPair<double> {
 };
 
 // Radius beyond which a diffusion target is summed by k_diff_far: 5 % of
-// the root's radius.  At most t.fcap targets are listed (kFarCap or one in
-// 16 included vortices, whichever is more: sparse wakes have many); further
-// ones stay in their group's wave.
+// the root's radius.  At most t.fcap targets are listed (the most of kFarCap,
+// one in 16 included vortices and a quarter more than the last diffusion
+// found: sparse wakes have many); further ones stay in their group's wave.
 constexpr int64_t kFarCap = 4096;
 __device__ __forceinline__ double far_radius(const TreeDev &t) {
     const double4 b = t.nbb[t.root];
@@ -2077,6 +2078,7 @@ void tree_release(vx_case *c) {
     t.gcs.release(); t.gcs2.release(); t.gflag.release(); t.gscan.release(); t.gfirst.release(); t.pcount.release(); t.arr.release(); t.sizes.release();
     t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.fbr2.release(); t.hpart.release();
     t.sd.release();
+    if (t.h_far) (void)hipHostFree(t.h_far);
     if (t.ev) (void)hipEventDestroy(t.ev);
     delete c->tree;
     c->tree = nullptr;
@@ -2334,7 +2336,18 @@ int tree_diffusion_pre(vx_case *c) {
     TreeBufs &t = *c->tree;
     hipStream_t st = c->stream;
     const int64_t nchunk = (t.nleaf + 63) / 64;
-    t.fcap = std::max<int64_t>(kFarCap, t.m / 16);
+    // The last diffusion's count comes from pinned memory, copied at the end
+    // of its far part; a caller that has not synchronised since reads an
+    // older count, which costs speed only.  The growth stays within a 4 GiB
+    // budget for the chunk partials.
+    if (!t.h_far) {
+        VX_CHECK(hipHostMalloc((void **)&t.h_far, sizeof(unsigned long long), hipHostMallocDefault));
+        *t.h_far = 0;
+    }
+    const int64_t last = (int64_t)*(volatile unsigned long long *)t.h_far;
+    const int64_t budget = (int64_t(1) << 32) / (3 * (int64_t)sizeof(double) * std::max<int64_t>(nchunk, 1));
+    const int64_t grow = std::min<int64_t>({last + last / 4, t.m, budget});
+    t.fcap = std::max<int64_t>({kFarCap, t.m / 16, grow});
     if (t.hpart.ensure((size_t)(3 * nchunk * t.fcap), st) || t.cbox.ensure((size_t)nchunk, st) ||
         t.fpairs.ensure((size_t)(nchunk * t.fcap), st) || t.ftgt.ensure((size_t)t.fcap, st) ||
         t.sd.ensure((size_t)t.m, st))
@@ -2357,6 +2370,7 @@ int tree_diffusion_far(vx_case *c, hipStream_t st) {
     // all counts stay on the device (grid-stride kernels sized for the list's capacity)
     const int64_t nchunk = (t.nleaf + 63) / 64;
     hipLaunchKernelGGL(k_far_zero, dim3(512), dim3(256), 0, st, t.hpart.p, nchunk, cnt, t.fcap);
+    VX_CHECK(hipMemcpyAsync(t.h_far, cnt + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)((nchunk + 3) / 4)), dim3(256), 0, st, tv, nchunk, t.cbox.p);
     hipLaunchKernelGGL(k_far_pairs, dim3(256), dim3(256), 0, st, tv, t.fb.p, cnt, nchunk, t.cbox.p, cut, c->d.p,
                        t.fpairs.p, cnt + 3, t.ftgt.p);
