@@ -147,6 +147,34 @@ def test_tree_diffusion_vs_direct_oracle(ref, prec, tol):
     assert info["heavy"] > 0  # isolated vortices go through the split direct kernel
 
 
+@pytest.mark.parametrize("prec,tol", [(VX_PREC_F64, 1e-9), (VX_PREC_MIXED, 1e-3)])
+def test_tree_diffusion_far_list_grows_past_its_first_capacity(ref, prec, tol):
+    """A sparse cloud in which most targets reach beyond 5 % of the domain:
+    the first diffusion lists 4096 of them and leaves the rest in their
+    groups' waves; the next one sizes its list from that count and lists
+    more.  Both match the direct sums."""
+    n = 20000
+    rng = np.random.default_rng(11)
+    x, y = rng.uniform(-1.5, 9.5, n), rng.uniform(-9.0, 9.0, n)
+    g = rng.uniform(0.001, 0.004, n)  # one sign: no cancellation in the weight sums
+    s = (n + 31) // 32 * 32
+    X, Y, G = tail(x, y, g, s)
+    eps, nu = 0.004, 1e-3
+    pan = geo.circle(200, 0.5)
+    c = Case(make_params(pan, precision=prec, ve_size=eps, nu=nu, order=32), pan)
+    c.set_state(X, Y, G)
+    heavy = []
+    for _ in range(2):
+        c.tree_biot_savart(s, (0.0, 0.0))
+        u0, v0, d = c.velocity(s)
+        c.tree_diffusion()
+        u, v, _ = c.velocity(s)
+        ur, vr = ref.diffusion(X, Y, G, d, nu, eps, u0, v0)
+        assert rel_err(u - u0, ur - u0) < tol and rel_err(v - v0, vr - v0) < tol
+        heavy.append(c.tree_stats()["heavy"])
+    assert heavy[0] == 4096 and heavy[1] > 4096
+
+
 def test_tree_and_direct_speed_agree_inside_a_time_step(ref):
     """Whole time steps with the tree switched on from 2000 vortices against
     the direct path on the same start state (FP64)."""
