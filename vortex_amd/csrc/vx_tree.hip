@@ -1464,10 +1464,11 @@ void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y
 // The cut nodes partition the buckets: those whose box lies within the bound
 // are listed in LDS.  The nearest one's buckets (a contiguous range) are
 // scanned first, and the bound tightened to the fourth smallest found; then
-// each wave takes listed cut nodes within the new bound.  A bucket within the
-// bound is read one particle per lane into the lane's private four smallest;
-// the lanes' lists are merged at the end.  Should the list overflow (or the
-// tree be a single bucket), every bucket is tested against the first bound.
+// each wave takes listed cut nodes within the bound, tightening its own copy
+// after each.  A bucket within the bound is read one particle per lane into
+// the lane's private four smallest; the lanes' lists are merged at the end.
+// Should the list overflow (or the tree be a single bucket), every bucket is
+// tested instead, each wave tightening its bound as it goes.
 __device__ __forceinline__ double box_d2(const double4 &b, double ax, double ay) {
     const double ex = fmax(0.0, fmax(b.x - ax, ax - b.y));
     const double ey = fmax(0.0, fmax(b.z - ay, ay - b.w));
@@ -1600,23 +1601,35 @@ __global__ __launch_bounds__(1024) void k_tree_nearest_fallback(TreeDev t, const
                 near4_block(lane, w, m0, m1, m2, m3, wr, r);
                 lim = fmin(lim, r[3]);
             }
-            // the others within the tightened bound, one cut node per wave
+            // the others within the bound, one cut node per wave; each wave
+            // tightens its own copy of the bound to the fourth smallest of
+            // its lanes (four real distances bound the fourth smallest)
             for (int e = w; e < L; e += 16) {
                 if (e == e0 || lsd[e] > lim) continue;
                 const int32_t u = lst[e];
                 const int64_t f = t.first[u], e1 = f + t.count[u];
                 const int64_t b0 = t.bscan[f], b1 = e1 < t.m ? (int64_t)t.bscan[e1] : t.nleaf;
+                uint64_t any = 0;
                 for (int64_t bb = b0; bb < b1; bb += 64) {
                     const int64_t b = bb + lane;
-                    const bool in = b < b1 && box_d2(t.nbb[b], ax, ay) <= lim;
-                    near4_buckets(t, __ballot(in), (int)b, lane, ax, ay, m0, m1, m2, m3);
+                    const uint64_t bm = __ballot(b < b1 && box_d2(t.nbb[b], ax, ay) <= lim);
+                    any |= bm;
+                    near4_buckets(t, bm, (int)b, lane, ax, ay, m0, m1, m2, m3);
+                }
+                if (any) {
+                    near4_merge(lane, m0, m1, m2, m3, r);
+                    lim = fmin(lim, r[3]);
                 }
             }
         } else {
             for (int64_t bb = (int64_t)w * 64; bb < t.nleaf; bb += 1024) {
                 const int64_t b = bb + lane;
-                const bool in = b < t.nleaf && box_d2(t.nbb[b], ax, ay) <= lim;
-                near4_buckets(t, __ballot(in), (int)b, lane, ax, ay, m0, m1, m2, m3);
+                const uint64_t bm = __ballot(b < t.nleaf && box_d2(t.nbb[b], ax, ay) <= lim);
+                near4_buckets(t, bm, (int)b, lane, ax, ay, m0, m1, m2, m3);
+                if (bm) {
+                    near4_merge(lane, m0, m1, m2, m3, r);
+                    lim = fmin(lim, r[3]);
+                }
             }
         }
         near4_block(lane, w, m0, m1, m2, m3, wr, r);
