@@ -1467,8 +1467,8 @@ void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y
 // each wave takes listed cut nodes within the bound, tightening its own copy
 // after each.  A bucket within the bound is read one particle per lane into
 // the lane's private four smallest; the lanes' lists are merged at the end.
-// Should the list overflow (or the tree be a single bucket), every bucket is
-// tested instead, each wave tightening its bound as it goes.
+// Should the list overflow (or the tree be a single bucket), every other
+// bucket is tested instead, each wave tightening its bound as it goes.
 __device__ __forceinline__ double box_d2(const double4 &b, double ax, double ay) {
     const double ex = fmax(0.0, fmax(b.x - ax, ax - b.y));
     const double ey = fmax(0.0, fmax(b.z - ay, ay - b.w));
@@ -1568,8 +1568,9 @@ __global__ __launch_bounds__(1024) void k_tree_nearest_fallback(TreeDev t, const
             best = ~0ull;
         }
         __syncthreads();
-        // the cut nodes within the bound, and the nearest one (box distance,
-        // then list slot: a non-negative double orders as its bit pattern)
+        // the cut nodes within the bound, and the nearest one (box distance
+        // with its low mantissa bits replaced by the cut-list index: a
+        // non-negative double orders as its bit pattern; any choice is exact)
         for (int k = threadIdx.x; k < nc; k += 1024) {
             const int32_t u = cut[k];
             const double b2 = box_d2(t.nbb[u], ax, ay);
@@ -1578,34 +1579,37 @@ __global__ __launch_bounds__(1024) void k_tree_nearest_fallback(TreeDev t, const
                 if (p < kFbList) {
                     lst[p] = u;
                     lsd[p] = b2;
-                    atomicMin(&best, ((unsigned long long)__double_as_longlong(b2) & ~0x3ffull) | (unsigned)p);
                 }
+                atomicMin(&best, ((unsigned long long)__double_as_longlong(b2) & ~0xffffffull) |
+                                     (unsigned long long)(k & 0xffffff));
             }
         }
         __syncthreads();
         const int L = nl;
         double m0 = 1e5, m1 = 1e5, m2 = 1e5, m3 = 1e5, r[4];
-        if (L <= kFbList && t.nleaf > 1) {
-            // the nearest cut node first, its buckets spread over the waves
-            // (16 each); its four smallest tighten the bound
-            const int e0 = L > 0 ? (int)(best & 0x3ffull) : -1;
-            if (e0 >= 0) {
-                const int32_t u = lst[e0];
-                const int64_t f = t.first[u], e1 = f + t.count[u];
-                const int64_t b0 = t.bscan[f], b1 = e1 < t.m ? (int64_t)t.bscan[e1] : t.nleaf;
-                for (int64_t bb = b0 + w * 16; bb < b1; bb += 256) {
-                    const int64_t b = bb + lane;
-                    const bool in = lane < 16 && b < b1 && box_d2(t.nbb[b], ax, ay) <= lim;
-                    near4_buckets(t, __ballot(in), (int)b, lane, ax, ay, m0, m1, m2, m3);
-                }
-                near4_block(lane, w, m0, m1, m2, m3, wr, r);
-                lim = fmin(lim, r[3]);
+        // the nearest cut node first, its buckets spread over the waves (16
+        // each); its four smallest tighten the bound
+        int32_t us = -1;
+        int64_t ub0 = 0, ub1 = 0;
+        if (t.nleaf > 1 && best != ~0ull) {
+            us = cut[best & 0xffffffull];
+            const int64_t f = t.first[us], e1 = f + t.count[us];
+            ub0 = t.bscan[f];
+            ub1 = e1 < t.m ? (int64_t)t.bscan[e1] : t.nleaf;
+            for (int64_t bb = ub0 + w * 16; bb < ub1; bb += 256) {
+                const int64_t b = bb + lane;
+                const bool in = lane < 16 && b < ub1 && box_d2(t.nbb[b], ax, ay) <= lim;
+                near4_buckets(t, __ballot(in), (int)b, lane, ax, ay, m0, m1, m2, m3);
             }
+            near4_block(lane, w, m0, m1, m2, m3, wr, r);
+            lim = fmin(lim, r[3]);
+        }
+        if (L <= kFbList && t.nleaf > 1) {
             // the others within the bound, one cut node per wave; each wave
             // tightens its own copy of the bound to the fourth smallest of
             // its lanes (four real distances bound the fourth smallest)
             for (int e = w; e < L; e += 16) {
-                if (e == e0 || lsd[e] > lim) continue;
+                if (lst[e] == us || lsd[e] > lim) continue;
                 const int32_t u = lst[e];
                 const int64_t f = t.first[u], e1 = f + t.count[u];
                 const int64_t b0 = t.bscan[f], b1 = e1 < t.m ? (int64_t)t.bscan[e1] : t.nleaf;
@@ -1622,9 +1626,10 @@ __global__ __launch_bounds__(1024) void k_tree_nearest_fallback(TreeDev t, const
                 }
             }
         } else {
+            // every bucket outside the nearest cut node's range
             for (int64_t bb = (int64_t)w * 64; bb < t.nleaf; bb += 1024) {
                 const int64_t b = bb + lane;
-                const uint64_t bm = __ballot(b < t.nleaf && box_d2(t.nbb[b], ax, ay) <= lim);
+                const uint64_t bm = __ballot(b < t.nleaf && (b < ub0 || b >= ub1) && box_d2(t.nbb[b], ax, ay) <= lim);
                 near4_buckets(t, bm, (int)b, lane, ax, ay, m0, m1, m2, m3);
                 if (bm) {
                     near4_merge(lane, m0, m1, m2, m3, r);
