@@ -1686,6 +1686,11 @@ struct DiffPair<double> {
 // one in 16 included vortices and a quarter more than the last diffusion
 // found: sparse wakes have many); further ones stay in their group's wave.
 constexpr int64_t kFarCap = 4096;
+// A target goes to k_diff_far when kFarCut d (or its cut-off times d, if
+// smaller) exceeds far_radius(): the same targets for both cut-offs, so the
+// FP64 path's 30 d widens some group waves rather than listing a large part
+// of a sparse wake.
+constexpr double kFarCut = 18.0;
 __device__ __forceinline__ double far_radius(const TreeDev &t) {
     const double4 b = t.nbb[t.root];
     const double rr = 0.5 * sqrt((b.y - b.x) * (b.y - b.x) + (b.w - b.z) * (b.w - b.z));
@@ -1820,7 +1825,7 @@ void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
 }
 
 // The diffusion's d in sorted order, negated for the targets listed for
-// k_diff_far (radius cut * d beyond far_radius(), at most t.fcap of them,
+// k_diff_far (min(cut, kFarCut) d beyond far_radius(), at most t.fcap of them,
 // appended with one atomic per work-group; cnt[2] counts every such target,
 // so targets past the capacity stay with their group's wave).
 __global__ __launch_bounds__(256) void k_far_list(TreeDev t, double cut, const double *__restrict__ dd,
@@ -1834,7 +1839,7 @@ __global__ __launch_bounds__(256) void k_far_list(TreeDev t, double cut, const d
     bool far = false;
     if (ti < t.m) {
         di = dd[t.perm[ti]];
-        far = cut * di > far_radius(t);
+        far = fmin(cut, kFarCut) * di > far_radius(t);
     }
     const uint64_t m = __ballot(far);
     if (lane == 0) wcnt[w] = __popcll(m);
