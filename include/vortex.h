@@ -166,6 +166,13 @@ typedef struct vx_tree_info {
                                         evaluates m2p_evals directly), last BS */
 } vx_tree_info;
 int vx_tree_stats(vx_case *c, vx_tree_info *info);
+/* The last tree build's sorted Morton keys and slot permutation (its first
+ * count entries, count <= the slots it keyed): keys[k] is the k-th smallest
+ * key and perm[k] its slot; equal keys keep the slot order.  Excluded slots
+ * (g = 0 farther than 400 from the deletion box) carry key 0xFFFFFFFF and
+ * sort last.  No reference counterpart: the reference's tree
+ * (src/kernel.cu:79-763) is a bisection kd-tree and is inactive. */
+int vx_tree_order(vx_case *c, uint32_t *keys, int32_t *perm, int64_t count);
 int vx_step_move(vx_case *c, double out4[4]);
 int vx_compact(vx_case *c);
 int vx_setka(vx_case *c, int kind, int32_t *COL);                           /* COL[n] out */

@@ -620,3 +620,59 @@ void vxr_time_step(double *x, double *y, double *g, double *u, double *v, double
     vxr_Step(x, y, g, u, v, n, pan, P, prm, forces4);
     *d_g = forces4[0];
 }
+
+/* ---- Morton keys of the velocity tree (build-new, no reference line) ----- */
+uint32_t vxr_spread16(uint32_t v) {
+    /* bit b of v moves to bit 2b, one bit at a time */
+    uint32_t r = 0;
+    for (int b = 0; b < 16; ++b) r |= ((v >> b) & 1u) << (2 * b);
+    return r;
+}
+
+int64_t vxr_morton_keys(const double *x, const double *y, const double *g, int64_t s,
+                        const double box[4], uint32_t *keys, double out_box[4]) {
+    double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+    int64_t m = 0;
+    char *in = (char *)malloc((size_t)(s > 0 ? s : 1));
+    for (int64_t i = 0; i < s; ++i) {
+        int inc = 0;
+        if (isfinite(x[i]) && isfinite(y[i])) {
+            const double ex = fmax(0.0, fmax(box[0] - x[i], x[i] - box[1]));
+            const double ey = fmax(0.0, fmax(box[2] - y[i], y[i] - box[3]));
+            inc = (g[i] != 0.0) || (ex * ex + ey * ey <= 400.0 * 400.0);
+        }
+        in[i] = (char)inc;
+        if (inc) {
+            ++m;
+            x0 = fmin(x0, x[i]); x1 = fmax(x1, x[i]);
+            y0 = fmin(y0, y[i]); y1 = fmax(y1, y[i]);
+        }
+    }
+    if (!(x0 <= x1)) { x0 = 0.0; x1 = 1.0; y0 = 0.0; y1 = 1.0; }
+    const double wx = x1 - x0, wy = y1 - y0;
+    for (int64_t i = 0; i < s; ++i) {
+        if (!in[i]) { keys[i] = 0xFFFFFFFFu; continue; }
+        double fx = wx > 0 ? (x[i] - x0) / wx * 65534.0 : 0.0;
+        double fy = wy > 0 ? (y[i] - y0) / wy * 65534.0 : 0.0;
+        fx = fmin(fmax(fx, 0.0), 65534.0);
+        fy = fmin(fmax(fy, 0.0), 65534.0);
+        keys[i] = vxr_spread16((uint32_t)fx) | (vxr_spread16((uint32_t)fy) << 1);
+    }
+    free(in);
+    if (out_box) { out_box[0] = x0; out_box[1] = x1; out_box[2] = y0; out_box[3] = y1; }
+    return m;
+}
+
+/* Stable counting sort, one key byte at a time from the least significant. */
+void vxr_stable_order(const uint32_t *keys, int64_t s, int32_t *perm) {
+    int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * (size_t)(s > 0 ? s : 1));
+    for (int64_t i = 0; i < s; ++i) perm[i] = (int32_t)i;
+    for (int shift = 0; shift < 32; shift += 8) {
+        int64_t cnt[257] = {0};
+        for (int64_t i = 0; i < s; ++i) cnt[((keys[perm[i]] >> shift) & 255u) + 1]++;
+        for (int b = 0; b < 256; ++b) cnt[b + 1] += cnt[b];
+        for (int64_t i = 0; i < s; ++i) tmp[cnt[(keys[perm[i]] >> shift) & 255u]++] = perm[i];
+        memcpy(perm, tmp, sizeof(int32_t) * (size_t)s);
+    }
+    free(tmp);
+}

@@ -172,6 +172,21 @@ void vxr_time_step(double *x, double *y, double *g, double *u, double *v, double
                    const vxr_params *prm, double vinf_x, double vinf_y,
                    double *d_g, double forces4[4]);
 
+/* Morton keys of the velocity tree (a build-new structure: the reference's
+ * own tree, src/kernel.cu:79-763, is a bisection kd-tree, inactive in every
+ * shipped conf).  A slot is included when its position is finite and g != 0
+ * or it lies within 400 of the deletion box (box = {x_min, x_max, y_min,
+ * y_max}); the included slots' bounding box (out_box, {x0, x1, y0, y1}; the
+ * unit box when none is included) is mapped to [0, 65534] per axis and the
+ * two 16-bit integers interleaved, x in the even bits.  Excluded slots get
+ * 0xFFFFFFFF.  Returns the included count. */
+uint32_t vxr_spread16(uint32_t v);
+int64_t vxr_morton_keys(const double *x, const double *y, const double *g, int64_t s,
+                        const double box[4], uint32_t *keys, double out_box[4]);
+/* Stable order of the keys (ties keep the index order): perm[k] is the slot
+ * of the k-th smallest key. */
+void vxr_stable_order(const uint32_t *keys, int64_t s, int32_t *perm);
+
 #ifdef __cplusplus
 }
 #endif

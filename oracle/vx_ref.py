@@ -77,6 +77,13 @@ def lib():
         L.vxr_num_threads.restype = C.c_int
         L.vxr_set_num_threads.argtypes = [C.c_int]
         L.vxr_sizeof_panel.restype = C.c_size_t
+        _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+        _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.vxr_spread16.argtypes = [C.c_uint32]
+        L.vxr_spread16.restype = C.c_uint32
+        L.vxr_morton_keys.argtypes = [_dp, _dp, _dp, _i64, _dp, _u32p, _dp]
+        L.vxr_morton_keys.restype = _i64
+        L.vxr_stable_order.argtypes = [_u32p, _i64, _i32p]
         _lib = L
     return _lib
 
@@ -232,6 +239,22 @@ def vort_creation(x, y, g, n, M, pan, prm: Params, vinf, d_g):
     lib().vxr_vort_creation(x, y, g, n, _f(M).reshape(-1), _pp(pan), pan.size, C.byref(prm),
                             vinf[0], vinf[1], d_g)
     return x, y, g
+
+
+def morton_keys(x, y, g, box):
+    """Tree keys of slots [0, len(x)): (keys uint32, included count, key box)."""
+    x, y, g = _f(x), _f(y), _f(g)
+    keys = np.zeros(x.size, np.uint32)
+    ob = np.zeros(4)
+    m = lib().vxr_morton_keys(x, y, g, x.size, _f(box), keys, ob)
+    return keys, int(m), ob
+
+
+def stable_order(keys):
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    perm = np.zeros(keys.size, np.int32)
+    lib().vxr_stable_order(keys, keys.size, perm)
+    return perm
 
 
 class State:

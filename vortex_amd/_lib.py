@@ -28,7 +28,7 @@ EXPORTS = [
     "vx_velocity_probe", "vx_right_part", "vx_birth", "vx_biot_savart", "vx_diffusion",
     "vx_diffusion_wall", "vx_step_move", "vx_compact", "vx_setka", "vx_collapse",
     "vx_case_timings", "vx_case_enable_timing", "vx_tree_biot_savart", "vx_tree_diffusion",
-    "vx_tree_stats", "vx_points_move", "vx_points_relax",
+    "vx_tree_stats", "vx_points_move", "vx_points_relax", "vx_tree_order",
 ]
 
 
@@ -114,6 +114,7 @@ def lib():
         "vx_tree_stats": ([vp, C.POINTER(TreeInfo)], C.c_int),
         "vx_points_move": ([vp, dp, dp, dp, dp, dp, i64], C.c_int),
         "vx_points_relax": ([dp, dp, dp, dp, dp, i64, d, d], C.c_int),
+        "vx_tree_order": ([vp, dp, dp, i64], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -61,6 +61,8 @@ int vx_device_count(int *count) {
     return 0;
 }
 
+static int case_init(vx_case *c, const vx_params *prm, const vx_panel *panels, int64_t P);
+
 int vx_case_create(const vx_params *prm, const vx_panel *panels, int64_t P, vx_case **out) {
     VX_ARG(prm && panels && out, "null pointer");
     VX_ARG(P >= 2, "need at least two panels");
@@ -74,6 +76,16 @@ int vx_case_create(const vx_params *prm, const vx_panel *panels, int64_t P, vx_c
         return 1;
     }
     vx_case *c = new vx_case();
+    const int rc = case_init(c, prm, panels, P);
+    if (rc) {
+        vx_case_destroy(c);  // releases whatever the failed set-up allocated
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+static int case_init(vx_case *c, const vx_params *prm, const vx_panel *panels, int64_t P) {
     c->prm = *prm;
     if (c->prm.n_col < 0) c->prm.n_col = 0;
     if (c->prm.n_of_points <= 0) c->prm.n_of_points = 20;
@@ -155,12 +167,8 @@ int vx_case_create(const vx_params *prm, const vx_panel *panels, int64_t P, vx_c
         }
         c->births_near = near;
     }
-    if (reserve(c, c->prm.inc_step > 0 ? c->prm.inc_step : 8192)) {
-        vx_case_destroy(c);
-        return 1;
-    }
+    if (reserve(c, c->prm.inc_step > 0 ? c->prm.inc_step : 8192)) return 1;
     VX_CHECK(hipStreamSynchronize(c->stream));
-    *out = c;
     return 0;
 }
 
@@ -174,7 +182,7 @@ int vx_case_destroy(vx_case *c) {
     c->xf.release(); c->yf.release(); c->gf.release();
     c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();
     c->keys.release(); c->keys_alt.release(); c->vals.release(); c->vals_alt.release();
-    c->cub_tmp.release(); c->gridp.release(); c->dcount.release(); c->colseg.release();
+    c->ptmp.release(); c->gridp.release(); c->dcount.release(); c->colseg.release();
     if (c->side) (void)hipStreamSynchronize(c->side);
     c->wlist.release(); c->wtmp.release(); c->wstage.release();
     tree_release(c);
@@ -242,6 +250,9 @@ int vx_case_set_state(vx_case *c, const double *x, const double *y, const double
         VX_CHECK(hipMemcpyAsync(c->y.p, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
         VX_CHECK(hipMemcpyAsync(c->g.p, g, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     }
+    // the padding that the stages read beyond n ([n, ceil32(n + P))) holds
+    // the far tail (g = 0), whatever an earlier, larger state left there
+    if (launch_fill_tail(c, n, std::min(c->cap, ceil32(n + c->P)))) return 1;
     VX_CHECK(hipStreamSynchronize(c->stream));
     c->n = n;
     return 0;
@@ -356,6 +367,11 @@ int vx_tree_diffusion(vx_case *c) {
 int vx_tree_stats(vx_case *c, vx_tree_info *info) {
     VX_ARG(c && info, "null pointer");
     return tree_stats(c, info);
+}
+
+int vx_tree_order(vx_case *c, uint32_t *keys, int32_t *perm, int64_t count) {
+    VX_ARG(c && count >= 0, "case/count");
+    return tree_copy_order(c, keys, perm, count);
 }
 
 int vx_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src) {

@@ -136,12 +136,12 @@ struct vx_case {
     vx::DBuf<int32_t> flags, scan, lists, col, misc;
     vx::DBuf<uint64_t> keys, keys_alt;
     vx::DBuf<int32_t> vals, vals_alt;
-    vx::DBuf<unsigned char> cub_tmp;
+    vx::DBuf<int32_t> ptmp;                            // tile totals of the scans and selections (main stream)
     vx::DBuf<unsigned char> gridp;                     // collapse grid parameters (device)
     vx::DBuf<int64_t> dcount;                         // device counters
     vx::DBuf<int32_t> colseg;                         // collapse candidate counts per list segment
     vx::DBuf<int32_t> wlist;                          // wall targets (side stream)
-    vx::DBuf<unsigned char> wtmp;                     // their selection's scratch
+    vx::DBuf<int32_t> wtmp;                           // their selection's tile totals
     vx::DBuf<double> wstage;                          // their velocity terms, per list slot
     int dn_slot = 1;                                  // dcount slot of the step's current count (1 or 8)
     int64_t *h_count = nullptr;                       // pinned read-back slots (16)
@@ -220,6 +220,7 @@ int tree_diffusion_pre(vx_case *c);
 int tree_diffusion_far(vx_case *c, hipStream_t st);
 int tree_diffusion_main(vx_case *c);
 int tree_stats(vx_case *c, vx_tree_info *info);
+int tree_copy_order(vx_case *c, uint32_t *keys, int32_t *perm, int64_t count);
 void tree_release(vx_case *c);
 // vx_step.hip
 // The step stages run on a device-resident vortex count *dn (dcount[1]):

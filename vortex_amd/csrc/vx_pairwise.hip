@@ -9,9 +9,9 @@
 // targets spaced one block apart so that target loads and stores coalesce.
 #include <cfloat>
 
-#include <hipcub/hipcub.hpp>
 
 #include "vx_internal.h"
+#include "vx_prim.h"
 
 namespace vx {
 
@@ -705,11 +705,8 @@ int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
     // sources nearer than 2 R_b, in index order
     {
         const RhsNear pred{c->x.p, c->y.p, c->g.p, cx, cy, rnear2};
-        hipcub::CountingInputIterator<int32_t> it(0);
-        size_t bytes = 0;
-        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, c->lists.p, nlist, n_src, pred, st));
-        if (c->cub_tmp.ensure(bytes, st)) return 1;
-        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, c->lists.p, nlist, n_src, pred, st));
+        if (c->ptmp.ensure((size_t)prim::tiles_of(n_src) + 1, st)) return 1;
+        VX_CHECK(prim::select_flags(prim::pred_flag(pred), n_src, prim::WriteIndex{c->lists.p}, c->ptmp.p, nlist, st));
     }
     // pairs within 2 R_s of each segment
     if (mixed(c)) {

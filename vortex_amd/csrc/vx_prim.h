@@ -1,0 +1,379 @@
+// Device-wide primitives for wave64 (gfx950): exclusive/inclusive scans,
+// order-preserving selection (stream compaction) and an LSD radix sort of
+// 32-bit keys with 32-bit values.
+//
+// Every primitive is reduce-then-scan over tiles of kTile items: one launch
+// reduces each tile, one work-group scans the tile totals, one launch scans
+// each tile again from its offset.  No work-group ever waits on another
+// (no decoupled look-back), so a primitive on a second stream beside a long
+// kernel is only slowed by sharing the CUs, never held up by predecessors
+// that cannot be scheduled.
+//
+// Items of a tile are read striped (round r, thread t: item r * kT + t), so
+// loads coalesce; the scan order is still the item order.  Flags (0/1) are
+// scanned with one ballot and popcount per wave and round; general values
+// with shuffles.
+//
+// Functor interfaces (all __device__, called only for i < n of the launch):
+//   flag:   int operator()(int64_t i) const            -> 0 or 1
+//   value:  int32_t operator()(int64_t i) const
+//   writer: void operator()(int64_t i, int32_t pos, int flag) const  (select / flag scan)
+//           void operator()(int64_t i, int32_t v) const              (value scan)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vx {
+namespace prim {
+
+constexpr int kT = 256;             // threads per work-group
+constexpr int kIPT = 16;            // items per thread
+constexpr int kTile = kT * kIPT;    // items per tile
+constexpr int kW = kT / 64;         // waves per work-group
+
+inline int64_t tiles_of(int64_t n) { return n > 0 ? (n + kTile - 1) / kTile : 0; }
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+__device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
+
+struct OpSum {
+    static constexpr int32_t id = 0;
+    __device__ __forceinline__ int32_t operator()(int32_t a, int32_t b) const { return a + b; }
+};
+struct OpMax {
+    static constexpr int32_t id = INT32_MIN;
+    __device__ __forceinline__ int32_t operator()(int32_t a, int32_t b) const { return a > b ? a : b; }
+};
+
+template <class Op>
+__device__ __forceinline__ int32_t wave_reduce(int32_t a, Op op) {
+    for (int o = 32; o > 0; o >>= 1) a = op(a, __shfl_xor(a, o, 64));
+    return a;
+}
+
+// Inclusive scan over the 64 lanes of a wave.
+template <class Op>
+__device__ __forceinline__ int32_t wave_inclusive(int32_t a, Op op) {
+    const int lane = lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t b = __shfl_up(a, o, 64);
+        if (lane >= o) a = op(a, b);
+    }
+    return a;
+}
+
+// ---- flags: count, scan, select ------------------------------------------------
+// Phase 1: flags per tile.
+template <class F>
+__global__ __launch_bounds__(kT) void k_flag_count(F f, int64_t n, int32_t *__restrict__ tile_tot) {
+    __shared__ int32_t wc[kW];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    int32_t c = 0;
+#pragma unroll 4
+    for (int r = 0; r < kIPT; ++r) {
+        const int64_t i = base + r * kT + threadIdx.x;
+        const int fl = i < n ? f(i) : 0;
+        c += __popcll(__ballot(fl));
+    }
+    if (lane_id() == 0) wc[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t s = 0;
+        for (int w = 0; w < kW; ++w) s += wc[w];
+        tile_tot[blockIdx.x] = s;
+    }
+}
+
+// Phase 2: exclusive scan of the tile totals in place, in one work-group of
+// 1024 threads (each takes a contiguous run of tiles); the grand total goes
+// to *total (converted to TT).
+template <class TT, class Op = OpSum>
+__global__ __launch_bounds__(1024) void k_tile_scan(int32_t *__restrict__ tot, int64_t ntiles, TT *__restrict__ total,
+                                                    Op op = Op()) {
+    __shared__ int32_t ws[16];
+    const int64_t per = (ntiles + 1023) / 1024;
+    const int64_t b = (int64_t)threadIdx.x * per, e = b + per < ntiles ? b + per : ntiles;
+    int32_t s = Op::id;
+    for (int64_t k = b; k < e; ++k) s = op(s, tot[k]);
+    // exclusive scan of the per-thread sums over the work-group
+    const int32_t inc = wave_inclusive(s, op);
+    if (lane_id() == 63) ws[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int32_t v = threadIdx.x < 16 ? ws[threadIdx.x] : Op::id;
+        const int32_t vi = wave_inclusive(v, op);
+        if (threadIdx.x < 16) ws[threadIdx.x] = vi;  // inclusive over waves
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    int32_t pre = w > 0 ? ws[w - 1] : Op::id;
+    const int32_t excl_lane = __shfl_up(inc, 1, 64);
+    if (lane_id() > 0) pre = op(pre, excl_lane);
+    for (int64_t k = b; k < e; ++k) {
+        const int32_t v = tot[k];
+        tot[k] = pre;
+        pre = op(pre, v);
+    }
+    if (threadIdx.x == 1023 && total) *total = (TT)(ws[15]);
+}
+
+// Phase 3: each flagged item's rank among the flagged items (its output
+// position); the writer sees every item i < n with its flag.
+template <class F, class Wr>
+__global__ __launch_bounds__(kT) void k_flag_apply(F f, int64_t n, const int32_t *__restrict__ tile_off, Wr wr) {
+    __shared__ int32_t wc[2][kW];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    const int w = threadIdx.x >> 6;
+    int32_t run = tile_off[blockIdx.x];
+    for (int r = 0; r < kIPT; ++r) {
+        const int64_t i = base + r * kT + threadIdx.x;
+        const int fl = i < n ? f(i) : 0;
+        const uint64_t bal = __ballot(fl);
+        if (lane_id() == 0) wc[r & 1][w] = __popcll(bal);
+        __syncthreads();
+        int32_t off = run, all = 0;
+#pragma unroll
+        for (int q = 0; q < kW; ++q) {
+            const int32_t v = wc[r & 1][q];
+            off += q < w ? v : 0;
+            all += v;
+        }
+        if (i < n) wr(i, off + (int32_t)__popcll(bal & lanes_below()), fl);
+        run += all;
+    }
+}
+
+// ---- general values: scan ---------------------------------------------------------
+template <class V, class Op>
+__global__ __launch_bounds__(kT) void k_value_reduce(V val, int64_t n, int32_t *__restrict__ tile_tot, Op op) {
+    __shared__ int32_t wc[kW];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    int32_t a = Op::id;
+#pragma unroll 4
+    for (int r = 0; r < kIPT; ++r) {
+        const int64_t i = base + r * kT + threadIdx.x;
+        if (i < n) a = op(a, val(i));
+    }
+    a = wave_reduce(a, op);
+    if (lane_id() == 0) wc[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t s = Op::id;
+        for (int w = 0; w < kW; ++w) s = op(s, wc[w]);
+        tile_tot[blockIdx.x] = s;
+    }
+}
+
+// INCL: the writer gets the inclusive scan, else the exclusive one.
+template <class V, class Op, bool INCL, class Wr>
+__global__ __launch_bounds__(kT) void k_value_apply(V val, int64_t n, const int32_t *__restrict__ tile_off, Op op,
+                                                    Wr wr) {
+    __shared__ int32_t wc[2][kW];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    const int w = threadIdx.x >> 6;
+    int32_t run = tile_off[blockIdx.x];
+    for (int r = 0; r < kIPT; ++r) {
+        const int64_t i = base + r * kT + threadIdx.x;
+        const int32_t v = i < n ? val(i) : Op::id;
+        const int32_t inc = wave_inclusive(v, op);
+        if (lane_id() == 63) wc[r & 1][w] = inc;
+        __syncthreads();
+        int32_t pre = run, all = run;
+#pragma unroll
+        for (int q = 0; q < kW; ++q) {
+            const int32_t t = wc[r & 1][q];
+            if (q < w) pre = op(pre, t);
+            all = op(all, t);
+        }
+        const int32_t ex = __shfl_up(inc, 1, 64);  // outside the branch: every lane takes part
+        if (i < n) wr(i, INCL ? op(pre, inc) : (lane_id() > 0 ? op(pre, ex) : pre));
+        run = all;
+    }
+}
+
+// ---- host wrappers ---------------------------------------------------------------------
+// tmp must hold tiles_of(n) int32 entries.  total (device) receives the count
+// of flagged items (or the value scan's total), converted to TT.
+
+template <class F, class Wr, class TT>
+inline hipError_t select_flags(F f, int64_t n, Wr wr, int32_t *tmp, TT *total, hipStream_t st) {
+    const int64_t nt = tiles_of(n);
+    if (nt > 0) hipLaunchKernelGGL(k_flag_count<F>, dim3((unsigned)nt), dim3(kT), 0, st, f, n, tmp);
+    // with no tiles this only writes a zero total
+    hipLaunchKernelGGL((k_tile_scan<TT, OpSum>), dim3(1), dim3(1024), 0, st, tmp, nt, total, OpSum());
+    if (nt > 0)
+        hipLaunchKernelGGL((k_flag_apply<F, Wr>), dim3((unsigned)nt), dim3(kT), 0, st, f, n, (const int32_t *)tmp, wr);
+    return hipGetLastError();
+}
+
+template <bool INCL, class V, class Op, class Wr, class TT>
+inline hipError_t scan_values(V val, int64_t n, Op op, Wr wr, int32_t *tmp, TT *total, hipStream_t st) {
+    const int64_t nt = tiles_of(n);
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_value_reduce<V, Op>), dim3((unsigned)nt), dim3(kT), 0, st, val, n, tmp, op);
+    hipLaunchKernelGGL((k_tile_scan<TT, Op>), dim3(1), dim3(1024), 0, st, tmp, nt, total, op);
+    hipLaunchKernelGGL((k_value_apply<V, Op, INCL, Wr>), dim3((unsigned)nt), dim3(kT), 0, st, val, n,
+                       (const int32_t *)tmp, op, wr);
+    return hipGetLastError();
+}
+
+// ---- fill (instead of memsets in per-step code) --------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T *__restrict__ a, int64_t n, T v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+template <typename T>
+inline hipError_t fill(T *a, int64_t n, T v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)(nb < 4096 ? nb : 4096)), dim3(256), 0, st, a, n, v);
+    return hipGetLastError();
+}
+
+// ---- common functors -----------------------------------------------------------------
+template <class P>
+struct PredFlag {  // a boolean predicate of an int32 index as a flag
+    P p;
+    __device__ int operator()(int64_t i) const { return p((int32_t)i) ? 1 : 0; }
+};
+template <class P>
+inline PredFlag<P> pred_flag(P p) { return PredFlag<P>{p}; }
+
+struct ArrayFlag {  // a[i] != 0
+    const int32_t *a;
+    __device__ int operator()(int64_t i) const { return a[i] != 0; }
+};
+struct ArrayValue {
+    const int32_t *a;
+    __device__ int32_t operator()(int64_t i) const { return a[i]; }
+};
+struct WriteScan {  // out[i] = exclusive count of flags before i (flag scan) / the scan value (value scan)
+    int32_t *out;
+    __device__ void operator()(int64_t i, int32_t pos, int) const { out[i] = pos; }
+    __device__ void operator()(int64_t i, int32_t v) const { out[i] = v; }
+};
+struct WriteIndex {  // out[pos] = i for flagged items (ordered selection of indices)
+    int32_t *out;
+    __device__ void operator()(int64_t i, int32_t pos, int fl) const {
+        if (fl) out[pos] = (int32_t)i;
+    }
+};
+
+// ---- LSD radix sort: 32-bit keys, 32-bit values, 8-bit digits ---------------------------
+// Four stable passes.  Per pass: each tile's digit histogram (LDS atomics),
+// the digit-major exclusive offsets of (digit, tile) in one work-group, and
+// the scatter, where each key's rank within its tile is its wave's count of
+// earlier lanes with the same digit (8 ballots) plus the counts of the
+// tile's earlier waves and rounds.  Stable, so equal keys keep their input
+// order.
+constexpr int kRadixBits = 8;
+constexpr int kBins = 1 << kRadixBits;
+
+template <int DUMMY = 0>
+__global__ __launch_bounds__(kT) void k_radix_hist(const uint32_t *__restrict__ key, int64_t n, int shift,
+                                                   int32_t *__restrict__ hist, int64_t ntiles) {
+    __shared__ int32_t h[kBins];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll 4
+    for (int r = 0; r < kIPT; ++r) {
+        const int64_t i = base + r * kT + threadIdx.x;
+        if (i < n) atomicAdd(&h[(key[i] >> shift) & (kBins - 1)], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// One work-group of kBins threads: thread d scans digit d's row over the
+// tiles, then the digit totals are scanned across the work-group.
+template <int DUMMY = 0>
+__global__ __launch_bounds__(kBins) void k_radix_offsets(int32_t *__restrict__ hist, int64_t ntiles) {
+    __shared__ int32_t ws[kBins / 64];
+    int32_t *row = hist + (int64_t)threadIdx.x * ntiles;
+    int32_t s = 0;
+    for (int64_t t = 0; t < ntiles; ++t) s += row[t];
+    const int32_t inc = wave_inclusive(s, OpSum());
+    if (lane_id() == 63) ws[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    int32_t pre = inc - s;
+    for (int q = 0; q < (int)(threadIdx.x >> 6); ++q) pre += ws[q];
+    for (int64_t t = 0; t < ntiles; ++t) {
+        const int32_t v = row[t];
+        row[t] = pre;
+        pre += v;
+    }
+}
+
+template <int DUMMY = 0>
+__global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
+                                                      int64_t n, int shift, const int32_t *__restrict__ hist,
+                                                      int64_t ntiles, uint32_t *__restrict__ kout,
+                                                      int32_t *__restrict__ vout) {
+    __shared__ int32_t run[kBins];    // offset of the next key of each digit
+    __shared__ int32_t wcnt[kW][kBins];
+    const int w = threadIdx.x >> 6;
+    run[threadIdx.x] = hist[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    for (int r = 0; r < kIPT; ++r) {
+        for (int q = 0; q < kW; ++q) wcnt[q][threadIdx.x] = 0;
+        const int64_t i = base + r * kT + threadIdx.x;
+        const bool ok = i < n;
+        uint32_t k = 0;
+        int32_t v = 0;
+        if (ok) {
+            k = kin[i];
+            v = vin[i];
+        }
+        const int d = (int)((k >> shift) & (kBins - 1));
+        // lanes of this wave holding the same digit
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; ++b) {
+            const uint64_t bb = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        const int32_t below = (int32_t)__popcll(peers & lanes_below());
+        __syncthreads();  // wcnt cleared, run up to date
+        if (ok && below == 0) wcnt[w][d] = (int32_t)__popcll(peers);  // the digit's first lane in this wave
+        __syncthreads();
+        if (ok) {
+            int32_t pos = run[d] + below;
+            for (int q = 0; q < w; ++q) pos += wcnt[q][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        int32_t add = 0;
+#pragma unroll
+        for (int q = 0; q < kW; ++q) add += wcnt[q][threadIdx.x];
+        run[threadIdx.x] += add;
+    }
+}
+
+// Sorts (k0, v0) of n items; k1, v1 are the same size.  The result ends in
+// k0, v0 (four passes).  hist must hold kBins * tiles_of(n) int32 entries.
+inline hipError_t radix_sort_pairs(uint32_t *k0, uint32_t *k1, int32_t *v0, int32_t *v1, int64_t n,
+                                   int32_t *hist, hipStream_t st) {
+    const int64_t nt = tiles_of(n);
+    if (nt == 0) return hipSuccess;
+    uint32_t *ka = k0, *kb = k1;
+    int32_t *va = v0, *vb = v1;
+    for (int pass = 0; pass < 32 / kRadixBits; ++pass) {
+        const int shift = pass * kRadixBits;
+        hipLaunchKernelGGL(k_radix_hist<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka, n, shift, hist,
+                           nt);
+        hipLaunchKernelGGL(k_radix_offsets<0>, dim3(1), dim3(kBins), 0, st, hist, nt);
+        hipLaunchKernelGGL(k_radix_scatter<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka,
+                           (const int32_t *)va, n, shift, (const int32_t *)hist, nt, kb, vb);
+        uint32_t *tk = ka; ka = kb; kb = tk;
+        int32_t *tv = va; va = vb; vb = tv;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace prim
+}  // namespace vx

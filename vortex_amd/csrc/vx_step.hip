@@ -8,9 +8,9 @@
 // so they are run-to-run deterministic.
 #pragma clang fp contract(off)
 
-#include <hipcub/hipcub.hpp>
 
 #include "vx_internal.h"
+#include "vx_prim.h"
 
 namespace vx {
 
@@ -239,11 +239,8 @@ int launch_step_move(vx_case *c) {
     int32_t *ncand = (int32_t *)(c->dcount.p + 4);
     if (n > 0) {
         const HitCandidate pred{c->x.p, c->y.p, c->g.p, c->u.p, c->v.p, k};
-        hipcub::CountingInputIterator<int32_t> it(0);
-        size_t bytes = 0;
-        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cand, ncand, n, pred, st));
-        if (c->cub_tmp.ensure(bytes, st)) return 1;
-        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cand, ncand, n, pred, st));
+        if (c->ptmp.ensure((size_t)prim::tiles_of(n) + 1, st)) return 1;
+        VX_CHECK(prim::select_flags(prim::pred_flag(pred), n, prim::WriteIndex{cand}, c->ptmp.p, ncand, st));
         hipLaunchKernelGGL(k_hit_panels, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->u.p, c->v.p, c->pan, k,
                            cand, ncand, hitp);
     }
@@ -289,11 +286,8 @@ int launch_points_move(vx_case *c, int64_t m) {
     int32_t *hitp = c->misc.p, *cand = c->lists.p;
     int32_t *ncand = (int32_t *)(c->dcount.p + 6);
     const HitCandidate pred{c->px.p, c->py.p, c->pg.p, c->pu.p, c->pv.p, k};
-    hipcub::CountingInputIterator<int32_t> it(0);
-    size_t bytes = 0;
-    VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cand, ncand, m, pred, st));
-    if (c->cub_tmp.ensure(bytes, st)) return 1;
-    VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cand, ncand, m, pred, st));
+    if (c->ptmp.ensure((size_t)prim::tiles_of(m) + 1, st)) return 1;
+    VX_CHECK(prim::select_flags(prim::pred_flag(pred), m, prim::WriteIndex{cand}, c->ptmp.p, ncand, st));
     hipLaunchKernelGGL(k_hit_panels, dim3(256), dim3(256), 0, st, c->px.p, c->py.p, c->pu.p, c->pv.p, c->pan, k,
                        cand, ncand, hitp);
     hipLaunchKernelGGL(k_points_move, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, c->px.p, c->py.p, c->pg.p,
@@ -399,13 +393,10 @@ int launch_compact(vx_case *c, int64_t n_ub) {
         return 1;
     int64_t *dn = step_dn(c), *dn1 = c->dcount.p + (c->dn_slot == 1 ? 8 : 1);
     const unsigned nb = (unsigned)((n_ub + 255) / 256);
-    hipcub::CountingInputIterator<int64_t> it(0);
-    hipcub::TransformInputIterator<int32_t, LiveFlag, hipcub::CountingInputIterator<int64_t>> fl(it,
-                                                                                               LiveFlag{c->g.p, dn});
-    size_t bytes = 0;
-    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, fl, c->scan.p, (int)n_ub, st));
-    if (c->cub_tmp.ensure(bytes, st)) return 1;
-    VX_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, fl, c->scan.p, (int)n_ub, st));
+    // exclusive scan of the live flags (ballot and popcount per wave)
+    if (c->ptmp.ensure((size_t)prim::tiles_of(n_ub) + 1, st)) return 1;
+    VX_CHECK(prim::select_flags(LiveFlag{c->g.p, dn}, n_ub, prim::WriteScan{c->scan.p}, c->ptmp.p,
+                                (int32_t *)nullptr, st));
     int32_t *holes = c->lists.p, *donors = c->lists.p + n_ub;
     hipLaunchKernelGGL(k_compact_lists, dim3(nb), dim3(256), 0, st, c->g.p, c->scan.p, dn, dn1, holes, donors);
     const unsigned nb2 = (unsigned)((2 * n_ub + 255) / 256);
@@ -879,11 +870,8 @@ int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared) {
         hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, cb);
         hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, step_dn(c), cb);
         const ColCandidate pred{c->col.p, step_dn(c)};
-        hipcub::CountingInputIterator<int32_t> it(0);
-        size_t bytes = 0;
-        VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, cb.cand, cb.ncs, (int)n_ub, pred, st));
-        if (c->cub_tmp.ensure(bytes, st)) return 1;
-        VX_CHECK(hipcub::DeviceSelect::If(c->cub_tmp.p, bytes, it, cb.cand, cb.ncs, (int)n_ub, pred, st));
+        if (c->ptmp.ensure((size_t)prim::tiles_of(n_ub) + 1, st)) return 1;
+        VX_CHECK(prim::select_flags(prim::pred_flag(pred), n_ub, prim::WriteIndex{cb.cand}, c->ptmp.p, cb.ncs, st));
     }
     hipLaunchKernelGGL(k_col_leaves, dim3(nbc), dim3(256), 0, st, c->col.p, cb);
     hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, cb.rest, cb.nrest, cb.pcd, cb.state,

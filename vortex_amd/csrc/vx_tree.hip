@@ -29,11 +29,11 @@
 // the leaf centres (so close pairs keep FP32 relative accuracy), partial sums
 // per source leaf / far node accumulated in FP64.  FP64: absolute FP64
 // coordinates throughout.
-#include <hipcub/hipcub.hpp>
-
+#include <utility>
 #include <vector>
 
 #include "vx_internal.h"
+#include "vx_prim.h"
 
 namespace vx {
 
@@ -73,6 +73,7 @@ struct TreeBufs {
     DBuf<NodeRec> rec;
     DBuf<unsigned long long> cnt;
     DBuf<unsigned char> tmp;
+    int64_t s = 0;                         // slots keyed and sorted by the last build
     int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
     int64_t ngroup = 0;                    // target groups (gfirst)
     int32_t root = 0;
@@ -2107,16 +2108,6 @@ void tree_release(vx_case *c) {
     c->tree = nullptr;
 }
 
-// Key sort: rocPRIM's onesweep radix sort, forced at every size (its default
-// switches to a merge sort up to 2^20 items, several times slower here), with
-// 11-bit digits: three passes over the 32-bit keys instead of four (about
-// 20 us less at 2^20).
-using KeySortConfig = rocprim::radix_sort_config<
-    rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 11,
-                                        rocprim::block_radix_rank_algorithm::match>,
-    0>;
-
 static int tree_order(const vx_case *c) { return c->prm.order >= 32 ? 32 : 16; }
 
 // Builds the tree on stream st in two parts: A (inclusion, box, keys, sort;
@@ -2127,11 +2118,12 @@ static int tree_build_a(vx_case *c, int64_t s, hipStream_t st, Births B) {
     TreeBufs &t = tree_of(c);
     const int order = tree_order(c);
     t.order = order;
+    t.s = s;
     const int64_t nbs = (s + 255) / 256;
     if (t.keys.ensure(s, st) || t.keys_alt.ensure(s, st) || t.idx.ensure(s, st) || t.perm.ensure(s, st) ||
         t.incl.ensure(s, st) || t.part.ensure(nbs + 1, st) || t.pcount.ensure(nbs, st) || t.cnt.ensure(16, st))
         return 1;
-    VX_CHECK(hipMemsetAsync(t.cnt.p, 0, sizeof(unsigned long long) * 16, st));
+    VX_CHECK(prim::fill(t.cnt.p, 16, 0ull, st));
     const double4 dbox = make_double4(c->prm.x_min, c->prm.x_max, c->prm.y_min, c->prm.y_max);
     hipLaunchKernelGGL(k_incl_box, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, s, dbox, t.incl.p,
                        t.part.p, t.pcount.p, B);
@@ -2142,12 +2134,12 @@ static int tree_build_a(vx_case *c, int64_t s, hipStream_t st, Births B) {
     VX_CHECK(hipEventRecord(t.ev, st));
     hipLaunchKernelGGL(k_tree_keys, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, t.incl.p, s,
                        t.part.p + nbs, t.keys.p, t.idx.p, B);
-    size_t bytes = 0;
-    VX_CHECK(rocprim::radix_sort_pairs<KeySortConfig>(nullptr, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p,
-                                                      (unsigned)s, 0, 32, st));
-    if (t.tmp.ensure(bytes, st)) return 1;
-    VX_CHECK(rocprim::radix_sort_pairs<KeySortConfig>(t.tmp.p, bytes, t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p,
-                                                      (unsigned)s, 0, 32, st));
+    // stable LSD radix sort of (key, index); the sorted pairs end in keys and
+    // idx, which then take the names keys_alt and perm
+    if (t.tmp.ensure(sizeof(int32_t) * (size_t)(prim::kBins * prim::tiles_of(s) + 1), st)) return 1;
+    VX_CHECK(prim::radix_sort_pairs(t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p, s, (int32_t *)t.tmp.p, st));
+    std::swap(t.keys, t.keys_alt);
+    std::swap(t.idx, t.perm);
     return 0;
 }
 
@@ -2177,27 +2169,26 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
     const int dmin = 2 * (int)fmin(fmax(lv, 0.0), 15.0);
     {
         hipLaunchKernelGGL(k_group_cells, dim3(gm), dim3(256), 0, st, t.keys_alt.p, m, 32 - dmin, t.gcs.p);
-        size_t b1 = 0, b2 = 0;
-        VX_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, b1, t.gcs.p, t.gcs2.p, hipcub::Max(), (int)m, st));
-        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, t.gflag.p, t.gscan.p, (int)m, st));
-        if (t.tmp.ensure(std::max(b1, b2), st)) return 1;
-        VX_CHECK(hipcub::DeviceScan::InclusiveScan(t.tmp.p, b1, t.gcs.p, t.gcs2.p, hipcub::Max(), (int)m, st));
+        if (t.tmp.ensure(sizeof(int32_t) * (size_t)(prim::tiles_of(m) + 1), st)) return 1;
+        int32_t *tt = (int32_t *)t.tmp.p;
+        VX_CHECK(prim::scan_values<true>(prim::ArrayValue{t.gcs.p}, m, prim::OpMax(), prim::WriteScan{t.gcs2.p}, tt,
+                                         (int32_t *)nullptr, st));
         hipLaunchKernelGGL(k_group_flags, dim3(gm), dim3(256), 0, st, t.gcs2.p, m, t.gflag.p);
-        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(t.tmp.p, b2, t.gflag.p, t.gscan.p, (int)m, st));
+        VX_CHECK(prim::select_flags(prim::ArrayFlag{t.gflag.p}, m, prim::WriteScan{t.gscan.p}, tt, (int32_t *)nullptr,
+                                    st));
         hipLaunchKernelGGL(k_group_first, dim3(gm), dim3(256), 0, st, t.gflag.p, t.gscan.p, m, t.gfirst.p);
     }
     if (m > kLeaf) {
-        VX_CHECK(hipMemsetAsync(t.bstart.p, 0, sizeof(int32_t) * m, st));
+        VX_CHECK(prim::fill(t.bstart.p, m, 0, st));
         hipLaunchKernelGGL(k_radix, dim3(gm), dim3(256), 0, st, t.keys_alt.p, m, t.rlo.p, t.rhi.p, t.rleft.p,
                            t.rright.p);
         hipLaunchKernelGGL(k_bucket_mark, dim3(gm), dim3(256), 0, st, t.keys_alt.p, t.rlo.p, t.rhi.p, t.rleft.p,
                            t.rright.p, m, dmin, t.bstart.p, t.upper.p);
-        size_t b1 = 0, b2 = 0;
-        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, t.bstart.p, t.bscan.p, (int)m, st));
-        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, t.upper.p, t.uscan.p, (int)(m - 1), st));
-        if (t.tmp.ensure(std::max(b1, b2), st)) return 1;
-        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(t.tmp.p, b1, t.bstart.p, t.bscan.p, (int)m, st));
-        VX_CHECK(hipcub::DeviceScan::ExclusiveSum(t.tmp.p, b2, t.upper.p, t.uscan.p, (int)(m - 1), st));
+        int32_t *tt = (int32_t *)t.tmp.p;
+        VX_CHECK(prim::select_flags(prim::ArrayFlag{t.bstart.p}, m, prim::WriteScan{t.bscan.p}, tt,
+                                    (int32_t *)nullptr, st));
+        VX_CHECK(prim::select_flags(prim::ArrayFlag{t.upper.p}, m - 1, prim::WriteScan{t.uscan.p}, tt,
+                                    (int32_t *)nullptr, st));
     }
     // the group, bucket and upper-node counts in one read-back
     if (t.sizes.ensure(4, st)) return 1;
@@ -2222,7 +2213,7 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
         t.parent.ensure(nn, st) || t.toplist.ensure(nn, st) || t.rec.ensure(nn, st) || t.cd.ensure(nn * order, st) ||
         t.cf.ensure(nn * order, st))
         return 1;
-    VX_CHECK(hipMemsetAsync(t.parent.p, 0xff, sizeof(int32_t) * nn, st));
+    VX_CHECK(prim::fill(t.parent.p, nn, -1, st));
     if (m > kLeaf) {
         hipLaunchKernelGGL(k_bucket_first, dim3(gm), dim3(256), 0, st, t.bstart.p, t.bscan.p, m, t.first.p);
         hipLaunchKernelGGL(k_bucket_count, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, t.first.p, nb, m,
@@ -2430,6 +2421,19 @@ int tree_diffusion_main(vx_case *c) {
 
 int launch_tree_diffusion(vx_case *c) {
     return tree_diffusion_pre(c) || tree_diffusion_far(c, c->stream) || tree_diffusion_main(c);
+}
+
+int tree_copy_order(vx_case *c, uint32_t *keys, int32_t *perm, int64_t count) {
+    if (!c->tree || count > c->tree->s) {
+        set_error("bad argument: count beyond the slots of the last tree build");
+        return 2;
+    }
+    if (count <= 0) return 0;
+    TreeBufs &t = *c->tree;
+    if (keys) VX_CHECK(hipMemcpyAsync(keys, t.keys_alt.p, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, c->stream));
+    if (perm) VX_CHECK(hipMemcpyAsync(perm, t.perm.p, sizeof(int32_t) * count, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int tree_stats(vx_case *c, vx_tree_info *info) {

@@ -11,9 +11,9 @@
 //   V_i += nu II3 / II0.
 // Targets outside the box of birth points grown by 5 max(l) meet no panel and
 // are skipped without touching the panel list.
-#include <hipcub/hipcub.hpp>
 
 #include "vx_internal.h"
+#include "vx_prim.h"
 
 namespace vx {
 
@@ -139,11 +139,8 @@ int launch_wall_side(vx_case *c, int64_t n_tgt, hipStream_t st) {
         return 1;
     int32_t *cnt = (int32_t *)(c->dcount.p + 5);
     const InWallBox pred{c->x.p, c->y.p, c->wall_box};
-    hipcub::CountingInputIterator<int32_t> it(0);
-    size_t bytes = 0;
-    VX_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, c->wlist.p, cnt, n_tgt, pred, st));
-    if (c->wtmp.ensure(bytes, c->stream)) return 1;
-    VX_CHECK(hipcub::DeviceSelect::If(c->wtmp.p, bytes, it, c->wlist.p, cnt, n_tgt, pred, st));
+    if (c->wtmp.ensure((size_t)prim::tiles_of(n_tgt) + 1, c->stream)) return 1;
+    VX_CHECK(prim::select_flags(prim::pred_flag(pred), n_tgt, prim::WriteIndex{c->wlist.p}, c->wtmp.p, cnt, st));
     if (c->prm.precision == VX_PREC_MIXED)
         hipLaunchKernelGGL(k_diffusion_wall<float>, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->d.p,
                            c->wlist.p, cnt, c->pan, c->prm.nu, c->prm.n_of_points, c->wstage.p);

@@ -213,6 +213,13 @@ class Case:
         check(lib().vx_tree_stats(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in TreeInfo._fields_}
 
+    def tree_order(self, count):
+        """Sorted Morton keys and slot permutation of the last tree build."""
+        keys = np.zeros(count, np.uint32)
+        perm = np.zeros(count, np.int32)
+        check(lib().vx_tree_order(self._h, ptr(keys), ptr(perm), count))
+        return keys, perm
+
     def diffusion_wall(self, n_tgt):
         check(lib().vx_diffusion_wall(self._h, int(n_tgt)))
 
