@@ -123,6 +123,13 @@ int vx_time_step(vx_case *c, double vinf_x, double vinf_y, double forces4[4]);
 int vx_run_steps(vx_case *c, int64_t steps, double vinf_x, double vinf_y, double *forces4_per_step);
 int vx_velocity_probe(vx_case *c, const double *px, const double *py, int64_t m,
                       double vinf_x, double vinf_y, double *u, double *v);
+/* Probe points kept on the device (m = 0: none).  vx_time_step then computes
+ * their velocities every step after the birth, from the same sources as
+ * vx_velocity_probe, as the reference loop calls velocity_control
+ * (src/main.c:744, src/unit_1.cu:1254-1269); vx_case_get_probes copies the
+ * last step's m velocities. */
+int vx_case_set_probes(vx_case *c, const double *px, const double *py, int64_t m);
+int vx_case_get_probes(vx_case *c, double *u, double *v, int64_t m);
 
 /* ---- second component (tracer points) ------------------------------------
  * The reference advects two copies of the points of second_points_file each

@@ -37,14 +37,38 @@ def build() -> None:
 
 
 _lib = None
+_lib_path = LIB_PATH
+
+
+def host_has_avx512() -> bool:
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+    return all(f in flags for f in ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"))
+
+
+def select_build(fast: bool) -> str:
+    """Load the x86-64-v4 build of the same source (bench.py's CPU baseline)
+    when fast and the host supports AVX-512; returns the path in use."""
+    global _lib, _lib_path
+    want = LIB_PATH
+    if fast and host_has_avx512():
+        v4 = os.path.join(os.path.dirname(LIB_PATH), "libvx_ref_v4.so")
+        if os.path.exists(v4):
+            want = v4
+    if want != _lib_path:
+        _lib, _lib_path = None, want
+    lib()
+    return _lib_path
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if not os.path.exists(_lib_path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(_lib_path)
         L.vxr_panel_matrix.argtypes = [_vp, _i64, _dp]
         L.vxr_invert.argtypes = [_dp, _i64, _dp]
         L.vxr_invert.restype = C.c_int

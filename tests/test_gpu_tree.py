@@ -175,6 +175,34 @@ def test_tree_diffusion_far_list_grows_past_its_first_capacity(ref, prec, tol):
     assert heavy[0] == 4096 and heavy[1] > 4096
 
 
+@pytest.mark.parametrize("prec", [VX_PREC_F64, VX_PREC_MIXED])
+def test_tree_diffusion_overflowing_far_list_is_reproducible(prec):
+    """The first diffusion of a sparse cloud lists only the first 4096 of its
+    domain-scale targets (in sorted order) and leaves the rest in their
+    groups' waves; which targets take which path, and so every bit of u and
+    v, must not depend on scheduling.  Three fresh cases on the same state
+    agree bit for bit."""
+    n = 20000
+    rng = np.random.default_rng(11)
+    x, y = rng.uniform(-1.5, 9.5, n), rng.uniform(-9.0, 9.0, n)
+    g = rng.uniform(0.001, 0.004, n)
+    s = (n + 31) // 32 * 32
+    X, Y, G = tail(x, y, g, s)
+    pan = geo.circle(200, 0.5)
+    outs = []
+    for _ in range(3):
+        c = Case(make_params(pan, precision=prec, ve_size=0.004, nu=1e-3), pan)
+        c.set_state(X, Y, G)
+        c.tree_biot_savart(s, (0.0, 0.0))
+        c.tree_diffusion()
+        assert c.tree_stats()["heavy"] == 4096  # the list overflowed
+        u, v, d = c.velocity(s)
+        outs.append((u.copy(), v.copy()))
+        c.close()
+    for u, v in outs[1:]:
+        assert np.array_equal(u, outs[0][0]) and np.array_equal(v, outs[0][1])
+
+
 def test_tree_and_direct_speed_agree_inside_a_time_step(ref):
     """Whole time steps with the tree switched on from 2000 vortices against
     the direct path on the same start state (FP64)."""

@@ -341,3 +341,23 @@ def test_time_loop_circulation_bookkeeping(ref):
         tot = st.g[:st.n].sum() + st.d_g
         assert abs(tot) < 1e-12 * max(1.0, np.abs(st.g[:st.n]).sum()), (k, tot)
     assert st.n > 64
+
+
+def test_step_kind_order_known_answer(ref):
+    """Step runs the opposite-sign collapse passes before the same-sign ones
+    (src/unit_1.cu:1138-1216).  A(+1) at 0, B(+1) at 0.3 eps, C(-0.5) at
+    0.7 eps, both radii 0.5 eps: B and C merge first (0.4 eps apart; |g|-
+    weighted centroid 0.65/1.5 eps, g = 0.5), then A and the merged vortex
+    (0.433 eps apart; g-weighted centroid at 0.2167/1.5 eps, g = 1.5): one
+    vortex.  Same-sign first would merge A and B and leave C 0.55 eps away."""
+    pan = geo.circle(64, 0.5)
+    eps = float(pan["length"][0])
+    prm = params(pan, ve_size=eps, r_col_diff_sign=0.5, r_col_same_sign=0.5, max_ve_g=10.0, n_col=1)
+    x = np.array([2.0, 2.0 + 0.3 * eps, 2.0 + 0.7 * eps])
+    y = np.zeros(3)
+    g = np.array([1.0, 1.0, -0.5])
+    xr, yr, gr, nr, _ = ref.Step(x, y, g, np.zeros(3), np.zeros(3), 3, pan, prm)
+    assert nr == 1
+    assert gr[0] == 1.5 and yr[0] == 0.0
+    xb = (x[1] * 1.0 + x[2] * 0.5) / 1.5
+    assert abs(xr[0] - (x[0] * 1.0 + xb * 0.5) / 1.5) < 1e-14

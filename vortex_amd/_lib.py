@@ -29,6 +29,7 @@ EXPORTS = [
     "vx_diffusion_wall", "vx_step_move", "vx_compact", "vx_setka", "vx_collapse",
     "vx_case_timings", "vx_case_enable_timing", "vx_tree_biot_savart", "vx_tree_diffusion",
     "vx_tree_stats", "vx_points_move", "vx_points_relax", "vx_tree_order",
+    "vx_case_set_probes", "vx_case_get_probes",
 ]
 
 
@@ -115,6 +116,8 @@ def lib():
         "vx_points_move": ([vp, dp, dp, dp, dp, dp, i64], C.c_int),
         "vx_points_relax": ([dp, dp, dp, dp, dp, i64, d, d], C.c_int),
         "vx_tree_order": ([vp, dp, dp, i64], C.c_int),
+        "vx_case_set_probes": ([vp, dp, dp, i64], C.c_int),
+        "vx_case_get_probes": ([vp, dp, dp, i64], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -177,7 +177,7 @@ int vx_case_destroy(vx_case *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     vx::DBuf<double> *db[] = {&c->L, &c->M, &c->x, &c->y, &c->g, &c->u, &c->v, &c->d, &c->rp, &c->gsheet,
                               &c->partial, &c->red, &c->tx, &c->ty, &c->tg, &c->px, &c->py, &c->pu, &c->pv, &c->pg,
-                              &c->rhs_s};
+                              &c->rhs_s, &c->prb_x, &c->prb_y, &c->prb_u, &c->prb_v, &c->prb_part};
     for (auto *p : db) p->release();
     c->xf.release(); c->yf.release(); c->gf.release();
     c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();
@@ -529,7 +529,10 @@ int vx_step(vx_case *c, double out4[4]) {
 int vx_time_step(vx_case *c, double vinf_x, double vinf_y, double forces4[4]) {
     VX_ARG(c && forces4, "null pointer");
     c->overlap_births = true;
-    const int rc = vx_vort_creation(c, vinf_x, vinf_y) || vx_speed(c, vinf_x, vinf_y);
+    // velocity_control after the birth (src/main.c:744), on the main stream
+    // beside the tree build
+    const int rc = vx_vort_creation(c, vinf_x, vinf_y) || launch_step_probes(c, vinf_x, vinf_y) ||
+                   vx_speed(c, vinf_x, vinf_y);
     c->overlap_births = false;
     if (c->born_at >= 0) {  // an early tree part left behind by a failed stage
         c->born_at = -1;
@@ -588,6 +591,32 @@ int vx_points_relax(const double *g, const double *uenv, const double *venv, dou
         u[i] = (uenv[i] - pu) / rel_t * dt + pu;
         v[i] = (venv[i] - pv) / rel_t * dt + pv;
     }
+    return 0;
+}
+
+int vx_case_set_probes(vx_case *c, const double *px, const double *py, int64_t m) {
+    VX_ARG(c && m >= 0 && (m == 0 || (px && py)), "probe points");
+    hipStream_t s = c->stream;
+    if (m) {
+        if (c->prb_x.ensure((size_t)m, s) || c->prb_y.ensure((size_t)m, s) || c->prb_u.ensure((size_t)m, s) ||
+            c->prb_v.ensure((size_t)m, s))
+            return 1;
+        VX_CHECK(hipMemcpyAsync(c->prb_x.p, px, sizeof(double) * m, hipMemcpyHostToDevice, s));
+        VX_CHECK(hipMemcpyAsync(c->prb_y.p, py, sizeof(double) * m, hipMemcpyHostToDevice, s));
+        VX_CHECK(hipMemsetAsync(c->prb_u.p, 0, sizeof(double) * m, s));
+        VX_CHECK(hipMemsetAsync(c->prb_v.p, 0, sizeof(double) * m, s));
+        VX_CHECK(hipStreamSynchronize(s));
+    }
+    c->prb_m = m;
+    return 0;
+}
+
+int vx_case_get_probes(vx_case *c, double *u, double *v, int64_t m) {
+    VX_ARG(c && u && v && m == c->prb_m, "m must equal the probe count");
+    if (!m) return 0;
+    VX_CHECK(hipMemcpyAsync(u, c->prb_u.p, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipMemcpyAsync(v, c->prb_v.p, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    VX_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
 

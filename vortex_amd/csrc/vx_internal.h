@@ -148,6 +148,10 @@ struct vx_case {
     double *h_red = nullptr;                          // pinned reduction read-back
     // probes
     vx::DBuf<double> px, py, pu, pv, pg;
+    // the per-step probes of vx_case_set_probes (velocity_control,
+    // src/main.c:744): points, last velocities, their split-K partials
+    vx::DBuf<double> prb_x, prb_y, prb_u, prb_v, prb_part;
+    int64_t prb_m = 0;
     // Morton tree of the last velocity stage
     vx::TreeBufs *tree = nullptr;
     // timing
@@ -196,6 +200,14 @@ int launch_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, 
 int launch_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src);
 int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y);  // -> c->rp
 int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vinf_y); // c->px.. -> c->pu,pv
+// The probe kernels on explicit buffers: part holds the split-K partials (at
+// most max_part doubles, max_part <= 0: as many as the split wants, see
+// probe_split); packed: mixed precision reads the FP32 copies (packed first)
+// instead of converting the FP64 state while staging.
+void probe_split(int64_t m, int64_t n_src, int64_t max_part, int64_t *chunk, int64_t *nchunk);
+int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, int64_t n_src, double vinf_x,
+                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed);
+int launch_step_probes(vx_case *c, double vinf_x, double vinf_y);  // c->prb_x.. -> c->prb_u, prb_v
 // vx_wall.hip
 // launch_wall_side queues the wall term on stream st (the main stream, or the
 // side stream after side_fork), into per-target staging; launch_wall_add

@@ -7,7 +7,10 @@
 // Sources are staged through LDS in 256-wide tiles (one wave64-aligned load
 // per thread, broadcast reads in the inner loop); each thread owns TPT
 // targets spaced one block apart so that target loads and stores coalesce.
+#include <algorithm>
 #include <cfloat>
+#include <mutex>
+#include <string>
 
 
 #include "vx_internal.h"
@@ -578,9 +581,29 @@ __global__ __launch_bounds__(kTile) void k_right_part_final(const double *__rest
 }
 
 // ---- A19: src/kernel.cu:1434-1475, split over source chunks ---------------
-template <typename T>
+// Sources stored as S (FP32 copies, or the FP64 state converted while staged:
+// the same FP32 values either way), arithmetic in T.
+template <typename T, typename S>
+__device__ __forceinline__ void stage_tile_cvt(Src4<T> *ls, const S *__restrict__ sx, const S *__restrict__ sy,
+                                               const S *__restrict__ sg, int64_t base, int64_t n_src) {
+    const int64_t j = base + threadIdx.x;
+    Src4<T> s;
+    if (j < n_src) {
+        s.x = (T)sx[j];
+        s.y = (T)sy[j];
+        s.g = (T)sg[j];
+    } else {
+        s.x = Arith<T>::kFar;
+        s.y = Arith<T>::kFar;
+        s.g = T(0);
+    }
+    s.pad = T(0);
+    ls[threadIdx.x] = s;
+}
+
+template <typename T, typename S = T>
 __global__ __launch_bounds__(kTile) void k_probe_partial(
-    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, int64_t n_src,
+    const S *__restrict__ sx, const S *__restrict__ sy, const S *__restrict__ sg, int64_t n_src,
     int64_t chunk, const double *__restrict__ px, const double *__restrict__ py, int64_t m, T eps2,
     double *__restrict__ part) {
     __shared__ Src4<T> ls[kTile];
@@ -593,7 +616,7 @@ __global__ __launch_bounds__(kTile) void k_probe_partial(
     double au = 0.0, av = 0.0;
     for (int64_t base = lo; base < hi; base += kTile) {
         __syncthreads();
-        stage_tile<T>(ls, sx, sy, sg, base, hi);
+        stage_tile_cvt<T, S>(ls, sx, sy, sg, base, hi);
         __syncthreads();
         const int cnt = (int)min((int64_t)kTile, hi - base);
         for (int jj = 0; jj < cnt; ++jj) {
@@ -734,28 +757,66 @@ int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
     return 0;
 }
 
-int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vinf_y) {
+// Probe chunking: at most max_part doubles of split-K partials (2 per probe
+// and chunk); max_part <= 0 lets the split choose.
+void probe_split(int64_t m, int64_t n_src, int64_t max_part, int64_t *chunk, int64_t *nchunk) {
+    const int64_t tb = (m + kTile - 1) / kTile;
+    split(n_src, tb, chunk, nchunk);
+    if (max_part > 0 && 2 * *nchunk * m > max_part) {
+        const int64_t want = max_part / (2 * m) > 1 ? max_part / (2 * m) : 1;
+        const int64_t tiles = (n_src + kTile - 1) / kTile;
+        const int64_t tiles_per = (tiles + want - 1) / want;
+        *chunk = (tiles_per > 0 ? tiles_per : 1) * kTile;
+        *nchunk = n_src > 0 ? (n_src + *chunk - 1) / *chunk : 1;
+    }
+}
+
+int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, int64_t n_src, double vinf_x,
+                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed) {
     if (m <= 0) return 0;
     const int64_t tb = (m + kTile - 1) / kTile;
     int64_t chunk, nchunk;
-    split(n_src, tb, &chunk, &nchunk);
-    if (c->partial.ensure((size_t)(2 * nchunk * m), c->stream)) return 1;
+    probe_split(m, n_src, max_part, &chunk, &nchunk);
     const double eps2 = c->prm.ve_size * c->prm.ve_size;
-    if (mixed(c)) {
+    const dim3 grid((unsigned)tb, (unsigned)nchunk);
+    if (mixed(c) && packed) {
         if (launch_pack_f32(c, n_src)) return 1;
-        hipLaunchKernelGGL(k_probe_partial<float>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0,
-                           c->stream, c->xf.p, c->yf.p, c->gf.p, n_src, chunk, c->px.p, c->py.p, m,
-                           (float)eps2, c->partial.p);
+        hipLaunchKernelGGL((k_probe_partial<float, float>), grid, dim3(kTile), 0, c->stream, c->xf.p, c->yf.p,
+                           c->gf.p, n_src, chunk, px, py, m, (float)eps2, part);
+    } else if (mixed(c)) {
+        hipLaunchKernelGGL((k_probe_partial<float, double>), grid, dim3(kTile), 0, c->stream, c->x.p, c->y.p,
+                           c->g.p, n_src, chunk, px, py, m, (float)eps2, part);
     } else {
-        hipLaunchKernelGGL(k_probe_partial<double>, dim3((unsigned)tb, (unsigned)nchunk), dim3(kTile), 0,
-                           c->stream, c->x.p, c->y.p, c->g.p, n_src, chunk, c->px.p, c->py.p, m, eps2,
-                           c->partial.p);
+        hipLaunchKernelGGL((k_probe_partial<double, double>), grid, dim3(kTile), 0, c->stream, c->x.p, c->y.p,
+                           c->g.p, n_src, chunk, px, py, m, eps2, part);
     }
     VX_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_probe_final, dim3((unsigned)m), dim3(kTile), 0, c->stream, c->partial.p, nchunk, m,
-                       vinf_x, vinf_y, c->pu.p, c->pv.p);
+    hipLaunchKernelGGL(k_probe_final, dim3((unsigned)m), dim3(kTile), 0, c->stream, part, nchunk, m, vinf_x, vinf_y,
+                       pu, pv);
     VX_CHECK(hipGetLastError());
     return 0;
+}
+
+int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vinf_y) {
+    if (m <= 0) return 0;
+    int64_t chunk, nchunk;
+    probe_split(m, n_src, 0, &chunk, &nchunk);
+    if (c->partial.ensure((size_t)(2 * nchunk * m), c->stream)) return 1;
+    return launch_probe_to(c, c->px.p, c->py.p, m, n_src, vinf_x, vinf_y, c->pu.p, c->pv.p, c->partial.p, 0, true);
+}
+
+// The probes of vx_case_set_probes, sources [0, ceil32(n)) of the current
+// state, into c->prb_u / c->prb_v (own partials, so the step's split-K
+// buffers are left alone).
+int launch_step_probes(vx_case *c, double vinf_x, double vinf_y) {
+    const int64_t m = c->prb_m;
+    if (m <= 0) return 0;
+    const int64_t n_src = std::min<int64_t>((c->n + 31) / 32 * 32, c->cap);
+    int64_t chunk, nchunk;
+    probe_split(m, n_src, 0, &chunk, &nchunk);
+    if (c->prb_part.ensure((size_t)(2 * nchunk * m), c->stream)) return 1;
+    return launch_probe_to(c, c->prb_x.p, c->prb_y.p, m, n_src, vinf_x, vinf_y, c->prb_u.p, c->prb_v.p,
+                           c->prb_part.p, 0, false);
 }
 
 }  // namespace vx

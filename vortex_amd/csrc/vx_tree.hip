@@ -1826,40 +1826,28 @@ void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
 }
 
 // The diffusion's d in sorted order, negated for the targets listed for
-// k_diff_far (min(cut, kFarCut) d beyond far_radius(), at most t.fcap of them,
-// appended with one atomic per work-group; cnt[2] counts every such target,
-// so targets past the capacity stay with their group's wave).
-__global__ __launch_bounds__(256) void k_far_list(TreeDev t, double cut, const double *__restrict__ dd,
-                                                  double *__restrict__ sd, int32_t *__restrict__ lists,
-                                                  unsigned long long *__restrict__ cnt) {
-    __shared__ int wcnt[4];
-    __shared__ unsigned long long base;
-    const int64_t ti = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double di = 0.0;
-    bool far = false;
-    if (ti < t.m) {
-        di = dd[t.perm[ti]];
-        far = fmin(cut, kFarCut) * di > far_radius(t);
+// k_diff_far: those with min(cut, kFarCut) d beyond far_radius(), the first
+// t.fcap of them in sorted order (an ordered selection, so the listed set
+// does not depend on scheduling); cnt[2] counts every such target, and
+// targets past the capacity stay with their group's wave.
+struct FarFlag {
+    TreeDev t;
+    double cut;
+    const double *dd;
+    __device__ int operator()(int64_t ti) const { return fmin(cut, kFarCut) * dd[t.perm[ti]] > far_radius(t); }
+};
+struct FarWrite {
+    TreeDev t;
+    const double *dd;
+    double *sd;
+    int32_t *lists;
+    __device__ void operator()(int64_t ti, int32_t pos, int fl) const {
+        const double di = dd[t.perm[ti]];
+        const bool listed = fl && pos < t.fcap;
+        if (listed) lists[pos] = (int32_t)ti;
+        sd[ti] = listed ? -di : di;
     }
-    const uint64_t m = __ballot(far);
-    if (lane == 0) wcnt[w] = __popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-        base = tot ? atomicAdd(cnt + 2, (unsigned long long)tot) : 0ull;
-    }
-    __syncthreads();
-    if (far) {
-        unsigned long long k = base + __popcll(m & ((1ull << lane) - 1));
-        for (int q = 0; q < w; ++q) k += wcnt[q];
-        if (k < (unsigned long long)t.fcap)
-            lists[k] = (int32_t)ti;
-        else
-            far = false;
-    }
-    if (ti < t.m) sd[ti] = far ? -di : di;
-}
+};
 
 // Targets whose radius exceeds far_radius(), listed by k_tree_diffusion.  The
 // buckets are cut into chunks of 64 consecutive ones (Morton order) with a
@@ -2368,9 +2356,10 @@ int tree_diffusion_pre(vx_case *c) {
         return 1;
     unsigned long long *cnt = t.cnt.p + 4;  // pairs, overflow, far targets, far (target, chunk) pairs
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 0xdu);  // not the shared overflow flag
-    hipLaunchKernelGGL(k_far_list, dim3((unsigned)((t.m + 255) / 256)), dim3(256), 0, st, view(t), diff_cut_of(c),
-                       c->d.p, t.sd.p, t.fb.p, cnt);
-    VX_CHECK(hipGetLastError());
+    if (c->ptmp.ensure((size_t)prim::tiles_of(t.m) + 1, st)) return 1;
+    const TreeDev tv = view(t);
+    VX_CHECK(prim::select_flags(FarFlag{tv, diff_cut_of(c), c->d.p}, t.m, FarWrite{tv, c->d.p, t.sd.p, t.fb.p},
+                                c->ptmp.p, cnt + 2, st));
     return 0;
 }
 

@@ -213,6 +213,18 @@ class Case:
         check(lib().vx_tree_stats(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in TreeInfo._fields_}
 
+    def set_probes(self, px, py):
+        """Probe points kept on the device; time_step computes their velocities."""
+        px, py = _f64(px), _f64(py)
+        check(lib().vx_case_set_probes(self._h, ptr(px), ptr(py), px.size))
+        self._nprobe = px.size
+
+    def probes(self):
+        m = getattr(self, "_nprobe", 0)
+        u, v = np.zeros(m), np.zeros(m)
+        check(lib().vx_case_get_probes(self._h, ptr(u), ptr(v), m))
+        return u, v
+
     def tree_order(self, count):
         """Sorted Morton keys and slot permutation of the last tree build."""
         keys = np.zeros(count, np.uint32)

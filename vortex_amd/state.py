@@ -52,3 +52,35 @@ def uniform_wake(n: int, seed: int = 1, max_ve_g: float = 0.00628162,
     mag = rng.uniform(0.0, max_ve_g, n)
     sign = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     return x, y, mag * sign
+
+
+def jittered_wake(n: int, seed: int = 1, max_ve_g: float = 0.00628162, min_sep: float = 0.002,
+                  box=(-0.6, 10.0, -2.0, 2.0), body_radius: float = 0.5):
+    """A wake of exactly n vortices that holds its size: one vortex per cell
+    of a square grid over ``box`` minus the disk r < body_radius + min_sep,
+    jittered inside its cell so that any two stay at least ``min_sep`` apart
+    (above the collapse radii, so the first steps do not merge thousands of
+    pairs as a uniform random cloud does).  Circulations +-U(0, max_ve_g),
+    alternating in sign over a random order of the cells."""
+    rng = np.random.default_rng(seed)
+    area = (box[1] - box[0]) * (box[3] - box[2]) - np.pi * (body_radius + min_sep) ** 2
+    h = np.sqrt(area / (n * 1.02))
+    if h <= min_sep:
+        raise ValueError("n too large for the minimum separation")
+    nx = int(np.floor((box[1] - box[0]) / h))
+    ny = int(np.floor((box[3] - box[2]) / h))
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    cx = box[0] + (ix.ravel() + 0.5) * h
+    cy = box[2] + (iy.ravel() + 0.5) * h
+    amp = 0.5 * (h - min_sep)
+    x = cx + rng.uniform(-amp, amp, cx.size)
+    y = cy + rng.uniform(-amp, amp, cy.size)
+    keep = np.hypot(x, y) >= body_radius + min_sep
+    x, y = x[keep], y[keep]
+    if x.size < n:
+        raise ValueError("grid holds fewer than n cells outside the body")
+    pick = rng.permutation(x.size)[:n]
+    x, y = x[pick], y[pick]
+    mag = rng.uniform(0.0, max_ve_g, n)
+    sign = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    return x, y, mag * sign
