@@ -3,12 +3,14 @@
 
 Workload: circular cylinder, radius 0.5, 1000 equal panels, U = (1, 0),
 Re = 1000 (nu = 1e-3), dt = 0.005, blob radius = panel length (pi/1000),
-with a synthetic wake of N = 2^20 + 4096 vortices resident in HBM: one
+with a synthetic wake of N = 2^20 + 16384 vortices resident in HBM: one
 vortex per cell of a square grid over [-0.6, 10] x [-2, 2] minus the disk,
-jittered so that no two lie closer than 0.002 (above the collapse radii), with
-circulations +-U(0, 2 eps) alternating in sign (seed 1).  The wake keeps
-N >= 2^20 through every warm-up and timed step (births replace what leaves the
-box and what collapses); the line reports the smallest and largest N timed.
+jittered so that no two lie closer than 0.003 (above the collapse radii), in
+two shear layers: circulation -U(1e-5/3, 1e-5) above y = 0 and +U(...) below
+(seed 1).  Births replace
+what leaves the box and what collapses, so the count stays near 2^20 through
+the warm-up and timed steps; the line reports the smallest and largest N
+timed and labels the metric N >= 2^20 only when the smallest is.
 
 One step = one full time step of the reference loop: birth from the panel
 system, the 500 probe velocities (velocity_control), Biot-Savart + nearest-3,
@@ -43,7 +45,7 @@ DIFF_FLOP_PER_PAIR = 17
 
 TREE_MIN_N = 65536   # direct sums below 64k vortices, Morton tree above
 ORDER, THETA = 16, 0.5
-N_DEFAULT = (1 << 20) + 4096
+N_DEFAULT = (1 << 20) + 16384
 NPROBE = 500
 
 
@@ -58,9 +60,12 @@ def probe_points():
 
 
 def wake_state(n):
+    """Two shear layers (negative circulation above y = 0, positive below) of
+    U(1e-5 / 3, 1e-5) per vortex, about 5 per layer in all, as a wake of
+    length 10 behind a unit body at U = 1 carries; 0.003 minimum spacing, above
+    the collapse radii.  Births replace what leaves the box at x = 10."""
     from vortex_amd.state import jittered_wake
-    eps = math.pi / 1000
-    return jittered_wake(n, seed=1, max_ve_g=2 * eps, min_sep=0.002)
+    return jittered_wake(n, seed=1, max_ve_g=1e-5, min_sep=0.003, layers=True)
 
 
 def make_case(prec, P=1000, dt=0.005, tree_min_n=TREE_MIN_N, diff_cut=0.0, cap=0):
@@ -165,6 +170,39 @@ def c0_case_params():
     return pan, dict(dt=0.01, nu=1e-3, rho=1.0, vinf=(1.0, 0.0), box=(-2.0, 10.0, -10.0, 10.0),
                      r_col_diff_sign=1.0, r_col_same_sign=0.3, max_ve_g=0.00628162, n_col=2,
                      birth_offset_rel=0.5, tree_min_n=0)
+
+
+def c0_step_rates(steps=500):
+    """Configuration C0 run as specified (500 steps from the empty start,
+    direct FP64 sums, and the same in mixed precision) on the GPU, once as
+    stream steps (one host wait per step, as the reference loop) and once as
+    replays of the captured step graph (one wait at the end).  The difference
+    per step is the launch and host-wait overhead the graph removes."""
+    from vortex_amd.solver import Case, make_params, VX_PREC_F64, VX_PREC_MIXED
+    pan, kw = c0_case_params()
+    out = {}
+    for name, prec in (("f64", VX_PREC_F64), ("mixed", VX_PREC_MIXED)):
+        for mode in ("stream", "graph"):
+            c = Case(make_params(pan, precision=prec, **kw), pan)
+            c.build_matrix()
+            c.reserve(steps * pan.size + 1024)
+            c.sync()
+            t = time.perf_counter()
+            if mode == "stream":
+                for _ in range(steps):
+                    c.time_step((1.0, 0.0))
+            else:
+                c.run_steps_graph(steps, (1.0, 0.0))
+            c.sync()
+            dt = time.perf_counter() - t
+            out[f"{name}_{mode}_steps_per_s"] = steps / dt
+            out[f"{name}_n_end"] = c.n
+            c.close()
+        out[f"{name}_overhead_us_per_step"] = 1e6 * (1.0 / out[f"{name}_stream_steps_per_s"] -
+                                                     1.0 / out[f"{name}_graph_steps_per_s"])
+    log(f"[bench] C0 GPU step rates ({steps} steps): " +
+        ", ".join(f"{k} {v:.4g}" for k, v in out.items()))
+    return out
 
 
 def c0_accuracy_and_cpu(vx_ref, steps=500, samples=(50, 100, 200, 300, 400, 500), window=200):
@@ -335,6 +373,7 @@ def main():
     ap.add_argument("--cpu-targets", type=int, default=8192)
     ap.add_argument("--c0-steps", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c0", action="store_true", help="skip the C0 GPU step-rate runs")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--direct", action="store_true",
                     help="direct O(N^2) sums at every N (the reference's algorithm) instead of the tree")
@@ -402,6 +441,8 @@ def main():
         log(f"[bench] one-step at N={args.n}: Cd mixed {cd_m:.9g} f64 {cd_f:.9g} rel diff {dcd:.3g}; "
             f"Cl diff / Cd {dcl:.3g}")
 
+    c0_rates = None if args.no_c0 else c0_step_rates(args.c0_steps)
+
     cpu = None
     c0 = None
     if not args.no_cpu:
@@ -433,8 +474,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32+fp64",
-        "data": "synthetic (jittered wake, seed 1)",
-        "config": {"workload": "cylinder P=1000, Re=1000, dt=0.005, eps=pi/1000, N=2^20+4096 jittered wake, "
+        "data": "synthetic (jittered two-layer wake, seed 1)",
+        "config": {"workload": "cylinder P=1000, Re=1000, dt=0.005, eps=pi/1000, N=2^20+16384 jittered wake, "
                                "500 probes per step; " +
                                ("direct O(N^2) sums" if args.direct else "tree leaf 64, order 16, theta 0.5"),
                    "n_vortices_min": nmin, "n_vortices_max": nmax, "n_vortices_mean": n_avg,
@@ -454,6 +495,7 @@ def main():
         "delta_cd_rel_one_step_1M": dcd,
         "delta_cl_rel_to_cd_one_step_1M": dcl,
         "accuracy_c0_vs_fp64_oracle": c0,
+        "c0_gpu_step_rates": c0_rates,
         "stage_ms": {k: mixed[k] for k in ("creation_ms", "speed_ms", "tree_build_ms", "bs_kernel_ms", "diff_ms",
                                            "diff_kernel_ms", "wall_ms", "step_ms")},
         "stage_steps": mixed["prof_steps"],  # stage_ms: further steps with stage timing on

@@ -88,6 +88,13 @@ typedef struct vx_params {
 /* Recompute the nearest-3 distance of every target by the exact search
  * instead of only where the tree's near field cannot guarantee it. */
 #define VX_FLAG_EXACT_NEAREST 1
+/* Solve the panel system at every birth by FP32 LU with partial pivoting
+ * (MFMA trailing updates, factors inverted once per case) plus FP64-residual
+ * iterative refinement against the FP64 matrix, instead of multiplying by
+ * the FP64 Gauss-Jordan inverse (src/unit_1.cu:115-254, src/kernel.cu:849-873).
+ * vx_case_build_matrix and vx_case_set_matrix then factor the panel matrix
+ * (vx_case_set_matrix keeps M only for vx_case_get_matrix). */
+#define VX_FLAG_LU_REFINE 2
 
 typedef struct vx_case vx_case;
 
@@ -102,6 +109,10 @@ int vx_case_build_matrix(vx_case *c);                        /* L on the GPU + G
 int vx_case_set_matrix(vx_case *c, const double *M, int64_t m); /* m = P+1, row-major      */
 int vx_case_get_matrix(vx_case *c, double *M, int64_t m);
 int vx_case_get_panel_matrix(vx_case *c, double *L, int64_t m);  /* L before inversion     */
+/* With VX_FLAG_LU_REFINE: the last solve's refinement corrections, whether
+ * it met the test ||r|| <= ||x|| ||L|| eps64 sqrt(P+1), and its
+ * ||r|| / (||L|| ||x||) (infinity norms). */
+int vx_case_solve_info(vx_case *c, int32_t *iters, int32_t *converged, double *resid_rel);
 int vx_case_reserve(vx_case *c, int64_t capacity);
 int vx_case_set_state(vx_case *c, const double *x, const double *y, const double *g, int64_t n);
 /* Copies slots [0, count) (count may exceed n: the tail beyond n is included). */
@@ -121,6 +132,33 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y);
 int vx_step(vx_case *c, double out4[4]);
 int vx_time_step(vx_case *c, double vinf_x, double vinf_y, double forces4[4]);
 int vx_run_steps(vx_case *c, int64_t steps, double vinf_x, double vinf_y, double *forces4_per_step);
+/* vx_run_steps as replays of one HIP graph of the whole time step (birth,
+ * probes, Biot-Savart, diffusion, wall, move, compaction, collapse) on the
+ * direct path: every launch reads the vortex count on the device, the grids
+ * are sized for the capacity (grown once up front for all the steps), and the
+ * host waits once at the end instead of after every step as the reference
+ * loop does (src/main.c:652-785).  The results equal vx_run_steps bit for bit.
+ * The count must stay below tree_min_n for every step (else error 2).  The
+ * graph is kept for the next call with the same capacity and free stream.
+ * counts_per_step (optional) gets the vortex count after each step. */
+int vx_run_steps_graph(vx_case *c, int64_t steps, double vinf_x, double vinf_y, double *forces4_per_step,
+                       int64_t *counts_per_step);
+
+/* ---- batches of independent cases ------------------------------------------
+ * No counterpart in the reference, which runs one case per process (globals
+ * at src/main.c:17-20, __constant__ parameters at src/unit_1.cu:20-44).  A
+ * batch advances k cases of any panel and vortex counts together: each
+ * case's graph step (as vx_run_steps_graph) is one branch of a single graph,
+ * so one graph launch advances every case by one step and their kernels run
+ * side by side.  Each case's results equal its own vx_run_steps bit for bit.
+ * The caller keeps ownership of the cases and must not use them on other
+ * calls while vx_batch_run_steps runs.  vinf holds 2 values per case;
+ * forces4 (4 per step and case) and counts (one per step and case) are
+ * case-major and optional. */
+typedef struct vx_batch vx_batch;
+int vx_batch_create(vx_case *const *cases, int64_t k, vx_batch **out);
+int vx_batch_run_steps(vx_batch *b, int64_t steps, const double *vinf, double *forces4, int64_t *counts);
+int vx_batch_destroy(vx_batch *b);
 int vx_velocity_probe(vx_case *c, const double *px, const double *py, int64_t m,
                       double vinf_x, double vinf_y, double *u, double *v);
 /* Probe points kept on the device (m = 0: none).  vx_time_step then computes
@@ -130,7 +168,6 @@ int vx_velocity_probe(vx_case *c, const double *px, const double *py, int64_t m,
  * last step's m velocities. */
 int vx_case_set_probes(vx_case *c, const double *px, const double *py, int64_t m);
 int vx_case_get_probes(vx_case *c, double *u, double *v, int64_t m);
-
 /* ---- second component (tracer points) ------------------------------------
  * The reference advects two copies of the points of second_points_file each
  * step (src/main.c:729-741): passive ones with the flow velocity
@@ -184,6 +221,29 @@ int vx_step_move(vx_case *c, double out4[4]);
 int vx_compact(vx_case *c);
 int vx_setka(vx_case *c, int kind, int32_t *COL);                           /* COL[n] out */
 int vx_collapse(vx_case *c, int kind, const int32_t *COL);
+
+/* ---- launch-shape tuning --------------------------------------------------
+ * No counterpart in the reference (fixed 32-thread blocks, include/definitions.h:35).
+ *   tpt_bs, tpt_diff  targets per thread of the direct Biot-Savart and
+ *                     diffusion kernels: 1, 2, 4 or 8 (0: default, 4 in mixed
+ *                     precision, 2 in FP64).  Results do not depend on them.
+ *   leaf_off          the tree's bucket and target-group depth: quadtree
+ *                     levels above the cell that holds 64 vortices at the mean
+ *                     density, 0..3 (-1: default 1).  It changes which pairs
+ *                     are summed directly, within the tree's error bound.
+ * vx_case_autotune times every choice on the case's current state (the
+ * direct kernels one by one below tree_min_n, the whole velocity stage
+ * above it), keeps the fastest and reports the default's and the chosen
+ * configuration's time in milliseconds.  It overwrites u, v and d only. */
+typedef struct vx_tuning {
+    int32_t tpt_bs, tpt_diff;
+    int32_t leaf_off;
+    int32_t trials;                 /* out: configurations timed */
+    double ms_default, ms_best;     /* out: timed stage, default and chosen */
+} vx_tuning;
+int vx_case_autotune(vx_case *c, double vinf_x, double vinf_y, vx_tuning *out);
+int vx_case_set_tuning(vx_case *c, const vx_tuning *t);
+int vx_case_get_tuning(vx_case *c, vx_tuning *t);
 
 /* ---- timing of the last stages on the case stream (milliseconds) -------- */
 typedef struct vx_timings {

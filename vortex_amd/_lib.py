@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "libvortex_amd.so")
 
 VX_PREC_F64 = 0
 VX_PREC_MIXED = 1
+VX_FLAG_EXACT_NEAREST = 1
+VX_FLAG_LU_REFINE = 2
 
 EXPORTS = [
     "vx_abi_version", "vx_last_error", "vx_device_count",
@@ -30,6 +32,8 @@ EXPORTS = [
     "vx_case_timings", "vx_case_enable_timing", "vx_tree_biot_savart", "vx_tree_diffusion",
     "vx_tree_stats", "vx_points_move", "vx_points_relax", "vx_tree_order",
     "vx_case_set_probes", "vx_case_get_probes",
+    "vx_case_solve_info", "vx_run_steps_graph", "vx_batch_create", "vx_batch_run_steps", "vx_batch_destroy",
+    "vx_case_autotune", "vx_case_set_tuning", "vx_case_get_tuning",
 ]
 
 
@@ -50,6 +54,12 @@ class TreeInfo(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("m", "leaves", "nodes", "fallback", "bs_pairs", "m2p_evals",
                                           "diff_pairs")] + [("order", C.c_int32), ("heavy", C.c_int32),
                                                             ("m2l", C.c_int64)]
+
+
+class Tuning(C.Structure):
+    """Mirror of vx_tuning (include/vortex.h)."""
+    _fields_ = [("tpt_bs", C.c_int32), ("tpt_diff", C.c_int32), ("leaf_off", C.c_int32), ("trials", C.c_int32),
+                ("ms_default", C.c_double), ("ms_best", C.c_double)]
 
 
 class Timings(C.Structure):
@@ -118,6 +128,14 @@ def lib():
         "vx_tree_order": ([vp, dp, dp, i64], C.c_int),
         "vx_case_set_probes": ([vp, dp, dp, i64], C.c_int),
         "vx_case_get_probes": ([vp, dp, dp, i64], C.c_int),
+        "vx_run_steps_graph": ([vp, i64, d, d, dp, dp], C.c_int),
+        "vx_batch_create": ([vp, i64, C.POINTER(vp)], C.c_int),
+        "vx_batch_run_steps": ([vp, i64, dp, dp, dp], C.c_int),
+        "vx_batch_destroy": ([vp], C.c_int),
+        "vx_case_autotune": ([vp, d, d, C.POINTER(Tuning)], C.c_int),
+        "vx_case_set_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
+        "vx_case_get_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
+        "vx_case_solve_info": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

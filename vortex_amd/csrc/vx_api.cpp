@@ -62,6 +62,7 @@ int vx_device_count(int *count) {
 }
 
 static int case_init(vx_case *c, const vx_params *prm, const vx_panel *panels, int64_t P);
+static void graph_release(vx_case *c);
 
 int vx_case_create(const vx_params *prm, const vx_panel *panels, int64_t P, vx_case **out) {
     VX_ARG(prm && panels && out, "null pointer");
@@ -186,6 +187,9 @@ int vx_case_destroy(vx_case *c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     c->wlist.release(); c->wtmp.release(); c->wstage.release();
     tree_release(c);
+    lu_release(c);
+    graph_release(c);
+    c->gr_ring.release();
     if (c->pan_block) (void)hipFree(c->pan_block);
     if (c->segs) (void)hipFree(c->segs);
     if (c->bpos) (void)hipFree(c->bpos);
@@ -201,9 +205,13 @@ int vx_case_destroy(vx_case *c) {
     return 0;
 }
 
+static bool lu_mode(const vx_case *c) { return (c->prm.flags & VX_FLAG_LU_REFINE) != 0; }
+
 int vx_case_build_matrix(vx_case *c) {
     VX_ARG(c, "case");
-    if (launch_panel_matrix(c) || launch_invert(c)) return 1;
+    if (launch_panel_matrix(c)) return 1;
+    if (lu_mode(c) ? launch_lu_setup(c) : launch_invert(c)) return 1;
+    c->have_inverse = !lu_mode(c);
     c->have_matrix = true;
     return 0;
 }
@@ -214,13 +222,20 @@ int vx_case_set_matrix(vx_case *c, const double *M, int64_t m) {
     if (c->M.ensure((size_t)(m * m), c->stream)) return 1;
     VX_CHECK(hipMemcpyAsync(c->M.p, M, sizeof(double) * m * m, hipMemcpyHostToDevice, c->stream));
     VX_CHECK(hipStreamSynchronize(c->stream));
+    c->have_inverse = true;
+    if (lu_mode(c) && !c->lu && (launch_panel_matrix(c) || launch_lu_setup(c))) return 1;
     c->have_matrix = true;
     return 0;
 }
 
+int vx_case_solve_info(vx_case *c, int32_t *iters, int32_t *converged, double *resid_rel) {
+    VX_ARG(c, "case");
+    return lu_info(c, iters, converged, resid_rel);
+}
+
 int vx_case_get_matrix(vx_case *c, double *M, int64_t m) {
     VX_ARG(c && M, "null pointer");
-    VX_ARG(m == c->P + 1 && c->have_matrix, "no matrix of that size");
+    VX_ARG(m == c->P + 1 && c->have_matrix && c->have_inverse, "no inverse matrix of that size");
     VX_CHECK(hipMemcpyAsync(M, c->M.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->stream));
     VX_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -337,7 +352,7 @@ int vx_birth(vx_case *c, const double *R_p, double d_g, double *gsheet) {
     }
     if (launch_birth(c, d_g)) return 1;
     if (gsheet)
-        VX_CHECK(hipMemcpyAsync(gsheet, c->gsheet.p, sizeof(double) * c->P, hipMemcpyDeviceToHost, c->stream));
+        VX_CHECK(hipMemcpyAsync(gsheet, birth_sheet(c), sizeof(double) * c->P, hipMemcpyDeviceToHost, c->stream));
     VX_CHECK(hipStreamSynchronize(c->stream));
     c->n += c->P;
     return 0;
@@ -485,13 +500,17 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
     int rc = 0;
     {
         Timer td(c, &c->tm.diff_ms, 3);
+        // The wall term first, on the main stream: beside the diffusion
+        // kernel its short selection and one-wave-per-target kernel ran
+        // several times slower than alone.  Its terms are staged and added
+        // after the diffusion, in the same order as before.
+        if (c->timing) (void)hipEventRecord(c->ev[14], c->stream);
+        if (launch_wall_side(c, s, c->stream)) return 1;
+        if (c->timing) (void)hipEventRecord(c->ev[15], c->stream);
         if ((tree && tree_diffusion_pre(c)) || side_fork(c)) return 1;
         // the long main-stream kernel is issued first; the side stream's
         // work is ordered after the fork point only
         rc = tree ? tree_diffusion_main(c) : launch_diffusion(c, s, s);
-        if (!rc && c->timing) (void)hipEventRecord(c->ev[14], c->side);
-        if (!rc) rc = launch_wall_side(c, s, c->side);
-        if (!rc && c->timing) (void)hipEventRecord(c->ev[15], c->side);
         if (!rc && tree) rc = tree_diffusion_far(c, c->side);
         const int rj = side_join(c);
         if (!rc) rc = rj ? rj : launch_wall_add(c);
@@ -547,6 +566,400 @@ int vx_run_steps(vx_case *c, int64_t steps, double vinf_x, double vinf_y, double
     double tmp[4];
     for (int64_t k = 0; k < steps; ++k)
         if (vx_time_step(c, vinf_x, vinf_y, f4 ? f4 + 4 * k : tmp)) return 1;
+    return 0;
+}
+
+// ---- launch-shape tuning ------------------------------------------------------
+// Mean milliseconds of `what` over reps runs after one untimed run, by events
+// on the case stream (work on the side stream is joined into it).
+}  // extern "C"
+
+template <class F>
+static int time_ms(vx_case *c, int reps, F what, double *ms) {
+    if (what()) return 1;
+    hipEvent_t a = nullptr, b = nullptr;
+    VX_CHECK(hipEventCreate(&a));
+    VX_CHECK(hipEventCreate(&b));
+    int rc = hipEventRecord(a, c->stream) != hipSuccess;
+    for (int r = 0; r < reps && !rc; ++r) rc = what();
+    if (!rc) rc = hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess;
+    float t = 0.f;
+    if (!rc) rc = hipEventElapsedTime(&t, a, b) != hipSuccess;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) {
+        if (vx_last_error()[0] == 0) set_error("autotune: timing failed");
+        return 1;
+    }
+    *ms = (double)t / reps;
+    return 0;
+}
+
+extern "C" {
+
+int vx_case_autotune(vx_case *c, double vinf_x, double vinf_y, vx_tuning *out) {
+    VX_ARG(c, "case");
+    VX_ARG(!c->timing && !c->devcount, "autotune runs with stage timing off");
+    const int64_t s = ceil32(c->n);
+    VX_ARG(s > 0, "autotune needs a state");
+    if (reserve(c, s)) return 1;
+    const bool tree = c->prm.tree_min_n > 0 && s >= c->prm.tree_min_n;
+    vx_tuning best = c->tune;
+    best.trials = 0;
+    constexpr int kReps = 3;
+    if (tree) {
+        // the whole velocity stage for each bucket depth
+        double tdef = 0.0, tbest = 1e300;
+        for (int off = 0; off <= 3; ++off) {
+            c->tune.leaf_off = off;
+            double t = 0.0;
+            if (time_ms(c, kReps, [&] { return vx_speed(c, vinf_x, vinf_y); }, &t)) return 1;
+            ++best.trials;
+            if (off == 1) tdef = t;
+            if (t < tbest) {
+                tbest = t;
+                best.leaf_off = off;
+            }
+        }
+        best.ms_default = tdef;
+        best.ms_best = tbest;
+    } else {
+        // each direct kernel for each number of targets per thread
+        const int tpts[4] = {1, 2, 4, 8};
+        const int def = c->prm.precision == VX_PREC_MIXED ? 4 : 2;
+        double bdef = 0.0, bbest = 1e300, ddef = 0.0, dbest = 1e300;
+        for (int q = 0; q < 4; ++q) {
+            c->tune.tpt_bs = tpts[q];
+            double t = 0.0;
+            if (time_ms(c, kReps, [&] { return launch_biot_savart(c, s, s, vinf_x, vinf_y); }, &t)) return 1;
+            ++best.trials;
+            if (tpts[q] == def) bdef = t;
+            if (t < bbest) {
+                bbest = t;
+                best.tpt_bs = tpts[q];
+            }
+        }
+        c->tune.tpt_bs = best.tpt_bs;
+        for (int q = 0; q < 4; ++q) {
+            c->tune.tpt_diff = tpts[q];
+            double t = 0.0;
+            if (time_ms(c, kReps, [&] { return launch_diffusion(c, s, s); }, &t)) return 1;
+            ++best.trials;
+            if (tpts[q] == def) ddef = t;
+            if (t < dbest) {
+                dbest = t;
+                best.tpt_diff = tpts[q];
+            }
+        }
+        best.ms_default = bdef + ddef;
+        best.ms_best = bbest + dbest;
+    }
+    c->tune = best;
+    if (out) *out = best;
+    return 0;
+}
+
+int vx_case_set_tuning(vx_case *c, const vx_tuning *t) {
+    VX_ARG(c && t, "null pointer");
+    auto ok_tpt = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8; };
+    VX_ARG(ok_tpt(t->tpt_bs) && ok_tpt(t->tpt_diff), "targets per thread must be 0, 1, 2, 4 or 8");
+    VX_ARG(t->leaf_off >= -1 && t->leaf_off <= 3, "leaf_off must be -1..3");
+    c->tune.tpt_bs = t->tpt_bs;
+    c->tune.tpt_diff = t->tpt_diff;
+    c->tune.leaf_off = t->leaf_off;
+    return 0;
+}
+
+int vx_case_get_tuning(vx_case *c, vx_tuning *t) {
+    VX_ARG(c && t, "null pointer");
+    *t = c->tune;
+    return 0;
+}
+
+// One time step on device counts (c->devcount): the loop's order of
+// vx_time_step on the direct path, all on the main stream, no host wait,
+// ending with the step's record in gr_ring.  Launch sizes depend only on
+// c->n_ub and the case, so the same launches repeat every step.
+static int device_step(vx_case *c, double vinf_x, double vinf_y) {
+    const int64_t n_ub = c->n_ub;
+    if (launch_right_part(c, cnt_s(c), vinf_x, vinf_y) || launch_birth(c, 0.0) ||
+        launch_step_probes(c, vinf_x, vinf_y))
+        return 1;
+    const Cnt s = cnt_s(c);
+    if (launch_biot_savart(c, s, s, vinf_x, vinf_y) || launch_diffusion(c, s, s) || launch_diffusion_wall(c, s))
+        return 1;
+    if (launch_set_dn(c, n_ub) || launch_step_move(c) || launch_compact(c, n_ub)) return 1;
+    for (int kind = 1; kind <= 2; ++kind)
+        for (int64_t cc = 0; cc < c->prm.n_col; ++cc)
+            if (launch_setka(c, kind, n_ub, true) || launch_collapse(c, kind, n_ub, true) || launch_compact(c, n_ub))
+                return 1;
+    return launch_step_record(c);
+}
+
+static void graph_release(vx_case *c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_cap = -1;
+}
+
+__global__ void k_graph_begin(int64_t *__restrict__ dcount, int64_t n) {
+    dcount[1] = n;
+    dcount[3] = 0;  // collapse error flag, sticky over the steps
+    dcount[9] = 0;  // ring slot
+}
+
+// Prepares c for `steps` device-count steps: capacity for every step, the
+// record ring, the count, error flag and ring slot on the device, and the
+// absorbed circulation in red[0].  Returns with c->devcount set.
+static int devcount_begin(vx_case *c, int64_t steps, bool *ring_moved) {
+    VX_ARG(c->have_matrix, "no panel matrix");
+    VX_ARG(!c->timing, "stage timing synchronises the host; switch it off for graph steps");
+    const int64_t n_max = c->n + steps * c->P;
+    VX_ARG(c->prm.tree_min_n <= 0 || ceil32(n_max) < c->prm.tree_min_n,
+           "graph steps run the direct path: the count must stay below tree_min_n");
+    if (reserve(c, ceil32(n_max + c->P) + 32)) return 1;
+    hipStream_t s = c->stream;
+    if (c->red.ensure(8, s) || c->dcount.ensure(16, s)) return 1;
+    *ring_moved = false;
+    if (steps > c->gr_cap) {
+        if (c->gr_ring.ensure((size_t)(5 * steps), s)) return 1;
+        c->gr_cap = (int64_t)(c->gr_ring.n / 5);
+        *ring_moved = true;  // the ring's address is part of captured launches
+    }
+    c->n_ub = c->cap;
+    VX_CHECK(hipMemcpyAsync(c->red.p, &c->d_g, sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_graph_begin, dim3(1), dim3(1), 0, s, c->dcount.p, c->n);
+    VX_CHECK(hipGetLastError());
+    c->devcount = true;
+    c->dn_slot = 1;
+    return 0;
+}
+
+// Reads the record of `steps` device-count steps: forces and counts per step;
+// c->n and c->d_g follow the last one.
+static int devcount_end(vx_case *c, int64_t steps, double *f4, int64_t *counts) {
+    hipStream_t s = c->stream;
+    std::vector<double> ring((size_t)(5 * c->gr_cap));
+    int64_t flags[16];
+    VX_CHECK(hipMemcpyAsync(ring.data(), c->gr_ring.p, sizeof(double) * 5 * c->gr_cap, hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipMemcpyAsync(flags, c->dcount.p, sizeof(flags), hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipStreamSynchronize(s));
+    if (flags[9] != steps) {
+        set_error("graph steps: " + std::to_string(flags[9]) + " of " + std::to_string(steps) + " recorded");
+        return 1;
+    }
+    if ((int32_t)flags[3]) {
+        set_error("collapse resolution did not converge");
+        return 1;
+    }
+    const int64_t *nr = reinterpret_cast<const int64_t *>(ring.data() + 4 * c->gr_cap);
+    for (int64_t k = 0; k < steps; ++k) {
+        if (f4)
+            for (int q = 0; q < 4; ++q) f4[4 * k + q] = ring[(size_t)(4 * k + q)];
+        if (counts) counts[k] = nr[k];
+    }
+    c->n = nr[steps - 1];
+    c->d_g = ring[(size_t)(4 * (steps - 1))];
+    return 0;
+}
+
+static int instantiate(hipGraph_t g, hipGraphExec_t *exec) {
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        set_error(std::string("graph instantiate: ") + hipGetErrorString(ei));
+        *exec = nullptr;
+        return 1;
+    }
+    return 0;
+}
+
+// steps time steps of the direct path as replays of one captured graph: the
+// first step runs on device counts uncaptured (it sizes every buffer for the
+// capacity bound), the second is captured, and the graph is replayed for the
+// rest.  The host waits once, at the end.
+int vx_run_steps_graph(vx_case *c, int64_t steps, double vinf_x, double vinf_y, double *f4, int64_t *counts) {
+    VX_ARG(c && steps >= 0, "steps");
+    if (steps == 0) return 0;
+    bool moved = false;
+    if (devcount_begin(c, steps, &moved)) {
+        c->devcount = false;
+        return 1;
+    }
+    if (moved || c->graph_cap != c->cap || c->graph_vinf[0] != vinf_x || c->graph_vinf[1] != vinf_y)
+        graph_release(c);
+    hipStream_t s = c->stream;
+    int rc = device_step(c, vinf_x, vinf_y);
+    if (!rc && steps > 1 && !c->graph_exec) {
+        VX_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = device_step(c, vinf_x, vinf_y);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (!rc && e != hipSuccess) {
+            set_error(std::string("graph capture: ") + hipGetErrorString(e));
+            rc = 1;
+        }
+        if (!rc) rc = instantiate(g, &c->graph_exec);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+        } else {
+            c->graph = g;
+            c->graph_cap = c->cap;
+            c->graph_vinf[0] = vinf_x;
+            c->graph_vinf[1] = vinf_y;
+        }
+    }
+    for (int64_t k = 1; !rc && k < steps; ++k) {
+        const hipError_t e = hipGraphLaunch(c->graph_exec, s);
+        if (e != hipSuccess) {
+            set_error(std::string("graph launch: ") + hipGetErrorString(e));
+            rc = 1;
+        }
+    }
+    c->devcount = false;
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    return devcount_end(c, steps, f4, counts);
+}
+
+}  // extern "C"
+
+// ---- batches of cases ---------------------------------------------------------
+// Independent cases advanced together: every case's device-count step is a
+// branch of one graph (captured from a batch stream that forks to the cases'
+// streams and joins them back), so one graph launch advances every case by a
+// step and the GPU runs the cases' kernels side by side.
+struct vx_batch {
+    std::vector<vx_case *> cases;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int64_t> caps;
+    std::vector<double> vinf;
+};
+
+static void batch_graph_release(vx_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+extern "C" {
+
+int vx_batch_create(vx_case *const *cases, int64_t k, vx_batch **out) {
+    VX_ARG(cases && k > 0 && out, "cases");
+    for (int64_t i = 0; i < k; ++i) {
+        VX_ARG(cases[i], "null case");
+        for (int64_t j = 0; j < i; ++j) VX_ARG(cases[i] != cases[j], "a case appears twice");
+    }
+    vx_batch *b = new vx_batch();
+    b->cases.assign(cases, cases + k);
+    b->join.assign((size_t)k, nullptr);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (int64_t i = 0; e == hipSuccess && i < k; ++i) e = hipEventCreateWithFlags(&b->join[(size_t)i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        set_error(std::string("batch: ") + hipGetErrorString(e));
+        vx_batch_destroy(b);
+        return 1;
+    }
+    *out = b;
+    return 0;
+}
+
+int vx_batch_destroy(vx_batch *b) {
+    if (!b) return 0;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_graph_release(b);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+// Every case advances `steps` steps (vinf: 2 per case); forces4 (4 per case
+// and step, case-major) and counts (per case and step) are optional.
+int vx_batch_run_steps(vx_batch *b, int64_t steps, const double *vinf, double *forces4, int64_t *counts) {
+    VX_ARG(b && vinf && steps >= 0, "batch arguments");
+    if (steps == 0) return 0;
+    const size_t k = b->cases.size();
+    bool stale = b->caps.size() != k || b->vinf.size() != 2 * k;
+    int rc = 0;
+    for (size_t i = 0; i < k && !rc; ++i) {
+        vx_case *c = b->cases[i];
+        bool moved = false;
+        rc = devcount_begin(c, steps, &moved);
+        stale = stale || moved || b->caps[i] != c->cap || b->vinf[2 * i] != vinf[2 * i] ||
+                b->vinf[2 * i + 1] != vinf[2 * i + 1];
+    }
+    // the first step of every case on its own stream, uncaptured
+    for (size_t i = 0; i < k && !rc; ++i) rc = device_step(b->cases[i], vinf[2 * i], vinf[2 * i + 1]);
+    if (!rc && steps > 1) {
+        if (stale) batch_graph_release(b);
+        // the batch stream waits for every case's first step
+        for (size_t i = 0; i < k && !rc; ++i)
+            if (hipEventRecord(b->join[i], b->cases[i]->stream) != hipSuccess ||
+                hipStreamWaitEvent(b->stream, b->join[i], 0) != hipSuccess)
+                rc = 1;
+        if (!rc && !b->exec) {
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess ||
+                hipEventRecord(b->fork, b->stream) != hipSuccess) {
+                set_error("batch: capture could not start");
+                rc = 1;
+            }
+            for (size_t i = 0; i < k && !rc; ++i) {
+                vx_case *c = b->cases[i];
+                if (hipStreamWaitEvent(c->stream, b->fork, 0) != hipSuccess) rc = 1;
+                if (!rc) rc = device_step(c, vinf[2 * i], vinf[2 * i + 1]);
+                if (!rc && (hipEventRecord(b->join[i], c->stream) != hipSuccess ||
+                            hipStreamWaitEvent(b->stream, b->join[i], 0) != hipSuccess))
+                    rc = 1;
+            }
+            const hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (!rc && e != hipSuccess) {
+                set_error(std::string("batch capture: ") + hipGetErrorString(e));
+                rc = 1;
+            }
+            if (!rc) rc = instantiate(g, &b->exec);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+            } else {
+                b->graph = g;
+                b->caps.resize(k);
+                b->vinf.assign(vinf, vinf + 2 * k);
+                for (size_t i = 0; i < k; ++i) b->caps[i] = b->cases[i]->cap;
+            }
+        }
+        for (int64_t s = 1; !rc && s < steps; ++s)
+            if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) {
+                set_error("batch: graph launch failed");
+                rc = 1;
+            }
+        // every case stream then waits for the replays
+        if (!rc && hipEventRecord(b->fork, b->stream) != hipSuccess) rc = 1;
+        for (size_t i = 0; i < k && !rc; ++i)
+            if (hipStreamWaitEvent(b->cases[i]->stream, b->fork, 0) != hipSuccess) rc = 1;
+    }
+    for (size_t i = 0; i < k; ++i) b->cases[i]->devcount = false;
+    if (rc) {
+        (void)hipStreamSynchronize(b->stream);
+        for (size_t i = 0; i < k; ++i) (void)hipStreamSynchronize(b->cases[i]->stream);
+        if (vx_last_error()[0] == 0) set_error("batch: stream ordering failed");
+        return 1;
+    }
+    for (size_t i = 0; i < k; ++i)
+        if (devcount_end(b->cases[i], steps, forces4 ? forces4 + 4 * steps * (int64_t)i : nullptr,
+                         counts ? counts + steps * (int64_t)i : nullptr))
+            return 1;
     return 0;
 }
 

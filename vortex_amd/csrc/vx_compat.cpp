@@ -316,6 +316,7 @@ int vort_creation(Vortex *pos, TVctr *V_infDev, size_t n_of_birth, size_t n_of_b
         VX_CHECK(hipMemcpy2DAsync(c->M.p, sizeof(double) * m, M_Dev, sizeof(double) * b, sizeof(double) * m, m,
                                   hipMemcpyDeviceToDevice, c->stream));
         c->have_matrix = true;
+        c->have_inverse = true;
         g_compat.M_src = M_Dev;
     }
     double vinf[2], dg = 0.0;

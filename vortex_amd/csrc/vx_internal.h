@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -36,6 +37,28 @@ void set_error(const std::string &msg);
     } while (0)
 
 struct TreeBufs;
+struct LuBufs;
+
+// A vortex count for a launch.  Host form (dev == nullptr): the value ub.
+// Device form: *dev read by the kernel, rounded up to a multiple of pad and
+// clamped to ub, the host's upper bound that sized the grid; this is what a
+// captured graph uses, since its launches cannot take per-step host values.
+struct Cnt {
+    const int64_t *dev;
+    int64_t ub;
+    int32_t pad;
+    __host__ __device__ __forceinline__ int64_t get() const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (!dev) return ub;
+        const int64_t v = *dev;
+        const int64_t r = pad > 1 ? (v + pad - 1) / pad * pad : v;
+        return r < ub ? r : ub;
+#else
+        return ub;
+#endif
+    }
+};
+inline Cnt host_cnt(int64_t n) { return Cnt{nullptr, n, 1}; }
 
 // Panels on the device as structure-of-arrays (one allocation).
 struct PanelsSoA {
@@ -122,7 +145,8 @@ struct vx_case {
     double4 wall_box{};             // birth-point box grown by 5 max(length)
     // panel system
     vx::DBuf<double> L, M;          // (P+1)^2
-    bool have_matrix = false;
+    bool have_matrix = false;       // the births can be solved (inverse or LU)
+    bool have_inverse = false;      // M holds the inverse
     // vortex state, FP64 SoA
     int64_t n = 0, cap = 0;
     double d_g = 0.0;               // circulation absorbed by the last Step (host copy)
@@ -154,6 +178,23 @@ struct vx_case {
     int64_t prb_m = 0;
     // Morton tree of the last velocity stage
     vx::TreeBufs *tree = nullptr;
+    // FP32 LU of the panel system with FP64 refinement (VX_FLAG_LU_REFINE)
+    vx::LuBufs *lu = nullptr;
+    // Device-count steps (vx_run_steps_graph): every launch takes the vortex
+    // count from dcount[1] (Cnt device form) with grids sized for n_ub, the
+    // absorbed circulation from red[0], and each step ends by recording its
+    // forces and count in gr_ring (slot dcount[9]); the step is captured once
+    // as a graph and replayed.
+    bool devcount = false;
+    int64_t n_ub = 0;
+    // launch shapes (vx_case_autotune / vx_case_set_tuning)
+    vx_tuning tune{0, 0, -1, 0, 0.0, 0.0};
+    vx::DBuf<double> gr_ring;                         // 4 forces + 1 count per step
+    int64_t gr_cap = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_cap = -1;                           // capacity the graph was captured for
+    double graph_vinf[2] = {0.0, 0.0};
     // timing
     bool timing = false;
     hipEvent_t ev[18] = {};
@@ -195,10 +236,26 @@ int side_stream(vx_case *c);  // creates the side stream and its events once
 int side_fork(vx_case *c);  // the side stream waits for the work queued so far on the main stream
 int side_join(vx_case *c);  // the main stream waits for the work queued so far on the side stream
 int launch_pack_f32(vx_case *c, int64_t count);
+int launch_pack_f32(vx_case *c, Cnt count);
+int launch_count_add(vx_case *c, int64_t *a, int64_t v);  // *a += v (stream-ordered)
+int launch_step_record(vx_case *c);  // count back to slot 1; forces and count into gr_ring
+// The counts of the current stage: host values, or in device-count mode the
+// count slot dcount[1] (n, or rounded up to a multiple of 32: s).
+inline Cnt cnt_n(const vx_case *c) {
+    return c->devcount ? Cnt{c->dcount.p + 1, c->n_ub, 1} : host_cnt(c->n);
+}
+inline Cnt cnt_s(const vx_case *c) {
+    const int64_t s_ub = (c->devcount ? c->n_ub : c->n) + 31;
+    return c->devcount ? Cnt{c->dcount.p + 1, std::min(s_ub / 32 * 32, c->cap), 32}
+                       : host_cnt(std::min((c->n + 31) / 32 * 32, c->cap));
+}
 // vx_pairwise.hip
 int launch_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, double vinf_y);
+int launch_biot_savart(vx_case *c, Cnt n_tgt, Cnt n_src, double vinf_x, double vinf_y);
 int launch_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src);
+int launch_diffusion(vx_case *c, Cnt n_tgt, Cnt n_src);
 int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y);  // -> c->rp
+int launch_right_part(vx_case *c, Cnt n_src, double vinf_x, double vinf_y);
 int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vinf_y); // c->px.. -> c->pu,pv
 // The probe kernels on explicit buffers: part holds the split-K partials (at
 // most max_part doubles, max_part <= 0: as many as the split wants, see
@@ -207,6 +264,8 @@ int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vin
 void probe_split(int64_t m, int64_t n_src, int64_t max_part, int64_t *chunk, int64_t *nchunk);
 int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, int64_t n_src, double vinf_x,
                     double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed);
+int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, Cnt n_src, double vinf_x,
+                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed);
 int launch_step_probes(vx_case *c, double vinf_x, double vinf_y);  // c->prb_x.. -> c->prb_u, prb_v
 // vx_wall.hip
 // launch_wall_side queues the wall term on stream st (the main stream, or the
@@ -214,13 +273,23 @@ int launch_step_probes(vx_case *c, double vinf_x, double vinf_y);  // c->prb_x..
 // (main stream, after side_join if st was the side stream) adds the terms to
 // u, v.  launch_diffusion_wall is the two in a row on the main stream.
 int launch_wall_side(vx_case *c, int64_t n_tgt, hipStream_t st);
+int launch_wall_side(vx_case *c, Cnt n_tgt, hipStream_t st);
 int launch_wall_add(vx_case *c);
 int launch_diffusion_wall(vx_case *c, int64_t n_tgt);
+int launch_diffusion_wall(vx_case *c, Cnt n_tgt);
 // vx_panel.hip
 int launch_panel_matrix(vx_case *c);                   // -> c->L
 int launch_invert(vx_case *c);                         // c->L -> c->M (L destroyed copy)
-int launch_birth(vx_case *c, double d_g);              // c->rp -> c->gsheet, newborns at [n, n+P)
+int launch_birth(vx_case *c, double d_g);              // c->rp -> sheet, newborns at [n, n+P)
+const double *birth_sheet(vx_case *c);                 // the last birth's sheet strengths (device)
 int launch_birth_positions(vx_case *c);                // -> c->bpos (once per case)
+// vx_lu.hip: FP32 LU of c->L and its triangular inverses (once per case),
+// then per birth the refined solve of L x = [R; d_g] into lu_solution(c)
+int launch_lu_setup(vx_case *c);
+int launch_lu_solve(vx_case *c, const double *R, double d_g, const double *d_g_dev = nullptr);
+const double *lu_solution(vx_case *c);
+int lu_info(vx_case *c, int32_t *iters, int32_t *converged, double *resid);
+void lu_release(vx_case *c);
 // vx_tree.hip
 int tree_build(vx_case *c, int64_t s);
 int tree_build_early(vx_case *c, int64_t s, int64_t at);  // part A on the side stream (birth overlap)

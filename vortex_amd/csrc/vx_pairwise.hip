@@ -76,11 +76,12 @@ __device__ __forceinline__ void stage_tile(Src4<T> *ls, const T *__restrict__ sx
 // dropped, which equals the three smallest over j != i.
 template <typename T, int TPT>
 __global__ __launch_bounds__(kTile) void k_biot_savart(
-    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, int64_t n_src,
-    int64_t n_tgt, T eps2, double vinf_x, double vinf_y, double *__restrict__ u,
+    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, Cnt cn_src,
+    Cnt cn_tgt, T eps2, double vinf_x, double vinf_y, double *__restrict__ u,
     double *__restrict__ v, double *__restrict__ d) {
     __shared__ Src4<T> ls[kTile];
     using A = Arith<T>;
+    const int64_t n_src = cn_src.get(), n_tgt = cn_tgt.get();
     const int64_t t0 = (int64_t)blockIdx.x * (kTile * TPT) + threadIdx.x;
     T ax[TPT], ay[TPT], m0[TPT], m1[TPT], m2[TPT], m3[TPT], su[TPT], sv[TPT];
     double au[TPT], av[TPT];
@@ -142,10 +143,11 @@ __global__ __launch_bounds__(kTile) void k_biot_savart(
 // I1 += g_j e^{-r/d_i}; if |I1| > 1e-5: V_i += -nu I2 / (I1 d_i).
 template <typename T, int TPT>
 __global__ __launch_bounds__(kTile) void k_diffusion(
-    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, int64_t n_src,
-    int64_t n_tgt, const double *__restrict__ dd, double nu, double thr,
+    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sg, Cnt cn_src,
+    Cnt cn_tgt, const double *__restrict__ dd, double nu, double thr,
     double *__restrict__ u, double *__restrict__ v) {
     __shared__ Src4<T> ls[kTile];
+    const int64_t n_src = cn_src.get(), n_tgt = cn_tgt.get();
     const int64_t t0 = (int64_t)blockIdx.x * (kTile * TPT) + threadIdx.x;
     T ax[TPT], ay[TPT], kd[TPT], s2x[TPT], s2y[TPT], s1[TPT];
     double a2x[TPT], a2y[TPT], a1[TPT];
@@ -305,9 +307,14 @@ __global__ __launch_bounds__(kSegTile) void k_right_part_partial(
     const int64_t lo = c * kRhsChunk, hi = min(n_src, lo + kRhsChunk);
     const bool live = (int)threadIdx.x < S.count;
     const int64_t i = S.first + (live ? threadIdx.x : 0);
-    const T l0 = (T)pan.lx[i], l1 = (T)pan.ly[i];
-    const T r0 = (T)pan.rx[i], r1 = (T)pan.ry[i];
-    const T d0 = r0 - l0, d1 = r1 - l1;
+    // FP32: panel ends and sources relative to the segment centre, the panel
+    // vector differenced in FP64 (absolute FP32 coordinates would leave the
+    // short panels' vectors with relative errors of ulp(|z|) / length)
+    constexpr bool F32 = sizeof(T) == 4;
+    const double ox = F32 ? S.cx : 0.0, oy = F32 ? S.cy : 0.0;
+    const T l0 = (T)(pan.lx[i] - ox), l1 = (T)(pan.ly[i] - oy);
+    const T r0 = (T)(pan.rx[i] - ox), r1 = (T)(pan.ry[i] - oy);
+    const T d0 = F32 ? (T)(pan.rx[i] - pan.lx[i]) : r0 - l0, d1 = F32 ? (T)(pan.ry[i] - pan.ly[i]) : r1 - l1;
     const T dd2 = d0 * d0 + d1 * d1;
     const double rn2 = 4.0 * S.r * S.r;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -321,8 +328,8 @@ __global__ __launch_bounds__(kSegTile) void k_right_part_partial(
             const int32_t k = list[j];
             const double ex = x[k] - S.cx, ey = y[k] - S.cy;
             near = ex * ex + ey * ey < rn2;
-            q.x = sx[k];
-            q.y = sy[k];
+            q.x = F32 ? (T)ex : sx[k];
+            q.y = F32 ? (T)ey : sy[k];
             q.g = sg[k];
             q.pad = T(0);
         }
@@ -372,9 +379,10 @@ __global__ __launch_bounds__(kSegTile) void k_right_part_partial(
 // Far sources: per work-group partial S_k (FP64), k < kLoc, grid-stride over
 // the sources; spart[(blk * kLoc + k) * 2 + {0,1}].
 __global__ __launch_bounds__(256) void k_rhs_far_partial(const double *__restrict__ x, const double *__restrict__ y,
-                                                         const double *__restrict__ g, int64_t n_src, double cx,
+                                                         const double *__restrict__ g, Cnt cn_src, double cx,
                                                          double cy, double r2, double *__restrict__ spart) {
     __shared__ double red[16][256];
+    const int64_t n_src = cn_src.get();
     double sr[kLoc], si[kLoc];
 #pragma unroll
     for (int k = 0; k < kLoc; ++k) sr[k] = si[k] = 0.0;
@@ -603,11 +611,12 @@ __device__ __forceinline__ void stage_tile_cvt(Src4<T> *ls, const S *__restrict_
 
 template <typename T, typename S = T>
 __global__ __launch_bounds__(kTile) void k_probe_partial(
-    const S *__restrict__ sx, const S *__restrict__ sy, const S *__restrict__ sg, int64_t n_src,
+    const S *__restrict__ sx, const S *__restrict__ sy, const S *__restrict__ sg, Cnt cn_src,
     int64_t chunk, const double *__restrict__ px, const double *__restrict__ py, int64_t m, T eps2,
     double *__restrict__ part) {
     __shared__ Src4<T> ls[kTile];
     using A = Arith<T>;
+    const int64_t n_src = cn_src.get();
     const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
     const int64_t c = blockIdx.y;
     const int64_t lo = c * chunk, hi = min(n_src, lo + chunk);
@@ -670,47 +679,95 @@ static void split(int64_t n_src, int64_t tgt_blocks, int64_t *chunk, int64_t *nc
 }
 
 int launch_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, double vinf_y) {
-    if (n_tgt <= 0) return 0;
+    return launch_biot_savart(c, host_cnt(n_tgt), host_cnt(n_src), vinf_x, vinf_y);
+}
+
+// Targets per thread of the direct kernels: the case's tuning (see
+// vx_case_autotune) or the default, 4 in FP32 and 2 in FP64.  Every choice
+// sums each target's sources in the same order, so the results do not depend
+// on it.
+static int tpt_of(int want, bool f32) {
+    if (want == 1 || want == 2 || want == 4 || want == 8) return want;
+    return f32 ? 4 : 2;
+}
+
+template <typename T, int TPT>
+static void bs_launch(vx_case *c, const T *sx, const T *sy, const T *sg, Cnt n_src, Cnt n_tgt, T eps2, double vinf_x,
+                      double vinf_y) {
+    const unsigned nb = (unsigned)((n_tgt.ub + kTile * TPT - 1) / (kTile * TPT));
+    hipLaunchKernelGGL((k_biot_savart<T, TPT>), dim3(nb), dim3(kTile), 0, c->stream, sx, sy, sg, n_src, n_tgt, eps2,
+                       vinf_x, vinf_y, c->u.p, c->v.p, c->d.p);
+}
+
+template <typename T>
+static void bs_dispatch(vx_case *c, int tpt, const T *sx, const T *sy, const T *sg, Cnt n_src, Cnt n_tgt, T eps2,
+                        double vinf_x, double vinf_y) {
+    switch (tpt) {
+        case 1: bs_launch<T, 1>(c, sx, sy, sg, n_src, n_tgt, eps2, vinf_x, vinf_y); break;
+        case 2: bs_launch<T, 2>(c, sx, sy, sg, n_src, n_tgt, eps2, vinf_x, vinf_y); break;
+        case 4: bs_launch<T, 4>(c, sx, sy, sg, n_src, n_tgt, eps2, vinf_x, vinf_y); break;
+        default: bs_launch<T, 8>(c, sx, sy, sg, n_src, n_tgt, eps2, vinf_x, vinf_y); break;
+    }
+}
+
+int launch_biot_savart(vx_case *c, Cnt n_tgt, Cnt n_src, double vinf_x, double vinf_y) {
+    if (n_tgt.ub <= 0) return 0;
     const double eps2 = c->prm.ve_size * c->prm.ve_size;
     if (mixed(c)) {
-        constexpr int TPT = 4;
         if (launch_pack_f32(c, n_src)) return 1;
-        const unsigned nb = (unsigned)((n_tgt + kTile * TPT - 1) / (kTile * TPT));
-        hipLaunchKernelGGL((k_biot_savart<float, TPT>), dim3(nb), dim3(kTile), 0, c->stream, c->xf.p,
-                           c->yf.p, c->gf.p, n_src, n_tgt, (float)eps2, vinf_x, vinf_y, c->u.p, c->v.p,
-                           c->d.p);
+        bs_dispatch<float>(c, tpt_of(c->tune.tpt_bs, true), c->xf.p, c->yf.p, c->gf.p, n_src, n_tgt, (float)eps2,
+                           vinf_x, vinf_y);
     } else {
-        constexpr int TPT = 2;
-        const unsigned nb = (unsigned)((n_tgt + kTile * TPT - 1) / (kTile * TPT));
-        hipLaunchKernelGGL((k_biot_savart<double, TPT>), dim3(nb), dim3(kTile), 0, c->stream, c->x.p,
-                           c->y.p, c->g.p, n_src, n_tgt, eps2, vinf_x, vinf_y, c->u.p, c->v.p, c->d.p);
+        bs_dispatch<double>(c, tpt_of(c->tune.tpt_bs, false), c->x.p, c->y.p, c->g.p, n_src, n_tgt, eps2, vinf_x,
+                            vinf_y);
     }
     VX_CHECK(hipGetLastError());
     return 0;
 }
 
 int launch_diffusion(vx_case *c, int64_t n_tgt, int64_t n_src) {
-    if (n_tgt <= 0) return 0;
+    return launch_diffusion(c, host_cnt(n_tgt), host_cnt(n_src));
+}
+
+template <typename T, int TPT>
+static void diff_launch(vx_case *c, const T *sx, const T *sy, const T *sg, Cnt n_src, Cnt n_tgt, double thr) {
+    const unsigned nb = (unsigned)((n_tgt.ub + kTile * TPT - 1) / (kTile * TPT));
+    hipLaunchKernelGGL((k_diffusion<T, TPT>), dim3(nb), dim3(kTile), 0, c->stream, sx, sy, sg, n_src, n_tgt, c->d.p,
+                       c->prm.nu, thr, c->u.p, c->v.p);
+}
+
+template <typename T>
+static void diff_dispatch(vx_case *c, int tpt, const T *sx, const T *sy, const T *sg, Cnt n_src, Cnt n_tgt,
+                          double thr) {
+    switch (tpt) {
+        case 1: diff_launch<T, 1>(c, sx, sy, sg, n_src, n_tgt, thr); break;
+        case 2: diff_launch<T, 2>(c, sx, sy, sg, n_src, n_tgt, thr); break;
+        case 4: diff_launch<T, 4>(c, sx, sy, sg, n_src, n_tgt, thr); break;
+        default: diff_launch<T, 8>(c, sx, sy, sg, n_src, n_tgt, thr); break;
+    }
+}
+
+int launch_diffusion(vx_case *c, Cnt n_tgt, Cnt n_src) {
+    if (n_tgt.ub <= 0) return 0;
     const double thr = 0.001 * c->prm.ve_size;
     if (mixed(c)) {
-        constexpr int TPT = 4;
         if (launch_pack_f32(c, n_src)) return 1;
-        const unsigned nb = (unsigned)((n_tgt + kTile * TPT - 1) / (kTile * TPT));
-        hipLaunchKernelGGL((k_diffusion<float, TPT>), dim3(nb), dim3(kTile), 0, c->stream, c->xf.p,
-                           c->yf.p, c->gf.p, n_src, n_tgt, c->d.p, c->prm.nu, thr, c->u.p, c->v.p);
+        diff_dispatch<float>(c, tpt_of(c->tune.tpt_diff, true), c->xf.p, c->yf.p, c->gf.p, n_src, n_tgt, thr);
     } else {
-        constexpr int TPT = 2;
-        const unsigned nb = (unsigned)((n_tgt + kTile * TPT - 1) / (kTile * TPT));
-        hipLaunchKernelGGL((k_diffusion<double, TPT>), dim3(nb), dim3(kTile), 0, c->stream, c->x.p,
-                           c->y.p, c->g.p, n_src, n_tgt, c->d.p, c->prm.nu, thr, c->u.p, c->v.p);
+        diff_dispatch<double>(c, tpt_of(c->tune.tpt_diff, false), c->x.p, c->y.p, c->g.p, n_src, n_tgt, thr);
     }
     VX_CHECK(hipGetLastError());
     return 0;
 }
 
 int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
+    return launch_right_part(c, host_cnt(n_src), vinf_x, vinf_y);
+}
+
+int launch_right_part(vx_case *c, Cnt cn_src, double vinf_x, double vinf_y) {
     const int64_t P = c->P, K = c->nseg;
     hipStream_t st = c->stream;
+    const int64_t n_src = cn_src.ub;  // bound: grids, list and partial sizes
     const int64_t nchunk = n_src > 0 ? (n_src + kRhsChunk - 1) / kRhsChunk : 1;
     constexpr int64_t kFarBlocks = 512;
     const size_t ns_far = (size_t)(2 * kLoc * kFarBlocks), ns_glob = 2 * kLoc,
@@ -729,11 +786,12 @@ int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
     {
         const RhsNear pred{c->x.p, c->y.p, c->g.p, cx, cy, rnear2};
         if (c->ptmp.ensure((size_t)prim::tiles_of(n_src) + 1, st)) return 1;
-        VX_CHECK(prim::select_flags(prim::pred_flag(pred), n_src, prim::WriteIndex{c->lists.p}, c->ptmp.p, nlist, st));
+        VX_CHECK(prim::select_flags(prim::pred_flag(pred, cn_src), n_src, prim::WriteIndex{c->lists.p}, c->ptmp.p,
+                                    nlist, st));
     }
     // pairs within 2 R_s of each segment
     if (mixed(c)) {
-        if (launch_pack_f32(c, n_src)) return 1;
+        if (launch_pack_f32(c, cn_src)) return 1;
         hipLaunchKernelGGL(k_right_part_partial<float>, dim3((unsigned)K, (unsigned)nchunk), dim3(kSegTile), 0, st,
                            c->xf.p, c->yf.p, c->gf.p, c->x.p, c->y.p, c->lists.p, nlist, c->pan, c->segs,
                            c->partial.p);
@@ -744,7 +802,7 @@ int launch_right_part(vx_case *c, int64_t n_src, double vinf_x, double vinf_y) {
     }
     // the global expansion (sources beyond 2 R_b), the segments' own
     // sources, and each segment's total about c_s
-    hipLaunchKernelGGL(k_rhs_far_partial, dim3((unsigned)kFarBlocks), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, n_src,
+    hipLaunchKernelGGL(k_rhs_far_partial, dim3((unsigned)kFarBlocks), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, cn_src,
                        cx, cy, rnear2, c->rhs_s.p);
     hipLaunchKernelGGL(k_rhs_far_final, dim3(2 * kLoc), dim3(256), 0, st, c->rhs_s.p, kFarBlocks, S);
     hipLaunchKernelGGL(k_rhs_seg_partial, dim3((unsigned)K, (unsigned)kSegBlocks), dim3(256), 0, st, c->x.p, c->y.p,
@@ -773,22 +831,28 @@ void probe_split(int64_t m, int64_t n_src, int64_t max_part, int64_t *chunk, int
 
 int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, int64_t n_src, double vinf_x,
                     double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed) {
+    return launch_probe_to(c, px, py, m, host_cnt(n_src), vinf_x, vinf_y, pu, pv, part, max_part, packed);
+}
+
+int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, Cnt cn_src, double vinf_x,
+                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed) {
     if (m <= 0) return 0;
     const int64_t tb = (m + kTile - 1) / kTile;
+    const int64_t n_src = cn_src.ub;  // the chunking follows the bound
     int64_t chunk, nchunk;
     probe_split(m, n_src, max_part, &chunk, &nchunk);
     const double eps2 = c->prm.ve_size * c->prm.ve_size;
     const dim3 grid((unsigned)tb, (unsigned)nchunk);
     if (mixed(c) && packed) {
-        if (launch_pack_f32(c, n_src)) return 1;
+        if (launch_pack_f32(c, cn_src)) return 1;
         hipLaunchKernelGGL((k_probe_partial<float, float>), grid, dim3(kTile), 0, c->stream, c->xf.p, c->yf.p,
-                           c->gf.p, n_src, chunk, px, py, m, (float)eps2, part);
+                           c->gf.p, cn_src, chunk, px, py, m, (float)eps2, part);
     } else if (mixed(c)) {
         hipLaunchKernelGGL((k_probe_partial<float, double>), grid, dim3(kTile), 0, c->stream, c->x.p, c->y.p,
-                           c->g.p, n_src, chunk, px, py, m, (float)eps2, part);
+                           c->g.p, cn_src, chunk, px, py, m, (float)eps2, part);
     } else {
         hipLaunchKernelGGL((k_probe_partial<double, double>), grid, dim3(kTile), 0, c->stream, c->x.p, c->y.p,
-                           c->g.p, n_src, chunk, px, py, m, eps2, part);
+                           c->g.p, cn_src, chunk, px, py, m, eps2, part);
     }
     VX_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_probe_final, dim3((unsigned)m), dim3(kTile), 0, c->stream, part, nchunk, m, vinf_x, vinf_y,
@@ -811,12 +875,12 @@ int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vin
 int launch_step_probes(vx_case *c, double vinf_x, double vinf_y) {
     const int64_t m = c->prb_m;
     if (m <= 0) return 0;
-    const int64_t n_src = std::min<int64_t>((c->n + 31) / 32 * 32, c->cap);
+    const Cnt s = cnt_s(c);
     int64_t chunk, nchunk;
-    probe_split(m, n_src, 0, &chunk, &nchunk);
+    probe_split(m, s.ub, 0, &chunk, &nchunk);
     if (c->prb_part.ensure((size_t)(2 * nchunk * m), c->stream)) return 1;
-    return launch_probe_to(c, c->prb_x.p, c->prb_y.p, m, n_src, vinf_x, vinf_y, c->prb_u.p, c->prb_v.p,
-                           c->prb_part.p, 0, false);
+    return launch_probe_to(c, c->prb_x.p, c->prb_y.p, m, s, vinf_x, vinf_y, c->prb_u.p, c->prb_v.p, c->prb_part.p,
+                           0, false);
 }
 
 }  // namespace vx

@@ -212,7 +212,9 @@ int launch_invert(vx_case *c) {
 // coalesced over the row, xor-shuffle reduction), then newborn i at the panel
 // midpoint + offset * normal with g = (gsheet_i l_i + gsheet_next l_next)/2.
 __global__ __launch_bounds__(256) void k_birth_gemv(const double *__restrict__ M, const double *__restrict__ R,
-                                                   double d_g, int64_t P, double *__restrict__ gs) {
+                                                   double d_g_host, const double *__restrict__ d_g_dev, int64_t P,
+                                                   double *__restrict__ gs) {
+    const double d_g = d_g_dev ? *d_g_dev : d_g_host;
     const int lane = threadIdx.x & 63;
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= P) return;
@@ -241,24 +243,38 @@ int launch_birth_positions(vx_case *c) {
     return 0;
 }
 
-__global__ void k_birth_place(const double *__restrict__ gs, PanelsSoA pan, const double *__restrict__ bp, int64_t n,
-                              double *__restrict__ x, double *__restrict__ y, double *__restrict__ g) {
+__global__ void k_birth_place(const double *__restrict__ gs, PanelsSoA pan, const double *__restrict__ bp, Cnt cn,
+                              int64_t cap, double *__restrict__ x, double *__restrict__ y, double *__restrict__ g) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= pan.P) return;
+    const int64_t n = cn.get();
+    if (i >= pan.P || n + i >= cap) return;
     const int64_t nx = pan.rp[i];
     x[n + i] = bp[i];
     y[n + i] = bp[pan.P + i];
     g[n + i] = 0.5 * (gs[i] * pan.len[i] + gs[nx] * pan.len[nx]);
 }
 
+// The sheet strengths of the last birth: the inverse's product, or with the
+// FP32 LU the first P entries of the refined solution.
+const double *birth_sheet(vx_case *c) { return c->lu ? lu_solution(c) : c->gsheet.p; }
+
+// In device-count mode the absorbed circulation comes from red[0] (the
+// previous step's sum) and the newborns go to the device count's slots,
+// which then grows by P.
 int launch_birth(vx_case *c, double d_g) {
     const int64_t P = c->P;
-    if (c->gsheet.ensure((size_t)P, c->stream)) return 1;
-    hipLaunchKernelGGL(k_birth_gemv, dim3((unsigned)((P + 3) / 4)), dim3(256), 0, c->stream, c->M.p, c->rp.p, d_g,
-                       P, c->gsheet.p);
-    hipLaunchKernelGGL(k_birth_place, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream, c->gsheet.p,
-                       c->pan, c->bpos, c->n, c->x.p, c->y.p, c->g.p);
+    const double *dgp = c->devcount ? c->red.p : nullptr;
+    if (c->lu) {
+        if (launch_lu_solve(c, c->rp.p, d_g, dgp)) return 1;
+    } else {
+        if (c->gsheet.ensure((size_t)P, c->stream)) return 1;
+        hipLaunchKernelGGL(k_birth_gemv, dim3((unsigned)((P + 3) / 4)), dim3(256), 0, c->stream, c->M.p, c->rp.p,
+                           d_g, dgp, P, c->gsheet.p);
+    }
+    hipLaunchKernelGGL(k_birth_place, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream, birth_sheet(c),
+                       c->pan, c->bpos, cnt_n(c), c->cap, c->x.p, c->y.p, c->g.p);
     VX_CHECK(hipGetLastError());
+    if (c->devcount && launch_count_add(c, c->dcount.p + 1, P)) return 1;
     return 0;
 }
 

@@ -241,6 +241,14 @@ struct PredFlag {  // a boolean predicate of an int32 index as a flag
 };
 template <class P>
 inline PredFlag<P> pred_flag(P p) { return PredFlag<P>{p}; }
+template <class P, class N>
+struct PredFlagN {  // the predicate below a count read on the device (N::get())
+    P p;
+    N n;
+    __device__ int operator()(int64_t i) const { return (i < n.get() && p((int32_t)i)) ? 1 : 0; }
+};
+template <class P, class N>
+inline PredFlagN<P, N> pred_flag(P p, N n) { return PredFlagN<P, N>{p, n}; }
 
 struct ArrayFlag {  // a[i] != 0
     const int32_t *a;
@@ -263,62 +271,67 @@ struct WriteIndex {  // out[pos] = i for flagged items (ordered selection of ind
 };
 
 // ---- LSD radix sort: 32-bit keys, 32-bit values, 8-bit digits ---------------------------
-// Four stable passes.  Per pass: each tile's digit histogram (LDS atomics),
-// the digit-major exclusive offsets of (digit, tile) in one work-group, and
-// the scatter, where each key's rank within its tile is its wave's count of
-// earlier lanes with the same digit (8 ballots) plus the counts of the
-// tile's earlier waves and rounds.  Stable, so equal keys keep their input
-// order.
+// Four stable passes of two launches each.  The histogram launch counts each
+// tile's digits in LDS, stores them digit-major and adds them to the pass's
+// digit totals.  The scatter launch starts each digit of its tile at the
+// digit's base (an exclusive scan of the totals, redone by every work-group)
+// plus the counts of the earlier tiles (a sum over its row), and ranks each
+// key within the tile by its wave's count of earlier lanes with the same digit
+// (8 ballots) plus the counts of the tile's earlier waves and rounds.  Stable,
+// so equal keys keep their input order.  No work-group waits on another.
 constexpr int kRadixBits = 8;
 constexpr int kBins = 1 << kRadixBits;
+constexpr int kSortIPT = 8;                 // keys per thread per tile
+constexpr int kSortTile = kT * kSortIPT;
+
+inline int64_t sort_tiles_of(int64_t n) { return n > 0 ? (n + kSortTile - 1) / kSortTile : 0; }
+// int32 entries of scratch radix_sort_pairs needs for n keys
+inline int64_t radix_tmp_size(int64_t n) { return (int64_t)kBins * (sort_tiles_of(n) + 4); }
 
 template <int DUMMY = 0>
 __global__ __launch_bounds__(kT) void k_radix_hist(const uint32_t *__restrict__ key, int64_t n, int shift,
-                                                   int32_t *__restrict__ hist, int64_t ntiles) {
+                                                   int32_t *__restrict__ hist, int64_t ntiles,
+                                                   int32_t *__restrict__ totals) {
     __shared__ int32_t h[kBins];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kTile;
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
 #pragma unroll 4
-    for (int r = 0; r < kIPT; ++r) {
+    for (int r = 0; r < kSortIPT; ++r) {
         const int64_t i = base + r * kT + threadIdx.x;
         if (i < n) atomicAdd(&h[(key[i] >> shift) & (kBins - 1)], 1);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// One work-group of kBins threads: thread d scans digit d's row over the
-// tiles, then the digit totals are scanned across the work-group.
-template <int DUMMY = 0>
-__global__ __launch_bounds__(kBins) void k_radix_offsets(int32_t *__restrict__ hist, int64_t ntiles) {
-    __shared__ int32_t ws[kBins / 64];
-    int32_t *row = hist + (int64_t)threadIdx.x * ntiles;
-    int32_t s = 0;
-    for (int64_t t = 0; t < ntiles; ++t) s += row[t];
-    const int32_t inc = wave_inclusive(s, OpSum());
-    if (lane_id() == 63) ws[threadIdx.x >> 6] = inc;
-    __syncthreads();
-    int32_t pre = inc - s;
-    for (int q = 0; q < (int)(threadIdx.x >> 6); ++q) pre += ws[q];
-    for (int64_t t = 0; t < ntiles; ++t) {
-        const int32_t v = row[t];
-        row[t] = pre;
-        pre += v;
-    }
+    const int32_t c = h[threadIdx.x];
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = c;
+    if (c) atomicAdd(totals + threadIdx.x, c);
 }
 
 template <int DUMMY = 0>
 __global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
                                                       int64_t n, int shift, const int32_t *__restrict__ hist,
-                                                      int64_t ntiles, uint32_t *__restrict__ kout,
-                                                      int32_t *__restrict__ vout) {
+                                                      int64_t ntiles, const int32_t *__restrict__ totals,
+                                                      uint32_t *__restrict__ kout, int32_t *__restrict__ vout) {
     __shared__ int32_t run[kBins];    // offset of the next key of each digit
     __shared__ int32_t wcnt[kW][kBins];
+    __shared__ int32_t ws[kW];
     const int w = threadIdx.x >> 6;
-    run[threadIdx.x] = hist[(int64_t)threadIdx.x * ntiles + blockIdx.x];
-    const int64_t base = (int64_t)blockIdx.x * kTile;
-    for (int r = 0; r < kIPT; ++r) {
+    {
+        // digit base: exclusive scan of the totals over the work-group
+        const int32_t tot = totals[threadIdx.x];
+        const int32_t inc = wave_inclusive(tot, OpSum());
+        if (lane_id() == 63) ws[w] = inc;
+        // this tile's start within the digit: the earlier tiles' counts
+        const int32_t *row = hist + (int64_t)threadIdx.x * ntiles;
+        int32_t before = 0;
+        for (int64_t t = 0; t < (int64_t)blockIdx.x; ++t) before += row[t];
+        __syncthreads();
+        int32_t pre = inc - tot;
+        for (int q = 0; q < w; ++q) pre += ws[q];
+        run[threadIdx.x] = pre + before;
+    }
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortIPT; ++r) {
         for (int q = 0; q < kW; ++q) wcnt[q][threadIdx.x] = 0;
         const int64_t i = base + r * kT + threadIdx.x;
         const bool ok = i < n;
@@ -355,20 +368,23 @@ __global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict
 }
 
 // Sorts (k0, v0) of n items; k1, v1 are the same size.  The result ends in
-// k0, v0 (four passes).  hist must hold kBins * tiles_of(n) int32 entries.
-inline hipError_t radix_sort_pairs(uint32_t *k0, uint32_t *k1, int32_t *v0, int32_t *v1, int64_t n,
-                                   int32_t *hist, hipStream_t st) {
-    const int64_t nt = tiles_of(n);
+// k0, v0 (four passes).  tmp must hold radix_tmp_size(n) int32 entries.
+inline hipError_t radix_sort_pairs(uint32_t *k0, uint32_t *k1, int32_t *v0, int32_t *v1, int64_t n, int32_t *tmp,
+                                   hipStream_t st) {
+    const int64_t nt = sort_tiles_of(n);
     if (nt == 0) return hipSuccess;
+    int32_t *hist = tmp, *totals = tmp + kBins * nt;  // 4 passes x kBins totals
+    hipError_t e = fill(totals, 4 * kBins, 0, st);
+    if (e != hipSuccess) return e;
     uint32_t *ka = k0, *kb = k1;
     int32_t *va = v0, *vb = v1;
     for (int pass = 0; pass < 32 / kRadixBits; ++pass) {
         const int shift = pass * kRadixBits;
+        int32_t *tot = totals + pass * kBins;
         hipLaunchKernelGGL(k_radix_hist<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka, n, shift, hist,
-                           nt);
-        hipLaunchKernelGGL(k_radix_offsets<0>, dim3(1), dim3(kBins), 0, st, hist, nt);
+                           nt, tot);
         hipLaunchKernelGGL(k_radix_scatter<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka,
-                           (const int32_t *)va, n, shift, (const int32_t *)hist, nt, kb, vb);
+                           (const int32_t *)va, n, shift, (const int32_t *)hist, nt, (const int32_t *)tot, kb, vb);
         uint32_t *tk = ka; ka = kb; kb = tk;
         int32_t *tv = va; va = vb; vb = tv;
     }

@@ -61,22 +61,52 @@ int launch_fill_tail(vx_case *c, int64_t from, int64_t to) {
 
 __global__ void k_pack_f32(const double *__restrict__ x, const double *__restrict__ y,
                            const double *__restrict__ g, float *__restrict__ xf,
-                           float *__restrict__ yf, float *__restrict__ gf, int64_t n) {
+                           float *__restrict__ yf, float *__restrict__ gf, Cnt n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n.get()) return;
     xf[i] = (float)x[i];
     yf[i] = (float)y[i];
     gf[i] = (float)g[i];
 }
 
-int launch_pack_f32(vx_case *c, int64_t count) {
-    if (count <= 0) return 0;
+int launch_pack_f32(vx_case *c, Cnt count) {
+    if (count.ub <= 0) return 0;
     if (c->xf.ensure((size_t)c->cap, c->stream) || c->yf.ensure((size_t)c->cap, c->stream) ||
         c->gf.ensure((size_t)c->cap, c->stream))
         return 1;
-    hipLaunchKernelGGL(k_pack_f32, dim3((unsigned)((count + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_pack_f32, dim3((unsigned)((count.ub + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        c->stream, c->x.p, c->y.p, c->g.p, c->xf.p, c->yf.p, c->gf.p, count);
     VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_pack_f32(vx_case *c, int64_t count) { return launch_pack_f32(c, host_cnt(count)); }
+
+// Device-count helpers for captured steps: *a += v; the step's count back to
+// slot 1 and the step's forces and count into the ring (slot k = st[0]++).
+__global__ void k_count_add(int64_t *__restrict__ a, int64_t v) { *a += v; }
+
+__global__ void k_step_record(int64_t *__restrict__ dcount, int slot, const double *__restrict__ red,
+                              double *__restrict__ ring, int64_t *__restrict__ nring, int64_t *__restrict__ k) {
+    const int64_t n = dcount[slot];
+    dcount[1] = n;
+    const int64_t s = *k;
+    for (int q = 0; q < 4; ++q) ring[4 * s + q] = red[q];
+    nring[s] = n;
+    *k = s + 1;
+}
+
+int launch_count_add(vx_case *c, int64_t *a, int64_t v) {
+    hipLaunchKernelGGL(k_count_add, dim3(1), dim3(1), 0, c->stream, a, v);
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_step_record(vx_case *c) {
+    hipLaunchKernelGGL(k_step_record, dim3(1), dim3(1), 0, c->stream, c->dcount.p, c->dn_slot, c->red.p,
+                       c->gr_ring.p, (int64_t *)(c->gr_ring.p + 4 * c->gr_cap), c->dcount.p + 9);
+    VX_CHECK(hipGetLastError());
+    c->dn_slot = 1;
     return 0;
 }
 

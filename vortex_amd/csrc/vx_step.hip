@@ -111,10 +111,11 @@ __global__ __launch_bounds__(256) void k_hit_panels(const double *__restrict__ x
 // grid covers).
 __global__ __launch_bounds__(256) void k_step_move(double *__restrict__ x, double *__restrict__ y,
                                                    double *__restrict__ g, const double *__restrict__ u,
-                                                   const double *__restrict__ v, int64_t n, int64_t P,
+                                                   const double *__restrict__ v, Cnt cn, int64_t P,
                                                    PanelsSoA pan, StepConsts k, const int32_t *__restrict__ hitp,
                                                    double *__restrict__ part) {
     __shared__ double red[8][256];
+    const int64_t n = cn.get();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double dg = 0.0, fx = 0.0, fy = 0.0, mm = 0.0;
     double bx0 = 1e300, bx1 = 1e300, by0 = 1e300, by1 = 1e300;
@@ -217,16 +218,19 @@ __global__ void k_step_init(int64_t *__restrict__ dcount, int64_t n) {
     dcount[3] = 0;
 }
 
+// In device-count mode the count is already in dcount[1] and the error flag
+// stays set over the steps of the sequence (cleared when it starts).
 int launch_set_dn(vx_case *c, int64_t n) {
     if (c->dcount.ensure(16, c->stream)) return 1;
-    hipLaunchKernelGGL(k_step_init, dim3(1), dim3(1), 0, c->stream, c->dcount.p, n);
+    if (!c->devcount) hipLaunchKernelGGL(k_step_init, dim3(1), dim3(1), 0, c->stream, c->dcount.p, n);
     VX_CHECK(hipGetLastError());
     c->dn_slot = 1;
     return 0;
 }
 
 int launch_step_move(vx_case *c) {
-    const int64_t n = c->n;
+    const Cnt cn = cnt_n(c);
+    const int64_t n = cn.ub;  // bound: grids and list sizes
     const int64_t nb = n > 0 ? (n + 255) / 256 : 1;
     hipStream_t st = c->stream;
     if (c->partial.ensure((size_t)(8 * nb), st) || c->red.ensure(8, st) ||
@@ -240,11 +244,11 @@ int launch_step_move(vx_case *c) {
     if (n > 0) {
         const HitCandidate pred{c->x.p, c->y.p, c->g.p, c->u.p, c->v.p, k};
         if (c->ptmp.ensure((size_t)prim::tiles_of(n) + 1, st)) return 1;
-        VX_CHECK(prim::select_flags(prim::pred_flag(pred), n, prim::WriteIndex{cand}, c->ptmp.p, ncand, st));
+        VX_CHECK(prim::select_flags(prim::pred_flag(pred, cn), n, prim::WriteIndex{cand}, c->ptmp.p, ncand, st));
         hipLaunchKernelGGL(k_hit_panels, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->u.p, c->v.p, c->pan, k,
                            cand, ncand, hitp);
     }
-    hipLaunchKernelGGL(k_step_move, dim3((unsigned)nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->u.p, c->v.p, n,
+    hipLaunchKernelGGL(k_step_move, dim3((unsigned)nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->u.p, c->v.p, cn,
                        c->P, c->pan, k, hitp, c->partial.p);
     hipLaunchKernelGGL(k_sum4, dim3(1), dim3(256), 0, st, c->partial.p, nb, p.rho, p.dt, c->red.p);
     VX_CHECK(hipGetLastError());

@@ -2124,7 +2124,7 @@ static int tree_build_a(vx_case *c, int64_t s, hipStream_t st, Births B) {
                        t.part.p + nbs, t.keys.p, t.idx.p, B);
     // stable LSD radix sort of (key, index); the sorted pairs end in keys and
     // idx, which then take the names keys_alt and perm
-    if (t.tmp.ensure(sizeof(int32_t) * (size_t)(prim::kBins * prim::tiles_of(s) + 1), st)) return 1;
+    if (t.tmp.ensure(sizeof(int32_t) * (size_t)prim::radix_tmp_size(s), st)) return 1;
     VX_CHECK(prim::radix_sort_pairs(t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p, s, (int32_t *)t.tmp.p, st));
     std::swap(t.keys, t.keys_alt);
     std::swap(t.idx, t.perm);
@@ -2153,7 +2153,7 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
     // cell holding kLeaf particles at the mean density.  Keys normalise each
     // axis of the bounding box separately, so a level-L cell holds m / 4^L
     // particles on average whatever the box's aspect ratio.
-    const double lv = floor(0.5 * log2((double)m / kLeaf)) - 1.0;
+    const double lv = floor(0.5 * log2((double)m / kLeaf)) - (double)(c->tune.leaf_off >= 0 ? c->tune.leaf_off : 1);
     const int dmin = 2 * (int)fmin(fmax(lv, 0.0), 15.0);
     {
         hipLaunchKernelGGL(k_group_cells, dim3(gm), dim3(256), 0, st, t.keys_alt.p, m, 32 - dmin, t.gcs.p);

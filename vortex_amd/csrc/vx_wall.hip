@@ -42,20 +42,26 @@ __device__ __forceinline__ void quad_i0_i3(T ra0, T ra1, T rb0, T rb1, T n0, T n
 // One target per wave: the lanes take panels lane, lane + 64, ...  The first panel in
 // index order with r <= 0.1 l decides the result alone (the sequential loop
 // overwrites II0 and II3 there and stops); without one, the mid-range terms
-// are summed, lane partials in FP64 and then across the wave.
+// are summed, lane partials in FP64 and then across the wave.  FP32 works in
+// coordinates relative to the target (differences taken in FP64), so that a
+// target a fraction of a short panel away keeps FP32's relative precision.
 template <typename T>
-__device__ void wave_wall(const PanelsSoA &pan, int lane, T a0, T a1, T dd, int64_t npts, double &II0,
+__device__ void wave_wall(const PanelsSoA &pan, int lane, double x, double y, T dd, int64_t npts, double &II0,
                           double &II30, double &II31) {
+    constexpr bool F32 = sizeof(T) == 4;
+    const double ox = F32 ? x : 0.0, oy = F32 ? y : 0.0;
+    const T a0 = (T)(x - ox), a1 = (T)(y - oy);
     double c0 = 0.0, c30 = 0.0, c31 = 0.0;
     int64_t fb = pan.P;
     for (int64_t f = lane; f < pan.P; f += 64) {
         const T dL = (T)pan.len[f];
-        const T q0 = a0 - (T)pan.bx[f], q1 = a1 - (T)pan.by[f];
+        const T q0 = F32 ? (T)(x - pan.bx[f]) : a0 - (T)pan.bx[f];
+        const T q1 = F32 ? (T)(y - pan.by[f]) : a1 - (T)pan.by[f];
         const T r = sqrt(q0 * q0 + q1 * q1);
         if ((r < T(5) * dL) && (r > T(0.1) * dL)) {
             T R0, R30, R31;
-            quad_i0_i3<T>((T)pan.lx[f], (T)pan.ly[f], (T)pan.rx[f], (T)pan.ry[f], -(T)pan.nx[f], -(T)pan.ny[f], a0,
-                          a1, dL, dd, npts, R0, R30, R31);
+            quad_i0_i3<T>((T)(pan.lx[f] - ox), (T)(pan.ly[f] - oy), (T)(pan.rx[f] - ox), (T)(pan.ry[f] - oy),
+                          -(T)pan.nx[f], -(T)pan.ny[f], a0, a1, dL, dd, npts, R0, R30, R31);
             c0 += (double)((-dd) * R0);
             c30 += (double)R30;
             c31 += (double)R31;
@@ -111,7 +117,7 @@ __global__ __launch_bounds__(256) void k_diffusion_wall(const double *__restrict
     for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total; t += nw) {
         const int64_t i = list[t];
         double II0, II30, II31;
-        wave_wall<T>(pan, lane, (T)x[i], (T)y[i], (T)d[i], npts, II0, II30, II31);
+        wave_wall<T>(pan, lane, x[i], y[i], (T)d[i], npts, II0, II30, II31);
         if (lane == 0) {
             wst[2 * t] = nu * II30 / II0;
             wst[2 * t + 1] = nu * II31 / II0;
@@ -131,6 +137,11 @@ __global__ __launch_bounds__(256) void k_wall_add(const int32_t *__restrict__ li
 }
 
 int launch_wall_side(vx_case *c, int64_t n_tgt, hipStream_t st) {
+    return launch_wall_side(c, host_cnt(n_tgt), st);
+}
+
+int launch_wall_side(vx_case *c, Cnt cn_tgt, hipStream_t st) {
+    const int64_t n_tgt = cn_tgt.ub;  // bound: list sizes and the selection's grid
     if (n_tgt <= 0 || c->P <= 0) return 0;
     // buffers grow on the main stream: a reallocation synchronises it, and
     // with it every earlier side-stream pass (joined into it)
@@ -140,7 +151,8 @@ int launch_wall_side(vx_case *c, int64_t n_tgt, hipStream_t st) {
     int32_t *cnt = (int32_t *)(c->dcount.p + 5);
     const InWallBox pred{c->x.p, c->y.p, c->wall_box};
     if (c->wtmp.ensure((size_t)prim::tiles_of(n_tgt) + 1, c->stream)) return 1;
-    VX_CHECK(prim::select_flags(prim::pred_flag(pred), n_tgt, prim::WriteIndex{c->wlist.p}, c->wtmp.p, cnt, st));
+    VX_CHECK(prim::select_flags(prim::pred_flag(pred, cn_tgt), n_tgt, prim::WriteIndex{c->wlist.p}, c->wtmp.p, cnt,
+                                st));
     if (c->prm.precision == VX_PREC_MIXED)
         hipLaunchKernelGGL(k_diffusion_wall<float>, dim3(1024), dim3(256), 0, st, c->x.p, c->y.p, c->d.p,
                            c->wlist.p, cnt, c->pan, c->prm.nu, c->prm.n_of_points, c->wstage.p);
@@ -164,5 +176,7 @@ int launch_wall_add(vx_case *c) {
 int launch_diffusion_wall(vx_case *c, int64_t n_tgt) {
     return launch_wall_side(c, n_tgt, c->stream) || launch_wall_add(c);
 }
+
+int launch_diffusion_wall(vx_case *c, Cnt n_tgt) { return launch_wall_side(c, n_tgt, c->stream) || launch_wall_add(c); }
 
 }  // namespace vx

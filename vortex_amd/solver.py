@@ -13,9 +13,10 @@ import ctypes as C
 import numpy as np
 
 from . import geometry as geo
-from ._lib import VX_PREC_F64, VX_PREC_MIXED, Params, Timings, check, lib, ptr
+from ._lib import (VX_FLAG_EXACT_NEAREST, VX_FLAG_LU_REFINE, VX_PREC_F64, VX_PREC_MIXED, Params, Timings, Tuning,
+                   check, lib, ptr)
 
-__all__ = ["Case", "make_params", "VX_PREC_F64", "VX_PREC_MIXED"]
+__all__ = ["Case", "Batch", "make_params", "VX_PREC_F64", "VX_PREC_MIXED", "VX_FLAG_EXACT_NEAREST", "VX_FLAG_LU_REFINE"]
 
 
 def make_params(panels: np.ndarray, *, dt=0.01, ve_size=None, nu=1e-3, rho=1.0, vinf=(1.0, 0.0),
@@ -169,6 +170,30 @@ class Case:
         check(lib().vx_run_steps(self._h, int(k), vinf[0], vinf[1], ptr(out)))
         return out[:4 * k].reshape(k, 4)
 
+    def autotune(self, vinf=(1.0, 0.0)) -> dict:
+        """Time the launch-shape choices on the current state and keep the
+        fastest (vx_case_autotune)."""
+        t = Tuning()
+        check(lib().vx_case_autotune(self._h, float(vinf[0]), float(vinf[1]), C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def set_tuning(self, tpt_bs=0, tpt_diff=0, leaf_off=-1):
+        t = Tuning(tpt_bs=tpt_bs, tpt_diff=tpt_diff, leaf_off=leaf_off)
+        check(lib().vx_case_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(lib().vx_case_get_tuning(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def run_steps_graph(self, k, vinf):
+        """k time steps as replays of one captured HIP graph (direct path);
+        returns the forces (k x 4) and the count after each step."""
+        f4 = np.zeros((k, 4))
+        counts = np.zeros(k, dtype=np.int64)
+        check(lib().vx_run_steps_graph(self._h, int(k), float(vinf[0]), float(vinf[1]), ptr(f4), ptr(counts)))
+        return f4, counts
+
     def velocity_probe(self, px, py, vinf):
         px, py = _f64(px), _f64(py)
         u, v = np.zeros(px.size), np.zeros(px.size)
@@ -212,6 +237,13 @@ class Case:
         t = TreeInfo()
         check(lib().vx_tree_stats(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in TreeInfo._fields_}
+
+    def solve_info(self) -> dict:
+        """Refinement corrections, convergence and relative residual of the
+        last panel-system solve (VX_FLAG_LU_REFINE cases)."""
+        it, ok, r = C.c_int32(), C.c_int32(), C.c_double()
+        check(lib().vx_case_solve_info(self._h, C.byref(it), C.byref(ok), C.byref(r)))
+        return {"iters": it.value, "converged": bool(ok.value), "resid_rel": r.value}
 
     def set_probes(self, px, py):
         """Probe points kept on the device; time_step computes their velocities."""
@@ -259,3 +291,31 @@ class Case:
         t = Timings()
         check(lib().vx_case_timings(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in Timings._fields_}
+
+
+class Batch:
+    """Independent cases advanced together by one graph launch per step
+    (vx_batch_*).  The cases stay owned by the caller."""
+
+    def __init__(self, cases):
+        self.cases = list(cases)
+        arr = (C.c_void_p * len(self.cases))(*[c._h.value for c in self.cases])
+        h = C.c_void_p()
+        check(lib().vx_batch_create(arr, len(self.cases), C.byref(h)))
+        self._h = h
+
+    def run_steps(self, k, vinf):
+        """vinf: one (vx, vy) per case; returns forces (cases x k x 4) and counts (cases x k)."""
+        v = np.ascontiguousarray(np.asarray(vinf, dtype=np.float64).reshape(len(self.cases), 2))
+        f4 = np.zeros((len(self.cases), k, 4))
+        counts = np.zeros((len(self.cases), k), dtype=np.int64)
+        check(lib().vx_batch_run_steps(self._h, int(k), ptr(v), ptr(f4), ptr(counts)))
+        return f4, counts
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -55,13 +55,17 @@ def uniform_wake(n: int, seed: int = 1, max_ve_g: float = 0.00628162,
 
 
 def jittered_wake(n: int, seed: int = 1, max_ve_g: float = 0.00628162, min_sep: float = 0.002,
-                  box=(-0.6, 10.0, -2.0, 2.0), body_radius: float = 0.5):
+                  box=(-0.6, 10.0, -2.0, 2.0), body_radius: float = 0.5, layers: bool = False):
     """A wake of exactly n vortices that holds its size: one vortex per cell
     of a square grid over ``box`` minus the disk r < body_radius + min_sep,
     jittered inside its cell so that any two stay at least ``min_sep`` apart
     (above the collapse radii, so the first steps do not merge thousands of
-    pairs as a uniform random cloud does).  Circulations +-U(0, max_ve_g),
-    alternating in sign over a random order of the cells."""
+    pairs as a uniform random cloud does).  Circulations: U(0, max_ve_g) in
+    magnitude, alternating in sign over a random order of the cells; with
+    ``layers`` instead U(max_ve_g / 3, max_ve_g), negative above y = 0 and
+    positive below, as in the two shear layers behind a body in a flow along
+    x (random signs make the weights of the diffusive velocity cancel, and the
+    ratio of its sums then throws vortices out of the box)."""
     rng = np.random.default_rng(seed)
     area = (box[1] - box[0]) * (box[3] - box[2]) - np.pi * (body_radius + min_sep) ** 2
     h = np.sqrt(area / (n * 1.02))
@@ -81,6 +85,9 @@ def jittered_wake(n: int, seed: int = 1, max_ve_g: float = 0.00628162, min_sep: 
         raise ValueError("grid holds fewer than n cells outside the body")
     pick = rng.permutation(x.size)[:n]
     x, y = x[pick], y[pick]
+    if layers:
+        mag = rng.uniform(max_ve_g / 3.0, max_ve_g, n)
+        return x, y, np.where(y > 0.0, -mag, mag)
     mag = rng.uniform(0.0, max_ve_g, n)
     sign = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     return x, y, mag * sign
