@@ -168,6 +168,17 @@ int vx_velocity_probe(vx_case *c, const double *px, const double *py, int64_t m,
  * last step's m velocities. */
 int vx_case_set_probes(vx_case *c, const double *px, const double *py, int64_t m);
 int vx_case_get_probes(vx_case *c, double *u, double *v, int64_t m);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): vx_velocity_probe on caller-owned DEVICE arrays px, py
+ * (inputs) and u, v (outputs) of m doubles, queued on the case stream with no
+ * host wait (vx_case_sync waits).  Its split-K partials live in a file-scope
+ * device array of 2^21 doubles (16 MiB per GPU, held until the process ends),
+ * so m may be at most 2^20 (larger m: error 2) and no call allocates.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU race. */
+int vx_velocity_probe_noalloc(vx_case *c, const double *px, const double *py, int64_t m,
+                              double vinf_x, double vinf_y, double *u, double *v);
+
 /* ---- second component (tracer points) ------------------------------------
  * The reference advects two copies of the points of second_points_file each
  * step (src/main.c:729-741): passive ones with the flow velocity

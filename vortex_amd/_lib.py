@@ -31,7 +31,7 @@ EXPORTS = [
     "vx_diffusion_wall", "vx_step_move", "vx_compact", "vx_setka", "vx_collapse",
     "vx_case_timings", "vx_case_enable_timing", "vx_tree_biot_savart", "vx_tree_diffusion",
     "vx_tree_stats", "vx_points_move", "vx_points_relax", "vx_tree_order",
-    "vx_case_set_probes", "vx_case_get_probes",
+    "vx_case_set_probes", "vx_case_get_probes", "vx_velocity_probe_noalloc",
     "vx_case_solve_info", "vx_run_steps_graph", "vx_batch_create", "vx_batch_run_steps", "vx_batch_destroy",
     "vx_case_autotune", "vx_case_set_tuning", "vx_case_get_tuning",
 ]
@@ -128,6 +128,7 @@ def lib():
         "vx_tree_order": ([vp, dp, dp, i64], C.c_int),
         "vx_case_set_probes": ([vp, dp, dp, i64], C.c_int),
         "vx_case_get_probes": ([vp, dp, dp, i64], C.c_int),
+        "vx_velocity_probe_noalloc": ([vp, vp, vp, i64, d, d, vp, vp], C.c_int),
         "vx_run_steps_graph": ([vp, i64, d, d, dp, dp], C.c_int),
         "vx_batch_create": ([vp, i64, C.POINTER(vp)], C.c_int),
         "vx_batch_run_steps": ([vp, i64, dp, dp, dp], C.c_int),

@@ -1051,4 +1051,12 @@ int vx_velocity_probe(vx_case *c, const double *px, const double *py, int64_t m,
     return 0;
 }
 
+// vx_velocity_probe on caller-owned device buffers, allocating nothing:
+// queued on the case stream, no host wait.
+int vx_velocity_probe_noalloc(vx_case *c, const double *px, const double *py, int64_t m, double vinf_x,
+                              double vinf_y, double *u, double *v) {
+    VX_ARG(c && m >= 0 && (m == 0 || (px && py && u && v)), "null pointer");
+    return launch_probe_noalloc(c, px, py, m, vinf_x, vinf_y, u, v);
+}
+
 }  // extern "C"

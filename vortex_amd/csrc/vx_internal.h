@@ -267,6 +267,11 @@ int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, i
 int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, Cnt n_src, double vinf_x,
                     double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed);
 int launch_step_probes(vx_case *c, double vinf_x, double vinf_y);  // c->prb_x.. -> c->prb_u, prb_v
+// The probes on caller-owned device buffers with the partials in the
+// library's file-scope device scratch (no allocation; calls that overlap on
+// one GPU race for it).
+int launch_probe_noalloc(vx_case *c, const double *px, const double *py, int64_t m, double vinf_x, double vinf_y,
+                         double *pu, double *pv);
 // vx_wall.hip
 // launch_wall_side queues the wall term on stream st (the main stream, or the
 // side stream after side_fork), into per-target staging; launch_wall_add

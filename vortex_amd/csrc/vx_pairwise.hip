@@ -883,4 +883,50 @@ int launch_step_probes(vx_case *c, double vinf_x, double vinf_y) {
                            0, false);
 }
 
+// ---- velocity probes without allocation ---------------------------------------
+// vx_velocity_probe_noalloc keeps its split-K partials in this file-scope
+// device array (one copy per GPU, held until the process ends) instead of
+// the case's growable buffer.  Host code never uses the array's name as an
+// address: its device address comes from hipGetSymbolAddress, once per
+// device, and is kept in scratch_ptr.
+constexpr int64_t kProbeScratch = int64_t(1) << 21;  // doubles (16 MiB)
+static __device__ __align__(256) double scratch[kProbeScratch];
+
+static std::mutex scratch_mu;
+static double *scratch_ptr[256] = {};
+
+static int probe_scratch(int dev, double **out) {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (dev < 0 || dev >= 256) {
+        set_error("bad argument: device index");
+        return 2;
+    }
+    if (!scratch_ptr[dev]) {
+        int cur = -1;
+        VX_CHECK(hipGetDevice(&cur));
+        if (cur != dev) VX_CHECK(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        VX_CHECK(e);
+        scratch_ptr[dev] = (double *)p;
+    }
+    *out = scratch_ptr[dev];
+    return 0;
+}
+
+int launch_probe_noalloc(vx_case *c, const double *px, const double *py, int64_t m, double vinf_x, double vinf_y,
+                         double *pu, double *pv) {
+    if (m <= 0) return 0;
+    if (2 * m > kProbeScratch) {
+        set_error("bad argument: vx_velocity_probe_noalloc takes at most " + std::to_string(kProbeScratch / 2) +
+                  " probes");
+        return 2;
+    }
+    double *part = nullptr;
+    if (const int rc = probe_scratch(c->device, &part)) return rc;
+    const int64_t n_src = std::min<int64_t>((c->n + 31) / 32 * 32, c->cap);
+    return launch_probe_to(c, px, py, m, n_src, vinf_x, vinf_y, pu, pv, part, kProbeScratch, false);
+}
+
 }  // namespace vx
