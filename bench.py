@@ -131,7 +131,15 @@ def run_path(prec, state, steps, warmup, M=None, tree_min_n=TREE_MIN_N, calibrat
     c.sync()
     t1 = c.timings()
     c.enable_timing(False)
-    tree = c.tree_stats() if tree_min_n else None
+    tree = None
+    if tree_min_n:
+        # one more step with the diffusion's census on: the pairs it evaluates
+        # that lie within their target's cut-off radius
+        c.tree_census(True)
+        c.time_step(vinf)
+        c.sync()
+        tree = c.tree_stats()
+        c.tree_census(False)
     per = lambda k: (t1[k] - t0[k]) / max(1, prof_steps)
     res = {
         "tree": tree, "bs_kernel_ms": per("bs_kernel_ms"), "tree_build_ms": per("tree_build_ms"),
@@ -202,6 +210,67 @@ def c0_step_rates(steps=500):
                                                      1.0 / out[f"{name}_graph_steps_per_s"])
     log(f"[bench] C0 GPU step rates ({steps} steps): " +
         ", ".join(f"{k} {v:.4g}" for k, v in out.items()))
+    return out
+
+
+def c3_batch_rates(ncases=64, steps=20, seed=3):
+    """Configuration C3's batch: flat plates with panel counts drawn from
+    {200, 1000, 2040, 3320}, angle of attack uniform in [-10, 20] degrees, Re
+    log-uniform in [500, 5000], dt = 0.005, mixed precision with the FP32 LU
+    and FP64 refinement, each case from rest for `steps` steps (direct sums:
+    N <= steps P).  Timed twice on fresh cases: advanced together (vx_batch:
+    one graph launch for the steps of all cases) and one case after another
+    with stream steps.  Reported as case-steps per second."""
+    from vortex_amd import geometry as geo
+    from vortex_amd.solver import Batch, Case, make_params, VX_PREC_MIXED, VX_FLAG_LU_REFINE
+    rng = np.random.default_rng(seed)
+    spec = []
+    for _ in range(ncases):
+        P = int(rng.choice([200, 1000, 2040, 3320]))
+        alpha = float(rng.uniform(-10.0, 20.0))
+        re = float(np.exp(rng.uniform(np.log(500.0), np.log(5000.0))))
+        spec.append((geo.flat_plate(P, chord=1.0, thickness=0.02, alpha_deg=alpha), 1.0 / re))
+
+    def make():
+        cs = []
+        for pan, nu in spec:
+            c = Case(make_params(pan, dt=0.005, nu=nu, r_col_diff_sign=1.0, r_col_same_sign=0.3,
+                                 precision=VX_PREC_MIXED, flags=VX_FLAG_LU_REFINE), pan)
+            c.build_matrix()
+            c.reserve(steps * pan.size + 1024)
+            cs.append(c)
+        for c in cs:
+            c.sync()
+        return cs
+
+    out = {"cases": ncases, "steps": steps, "panels": [int(pan.size) for pan, _ in spec]}
+    cs = make()
+    t = time.perf_counter()
+    for c in cs:
+        c.run_steps(steps, (1.0, 0.0))
+    for c in cs:
+        c.sync()
+    t_one = time.perf_counter() - t
+    n_one = [c.n for c in cs]
+    for c in cs:
+        c.close()
+    cs = make()
+    b = Batch(cs)
+    t = time.perf_counter()
+    b.run_steps(steps, [(1.0, 0.0)] * ncases)
+    t_batch = time.perf_counter() - t
+    n_batch = [c.n for c in cs]
+    b.close()
+    for c in cs:
+        c.close()
+    out.update({"one_by_one_case_steps_per_s": ncases * steps / t_one,
+                "batch_case_steps_per_s": ncases * steps / t_batch,
+                "batch_over_one_by_one": t_one / t_batch,
+                "n_end_min": min(n_batch), "n_end_max": max(n_batch),
+                "same_counts": n_one == n_batch})
+    log(f"[bench] C3 batch ({ncases} cases x {steps} steps, N per case up to {max(n_batch)}): batch "
+        f"{out['batch_case_steps_per_s']:.4g} case-steps/s, one by one {out['one_by_one_case_steps_per_s']:.4g}, "
+        f"x{out['batch_over_one_by_one']:.2f}")
     return out
 
 
@@ -375,6 +444,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c0", action="store_true", help="skip the C0 GPU step-rate runs")
     ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the C3 batch runs")
+    ap.add_argument("--c3-cases", type=int, default=64)
+    ap.add_argument("--c3-steps", type=int, default=20)
     ap.add_argument("--direct", action="store_true",
                     help="direct O(N^2) sums at every N (the reference's algorithm) instead of the tree")
     ap.add_argument("--calibrate", action="store_true",
@@ -404,6 +476,9 @@ def main():
         f"creation {mixed['creation_ms']:.2f} ms, step {mixed['step_ms']:.2f} ms")
     tr = mixed["tree"] or {}
     log(f"[bench] tree: {tr}")
+    diff_waste = (tr["diff_pairs"] / tr["diff_in_radius"]) if tr.get("diff_in_radius", -1) > 0 else None
+    if diff_waste:
+        log(f"[bench] diffusion: evaluated / in-radius pairs {diff_waste:.3f}")
     # dominant kernel of the step and its algorithmic FLOPs per launch
     # mixed far field: one M2L per (bucket, far node), 4p^2 + 24p FLOP (the
     # p x p binomial product, powers and scaling), and one L2P per target,
@@ -442,6 +517,7 @@ def main():
             f"Cl diff / Cd {dcl:.3g}")
 
     c0_rates = None if args.no_c0 else c0_step_rates(args.c0_steps)
+    c3 = None if args.no_c3 else c3_batch_rates(args.c3_cases, args.c3_steps)
 
     cpu = None
     c0 = None
@@ -486,7 +562,8 @@ def main():
         "evaluated": ({"bs_pairs": s_bs * s_bs, "diff_pairs": s_bs * s_bs} if args.direct else
                       {"bs_pairs": tr.get("bs_pairs"), "far_interactions": tr.get("m2p_evals"),
                        "m2l_translations": tr.get("m2l"),
-                       "diff_pairs": tr.get("diff_pairs"), "nearest3_fallbacks": tr.get("fallback")}),
+                       "diff_pairs": tr.get("diff_pairs"), "diff_pairs_in_radius": tr.get("diff_in_radius"),
+                       "diff_evaluated_over_in_radius": diff_waste, "nearest3_fallbacks": tr.get("fallback")}),
         "diff_cut": {"mixed": mixed["diff_cut"], "f64": f64["diff_cut"] if f64 else None},
         "f64_steps_per_s": (args.f64_steps / f64["wall"]) if f64 else None,
         "mixed_over_f64": (sps / (args.f64_steps / f64["wall"])) if f64 else None,
@@ -496,6 +573,7 @@ def main():
         "delta_cl_rel_to_cd_one_step_1M": dcl,
         "accuracy_c0_vs_fp64_oracle": c0,
         "c0_gpu_step_rates": c0_rates,
+        "c3_batch": c3,
         "stage_ms": {k: mixed[k] for k in ("creation_ms", "speed_ms", "tree_build_ms", "bs_kernel_ms", "diff_ms",
                                            "diff_kernel_ms", "wall_ms", "step_ms")},
         "stage_steps": mixed["prof_steps"],  # stage_ms: further steps with stage timing on

@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define VX_ABI_VERSION 1
+#define VX_ABI_VERSION 2
 
 /* Byte-identical to the reference tPanel (include/definitions.h:92-113). */
 typedef struct vx_panel {
@@ -81,7 +81,7 @@ typedef struct vx_params {
     int32_t tree_min_n;                         /* Morton-tree velocities at N >= this (0: never) */
     double  theta;                              /* tree opening angle (0: 0.5)   */
     double  diff_cut;                           /* diffusion cut-off / d_i (0: 18 mixed, 30 FP64) */
-    int32_t order;                              /* multipole terms, 16 or 32 (0: 16) */
+    int32_t order;                              /* multipole terms, 16, 20 or 32 (0: 16) */
     int32_t flags;                              /* VX_FLAG_*                     */
 } vx_params;
 
@@ -121,10 +121,25 @@ int vx_case_get_velocity(vx_case *c, double *u, double *v, double *d, int64_t co
 int vx_case_set_velocity(vx_case *c, const double *u, const double *v, const double *d, int64_t count);
 int64_t vx_case_n(const vx_case *c);
 int64_t vx_case_capacity(const vx_case *c);
+/* Sets the count only (raw): the slots from n on keep their contents, which
+ * the stages read up to ceil32(n + P); vx_case_set_state refills them. */
 int vx_case_set_n(vx_case *c, int64_t n);
 double  vx_case_d_g(const vx_case *c);
 int vx_case_set_d_g(vx_case *c, double d_g);
 int vx_case_sync(vx_case *c);
+/* Exact checkpoint (no reference counterpart: the reference restarts from a
+ * Kadr text snapshot printed with %lf, src/main.c:190-203,558-605, which
+ * rounds the state).  vx_case_save writes the case's n, capacity, d_g, every
+ * slot of x, y, g (the tail beyond n takes part in the nearest-3 search), the
+ * tree's domain-scale target hint and the caller's blob `user` (user_bytes,
+ * may be 0) to `path` in native byte order.  vx_case_load restores them
+ * into a case created with the same panels and parameters (P, precision,
+ * dt, ve_size and nu are checked: error 2), so that its following steps
+ * equal those of the saved case bit for bit; the blob goes to `user`
+ * (capacity user_cap bytes, error 2 if smaller) and its size to
+ * *user_bytes.  Velocities are not saved: every step recomputes them. */
+int vx_case_save(vx_case *c, const char *path, const void *user, int64_t user_bytes);
+int vx_case_load(vx_case *c, const char *path, void *user, int64_t user_cap, int64_t *user_bytes);
 
 /* ---- per-step stages (device resident) ---------------------------------- */
 int vx_vort_creation(vx_case *c, double vinf_x, double vinf_y);
@@ -219,8 +234,16 @@ typedef struct vx_tree_info {
     int64_t m2l;                     /* far nodes translated into bucket local
                                         expansions (FP32 path; the FP64 path
                                         evaluates m2p_evals directly), last BS */
+    int64_t diff_in_radius;          /* of the target-group kernel's diff_pairs,
+                                        those within the target's cut-off radius
+                                        (census on), else -1                  */
 } vx_tree_info;
 int vx_tree_stats(vx_case *c, vx_tree_info *info);
+/* Census of the tree diffusion (no reference counterpart): with on != 0 the
+ * following diffusions also count the evaluated pairs that lie within their
+ * target's cut-off radius (vx_tree_info.diff_in_radius), at the cost of one
+ * more pass over each staged leaf.  Results do not change. */
+int vx_tree_census(vx_case *c, int on);
 /* The last tree build's sorted Morton keys and slot permutation (its first
  * count entries, count <= the slots it keyed): keys[k] is the k-th smallest
  * key and perm[k] its slot; equal keys keep the slot order.  Excluded slots
@@ -242,6 +265,11 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL);
  *                     levels above the cell that holds 64 vortices at the mean
  *                     density, 0..3 (-1: default 1).  It changes which pairs
  *                     are summed directly, within the tree's error bound.
+ *   bs_targets        the tree Biot-Savart's targets per wave: 1 a bucket
+ *                     (at most 64 vortices, fewer on average), 2 a target
+ *                     group (runs of 64 in Morton order, fuller waves, a wider
+ *                     opening box); 0: default 1.  Within the tree's error
+ *                     bound, like leaf_off.
  * vx_case_autotune times every choice on the case's current state (the
  * direct kernels one by one below tree_min_n, the whole velocity stage
  * above it), keeps the fastest and reports the default's and the chosen
@@ -249,7 +277,9 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL);
 typedef struct vx_tuning {
     int32_t tpt_bs, tpt_diff;
     int32_t leaf_off;
+    int32_t bs_targets;
     int32_t trials;                 /* out: configurations timed */
+    int32_t reserved;
     double ms_default, ms_best;     /* out: timed stage, default and chosen */
 } vx_tuning;
 int vx_case_autotune(vx_case *c, double vinf_x, double vinf_y, vx_tuning *out);

@@ -49,10 +49,17 @@ extern "C" {
 typedef double TVars;
 typedef TVars TVctr[2];
 
+/* The reference's default build carries the tree field (32 B); built with
+ * NO_TREE it has none (24 B, include/definitions.h:61-69).  A NO_TREE caller
+ * gets the 24-byte record here, and its init_device_conf_values() call
+ * (which the reference's main makes before any other, src/main.c:446)
+ * selects that layout for every later call that takes Vortex arrays. */
 typedef struct Vortex {
     TVars r[2];
     TVars g;
+#ifndef NO_TREE
     unsigned int tree_id;
+#endif
 } Vortex;
 
 typedef struct PVortex {
@@ -130,6 +137,13 @@ int first_speed(Vortex *pos, TVctr *V_inf, size_t n, Vortex *second_pos, PVortex
 
 /* Select the arithmetic of the compat entry points (VX_PREC_F64 by default). */
 int vx_compat_set_precision(int precision);
+/* init_device_conf_values for callers of the 24-byte NO_TREE Vortex (no
+ * reference counterpart: the reference selects the layout at compile time);
+ * a NO_TREE caller's init_device_conf_values() resolves to it. */
+int vx_compat_init_no_tree(void);
+#ifdef NO_TREE
+#define init_device_conf_values() vx_compat_init_no_tree()
+#endif
 
 #ifdef __cplusplus
 }

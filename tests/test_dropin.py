@@ -16,6 +16,7 @@ from vortex_amd import geometry as geo
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "vortex_amd", "vortex")
 DRIVER = os.path.join(ROOT, "vortex_amd", "compat_driver")
+DRIVER_NT = os.path.join(ROOT, "vortex_amd", "compat_driver_nt")  # built with NO_TREE (24-byte Vortex)
 REF = "/root/reference"
 
 RUN_CONF = """steps {steps}                     # number of steps
@@ -177,14 +178,17 @@ def test_cli_restart_from_snapshot(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["f64", "mixed"])
-def test_compat_driver_matches_native_api(tmp_path, prec):
+@pytest.mark.parametrize("prec,driver", [("f64", DRIVER), ("mixed", DRIVER), ("f64", DRIVER_NT)])
+def test_compat_driver_matches_native_api(tmp_path, prec, driver):
+    """The compat driver (the reference's loop on the unit_1.cuh calls), built
+    with the default 32-byte Vortex and with NO_TREE's 24-byte one, against
+    the native API step by step."""
     from vortex_amd.solver import Case, make_params, VX_PREC_F64, VX_PREC_MIXED
     tmp = str(tmp_path)
     pan = geo.circle(64, 0.5)
     geo.write_profile(os.path.join(tmp, "profile.txt"), pan)
     steps = 8
-    args = [DRIVER, "profile.txt", str(steps)] + (["mixed"] if prec == "mixed" else [])
+    args = [driver, "profile.txt", str(steps)] + (["mixed"] if prec == "mixed" else [])
     out = subprocess.run(args, cwd=tmp, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     rows = np.array([[float(v) for v in line.split()] for line in out.stdout.strip().splitlines()
@@ -246,9 +250,10 @@ def test_cli_second_component_matches_native_api(tmp_path):
 
 
 @pytest.mark.gpu
-def test_compat_second_component_matches_native_api(tmp_path):
+@pytest.mark.parametrize("driver", [DRIVER, DRIVER_NT])
+def test_compat_second_component_matches_native_api(tmp_path, driver):
     """second_speed / first_speed through the reference-named entry points
-    against the same loop on the native API (FP64)."""
+    (32- and 24-byte Vortex) against the same loop on the native API (FP64)."""
     from vortex_amd.solver import Case, make_params, points_relax
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "output", "vels"))
@@ -257,7 +262,7 @@ def test_compat_second_component_matches_native_api(tmp_path):
     px, py, pg = tracer_points(24)
     write_points(os.path.join(tmp, "pts.txt"), px, py, pg)
     steps = 5
-    out = subprocess.run([DRIVER, "profile.txt", str(steps), "f64", "pts.txt"], cwd=tmp, capture_output=True,
+    out = subprocess.run([driver, "profile.txt", str(steps), "f64", "pts.txt"], cwd=tmp, capture_output=True,
                          text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     pts = {"first": [], "second": []}

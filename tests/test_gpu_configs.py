@@ -233,8 +233,11 @@ def test_capacity_grows_across_inc_step_like_the_oracle(ref):
     passes 1400 in 20 steps): the capacity grows from 512 to 2048 in three
     quanta while vortices are live (incr_vort_quant, src/unit_1.cu:511-558).
     The growing case equals one whose capacity was reserved up front bit for
-    bit, and follows the oracle, which grows its arrays alike, to the FP64
-    tolerances of a 20-step run."""
+    bit over all 20 steps.  It follows the oracle, which grows its arrays
+    alike, over the first 8 steps (through the first growth) to the FP64
+    tolerances: this flow's vortex count is chaotic beyond that, the oracle
+    itself changing its count at step 10 or 11 when 1e-13 noise is added to
+    the positions after each step."""
     import vx_ref
     pan = geo.circle(128, 0.5)
     prm = make_params(pan, dt=0.01, r_col_same_sign=0.1, r_col_diff_sign=0.0, birth_offset_rel=0.5, inc_step=512,
@@ -248,20 +251,26 @@ def test_capacity_grows_across_inc_step_like_the_oracle(ref):
     big.reserve(64 * 1024)
     caps = {c.capacity}
     fs, fb, fr = [], [], []
+    k_ref = 8
     for k in range(20):
         fs.append(c.time_step(VINF))
         fb.append(big.time_step(VINF))
-        fr.append(st.step(VINF))
         caps.add(c.capacity)
-        assert c.n == st.n == big.n, (k, c.n, st.n, big.n)
+        assert c.n == big.n, (k, c.n, big.n)
+        if k < k_ref:
+            fr.append(st.step(VINF))
+            assert c.n == st.n, (k, c.n, st.n)
+            if k == k_ref - 1:
+                assert len(caps) >= 2, caps
+                fx_rms = math.sqrt(np.mean([f[1] ** 2 for f in fr]))
+                assert max(abs(a[1] - b[1]) for a, b in zip(fs, fr)) < 1e-9 * fx_rms
+                x, y, g = c.state()
+                assert np.max(np.abs(x - st.x[:st.n])) < 1e-9
+                assert np.max(np.abs(g - st.g[:st.n])) < 1e-12 * np.max(np.abs(g))
     assert len(caps) >= 4 and max(caps) >= 2048, caps
     assert np.array_equal(np.array(fs), np.array(fb))
     for a, b in zip(c.state(), big.state()):
         assert np.array_equal(a, b)
-    fx_rms = math.sqrt(np.mean([f[1] ** 2 for f in fr]))
-    assert max(abs(a[1] - b[1]) for a, b in zip(fs, fr)) < 1e-8 * fx_rms
-    x, y, g = c.state()
-    assert np.max(np.abs(x - st.x[:st.n])) < 1e-9 and np.max(np.abs(g - st.g[:st.n])) < 1e-12 * np.max(np.abs(g))
     c.close()
     big.close()
 

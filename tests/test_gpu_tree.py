@@ -61,9 +61,12 @@ def cloud40k(ref):
     return X, Y, G, s, eps, u, v, d
 
 
-@pytest.mark.parametrize("prec,order,tol", [(VX_PREC_F64, 32, 1e-8), (VX_PREC_F64, 16, 2e-4),
-                                            (VX_PREC_MIXED, 16, 3e-4), (VX_PREC_MIXED, 32, 3e-5)])
+@pytest.mark.parametrize("prec,order,tol", [(VX_PREC_F64, 32, 1e-8), (VX_PREC_F64, 20, 2e-5),
+                                            (VX_PREC_F64, 16, 2e-4), (VX_PREC_MIXED, 16, 3e-4),
+                                            (VX_PREC_MIXED, 20, 5e-5), (VX_PREC_MIXED, 32, 3e-5)])
 def test_tree_biot_savart_vs_direct_oracle(cloud40k, prec, order, tol):
+    """Orders 16, 20 and 32 (truncation about theta^order, theta = 0.5;
+    FP32 far fields through the matrix-core M2L) against the direct sums."""
     X, Y, G, s, eps, ur, vr, dr = cloud40k
     pan = geo.circle(200, 0.5)
     c = Case(make_params(pan, precision=prec, ve_size=eps, order=order), pan)
@@ -71,8 +74,31 @@ def test_tree_biot_savart_vs_direct_oracle(cloud40k, prec, order, tol):
     c.tree_biot_savart(s, (1.0, 0.0))
     u, v, d = c.velocity(s)
     info = c.tree_stats()
+    print(f"[tree] prec={prec} order={order}: rel err u {rel_err(u - 1.0, ur - 1.0):.3g} v {rel_err(v, vr):.3g}",
+          flush=True)
+    assert info["order"] == order
     assert info["m"] == s  # every slot is near the box here
     assert info["m2p_evals"] > 0 and info["bs_pairs"] < s * s / 10
+    assert rel_err(u - 1.0, ur - 1.0) < tol and rel_err(v, vr) < tol
+    dtol = 1e-13 if prec == VX_PREC_F64 else 1e-4
+    assert np.max(np.abs(d - dr) / dr) < dtol
+
+
+@pytest.mark.parametrize("prec,order,tol", [(VX_PREC_F64, 16, 2e-4), (VX_PREC_MIXED, 16, 3e-4),
+                                            (VX_PREC_MIXED, 32, 3e-5)])
+def test_tree_biot_savart_target_groups_vs_direct_oracle(cloud40k, prec, order, tol):
+    """Target groups (64 consecutive particles of a cell per wave) instead of
+    buckets: the same bounds against the direct sums."""
+    X, Y, G, s, eps, ur, vr, dr = cloud40k
+    pan = geo.circle(200, 0.5)
+    c = Case(make_params(pan, precision=prec, ve_size=eps, order=order), pan)
+    c.set_tuning(bs_targets=2)
+    c.set_state(X, Y, G)
+    c.tree_biot_savart(s, (1.0, 0.0))
+    u, v, d = c.velocity(s)
+    print(f"[tree] groups prec={prec} order={order}: rel err u {rel_err(u - 1.0, ur - 1.0):.3g} "
+          f"v {rel_err(v, vr):.3g}", flush=True)
+    c.close()
     assert rel_err(u - 1.0, ur - 1.0) < tol and rel_err(v, vr) < tol
     dtol = 1e-13 if prec == VX_PREC_F64 else 1e-4
     assert np.max(np.abs(d - dr) / dr) < dtol
@@ -145,6 +171,49 @@ def test_tree_diffusion_vs_direct_oracle(ref, prec, tol):
     info = c.tree_stats()
     assert 0 < info["diff_pairs"] < s * s / 4  # the cut-off prunes
     assert info["heavy"] > 0  # isolated vortices go through the split direct kernel
+
+
+@pytest.mark.parametrize("prec", [VX_PREC_F64, VX_PREC_MIXED])
+def test_tree_diffusion_census_counts_in_radius_pairs(ref, prec):
+    """The census (vx_tree_census) leaves the velocities bit for bit and
+    counts, of the evaluated pairs, those within each target's cut-off
+    radius; on a strip dense enough that no target's radius reaches 5 % of
+    the domain (none goes to the domain-scale kernel), that count equals the
+    brute-force count of (target, source) pairs with r < cut d."""
+    n = 8000
+    rng = np.random.default_rng(12)
+    x, y = rng.uniform(0.0, 40.0, n), rng.uniform(0.0, 0.05, n)
+    g = rng.uniform(-0.004, 0.004, n)
+    s = (n + 31) // 32 * 32
+    X, Y, G = tail(x, y, g, s)
+    eps, nu = 0.004, 1e-3
+    pan = geo.circle(64, 0.05)
+    out = []
+    for census in (False, True):
+        c = Case(make_params(pan, precision=prec, ve_size=eps, nu=nu), pan)
+        c.set_state(X, Y, G)
+        c.tree_census(census)
+        c.tree_biot_savart(s, (0.0, 0.0))
+        _, _, d = c.velocity(s)
+        c.tree_diffusion()
+        u, v, _ = c.velocity(s)
+        out.append((u, v, d, c.tree_stats()))
+        c.close()
+    (u0, v0, d, i0), (u1, v1, _, i1) = out
+    assert np.array_equal(u0, u1) and np.array_equal(v0, v1)
+    assert i0["diff_in_radius"] == -1 and i1["heavy"] == 0
+    cut = 18.0 if prec == VX_PREC_MIXED else 30.0
+    m = i1["m"]
+    within = 0
+    for a in range(0, n, 1000):
+        dx = x[a:a + 1000, None] - x[None, :]
+        dy = y[a:a + 1000, None] - y[None, :]
+        within += int((dx * dx + dy * dy < (cut * d[a:a + 1000, None]) ** 2).sum())
+    print(f"[census] prec={prec}: evaluated {i1['diff_pairs']} in radius {i1['diff_in_radius']} "
+          f"brute force {within} (m={m})", flush=True)
+    # FP32 distances may move pairs right at the radius across it
+    assert abs(i1["diff_in_radius"] - within) <= 1e-4 * within
+    assert within <= i1["diff_pairs"]
 
 
 @pytest.mark.parametrize("prec,tol", [(VX_PREC_F64, 1e-9), (VX_PREC_MIXED, 1e-3)])

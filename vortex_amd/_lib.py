@@ -25,12 +25,12 @@ EXPORTS = [
     "vx_case_create", "vx_case_destroy", "vx_case_build_matrix", "vx_case_set_matrix",
     "vx_case_get_matrix", "vx_case_get_panel_matrix", "vx_case_reserve", "vx_case_set_state",
     "vx_case_get_state", "vx_case_get_velocity", "vx_case_set_velocity", "vx_case_n",
-    "vx_case_capacity", "vx_case_set_n", "vx_case_d_g", "vx_case_set_d_g", "vx_case_sync",
+    "vx_case_capacity", "vx_case_set_n", "vx_case_d_g", "vx_case_set_d_g", "vx_case_sync", "vx_case_save", "vx_case_load",
     "vx_vort_creation", "vx_speed", "vx_step", "vx_time_step", "vx_run_steps",
     "vx_velocity_probe", "vx_right_part", "vx_birth", "vx_biot_savart", "vx_diffusion",
     "vx_diffusion_wall", "vx_step_move", "vx_compact", "vx_setka", "vx_collapse",
     "vx_case_timings", "vx_case_enable_timing", "vx_tree_biot_savart", "vx_tree_diffusion",
-    "vx_tree_stats", "vx_points_move", "vx_points_relax", "vx_tree_order",
+    "vx_tree_stats", "vx_points_move", "vx_points_relax", "vx_tree_order", "vx_tree_census",
     "vx_case_set_probes", "vx_case_get_probes", "vx_velocity_probe_noalloc",
     "vx_case_solve_info", "vx_run_steps_graph", "vx_batch_create", "vx_batch_run_steps", "vx_batch_destroy",
     "vx_case_autotune", "vx_case_set_tuning", "vx_case_get_tuning",
@@ -53,13 +53,13 @@ class Params(C.Structure):
 class TreeInfo(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("m", "leaves", "nodes", "fallback", "bs_pairs", "m2p_evals",
                                           "diff_pairs")] + [("order", C.c_int32), ("heavy", C.c_int32),
-                                                            ("m2l", C.c_int64)]
+                                                            ("m2l", C.c_int64), ("diff_in_radius", C.c_int64)]
 
 
 class Tuning(C.Structure):
     """Mirror of vx_tuning (include/vortex.h)."""
-    _fields_ = [("tpt_bs", C.c_int32), ("tpt_diff", C.c_int32), ("leaf_off", C.c_int32), ("trials", C.c_int32),
-                ("ms_default", C.c_double), ("ms_best", C.c_double)]
+    _fields_ = [("tpt_bs", C.c_int32), ("tpt_diff", C.c_int32), ("leaf_off", C.c_int32), ("bs_targets", C.c_int32),
+                ("trials", C.c_int32), ("reserved", C.c_int32), ("ms_default", C.c_double), ("ms_best", C.c_double)]
 
 
 class Timings(C.Structure):
@@ -103,6 +103,8 @@ def lib():
         "vx_case_d_g": ([vp], d),
         "vx_case_set_d_g": ([vp, d], C.c_int),
         "vx_case_sync": ([vp], C.c_int),
+        "vx_case_save": ([vp, C.c_char_p, vp, i64], C.c_int),
+        "vx_case_load": ([vp, C.c_char_p, vp, i64, C.POINTER(C.c_int64)], C.c_int),
         "vx_vort_creation": ([vp, d, d], C.c_int),
         "vx_speed": ([vp, d, d], C.c_int),
         "vx_step": ([vp, dp], C.c_int),
@@ -126,6 +128,7 @@ def lib():
         "vx_points_move": ([vp, dp, dp, dp, dp, dp, i64], C.c_int),
         "vx_points_relax": ([dp, dp, dp, dp, dp, i64, d, d], C.c_int),
         "vx_tree_order": ([vp, dp, dp, i64], C.c_int),
+        "vx_tree_census": ([vp, C.c_int], C.c_int),
         "vx_case_set_probes": ([vp, dp, dp, i64], C.c_int),
         "vx_case_get_probes": ([vp, dp, dp, i64], C.c_int),
         "vx_velocity_probe_noalloc": ([vp, vp, vp, i64, d, d, vp, vp], C.c_int),

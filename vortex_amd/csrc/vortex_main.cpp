@@ -91,6 +91,47 @@ struct Points {
     }
 };
 
+// The driver's own state in an exact checkpoint's blob: step, the force
+// line still to be logged, V_inf, the probe rows since the last save and the
+// tracer points.
+struct Blob {
+    std::vector<char> b;
+    template <class T>
+    void put(const T &v) {
+        const char *p = reinterpret_cast<const char *>(&v);
+        b.insert(b.end(), p, p + sizeof(T));
+    }
+    void put_vec(const std::vector<double> &v) {
+        put((int64_t)v.size());
+        const char *p = reinterpret_cast<const char *>(v.data());
+        b.insert(b.end(), p, p + sizeof(double) * v.size());
+    }
+};
+struct BlobReader {
+    const char *p, *e;
+    template <class T>
+    bool get(T &v) {
+        if (e - p < (std::ptrdiff_t)sizeof(T)) return false;
+        memcpy(&v, p, sizeof(T));
+        p += sizeof(T);
+        return true;
+    }
+    bool get_vec(std::vector<double> &v) {
+        int64_t n = 0;
+        if (!get(n) || n < 0 || (e - p) / (std::ptrdiff_t)sizeof(double) < n) return false;
+        v.resize((size_t)n);
+        if (n) memcpy(v.data(), p, sizeof(double) * (size_t)n);
+        p += sizeof(double) * (size_t)n;
+        return true;
+    }
+};
+void put_points(Blob &bl, const Points &q) {
+    bl.put_vec(q.x); bl.put_vec(q.y); bl.put_vec(q.g); bl.put_vec(q.u); bl.put_vec(q.v);
+}
+bool get_points(BlobReader &r, Points &q) {
+    return r.get_vec(q.x) && r.get_vec(q.y) && r.get_vec(q.g) && r.get_vec(q.u) && r.get_vec(q.v);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -209,7 +250,64 @@ int main(int argc, char **argv) {
     }
 
     int64_t step = 0;
-    if (!conf.kadr_file.empty()) {
+    // probe points (src/main.c:521-526)
+    const int64_t NPROBE = 500;
+    const int64_t save = conf.saving_step > 0 ? conf.saving_step : 0;
+    std::vector<double> px(NPROBE), py(NPROBE), pu(NPROBE), pv(NPROBE);
+    for (int64_t k = 0; k < NPROBE; ++k) {
+        px[k] = -0.15;
+        py[k] = 0.01 + 0.002 * (double)k;
+    }
+    std::vector<double> vbuf((size_t)(2 * NPROBE * (save > 0 ? save : 1)), 0.0);
+    int64_t vslot = 0;
+    double vinf[2] = {conf.v_inf_x, conf.v_inf_y}, dv[2] = {0.0, 0.0};
+    if (conf.v_inf_incr_steps) {
+        dv[0] = conf.v_inf_x / (double)conf.v_inf_incr_steps;
+        dv[1] = conf.v_inf_y / (double)conf.v_inf_incr_steps;
+        vinf[0] = vinf[1] = 0.0;
+    }
+    double forces[4] = {0.0, 0.0, 0.0, 0.0};
+    bool from_ckpt = false;
+    if (!conf.checkpoint_file.empty()) {
+        // exact restart: the case state and this driver's own
+        std::vector<char> blob((size_t)(64 << 20));
+        int64_t got = 0;
+        if (vx_case_load(c, conf.checkpoint_file.c_str(), blob.data(), (int64_t)blob.size(), &got))
+            return fail(c, "checkpoint load");
+        BlobReader r{blob.data(), blob.data() + got};
+        std::vector<double> vb;
+        if (!r.get(step) || !r.get(forces) || !r.get(vinf) || !r.get(vslot) || !r.get_vec(vb) ||
+            !get_points(r, second) || !get_points(r, first) || vb.size() != vbuf.size() ||
+            second.size() != first.size() || vslot < 0 || vslot > (save > 0 ? save : 1)) {
+            logf(1, "checkpoint %s: driver state does not match this run file", conf.checkpoint_file.c_str());
+            vx_case_destroy(c);
+            return 1;
+        }
+        vbuf = vb;
+        conf.steps += step;
+        from_ckpt = true;
+        logf(3, "restart from checkpoint %s at step %" PRId64, conf.checkpoint_file.c_str(), step);
+    }
+    const int64_t ckpt = conf.checkpoint_step > 0 ? conf.checkpoint_step : 0;
+    if (ckpt && make_dir("output/checkpoints")) {
+        logf(1, "cannot create output/checkpoints: %s", strerror(errno));
+        vx_case_destroy(c);
+        return 1;
+    }
+    auto save_ckpt = [&](int64_t at) -> int {
+        Blob bl;
+        bl.put(at);
+        bl.put(forces);
+        bl.put(vinf);
+        bl.put(vslot);
+        bl.put_vec(vbuf);
+        put_points(bl, second);
+        put_points(bl, first);
+        char fn[256];
+        snprintf(fn, sizeof(fn), "output/checkpoints/Ckpt%06" PRId64 ".bin", at);
+        return vx_case_save(c, fn, bl.b.data(), (int64_t)bl.b.size());
+    };
+    if (!from_ckpt && !conf.kadr_file.empty()) {
         std::vector<double> x, y, g;
         if (vxio::read_kadr(conf.kadr_file, x, y, g, &err)) {
             logf(1, "%s", err.c_str());
@@ -224,7 +322,7 @@ int main(int argc, char **argv) {
         }
     }
     const int64_t first_step = step;
-    if (step && have_points) {
+    if (step && have_points && !from_ckpt) {
         // restart of the points from their own snapshots (src/main.c:606-648)
         char fs[256], ff[256];
         snprintf(fs, sizeof(fs), "output/second/Kadr%06" PRId64 ".txt", step);
@@ -246,30 +344,14 @@ int main(int argc, char **argv) {
         }
     }
 
-    // probe points (src/main.c:521-526)
-    const int64_t NPROBE = 500;
-    const int64_t save = conf.saving_step > 0 ? conf.saving_step : 0;
-    std::vector<double> px(NPROBE), py(NPROBE), pu(NPROBE), pv(NPROBE);
-    for (int64_t k = 0; k < NPROBE; ++k) {
-        px[k] = -0.15;
-        py[k] = 0.01 + 0.002 * (double)k;
-    }
-    std::vector<double> vbuf((size_t)(2 * NPROBE * (save > 0 ? save : 1)), 0.0);
-    int64_t vslot = 0;
-
-    double vinf[2] = {conf.v_inf_x, conf.v_inf_y}, dv[2] = {0.0, 0.0};
-    if (conf.v_inf_incr_steps) {
-        dv[0] = conf.v_inf_x / (double)conf.v_inf_incr_steps;
-        dv[1] = conf.v_inf_y / (double)conf.v_inf_incr_steps;
-        vinf[0] = vinf[1] = 0.0;
-    }
-    FILE *fp = fopen("output/F_p.txt", "w");
-    double forces[4] = {0.0, 0.0, 0.0, 0.0};
+    // a checkpoint restart continues the force history
+    FILE *fp = fopen("output/F_p.txt", from_ckpt ? "a" : "w");
     vx_case_enable_timing(c, 1);
     const double eps_out = 0.008;  // src/main.c:517
     std::vector<double> hx, hy, hg;
     logf(2, "Start from step %" PRId64, step);
     for (; step < conf.steps; ++step) {
+        if (ckpt && step % ckpt == 0 && step != first_step && save_ckpt(step)) return fail(c, "checkpoint");
         if (step < conf.v_inf_incr_steps) {
             vinf[0] += dv[0];
             vinf[1] += dv[1];
@@ -311,7 +393,7 @@ int main(int argc, char **argv) {
             std::vector<double> ue(m), ve(m);
             if (vx_velocity_probe(c, second.x.data(), second.y.data(), m, vinf[0], vinf[1], ue.data(), ve.data()))
                 return fail(c, "second Speed evaluation ERROR!");
-            if (step == first_step) {
+            if (step == first_step && !from_ckpt) {
                 second.u = ue;
                 second.v = ve;
             }
@@ -349,6 +431,8 @@ int main(int argc, char **argv) {
         if (vx_speed(c, vinf[0], vinf[1])) return fail(c, "Speed evaluation ERROR!");
         if (vx_step(c, forces)) return fail(c, "Moving ERROR!");
     }
+    if (ckpt && conf.steps % ckpt == 0 && conf.steps != first_step && save_ckpt(conf.steps))
+        return fail(c, "checkpoint");
     {
         vx_timings tm;
         vx_case_timings(c, &tm);

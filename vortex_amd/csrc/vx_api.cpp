@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -300,6 +301,103 @@ int vx_case_set_velocity(vx_case *c, const double *u, const double *v, const dou
     return 0;
 }
 
+// ---- exact checkpoint ----------------------------------------------------------
+// File: header (magic, sizes, d_g, the tree's far-target hint, the case
+// parameters), then x, y, g over all `cap` slots (the tail beyond n takes
+// part in the nearest-3 search), then the caller's blob.  Native byte order.
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int64_t P, n, cap, far_hint, user_bytes;
+    double d_g;
+    vx_params prm;
+};
+constexpr char kCkptMagic[8] = {'V', 'X', 'C', 'K', 'P', 'T', '0', '1'};
+
+bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
+}  // namespace
+
+int vx_case_save(vx_case *c, const char *path, const void *user, int64_t user_bytes) {
+    VX_ARG(c && path && user_bytes >= 0 && (user_bytes == 0 || user), "checkpoint arguments");
+    CkptHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, kCkptMagic, sizeof(h.magic));
+    h.P = c->P;
+    h.n = c->n;
+    h.cap = c->cap;
+    h.far_hint = tree_far_hint(c);
+    h.user_bytes = user_bytes;
+    h.d_g = c->d_g;
+    h.prm = c->prm;
+    std::vector<double> buf((size_t)(3 * c->cap));
+    hipStream_t s = c->stream;
+    VX_CHECK(hipMemcpyAsync(buf.data(), c->x.p, sizeof(double) * c->cap, hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipMemcpyAsync(buf.data() + c->cap, c->y.p, sizeof(double) * c->cap, hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipMemcpyAsync(buf.data() + 2 * c->cap, c->g.p, sizeof(double) * c->cap, hipMemcpyDeviceToHost, s));
+    VX_CHECK(hipStreamSynchronize(s));
+    FILE *f = std::fopen(path, "wb");
+    if (!f) {
+        set_error(std::string("checkpoint: cannot write ") + path);
+        return 1;
+    }
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(buf.data(), sizeof(double), buf.size(), f) ==
+                                                              buf.size();
+    if (ok && user_bytes) ok = std::fwrite(user, 1, (size_t)user_bytes, f) == (size_t)user_bytes;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        set_error(std::string("checkpoint: write failed: ") + path);
+        return 1;
+    }
+    return 0;
+}
+
+int vx_case_load(vx_case *c, const char *path, void *user, int64_t user_cap, int64_t *user_bytes) {
+    VX_ARG(c && path && user_cap >= 0 && (user_cap == 0 || user), "checkpoint arguments");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) {
+        set_error(std::string("checkpoint: cannot read ") + path);
+        return 1;
+    }
+    CkptHeader h;
+    std::vector<double> buf;
+    std::string bad;
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kCkptMagic, sizeof(h.magic)) != 0)
+        bad = "not a checkpoint of this library";
+    else if (h.P != c->P || h.prm.precision != c->prm.precision || !same_bits(h.prm.dt, c->prm.dt) ||
+             !same_bits(h.prm.ve_size, c->prm.ve_size) || !same_bits(h.prm.nu, c->prm.nu))
+        bad = "checkpoint of another case (panels, precision, dt, ve_size or nu differ)";
+    else if (h.n < 0 || h.cap < h.n || h.user_bytes < 0)
+        bad = "corrupt checkpoint header";
+    else if (h.user_bytes > user_cap)
+        bad = "checkpoint: the caller's buffer is smaller than its blob (" + std::to_string(h.user_bytes) + " B)";
+    if (bad.empty()) {
+        buf.resize((size_t)(3 * h.cap));
+        if (std::fread(buf.data(), sizeof(double), buf.size(), f) != buf.size() ||
+            (h.user_bytes && std::fread(user, 1, (size_t)h.user_bytes, f) != (size_t)h.user_bytes))
+            bad = "truncated checkpoint";
+    }
+    std::fclose(f);
+    if (!bad.empty()) {
+        set_error(bad + ": " + path);
+        return 2;
+    }
+    if (reserve(c, h.cap)) return 1;
+    hipStream_t s = c->stream;
+    if (h.cap) {
+        VX_CHECK(hipMemcpyAsync(c->x.p, buf.data(), sizeof(double) * h.cap, hipMemcpyHostToDevice, s));
+        VX_CHECK(hipMemcpyAsync(c->y.p, buf.data() + h.cap, sizeof(double) * h.cap, hipMemcpyHostToDevice, s));
+        VX_CHECK(hipMemcpyAsync(c->g.p, buf.data() + 2 * h.cap, sizeof(double) * h.cap, hipMemcpyHostToDevice, s));
+    }
+    // slots the saved case had not grown to yet: the far tail they get on growth
+    if (c->cap > h.cap && launch_fill_tail(c, h.cap, c->cap)) return 1;
+    VX_CHECK(hipStreamSynchronize(s));
+    c->n = h.n;
+    c->d_g = h.d_g;
+    if ((h.far_hint > 0 || c->tree) && tree_set_far_hint(c, h.far_hint)) return 1;
+    if (user_bytes) *user_bytes = h.user_bytes;
+    return 0;
+}
+
 int64_t vx_case_n(const vx_case *c) { return c ? c->n : -1; }
 int64_t vx_case_capacity(const vx_case *c) { return c ? c->cap : -1; }
 double vx_case_d_g(const vx_case *c) { return c ? c->d_g : 0.0; }
@@ -382,6 +480,12 @@ int vx_tree_diffusion(vx_case *c) {
 int vx_tree_stats(vx_case *c, vx_tree_info *info) {
     VX_ARG(c && info, "null pointer");
     return tree_stats(c, info);
+}
+
+int vx_tree_census(vx_case *c, int on) {
+    VX_ARG(c, "null pointer");
+    c->census = on != 0;
+    return 0;
 }
 
 int vx_tree_order(vx_case *c, uint32_t *keys, int32_t *perm, int64_t count) {
@@ -608,19 +712,22 @@ int vx_case_autotune(vx_case *c, double vinf_x, double vinf_y, vx_tuning *out) {
     best.trials = 0;
     constexpr int kReps = 3;
     if (tree) {
-        // the whole velocity stage for each bucket depth
+        // the whole velocity stage for each bucket depth and kind of target
         double tdef = 0.0, tbest = 1e300;
-        for (int off = 0; off <= 3; ++off) {
-            c->tune.leaf_off = off;
-            double t = 0.0;
-            if (time_ms(c, kReps, [&] { return vx_speed(c, vinf_x, vinf_y); }, &t)) return 1;
-            ++best.trials;
-            if (off == 1) tdef = t;
-            if (t < tbest) {
-                tbest = t;
-                best.leaf_off = off;
+        for (int tg = 1; tg <= 2; ++tg)
+            for (int off = 0; off <= 3; ++off) {
+                c->tune.leaf_off = off;
+                c->tune.bs_targets = tg;
+                double t = 0.0;
+                if (time_ms(c, kReps, [&] { return vx_speed(c, vinf_x, vinf_y); }, &t)) return 1;
+                ++best.trials;
+                if (off == 1 && tg == 1) tdef = t;
+                if (t < tbest) {
+                    tbest = t;
+                    best.leaf_off = off;
+                    best.bs_targets = tg;
+                }
             }
-        }
         best.ms_default = tdef;
         best.ms_best = tbest;
     } else {
@@ -664,9 +771,11 @@ int vx_case_set_tuning(vx_case *c, const vx_tuning *t) {
     auto ok_tpt = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8; };
     VX_ARG(ok_tpt(t->tpt_bs) && ok_tpt(t->tpt_diff), "targets per thread must be 0, 1, 2, 4 or 8");
     VX_ARG(t->leaf_off >= -1 && t->leaf_off <= 3, "leaf_off must be -1..3");
+    VX_ARG(t->bs_targets >= 0 && t->bs_targets <= 2, "bs_targets must be 0, 1 or 2");
     c->tune.tpt_bs = t->tpt_bs;
     c->tune.tpt_diff = t->tpt_diff;
     c->tune.leaf_off = t->leaf_off;
+    c->tune.bs_targets = t->bs_targets;
     return 0;
 }
 

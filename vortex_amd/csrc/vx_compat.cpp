@@ -22,12 +22,13 @@ __attribute__((weak)) int first_step = 0;
 namespace vx {
 namespace {
 
-static_assert(sizeof(Vortex) == 32, "Vortex layout of the reference's default build");
+static_assert(sizeof(Vortex) == 32, "Vortex layout of the reference's default build (NO_TREE: 24 B, at run time)");
 static_assert(sizeof(PVortex) == 16, "PVortex layout");
 static_assert(sizeof(tPanel) == sizeof(vx_panel), "tPanel layout");
 
 struct Compat {
     vx_case *c = nullptr;
+    int64_t vb = sizeof(Vortex);  // the caller's Vortex: 32 B, or 24 B built with NO_TREE
     const tPanel *panels_dev = nullptr;
     size_t P = 0;
     int precision = VX_PREC_F64;
@@ -43,23 +44,26 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-__global__ void k_aos_in(const Vortex *__restrict__ pos, int64_t cnt, double *__restrict__ x, double *__restrict__ y,
-                         double *__restrict__ g) {
+// Vortex records of vb bytes: r[0], r[1], g at offsets 0, 8, 16, and with
+// vb = 32 the tree_id at 24 (include/definitions.h:61-69).
+__global__ void k_aos_in(const char *__restrict__ pos, int64_t vb, int64_t cnt, double *__restrict__ x,
+                         double *__restrict__ y, double *__restrict__ g) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cnt) return;
-    const Vortex p = pos[i];
-    x[i] = p.r[0];
-    y[i] = p.r[1];
-    g[i] = p.g;
+    const double *p = reinterpret_cast<const double *>(pos + i * vb);
+    x[i] = p[0];
+    y[i] = p[1];
+    g[i] = p[2];
 }
 
 __global__ void k_aos_out(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ g,
-                          int64_t from, int64_t to, Vortex *__restrict__ pos) {
+                          int64_t from, int64_t to, char *__restrict__ pos, int64_t vb) {
     const int64_t i = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= to) return;
-    pos[i].r[0] = x[i];
-    pos[i].r[1] = y[i];
-    pos[i].g = g[i];
+    double *p = reinterpret_cast<double *>(pos + i * vb);
+    p[0] = x[i];
+    p[1] = y[i];
+    p[2] = g[i];
 }
 
 __global__ void k_pv_in(const PVortex *__restrict__ v, int64_t cnt, double *__restrict__ u, double *__restrict__ w) {
@@ -79,14 +83,31 @@ __global__ void k_pv_out(const double *__restrict__ u, const double *__restrict_
 
 // zero_Kernel (src/kernel.cu:767-777): g = 0 far away; the factor is a hash
 // of the slot index instead of rand().
-__global__ void k_aos_tail(Vortex *__restrict__ pos, int64_t from, int64_t to) {
+__global__ void k_aos_tail(char *__restrict__ pos, int64_t vb, int64_t from, int64_t to) {
     const int64_t i = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= to) return;
     const double h = (double)(mix64((uint64_t)i ^ 12345ull) >> 33);
-    pos[i].r[0] = 2.0e5 * h + 2.0e5;
-    pos[i].r[1] = 2.0e5 * h + 2.0e5;
-    pos[i].g = 0.0;
-    pos[i].tree_id = 0;
+    double *p = reinterpret_cast<double *>(pos + i * vb);
+    p[0] = 2.0e5 * h + 2.0e5;
+    p[1] = 2.0e5 * h + 2.0e5;
+    p[2] = 0.0;
+    if (vb >= 28) *reinterpret_cast<unsigned int *>(pos + i * vb + 24) = 0u;
+}
+
+// Host copies of Vortex records of the caller's layout.
+struct HostPts {
+    std::vector<char> raw;
+    int64_t vb = 32;
+    size_t m = 0;
+    double *rec(size_t i) { return reinterpret_cast<double *>(raw.data() + i * vb); }
+    const double *rec(size_t i) const { return reinterpret_cast<const double *>(raw.data() + i * vb); }
+};
+int read_pts(HostPts &h, const void *dev, size_t m) {
+    h.vb = g_compat.vb;
+    h.m = m;
+    h.raw.assign(m * (size_t)h.vb, 0);
+    if (m) VX_CHECK(hipMemcpy(h.raw.data(), dev, h.raw.size(), hipMemcpyDeviceToHost));
+    return 0;
 }
 
 unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -149,7 +170,8 @@ vx_case *compat_case(const tPanel *panels_dev) {
 
 int stage_in(vx_case *c, const Vortex *pos, int64_t cnt) {
     if (cnt <= 0) return 0;
-    hipLaunchKernelGGL(k_aos_in, dim3(blocks(cnt)), dim3(256), 0, c->stream, pos, cnt, c->x.p, c->y.p, c->g.p);
+    hipLaunchKernelGGL(k_aos_in, dim3(blocks(cnt)), dim3(256), 0, c->stream, (const char *)pos, g_compat.vb, cnt,
+                       c->x.p, c->y.p, c->g.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }
@@ -157,7 +179,7 @@ int stage_in(vx_case *c, const Vortex *pos, int64_t cnt) {
 int stage_out(vx_case *c, Vortex *pos, int64_t from, int64_t to) {
     if (to <= from) return 0;
     hipLaunchKernelGGL(k_aos_out, dim3(blocks(to - from)), dim3(256), 0, c->stream, c->x.p, c->y.p, c->g.p, from, to,
-                       pos);
+                       (char *)pos, g_compat.vb);
     VX_CHECK(hipGetLastError());
     return 0;
 }
@@ -204,6 +226,13 @@ int vx_compat_set_precision(int precision) {
 
 int init_device_conf_values(void) {
     g_compat.dirty = true;
+    g_compat.vb = sizeof(Vortex);
+    return 0;
+}
+
+int vx_compat_init_no_tree(void) {
+    g_compat.dirty = true;
+    g_compat.vb = 24;
     return 0;
 }
 
@@ -253,11 +282,12 @@ TVars *load_matrix(size_t *p) {
 
 int allocate_arrays(Vortex **p_host, Vortex **p_dev, PVortex **v_host, PVortex **v_dev, TVars **d_dev, size_t size) {
     VX_ARG(p_host && p_dev && v_host && v_dev && d_dev && size, "wrong parameters");
-    *p_host = (Vortex *)calloc(size, sizeof(Vortex));
+    const size_t vb = (size_t)g_compat.vb;
+    *p_host = (Vortex *)calloc(size, vb);
     *v_host = (PVortex *)calloc(size, sizeof(PVortex));
     VX_ARG(*p_host && *v_host, "host allocation");
-    VX_CHECK(hipMalloc((void **)p_dev, size * sizeof(Vortex)));
-    VX_CHECK(hipMemset(*p_dev, 0, size * sizeof(Vortex)));
+    VX_CHECK(hipMalloc((void **)p_dev, size * vb));
+    VX_CHECK(hipMemset(*p_dev, 0, size * vb));
     VX_CHECK(hipMalloc((void **)d_dev, size * sizeof(TVars)));
     VX_CHECK(hipMemset(*d_dev, 0, size * sizeof(TVars)));
     VX_CHECK(hipMalloc((void **)v_dev, size * sizeof(PVortex)));
@@ -268,7 +298,7 @@ int allocate_arrays(Vortex **p_host, Vortex **p_dev, PVortex **v_host, PVortex *
 int randomize_tail(Vortex **p_dev, size_t new_size, size_t increased) {
     VX_ARG(p_dev && *p_dev && increased <= new_size, "wrong parameters");
     if (!increased) return 0;
-    hipLaunchKernelGGL(k_aos_tail, dim3(blocks((int64_t)increased)), dim3(256), 0, 0, *p_dev,
+    hipLaunchKernelGGL(k_aos_tail, dim3(blocks((int64_t)increased)), dim3(256), 0, 0, (char *)*p_dev, g_compat.vb,
                        (int64_t)(new_size - increased), (int64_t)new_size);
     VX_CHECK(hipGetLastError());
     VX_CHECK(hipDeviceSynchronize());
@@ -280,12 +310,12 @@ int incr_vort_quant(Vortex **p_host, Vortex **p_dev, PVortex **v_host, PVortex *
     const size_t inc = conf.inc_step ? conf.inc_step : 8192;
     if (*p_host && *p_dev && *v_host && *v_dev && *d_dev) {
         Vortex *nd = nullptr;
-        const size_t ns = *size + inc;
-        VX_CHECK(hipMalloc((void **)&nd, ns * sizeof(Vortex)));
-        VX_CHECK(hipMemcpy(nd, *p_dev, *size * sizeof(Vortex), hipMemcpyDeviceToDevice));
+        const size_t ns = *size + inc, vb = (size_t)g_compat.vb;
+        VX_CHECK(hipMalloc((void **)&nd, ns * vb));
+        VX_CHECK(hipMemcpy(nd, *p_dev, *size * vb, hipMemcpyDeviceToDevice));
         free(*p_host);
         free(*v_host);
-        *p_host = (Vortex *)calloc(ns, sizeof(Vortex));
+        *p_host = (Vortex *)calloc(ns, vb);
         *v_host = (PVortex *)calloc(ns, sizeof(PVortex));
         VX_CHECK(hipFree(*p_dev));
         VX_CHECK(hipFree(*d_dev));
@@ -401,12 +431,12 @@ int velocity_control(Vortex *pos, TVctr *V_inf, size_t n, Vortex *Contr_points, 
     VX_ARG(c, "no case yet: call vort_creation, Speed or Step first");
     double vinf[2];
     if (read_dev(c->stream, vinf, V_inf, sizeof(vinf))) return 1;
-    std::vector<Vortex> cp(n_contr);
-    if (n_contr) VX_CHECK(hipMemcpy(cp.data(), Contr_points, sizeof(Vortex) * n_contr, hipMemcpyDeviceToHost));
+    HostPts cp;
+    if (read_pts(cp, Contr_points, n_contr)) return 1;
     std::vector<double> px(n_contr), py(n_contr), u(n_contr), w(n_contr);
     for (size_t k = 0; k < n_contr; ++k) {
-        px[k] = cp[k].r[0];
-        py[k] = cp[k].r[1];
+        px[k] = cp.rec(k)[0];
+        py[k] = cp.rec(k)[1];
     }
     const int64_t src = ceil32((int64_t)n);
     if (vx_case_reserve(c, src) || stage_in(c, pos, src)) return 1;
@@ -425,36 +455,36 @@ int velocity_control(Vortex *pos, TVctr *V_inf, size_t n, Vortex *Contr_points, 
 namespace {
 
 // save_vel_to_file (src/unit_1.cu:616-631)
-void compat_write_vels(const std::vector<Vortex> &pos, const std::vector<PVortex> &vel, int stage) {
+void compat_write_vels(const HostPts &pos, const std::vector<PVortex> &vel, int stage) {
     char fn[256];
     snprintf(fn, sizeof(fn), "output/vels/Vel%06d_%d.txt", current_step, stage);
     FILE *f = fopen(fn, "w");
     if (!f) return;
-    fprintf(f, "%zu\n", pos.size());
-    for (size_t i = 0; i < pos.size(); ++i)
-        fprintf(f, "%zu %lf %lf %lf %lf\n", i, pos[i].r[0], pos[i].r[1], vel[i].v[0], vel[i].v[1]);
+    fprintf(f, "%zu\n", pos.m);
+    for (size_t i = 0; i < pos.m; ++i)
+        fprintf(f, "%zu %lf %lf %lf %lf\n", i, pos.rec(i)[0], pos.rec(i)[1], vel[i].v[0], vel[i].v[1]);
     fclose(f);
 }
 
 // step_Kernel on the points with null force pointers: positions and flags of
 // the caller's device AoS points moved by the device velocities.
-int compat_points_move(vx_case *c, Vortex *pts_dev, std::vector<Vortex> &pts, const std::vector<PVortex> &vel) {
-    const size_t m = pts.size();
+int compat_points_move(vx_case *c, Vortex *pts_dev, HostPts &pts, const std::vector<PVortex> &vel) {
+    const size_t m = pts.m;
     std::vector<double> x(m), y(m), g(m), u(m), v(m);
     for (size_t i = 0; i < m; ++i) {
-        x[i] = pts[i].r[0];
-        y[i] = pts[i].r[1];
-        g[i] = pts[i].g;
+        x[i] = pts.rec(i)[0];
+        y[i] = pts.rec(i)[1];
+        g[i] = pts.rec(i)[2];
         u[i] = vel[i].v[0];
         v[i] = vel[i].v[1];
     }
     if (vx_points_move(c, x.data(), y.data(), g.data(), u.data(), v.data(), (int64_t)m)) return 1;
     for (size_t i = 0; i < m; ++i) {
-        pts[i].r[0] = x[i];
-        pts[i].r[1] = y[i];
-        pts[i].g = g[i];
+        pts.rec(i)[0] = x[i];
+        pts.rec(i)[1] = y[i];
+        pts.rec(i)[2] = g[i];
     }
-    VX_CHECK(hipMemcpy(pts_dev, pts.data(), sizeof(Vortex) * m, hipMemcpyHostToDevice));
+    if (m) VX_CHECK(hipMemcpy(pts_dev, pts.raw.data(), pts.raw.size(), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -467,10 +497,10 @@ int first_speed(Vortex *pos, TVctr *V_inf, size_t n, Vortex *second_pos, PVortex
     vx_case *c = compat_case(panels);
     if (!c) return 1;
     const size_t m = *n_second;
-    std::vector<Vortex> pts(m);
+    HostPts pts;
     std::vector<PVortex> vel(m);
-    VX_CHECK(hipMemcpy(pts.data(), second_pos, sizeof(Vortex) * m, hipMemcpyDeviceToHost));
-    VX_CHECK(hipMemcpy(vel.data(), v_second, sizeof(PVortex) * m, hipMemcpyDeviceToHost));
+    if (read_pts(pts, second_pos, m)) return 1;
+    if (m) VX_CHECK(hipMemcpy(vel.data(), v_second, sizeof(PVortex) * m, hipMemcpyDeviceToHost));
     compat_write_vels(pts, vel, 5);
     return compat_points_move(c, second_pos, pts, vel);
 }
@@ -485,15 +515,17 @@ int second_speed(Vortex *pos, TVctr *V_inf, size_t n, Vortex *second_pos, PVorte
     const size_t m = *n_second;
     if (current_step == first_step)
         VX_CHECK(hipMemcpy(v_second, v_env, sizeof(PVortex) * m, hipMemcpyDeviceToDevice));
-    std::vector<Vortex> pts(m);
+    HostPts pts;
     std::vector<PVortex> env(m), vel(m);
-    VX_CHECK(hipMemcpy(pts.data(), second_pos, sizeof(Vortex) * m, hipMemcpyDeviceToHost));
-    VX_CHECK(hipMemcpy(env.data(), v_env, sizeof(PVortex) * m, hipMemcpyDeviceToHost));
-    VX_CHECK(hipMemcpy(vel.data(), v_second, sizeof(PVortex) * m, hipMemcpyDeviceToHost));
+    if (read_pts(pts, second_pos, m)) return 1;
+    if (m) {
+        VX_CHECK(hipMemcpy(env.data(), v_env, sizeof(PVortex) * m, hipMemcpyDeviceToHost));
+        VX_CHECK(hipMemcpy(vel.data(), v_second, sizeof(PVortex) * m, hipMemcpyDeviceToHost));
+    }
     compat_write_vels(pts, env, 4);
     std::vector<double> g(m), ue(m), ve(m), u(m), v(m);
     for (size_t i = 0; i < m; ++i) {
-        g[i] = pts[i].g;
+        g[i] = pts.rec(i)[2];
         ue[i] = env[i].v[0];
         ve[i] = env[i].v[1];
         u[i] = vel[i].v[0];

@@ -147,6 +147,7 @@ struct vx_case {
     vx::DBuf<double> L, M;          // (P+1)^2
     bool have_matrix = false;       // the births can be solved (inverse or LU)
     bool have_inverse = false;      // M holds the inverse
+    bool census = false;            // tree diffusion also counts in-radius pairs
     // vortex state, FP64 SoA
     int64_t n = 0, cap = 0;
     double d_g = 0.0;               // circulation absorbed by the last Step (host copy)
@@ -188,7 +189,7 @@ struct vx_case {
     bool devcount = false;
     int64_t n_ub = 0;
     // launch shapes (vx_case_autotune / vx_case_set_tuning)
-    vx_tuning tune{0, 0, -1, 0, 0.0, 0.0};
+    vx_tuning tune{0, 0, -1, 0, 0, 0, 0.0, 0.0};
     vx::DBuf<double> gr_ring;                         // 4 forces + 1 count per step
     int64_t gr_cap = 0;
     hipGraph_t graph = nullptr;
@@ -307,6 +308,8 @@ int tree_diffusion_far(vx_case *c, hipStream_t st);
 int tree_diffusion_main(vx_case *c);
 int tree_stats(vx_case *c, vx_tree_info *info);
 int tree_copy_order(vx_case *c, uint32_t *keys, int32_t *perm, int64_t count);
+int64_t tree_far_hint(vx_case *c);
+int tree_set_far_hint(vx_case *c, int64_t h);
 void tree_release(vx_case *c);
 // vx_step.hip
 // The step stages run on a device-resident vortex count *dn (dcount[1]):

@@ -91,6 +91,8 @@ int read_conf(const std::string &path, Conf &c, std::string *err) {
         NUM(diff_cut);
         NUM(birth_offset);
         NUM(matrix_precise);
+        NUM(checkpoint_step);
+        STR(checkpoint_file);
         else if (starts(buf, "precision")) {
             std::string v;
             scan_str(buf, v);

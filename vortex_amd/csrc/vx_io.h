@@ -32,6 +32,8 @@ struct Conf {
     double diff_cut = 0;          // "diff_cut c"
     double birth_offset = 0;      // "birth_offset r" (multiple of the panel length)
     int matrix_precise = 0;       // "matrix_precise 1": M.txt with 17 digits
+    int64_t checkpoint_step = 0;  // "checkpoint_step k": exact checkpoint every k steps
+    std::string checkpoint_file;  // "checkpoint_file path": restart from an exact checkpoint
 };
 
 // Parses "key value # comment" lines; a key matches when the line starts with

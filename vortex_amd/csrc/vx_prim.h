@@ -271,68 +271,88 @@ struct WriteIndex {  // out[pos] = i for flagged items (ordered selection of ind
 };
 
 // ---- LSD radix sort: 32-bit keys, 32-bit values, 8-bit digits ---------------------------
-// Four stable passes of two launches each.  The histogram launch counts each
-// tile's digits in LDS, stores them digit-major and adds them to the pass's
-// digit totals.  The scatter launch starts each digit of its tile at the
-// digit's base (an exclusive scan of the totals, redone by every work-group)
-// plus the counts of the earlier tiles (a sum over its row), and ranks each
-// key within the tile by its wave's count of earlier lanes with the same digit
-// (8 ballots) plus the counts of the tile's earlier waves and rounds.  Stable,
-// so equal keys keep their input order.  No work-group waits on another.
+// Four stable passes.  Per pass: the histogram launch counts each tile's
+// digits (one LDS add per run of equal digits in a wave, found with 8
+// ballots) and stores them digit-major, hist[d * ntiles + tile]; one
+// exclusive scan of that array (scan_values) gives every (digit, tile) its
+// output offset, and the scan's total, n, lands after it, so the tile's count
+// of digit d is off[d * ntiles + tile + 1] - off[d * ntiles + tile].  The
+// scatter launch ranks each key within its tile (the wave's earlier lanes
+// with the same digit, plus the tile's earlier waves and rounds), places the
+// tile in LDS sorted by digit, and writes it out in that order, so each
+// digit's run of the tile goes to consecutive addresses.  Stable: equal keys
+// keep their input order.  No work-group waits on another.
 constexpr int kRadixBits = 8;
 constexpr int kBins = 1 << kRadixBits;
 constexpr int kSortIPT = 8;                 // keys per thread per tile
 constexpr int kSortTile = kT * kSortIPT;
 
 inline int64_t sort_tiles_of(int64_t n) { return n > 0 ? (n + kSortTile - 1) / kSortTile : 0; }
-// int32 entries of scratch radix_sort_pairs needs for n keys
-inline int64_t radix_tmp_size(int64_t n) { return (int64_t)kBins * (sort_tiles_of(n) + 4); }
+// int32 entries of scratch radix_sort_pairs needs for n keys: the offsets
+// (plus the total) and the offset scan's tile sums
+inline int64_t radix_tmp_size(int64_t n) {
+    const int64_t h = (int64_t)kBins * sort_tiles_of(n);
+    return h + 1 + tiles_of(h) + 1;
+}
+
+// Lanes of this wave whose digit equals this lane's (among lanes with ok).
+__device__ __forceinline__ uint64_t digit_peers(int d, bool ok) {
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+        const uint64_t bb = __ballot((d >> b) & 1);
+        peers &= ((d >> b) & 1) ? bb : ~bb;
+    }
+    return peers;
+}
 
 template <int DUMMY = 0>
 __global__ __launch_bounds__(kT) void k_radix_hist(const uint32_t *__restrict__ key, int64_t n, int shift,
-                                                   int32_t *__restrict__ hist, int64_t ntiles,
-                                                   int32_t *__restrict__ totals) {
+                                                   int32_t *__restrict__ hist, int64_t ntiles) {
     __shared__ int32_t h[kBins];
     h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
-#pragma unroll 4
+#pragma unroll 2
     for (int r = 0; r < kSortIPT; ++r) {
         const int64_t i = base + r * kT + threadIdx.x;
-        if (i < n) atomicAdd(&h[(key[i] >> shift) & (kBins - 1)], 1);
+        const bool ok = i < n;
+        const int d = ok ? (int)((key[i] >> shift) & (kBins - 1)) : 0;
+        const uint64_t peers = digit_peers(d, ok);
+        if (ok && (peers & lanes_below()) == 0) atomicAdd(&h[d], (int32_t)__popcll(peers));
     }
     __syncthreads();
-    const int32_t c = h[threadIdx.x];
-    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = c;
-    if (c) atomicAdd(totals + threadIdx.x, c);
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
 template <int DUMMY = 0>
 __global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
-                                                      int64_t n, int shift, const int32_t *__restrict__ hist,
-                                                      int64_t ntiles, const int32_t *__restrict__ totals,
-                                                      uint32_t *__restrict__ kout, int32_t *__restrict__ vout) {
-    __shared__ int32_t run[kBins];    // offset of the next key of each digit
+                                                      int64_t n, int shift, const int32_t *__restrict__ off,
+                                                      int64_t ntiles, uint32_t *__restrict__ kout,
+                                                      int32_t *__restrict__ vout) {
+    __shared__ uint32_t sk[kSortTile];   // the tile sorted by digit
+    __shared__ int32_t sv[kSortTile];
+    __shared__ int32_t loc[kBins];       // first slot of each digit in sk
+    __shared__ int32_t gbase[kBins];     // first output position of each digit's run
+    __shared__ int32_t run[kBins];       // next free slot of each digit in sk
     __shared__ int32_t wcnt[kW][kBins];
     __shared__ int32_t ws[kW];
     const int w = threadIdx.x >> 6;
     {
-        // digit base: exclusive scan of the totals over the work-group
-        const int32_t tot = totals[threadIdx.x];
-        const int32_t inc = wave_inclusive(tot, OpSum());
+        const int64_t q = (int64_t)threadIdx.x * ntiles + blockIdx.x;
+        const int32_t g = off[q], cnt = off[q + 1] - g;
+        const int32_t inc = wave_inclusive(cnt, OpSum());
         if (lane_id() == 63) ws[w] = inc;
-        // this tile's start within the digit: the earlier tiles' counts
-        const int32_t *row = hist + (int64_t)threadIdx.x * ntiles;
-        int32_t before = 0;
-        for (int64_t t = 0; t < (int64_t)blockIdx.x; ++t) before += row[t];
         __syncthreads();
-        int32_t pre = inc - tot;
-        for (int q = 0; q < w; ++q) pre += ws[q];
-        run[threadIdx.x] = pre + before;
+        int32_t pre = inc - cnt;
+        for (int k = 0; k < w; ++k) pre += ws[k];
+        loc[threadIdx.x] = pre;
+        run[threadIdx.x] = pre;
+        gbase[threadIdx.x] = g;
     }
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
     for (int r = 0; r < kSortIPT; ++r) {
-        for (int q = 0; q < kW; ++q) wcnt[q][threadIdx.x] = 0;
+        for (int k = 0; k < kW; ++k) wcnt[k][threadIdx.x] = 0;
         const int64_t i = base + r * kT + threadIdx.x;
         const bool ok = i < n;
         uint32_t k = 0;
@@ -342,13 +362,7 @@ __global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict
             v = vin[i];
         }
         const int d = (int)((k >> shift) & (kBins - 1));
-        // lanes of this wave holding the same digit
-        uint64_t peers = __ballot(ok);
-#pragma unroll
-        for (int b = 0; b < kRadixBits; ++b) {
-            const uint64_t bb = __ballot((d >> b) & 1);
-            peers &= ((d >> b) & 1) ? bb : ~bb;
-        }
+        const uint64_t peers = digit_peers(d, ok);
         const int32_t below = (int32_t)__popcll(peers & lanes_below());
         __syncthreads();  // wcnt cleared, run up to date
         if (ok && below == 0) wcnt[w][d] = (int32_t)__popcll(peers);  // the digit's first lane in this wave
@@ -356,14 +370,23 @@ __global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict
         if (ok) {
             int32_t pos = run[d] + below;
             for (int q = 0; q < w; ++q) pos += wcnt[q][d];
-            kout[pos] = k;
-            vout[pos] = v;
+            sk[pos] = k;
+            sv[pos] = v;
         }
         __syncthreads();
         int32_t add = 0;
 #pragma unroll
         for (int q = 0; q < kW; ++q) add += wcnt[q][threadIdx.x];
         run[threadIdx.x] += add;
+    }
+    __syncthreads();
+    const int64_t m = n - base < kSortTile ? n - base : kSortTile;
+    for (int j = threadIdx.x; j < m; j += kT) {
+        const uint32_t k = sk[j];
+        const int d = (int)((k >> shift) & (kBins - 1));
+        const int64_t pos = (int64_t)gbase[d] + (j - loc[d]);
+        kout[pos] = k;
+        vout[pos] = sv[j];
     }
 }
 
@@ -373,18 +396,19 @@ inline hipError_t radix_sort_pairs(uint32_t *k0, uint32_t *k1, int32_t *v0, int3
                                    hipStream_t st) {
     const int64_t nt = sort_tiles_of(n);
     if (nt == 0) return hipSuccess;
-    int32_t *hist = tmp, *totals = tmp + kBins * nt;  // 4 passes x kBins totals
-    hipError_t e = fill(totals, 4 * kBins, 0, st);
-    if (e != hipSuccess) return e;
+    const int64_t h = (int64_t)kBins * nt;
+    int32_t *off = tmp, *stmp = tmp + h + 1;
     uint32_t *ka = k0, *kb = k1;
     int32_t *va = v0, *vb = v1;
     for (int pass = 0; pass < 32 / kRadixBits; ++pass) {
         const int shift = pass * kRadixBits;
-        int32_t *tot = totals + pass * kBins;
-        hipLaunchKernelGGL(k_radix_hist<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka, n, shift, hist,
-                           nt, tot);
+        hipLaunchKernelGGL(k_radix_hist<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka, n, shift, off,
+                           nt);
+        // in place: each offset is read and then written by the same thread
+        const hipError_t e = scan_values<false>(ArrayValue{off}, h, OpSum(), WriteScan{off}, stmp, off + h, st);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_radix_scatter<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka,
-                           (const int32_t *)va, n, shift, (const int32_t *)hist, nt, (const int32_t *)tot, kb, vb);
+                           (const int32_t *)va, n, shift, (const int32_t *)off, nt, kb, vb);
         uint32_t *tk = ka; ka = kb; kb = tk;
         int32_t *tv = va; va = vb; vb = tv;
     }

@@ -77,7 +77,8 @@ struct TreeBufs {
     int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
     int64_t ngroup = 0;                    // target groups (gfirst)
     int32_t root = 0;
-    int order = 16;
+    int order = 16;  // coefficients stored per node
+    int terms = 16;  // of them used by the far field
     int64_t fcap = 4096;  // domain-scale diffusion targets listed at most (set per diffusion)
     unsigned long long *h_far = nullptr;  // pinned: the last diffusion's domain-scale target count
     hipEvent_t ev = nullptr;                       // read-back completion
@@ -977,6 +978,36 @@ __device__ __forceinline__ void stage_leaf(Src4T<T> *lds, int lane, int64_t j0, 
     __builtin_amdgcn_wave_barrier();
 }
 
+// One source leaf staged like stage_leaf, keeping only the sources within
+// sqrt(R2) of the box [bx0, bx1] x [by0, by1] (in the staged coordinates),
+// packed to the front of the slot in leaf order; returns their count.
+template <typename T>
+__device__ __forceinline__ int stage_leaf_near(Src4T<T> *lds, int lane, int64_t j0, int cnt, const TreeDev &t, T bx0,
+                                               T bx1, T by0, T by1, T R2) {
+    __builtin_amdgcn_wave_barrier();
+    Src4T<T> q;
+    bool keep = false;
+    if (lane < cnt) {
+        if constexpr (sizeof(T) == 4) {
+            q.x = t.fx[j0 + lane];
+            q.y = t.fy[j0 + lane];
+            q.g = t.fg[j0 + lane];
+        } else {
+            q.x = t.sx[j0 + lane];
+            q.y = t.sy[j0 + lane];
+            q.g = t.sg[j0 + lane];
+        }
+        q.pad = T(0);
+        const T ex = q.x < bx0 ? bx0 - q.x : (q.x > bx1 ? q.x - bx1 : T(0));
+        const T ey = q.y < by0 ? by0 - q.y : (q.y > by1 ? q.y - by1 : T(0));
+        keep = ex * ex + ey * ey < R2;
+    }
+    const uint64_t m = __ballot(keep);
+    if (keep) lds[__popcll(m & ((1ull << lane) - 1ull))] = q;
+    __builtin_amdgcn_wave_barrier();
+    return (int)__popcll(m);
+}
+
 // Pair loop over one staged leaf: Biot-Savart sums and, with TRACK, the four
 // smallest squared distances (the target itself included).  Without CLAMP
 // the weight is g / r2: the caller has shown r2 >= eps2 for every pair.
@@ -1110,97 +1141,154 @@ __device__ __forceinline__ float readlane_f(float a, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), l));
 }
 
-// M2L (FP32): far source nodes translated into the target bucket's local
-// expansion about its centre c_T (radius R_T), one node per lane:
+// M2L (FP32) on the matrix cores: far source nodes translated into the
+// target bucket's local expansion about its centre c_T (radius R_T):
 //   F(z) = sum_j g_j / (z - z_j) = sum_l L_l u^l,  u = (z - c_T) / R_T,
-//   L_l += q (-beta)^l sum_k C(k + l, l) alpha^k b_k,
+//   L_l += q (-beta)^l T_l,  T_l = sum_k C(k + l, l) a_k,  a_k = alpha^k b_k,
 //   q = 1 / (c_T - c_S),  alpha = R_S q,  beta = R_T q,
 // with b_k the node's multipole (about c_S, scaled by R_S).  It follows from
 // 1/(D + w)^(k+1) = sum_l C(k + l, l) (-w)^l / D^(k+l+1), D = c_T - c_S,
 // w = z - c_T.  The opening test keeps (R_S + R_T) / |D| below theta, so the
-// truncation at ORDER terms on both sides leaves about theta^ORDER of the
-// node's field.  The nodes are fq[0, count) of this wave's queue; gap
-// takes the lane's least |c_T - c_S| - R_S - R_T.  The
-// lanes' terms are summed over the wave through LDS four coefficients at a
-// time (tr: this wave's 4 x 64 slots): lane (c, e), c = lane & 3, adds the
-// terms of lanes 4e .. 4e + 3 for coefficient 4 j + c into acc[j]; the
-// partial sums over e are combined once, by l2p_coeffs().
-template <int ORDER>
-__device__ __noinline__ void m2l_batch(const TreeDev &t, const int32_t *fq, int count, int lane, double cTx,
-                                          double cTy, float rT, float2 (*tr)[64], float2 (&acc)[ORDER / 4],
-                                          float &gap) {
-    const int node = lane < count ? fq[lane] : -1;
-    float akr[ORDER], aki[ORDER];  // alpha^k b_k
-    float qbr = 0.f, qbi = 0.f;    // q (-beta)^l
-    float mbr = 0.f, mbi = 0.f;    // -beta
+// truncation at P terms on both sides leaves about theta^P of the node's
+// field.
+//
+// The contraction over k is a product with the fixed binomial matrix,
+// T (P x 16 nodes) = B (P x P) A (P x 16 nodes), run as
+// v_mfma_f32_16x16x4_f32 (operand A: lane i + 16 k holds A[i][k]; B: lane
+// j + 16 k holds B[k][j]; D: lane j + 16 (i / 4), register i % 4 holds
+// D[i][j]) over 16 nodes at a time, real and imaginary parts of a_k as two
+// products.  Lane j + 16 g works on node j of the 16: it forms a_k for
+// k = g mod 4 (the B operand of each 4-deep step), and receives T_l for
+// l = 16 mt + 4 g + r (r = 0..3), which it weights by q (-beta)^l and adds
+// to acc[mt][r].  The lanes' sums over nodes are combined once, by
+// l2p_coeffs().  Rows l >= P of the binomial operand are zero.  gap takes
+// the lane's least |c_T - c_S| - R_S - R_T.
+template <int P>
+struct M2LBinom {
+    float v[P][P];  // v[k][l] = C(k + l, l)
+};
+template <int P>
+constexpr M2LBinom<P> m2l_binom() {
+    M2LBinom<P> b{};
+    for (int k = 0; k < P; ++k)
+        for (int l = 0; l < P; ++l) b.v[k][l] = (float)binom(k + l, l);
+    return b;
+}
+__device__ const M2LBinom<16> kM2LBin16 = m2l_binom<16>();
+__device__ const M2LBinom<20> kM2LBin20 = m2l_binom<20>();
+__device__ const M2LBinom<32> kM2LBin32 = m2l_binom<32>();
+template <int P>
+__device__ __forceinline__ const M2LBinom<P> &m2l_bin();
+template <>
+__device__ __forceinline__ const M2LBinom<16> &m2l_bin<16>() { return kM2LBin16; }
+template <>
+__device__ __forceinline__ const M2LBinom<20> &m2l_bin<20>() { return kM2LBin20; }
+template <>
+__device__ __forceinline__ const M2LBinom<32> &m2l_bin<32>() { return kM2LBin32; }
+
+typedef float m2l_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int P>
+struct M2LShape {
+    static constexpr int KS = P / 4;          // 4-deep steps over k
+    static constexpr int MT = (P + 15) / 16;  // 16-row tiles over l
+};
+
+template <int P, int STRIDE>
+__device__ __forceinline__ void m2l_mfma(const TreeDev &t, const int32_t *fq, int count, int lane, double cTx,
+                                         double cTy, float rT, float2 (&acc)[M2LShape<P>::MT][4], float &gap) {
+    constexpr int KS = M2LShape<P>::KS, MT = M2LShape<P>::MT;
+    static_assert(P % 4 == 0 && P <= STRIDE, "terms: a multiple of 4 within the stored coefficients");
+    const int j = lane & 15, g = lane >> 4;
+    // binomial operand of this lane: A[i][kk] = C(k + l, l), k = 4 m + kk, l = 16 mt + i
+    float ab[MT][KS];
+    {
+        const M2LBinom<P> &B = m2l_bin<P>();
 #pragma unroll
-    for (int k = 0; k < ORDER; ++k) akr[k] = aki[k] = 0.f;
-    if (node >= 0) {
-        const NodeRec S = t.rec[node];
-        const float dx = (float)(cTx - S.cx), dy = (float)(cTy - S.cy);
-        const float d2 = dx * dx + dy * dy;
-        const float inv = __builtin_amdgcn_rcpf(d2);
-        qbr = dx * inv;
-        qbi = -dy * inv;
-        const float rs = fmaxf((float)S.r, 1e-30f);
-        // nearest possible particle of the node: |c_T - c_S| - R_S - R_T
-        gap = fminf(gap, __builtin_amdgcn_sqrtf(d2) - rs - rT);
-        const float alr = rs * qbr, ali = rs * qbi;
-        mbr = -rT * qbr;
-        mbi = -rT * qbi;
-        const float2 *b = t.cf + (int64_t)node * ORDER;
-        float pr = 1.f, pi = 0.f;
+        for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int k = 0; k < ORDER; ++k) {
-            const float2 bk = b[k];
-            akr[k] = pr * bk.x - pi * bk.y;
-            aki[k] = pr * bk.y + pi * bk.x;
-            const float npr = pr * alr - pi * ali;
-            pi = pr * ali + pi * alr;
-            pr = npr;
-        }
-    }
-    const int c4 = lane & 3, e4 = lane >> 2;
-#pragma unroll
-    for (int l = 0; l < ORDER; ++l) {
-        float sr = 0.f, si = 0.f;
-#pragma unroll
-        for (int k = 0; k < ORDER; ++k) {
-            const float c = (float)binom(k + l, l);
-            sr += c * akr[k];
-            si += c * aki[k];
-        }
-        tr[l & 3][lane] = make_float2(qbr * sr - qbi * si, qbr * si + qbi * sr);
-        const float nqr = qbr * mbr - qbi * mbi;
-        qbi = qbr * mbi + qbi * mbr;
-        qbr = nqr;
-        if ((l & 3) == 3) {
-            __builtin_amdgcn_wave_barrier();
-            float2 a = acc[l >> 2];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float2 q = tr[c4][4 * e4 + j];
-                a.x += q.x;
-                a.y += q.y;
+            for (int m = 0; m < KS; ++m) {
+                const int l = 16 * mt + j;
+                ab[mt][m] = l < P ? B.v[4 * m + g][l] : 0.f;
             }
-            acc[l >> 2] = a;
-            __builtin_amdgcn_wave_barrier();
+    }
+    for (int base = 0; base < count; base += 16) {
+        const int node = base + j < count ? fq[base + j] : -1;
+        float2 q = make_float2(0.f, 0.f), al = q, nb = q;
+        if (node >= 0) {
+            const NodeRec S = t.rec[node];
+            const float dx = (float)(cTx - S.cx), dy = (float)(cTy - S.cy);
+            const float d2 = dx * dx + dy * dy;
+            const float inv = __builtin_amdgcn_rcpf(d2);
+            q = make_float2(dx * inv, -dy * inv);
+            const float rs = fmaxf((float)S.r, 1e-30f);
+            // nearest possible particle of the node: |c_T - c_S| - R_S - R_T
+            gap = fminf(gap, __builtin_amdgcn_sqrtf(d2) - rs - rT);
+            al = make_float2(rs * q.x, rs * q.y);
+            nb = make_float2(-rT * q.x, -rT * q.y);
+        }
+        // a_k = alpha^k b_k for k = g, g + 4, ...
+        const float2 a2 = cmulf(al, al), a3 = cmulf(a2, al), a4 = cmulf(a2, a2);
+        float2 pw = g == 0 ? make_float2(1.f, 0.f) : (g == 1 ? al : (g == 2 ? a2 : a3));
+        m2l_f4 Tr[MT], Ti[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            Tr[mt] = m2l_f4{0.f, 0.f, 0.f, 0.f};
+            Ti[mt] = m2l_f4{0.f, 0.f, 0.f, 0.f};
+        }
+        const float2 *b = t.cf + (int64_t)(node >= 0 ? node : 0) * STRIDE + g;
+#pragma unroll
+        for (int m = 0; m < KS; ++m) {
+            const float2 bk = node >= 0 ? b[4 * m] : make_float2(0.f, 0.f);
+            const float2 ak = cmulf(pw, bk);
+            pw = cmulf(pw, a4);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                Tr[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab[mt][m], ak.x, Tr[mt], 0, 0, 0);
+                Ti[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab[mt][m], ak.y, Ti[mt], 0, 0, 0);
+            }
+        }
+        // weights q (-beta)^l, l = 16 mt + 4 g + r
+        const float2 n2 = cmulf(nb, nb), n4 = cmulf(n2, n2), n8 = cmulf(n4, n4), n12 = cmulf(n8, n4);
+        float2 w = cmulf(q, g == 0 ? make_float2(1.f, 0.f) : (g == 1 ? n4 : (g == 2 ? n8 : n12)));
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float tr_ = Tr[mt][r], ti_ = Ti[mt][r];
+                acc[mt][r].x += w.x * tr_ - w.y * ti_;
+                acc[mt][r].y += w.x * ti_ + w.y * tr_;
+                w = cmulf(w, nb);
+            }
+            if (mt + 1 < MT) w = cmulf(w, n12);  // from (-beta)^(16 mt + 4 g + 4) to the next tile's start
         }
     }
 }
 
-// The local coefficients: acc[j] summed over the lanes sharing lane & 3, so
-// that every lane with lane & 3 == c holds L_{4 j + c}.
-template <int ORDER>
-__device__ __forceinline__ void l2p_coeffs(float2 (&acc)[ORDER / 4]) {
+// The local coefficients: acc summed over the 16 lanes of each group g, so
+// that every lane of group g holds L_l for l = 16 mt + 4 g + r in acc[mt][r].
+template <int P>
+__device__ __forceinline__ void l2p_coeffs(float2 (&acc)[M2LShape<P>::MT][4]) {
 #pragma unroll
-    for (int j = 0; j < ORDER / 4; ++j) {
+    for (int mt = 0; mt < M2LShape<P>::MT; ++mt)
 #pragma unroll
-        for (int o = 4; o < 64; o <<= 1) {
-            acc[j].x += __shfl_xor(acc[j].x, o, 64);
-            acc[j].y += __shfl_xor(acc[j].y, o, 64);
-        }
-    }
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                acc[mt][r].x += __shfl_xor(acc[mt][r].x, o, 64);
+                acc[mt][r].y += __shfl_xor(acc[mt][r].y, o, 64);
+            }
+}
+
+// L_l of the combined coefficients, broadcast from lane 16 g.
+template <int P>
+__device__ __forceinline__ float2 local_coeff(const float2 (&acc)[M2LShape<P>::MT][4], int l) {
+    const int mt = l >> 4, g = (l >> 2) & 3, r = l & 3;
+    return make_float2(readlane_f(acc[mt][r].x, 16 * g), readlane_f(acc[mt][r].y, 16 * g));
 }
 
 // ---- Biot-Savart + nearest-3 --------------------------------------------------
@@ -1216,7 +1304,11 @@ __device__ __forceinline__ int64_t xcd_block() {
 constexpr int kBsStack = 256;  // traversal stack entries per wave
 constexpr int kFarQ = 512;     // FP32: far nodes queued per wave before any M2L
 
-template <typename T, int ORDER>
+// Targets of a wave: one bucket (GROUPS false), or one target group, a run of
+// up to 64 consecutive particles of the Morton order within a cell, which
+// fills more lanes than a bucket (buckets hold 64 at most, about 40 on
+// average) at the price of a wider box for the opening test.
+template <typename T, int ORDER, int STRIDE, bool GROUPS>
 __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double eps, double vinf_x, double vinf_y,
                                         double *__restrict__ u, double *__restrict__ v, double *__restrict__ d,
                                         int32_t *__restrict__ fb, double *__restrict__ fbr2,
@@ -1229,24 +1321,47 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     __shared__ int32_t fq_all[4][kFarQ];  // FP32: far nodes waiting for M2L
     __shared__ int32_t stk_all[4][kBsStack];
     __shared__ int32_t lq_all[4][64];     // near leaves of the popped nodes
-    __shared__ float2 tr_all[4][4][64];   // FP32: M2L sums over the wave
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
     int32_t *fq = fq_all[threadIdx.x >> 6];
-    float2(*tr)[64] = tr_all[threadIdx.x >> 6];
-    // targets: one bucket per wave (a compact node, which keeps the MAC's
-    // opening radius small)
-    const int64_t leaf = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
-    if (leaf >= t.nleaf) return;
-    const NodeRec L = t.rec[leaf];
-    const int64_t ti = (int64_t)(-1 - L.a) + lane;
-    const int nvalid = L.b;
-    const bool ok = lane < nvalid;
-    const double cAx = L.cx, cAy = L.cy, rA = L.r;
+    const int64_t wv = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
+    if (wv >= (GROUPS ? t.ngroup : t.nleaf)) return;
+    int64_t ti;
+    int nvalid;
+    bool ok;
+    double cAx, cAy, rA;
+    int leaf;  // the bucket summed first (holding the targets, or the group's first)
+    if constexpr (GROUPS) {
+        const TargetGroup G = target_group(t, wv, lane);
+        ti = G.ti;
+        nvalid = G.nvalid;
+        ok = G.ok;
+        cAx = G.cx;
+        cAy = G.cy;
+        rA = G.r;
+        // bucket of particle f: the bucket starts before it, less one unless f starts one
+        const int64_t f = t.gfirst[wv];
+        leaf = t.nleaf == 1 ? 0 : t.bscan[f] + t.bstart[f] - 1;
+    } else {
+        leaf = (int)wv;
+        const NodeRec L = t.rec[leaf];
+        ti = (int64_t)(-1 - L.a) + lane;
+        nvalid = L.b;
+        ok = lane < nvalid;
+        cAx = L.cx;
+        cAy = L.cy;
+        rA = L.r;
+    }
     T ax, ay;
     if constexpr (F32) {
-        ax = ok ? t.fx[ti] : 0.f;
-        ay = ok ? t.fy[ti] : 0.f;
+        // relative to the targets' centre (the bucket centre for a bucket)
+        if constexpr (GROUPS) {
+            ax = ok ? (float)(t.sx[ti] - cAx) : 0.f;
+            ay = ok ? (float)(t.sy[ti] - cAy) : 0.f;
+        } else {
+            ax = ok ? t.fx[ti] : 0.f;
+            ay = ok ? t.fy[ti] : 0.f;
+        }
     } else {
         ax = ok ? t.sx[ti] : cAx;
         ay = ok ? t.sy[ti] : cAy;
@@ -1259,20 +1374,28 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     T m0 = T(1e5), m1 = T(1e5), m2 = T(1e5), m3 = T(1e5);
     T farmin = T(1e30);         // FP64: per target, nearest point of a far node
     double farmin_w = 1e30;     // FP32: per bucket, |c_T - c_S| - R_S - R_T
-    float gap = 1e30f;          // FP32: this lane's share of it (m2l_batch)
+    float gap = 1e30f;          // FP32: this lane's share of it (m2l_mfma)
     int nq = 0;                 // FP32: far nodes queued
-    float2 lacc[ORDER / 4];     // FP32: local coefficients, partial (m2l_batch)
+    float2 lacc[M2LShape<ORDER>::MT][4];  // FP32: local coefficients, partial (m2l_mfma)
 #pragma unroll
-    for (int j = 0; j < ORDER / 4; ++j) lacc[j] = make_float2(0.f, 0.f);
+    for (int mt = 0; mt < M2LShape<ORDER>::MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) lacc[mt][r] = make_float2(0.f, 0.f);
     const float rT = fmaxf((float)rA, 1e-30f);  // FP32: kTiny would round to zero
     // the own bucket first: its particles are usually the nearest, which
     // tightens the bound that lets most other leaves skip the nearest-3 updates
     {
-        stage_leaf<T>(lds, lane, -1 - L.a, nvalid, t);
+        const NodeRec L = t.rec[leaf];
+        stage_leaf<T>(lds, lane, -1 - L.a, L.b, t);
+        T ox = ax, oy = ay;
+        if constexpr (F32 && GROUPS) {
+            ox = ax - (float)(L.cx - cAx);
+            oy = ay - (float)(L.cy - cAy);
+        }
         T pu = 0, pv = 0;
-        p2p_leaf<T, true>(lds, nvalid, ax, ay, eps2, pu, pv, m0, m1, m2, m3);
+        p2p_leaf<T, true>(lds, L.b, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
         __builtin_amdgcn_wave_barrier();
-        npairs += nvalid;
+        npairs += L.b;
         su += (double)pu;
         sv += (double)pv;
     }
@@ -1287,7 +1410,22 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     int32_t *lq = lq_all[threadIdx.x >> 6];
     int sp = 1;
     if (lane == 0) stk[0] = t.root;
-    while (sp > 0) {
+    while (sp > 0 || (F32 && nq > 0)) {
+        if constexpr (F32) {
+            // queued far nodes go to M2L into the bucket's local expansion
+            // after the traversal, 64 at a time; only a nearly full queue
+            // (rare: a bucket meets about 150 far nodes at 2^20) is
+            // translated before it ends
+            if (sp == 0 || nq > kFarQ - 64) {
+                const int nb_ = nq < 64 ? nq : 64;
+                __builtin_amdgcn_wave_barrier();
+                m2l_mfma<ORDER, STRIDE>(t, fq + nq - nb_, nb_, lane, cAx, cAy, rT, lacc, gap);
+                __builtin_amdgcn_wave_barrier();
+                cm2l += (unsigned long long)nb_;
+                nq -= nb_;
+                continue;
+            }
+        }
         const int k = sp + 64 + 64 <= kBsStack ? (sp < 64 ? sp : 64) : 1;
         __builtin_amdgcn_wave_barrier();
         const int node = lane < k ? stk[sp - 1 - lane] : -1;
@@ -1295,7 +1433,7 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
         sp -= k;
         bool isfar = false, isleaf = false, isopen = false;
         int ca = 0, cb = 0;
-        if (node >= 0 && node != (int)leaf) {
+        if (node >= 0 && node != leaf) {
             const NodeRec nd = t.rec[node];
             const double ddx = nd.cx - cAx, ddy = nd.cy - cAy;
             const double d2 = ddx * ddx + ddy * ddy;
@@ -1322,19 +1460,8 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
         const uint64_t fm = __ballot(isfar);
         nfar += __popcll(fm);
         if constexpr (F32) {
-            // queued for M2L into the bucket's local expansion after the
-            // traversal, 64 at a time; only a nearly full queue (rare: a
-            // bucket meets about 140 far nodes at 2^20) is translated here,
-            // so the traversal keeps its registers
             if (isfar) fq[nq + __popcll(fm & ((1ull << lane) - 1))] = node;
             nq += __popcll(fm);
-            if (nq > kFarQ - 64) {
-                __builtin_amdgcn_wave_barrier();
-                m2l_batch<ORDER>(t, fq + nq - 64, 64, lane, cAx, cAy, rT, tr, lacc, gap);
-                __builtin_amdgcn_wave_barrier();
-                cm2l += 64ull;
-                nq -= 64;
-            }
         } else {
             // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
             for (uint64_t m = fm; m; m &= m - 1) {
@@ -1346,7 +1473,7 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
                 const T qx = Dx * inv, qy = -Dy * inv;
                 const T R = (T)fmax(nd.r, kTiny);
                 const T tx = R * qx, ty = R * qy;
-                const double2 *bm = t.cd + (int64_t)fnode * ORDER;
+                const double2 *bm = t.cd + (int64_t)fnode * STRIDE;
                 T sr = bm[ORDER - 1].x, si = bm[ORDER - 1].y;
 #pragma unroll
                 for (int q = ORDER - 2; q >= 0; --q) {
@@ -1402,26 +1529,19 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     su += (double)fu;
     sv += (double)fv;
     if constexpr (F32) {
-        while (nq > 0) {
-            const int nb_ = nq < 64 ? nq : 64;
-            __builtin_amdgcn_wave_barrier();
-            m2l_batch<ORDER>(t, fq + nq - nb_, nb_, lane, cAx, cAy, rT, tr, lacc, gap);
-            __builtin_amdgcn_wave_barrier();
-            cm2l += (unsigned long long)nb_;
-            nq -= nb_;
-        }
         for (int o = 32; o > 0; o >>= 1) gap = fminf(gap, __shfl_xor(gap, o, 64));
         farmin_w = (double)gap;
         // L2P: F = sum_l L_l u^l at u = (z - c_T) / R_T (ax, ay are relative
         // to the bucket centre); u += Im F, v += Re F
         l2p_coeffs<ORDER>(lacc);
         const float ur = ax / rT, ui = ay / rT;
-        float Fr = readlane_f(lacc[(ORDER - 1) >> 2].x, (ORDER - 1) & 3);
-        float Fi = readlane_f(lacc[(ORDER - 1) >> 2].y, (ORDER - 1) & 3);
+        const float2 Lt = local_coeff<ORDER>(lacc, ORDER - 1);
+        float Fr = Lt.x, Fi = Lt.y;
 #pragma unroll
         for (int l = ORDER - 2; l >= 0; --l) {
-            const float nr_ = Fr * ur - Fi * ui + readlane_f(lacc[l >> 2].x, l & 3);
-            Fi = Fr * ui + Fi * ur + readlane_f(lacc[l >> 2].y, l & 3);
+            const float2 Ll = local_coeff<ORDER>(lacc, l);
+            const float nr_ = Fr * ur - Fi * ui + Ll.x;
+            Fi = Fr * ui + Fi * ur + Ll.y;
             Fr = nr_;
         }
         su += (double)Fi;
@@ -1445,7 +1565,9 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     }
 }
 
-template <typename T, int ORDER>
+// ORDER terms of coefficients stored STRIDE per node; targets per wave as
+// bs_wave's GROUPS.
+template <typename T, int ORDER, int STRIDE, bool GROUPS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && ORDER == 16 ? 6 : 1)))
 void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
                                                  double *__restrict__ u, double *__restrict__ v,
@@ -1453,7 +1575,7 @@ void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y
                                                  double *__restrict__ fbr2, unsigned long long *__restrict__ nfb,
                                                  int exact_all) {
     unsigned long long cp = 0, cf = 0, cl = 0;
-    bs_wave<T, ORDER>(t, theta, eps, vinf_x, vinf_y, u, v, d, fb, fbr2, nfb, exact_all, cp, cf, cl);
+    bs_wave<T, ORDER, STRIDE, GROUPS>(t, theta, eps, vinf_x, vinf_y, u, v, d, fb, fbr2, nfb, exact_all, cp, cf, cl);
     block_count_add(cp, cf, nfb + 1, nfb + 2);
     block_count_add(cl, 0ull, nfb + 7, nullptr);
 }
@@ -1710,11 +1832,11 @@ __device__ __forceinline__ int64_t far_count(const unsigned long long *cnt, int6
 // own radius reaches it.  A lane whose radius exceeds far_radius() is listed for
 // k_diff_far instead and does not widen the wave's radius.  cnt[0] +=
 // evaluated pairs, cnt[1] stack overflows, cnt[2] the list length.
-template <typename T>
+template <typename T, bool CENSUS>
 __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, double nu, double thr,
                                                const double *__restrict__ sd, double *__restrict__ u,
                                                double *__restrict__ v, unsigned long long *__restrict__ cnt,
-                                               unsigned long long &cpairs) {
+                                               unsigned long long &cpairs, unsigned long long &cin) {
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
     const int lane = threadIdx.x & 63;
@@ -1746,7 +1868,9 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
     }
     const T thr2 = F32 ? (T)(thr * thr) : (T)thr;
     const T gap2 = (T)(4.0 * thr * thr);
-    int64_t npairs = 0;
+    // sources farther than R from the group's box reach no target's radius
+    const T Rs2 = F32 ? (T)(R2 * (1.0 + 1e-4)) : (T)R2;
+    int64_t npairs = 0, nin = 0;
     double a2x = 0.0, a2y = 0.0, a1 = 0.0;
     LaneStack st;
     if (R > 0.0) st.push(lane, t.root);
@@ -1770,18 +1894,34 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
         if (__ballot(lx * lx + ly * ly < ri2) == 0) continue;
         if (t.child0[node] < 0) {
             const int64_t j0 = t.first[node];
-            const int cnt_j = t.count[node];
-            T ox, oy;
+            T ox, oy, bx0, bx1, by0, by1;
             if constexpr (F32) {
-                ox = ax + (float)(cAx - t.ncx[node]);
-                oy = ay + (float)(cAy - t.ncy[node]);
+                const double lcx = t.ncx[node], lcy = t.ncy[node];
+                ox = ax + (float)(cAx - lcx);
+                oy = ay + (float)(cAy - lcy);
+                bx0 = (float)(bA.x - lcx);
+                bx1 = (float)(bA.y - lcx);
+                by0 = (float)(bA.z - lcy);
+                by1 = (float)(bA.w - lcy);
             } else {
                 ox = ax;
                 oy = ay;
+                bx0 = bA.x;
+                bx1 = bA.y;
+                by0 = bA.z;
+                by1 = bA.w;
             }
-            stage_leaf<T>(lds, lane, j0, cnt_j, t);
+            const int cnt_j = stage_leaf_near<T>(lds, lane, j0, t.count[node], t, bx0, bx1, by0, by1, Rs2);
             T s2x = 0, s2y = 0, s1 = 0;
             npairs += cnt_j;
+            if constexpr (CENSUS) {
+                if (mine)
+                    for (int jj = 0; jj < cnt_j; ++jj) {
+                        const Src4T<T> q = lds[jj];
+                        const T dx = ox - q.x, dy = oy - q.y;
+                        nin += dx * dx + dy * dy < (F32 ? ri2 / (T)(1.0 + 1e-4) : ri2) ? 1 : 0;
+                    }
+            }
             // the threshold test is needed only where a pair can come within
             // it: a leaf box nearer than twice the threshold to some target
             if (__ballot(mine && !(lx * lx + ly * ly > gap2))) {
@@ -1806,6 +1946,10 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
     }
     const int nmine = __popcll(__ballot(mine));
     cpairs = (unsigned long long)(npairs * nmine);
+    if constexpr (CENSUS) {
+        for (int o = 32; o > 0; o >>= 1) nin += __shfl_xor(nin, o, 64);
+        cin = (unsigned long long)nin;
+    }
     if (lane == 0 && st.overflow) atomicAdd(cnt + 1, 1ull);
     if (!mine) return;
     if (fabs(a1) > 1e-5) {
@@ -1814,15 +1958,16 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
     }
 }
 
-template <typename T>
+// CENSUS also counts the pairs within each target's own radius (cnt[5]).
+template <typename T, bool CENSUS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
 void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
                                                         const double *__restrict__ sd,
                                                         double *__restrict__ u, double *__restrict__ v,
                                                         unsigned long long *__restrict__ cnt) {
-    unsigned long long cp = 0;
-    diffusion_wave<T>(t, cut, nu, thr, sd, u, v, cnt, cp);
-    block_count_add(cp, 0ull, cnt, nullptr);
+    unsigned long long cp = 0, ci = 0;
+    diffusion_wave<T, CENSUS>(t, cut, nu, thr, sd, u, v, cnt, cp, ci);
+    block_count_add(cp, ci, cnt, CENSUS ? cnt + 5 : nullptr);
 }
 
 // The diffusion's d in sorted order, negated for the targets listed for
@@ -2096,7 +2241,10 @@ void tree_release(vx_case *c) {
     c->tree = nullptr;
 }
 
-static int tree_order(const vx_case *c) { return c->prm.order >= 32 ? 32 : 16; }
+// Coefficients stored per node (the upward pass works on groups of 16 or 32
+// lanes) and terms used by the far field: 16, 20 or 32.
+static int tree_order(const vx_case *c) { return c->prm.order > 16 ? 32 : 16; }
+static int tree_terms(const vx_case *c) { return c->prm.order >= 32 ? 32 : (c->prm.order >= 20 ? 20 : 16); }
 
 // Builds the tree on stream st in two parts: A (inclusion, box, keys, sort;
 // no host wait) and B (the rest, after the included count is back).  With
@@ -2106,6 +2254,7 @@ static int tree_build_a(vx_case *c, int64_t s, hipStream_t st, Births B) {
     TreeBufs &t = tree_of(c);
     const int order = tree_order(c);
     t.order = order;
+    t.terms = tree_terms(c);
     t.s = s;
     const int64_t nbs = (s + 255) / 256;
     if (t.keys.ensure(s, st) || t.keys_alt.ensure(s, st) || t.idx.ensure(s, st) || t.perm.ensure(s, st) ||
@@ -2293,27 +2442,39 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
                            vinf_x, vinf_y, c->u.p, c->v.p, c->d.p);
     if (t.m == 0) return 0;
     const TreeDev tv = view(t);
-    const unsigned lb = (unsigned)((t.nleaf + 3) / 4 + 7) / 8 * 8;  // padded for xcd_block()
+    const bool groups = c->tune.bs_targets == 2;
+    const int64_t nw = groups ? t.ngroup : t.nleaf;  // waves
+    const unsigned lb = (unsigned)((nw + 3) / 4 + 7) / 8 * 8;  // padded for xcd_block()
     const double th = theta_of(c), eps = c->prm.ve_size;
     unsigned long long *nfb = t.cnt.p + 1;
     const bool mixed = c->prm.precision == VX_PREC_MIXED;
     const int exact_all = (c->prm.flags & VX_FLAG_EXACT_NEAREST) ? 1 : 0;
     Timer tk(c, &c->tm.bs_kernel_ms, 6);
+#define VX_TREE_BS(T_, P_, S_)                                                                                       \
+    do {                                                                                                             \
+        if (groups)                                                                                                  \
+            hipLaunchKernelGGL((k_tree_bs<T_, P_, S_, true>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x,       \
+                               vinf_y, c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);                   \
+        else                                                                                                         \
+            hipLaunchKernelGGL((k_tree_bs<T_, P_, S_, false>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x,      \
+                               vinf_y, c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);                   \
+    } while (0)
     if (mixed) {
-        if (t.order == 32)
-            hipLaunchKernelGGL((k_tree_bs<float, 32>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,
-                               c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
+        if (t.terms == 32)
+            VX_TREE_BS(float, 32, 32);
+        else if (t.terms == 20)
+            VX_TREE_BS(float, 20, 32);
         else
-            hipLaunchKernelGGL((k_tree_bs<float, 16>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,
-                               c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
+            VX_TREE_BS(float, 16, 16);
     } else {
-        if (t.order == 32)
-            hipLaunchKernelGGL((k_tree_bs<double, 32>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y,
-                               c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
+        if (t.terms == 32)
+            VX_TREE_BS(double, 32, 32);
+        else if (t.terms == 20)
+            VX_TREE_BS(double, 20, 32);
         else
-            hipLaunchKernelGGL((k_tree_bs<double, 16>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y,
-                               c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);
+            VX_TREE_BS(double, 16, 16);
     }
+#undef VX_TREE_BS
     tk.stop();
     hipLaunchKernelGGL(k_tree_nearest_fallback, dim3(512), dim3(1024), 0, st, tv, t.arr.p, t.cnt.p + 10, t.fb.p,
                        t.fbr2.p, nfb, c->d.p);
@@ -2354,8 +2515,10 @@ int tree_diffusion_pre(vx_case *c) {
         t.fpairs.ensure((size_t)(nchunk * t.fcap), st) || t.ftgt.ensure((size_t)t.fcap, st) ||
         t.sd.ensure((size_t)t.m, st))
         return 1;
-    unsigned long long *cnt = t.cnt.p + 4;  // pairs, overflow, far targets, far (target, chunk) pairs
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 0xdu);  // not the shared overflow flag
+    // pairs, overflow, far targets, far (target, chunk) pairs; [4] is the
+    // BS's M2L count, [5] the census of pairs within the targets' radii
+    unsigned long long *cnt = t.cnt.p + 4;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 0x2du);  // not the overflow flag nor M2L count
     if (c->ptmp.ensure((size_t)prim::tiles_of(t.m) + 1, st)) return 1;
     const TreeDev tv = view(t);
     VX_CHECK(prim::select_flags(FarFlag{tv, diff_cut_of(c), c->d.p}, t.m, FarWrite{tv, c->d.p, t.sd.p, t.fb.p},
@@ -2398,18 +2561,41 @@ int tree_diffusion_main(vx_case *c) {
     const double cut = diff_cut_of(c), thr = 0.001 * c->prm.ve_size;
     unsigned long long *cnt = t.cnt.p + 4;
     Timer tk(c, &c->tm.diff_kernel_ms, 8);
-    if (mixed)
-        hipLaunchKernelGGL(k_tree_diffusion<float>, dim3(lb8), dim3(256), 0, st, view(t), cut, c->prm.nu, thr,
-                           t.sd.p, c->u.p, c->v.p, cnt);
-    else
-        hipLaunchKernelGGL(k_tree_diffusion<double>, dim3(lb8), dim3(256), 0, st, view(t), cut, c->prm.nu, thr,
-                           t.sd.p, c->u.p, c->v.p, cnt);
+    if (c->census) {
+        if (mixed)
+            hipLaunchKernelGGL((k_tree_diffusion<float, true>), dim3(lb8), dim3(256), 0, st, view(t), cut, c->prm.nu,
+                               thr, t.sd.p, c->u.p, c->v.p, cnt);
+        else
+            hipLaunchKernelGGL((k_tree_diffusion<double, true>), dim3(lb8), dim3(256), 0, st, view(t), cut,
+                               c->prm.nu, thr, t.sd.p, c->u.p, c->v.p, cnt);
+    } else if (mixed) {
+        hipLaunchKernelGGL((k_tree_diffusion<float, false>), dim3(lb8), dim3(256), 0, st, view(t), cut, c->prm.nu,
+                           thr, t.sd.p, c->u.p, c->v.p, cnt);
+    } else {
+        hipLaunchKernelGGL((k_tree_diffusion<double, false>), dim3(lb8), dim3(256), 0, st, view(t), cut, c->prm.nu,
+                           thr, t.sd.p, c->u.p, c->v.p, cnt);
+    }
     VX_CHECK(hipGetLastError());
     return 0;
 }
 
 int launch_tree_diffusion(vx_case *c) {
     return tree_diffusion_pre(c) || tree_diffusion_far(c, c->stream) || tree_diffusion_main(c);
+}
+
+// The domain-scale target count the next diffusion sizes its list from (the
+// last diffusion's, read after the caller's synchronisation); part of an
+// exact checkpoint, since the list's capacity decides which targets the far
+// kernel sums.
+int64_t tree_far_hint(vx_case *c) {
+    return (c->tree && c->tree->h_far) ? (int64_t) * (volatile unsigned long long *)c->tree->h_far : 0;
+}
+
+int tree_set_far_hint(vx_case *c, int64_t h) {
+    TreeBufs &t = tree_of(c);
+    if (!t.h_far) VX_CHECK(hipHostMalloc((void **)&t.h_far, sizeof(unsigned long long), hipHostMallocDefault));
+    *t.h_far = (unsigned long long)(h > 0 ? h : 0);
+    return 0;
 }
 
 int tree_copy_order(vx_case *c, uint32_t *keys, int32_t *perm, int64_t count) {
@@ -2442,10 +2628,11 @@ int tree_stats(vx_case *c, vx_tree_info *info) {
         set_error("tree walk stack overflow");
         return 1;
     }
-    info->order = c->tree->order;
+    info->order = c->tree->terms;
     const unsigned long long cap = (unsigned long long)c->tree->fcap;
     info->heavy = (int32_t)(h[6] < cap ? h[6] : cap);
     info->m2l = (int64_t)h[8];
+    info->diff_in_radius = c->census ? (int64_t)h[9] : -1;
     return 0;
 }
 

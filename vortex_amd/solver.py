@@ -148,6 +148,18 @@ class Case:
     def sync(self):
         check(lib().vx_case_sync(self._h))
 
+    def save(self, path, user=b""):
+        """Exact checkpoint of the case state plus an opaque blob (vx_case_save)."""
+        buf = C.create_string_buffer(bytes(user), len(user)) if user else None
+        check(lib().vx_case_save(self._h, str(path).encode(), buf, len(user)))
+
+    def load(self, path, user_cap=1 << 20) -> bytes:
+        """Restore a checkpoint written by save(); returns its blob."""
+        buf = C.create_string_buffer(max(1, user_cap))
+        got = C.c_int64()
+        check(lib().vx_case_load(self._h, str(path).encode(), buf, user_cap, C.byref(got)))
+        return buf.raw[:got.value]
+
     # -- stages -----------------------------------------------------------
     def vort_creation(self, vinf):
         check(lib().vx_vort_creation(self._h, vinf[0], vinf[1]))
@@ -177,8 +189,8 @@ class Case:
         check(lib().vx_case_autotune(self._h, float(vinf[0]), float(vinf[1]), C.byref(t)))
         return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
-    def set_tuning(self, tpt_bs=0, tpt_diff=0, leaf_off=-1):
-        t = Tuning(tpt_bs=tpt_bs, tpt_diff=tpt_diff, leaf_off=leaf_off)
+    def set_tuning(self, tpt_bs=0, tpt_diff=0, leaf_off=-1, bs_targets=0):
+        t = Tuning(tpt_bs=tpt_bs, tpt_diff=tpt_diff, leaf_off=leaf_off, bs_targets=bs_targets)
         check(lib().vx_case_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:
@@ -231,6 +243,11 @@ class Case:
 
     def tree_diffusion(self):
         check(lib().vx_tree_diffusion(self._h))
+
+    def tree_census(self, on=True):
+        """Count the diffusion pairs within each target's radius from the next
+        tree diffusion on (tree_stats()["diff_in_radius"])."""
+        check(lib().vx_tree_census(self._h, int(bool(on))))
 
     def tree_stats(self) -> dict:
         from ._lib import TreeInfo
