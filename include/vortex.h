@@ -265,11 +265,6 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL);
  *                     levels above the cell that holds 64 vortices at the mean
  *                     density, 0..3 (-1: default 1).  It changes which pairs
  *                     are summed directly, within the tree's error bound.
- *   bs_targets        the tree Biot-Savart's targets per wave: 1 a bucket
- *                     (at most 64 vortices, fewer on average), 2 a target
- *                     group (runs of 64 in Morton order, fuller waves, a wider
- *                     opening box); 0: default 1.  Within the tree's error
- *                     bound, like leaf_off.
  * vx_case_autotune times every choice on the case's current state (the
  * direct kernels one by one below tree_min_n, the whole velocity stage
  * above it), keeps the fastest and reports the default's and the chosen
@@ -277,9 +272,7 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL);
 typedef struct vx_tuning {
     int32_t tpt_bs, tpt_diff;
     int32_t leaf_off;
-    int32_t bs_targets;
     int32_t trials;                 /* out: configurations timed */
-    int32_t reserved;
     double ms_default, ms_best;     /* out: timed stage, default and chosen */
 } vx_tuning;
 int vx_case_autotune(vx_case *c, double vinf_x, double vinf_y, vx_tuning *out);

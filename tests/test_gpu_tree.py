@@ -84,26 +84,6 @@ def test_tree_biot_savart_vs_direct_oracle(cloud40k, prec, order, tol):
     assert np.max(np.abs(d - dr) / dr) < dtol
 
 
-@pytest.mark.parametrize("prec,order,tol", [(VX_PREC_F64, 16, 2e-4), (VX_PREC_MIXED, 16, 3e-4),
-                                            (VX_PREC_MIXED, 32, 3e-5)])
-def test_tree_biot_savart_target_groups_vs_direct_oracle(cloud40k, prec, order, tol):
-    """Target groups (64 consecutive particles of a cell per wave) instead of
-    buckets: the same bounds against the direct sums."""
-    X, Y, G, s, eps, ur, vr, dr = cloud40k
-    pan = geo.circle(200, 0.5)
-    c = Case(make_params(pan, precision=prec, ve_size=eps, order=order), pan)
-    c.set_tuning(bs_targets=2)
-    c.set_state(X, Y, G)
-    c.tree_biot_savart(s, (1.0, 0.0))
-    u, v, d = c.velocity(s)
-    print(f"[tree] groups prec={prec} order={order}: rel err u {rel_err(u - 1.0, ur - 1.0):.3g} "
-          f"v {rel_err(v, vr):.3g}", flush=True)
-    c.close()
-    assert rel_err(u - 1.0, ur - 1.0) < tol and rel_err(v, vr) < tol
-    dtol = 1e-13 if prec == VX_PREC_F64 else 1e-4
-    assert np.max(np.abs(d - dr) / dr) < dtol
-
-
 @pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 200, 1000, 5000])
 @pytest.mark.parametrize("prec", [VX_PREC_F64, VX_PREC_MIXED])
 @pytest.mark.parametrize("exact", [0, 1])

@@ -58,8 +58,8 @@ class TreeInfo(C.Structure):
 
 class Tuning(C.Structure):
     """Mirror of vx_tuning (include/vortex.h)."""
-    _fields_ = [("tpt_bs", C.c_int32), ("tpt_diff", C.c_int32), ("leaf_off", C.c_int32), ("bs_targets", C.c_int32),
-                ("trials", C.c_int32), ("reserved", C.c_int32), ("ms_default", C.c_double), ("ms_best", C.c_double)]
+    _fields_ = [("tpt_bs", C.c_int32), ("tpt_diff", C.c_int32), ("leaf_off", C.c_int32), ("trials", C.c_int32),
+                ("ms_default", C.c_double), ("ms_best", C.c_double)]
 
 
 class Timings(C.Structure):

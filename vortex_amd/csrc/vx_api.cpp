@@ -712,22 +712,19 @@ int vx_case_autotune(vx_case *c, double vinf_x, double vinf_y, vx_tuning *out) {
     best.trials = 0;
     constexpr int kReps = 3;
     if (tree) {
-        // the whole velocity stage for each bucket depth and kind of target
+        // the whole velocity stage for each bucket depth
         double tdef = 0.0, tbest = 1e300;
-        for (int tg = 1; tg <= 2; ++tg)
-            for (int off = 0; off <= 3; ++off) {
-                c->tune.leaf_off = off;
-                c->tune.bs_targets = tg;
-                double t = 0.0;
-                if (time_ms(c, kReps, [&] { return vx_speed(c, vinf_x, vinf_y); }, &t)) return 1;
-                ++best.trials;
-                if (off == 1 && tg == 1) tdef = t;
-                if (t < tbest) {
-                    tbest = t;
-                    best.leaf_off = off;
-                    best.bs_targets = tg;
-                }
+        for (int off = 0; off <= 3; ++off) {
+            c->tune.leaf_off = off;
+            double t = 0.0;
+            if (time_ms(c, kReps, [&] { return vx_speed(c, vinf_x, vinf_y); }, &t)) return 1;
+            ++best.trials;
+            if (off == 1) tdef = t;
+            if (t < tbest) {
+                tbest = t;
+                best.leaf_off = off;
             }
+        }
         best.ms_default = tdef;
         best.ms_best = tbest;
     } else {
@@ -771,11 +768,9 @@ int vx_case_set_tuning(vx_case *c, const vx_tuning *t) {
     auto ok_tpt = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8; };
     VX_ARG(ok_tpt(t->tpt_bs) && ok_tpt(t->tpt_diff), "targets per thread must be 0, 1, 2, 4 or 8");
     VX_ARG(t->leaf_off >= -1 && t->leaf_off <= 3, "leaf_off must be -1..3");
-    VX_ARG(t->bs_targets >= 0 && t->bs_targets <= 2, "bs_targets must be 0, 1 or 2");
     c->tune.tpt_bs = t->tpt_bs;
     c->tune.tpt_diff = t->tpt_diff;
     c->tune.leaf_off = t->leaf_off;
-    c->tune.bs_targets = t->bs_targets;
     return 0;
 }
 

@@ -189,7 +189,7 @@ struct vx_case {
     bool devcount = false;
     int64_t n_ub = 0;
     // launch shapes (vx_case_autotune / vx_case_set_tuning)
-    vx_tuning tune{0, 0, -1, 0, 0, 0, 0.0, 0.0};
+    vx_tuning tune{0, 0, -1, 0, 0.0, 0.0};
     vx::DBuf<double> gr_ring;                         // 4 forces + 1 count per step
     int64_t gr_cap = 0;
     hipGraph_t graph = nullptr;

@@ -148,18 +148,31 @@ __global__ __launch_bounds__(256) void k_lu_norm(const double *__restrict__ L, i
 // [k0, np): per column, the largest magnitude below the diagonal (ties: the
 // first row) is swapped up within the panel, the column is divided by it and
 // the rest of the panel is updated.  ipiv[col] = the row swapped with col.
+// The panel is factored in a column-major copy Pt (64 x np, column c at
+// Pt + c np), so that the pivot search and every column update read and
+// write consecutive rows (the row-major matrix would put each thread on its
+// own row, 4 np bytes from the next), then copied back.
 __global__ __launch_bounds__(1024) void k_lu_panel(float *__restrict__ F, int64_t np, int64_t k0,
-                                                   int32_t *__restrict__ ipiv, unsigned long long *__restrict__ st) {
+                                                   int32_t *__restrict__ ipiv, unsigned long long *__restrict__ st,
+                                                   float *__restrict__ Pt) {
     __shared__ float bv[1024];
     __shared__ int bi[1024];
     __shared__ float prow[kLuB];
     const int tid = threadIdx.x;
+    const int64_t rows = np - k0;
+    for (int64_t e = tid; e < rows * kLuB; e += 1024) {
+        const int64_t i = k0 + e / kLuB;
+        const int c = (int)(e % kLuB);
+        Pt[c * np + i] = F[i * np + k0 + c];
+    }
+    __syncthreads();
     for (int jj = 0; jj < kLuB; ++jj) {
         const int64_t col = k0 + jj;
+        float *pc = Pt + (int64_t)jj * np;
         float best = -1.f;
         int besti = (int)col;
         for (int64_t i = col + tid; i < np; i += 1024) {
-            const float v = fabsf(F[i * np + col]);
+            const float v = fabsf(pc[i]);
             if (v > best) { best = v; besti = (int)i; }
         }
         bv[tid] = best;
@@ -176,12 +189,13 @@ __global__ __launch_bounds__(1024) void k_lu_panel(float *__restrict__ F, int64_
         const int64_t p = bi[0];
         if (tid == 0) ipiv[col] = (int32_t)p;
         if (p != col && tid < kLuB) {
-            const float a = F[col * np + k0 + tid], b = F[p * np + k0 + tid];
-            F[col * np + k0 + tid] = b;
-            F[p * np + k0 + tid] = a;
+            float *q = Pt + (int64_t)tid * np;
+            const float a = q[col], b = q[p];
+            q[col] = b;
+            q[p] = a;
         }
         __syncthreads();
-        if (tid < kLuB) prow[tid] = F[col * np + k0 + tid];
+        if (tid < kLuB) prow[tid] = Pt[(int64_t)tid * np + col];
         __syncthreads();
         const float piv = prow[jj];
         if (piv == 0.f) {
@@ -190,12 +204,16 @@ __global__ __launch_bounds__(1024) void k_lu_panel(float *__restrict__ F, int64_
             continue;
         }
         for (int64_t i = col + 1 + tid; i < np; i += 1024) {
-            float *row = F + i * np + k0;
-            const float l = row[jj] / piv;
-            row[jj] = l;
-            for (int c = jj + 1; c < kLuB; ++c) row[c] -= l * prow[c];
+            const float l = pc[i] / piv;
+            pc[i] = l;
+            for (int c = jj + 1; c < kLuB; ++c) Pt[(int64_t)c * np + i] -= l * prow[c];
         }
         __syncthreads();
+    }
+    for (int64_t e = tid; e < rows * kLuB; e += 1024) {
+        const int64_t i = k0 + e / kLuB;
+        const int c = (int)(e % kLuB);
+        F[i * np + k0 + c] = Pt[c * np + i];
     }
 }
 
@@ -421,7 +439,7 @@ int launch_lu_setup(vx_case *c) {
     hipLaunchKernelGGL(k_lu_perm_init, dim3(g1), dim3(256), 0, st, b.perm.p, np);
     // right-looking LU: panel, swaps, block row, trailing product
     for (int64_t k0 = 0; k0 < np; k0 += kLuB) {
-        hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(1024), 0, st, b.F.p, np, k0, b.ipiv.p, b.st.p);
+        hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(1024), 0, st, b.F.p, np, k0, b.ipiv.p, b.st.p, b.T.p);
         hipLaunchKernelGGL(k_lu_swap, dim3(g1), dim3(256), 0, st, b.F.p, np, k0, b.ipiv.p, b.perm.p);
         const int64_t rest = np - k0 - kLuB;
         if (rest > 0) {

@@ -1304,11 +1304,7 @@ __device__ __forceinline__ int64_t xcd_block() {
 constexpr int kBsStack = 256;  // traversal stack entries per wave
 constexpr int kFarQ = 512;     // FP32: far nodes queued per wave before any M2L
 
-// Targets of a wave: one bucket (GROUPS false), or one target group, a run of
-// up to 64 consecutive particles of the Morton order within a cell, which
-// fills more lanes than a bucket (buckets hold 64 at most, about 40 on
-// average) at the price of a wider box for the opening test.
-template <typename T, int ORDER, int STRIDE, bool GROUPS>
+template <typename T, int ORDER, int STRIDE>
 __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double eps, double vinf_x, double vinf_y,
                                         double *__restrict__ u, double *__restrict__ v, double *__restrict__ d,
                                         int32_t *__restrict__ fb, double *__restrict__ fbr2,
@@ -1324,44 +1320,20 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     const int lane = threadIdx.x & 63;
     Src4T<T> *lds = lds_all[threadIdx.x >> 6];
     int32_t *fq = fq_all[threadIdx.x >> 6];
-    const int64_t wv = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
-    if (wv >= (GROUPS ? t.ngroup : t.nleaf)) return;
-    int64_t ti;
-    int nvalid;
-    bool ok;
-    double cAx, cAy, rA;
-    int leaf;  // the bucket summed first (holding the targets, or the group's first)
-    if constexpr (GROUPS) {
-        const TargetGroup G = target_group(t, wv, lane);
-        ti = G.ti;
-        nvalid = G.nvalid;
-        ok = G.ok;
-        cAx = G.cx;
-        cAy = G.cy;
-        rA = G.r;
-        // bucket of particle f: the bucket starts before it, less one unless f starts one
-        const int64_t f = t.gfirst[wv];
-        leaf = t.nleaf == 1 ? 0 : t.bscan[f] + t.bstart[f] - 1;
-    } else {
-        leaf = (int)wv;
-        const NodeRec L = t.rec[leaf];
-        ti = (int64_t)(-1 - L.a) + lane;
-        nvalid = L.b;
-        ok = lane < nvalid;
-        cAx = L.cx;
-        cAy = L.cy;
-        rA = L.r;
-    }
+    // targets: one bucket per wave (a compact node, which keeps the MAC's
+    // opening radius small; runs of 64 particles of the Morton order fill
+    // more lanes but open about 1.6 times as many pairs)
+    const int leaf = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
+    if (leaf >= t.nleaf) return;
+    const NodeRec L = t.rec[leaf];
+    const int64_t ti = (int64_t)(-1 - L.a) + lane;
+    const int nvalid = L.b;
+    const bool ok = lane < nvalid;
+    const double cAx = L.cx, cAy = L.cy, rA = L.r;
     T ax, ay;
     if constexpr (F32) {
-        // relative to the targets' centre (the bucket centre for a bucket)
-        if constexpr (GROUPS) {
-            ax = ok ? (float)(t.sx[ti] - cAx) : 0.f;
-            ay = ok ? (float)(t.sy[ti] - cAy) : 0.f;
-        } else {
-            ax = ok ? t.fx[ti] : 0.f;
-            ay = ok ? t.fy[ti] : 0.f;
-        }
+        ax = ok ? t.fx[ti] : 0.f;  // relative to the bucket centre
+        ay = ok ? t.fy[ti] : 0.f;
     } else {
         ax = ok ? t.sx[ti] : cAx;
         ay = ok ? t.sy[ti] : cAy;
@@ -1385,17 +1357,11 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     // the own bucket first: its particles are usually the nearest, which
     // tightens the bound that lets most other leaves skip the nearest-3 updates
     {
-        const NodeRec L = t.rec[leaf];
-        stage_leaf<T>(lds, lane, -1 - L.a, L.b, t);
-        T ox = ax, oy = ay;
-        if constexpr (F32 && GROUPS) {
-            ox = ax - (float)(L.cx - cAx);
-            oy = ay - (float)(L.cy - cAy);
-        }
+        stage_leaf<T>(lds, lane, -1 - L.a, nvalid, t);
         T pu = 0, pv = 0;
-        p2p_leaf<T, true>(lds, L.b, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
+        p2p_leaf<T, true>(lds, nvalid, ax, ay, eps2, pu, pv, m0, m1, m2, m3);
         __builtin_amdgcn_wave_barrier();
-        npairs += L.b;
+        npairs += nvalid;
         su += (double)pu;
         sv += (double)pv;
     }
@@ -1565,9 +1531,8 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     }
 }
 
-// ORDER terms of coefficients stored STRIDE per node; targets per wave as
-// bs_wave's GROUPS.
-template <typename T, int ORDER, int STRIDE, bool GROUPS>
+// ORDER terms of coefficients stored STRIDE per node.
+template <typename T, int ORDER, int STRIDE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && ORDER == 16 ? 6 : 1)))
 void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
                                                  double *__restrict__ u, double *__restrict__ v,
@@ -1575,7 +1540,7 @@ void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y
                                                  double *__restrict__ fbr2, unsigned long long *__restrict__ nfb,
                                                  int exact_all) {
     unsigned long long cp = 0, cf = 0, cl = 0;
-    bs_wave<T, ORDER, STRIDE, GROUPS>(t, theta, eps, vinf_x, vinf_y, u, v, d, fb, fbr2, nfb, exact_all, cp, cf, cl);
+    bs_wave<T, ORDER, STRIDE>(t, theta, eps, vinf_x, vinf_y, u, v, d, fb, fbr2, nfb, exact_all, cp, cf, cl);
     block_count_add(cp, cf, nfb + 1, nfb + 2);
     block_count_add(cl, 0ull, nfb + 7, nullptr);
 }
@@ -2442,23 +2407,15 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
                            vinf_x, vinf_y, c->u.p, c->v.p, c->d.p);
     if (t.m == 0) return 0;
     const TreeDev tv = view(t);
-    const bool groups = c->tune.bs_targets == 2;
-    const int64_t nw = groups ? t.ngroup : t.nleaf;  // waves
-    const unsigned lb = (unsigned)((nw + 3) / 4 + 7) / 8 * 8;  // padded for xcd_block()
+    const unsigned lb = (unsigned)((t.nleaf + 3) / 4 + 7) / 8 * 8;  // padded for xcd_block()
     const double th = theta_of(c), eps = c->prm.ve_size;
     unsigned long long *nfb = t.cnt.p + 1;
     const bool mixed = c->prm.precision == VX_PREC_MIXED;
     const int exact_all = (c->prm.flags & VX_FLAG_EXACT_NEAREST) ? 1 : 0;
     Timer tk(c, &c->tm.bs_kernel_ms, 6);
 #define VX_TREE_BS(T_, P_, S_)                                                                                       \
-    do {                                                                                                             \
-        if (groups)                                                                                                  \
-            hipLaunchKernelGGL((k_tree_bs<T_, P_, S_, true>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x,       \
-                               vinf_y, c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);                   \
-        else                                                                                                         \
-            hipLaunchKernelGGL((k_tree_bs<T_, P_, S_, false>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x,      \
-                               vinf_y, c->u.p, c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all);                   \
-    } while (0)
+    hipLaunchKernelGGL((k_tree_bs<T_, P_, S_>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,     \
+                       c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all)
     if (mixed) {
         if (t.terms == 32)
             VX_TREE_BS(float, 32, 32);
