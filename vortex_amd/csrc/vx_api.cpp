@@ -627,9 +627,9 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
 }
 
 // Step (src/unit_1.cu:1077-1218).  The count stays on the device until the
-// end; every collapse pass runs: a pass after one that merged nothing finds
-// the same (empty) pairs again, so the result equals the reference's early
-// exit when n stops changing.
+// end; every collapse pass is queued, and one whose predecessor of the same
+// kind merged nothing searches no grid (launch_setka), which equals the
+// reference's early exit when n stops changing.
 int vx_step(vx_case *c, double out4[4]) {
     VX_ARG(c && out4, "null pointer");
     Timer t(c, &c->tm.step_ms, 4);
@@ -637,7 +637,8 @@ int vx_step(vx_case *c, double out4[4]) {
     if (launch_set_dn(c, n_ub) || launch_step_move(c) || launch_compact(c, n_ub)) return 1;
     for (int kind = 1; kind <= 2; ++kind)
         for (int64_t cc = 0; cc < c->prm.n_col; ++cc)
-            if (launch_setka(c, kind, n_ub, true) || launch_collapse(c, kind, n_ub, true) || launch_compact(c, n_ub))
+            if (launch_setka(c, kind, n_ub, true, cc > 0) || launch_collapse(c, kind, n_ub, true) ||
+                launch_compact(c, n_ub))
                 return 1;
     bool err = false;
     if (step_readback(c, out4, &err)) return 1;
@@ -795,7 +796,8 @@ static int device_step(vx_case *c, double vinf_x, double vinf_y) {
     if (launch_set_dn(c, n_ub) || launch_step_move(c) || launch_compact(c, n_ub)) return 1;
     for (int kind = 1; kind <= 2; ++kind)
         for (int64_t cc = 0; cc < c->prm.n_col; ++cc)
-            if (launch_setka(c, kind, n_ub, true) || launch_collapse(c, kind, n_ub, true) || launch_compact(c, n_ub))
+            if (launch_setka(c, kind, n_ub, true, cc > 0) || launch_collapse(c, kind, n_ub, true) ||
+                launch_compact(c, n_ub))
                 return 1;
     return launch_step_record(c);
 }

@@ -322,8 +322,10 @@ int launch_compact(vx_case *c, int64_t n_ub);          // [0, *dn) compacted, ne
 int launch_live_box(vx_case *c);                       // box of g != 0 in [0, *dn) -> c->red[4..7]
 // launch_setka -> c->col (box from c->red[4..7]); with prepare it also fills
 // the collapse counts and candidate list, and launch_collapse is then called
-// with prepared.  Collapse error flag in dcount[3].
-int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare);
+// with prepared.  With after_pass (a pass of the same kind and its compaction
+// just ran) it searches nothing if that pass left the count unchanged.
+// Collapse error flag in dcount[3].
+int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare, bool after_pass = false);
 int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared);
 
 }  // namespace vx

@@ -531,11 +531,17 @@ __device__ Grid grid_params(const double *__restrict__ box, double r2, int64_t m
 // one), work-group 0 stores it, and the cell heads are cleared to -1.  With
 // cb.pc set, the collapse arrays of [0, n_ub) are prepared as well (counts
 // and states 0, first INT32_MAX, both list counts 0).
+// With dn_prev set, a pass whose predecessor of the same kind left the count
+// unchanged (*dn == *dn_prev: it merged nothing, so the state and with it
+// the pairs are the same) gets an invalid grid: the search finds no pair, as
+// the reference's loop stops when n no longer changes (src/unit_1.cu:1138-1216).
 __global__ void k_grid_clear(int32_t *__restrict__ head, const double *__restrict__ box, double r2,
-                             int64_t max_cells, Grid *__restrict__ G, int64_t n_ub, ColBufs cb) {
+                             int64_t max_cells, Grid *__restrict__ G, int64_t n_ub, ColBufs cb,
+                             const int64_t *__restrict__ dn, const int64_t *__restrict__ dn_prev) {
     __shared__ Grid sq;
     if (threadIdx.x == 0) {
         sq = grid_params(box, r2, max_cells);
+        if (dn_prev && *dn == *dn_prev) sq.valid = false;
         if (blockIdx.x == 0) *G = sq;
     }
     __syncthreads();
@@ -673,7 +679,7 @@ int launch_live_box(vx_case *c) {
     return 0;
 }
 
-int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare) {
+int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare, bool after_pass) {
     hipStream_t st = c->stream;
     if (c->col.ensure((size_t)(n_ub > 0 ? n_ub : 1), st)) return 1;
     if (n_ub <= 0) return 0;
@@ -689,9 +695,11 @@ int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare) {
         c->gridp.ensure(sizeof(Grid), st))
         return 1;
     Grid *G = (Grid *)c->gridp.p;
-    hipLaunchKernelGGL(k_grid_clear, dim3(1024), dim3(256), 0, st, c->vals.p, c->red.p + 4, r2, max_cells, G, n_ub,
-                       cb);
     int64_t *dn = step_dn(c);
+    // after a pass of the same kind, the other slot holds the count before it
+    const int64_t *dn_prev = after_pass ? c->dcount.p + (c->dn_slot == 1 ? 8 : 1) : nullptr;
+    hipLaunchKernelGGL(k_grid_clear, dim3(1024), dim3(256), 0, st, c->vals.p, c->red.p + 4, r2, max_cells, G, n_ub,
+                       cb, (const int64_t *)dn, dn_prev);
     hipLaunchKernelGGL(k_grid_insert, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn, G, c->vals.p,
                        c->vals_alt.p);
     if (kind == 1)
