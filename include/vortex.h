@@ -260,7 +260,9 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL);
  * No counterpart in the reference (fixed 32-thread blocks, include/definitions.h:35).
  *   tpt_bs, tpt_diff  targets per thread of the direct Biot-Savart and
  *                     diffusion kernels: 1, 2, 4 or 8 (0: default, 4 in mixed
- *                     precision, 2 in FP64).  Results do not depend on them.
+ *                     precision, 2 in FP64, halved while the launch would give
+ *                     the GPU's 256 CUs fewer than two work-groups each).
+ *                     Results do not depend on them.
  *   leaf_off          the tree's bucket and target-group depth: quadtree
  *                     levels above the cell that holds 64 vortices at the mean
  *                     density, 0..3 (-1: default 1).  It changes which pairs

@@ -598,9 +598,9 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
             return 1;
         c->tm.bs_launches++;
     }
-    // The wall term and the tree's domain-scale diffusion targets run on the
-    // side stream, overlapping the groups' diffusion on the main stream; the
-    // wall's terms are added after the join.
+    // The tree's domain-scale diffusion targets, then the step's probes over
+    // the tree, run on the side stream, overlapping the groups' diffusion on
+    // the main stream; the wall's terms are added after the join.
     int rc = 0;
     {
         Timer td(c, &c->tm.diff_ms, 3);
@@ -616,6 +616,9 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
         // work is ordered after the fork point only
         rc = tree ? tree_diffusion_main(c) : launch_diffusion(c, s, s);
         if (!rc && tree) rc = tree_diffusion_far(c, c->side);
+        if (!rc && tree && c->prb_in_speed)
+            rc = launch_tree_probes(c, c->prb_x.p, c->prb_y.p, c->prb_m, vinf_x, vinf_y, c->prb_u.p, c->prb_v.p,
+                                    c->side);
         const int rj = side_join(c);
         if (!rc) rc = rj ? rj : launch_wall_add(c);
     }
@@ -653,10 +656,16 @@ int vx_step(vx_case *c, double out4[4]) {
 int vx_time_step(vx_case *c, double vinf_x, double vinf_y, double forces4[4]) {
     VX_ARG(c && forces4, "null pointer");
     c->overlap_births = true;
-    // velocity_control after the birth (src/main.c:744), on the main stream
-    // beside the tree build
-    const int rc = vx_vort_creation(c, vinf_x, vinf_y) || launch_step_probes(c, vinf_x, vinf_y) ||
-                   vx_speed(c, vinf_x, vinf_y);
+    // velocity_control after the birth (src/main.c:744): direct sums on the
+    // main stream beside the tree build, or, when the step's velocities come
+    // from the tree, over that tree on the second stream beside the diffusion
+    // (the state is the same until Step)
+    int rc = vx_vort_creation(c, vinf_x, vinf_y);
+    if (!rc) {
+        c->prb_in_speed = c->prb_m > 0 && c->prm.tree_min_n > 0 && ceil32(c->n) >= c->prm.tree_min_n;
+        rc = (!c->prb_in_speed && launch_step_probes(c, vinf_x, vinf_y)) || vx_speed(c, vinf_x, vinf_y);
+        c->prb_in_speed = false;
+    }
     c->overlap_births = false;
     if (c->born_at >= 0) {  // an early tree part left behind by a failed stage
         c->born_at = -1;
@@ -731,7 +740,7 @@ int vx_case_autotune(vx_case *c, double vinf_x, double vinf_y, vx_tuning *out) {
     } else {
         // each direct kernel for each number of targets per thread
         const int tpts[4] = {1, 2, 4, 8};
-        const int def = c->prm.precision == VX_PREC_MIXED ? 4 : 2;
+        const int def = direct_tpt_default(c->prm.precision == VX_PREC_MIXED, s);
         double bdef = 0.0, bbest = 1e300, ddef = 0.0, dbest = 1e300;
         for (int q = 0; q < 4; ++q) {
             c->tune.tpt_bs = tpts[q];

@@ -177,6 +177,7 @@ struct vx_case {
     // src/main.c:744): points, last velocities, their split-K partials
     vx::DBuf<double> prb_x, prb_y, prb_u, prb_v, prb_part;
     int64_t prb_m = 0;
+    bool prb_in_speed = false;  // vx_time_step: the step's probes go over the tree in vx_speed
     // Morton tree of the last velocity stage
     vx::TreeBufs *tree = nullptr;
     // FP32 LU of the panel system with FP64 refinement (VX_FLAG_LU_REFINE)
@@ -260,14 +261,16 @@ int launch_right_part(vx_case *c, Cnt n_src, double vinf_x, double vinf_y);
 int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vinf_y); // c->px.. -> c->pu,pv
 // The probe kernels on explicit buffers: part holds the split-K partials (at
 // most max_part doubles, max_part <= 0: as many as the split wants, see
-// probe_split); packed: mixed precision reads the FP32 copies (packed first)
-// instead of converting the FP64 state while staging.
+// probe_split).
 void probe_split(int64_t m, int64_t n_src, int64_t max_part, int64_t *chunk, int64_t *nchunk);
 int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, int64_t n_src, double vinf_x,
-                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed);
+                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part);
 int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, Cnt n_src, double vinf_x,
-                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed);
+                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part);
 int launch_step_probes(vx_case *c, double vinf_x, double vinf_y);  // c->prb_x.. -> c->prb_u, prb_v
+// The probes over the tree of the last launch_tree_biot_savart, on stream st
+int launch_tree_probes(vx_case *c, const double *px, const double *py, int64_t m, double vinf_x,
+                       double vinf_y, double *pu, double *pv, hipStream_t st);
 // The probes on caller-owned device buffers with the partials in the
 // library's file-scope device scratch (no allocation; calls that overlap on
 // one GPU race for it).
@@ -326,6 +329,8 @@ int launch_live_box(vx_case *c);                       // box of g != 0 in [0, *
 // just ran) it searches nothing if that pass left the count unchanged.
 // Collapse error flag in dcount[3].
 int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare, bool after_pass = false);
+// default targets per thread of the direct kernels for n_tgt targets (vx_pairwise.hip)
+int direct_tpt_default(bool f32, int64_t n_tgt);
 int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared);
 
 }  // namespace vx

@@ -589,61 +589,78 @@ __global__ __launch_bounds__(kTile) void k_right_part_final(const double *__rest
 }
 
 // ---- A19: src/kernel.cu:1434-1475, split over source chunks ---------------
-// Sources stored as S (FP32 copies, or the FP64 state converted while staged:
-// the same FP32 values either way), arithmetic in T.
-template <typename T, typename S>
-__device__ __forceinline__ void stage_tile_cvt(Src4<T> *ls, const S *__restrict__ sx, const S *__restrict__ sy,
-                                               const S *__restrict__ sg, int64_t base, int64_t n_src) {
-    const int64_t j = base + threadIdx.x;
-    Src4<T> s;
-    if (j < n_src) {
-        s.x = (T)sx[j];
-        s.y = (T)sy[j];
-        s.g = (T)sg[j];
-    } else {
-        s.x = Arith<T>::kFar;
-        s.y = Arith<T>::kFar;
-        s.g = T(0);
-    }
-    s.pad = T(0);
-    ls[threadIdx.x] = s;
-}
-
-template <typename T, typename S = T>
-__global__ __launch_bounds__(kTile) void k_probe_partial(
-    const S *__restrict__ sx, const S *__restrict__ sy, const S *__restrict__ sg, Cnt cn_src,
-    int64_t chunk, const double *__restrict__ px, const double *__restrict__ py, int64_t m, T eps2,
-    double *__restrict__ part) {
-    __shared__ Src4<T> ls[kTile];
-    using A = Arith<T>;
+__global__ __launch_bounds__(kTile) void k_probe_partial(const double *__restrict__ sx, const double *__restrict__ sy,
+                                                         const double *__restrict__ sg, Cnt cn_src, int64_t chunk,
+                                                         const double *__restrict__ px, const double *__restrict__ py,
+                                                         int64_t m, double eps2, double *__restrict__ part) {
+    __shared__ Src4<double> ls[kTile];
+    using A = Arith<double>;
     const int64_t n_src = cn_src.get();
     const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
     const int64_t c = blockIdx.y;
     const int64_t lo = c * chunk, hi = min(n_src, lo + chunk);
-    const T a0 = i < m ? (T)px[i] : T(0), a1 = i < m ? (T)py[i] : T(0);
-    T su = 0, sv = 0;
-    double au = 0.0, av = 0.0;
+    const double a0 = i < m ? px[i] : 0.0, a1 = i < m ? py[i] : 0.0;
+    double su = 0.0, sv = 0.0;
     for (int64_t base = lo; base < hi; base += kTile) {
         __syncthreads();
-        stage_tile_cvt<T, S>(ls, sx, sy, sg, base, hi);
+        const int64_t j = base + threadIdx.x;
+        Src4<double> s;
+        s.x = j < hi ? sx[j] : A::kFar;
+        s.y = j < hi ? sy[j] : A::kFar;
+        s.g = j < hi ? sg[j] : 0.0;
+        s.pad = 0.0;
+        ls[threadIdx.x] = s;
         __syncthreads();
         const int cnt = (int)min((int64_t)kTile, hi - base);
         for (int jj = 0; jj < cnt; ++jj) {
-            const Src4<T> s = ls[jj];
-            const T dx = a0 - s.x, dy = a1 - s.y;
-            const T w = A::div(s.g, A::max_(dx * dx + dy * dy, eps2));
+            const Src4<double> q = ls[jj];
+            const double dx = a0 - q.x, dy = a1 - q.y;
+            const double w = A::div(q.g, A::max_(dx * dx + dy * dy, eps2));
             sv += w * dx;
             su -= w * dy;
         }
-        if constexpr (sizeof(T) == 4) {
-            au += (double)su;
-            av += (double)sv;
-            su = sv = T(0);
-        }
     }
-    if constexpr (sizeof(T) == 8) {
-        au = su;
-        av = sv;
+    if (i < m) {
+        part[(c * 2 + 0) * m + i] = su;
+        part[(c * 2 + 1) * m + i] = sv;
+    }
+}
+
+// Mixed precision: each pair's separation is taken in FP64 and rounded once,
+// so a probe next to a source keeps FP32's relative accuracy in the
+// separation (FP32 positions would lose it in the coordinates' magnitude:
+// 1e-7 of |x| against separations down to the core radius); kernel and sums
+// in FP32 per tile, FP64 across tiles.
+__global__ __launch_bounds__(kTile) void k_probe_partial_mixed(
+    const double *__restrict__ sx, const double *__restrict__ sy, const double *__restrict__ sg, Cnt cn_src,
+    int64_t chunk, const double *__restrict__ px, const double *__restrict__ py, int64_t m, float eps2,
+    double *__restrict__ part) {
+    __shared__ double lx[kTile], ly[kTile];
+    __shared__ float lg[kTile];
+    using A = Arith<float>;
+    const int64_t n_src = cn_src.get();
+    const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    const int64_t lo = c * chunk, hi = min(n_src, lo + chunk);
+    const double a0 = i < m ? px[i] : 0.0, a1 = i < m ? py[i] : 0.0;
+    double au = 0.0, av = 0.0;
+    for (int64_t base = lo; base < hi; base += kTile) {
+        __syncthreads();
+        const int64_t j = base + threadIdx.x;
+        lx[threadIdx.x] = j < hi ? sx[j] : 0.0;
+        ly[threadIdx.x] = j < hi ? sy[j] : 0.0;
+        lg[threadIdx.x] = j < hi ? (float)sg[j] : 0.f;
+        __syncthreads();
+        const int cnt = (int)min((int64_t)kTile, hi - base);
+        float su = 0.f, sv = 0.f;
+        for (int jj = 0; jj < cnt; ++jj) {
+            const float dx = (float)(a0 - lx[jj]), dy = (float)(a1 - ly[jj]);
+            const float w = A::div(lg[jj], A::max_(dx * dx + dy * dy, eps2));
+            sv += w * dx;
+            su -= w * dy;
+        }
+        au += (double)su;
+        av += (double)sv;
     }
     if (i < m) {
         part[(c * 2 + 0) * m + i] = au;
@@ -686,9 +703,18 @@ int launch_biot_savart(vx_case *c, int64_t n_tgt, int64_t n_src, double vinf_x, 
 // vx_case_autotune) or the default, 4 in FP32 and 2 in FP64.  Every choice
 // sums each target's sources in the same order, so the results do not depend
 // on it.
-static int tpt_of(int want, bool f32) {
+// Default targets per thread of the direct kernels: 4 (FP32) or 2 (FP64),
+// fewer while the grid would give the 256 CUs less than two work-groups each
+// (the sums, and so the results, do not depend on it).
+int direct_tpt_default(bool f32, int64_t n_tgt) {
+    int tpt = f32 ? 4 : 2;
+    while (tpt > 1 && (n_tgt + kTile * tpt - 1) / (kTile * tpt) < 512) tpt >>= 1;
+    return tpt;
+}
+
+static int tpt_of(int want, bool f32, int64_t n_tgt) {
     if (want == 1 || want == 2 || want == 4 || want == 8) return want;
-    return f32 ? 4 : 2;
+    return direct_tpt_default(f32, n_tgt);
 }
 
 template <typename T, int TPT>
@@ -715,11 +741,11 @@ int launch_biot_savart(vx_case *c, Cnt n_tgt, Cnt n_src, double vinf_x, double v
     const double eps2 = c->prm.ve_size * c->prm.ve_size;
     if (mixed(c)) {
         if (launch_pack_f32(c, n_src)) return 1;
-        bs_dispatch<float>(c, tpt_of(c->tune.tpt_bs, true), c->xf.p, c->yf.p, c->gf.p, n_src, n_tgt, (float)eps2,
-                           vinf_x, vinf_y);
+        bs_dispatch<float>(c, tpt_of(c->tune.tpt_bs, true, n_tgt.ub), c->xf.p, c->yf.p, c->gf.p, n_src, n_tgt,
+                           (float)eps2, vinf_x, vinf_y);
     } else {
-        bs_dispatch<double>(c, tpt_of(c->tune.tpt_bs, false), c->x.p, c->y.p, c->g.p, n_src, n_tgt, eps2, vinf_x,
-                            vinf_y);
+        bs_dispatch<double>(c, tpt_of(c->tune.tpt_bs, false, n_tgt.ub), c->x.p, c->y.p, c->g.p, n_src, n_tgt, eps2,
+                            vinf_x, vinf_y);
     }
     VX_CHECK(hipGetLastError());
     return 0;
@@ -752,9 +778,11 @@ int launch_diffusion(vx_case *c, Cnt n_tgt, Cnt n_src) {
     const double thr = 0.001 * c->prm.ve_size;
     if (mixed(c)) {
         if (launch_pack_f32(c, n_src)) return 1;
-        diff_dispatch<float>(c, tpt_of(c->tune.tpt_diff, true), c->xf.p, c->yf.p, c->gf.p, n_src, n_tgt, thr);
+        diff_dispatch<float>(c, tpt_of(c->tune.tpt_diff, true, n_tgt.ub), c->xf.p, c->yf.p, c->gf.p, n_src, n_tgt,
+                             thr);
     } else {
-        diff_dispatch<double>(c, tpt_of(c->tune.tpt_diff, false), c->x.p, c->y.p, c->g.p, n_src, n_tgt, thr);
+        diff_dispatch<double>(c, tpt_of(c->tune.tpt_diff, false, n_tgt.ub), c->x.p, c->y.p, c->g.p, n_src, n_tgt,
+                              thr);
     }
     VX_CHECK(hipGetLastError());
     return 0;
@@ -830,12 +858,12 @@ void probe_split(int64_t m, int64_t n_src, int64_t max_part, int64_t *chunk, int
 }
 
 int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, int64_t n_src, double vinf_x,
-                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed) {
-    return launch_probe_to(c, px, py, m, host_cnt(n_src), vinf_x, vinf_y, pu, pv, part, max_part, packed);
+                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part) {
+    return launch_probe_to(c, px, py, m, host_cnt(n_src), vinf_x, vinf_y, pu, pv, part, max_part);
 }
 
 int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, Cnt cn_src, double vinf_x,
-                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part, bool packed) {
+                    double vinf_y, double *pu, double *pv, double *part, int64_t max_part) {
     if (m <= 0) return 0;
     const int64_t tb = (m + kTile - 1) / kTile;
     const int64_t n_src = cn_src.ub;  // the chunking follows the bound
@@ -843,15 +871,11 @@ int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, C
     probe_split(m, n_src, max_part, &chunk, &nchunk);
     const double eps2 = c->prm.ve_size * c->prm.ve_size;
     const dim3 grid((unsigned)tb, (unsigned)nchunk);
-    if (mixed(c) && packed) {
-        if (launch_pack_f32(c, cn_src)) return 1;
-        hipLaunchKernelGGL((k_probe_partial<float, float>), grid, dim3(kTile), 0, c->stream, c->xf.p, c->yf.p,
-                           c->gf.p, cn_src, chunk, px, py, m, (float)eps2, part);
-    } else if (mixed(c)) {
-        hipLaunchKernelGGL((k_probe_partial<float, double>), grid, dim3(kTile), 0, c->stream, c->x.p, c->y.p,
-                           c->g.p, cn_src, chunk, px, py, m, (float)eps2, part);
+    if (mixed(c)) {
+        hipLaunchKernelGGL(k_probe_partial_mixed, grid, dim3(kTile), 0, c->stream, c->x.p, c->y.p, c->g.p, cn_src,
+                           chunk, px, py, m, (float)eps2, part);
     } else {
-        hipLaunchKernelGGL((k_probe_partial<double, double>), grid, dim3(kTile), 0, c->stream, c->x.p, c->y.p,
+        hipLaunchKernelGGL(k_probe_partial, grid, dim3(kTile), 0, c->stream, c->x.p, c->y.p,
                            c->g.p, cn_src, chunk, px, py, m, eps2, part);
     }
     VX_CHECK(hipGetLastError());
@@ -866,7 +890,7 @@ int launch_probe(vx_case *c, int64_t m, int64_t n_src, double vinf_x, double vin
     int64_t chunk, nchunk;
     probe_split(m, n_src, 0, &chunk, &nchunk);
     if (c->partial.ensure((size_t)(2 * nchunk * m), c->stream)) return 1;
-    return launch_probe_to(c, c->px.p, c->py.p, m, n_src, vinf_x, vinf_y, c->pu.p, c->pv.p, c->partial.p, 0, true);
+    return launch_probe_to(c, c->px.p, c->py.p, m, n_src, vinf_x, vinf_y, c->pu.p, c->pv.p, c->partial.p, 0);
 }
 
 // The probes of vx_case_set_probes, sources [0, ceil32(n)) of the current
@@ -880,7 +904,7 @@ int launch_step_probes(vx_case *c, double vinf_x, double vinf_y) {
     probe_split(m, s.ub, 0, &chunk, &nchunk);
     if (c->prb_part.ensure((size_t)(2 * nchunk * m), c->stream)) return 1;
     return launch_probe_to(c, c->prb_x.p, c->prb_y.p, m, s, vinf_x, vinf_y, c->prb_u.p, c->prb_v.p, c->prb_part.p,
-                           0, false);
+                           0);
 }
 
 // ---- velocity probes without allocation ---------------------------------------
@@ -926,7 +950,7 @@ int launch_probe_noalloc(vx_case *c, const double *px, const double *py, int64_t
     double *part = nullptr;
     if (const int rc = probe_scratch(c->device, &part)) return rc;
     const int64_t n_src = std::min<int64_t>((c->n + 31) / 32 * 32, c->cap);
-    return launch_probe_to(c, px, py, m, n_src, vinf_x, vinf_y, pu, pv, part, kProbeScratch, false);
+    return launch_probe_to(c, px, py, m, n_src, vinf_x, vinf_y, pu, pv, part, kProbeScratch);
 }
 
 }  // namespace vx

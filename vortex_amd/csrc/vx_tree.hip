@@ -2395,6 +2395,182 @@ int tree_build(vx_case *c, int64_t s) {
 
 static double theta_of(const vx_case *c) { return c->prm.theta > 0 ? c->prm.theta : 0.5; }
 
+// Velocities at probe points over the tree of the preceding
+// launch_tree_biot_savart (A19's sums, src/kernel.cu:1434-1475: every source,
+// max(r^2, eps^2), no nearest-3).  One wave per 8 consecutive probes, 8 lanes
+// per probe: the traversal of k_tree_bs with the 8 probes' box as the target
+// circle; each probe's far nodes (M2P at the probe, as the FP64 Biot-Savart
+// path evaluates them) and each near bucket's sources are dealt round-robin
+// to its 8 lanes, and the lanes' sums are added in a fixed order at the end.
+// A few hundred probes give a hundred waves of short traversals, where 64
+// probes per wave left a dozen long ones.  FP32: positions relative to the
+// source bucket's centre or the node's, differenced in FP64; sums in FP64 per
+// bucket and node.
+template <typename T, int ORDER, int STRIDE>
+__global__ __launch_bounds__(256) void k_tree_probe(TreeDev t, const double *__restrict__ px,
+                                                    const double *__restrict__ py, int64_t m, double theta,
+                                                    double eps, double vinf_x, double vinf_y,
+                                                    double *__restrict__ pu, double *__restrict__ pv) {
+    using PM = PairMath<T>;
+    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int kSub = 8, kPer = 64 / kSub;
+    __shared__ Src4T<T> lds_all[4][kLeaf];
+    __shared__ int32_t stk_all[4][kBsStack];
+    __shared__ int32_t lq_all[4][64];
+    __shared__ int32_t fq_all[4][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, sub = lane & (kSub - 1);
+    Src4T<T> *lds = lds_all[wid];
+    int32_t *stk = stk_all[wid];
+    int32_t *lq = lq_all[wid];
+    int32_t *fq = fq_all[wid];
+    const int64_t base = ((int64_t)blockIdx.x * 4 + wid) * kPer;
+    if (base >= m) return;
+    const int64_t i = base + lane / kSub;
+    const bool ok = i < m;
+    const double qx = px[ok ? i : base], qy = py[ok ? i : base];
+    const double x0 = wave_min(qx), x1 = wave_max(qx), y0 = wave_min(qy), y1 = wave_max(qy);
+    const double cAx = 0.5 * (x0 + x1), cAy = 0.5 * (y0 + y1);
+    const double rA = 0.5 * sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+    const T eps2 = (T)(eps * eps);
+    const double th2 = theta * theta;
+    const uint64_t below = (1ull << lane) - 1;
+    double su = 0.0, sv = 0.0;
+    int sp = 1;
+    if (lane == 0) stk[0] = t.root;
+    while (sp > 0) {
+        const int k = sp + 64 + 64 <= kBsStack ? (sp < 64 ? sp : 64) : 1;
+        __builtin_amdgcn_wave_barrier();
+        const int node = lane < k ? stk[sp - 1 - lane] : -1;
+        __builtin_amdgcn_wave_barrier();
+        sp -= k;
+        bool isfar = false, isleaf = false, isopen = false;
+        int ca = 0, cb = 0;
+        if (node >= 0) {
+            const NodeRec nd = t.rec[node];
+            const double ddx = nd.cx - cAx, ddy = nd.cy - cAy;
+            const double d2 = ddx * ddx + ddy * ddy;
+            const double rs = nd.r + rA;
+            isfar = d2 * th2 > rs * rs && d2 > (rs + eps) * (rs + eps);
+            isleaf = !isfar && nd.a < 0;
+            isopen = !isfar && !isleaf;
+            ca = nd.a;
+            cb = nd.b;
+        }
+        const uint64_t om = __ballot(isopen);
+        if (isopen) {
+            const int pos = sp + 2 * __popcll(om & below);
+            stk[pos] = cb;
+            stk[pos + 1] = ca;
+        }
+        sp += 2 * __popcll(om);
+        const uint64_t lm = __ballot(isleaf);
+        if (isleaf) lq[__popcll(lm & below)] = node;
+        const int nl = __popcll(lm);
+        const uint64_t fm = __ballot(isfar);
+        if (isfar) fq[__popcll(fm & below)] = node;
+        const int nf = __popcll(fm);
+        __builtin_amdgcn_wave_barrier();
+        // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
+        for (int fi = sub; fi < nf; fi += kSub) {
+            const int fnode = fq[fi];
+            const NodeRec nd = t.rec[fnode];
+            const T Dx = (T)(qx - nd.cx), Dy = (T)(qy - nd.cy);
+            const T den = Dx * Dx + Dy * Dy;
+            const T inv = PM::div(T(1), den);
+            const T qre = Dx * inv, qim = -Dy * inv;
+            const T R = (T)fmax(nd.r, kTiny);
+            const T tx = R * qre, ty = R * qim;
+            T sr, si;
+            if constexpr (F32) {
+                const float2 *bm = t.cf + (int64_t)fnode * STRIDE;
+                sr = bm[ORDER - 1].x;
+                si = bm[ORDER - 1].y;
+#pragma unroll
+                for (int q = ORDER - 2; q >= 0; --q) {
+                    const T nr_ = sr * tx - si * ty + bm[q].x;
+                    si = sr * ty + si * tx + bm[q].y;
+                    sr = nr_;
+                }
+            } else {
+                const double2 *bm = t.cd + (int64_t)fnode * STRIDE;
+                sr = bm[ORDER - 1].x;
+                si = bm[ORDER - 1].y;
+#pragma unroll
+                for (int q = ORDER - 2; q >= 0; --q) {
+                    const T nr_ = sr * tx - si * ty + bm[q].x;
+                    si = sr * ty + si * tx + bm[q].y;
+                    sr = nr_;
+                }
+            }
+            su += (double)(sr * qim + si * qre);
+            sv += (double)(sr * qre - si * qim);
+        }
+        for (int li = 0; li < nl; ++li) {
+            const int lnode = __builtin_amdgcn_readfirstlane(lq[li]);
+            const NodeRec nd = t.rec[lnode];
+            const int cnt = nd.b;
+            stage_leaf<T>(lds, lane, -1 - nd.a, cnt, t);
+            // FP32 sources are staged relative to their bucket's centre
+            const T ox = F32 ? (T)(qx - nd.cx) : (T)qx, oy = F32 ? (T)(qy - nd.cy) : (T)qy;
+            T pu_ = 0, pv_ = 0;
+            for (int jj = sub; jj < cnt; jj += kSub) {
+                const Src4T<T> q = lds[jj];
+                const T dx = ox - q.x, dy = oy - q.y;
+                const T r2 = dx * dx + dy * dy;
+                const T w = PM::div(q.g, r2 > eps2 ? r2 : eps2);
+                pv_ += w * dx;
+                pu_ -= w * dy;
+            }
+            su += (double)pu_;
+            sv += (double)pv_;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int o = 1; o < kSub; o <<= 1) {
+        su += __shfl_xor(su, o);
+        sv += __shfl_xor(sv, o);
+    }
+    if (!ok || sub != 0) return;
+    pu[i] = su / (2 * kPi) + vinf_x;
+    pv[i] = sv / (2 * kPi) + vinf_y;
+}
+
+int launch_tree_probes(vx_case *c, const double *px, const double *py, int64_t m, double vinf_x, double vinf_y,
+                       double *pu, double *pv, hipStream_t st) {
+    if (m <= 0) return 0;
+    if (!c->tree || c->tree->m == 0) {
+        set_error("internal: tree probes without a tree");
+        return 1;
+    }
+    TreeBufs &t = *c->tree;
+    const TreeDev tv = view(t);
+    const unsigned nb = (unsigned)((m + 31) / 32);  // 4 waves of 8 probes
+    const double th = theta_of(c), eps = c->prm.ve_size;
+    const bool mixed = c->prm.precision == VX_PREC_MIXED;
+#define VX_TREE_PROBE(T_, P_, S_)                                                                               \
+    hipLaunchKernelGGL((k_tree_probe<T_, P_, S_>), dim3(nb), dim3(256), 0, st, tv, px, py, m, th, eps, vinf_x, \
+                       vinf_y, pu, pv)
+    if (mixed) {
+        if (t.terms == 32)
+            VX_TREE_PROBE(float, 32, 32);
+        else if (t.terms == 20)
+            VX_TREE_PROBE(float, 20, 32);
+        else
+            VX_TREE_PROBE(float, 16, 16);
+    } else {
+        if (t.terms == 32)
+            VX_TREE_PROBE(double, 32, 32);
+        else if (t.terms == 20)
+            VX_TREE_PROBE(double, 20, 32);
+        else
+            VX_TREE_PROBE(double, 16, 16);
+    }
+#undef VX_TREE_PROBE
+    VX_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y) {
     {
         Timer tb(c, &c->tm.tree_build_ms, 5);
