@@ -198,8 +198,10 @@ int vx_case_destroy(vx_case *c) {
     if (c->h_red) (void)hipHostFree(c->h_red);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {c->fork_ev, c->join_ev, c->pre_ev, c->born_ev})
+    for (hipEvent_t e : {c->fork_ev, c->join_ev, c->pre_ev, c->born_ev, c->aux_fork, c->aux_done[0], c->aux_done[1]})
         if (e) (void)hipEventDestroy(e);
+    for (hipStream_t a : c->aux)
+        if (a) (void)hipStreamDestroy(a);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -598,23 +600,21 @@ int vx_speed(vx_case *c, double vinf_x, double vinf_y) {
             return 1;
         c->tm.bs_launches++;
     }
-    // The tree's domain-scale diffusion targets, then the step's probes over
-    // the tree, run on the side stream, overlapping the groups' diffusion on
-    // the main stream; the wall's terms are added after the join.
+    // The wall term, the tree's domain-scale diffusion targets and the step's
+    // probes over the tree run on the side stream, overlapping the groups'
+    // diffusion on the main stream (the side stream's work, slowed down
+    // beside it, still ends first); the wall's terms are staged and added
+    // after the join.
     int rc = 0;
     {
         Timer td(c, &c->tm.diff_ms, 3);
-        // The wall term first, on the main stream: beside the diffusion
-        // kernel its short selection and one-wave-per-target kernel ran
-        // several times slower than alone.  Its terms are staged and added
-        // after the diffusion, in the same order as before.
-        if (c->timing) (void)hipEventRecord(c->ev[14], c->stream);
-        if (launch_wall_side(c, s, c->stream)) return 1;
-        if (c->timing) (void)hipEventRecord(c->ev[15], c->stream);
         if ((tree && tree_diffusion_pre(c)) || side_fork(c)) return 1;
         // the long main-stream kernel is issued first; the side stream's
         // work is ordered after the fork point only
         rc = tree ? tree_diffusion_main(c) : launch_diffusion(c, s, s);
+        if (c->timing) (void)hipEventRecord(c->ev[14], c->side);
+        if (!rc) rc = launch_wall_side(c, s, c->side);
+        if (c->timing) (void)hipEventRecord(c->ev[15], c->side);
         if (!rc && tree) rc = tree_diffusion_far(c, c->side);
         if (!rc && tree && c->prb_in_speed)
             rc = launch_tree_probes(c, c->prb_x.p, c->prb_y.p, c->prb_m, vinf_x, vinf_y, c->prb_u.p, c->prb_v.p,

@@ -125,6 +125,11 @@ struct vx_case {
     // fork_ev orders it after the main stream, join_ev the main stream after it
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // two more streams (default priority) for the independent chains of a
+    // tree build: aux_fork orders them after the build's stream, aux_done[k]
+    // the build's stream after stream k's chain
+    hipStream_t aux[2] = {nullptr, nullptr};
+    hipEvent_t aux_fork = nullptr, aux_done[2] = {nullptr, nullptr};
     bool wall_pending = false;                        // a wall pass forked and not yet joined
     // Within vx_time_step, the tree build of the speed stage starts on the
     // side stream while the birth stage runs on the main stream: pre_ev marks
@@ -237,6 +242,7 @@ int launch_fill_tail(vx_case *c, int64_t from, int64_t to);
 int side_stream(vx_case *c);  // creates the side stream and its events once
 int side_fork(vx_case *c);  // the side stream waits for the work queued so far on the main stream
 int side_join(vx_case *c);  // the main stream waits for the work queued so far on the side stream
+int aux_streams(vx_case *c);  // creates aux[] and their events once
 int launch_pack_f32(vx_case *c, int64_t count);
 int launch_pack_f32(vx_case *c, Cnt count);
 int launch_count_add(vx_case *c, int64_t *a, int64_t v);  // *a += v (stream-ordered)

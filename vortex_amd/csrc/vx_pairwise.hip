@@ -305,6 +305,7 @@ __global__ __launch_bounds__(kSegTile) void k_right_part_partial(
     const int64_t c = blockIdx.y;
     const int64_t n_src = *nlist;
     const int64_t lo = c * kRhsChunk, hi = min(n_src, lo + kRhsChunk);
+    if (lo >= n_src && lo > 0) return;  // a chunk k_right_part_final does not read
     const bool live = (int)threadIdx.x < S.count;
     const int64_t i = S.first + (live ? threadIdx.x : 0);
     // FP32: panel ends and sources relative to the segment centre, the panel
@@ -547,15 +548,19 @@ __device__ __forceinline__ void chunk_sum2(const double *__restrict__ part, int6
 }
 
 // One work-group per panel: the near chunk partials, plus the panel average
-// of its segment's expansion (thread 0).
+// of its segment's expansion (thread 0).  Only the chunks of the near list's
+// length hold partials (the grid is sized for its bound); the others would
+// add exact zeros.
 __global__ __launch_bounds__(kTile) void k_right_part_final(const double *__restrict__ part, int64_t nchunk,
-                                                            PanelsSoA pan, const double *__restrict__ Ss,
+                                                            const int32_t *__restrict__ nlist, PanelsSoA pan,
+                                                            const double *__restrict__ Ss,
                                                             const RhsSeg *__restrict__ segs,
                                                             const int32_t *__restrict__ seg_of, double vinf_x,
                                                             double vinf_y, double *__restrict__ R) {
     const int64_t i = blockIdx.x;
+    const int64_t used = ((int64_t)*nlist + kRhsChunk - 1) / kRhsChunk;
     double y0, y1;
-    chunk_sum2(part, nchunk, pan.P, i, y0, y1);
+    chunk_sum2(part, used < nchunk ? used : nchunk, pan.P, i, y0, y1);
     if (threadIdx.x != 0) return;
     const int32_t sg = seg_of[i];
     const double cx = segs[sg].cx, cy = segs[sg].cy;
@@ -837,8 +842,8 @@ int launch_right_part(vx_case *c, Cnt cn_src, double vinf_x, double vinf_y) {
                        c->g.p, c->lists.p, nlist, c->segs, spart);
     hipLaunchKernelGGL(k_rhs_seg_final, dim3((unsigned)K), dim3(64), 0, st, spart, S, cx, cy, c->segs, Ss);
     VX_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)P), dim3(kTile), 0, st, c->partial.p, nchunk, c->pan, Ss,
-                       c->segs, c->seg_of, vinf_x, vinf_y, c->rp.p);
+    hipLaunchKernelGGL(k_right_part_final, dim3((unsigned)P), dim3(kTile), 0, st, c->partial.p, nchunk, nlist, c->pan,
+                       Ss, c->segs, c->seg_of, vinf_x, vinf_y, c->rp.p);
     VX_CHECK(hipGetLastError());
     return 0;
 }

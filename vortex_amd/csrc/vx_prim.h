@@ -119,14 +119,41 @@ __global__ __launch_bounds__(1024) void k_tile_scan(int32_t *__restrict__ tot, i
     if (threadIdx.x == 1023 && total) *total = (TT)(ws[15]);
 }
 
+// The first value of a tile's run: with SUMS, tile holds the tiles' totals
+// and the work-group reduces those before it (few tiles: each reads at most
+// kFuseTiles of them), and the last work-group stores the grand total;
+// else tile holds the tiles' exclusive offsets (k_tile_scan).  Integer sums
+// and maxima do not depend on the order of reduction.
+constexpr int64_t kFuseTiles = 2048;
+
+template <bool SUMS, class Op, class TT>
+__device__ __forceinline__ int32_t tile_start(const int32_t *__restrict__ tile, int64_t nt, Op op, TT *total) {
+    if constexpr (!SUMS) {
+        return tile[blockIdx.x];
+    } else {
+        __shared__ int32_t ws[kW];
+        int32_t a = Op::id;
+        for (int64_t k = threadIdx.x; k < (int64_t)blockIdx.x; k += kT) a = op(a, tile[k]);
+        a = wave_reduce(a, op);
+        if (lane_id() == 0) ws[threadIdx.x >> 6] = a;
+        __syncthreads();
+        int32_t pre = Op::id;
+#pragma unroll
+        for (int q = 0; q < kW; ++q) pre = op(pre, ws[q]);
+        if (total && threadIdx.x == 0 && (int64_t)blockIdx.x == nt - 1) *total = (TT)op(pre, tile[blockIdx.x]);
+        return pre;
+    }
+}
+
 // Phase 3: each flagged item's rank among the flagged items (its output
 // position); the writer sees every item i < n with its flag.
-template <class F, class Wr>
-__global__ __launch_bounds__(kT) void k_flag_apply(F f, int64_t n, const int32_t *__restrict__ tile_off, Wr wr) {
+template <class F, class Wr, bool SUMS = false, class TT = int32_t>
+__global__ __launch_bounds__(kT) void k_flag_apply(F f, int64_t n, const int32_t *__restrict__ tile_off, Wr wr,
+                                                   TT *total = nullptr) {
     __shared__ int32_t wc[2][kW];
     const int64_t base = (int64_t)blockIdx.x * kTile;
     const int w = threadIdx.x >> 6;
-    int32_t run = tile_off[blockIdx.x];
+    int32_t run = tile_start<SUMS>(tile_off, (n + kTile - 1) / kTile, OpSum(), total);
     for (int r = 0; r < kIPT; ++r) {
         const int64_t i = base + r * kT + threadIdx.x;
         const int fl = i < n ? f(i) : 0;
@@ -167,13 +194,13 @@ __global__ __launch_bounds__(kT) void k_value_reduce(V val, int64_t n, int32_t *
 }
 
 // INCL: the writer gets the inclusive scan, else the exclusive one.
-template <class V, class Op, bool INCL, class Wr>
+template <class V, class Op, bool INCL, class Wr, bool SUMS = false, class TT = int32_t>
 __global__ __launch_bounds__(kT) void k_value_apply(V val, int64_t n, const int32_t *__restrict__ tile_off, Op op,
-                                                    Wr wr) {
+                                                    Wr wr, TT *total = nullptr) {
     __shared__ int32_t wc[2][kW];
     const int64_t base = (int64_t)blockIdx.x * kTile;
     const int w = threadIdx.x >> 6;
-    int32_t run = tile_off[blockIdx.x];
+    int32_t run = tile_start<SUMS>(tile_off, (n + kTile - 1) / kTile, op, total);
     for (int r = 0; r < kIPT; ++r) {
         const int64_t i = base + r * kT + threadIdx.x;
         const int32_t v = i < n ? val(i) : Op::id;
@@ -197,14 +224,22 @@ __global__ __launch_bounds__(kT) void k_value_apply(V val, int64_t n, const int3
 // tmp must hold tiles_of(n) int32 entries.  total (device) receives the count
 // of flagged items (or the value scan's total), converted to TT.
 
+// Up to kFuseTiles tiles the apply launch reduces the tile totals itself
+// (two launches); beyond, k_tile_scan turns them into offsets first.
 template <class F, class Wr, class TT>
 inline hipError_t select_flags(F f, int64_t n, Wr wr, int32_t *tmp, TT *total, hipStream_t st) {
     const int64_t nt = tiles_of(n);
     if (nt > 0) hipLaunchKernelGGL(k_flag_count<F>, dim3((unsigned)nt), dim3(kT), 0, st, f, n, tmp);
+    if (nt > 0 && nt <= kFuseTiles) {
+        hipLaunchKernelGGL((k_flag_apply<F, Wr, true, TT>), dim3((unsigned)nt), dim3(kT), 0, st, f, n,
+                           (const int32_t *)tmp, wr, total);
+        return hipGetLastError();
+    }
     // with no tiles this only writes a zero total
     hipLaunchKernelGGL((k_tile_scan<TT, OpSum>), dim3(1), dim3(1024), 0, st, tmp, nt, total, OpSum());
     if (nt > 0)
-        hipLaunchKernelGGL((k_flag_apply<F, Wr>), dim3((unsigned)nt), dim3(kT), 0, st, f, n, (const int32_t *)tmp, wr);
+        hipLaunchKernelGGL((k_flag_apply<F, Wr>), dim3((unsigned)nt), dim3(kT), 0, st, f, n, (const int32_t *)tmp, wr,
+                           (int32_t *)nullptr);
     return hipGetLastError();
 }
 
@@ -213,9 +248,14 @@ inline hipError_t scan_values(V val, int64_t n, Op op, Wr wr, int32_t *tmp, TT *
     const int64_t nt = tiles_of(n);
     if (nt == 0) return hipSuccess;
     hipLaunchKernelGGL((k_value_reduce<V, Op>), dim3((unsigned)nt), dim3(kT), 0, st, val, n, tmp, op);
+    if (nt <= kFuseTiles) {
+        hipLaunchKernelGGL((k_value_apply<V, Op, INCL, Wr, true, TT>), dim3((unsigned)nt), dim3(kT), 0, st, val, n,
+                           (const int32_t *)tmp, op, wr, total);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL((k_tile_scan<TT, Op>), dim3(1), dim3(1024), 0, st, tmp, nt, total, op);
     hipLaunchKernelGGL((k_value_apply<V, Op, INCL, Wr>), dim3((unsigned)nt), dim3(kT), 0, st, val, n,
-                       (const int32_t *)tmp, op, wr);
+                       (const int32_t *)tmp, op, wr, (int32_t *)nullptr);
     return hipGetLastError();
 }
 

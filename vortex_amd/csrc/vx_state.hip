@@ -37,6 +37,17 @@ int side_stream(vx_case *c) {
     return 0;
 }
 
+int aux_streams(vx_case *c) {
+    if (!c->aux[0]) {
+        for (int k = 0; k < 2; ++k) {
+            VX_CHECK(hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking));
+            VX_CHECK(hipEventCreateWithFlags(&c->aux_done[k], hipEventDisableTiming));
+        }
+        VX_CHECK(hipEventCreateWithFlags(&c->aux_fork, hipEventDisableTiming));
+    }
+    return 0;
+}
+
 int side_fork(vx_case *c) {
     if (side_stream(c)) return 1;
     VX_CHECK(hipEventRecord(c->fork_ev, c->stream));

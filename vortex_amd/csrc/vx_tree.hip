@@ -72,7 +72,7 @@ struct TreeBufs {
     DBuf<float2> cf;
     DBuf<NodeRec> rec;
     DBuf<unsigned long long> cnt;
-    DBuf<unsigned char> tmp;
+    DBuf<unsigned char> tmp, tmp2;  // scan scratch of the build's two concurrent chains
     int64_t s = 0;                         // slots keyed and sorted by the last build
     int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
     int64_t ngroup = 0;                    // target groups (gfirst)
@@ -936,6 +936,7 @@ struct PairMath<double> {
     __device__ static __forceinline__ double med3(double a, double b, double c) {
         return fmax(fmin(a, b), fmin(fmax(a, b), c));
     }
+    __device__ static __forceinline__ double min_(double a, double b) { return fmin(a, b); }
 };
 template <>
 struct PairMath<float> {
@@ -943,6 +944,8 @@ struct PairMath<float> {
     // the hardware square root (1 ulp), for bounds that carry a margin
     __device__ static __forceinline__ float sqrt_bound(float a) { return __builtin_amdgcn_sqrtf(a); }
     __device__ static __forceinline__ float med3(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
+    // min of two non-NaN values as one median (fminf adds a canonicalisation)
+    __device__ static __forceinline__ float min_(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -3.0e38f); }
 };
 
 // One source leaf staged into this wave's LDS slot: one coalesced load per
@@ -955,10 +958,25 @@ struct Src4T {
     T x, y, g, pad;
 };
 
+// Slots past a leaf's sources hold padding: no circulation, and far enough
+// from any target to stay out of the four smallest distances (first bound
+// 1e5).  FP32 slots are relative to their bucket's centre and the targets
+// that read them lie within a few bucket radii of it: at (1000, 1000), r^2
+// is about 2e6, and FP32 products of two squared distances stay finite.
+// FP64 slots hold absolute positions: at (1e30, 1e30).  Waves that give a
+// target several lanes read whole shares of the slot, padding included.
+template <typename T>
+__device__ __forceinline__ Src4T<T> pad_src() {
+    Src4T<T> q;
+    q.x = q.y = sizeof(T) == 4 ? T(1000) : T(1e30);
+    q.g = q.pad = T(0);
+    return q;
+}
+
 template <typename T>
 __device__ __forceinline__ void stage_leaf(Src4T<T> *lds, int lane, int64_t j0, int cnt, const TreeDev &t) {
     __builtin_amdgcn_wave_barrier();
-    Src4T<T> q;
+    Src4T<T> q = pad_src<T>();
     if (lane < cnt) {
         if constexpr (sizeof(T) == 4) {
             q.x = t.fx[j0 + lane];
@@ -969,18 +987,15 @@ __device__ __forceinline__ void stage_leaf(Src4T<T> *lds, int lane, int64_t j0, 
             q.y = t.sy[j0 + lane];
             q.g = t.sg[j0 + lane];
         }
-    } else {
-        q.x = q.y = T(0);
-        q.g = T(0);
     }
-    q.pad = T(0);
     lds[lane] = q;
     __builtin_amdgcn_wave_barrier();
 }
 
 // One source leaf staged like stage_leaf, keeping only the sources within
 // sqrt(R2) of the box [bx0, bx1] x [by0, by1] (in the staged coordinates),
-// packed to the front of the slot in leaf order; returns their count.
+// packed to the front of the slot in leaf order, padding after them;
+// returns their count.
 template <typename T>
 __device__ __forceinline__ int stage_leaf_near(Src4T<T> *lds, int lane, int64_t j0, int cnt, const TreeDev &t, T bx0,
                                                T bx1, T by0, T by1, T R2) {
@@ -1003,9 +1018,21 @@ __device__ __forceinline__ int stage_leaf_near(Src4T<T> *lds, int lane, int64_t 
         keep = ex * ex + ey * ey < R2;
     }
     const uint64_t m = __ballot(keep);
+    const int kept = (int)__popcll(m);
     if (keep) lds[__popcll(m & ((1ull << lane) - 1ull))] = q;
+    if (lane >= kept) lds[lane] = pad_src<T>();
     __builtin_amdgcn_wave_barrier();
-    return (int)__popcll(m);
+    return kept;
+}
+
+// v into the ascending four smallest m0 <= m1 <= m2 <= m3.
+template <typename T>
+__device__ __forceinline__ void near4_insert(T v, T &m0, T &m1, T &m2, T &m3) {
+    using PM = PairMath<T>;
+    m3 = PM::med3(m2, m3, v);
+    m2 = PM::med3(m1, m2, v);
+    m1 = PM::med3(m0, m1, v);
+    m0 = PM::min_(m0, v);
 }
 
 // Pair loop over one staged leaf: Biot-Savart sums and, with TRACK, the four
@@ -1019,9 +1046,10 @@ __device__ __forceinline__ void p2p_leaf(const Src4T<T> *lds, int cnt, T ox, T o
     if constexpr (sizeof(T) == 4) {
         // two sources at a time with one quarter-rate reciprocal:
         // 1/a = b / (a b), 1/b = a / (a b).  For a target lane, a, b lie in
-        // [eps^2, 1e6] (the included vortices lie within 400 of the domain),
-        // so the product neither overflows nor underflows in FP32; lanes
-        // without a target may produce inf or NaN, which are never stored.
+        // [eps^2, 4e6] (the included vortices lie within 400 of the domain,
+        // padding slots about 1400 from the bucket centre), so the product
+        // neither overflows nor underflows in FP32; lanes without a target
+        // may produce inf or NaN, which are never stored.
         for (; jj + 1 < cnt; jj += 2) {
             const Src4T<T> q = lds[jj], p = lds[jj + 1];
             const T dx = ox - q.x, dy = oy - q.y, ex = ox - p.x, ey = oy - p.y;
@@ -1030,11 +1058,11 @@ __device__ __forceinline__ void p2p_leaf(const Src4T<T> *lds, int cnt, T ox, T o
                 m3 = PM::med3(m2, m3, r2);
                 m2 = PM::med3(m1, m2, r2);
                 m1 = PM::med3(m0, m1, r2);
-                m0 = fmin(m0, r2);
+                m0 = PM::min_(m0, r2);
                 m3 = PM::med3(m2, m3, s2);
                 m2 = PM::med3(m1, m2, s2);
                 m1 = PM::med3(m0, m1, s2);
-                m0 = fmin(m0, s2);
+                m0 = PM::min_(m0, s2);
             }
             const T a = CLAMP ? fmax(r2, eps2) : r2, b = CLAMP ? fmax(s2, eps2) : s2;
             const T rab = __builtin_amdgcn_rcpf(a * b);
@@ -1053,7 +1081,7 @@ __device__ __forceinline__ void p2p_leaf(const Src4T<T> *lds, int cnt, T ox, T o
             m3 = PM::med3(m2, m3, r2);
             m2 = PM::med3(m1, m2, r2);
             m1 = PM::med3(m0, m1, r2);
-            m0 = fmin(m0, r2);
+            m0 = PM::min_(m0, r2);
         }
         const T w = PM::div(q.g, CLAMP ? fmax(r2, eps2) : r2);
         pv += w * dx;
@@ -1087,10 +1115,38 @@ __global__ void k_group_first(const int32_t *__restrict__ gflag, const int32_t *
     if (i < m && gflag[i]) gfirst[gscan[i]] = (int32_t)i;
 }
 
+// Lanes per target of a wave that holds nvalid targets: the largest power of
+// two S with nvalid S <= 64.  Target k is held by lanes k + s 64 / S
+// (s = 0 .. S-1), and lane share s sums the s-th of S equal runs of every
+// staged source slot, so a wave of few targets takes S times fewer pair
+// iterations than with one lane per target.
+struct LaneSplit {
+    int per;  // S
+    int tpw;  // targets per wave, 64 / S
+    int tl;   // this lane's target
+    int sub;  // this lane's share
+};
+
+__device__ __forceinline__ LaneSplit lane_split(int nvalid, int lane) {
+    LaneSplit L;
+    int lg = 0;
+    while (lg < 6 && nvalid << (lg + 1) <= 64) ++lg;
+    L.per = 1 << lg;
+    L.tpw = 64 >> lg;
+    L.tl = lane & (L.tpw - 1);
+    L.sub = lane >> (6 - lg);
+    return L;
+}
+
+// The lane's share of a slot of cnt staged sources: ceil(cnt / S) entries
+// from share * that (entries past cnt are padding).
+__device__ __forceinline__ int share_len(const LaneSplit &L, int cnt) { return (cnt + L.per - 1) / L.per; }
+
 struct TargetGroup {
     int64_t ti;
     bool ok;
     int nvalid;
+    LaneSplit ls;
     double cx, cy, r;
     double4 box;
 };
@@ -1099,9 +1155,10 @@ __device__ __forceinline__ TargetGroup target_group(const TreeDev &t, int64_t gr
     TargetGroup G;
     const int64_t f = t.gfirst[grp];
     const int64_t e = grp + 1 < t.ngroup ? (int64_t)t.gfirst[grp + 1] : t.m;
-    G.ti = f + lane;
     G.nvalid = (int)(e - f);
-    G.ok = lane < G.nvalid;
+    G.ls = lane_split(G.nvalid, lane);
+    G.ti = f + G.ls.tl;
+    G.ok = G.ls.tl < G.nvalid;
     const double xi = G.ok ? t.sx[G.ti] : 0.0, yi = G.ok ? t.sy[G.ti] : 0.0;
     const double x0 = wave_min(G.ok ? xi : 1e300), x1 = wave_max(G.ok ? xi : -1e300);
     const double y0 = wave_min(G.ok ? yi : 1e300), y1 = wave_max(G.ok ? yi : -1e300);
@@ -1326,9 +1383,11 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     const int leaf = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
     if (leaf >= t.nleaf) return;
     const NodeRec L = t.rec[leaf];
-    const int64_t ti = (int64_t)(-1 - L.a) + lane;
     const int nvalid = L.b;
-    const bool ok = lane < nvalid;
+    // a bucket of at most 32 particles gives each target several lanes
+    const LaneSplit ls = lane_split(nvalid, lane);
+    const int64_t ti = (int64_t)(-1 - L.a) + ls.tl;
+    const bool ok = ls.tl < nvalid;
     const double cAx = L.cx, cAy = L.cy, rA = L.r;
     T ax, ay;
     if constexpr (F32) {
@@ -1359,7 +1418,8 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     {
         stage_leaf<T>(lds, lane, -1 - L.a, nvalid, t);
         T pu = 0, pv = 0;
-        p2p_leaf<T, true>(lds, nvalid, ax, ay, eps2, pu, pv, m0, m1, m2, m3);
+        const int h = share_len(ls, nvalid);
+        p2p_leaf<T, true>(lds + ls.sub * h, h, ax, ay, eps2, pu, pv, m0, m1, m2, m3);
         __builtin_amdgcn_wave_barrier();
         npairs += nvalid;
         su += (double)pu;
@@ -1430,8 +1490,7 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
             nq += __popcll(fm);
         } else {
             // M2P: S = (1/D) sum_k b_k (R/D)^k, D = z - c_node; u += Im S, v += Re S
-            for (uint64_t m = fm; m; m &= m - 1) {
-                const int fnode = __builtin_amdgcn_readlane(node, __builtin_ctzll(m));
+            auto m2p = [&](int fnode) {
                 const NodeRec nd = t.rec[fnode];
                 const T Dx = ax - nd.cx, Dy = ay - nd.cy;
                 const T den = Dx * Dx + Dy * Dy;
@@ -1450,6 +1509,15 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
                 fu += sr * qy + si * qx;
                 fv += sr * qx - si * qy;
                 farmin = fmin(farmin, PM::sqrt_bound(den) - (T)nd.r);
+            };
+            if (ls.per == 1) {
+                for (uint64_t m = fm; m; m &= m - 1) m2p(__builtin_amdgcn_readlane(node, __builtin_ctzll(m)));
+            } else {
+                // a target's lanes take the far nodes in turn
+                if (isfar) fq[__popcll(fm & ((1ull << lane) - 1))] = node;
+                __builtin_amdgcn_wave_barrier();
+                const int nf = __popcll(fm);
+                for (int fi = ls.sub; fi < nf; fi += ls.per) m2p(fq[fi]);
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -1470,6 +1538,8 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
             stage_leaf<T>(lds, lane, j0, cnt, t);
             T pu = 0, pv = 0;
             npairs += cnt;
+            const int h = share_len(ls, cnt);
+            const Src4T<T> *sh = lds + ls.sub * h;
             // a pair at r2 >= m3 leaves the four smallest unchanged: skip the
             // updates when the leaf's circle lies beyond sqrt(m3) of every
             // target (a 1e-3 margin covers the FP32 distances)
@@ -1481,11 +1551,11 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
             const bool near3 = ok && !(dc2 > reach * reach * T(1.002));
             const bool neare = ok && !(dc2 > reache * reache * T(1.002));
             if (__ballot(near3))
-                p2p_leaf<T, true>(lds, cnt, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
+                p2p_leaf<T, true>(sh, h, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
             else if (__ballot(neare))
-                p2p_leaf<T, false>(lds, cnt, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
+                p2p_leaf<T, false>(sh, h, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
             else
-                p2p_leaf<T, false, false>(lds, cnt, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
+                p2p_leaf<T, false, false>(sh, h, ox, oy, eps2, pu, pv, m0, m1, m2, m3);
             __builtin_amdgcn_wave_barrier();
             su += (double)pu + (double)fu;
             sv += (double)pv + (double)fv;
@@ -1494,6 +1564,19 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     }
     su += (double)fu;
     sv += (double)fv;
+    // the target's shares: sums added in the same order on each of its
+    // lanes, the four smallest distances merged, the far bound the least
+    for (int o = ls.tpw; o < 64; o <<= 1) {
+        su += __shfl_xor(su, o, 64);
+        sv += __shfl_xor(sv, o, 64);
+        const T q0 = __shfl_xor(m0, o, 64), q1 = __shfl_xor(m1, o, 64);
+        const T q2 = __shfl_xor(m2, o, 64), q3 = __shfl_xor(m3, o, 64);
+        near4_insert<T>(q0, m0, m1, m2, m3);
+        near4_insert<T>(q1, m0, m1, m2, m3);
+        near4_insert<T>(q2, m0, m1, m2, m3);
+        near4_insert<T>(q3, m0, m1, m2, m3);
+        if constexpr (!F32) farmin = fmin(farmin, __shfl_xor(farmin, o, 64));
+    }
     if constexpr (F32) {
         for (int o = 32; o > 0; o >>= 1) gap = fminf(gap, __shfl_xor(gap, o, 64));
         farmin_w = (double)gap;
@@ -1515,7 +1598,7 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     }
     cpairs = (unsigned long long)(npairs * nvalid);
     cfar = (unsigned long long)(nfar * nvalid);
-    if (!ok) return;
+    if (!ok || ls.sub != 0) return;
     const int32_t i = t.perm[ti];
     u[i] = su / (2 * kPi) + vinf_x;
     v[i] = sv / (2 * kPi) + vinf_y;
@@ -1795,8 +1878,9 @@ __device__ __forceinline__ int64_t far_count(const unsigned long long *cnt, int6
 // target group: a node is visited when the box around the group within the
 // largest radius of its lanes reaches it and, lane by lane, when some lane's
 // own radius reaches it.  A lane whose radius exceeds far_radius() is listed for
-// k_diff_far instead and does not widen the wave's radius.  cnt[0] +=
-// evaluated pairs, cnt[1] stack overflows, cnt[2] the list length.
+// k_diff_far instead and does not widen the wave's radius.  A group of at
+// most 32 targets gives each several lanes (LaneSplit).  cnt[0] +=
+// evaluated pairs, cnt[2] the list length.
 template <typename T, bool CENSUS>
 __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, double nu, double thr,
                                                const double *__restrict__ sd, double *__restrict__ u,
@@ -1804,9 +1888,20 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
                                                unsigned long long &cpairs, unsigned long long &cin) {
     constexpr bool F32 = sizeof(T) == 4;
     __shared__ Src4T<T> lds_all[4][kLeaf];
-    const int lane = threadIdx.x & 63;
-    Src4T<T> *lds = lds_all[threadIdx.x >> 6];
-    const int64_t grp = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + (threadIdx.x >> 6)));
+    __shared__ int32_t stk_all[4][kBsStack];
+    __shared__ double4 lbox_all[4][64];  // the popped leaves: box,
+    __shared__ double2 lcen_all[4][64];  // centre,
+    __shared__ int2 lrng_all[4][64];     // first source and count,
+    __shared__ int32_t lsel_all[4][64];  // those summed (index | guard << 8)
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    Src4T<T> *lds = lds_all[wid];
+    int32_t *stk = stk_all[wid];
+    double4 *lbox = lbox_all[wid];
+    double2 *lcen = lcen_all[wid];
+    int2 *lrng = lrng_all[wid];
+    int32_t *lsel = lsel_all[wid];
+    const uint64_t below = (1ull << lane) - 1;
+    const int64_t grp = __builtin_amdgcn_readfirstlane((int)(xcd_block() * 4 + wid));
     if (grp >= t.ngroup) return;
     const TargetGroup G = target_group(t, grp, lane);
     const int64_t ti = G.ti;
@@ -1837,37 +1932,106 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
     const T Rs2 = F32 ? (T)(R2 * (1.0 + 1e-4)) : (T)R2;
     int64_t npairs = 0, nin = 0;
     double a2x = 0.0, a2y = 0.0, a1 = 0.0;
-    LaneStack st;
-    if (R > 0.0) st.push(lane, t.root);
-    while (st.sp > 0) {
-        const int node = st.pop();
-        const double4 b = t.nbb[node];
-        const double ex = fmax(0.0, fmax(b.x - bA.y, bA.x - b.y));
-        const double ey = fmax(0.0, fmax(b.z - bA.w, bA.z - b.w));
-        if (ex * ex + ey * ey >= R2) continue;
-        // lane test: distance from this lane's target to the node box
-        T lx, ly;
-        if constexpr (F32) {
-            const float x0 = (float)(b.x - cAx), x1 = (float)(b.y - cAx);
-            const float y0 = (float)(b.z - cAy), y1 = (float)(b.w - cAy);
-            lx = fmaxf(0.f, fmaxf(x0 - ax, ax - x1));
-            ly = fmaxf(0.f, fmaxf(y0 - ay, ay - y1));
-        } else {
-            lx = fmax(0.0, fmax(b.x - ax, ax - b.y));
-            ly = fmax(0.0, fmax(b.z - ay, ay - b.w));
+    // Traversal as in bs_wave: a stack in the wave's LDS slot popped up to 64
+    // nodes at a time, one per lane, each tested against the group's box
+    // widened by R.  The leaves found are listed with their box, centre and
+    // source range; those that some lane's own radius reaches are summed one
+    // at a time, each leaf's sources loaded while the previous one is summed.
+    int sp = R > 0.0 ? 1 : 0;
+    if (lane == 0) stk[0] = t.root;
+    while (sp > 0) {
+        const int k = sp + 64 + 64 <= kBsStack ? (sp < 64 ? sp : 64) : 1;
+        __builtin_amdgcn_wave_barrier();
+        const int node = lane < k ? stk[sp - 1 - lane] : -1;
+        __builtin_amdgcn_wave_barrier();
+        sp -= k;
+        bool isleaf = false, isopen = false;
+        NodeRec nd{};
+        double4 b{};
+        if (node >= 0) {
+            b = t.nbb[node];
+            const double ex = fmax(0.0, fmax(b.x - bA.y, bA.x - b.y));
+            const double ey = fmax(0.0, fmax(b.z - bA.w, bA.z - b.w));
+            if (ex * ex + ey * ey < R2) {
+                nd = t.rec[node];
+                isleaf = nd.a < 0;
+                isopen = !isleaf;
+            }
         }
-        if (__ballot(lx * lx + ly * ly < ri2) == 0) continue;
-        if (t.child0[node] < 0) {
-            const int64_t j0 = t.first[node];
+        // children of the opened nodes, the first child on top
+        const uint64_t om = __ballot(isopen);
+        if (isopen) {
+            const int pos = sp + 2 * __popcll(om & below);
+            stk[pos] = nd.b;
+            stk[pos + 1] = nd.a;
+        }
+        sp += 2 * __popcll(om);
+        const uint64_t lm = __ballot(isleaf);
+        if (isleaf) {
+            const int q = __popcll(lm & below);
+            lbox[q] = b;
+            lcen[q] = make_double2(nd.cx, nd.cy);
+            lrng[q] = make_int2(-1 - nd.a, nd.b);
+        }
+        const int nl = __popcll(lm);
+        __builtin_amdgcn_wave_barrier();
+        // the leaves some lane's own radius reaches, in list order
+        int nr = 0;
+        for (int li = 0; li < nl; ++li) {
+            const double4 lb = lbox[li];
+            T lx, ly;
+            if constexpr (F32) {
+                const float x0 = (float)(lb.x - cAx), x1 = (float)(lb.y - cAx);
+                const float y0 = (float)(lb.z - cAy), y1 = (float)(lb.w - cAy);
+                lx = fmaxf(0.f, fmaxf(x0 - ax, ax - x1));
+                ly = fmaxf(0.f, fmaxf(y0 - ay, ay - y1));
+            } else {
+                lx = fmax(0.0, fmax(lb.x - ax, ax - lb.y));
+                ly = fmax(0.0, fmax(lb.z - ay, ay - lb.w));
+            }
+            const T l2 = lx * lx + ly * ly;
+            if (__ballot(l2 < ri2) == 0) continue;
+            // the threshold test is needed only where a pair can come within
+            // it: a leaf box nearer than twice the threshold to some target
+            const int guard = __ballot(mine && !(l2 > gap2)) != 0;
+            if (lane == 0) lsel[nr] = li | (guard << 8);
+            ++nr;
+        }
+        __builtin_amdgcn_wave_barrier();
+        Src4T<T> qn = pad_src<T>();
+        // every lane loads (lanes past the count re-read the first source,
+        // which the filter drops): with no branch around them the loads stay
+        // in flight through the pair loop
+        auto fetch = [&](int r) {
+            const int2 rg = lrng[lsel[r] & 255];
+            const int64_t j = rg.x + (lane < rg.y ? lane : 0);
+            Src4T<T> q;
+            if constexpr (F32) {
+                q.x = t.fx[j];
+                q.y = t.fy[j];
+                q.g = t.fg[j];
+            } else {
+                q.x = t.sx[j];
+                q.y = t.sy[j];
+                q.g = t.sg[j];
+            }
+            q.pad = T(0);
+            return q;
+        };
+        if (nr > 0) qn = fetch(0);
+        for (int r = 0; r < nr; ++r) {
+            const int sel = lsel[r];
+            const int li = sel & 255;
+            const int cnt_l = lrng[li].y;
             T ox, oy, bx0, bx1, by0, by1;
             if constexpr (F32) {
-                const double lcx = t.ncx[node], lcy = t.ncy[node];
-                ox = ax + (float)(cAx - lcx);
-                oy = ay + (float)(cAy - lcy);
-                bx0 = (float)(bA.x - lcx);
-                bx1 = (float)(bA.y - lcx);
-                by0 = (float)(bA.z - lcy);
-                by1 = (float)(bA.w - lcy);
+                const double2 lc = lcen[li];
+                ox = ax + (float)(cAx - lc.x);
+                oy = ay + (float)(cAy - lc.y);
+                bx0 = (float)(bA.x - lc.x);
+                bx1 = (float)(bA.y - lc.x);
+                by0 = (float)(bA.z - lc.y);
+                by1 = (float)(bA.w - lc.y);
             } else {
                 ox = ax;
                 oy = ay;
@@ -1876,47 +2040,64 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
                 by0 = bA.z;
                 by1 = bA.w;
             }
-            const int cnt_j = stage_leaf_near<T>(lds, lane, j0, t.count[node], t, bx0, bx1, by0, by1, Rs2);
+            // sources within sqrt(Rs2) of the group's box, packed to the
+            // front of the slot, padding after them
+            bool keep = false;
+            if (lane < cnt_l) {
+                const T ex = qn.x < bx0 ? bx0 - qn.x : (qn.x > bx1 ? qn.x - bx1 : T(0));
+                const T ey = qn.y < by0 ? by0 - qn.y : (qn.y > by1 ? qn.y - by1 : T(0));
+                keep = ex * ex + ey * ey < Rs2;
+            }
+            const uint64_t km = __ballot(keep);
+            const int cnt_j = (int)__popcll(km);
+            __builtin_amdgcn_wave_barrier();
+            if (keep) lds[__popcll(km & below)] = qn;
+            if (lane >= cnt_j) lds[lane] = pad_src<T>();
+            __builtin_amdgcn_wave_barrier();
+            if (r + 1 < nr) qn = fetch(r + 1);
             T s2x = 0, s2y = 0, s1 = 0;
             npairs += cnt_j;
+            // this lane's share of the staged sources (padding adds nothing)
+            const int h = share_len(G.ls, cnt_j);
+            const Src4T<T> *sh = lds + G.ls.sub * h;
             if constexpr (CENSUS) {
                 if (mine)
-                    for (int jj = 0; jj < cnt_j; ++jj) {
-                        const Src4T<T> q = lds[jj];
+                    for (int jj = 0; jj < h; ++jj) {
+                        const Src4T<T> q = sh[jj];
                         const T dx = ox - q.x, dy = oy - q.y;
                         nin += dx * dx + dy * dy < (F32 ? ri2 / (T)(1.0 + 1e-4) : ri2) ? 1 : 0;
                     }
             }
-            // the threshold test is needed only where a pair can come within
-            // it: a leaf box nearer than twice the threshold to some target
-            if (__ballot(mine && !(lx * lx + ly * ly > gap2))) {
-                for (int jj = 0; jj < cnt_j; ++jj) {
-                    const Src4T<T> q = lds[jj];
+            if (sel >> 8) {
+                for (int jj = 0; jj < h; ++jj) {
+                    const Src4T<T> q = sh[jj];
                     DiffPair<T>::template add<true>(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
                 }
             } else {
-                for (int jj = 0; jj < cnt_j; ++jj) {
-                    const Src4T<T> q = lds[jj];
+                for (int jj = 0; jj < h; ++jj) {
+                    const Src4T<T> q = sh[jj];
                     DiffPair<T>::template add<false>(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
                 }
             }
-            __builtin_amdgcn_wave_barrier();
             a2x += (double)s2x;
             a2y += (double)s2y;
             a1 += (double)s1;
-        } else {
-            st.push(lane, t.child1[node]);
-            st.push(lane, t.child0[node]);
         }
+        __builtin_amdgcn_wave_barrier();
     }
-    const int nmine = __popcll(__ballot(mine));
+    const int nmine = __popcll(__ballot(mine && G.ls.sub == 0));
     cpairs = (unsigned long long)(npairs * nmine);
     if constexpr (CENSUS) {
         for (int o = 32; o > 0; o >>= 1) nin += __shfl_xor(nin, o, 64);
         cin = (unsigned long long)nin;
     }
-    if (lane == 0 && st.overflow) atomicAdd(cnt + 1, 1ull);
-    if (!mine) return;
+    // the target's shares, added in the same order on each of its lanes
+    for (int o = G.ls.tpw; o < 64; o <<= 1) {
+        a2x += __shfl_xor(a2x, o, 64);
+        a2y += __shfl_xor(a2y, o, 64);
+        a1 += __shfl_xor(a1, o, 64);
+    }
+    if (!mine || G.ls.sub != 0) return;
     if (fabs(a1) > 1e-5) {
         u[i] += nu * a2x / (a1 * di);
         v[i] += nu * a2y / (a1 * di);
@@ -2198,7 +2379,7 @@ void tree_release(vx_case *c) {
     t.fx.release(); t.fy.release(); t.fg.release(); t.nbb.release(); t.part.release(); t.rec.release();
     t.cbox.release(); t.fpairs.release(); t.ftgt.release();
     t.gcs.release(); t.gcs2.release(); t.gflag.release(); t.gscan.release(); t.gfirst.release(); t.pcount.release(); t.arr.release(); t.sizes.release();
-    t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.fbr2.release(); t.hpart.release();
+    t.cd.release(); t.cf.release(); t.cnt.release(); t.tmp.release(); t.tmp2.release(); t.fbr2.release(); t.hpart.release();
     t.sd.release();
     if (t.h_far) (void)hipHostFree(t.h_far);
     if (t.ev) (void)hipEventDestroy(t.ev);
@@ -2269,16 +2450,35 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
     // particles on average whatever the box's aspect ratio.
     const double lv = floor(0.5 * log2((double)m / kLeaf)) - (double)(c->tune.leaf_off >= 0 ? c->tune.leaf_off : 1);
     const int dmin = 2 * (int)fmin(fmax(lv, 0.0), 15.0);
+    // Three independent chains follow the sort: the target groups (aux[0]),
+    // the gather of the sorted particles (aux[1]) and the bucket and upper
+    // node marks (st).  st waits for the groups before the counts and for
+    // the gather before the buckets' multipoles.
+    if (t.tmp.ensure(sizeof(int32_t) * (size_t)(prim::tiles_of(m) + 1), st) ||
+        t.tmp2.ensure(sizeof(int32_t) * (size_t)(prim::tiles_of(m) + 1), st) || aux_streams(c))
+        return 1;
+    VX_CHECK(hipEventRecord(c->aux_fork, st));
     {
-        hipLaunchKernelGGL(k_group_cells, dim3(gm), dim3(256), 0, st, t.keys_alt.p, m, 32 - dmin, t.gcs.p);
-        if (t.tmp.ensure(sizeof(int32_t) * (size_t)(prim::tiles_of(m) + 1), st)) return 1;
-        int32_t *tt = (int32_t *)t.tmp.p;
+        hipStream_t sg = c->aux[0];
+        VX_CHECK(hipStreamWaitEvent(sg, c->aux_fork, 0));
+        hipLaunchKernelGGL(k_group_cells, dim3(gm), dim3(256), 0, sg, t.keys_alt.p, m, 32 - dmin, t.gcs.p);
+        int32_t *tt = (int32_t *)t.tmp2.p;
         VX_CHECK(prim::scan_values<true>(prim::ArrayValue{t.gcs.p}, m, prim::OpMax(), prim::WriteScan{t.gcs2.p}, tt,
-                                         (int32_t *)nullptr, st));
-        hipLaunchKernelGGL(k_group_flags, dim3(gm), dim3(256), 0, st, t.gcs2.p, m, t.gflag.p);
+                                         (int32_t *)nullptr, sg));
+        hipLaunchKernelGGL(k_group_flags, dim3(gm), dim3(256), 0, sg, t.gcs2.p, m, t.gflag.p);
         VX_CHECK(prim::select_flags(prim::ArrayFlag{t.gflag.p}, m, prim::WriteScan{t.gscan.p}, tt, (int32_t *)nullptr,
-                                    st));
-        hipLaunchKernelGGL(k_group_first, dim3(gm), dim3(256), 0, st, t.gflag.p, t.gscan.p, m, t.gfirst.p);
+                                    sg));
+        hipLaunchKernelGGL(k_group_first, dim3(gm), dim3(256), 0, sg, t.gflag.p, t.gscan.p, m, t.gfirst.p);
+        VX_CHECK(hipEventRecord(c->aux_done[0], sg));
+    }
+    {
+        // the sorted particles, newborns included
+        hipStream_t sa = c->aux[1];
+        VX_CHECK(hipStreamWaitEvent(sa, c->aux_fork, 0));
+        if (B.P > 0) VX_CHECK(hipStreamWaitEvent(sa, c->born_ev, 0));
+        hipLaunchKernelGGL(k_tree_gather, dim3(gm), dim3(256), 0, sa, c->x.p, c->y.p, c->g.p, t.perm.p, m, t.sx.p,
+                           t.sy.p, t.sg.p);
+        VX_CHECK(hipEventRecord(c->aux_done[1], sa));
     }
     if (m > kLeaf) {
         VX_CHECK(prim::fill(t.bstart.p, m, 0, st));
@@ -2294,14 +2494,11 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
     }
     // the group, bucket and upper-node counts in one read-back
     if (t.sizes.ensure(4, st)) return 1;
+    VX_CHECK(hipStreamWaitEvent(st, c->aux_done[0], 0));
     hipLaunchKernelGGL(k_tree_counts, dim3(1), dim3(1), 0, st, t.gscan.p, t.gflag.p, t.bscan.p, t.bstart.p, t.uscan.p,
                        t.upper.p, m, (int)(m > kLeaf), t.sizes.p);
     VX_CHECK(hipMemcpyAsync(c->h_count, t.sizes.p, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     VX_CHECK(hipEventRecord(t.ev, st));
-    // the sorted particles are gathered while the counts come back
-    if (B.P > 0) VX_CHECK(hipStreamWaitEvent(st, c->born_ev, 0));
-    hipLaunchKernelGGL(k_tree_gather, dim3(gm), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, t.perm.p, m, t.sx.p, t.sy.p,
-                       t.sg.p);
     VX_CHECK(hipEventSynchronize(t.ev));
     t.ngroup = c->h_count[0];
     nb = c->h_count[1];
@@ -2336,6 +2533,7 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
     hipLaunchKernelGGL(k_cut_list, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st,
                        CutNode{t.first.p, t.count.p, t.bscan.p, t.parent.p, nb, m}, nn, cut, ncut);
     const unsigned gc = (unsigned)std::min<int64_t>(nb, 2048);
+    VX_CHECK(hipStreamWaitEvent(st, c->aux_done[1], 0));  // the gathered particles
     if (order == 32) {
         hipLaunchKernelGGL(k_bucket_build<32>, dim3(lb), dim3(256), 0, st, t.sx.p, t.sy.p, t.sg.p, nb, t.first.p,
                            t.count.p, t.fx.p, t.fy.p, t.fg.p, t.ncx.p, t.ncy.p, t.nr.p, t.nbb.p, t.child0.p,
