@@ -641,7 +641,7 @@ int vx_step(vx_case *c, double out4[4]) {
     for (int kind = 1; kind <= 2; ++kind)
         for (int64_t cc = 0; cc < c->prm.n_col; ++cc)
             if (launch_setka(c, kind, n_ub, true, cc > 0) || launch_collapse(c, kind, n_ub, true) ||
-                launch_compact(c, n_ub))
+                launch_compact(c, n_ub, col_merged(c)))
                 return 1;
     bool err = false;
     if (step_readback(c, out4, &err)) return 1;
@@ -806,7 +806,7 @@ static int device_step(vx_case *c, double vinf_x, double vinf_y) {
     for (int kind = 1; kind <= 2; ++kind)
         for (int64_t cc = 0; cc < c->prm.n_col; ++cc)
             if (launch_setka(c, kind, n_ub, true, cc > 0) || launch_collapse(c, kind, n_ub, true) ||
-                launch_compact(c, n_ub))
+                launch_compact(c, n_ub, col_merged(c)))
                 return 1;
     return launch_step_record(c);
 }

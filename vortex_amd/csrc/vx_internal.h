@@ -327,7 +327,10 @@ int64_t *step_dn(vx_case *c);                          // the device count
 int launch_set_dn(vx_case *c, int64_t n);              // *dn = n (stream-ordered)
 int launch_step_move(vx_case *c);                      // sums and live box -> c->red[0..7]
 int launch_points_move(vx_case *c, int64_t m);         // c->px, py, pg moved by c->pu, pv
-int launch_compact(vx_case *c, int64_t n_ub);          // [0, *dn) compacted, new count -> *dn
+// [0, *dn) compacted, new count -> *dn; with merged given, the identity when
+// *merged == 0 (the count of candidates the preceding collapse pass listed)
+int launch_compact(vx_case *c, int64_t n_ub, const int64_t *merged = nullptr);
+inline int64_t *col_merged(vx_case *c) { return c->dcount.p + 11; }  // written by launch_collapse
 int launch_live_box(vx_case *c);                       // box of g != 0 in [0, *dn) -> c->red[4..7]
 // launch_setka -> c->col (box from c->red[4..7]); with prepare it also fills
 // the collapse counts and candidate list, and launch_collapse is then called

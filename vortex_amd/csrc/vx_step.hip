@@ -313,20 +313,30 @@ __device__ __forceinline__ int32_t live_at(const double *__restrict__ g, int64_t
     return (i < n && !(fabs(g[i]) < kDelt)) ? 1 : 0;
 }
 
-// The scan's input: live flags made on the fly.
+// The scan's input: live flags made on the fly.  With merged given and zero
+// (a collapse pass that listed no candidate: it merged and killed nothing,
+// so the compaction is the identity) every flag reads 0 and no vortex is read.
 struct LiveFlag {
     const double *g;
     const int64_t *dn;
-    __device__ int32_t operator()(int64_t i) const { return live_at(g, i, *dn); }
+    const int64_t *merged;
+    __device__ int32_t operator()(int64_t i) const { return (merged && *merged == 0) ? 0 : live_at(g, i, *dn); }
 };
+
+__device__ __forceinline__ bool compact_skip(const int64_t *merged) { return merged && *merged == 0; }
 
 // s = exclusive scan of the flags; n1 = total live (s[n-1] + f[n-1]), written
 // to the other count slot *dn1 by thread 0.
 __global__ void k_compact_lists(const double *__restrict__ g, const int32_t *__restrict__ s,
                                 const int64_t *__restrict__ dn, int64_t *__restrict__ dn1,
-                                int32_t *__restrict__ holes, int32_t *__restrict__ donors) {
+                                int32_t *__restrict__ holes, int32_t *__restrict__ donors,
+                                const int64_t *__restrict__ merged) {
     const int64_t n = *dn;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (compact_skip(merged)) {
+        if (i == 0) *dn1 = n;
+        return;
+    }
     const int64_t n1 = n > 0 ? (int64_t)s[n - 1] + live_at(g, n - 1, n) : 0;
     if (i == 0) *dn1 = n1;
     if (i >= n) return;
@@ -342,9 +352,10 @@ __global__ void k_compact_gather(const double *__restrict__ x, const double *__r
                                  const double *__restrict__ g, const int32_t *__restrict__ s,
                                  const int64_t *__restrict__ dn, const int64_t *__restrict__ dn1,
                                  const int32_t *__restrict__ holes, const int32_t *__restrict__ donors,
-                                 double *__restrict__ tx, double *__restrict__ ty, double *__restrict__ tg) {
+                                 double *__restrict__ tx, double *__restrict__ ty, double *__restrict__ tg,
+                                 const int64_t *__restrict__ merged) {
     const int64_t n = *dn;
-    if (n <= 0) return;
+    if (n <= 0 || compact_skip(merged)) return;
     const int64_t n1 = *dn1;
     const int64_t K = n1 > 0 ? (int64_t)(n1 - (n1 < n ? s[n1] : n1)) : 0;  // holes in [0, n1)
     const int64_t T = n - n1;
@@ -369,9 +380,9 @@ __global__ void k_compact_scatter(double *__restrict__ x, double *__restrict__ y
                                   const int32_t *__restrict__ s, const int64_t *__restrict__ dn,
                                   const int64_t *__restrict__ dn1, const int32_t *__restrict__ holes,
                                   const double *__restrict__ tx, const double *__restrict__ ty,
-                                  const double *__restrict__ tg) {
+                                  const double *__restrict__ tg, const int64_t *__restrict__ merged) {
     const int64_t n = *dn;
-    if (n <= 0) return;
+    if (n <= 0 || compact_skip(merged)) return;
     const int64_t n1 = *dn1;
     const int64_t K = n1 > 0 ? (int64_t)(n1 - (n1 < n ? s[n1] : n1)) : 0;
     const int64_t T = n - n1;
@@ -388,7 +399,7 @@ __global__ void k_compact_scatter(double *__restrict__ x, double *__restrict__ y
     g[dst] = tg[t];
 }
 
-int launch_compact(vx_case *c, int64_t n_ub) {
+int launch_compact(vx_case *c, int64_t n_ub, const int64_t *merged) {
     if (n_ub <= 0) return 0;
     hipStream_t st = c->stream;
     if (c->scan.ensure((size_t)n_ub, st) || c->lists.ensure((size_t)(2 * n_ub), st) ||
@@ -399,15 +410,16 @@ int launch_compact(vx_case *c, int64_t n_ub) {
     const unsigned nb = (unsigned)((n_ub + 255) / 256);
     // exclusive scan of the live flags (ballot and popcount per wave)
     if (c->ptmp.ensure((size_t)prim::tiles_of(n_ub) + 1, st)) return 1;
-    VX_CHECK(prim::select_flags(LiveFlag{c->g.p, dn}, n_ub, prim::WriteScan{c->scan.p}, c->ptmp.p,
+    VX_CHECK(prim::select_flags(LiveFlag{c->g.p, dn, merged}, n_ub, prim::WriteScan{c->scan.p}, c->ptmp.p,
                                 (int32_t *)nullptr, st));
     int32_t *holes = c->lists.p, *donors = c->lists.p + n_ub;
-    hipLaunchKernelGGL(k_compact_lists, dim3(nb), dim3(256), 0, st, c->g.p, c->scan.p, dn, dn1, holes, donors);
+    hipLaunchKernelGGL(k_compact_lists, dim3(nb), dim3(256), 0, st, c->g.p, c->scan.p, dn, dn1, holes, donors,
+                       merged);
     const unsigned nb2 = (unsigned)((2 * n_ub + 255) / 256);
     hipLaunchKernelGGL(k_compact_gather, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->scan.p, dn, dn1,
-                       holes, donors, c->tx.p, c->ty.p, c->tg.p);
+                       holes, donors, c->tx.p, c->ty.p, c->tg.p, merged);
     hipLaunchKernelGGL(k_compact_scatter, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->scan.p, dn, dn1,
-                       holes, c->tx.p, c->ty.p, c->tg.p);
+                       holes, c->tx.p, c->ty.p, c->tg.p, merged);
     VX_CHECK(hipGetLastError());
     c->dn_slot = c->dn_slot == 1 ? 8 : 1;
     return 0;
@@ -754,8 +766,15 @@ struct ColCandidate {
 // itself is only read, so every thread sees the counts from before any
 // publication.  No candidate points at a leaf, so its state slot is written
 // by this thread only.  The other candidates are listed in rest.
-__global__ __launch_bounds__(256) void k_col_leaves(const int32_t *__restrict__ col, ColBufs cb) {
+__global__ __launch_bounds__(256) void k_col_leaves(const int32_t *__restrict__ col, ColBufs cb,
+                                                    int64_t *__restrict__ merged) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        // candidates listed by the search: none means no merge in this pass
+        int64_t k = 0;
+        for (int q = 0; q < cb.nseg; ++q) k += cb.ncs[q];
+        *merged = k;
+    }
     const int32_t i = col_cand(cb, t);
     const bool leaf = i >= 0 && cb.pc[i] == 0;
     if (leaf) {
@@ -885,7 +904,7 @@ int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared) {
         if (c->ptmp.ensure((size_t)prim::tiles_of(n_ub) + 1, st)) return 1;
         VX_CHECK(prim::select_flags(prim::pred_flag(pred), n_ub, prim::WriteIndex{cb.cand}, c->ptmp.p, cb.ncs, st));
     }
-    hipLaunchKernelGGL(k_col_leaves, dim3(nbc), dim3(256), 0, st, c->col.p, cb);
+    hipLaunchKernelGGL(k_col_leaves, dim3(nbc), dim3(256), 0, st, c->col.p, cb, col_merged(c));
     hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, cb.rest, cb.nrest, cb.pcd, cb.state,
                        cb.first, (int32_t *)(c->dcount.p + 3));
     if (kind == 1)
