@@ -1836,6 +1836,29 @@ struct DiffPair<float> {
         s1 += ge;
     }
 };
+// FP32 with the circulation folded into the exponent: lg = log2|g| of a
+// source of known sign, h = 2^(r kd + lg) = |g| e^{-r/d}; NEG subtracts.
+// One multiply less per pair than g * e; the weight carries the rounding of
+// log2|g| (about 1e-6 relative).
+template <bool GUARD, bool NEG>
+__device__ __forceinline__ void diff_pair_log(float dx, float dy, float lg, float kd, float thr2, float &s2x,
+                                              float &s2y, float &s1) {
+    const float r2 = dx * dx + dy * dy;
+    const float r2c = (!GUARD || r2 > thr2) ? r2 : 1e30f;
+    const float rs = __builtin_amdgcn_rsqf(r2c);
+    const float h = __builtin_amdgcn_exp2f(__builtin_fmaf(r2c * rs, kd, lg));
+    const float w = h * rs;
+    if constexpr (NEG) {
+        s1 -= h;
+        s2x -= w * dx;
+        s2y -= w * dy;
+    } else {
+        s1 += h;
+        s2x += w * dx;
+        s2y += w * dy;
+    }
+}
+
 template <>
 struct DiffPair<double> {
     template <bool GUARD>
@@ -2047,6 +2070,61 @@ __device__ __forceinline__ void diffusion_wave(const TreeDev &t, double cut, dou
                 const T ex = qn.x < bx0 ? bx0 - qn.x : (qn.x > bx1 ? qn.x - bx1 : T(0));
                 const T ey = qn.y < by0 ? by0 - qn.y : (qn.y > by1 ? qn.y - by1 : T(0));
                 keep = ex * ex + ey * ey < Rs2;
+            }
+            if constexpr (F32) {
+                if (G.ls.per == 1) {
+                    // one lane per target: positive sources first, then
+                    // negative ones, each with log2|g| (no circulation: no
+                    // term), summed by the folded pair term
+                    const bool kp = keep && qn.g > 0.f, kn = keep && qn.g < 0.f;
+                    const uint64_t mp = __ballot(kp), mn = __ballot(kn);
+                    const int np_ = (int)__popcll(mp), nn_ = (int)__popcll(mn);
+                    Src4T<T> w = qn;
+                    w.g = __builtin_amdgcn_logf(fabsf(qn.g));
+                    __builtin_amdgcn_wave_barrier();
+                    if (kp) lds[__popcll(mp & below)] = w;
+                    if (kn) lds[np_ + __popcll(mn & below)] = w;
+                    __builtin_amdgcn_wave_barrier();
+                    if (r + 1 < nr) qn = fetch(r + 1);
+                    const int ne = np_ + nn_;
+                    npairs += ne;
+                    if constexpr (CENSUS) {
+                        if (mine)
+                            for (int jj = 0; jj < ne; ++jj) {
+                                const Src4T<T> q = lds[jj];
+                                const T dx = ox - q.x, dy = oy - q.y;
+                                nin += dx * dx + dy * dy < ri2 / (T)(1.0 + 1e-4) ? 1 : 0;
+                            }
+                    }
+                    float s2x = 0.f, s2y = 0.f, s1 = 0.f;
+                    if (sel >> 8) {
+#pragma unroll 4
+                        for (int jj = 0; jj < np_; ++jj) {
+                            const Src4T<T> q = lds[jj];
+                            diff_pair_log<true, false>(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
+                        }
+#pragma unroll 4
+                        for (int jj = np_; jj < ne; ++jj) {
+                            const Src4T<T> q = lds[jj];
+                            diff_pair_log<true, true>(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
+                        }
+                    } else {
+#pragma unroll 4
+                        for (int jj = 0; jj < np_; ++jj) {
+                            const Src4T<T> q = lds[jj];
+                            diff_pair_log<false, false>(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
+                        }
+#pragma unroll 4
+                        for (int jj = np_; jj < ne; ++jj) {
+                            const Src4T<T> q = lds[jj];
+                            diff_pair_log<false, true>(ox - q.x, oy - q.y, q.g, kd, thr2, s2x, s2y, s1);
+                        }
+                    }
+                    a2x += (double)s2x;
+                    a2y += (double)s2y;
+                    a1 += (double)s1;
+                    continue;
+                }
             }
             const uint64_t km = __ballot(keep);
             const int cnt_j = (int)__popcll(km);
