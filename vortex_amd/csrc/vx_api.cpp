@@ -298,7 +298,10 @@ int vx_case_set_velocity(vx_case *c, const double *u, const double *v, const dou
     VX_ARG(c && count >= 0 && count <= c->cap, "count beyond capacity");
     if (u) VX_CHECK(hipMemcpyAsync(c->u.p, u, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
     if (v) VX_CHECK(hipMemcpyAsync(c->v.p, v, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
-    if (d) VX_CHECK(hipMemcpyAsync(c->d.p, d, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
+    if (d) {
+        VX_CHECK(hipMemcpyAsync(c->d.p, d, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
+        tree_d_changed(c);
+    }
     VX_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }

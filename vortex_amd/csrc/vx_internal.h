@@ -275,6 +275,7 @@ int launch_probe_to(vx_case *c, const double *px, const double *py, int64_t m, C
                     double vinf_y, double *pu, double *pv, double *part, int64_t max_part);
 int launch_step_probes(vx_case *c, double vinf_x, double vinf_y);  // c->prb_x.. -> c->prb_u, prb_v
 // The probes over the tree of the last launch_tree_biot_savart, on stream st
+void tree_d_changed(vx_case *c);  // d was set by the caller: the tree's sorted copy is stale
 int launch_tree_probes(vx_case *c, const double *px, const double *py, int64_t m, double vinf_x,
                        double vinf_y, double *pu, double *pv, hipStream_t st);
 // The probes on caller-owned device buffers with the partials in the

@@ -743,6 +743,7 @@ static void bs_dispatch(vx_case *c, int tpt, const T *sx, const T *sy, const T *
 
 int launch_biot_savart(vx_case *c, Cnt n_tgt, Cnt n_src, double vinf_x, double vinf_y) {
     if (n_tgt.ub <= 0) return 0;
+    tree_d_changed(c);  // d from direct sums: a tree's sorted copy no longer matches
     const double eps2 = c->prm.ve_size * c->prm.ve_size;
     if (mixed(c)) {
         if (launch_pack_f32(c, n_src)) return 1;

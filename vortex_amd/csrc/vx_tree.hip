@@ -73,6 +73,7 @@ struct TreeBufs {
     DBuf<NodeRec> rec;
     DBuf<unsigned long long> cnt;
     DBuf<unsigned char> tmp, tmp2;  // scan scratch of the build's two concurrent chains
+    bool sd_fresh = false;          // sd holds d in sorted order, as the last Biot-Savart pass wrote it
     int64_t s = 0;                         // slots keyed and sorted by the last build
     int64_t m = 0, nleaf = 0, nnode = 0;  // nleaf: buckets, nodes [0, nleaf)
     int64_t ngroup = 0;                    // target groups (gfirst)
@@ -1364,6 +1365,7 @@ constexpr int kFarQ = 512;     // FP32: far nodes queued per wave before any M2L
 template <typename T, int ORDER, int STRIDE>
 __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double eps, double vinf_x, double vinf_y,
                                         double *__restrict__ u, double *__restrict__ v, double *__restrict__ d,
+                                        double *__restrict__ sdo,
                                         int32_t *__restrict__ fb, double *__restrict__ fbr2,
                                         unsigned long long *__restrict__ nfb, int exact_all,
                                         unsigned long long &cpairs, unsigned long long &cfar,
@@ -1602,7 +1604,9 @@ __device__ __forceinline__ void bs_wave(const TreeDev &t, double theta, double e
     const int32_t i = t.perm[ti];
     u[i] = su / (2 * kPi) + vinf_x;
     v[i] = sv / (2 * kPi) + vinf_y;
-    d[i] = (sqrt((double)m1) + sqrt((double)m2) + sqrt((double)m3)) / 3;
+    const double di = (sqrt((double)m1) + sqrt((double)m2) + sqrt((double)m3)) / 3;
+    d[i] = di;
+    sdo[ti] = di;  // in sorted order too, for the diffusion's far-target selection
     // exact only if no far node can hold a closer particle than the third
     // FP32 squared distances of close pairs carry up to ~1e-5 relative error,
     // hence the 1e-3 margins here and in the search bound
@@ -1619,11 +1623,11 @@ template <typename T, int ORDER, int STRIDE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && ORDER == 16 ? 6 : 1)))
 void k_tree_bs(TreeDev t, double theta, double eps, double vinf_x, double vinf_y,
                                                  double *__restrict__ u, double *__restrict__ v,
-                                                 double *__restrict__ d, int32_t *__restrict__ fb,
-                                                 double *__restrict__ fbr2, unsigned long long *__restrict__ nfb,
-                                                 int exact_all) {
+                                                 double *__restrict__ d, double *__restrict__ sdo,
+                                                 int32_t *__restrict__ fb, double *__restrict__ fbr2,
+                                                 unsigned long long *__restrict__ nfb, int exact_all) {
     unsigned long long cp = 0, cf = 0, cl = 0;
-    bs_wave<T, ORDER, STRIDE>(t, theta, eps, vinf_x, vinf_y, u, v, d, fb, fbr2, nfb, exact_all, cp, cf, cl);
+    bs_wave<T, ORDER, STRIDE>(t, theta, eps, vinf_x, vinf_y, u, v, d, sdo, fb, fbr2, nfb, exact_all, cp, cf, cl);
     block_count_add(cp, cf, nfb + 1, nfb + 2);
     block_count_add(cl, 0ull, nfb + 7, nullptr);
 }
@@ -1721,7 +1725,7 @@ __global__ __launch_bounds__(1024) void k_tree_nearest_fallback(TreeDev t, const
                                                                 const int32_t *__restrict__ fb,
                                                                 const double *__restrict__ fbr2,
                                                                 const unsigned long long *__restrict__ nfb,
-                                                                double *__restrict__ d) {
+                                                                double *__restrict__ d, double *__restrict__ sdo) {
     __shared__ int32_t lst[kFbList];
     __shared__ double lsd[kFbList];
     __shared__ int nl;
@@ -1809,7 +1813,11 @@ __global__ __launch_bounds__(1024) void k_tree_nearest_fallback(TreeDev t, const
             }
         }
         near4_block(lane, w, m0, m1, m2, m3, wr, r);
-        if (threadIdx.x == 0) d[t.perm[ti]] = (sqrt(r[1]) + sqrt(r[2]) + sqrt(r[3])) / 3;
+        if (threadIdx.x == 0) {
+            const double di = (sqrt(r[1]) + sqrt(r[2]) + sqrt(r[3])) / 3;
+            d[t.perm[ti]] = di;
+            sdo[ti] = di;
+        }
     }
 }
 
@@ -2202,21 +2210,27 @@ void k_tree_diffusion(TreeDev t, double cut, double nu, double thr,
 struct FarFlag {
     TreeDev t;
     double cut;
-    const double *dd;
-    __device__ int operator()(int64_t ti) const { return fmin(cut, kFarCut) * dd[t.perm[ti]] > far_radius(t); }
+    const double *sd;  // d in sorted order (all positive before the selection)
+    __device__ int operator()(int64_t ti) const { return fmin(cut, kFarCut) * sd[ti] > far_radius(t); }
 };
 struct FarWrite {
     TreeDev t;
-    const double *dd;
     double *sd;
     int32_t *lists;
     __device__ void operator()(int64_t ti, int32_t pos, int fl) const {
-        const double di = dd[t.perm[ti]];
-        const bool listed = fl && pos < t.fcap;
-        if (listed) lists[pos] = (int32_t)ti;
-        sd[ti] = listed ? -di : di;
+        if (fl && pos < t.fcap) {
+            lists[pos] = (int32_t)ti;
+            sd[ti] = -sd[ti];
+        }
     }
 };
+
+// sd[ti] = d[perm[ti]] (when the Biot-Savart pass's sorted copy is stale).
+__global__ void k_sd_gather(const int32_t *__restrict__ perm, const double *__restrict__ d, int64_t m,
+                            double *__restrict__ sd) {
+    const int64_t ti = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti < m) sd[ti] = d[perm[ti]];
+}
 
 // Targets whose radius exceeds far_radius(), listed by k_tree_diffusion.  The
 // buckets are cut into chunks of 64 consecutive ones (Morton order) with a
@@ -2441,6 +2455,10 @@ __global__ void k_tree_outside(const int32_t *__restrict__ perm, int64_t m, int6
 }
 
 // ---- host side ----------------------------------------------------------------
+void tree_d_changed(vx_case *c) {
+    if (c->tree) c->tree->sd_fresh = false;
+}
+
 static TreeBufs &tree_of(vx_case *c) {
     if (!c->tree) c->tree = new TreeBufs();
     return *c->tree;
@@ -2858,6 +2876,7 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
         hipLaunchKernelGGL(k_tree_outside, dim3((unsigned)((s - t.m + 255) / 256)), dim3(256), 0, st, t.perm.p, t.m, s,
                            vinf_x, vinf_y, c->u.p, c->v.p, c->d.p);
     if (t.m == 0) return 0;
+    if (t.sd.ensure((size_t)t.m, st)) return 1;
     const TreeDev tv = view(t);
     const unsigned lb = (unsigned)((t.nleaf + 3) / 4 + 7) / 8 * 8;  // padded for xcd_block()
     const double th = theta_of(c), eps = c->prm.ve_size;
@@ -2867,7 +2886,7 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
     Timer tk(c, &c->tm.bs_kernel_ms, 6);
 #define VX_TREE_BS(T_, P_, S_)                                                                                       \
     hipLaunchKernelGGL((k_tree_bs<T_, P_, S_>), dim3(lb), dim3(256), 0, st, tv, th, eps, vinf_x, vinf_y, c->u.p,     \
-                       c->v.p, c->d.p, t.fb.p, t.fbr2.p, nfb, exact_all)
+                       c->v.p, c->d.p, t.sd.p, t.fb.p, t.fbr2.p, nfb, exact_all)
     if (mixed) {
         if (t.terms == 32)
             VX_TREE_BS(float, 32, 32);
@@ -2886,8 +2905,9 @@ int launch_tree_biot_savart(vx_case *c, int64_t s, double vinf_x, double vinf_y)
 #undef VX_TREE_BS
     tk.stop();
     hipLaunchKernelGGL(k_tree_nearest_fallback, dim3(512), dim3(1024), 0, st, tv, t.arr.p, t.cnt.p + 10, t.fb.p,
-                       t.fbr2.p, nfb, c->d.p);
+                       t.fbr2.p, nfb, c->d.p, t.sd.p);
     VX_CHECK(hipGetLastError());
+    t.sd_fresh = true;
     return 0;
 }
 
@@ -2930,8 +2950,14 @@ int tree_diffusion_pre(vx_case *c) {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 0x2du);  // not the overflow flag nor M2L count
     if (c->ptmp.ensure((size_t)prim::tiles_of(t.m) + 1, st)) return 1;
     const TreeDev tv = view(t);
-    VX_CHECK(prim::select_flags(FarFlag{tv, diff_cut_of(c), c->d.p}, t.m, FarWrite{tv, c->d.p, t.sd.p, t.fb.p},
-                                c->ptmp.p, cnt + 2, st));
+    // d in sorted order: the Biot-Savart pass's copy unless d changed since
+    // or an earlier selection negated entries of it
+    if (!t.sd_fresh)
+        hipLaunchKernelGGL(k_sd_gather, dim3((unsigned)((t.m + 255) / 256)), dim3(256), 0, st, t.perm.p, c->d.p, t.m,
+                           t.sd.p);
+    t.sd_fresh = false;
+    VX_CHECK(prim::select_flags(FarFlag{tv, diff_cut_of(c), t.sd.p}, t.m, FarWrite{tv, t.sd.p, t.fb.p}, c->ptmp.p,
+                                cnt + 2, st));
     return 0;
 }
 
