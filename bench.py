@@ -180,12 +180,19 @@ def c0_case_params():
                      birth_offset_rel=0.5, tree_min_n=0)
 
 
+GRAPH_CHUNK = 25  # steps per graph call in the C0 graph leg
+
+
 def c0_step_rates(steps=500):
     """Configuration C0 run as specified (500 steps from the empty start,
     direct FP64 sums, and the same in mixed precision) on the GPU, once as
     stream steps (one host wait per step, as the reference loop) and once as
-    replays of the captured step graph (one wait at the end).  The difference
-    per step is the launch and host-wait overhead the graph removes."""
+    replays of the captured step graph, GRAPH_CHUNK steps per call (one host
+    wait per call).  A graph call sizes every launch for the most vortices
+    its steps can reach (no deaths), so the case reserves no more than that:
+    one 500-step call would size the launches for 1e5 slots while about
+    3e3 are live.  The difference per step is the launch and host-wait
+    overhead the graph removes."""
     from vortex_amd.solver import Case, make_params, VX_PREC_F64, VX_PREC_MIXED
     pan, kw = c0_case_params()
     out = {}
@@ -193,14 +200,17 @@ def c0_step_rates(steps=500):
         for mode in ("stream", "graph"):
             c = Case(make_params(pan, precision=prec, **kw), pan)
             c.build_matrix()
-            c.reserve(steps * pan.size + 1024)
+            if mode == "stream":
+                c.reserve(steps * pan.size + 1024)
             c.sync()
             t = time.perf_counter()
             if mode == "stream":
                 for _ in range(steps):
                     c.time_step((1.0, 0.0))
             else:
-                c.run_steps_graph(steps, (1.0, 0.0))
+                for _ in range(steps // GRAPH_CHUNK):
+                    c.run_steps_graph(GRAPH_CHUNK, (1.0, 0.0))
+                c.run_steps_graph(steps % GRAPH_CHUNK, (1.0, 0.0))
             c.sync()
             dt = time.perf_counter() - t
             out[f"{name}_{mode}_steps_per_s"] = steps / dt
