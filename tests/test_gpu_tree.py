@@ -156,6 +156,32 @@ def test_tree_diffusion_vs_direct_oracle(ref, prec, tol):
     assert info["heavy"] > 0  # isolated vortices go through the split direct kernel
 
 
+@pytest.mark.parametrize("prec,tol", [(VX_PREC_F64, 1e-9), (VX_PREC_MIXED, 1e-3)])
+def test_tree_diffusion_after_the_caller_sets_d(ref, prec, tol):
+    """The Biot-Savart pass leaves a copy of d in the tree's sorted order for
+    the diffusion's far-target selection; a d handed in by the caller
+    afterwards (vx_case_set_velocity) must replace that copy, so the
+    diffusion sees the caller's d everywhere, including which targets reach
+    the domain scale."""
+    n = 20000
+    x, y, g = wake_cloud(n, 4)
+    s = (n + 31) // 32 * 32
+    X, Y, G = tail(x, y, g, s)
+    eps, nu = 0.004, 1e-3
+    pan = geo.circle(200, 0.5)
+    c = Case(make_params(pan, precision=prec, ve_size=eps, nu=nu, order=32), pan)
+    c.set_state(X, Y, G)
+    c.tree_biot_savart(s, (0.0, 0.0))
+    u0, v0, d = c.velocity(s)
+    d2 = d * np.random.default_rng(5).uniform(0.5, 3.0, s)
+    c.set_velocity(u0, v0, d2)
+    c.tree_diffusion()
+    u, v, _ = c.velocity(s)
+    ur, vr = ref.diffusion(X, Y, G, d2, nu, eps, u0, v0)
+    assert rel_err(u - u0, ur - u0) < tol and rel_err(v - v0, vr - v0) < tol
+    c.close()
+
+
 @pytest.mark.parametrize("prec", [VX_PREC_F64, VX_PREC_MIXED])
 def test_tree_diffusion_census_counts_in_radius_pairs(ref, prec):
     """The census (vx_tree_census) leaves the velocities bit for bit and
