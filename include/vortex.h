@@ -267,6 +267,11 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL);
  *                     levels above the cell that holds 64 vortices at the mean
  *                     density, 0..3 (-1: default 1).  It changes which pairs
  *                     are summed directly, within the tree's error bound.
+ *   col_share_min     vortex count from which the collapse passes of a step
+ *                     keep one cell grid from pass to pass, linking the
+ *                     vortices a pass moves at their new cells (0: default
+ *                     2^17; below it every pass builds its grid).  Results
+ *                     do not depend on it.
  * vx_case_autotune times every choice on the case's current state (the
  * direct kernels one by one below tree_min_n, the whole velocity stage
  * above it), keeps the fastest and reports the default's and the chosen
@@ -276,6 +281,7 @@ typedef struct vx_tuning {
     int32_t leaf_off;
     int32_t trials;                 /* out: configurations timed */
     double ms_default, ms_best;     /* out: timed stage, default and chosen */
+    int64_t col_share_min;
 } vx_tuning;
 int vx_case_autotune(vx_case *c, double vinf_x, double vinf_y, vx_tuning *out);
 int vx_case_set_tuning(vx_case *c, const vx_tuning *t);

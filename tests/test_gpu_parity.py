@@ -286,15 +286,23 @@ def test_setka_collapse_bitexact(ref, cyl, kind, seed):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-def test_Step_bitexact_from_identical_velocities(ref, prec):
+@pytest.mark.parametrize("r_same,share", [(0.6, 0), (0.6, 1), (1.0, 1)])
+def test_Step_bitexact_from_identical_velocities(ref, prec, r_same, share):
+    """Step (move, compaction, both kinds' collapse passes) bit for bit against
+    the oracle: with a grid built for every collapse pass (the default below
+    2^17 vortices) and, forced by col_share_min, with grids kept from pass to
+    pass, one per kind (radii 1.0 and 0.6) or one for both (equal radii)."""
     pan = geo.circle(200, 0.5)
-    c, prm = make_case(pan, prec, dt=0.02, ve_size=0.004, r_col_diff_sign=1.0, r_col_same_sign=0.6,
+    c, prm = make_case(pan, prec, dt=0.02, ve_size=0.004, r_col_diff_sign=1.0, r_col_same_sign=r_same,
                        max_ve_g=0.006)
+    if share:
+        c.set_tuning(col_share_min=1)
     n = 8000
     x, y, g, u, v = crossing_state(pan, n, 7)
     c.set_state(x, y, g)
     c.set_velocity(u, v)
     out = c.step()
+    print(f"[Step] r_same={r_same} share={share}: {n} -> {c.n}", flush=True)
     X, Y, G = c.state(n)
     xr, yr, gr, nr, outr = ref.Step(x, y, g, u, v, n, pan, ref_params(prm))
     assert c.n == nr

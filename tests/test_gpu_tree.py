@@ -334,6 +334,43 @@ def test_fused_time_step_matches_the_stage_calls(prec, order):
     assert np.array_equal(f0, f1)
 
 
+@pytest.mark.parametrize("prec,r_same", [(VX_PREC_MIXED, 0.6), (VX_PREC_F64, 0.6), (VX_PREC_MIXED, 1.0)])
+def test_step_collapse_kept_grid_matches_rebuilt_grids(prec, r_same):
+    """Whole time steps (tree velocities) of a cloud with thousands of pairs
+    within the collapse radii, three passes per kind, each merging and filling
+    holes: with the collapse grid kept from pass to pass (col_share_min = 1;
+    the survivors and the vortices moved into holes take pool nodes), one per
+    kind (radii 1.0 and 0.6) or one for both (equal radii), the state ends bit
+    for bit as with a grid built for every pass."""
+    pan = geo.circle(300, 0.5)
+    eps = 0.004
+    prm = make_params(pan, dt=0.01, ve_size=eps, r_col_diff_sign=1.0, r_col_same_sign=r_same, max_ve_g=0.006,
+                      birth_offset_rel=0.5, n_col=3, precision=prec, tree_min_n=2000)
+    rng = np.random.default_rng(21)
+    n = 30000
+    x, y = rng.uniform(0.6, 3.0, n), rng.uniform(-0.5, 0.5, n)
+    g = rng.uniform(-0.004, 0.004, n)
+    res = []
+    for share in (1, 0):
+        c = Case(prm, pan)
+        c.build_matrix()
+        c.set_tuning(col_share_min=share)
+        c.set_state(x, y, g)
+        merged = 0
+        for _ in range(2):
+            before = c.n + pan.size  # the births come first
+            c.time_step((1.0, 0.0))
+            merged += before - c.n
+        res.append((c.n, c.state(), merged))
+        c.close()
+    (n0, s0, m0), (n1, s1, m1) = res
+    print(f"[collapse grid] prec={prec} r_same={r_same}: n {n0} {n1}, removed {m0} {m1}", flush=True)
+    assert m0 > 2000  # absorbed, deleted and merged; mostly merged here
+    assert n0 == n1
+    for a, b in zip(s0, s1):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("prec,order", [(VX_PREC_MIXED, 16), (VX_PREC_F64, 32)])
 def test_speed_stage_matches_the_separate_kernels(prec, order):
     """vx_speed runs the wall term and the domain-scale diffusion targets on

@@ -59,7 +59,7 @@ class TreeInfo(C.Structure):
 class Tuning(C.Structure):
     """Mirror of vx_tuning (include/vortex.h)."""
     _fields_ = [("tpt_bs", C.c_int32), ("tpt_diff", C.c_int32), ("leaf_off", C.c_int32), ("trials", C.c_int32),
-                ("ms_default", C.c_double), ("ms_best", C.c_double)]
+                ("ms_default", C.c_double), ("ms_best", C.c_double), ("col_share_min", C.c_int64)]
 
 
 class Timings(C.Structure):

@@ -183,7 +183,7 @@ int vx_case_destroy(vx_case *c) {
     for (auto *p : db) p->release();
     c->xf.release(); c->yf.release(); c->gf.release();
     c->flags.release(); c->scan.release(); c->lists.release(); c->col.release(); c->misc.release();
-    c->keys.release(); c->keys_alt.release(); c->vals.release(); c->vals_alt.release();
+    c->keys.release(); c->keys_alt.release(); c->vals.release(); c->vals_alt.release(); c->cnode.release();
     c->ptmp.release(); c->gridp.release(); c->dcount.release(); c->colseg.release();
     if (c->side) (void)hipStreamSynchronize(c->side);
     c->wlist.release(); c->wtmp.release(); c->wstage.release();
@@ -514,14 +514,14 @@ int vx_diffusion_wall(vx_case *c, int64_t n_tgt) {
 
 // Read back the step sums (c->red[0..3]), the device count (slot dn_slot)
 // and the collapse error flag; one synchronisation.
-static int step_readback(vx_case *c, double out4[4], bool *err) {
+static int step_readback(vx_case *c, double out4[4], int *err) {
     if (out4) VX_CHECK(hipMemcpyAsync(c->h_red, c->red.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     VX_CHECK(hipMemcpyAsync(c->h_count, c->dcount.p, 9 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     VX_CHECK(hipStreamSynchronize(c->stream));
     if (out4)
         for (int q = 0; q < 4; ++q) out4[q] = c->h_red[q];
     c->n = c->h_count[c->dn_slot];
-    if (err) *err = (int32_t)c->h_count[3] != 0;
+    if (err) *err = (int32_t)c->h_count[3];
     return 0;
 }
 
@@ -554,7 +554,7 @@ int vx_collapse(vx_case *c, int kind, const int32_t *COL) {
         if (c->n) VX_CHECK(hipMemcpyAsync(c->col.p, COL, sizeof(int32_t) * c->n, hipMemcpyHostToDevice, c->stream));
     }
     if (launch_set_dn(c, c->n) || launch_collapse(c, kind, c->n, false)) return 1;
-    bool err = false;
+    int err = 0;
     if (step_readback(c, nullptr, &err)) return 1;
     if (err) {
         set_error("collapse resolution did not converge");
@@ -640,16 +640,12 @@ int vx_step(vx_case *c, double out4[4]) {
     VX_ARG(c && out4, "null pointer");
     Timer t(c, &c->tm.step_ms, 4);
     const int64_t n_ub = c->n;
-    if (launch_set_dn(c, n_ub) || launch_step_move(c) || launch_compact(c, n_ub)) return 1;
-    for (int kind = 1; kind <= 2; ++kind)
-        for (int64_t cc = 0; cc < c->prm.n_col; ++cc)
-            if (launch_setka(c, kind, n_ub, true, cc > 0) || launch_collapse(c, kind, n_ub, true) ||
-                launch_compact(c, n_ub, col_merged(c)))
-                return 1;
-    bool err = false;
+    if (launch_set_dn(c, n_ub) || launch_step_move(c) || launch_compact(c, n_ub) || launch_step_collapse(c, n_ub))
+        return 1;
+    int err = 0;
     if (step_readback(c, out4, &err)) return 1;
     if (err) {
-        set_error("collapse resolution did not converge");
+        set_error(err & 2 ? "collapse grid node pool full" : "collapse resolution did not converge");
         return 1;
     }
     c->d_g = out4[0];
@@ -781,9 +777,11 @@ int vx_case_set_tuning(vx_case *c, const vx_tuning *t) {
     auto ok_tpt = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8; };
     VX_ARG(ok_tpt(t->tpt_bs) && ok_tpt(t->tpt_diff), "targets per thread must be 0, 1, 2, 4 or 8");
     VX_ARG(t->leaf_off >= -1 && t->leaf_off <= 3, "leaf_off must be -1..3");
+    VX_ARG(t->col_share_min >= 0, "col_share_min must be >= 0");
     c->tune.tpt_bs = t->tpt_bs;
     c->tune.tpt_diff = t->tpt_diff;
     c->tune.leaf_off = t->leaf_off;
+    c->tune.col_share_min = t->col_share_min;
     return 0;
 }
 
@@ -805,12 +803,9 @@ static int device_step(vx_case *c, double vinf_x, double vinf_y) {
     const Cnt s = cnt_s(c);
     if (launch_biot_savart(c, s, s, vinf_x, vinf_y) || launch_diffusion(c, s, s) || launch_diffusion_wall(c, s))
         return 1;
-    if (launch_set_dn(c, n_ub) || launch_step_move(c) || launch_compact(c, n_ub)) return 1;
-    for (int kind = 1; kind <= 2; ++kind)
-        for (int64_t cc = 0; cc < c->prm.n_col; ++cc)
-            if (launch_setka(c, kind, n_ub, true, cc > 0) || launch_collapse(c, kind, n_ub, true) ||
-                launch_compact(c, n_ub, col_merged(c)))
-                return 1;
+    if (launch_set_dn(c, n_ub) || launch_step_move(c) || launch_compact(c, n_ub) ||
+        launch_step_collapse(c, n_ub))
+        return 1;
     return launch_step_record(c);
 }
 
@@ -869,7 +864,7 @@ static int devcount_end(vx_case *c, int64_t steps, double *f4, int64_t *counts) 
         return 1;
     }
     if ((int32_t)flags[3]) {
-        set_error("collapse resolution did not converge");
+        set_error((int32_t)flags[3] & 2 ? "collapse grid node pool full" : "collapse resolution did not converge");
         return 1;
     }
     const int64_t *nr = reinterpret_cast<const int64_t *>(ring.data() + 4 * c->gr_cap);

@@ -166,6 +166,7 @@ struct vx_case {
     vx::DBuf<int32_t> flags, scan, lists, col, misc;
     vx::DBuf<uint64_t> keys, keys_alt;
     vx::DBuf<int32_t> vals, vals_alt;
+    vx::DBuf<int32_t> cnode;                           // step collapse grid: node links, then pool slots
     vx::DBuf<int32_t> ptmp;                            // tile totals of the scans and selections (main stream)
     vx::DBuf<unsigned char> gridp;                     // collapse grid parameters (device)
     vx::DBuf<int64_t> dcount;                         // device counters
@@ -195,7 +196,7 @@ struct vx_case {
     bool devcount = false;
     int64_t n_ub = 0;
     // launch shapes (vx_case_autotune / vx_case_set_tuning)
-    vx_tuning tune{0, 0, -1, 0, 0.0, 0.0};
+    vx_tuning tune{0, 0, -1, 0, 0.0, 0.0, 0};
     vx::DBuf<double> gr_ring;                         // 4 forces + 1 count per step
     int64_t gr_cap = 0;
     hipGraph_t graph = nullptr;
@@ -328,9 +329,18 @@ int64_t *step_dn(vx_case *c);                          // the device count
 int launch_set_dn(vx_case *c, int64_t n);              // *dn = n (stream-ordered)
 int launch_step_move(vx_case *c);                      // sums and live box -> c->red[0..7]
 int launch_points_move(vx_case *c, int64_t m);         // c->px, py, pg moved by c->pu, pv
+// A collapse grid kept over the passes of one step (launch_step_collapse):
+// a kernel that gives a slot new data (a merge's survivor, a compaction's
+// filled hole) pushes a pool node for the slot at its new cell.
+struct GridPush {
+    const void *grid;                      // the step's Grid on the device
+    int32_t *head, *next, *pslot;          // cell heads, node links, pool node -> slot
+    int32_t *pcount, *err;                 // pool nodes used; error flag (|= 2 when full)
+    int64_t base, cap;                     // first pool node id, pool capacity
+};
 // [0, *dn) compacted, new count -> *dn; with merged given, the identity when
 // *merged == 0 (the count of candidates the preceding collapse pass listed)
-int launch_compact(vx_case *c, int64_t n_ub, const int64_t *merged = nullptr);
+int launch_compact(vx_case *c, int64_t n_ub, const int64_t *merged = nullptr, const GridPush *gp = nullptr);
 inline int64_t *col_merged(vx_case *c) { return c->dcount.p + 11; }  // written by launch_collapse
 int launch_live_box(vx_case *c);                       // box of g != 0 in [0, *dn) -> c->red[4..7]
 // launch_setka -> c->col (box from c->red[4..7]); with prepare it also fills
@@ -341,6 +351,10 @@ int launch_live_box(vx_case *c);                       // box of g != 0 in [0, *
 int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare, bool after_pass = false);
 // default targets per thread of the direct kernels for n_tgt targets (vx_pairwise.hip)
 int direct_tpt_default(bool f32, int64_t n_tgt);
-int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared);
+int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared, const GridPush *gp = nullptr);
+// Every collapse pass of a step (both kinds, n_col each, with their
+// compactions), from 2^17 slots on over a grid kept from pass to pass.
+// Collapse error flag in dcount[3] (bit 2: the grid's node pool was full).
+int launch_step_collapse(vx_case *c, int64_t n_ub);
 
 }  // namespace vx

@@ -300,6 +300,43 @@ int launch_points_move(vx_case *c, int64_t m) {
     return 0;
 }
 
+struct Grid {
+    double x0, y0, inv_s;
+    int64_t nx, ny, cells;
+    int valid;
+};
+
+__device__ __forceinline__ void cell_of(const Grid &G, double x, double y, int64_t &cx, int64_t &cy, double &fx,
+                                        double &fy) {
+    fx = (x - G.x0) * G.inv_s;
+    fy = (y - G.y0) * G.inv_s;
+    if (!(fx >= 0.0)) fx = 0.0;  // also catches NaN
+    if (!(fy >= 0.0)) fy = 0.0;
+    cx = fx >= (double)(G.nx - 1) ? G.nx - 1 : (int64_t)fx;
+    cy = fy >= (double)(G.ny - 1) ? G.ny - 1 : (int64_t)fy;
+    fx -= (double)cx;
+    fy -= (double)cy;
+}
+
+// A pool node for slot at the cell of (px, py): taken from the pool's count,
+// linked in front of the cell's list.  The search that reads the lists runs
+// in a later kernel.
+__device__ __forceinline__ void grid_push(const GridPush &p, int32_t slot, double px, double py) {
+    const Grid G = *(const Grid *)p.grid;
+    if (!G.valid) return;
+    const int32_t q = atomicAdd(p.pcount, 1);
+    if ((int64_t)q >= p.cap) {
+        atomicOr(p.err, 2);
+        return;
+    }
+    int64_t cx, cy;
+    double fx, fy;
+    cell_of(G, px, py, cx, cy, fx, fy);
+    p.pslot[q] = slot;
+    const int32_t node = (int32_t)(p.base + q);
+    p.next[node] = atomicExch(&p.head[cy * G.nx + cx], node);
+}
+
 // ---- A11: src/kernel.cu:1159-1178 ------------------------------------------
 // Swap-with-last removal of |g| < 1e-12 in closed form.  With n1 live
 // entries, the j-th hole of [0, n1) (ascending) takes the j-th live entry of
@@ -380,7 +417,8 @@ __global__ void k_compact_scatter(double *__restrict__ x, double *__restrict__ y
                                   const int32_t *__restrict__ s, const int64_t *__restrict__ dn,
                                   const int64_t *__restrict__ dn1, const int32_t *__restrict__ holes,
                                   const double *__restrict__ tx, const double *__restrict__ ty,
-                                  const double *__restrict__ tg, const int64_t *__restrict__ merged) {
+                                  const double *__restrict__ tg, const int64_t *__restrict__ merged,
+                                  GridPush gp) {
     const int64_t n = *dn;
     if (n <= 0 || compact_skip(merged)) return;
     const int64_t n1 = *dn1;
@@ -397,9 +435,11 @@ __global__ void k_compact_scatter(double *__restrict__ x, double *__restrict__ y
     x[dst] = tx[t];
     y[dst] = ty[t];
     g[dst] = tg[t];
+    // a filled hole holds a live vortex that its cell's list does not reach
+    if (gp.grid && t < K) grid_push(gp, (int32_t)dst, tx[t], ty[t]);
 }
 
-int launch_compact(vx_case *c, int64_t n_ub, const int64_t *merged) {
+int launch_compact(vx_case *c, int64_t n_ub, const int64_t *merged, const GridPush *gp) {
     if (n_ub <= 0) return 0;
     hipStream_t st = c->stream;
     if (c->scan.ensure((size_t)n_ub, st) || c->lists.ensure((size_t)(2 * n_ub), st) ||
@@ -419,7 +459,7 @@ int launch_compact(vx_case *c, int64_t n_ub, const int64_t *merged) {
     hipLaunchKernelGGL(k_compact_gather, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->scan.p, dn, dn1,
                        holes, donors, c->tx.p, c->ty.p, c->tg.p, merged);
     hipLaunchKernelGGL(k_compact_scatter, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->scan.p, dn, dn1,
-                       holes, c->tx.p, c->ty.p, c->tg.p, merged);
+                       holes, c->tx.p, c->ty.p, c->tg.p, merged, gp ? *gp : GridPush{});
     VX_CHECK(hipGetLastError());
     c->dn_slot = c->dn_slot == 1 ? 8 : 1;
     return 0;
@@ -501,24 +541,6 @@ __device__ void block_append(bool flag, int32_t val, int32_t *__restrict__ out, 
     if (flag) out[base + off + __popcll(m & ((1ull << lane) - 1))] = val;
 }
 
-struct Grid {
-    double x0, y0, inv_s;
-    int64_t nx, ny, cells;
-    int valid;
-};
-
-__device__ __forceinline__ void cell_of(const Grid &G, double x, double y, int64_t &cx, int64_t &cy, double &fx,
-                                        double &fy) {
-    fx = (x - G.x0) * G.inv_s;
-    fy = (y - G.y0) * G.inv_s;
-    if (!(fx >= 0.0)) fx = 0.0;  // also catches NaN
-    if (!(fy >= 0.0)) fy = 0.0;
-    cx = fx >= (double)(G.nx - 1) ? G.nx - 1 : (int64_t)fx;
-    cy = fy >= (double)(G.ny - 1) ? G.ny - 1 : (int64_t)fy;
-    fx -= (double)cx;
-    fy -= (double)cy;
-}
-
 // box = (min x, max x, min y, max y) of the g != 0 vortices; at most
 // max_cells cells (a wider cell stays correct).
 __device__ Grid grid_params(const double *__restrict__ box, double r2, int64_t max_cells) {
@@ -549,12 +571,16 @@ __device__ Grid grid_params(const double *__restrict__ box, double r2, int64_t m
 // the reference's loop stops when n no longer changes (src/unit_1.cu:1138-1216).
 __global__ void k_grid_clear(int32_t *__restrict__ head, const double *__restrict__ box, double r2,
                              int64_t max_cells, Grid *__restrict__ G, int64_t n_ub, ColBufs cb,
-                             const int64_t *__restrict__ dn, const int64_t *__restrict__ dn_prev) {
+                             const int64_t *__restrict__ dn, const int64_t *__restrict__ dn_prev,
+                             int32_t *__restrict__ pool) {
     __shared__ Grid sq;
     if (threadIdx.x == 0) {
         sq = grid_params(box, r2, max_cells);
         if (dn_prev && *dn == *dn_prev) sq.valid = false;
-        if (blockIdx.x == 0) *G = sq;
+        if (blockIdx.x == 0) {
+            *G = sq;
+            if (pool) *pool = 0;
+        }
     }
     __syncthreads();
     const int64_t cells = sq.valid ? sq.cells : 0;
@@ -628,6 +654,87 @@ __global__ __launch_bounds__(256) void k_setka(const double *__restrict__ x, con
     if (!cb.pc) return;  // the same for every thread
     const bool hit = best != INT64_MAX;
     if (hit) {
+        atomicAdd(&cb.pc[best], 1);
+        atomicAdd(&cb.pcd[best], 1);
+    }
+    const int sg = (int)(blockIdx.x % kColSeg);
+    block_append(hit, (int32_t)i, cb.cand + sg * cb.cap, cb.ncs + sg);
+}
+
+// ---- one grid for the passes of a step ---------------------------------------
+// The passes of a step can share one grid over the box of the vortices alive
+// after the move (a cell of at least 2 sqrt(r2) keeps the four-cell search
+// exact).  Node k of [0, no) stands for slot k; a merge's survivor and a
+// compaction's filled hole get a pool node (id no + q, slot pslot[q]) at the
+// cell of their new position (grid_push), so every live slot keeps a node at
+// the cell of its current position.  Nodes whose slot has since taken other
+// data change nothing: the search tests each slot it reaches by the slot's
+// current circulation and position and keeps the smallest index found,
+// whatever else the lists hold.
+__global__ void k_grid_nodes(const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ g, const int64_t *__restrict__ dn,
+                             const Grid *__restrict__ Gp, int64_t no, int32_t *__restrict__ head,
+                             int32_t *__restrict__ next) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= no) return;
+    const Grid G = *Gp;
+    if (!G.valid || i >= *dn || g[i] == 0) return;
+    int64_t cx, cy;
+    double fx, fy;
+    cell_of(G, x[i], y[i], cx, cy, fx, fy);
+    next[i] = atomicExch(&head[cy * G.nx + cx], (int32_t)i);
+}
+
+// k_setka over the step's grid, recording only the hits (the collapse reads
+// COL of the listed candidates alone).  With dn_prev, a pass whose
+// predecessor of the same kind left the count unchanged searches nothing.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_setka_nodes(const double *__restrict__ x, const double *__restrict__ y,
+                                                     const double *__restrict__ g, const int64_t *__restrict__ dn,
+                                                     const int64_t *__restrict__ dn_prev,
+                                                     const Grid *__restrict__ Gp, double r2, double max_ve_g,
+                                                     const int32_t *__restrict__ head,
+                                                     const int32_t *__restrict__ next, int64_t no,
+                                                     const int32_t *__restrict__ pslot, int32_t *__restrict__ col,
+                                                     ColBufs cb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = *dn;
+    int64_t best = INT64_MAX;
+    if (i < n && !(dn_prev && n == *dn_prev)) {
+        const Grid G = *Gp;
+        const double gi = g[i];
+        if (G.valid && gi != 0) {
+            const double r0 = x[i], r1 = y[i];
+            int64_t cx, cy;
+            double fx, fy;
+            cell_of(G, r0, r1, cx, cy, fx, fy);
+            const int64_t ox = fx < 0.5 ? cx - 1 : cx + 1, oy = fy < 0.5 ? cy - 1 : cy + 1;
+            const int64_t xs[2] = {cx, ox}, ys[2] = {cy, oy};
+            int32_t hd[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t xx = xs[q & 1], yy = ys[q >> 1];
+                hd[q] = (xx >= 0 && xx < G.nx && yy >= 0 && yy < G.ny) ? head[yy * G.nx + xx] : -1;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                for (int32_t kk = hd[q]; kk >= 0; kk = next[kk]) {
+                    const int32_t j = kk < no ? kk : pslot[kk - no];
+                    if (j <= i || j >= best) continue;
+                    const double gj = g[j];
+                    bool hit;
+                    if (KIND == 1)
+                        hit = (gi * gj < 0) && (dist2(r0, r1, x[j], y[j]) < r2);
+                    else
+                        hit = (gi * gj > 0) && (dist2(r0, r1, x[j], y[j]) < r2) && (fabs(gi + gj) < max_ve_g);
+                    if (hit) best = j;
+                }
+            }
+        }
+    }
+    const bool hit = best != INT64_MAX;
+    if (hit) {
+        col[i] = (int32_t)best;
         atomicAdd(&cb.pc[best], 1);
         atomicAdd(&cb.pcd[best], 1);
     }
@@ -711,7 +818,7 @@ int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare, bool after_pa
     // after a pass of the same kind, the other slot holds the count before it
     const int64_t *dn_prev = after_pass ? c->dcount.p + (c->dn_slot == 1 ? 8 : 1) : nullptr;
     hipLaunchKernelGGL(k_grid_clear, dim3(1024), dim3(256), 0, st, c->vals.p, c->red.p + 4, r2, max_cells, G, n_ub,
-                       cb, (const int64_t *)dn, dn_prev);
+                       cb, (const int64_t *)dn, dn_prev, (int32_t *)nullptr);
     hipLaunchKernelGGL(k_grid_insert, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn, G, c->vals.p,
                        c->vals_alt.p);
     if (kind == 1)
@@ -854,7 +961,7 @@ __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict_
 // j's first merger alone.
 template <int KIND>
 __global__ void k_col_merge(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
-                            const int32_t *__restrict__ col, ColBufs cb) {
+                            const int32_t *__restrict__ col, ColBufs cb, GridPush gp) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t i = col_cand(cb, t);
     if (i < 0) return;
@@ -888,9 +995,10 @@ __global__ void k_col_merge(double *__restrict__ x, double *__restrict__ y, doub
         y[j] = 1e+10;
         g[j] = 0.0;
     }
+    if (gp.grid) grid_push(gp, i, nx, ny);  // the survivor may have left its cell
 }
 
-int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared) {
+int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared, const GridPush *gp) {
     if (n_ub <= 0) return 0;
     hipStream_t st = c->stream;
     ColBufs cb{};
@@ -907,12 +1015,79 @@ int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared) {
     hipLaunchKernelGGL(k_col_leaves, dim3(nbc), dim3(256), 0, st, c->col.p, cb, col_merged(c));
     hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, cb.rest, cb.nrest, cb.pcd, cb.state,
                        cb.first, (int32_t *)(c->dcount.p + 3));
+    const GridPush push = gp ? *gp : GridPush{};
     if (kind == 1)
-        hipLaunchKernelGGL(k_col_merge<1>, dim3(nbc), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb);
+        hipLaunchKernelGGL(k_col_merge<1>, dim3(nbc), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb, push);
     else
-        hipLaunchKernelGGL(k_col_merge<2>, dim3(nbc), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb);
+        hipLaunchKernelGGL(k_col_merge<2>, dim3(nbc), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb, push);
     VX_CHECK(hipGetLastError());
     return 0;
 }
 
+// Below this many slots (the default of tune.col_share_min) a grid costs
+// little more than its two launches, and the passes rebuild it: a pass that
+// merges much would otherwise lengthen the lists the next pass walks (dead
+// slots, survivors' second nodes).
+constexpr int64_t kColGridShareMin = 1 << 17;
+
+// The step's collapse (src/unit_1.cu:1138-1216): for each kind, n_col passes
+// of search, resolution and merge, each followed by its compaction.  From
+// tune.col_share_min slots on, the passes share a grid built after the move (see
+// k_grid_nodes): one for both kinds when their radii are equal, else one per
+// kind with cells for its own radius (cells sized for the larger radius would
+// lengthen the other kind's list walks); below it every pass builds its own.
+// A pass pushes at most one node per survivor and one per filled hole, two
+// per slot, so the pool holds 2 n_ub nodes for each pass over a grid; a full
+// pool is reported through the collapse error flag (bit 2).
+int launch_step_collapse(vx_case *c, int64_t n_ub) {
+    if (n_ub <= 0 || c->prm.n_col <= 0) return 0;
+    hipStream_t st = c->stream;
+    const double ve2 = c->prm.ve_size * c->prm.ve_size;
+    const double r2d = c->prm.r_col_diff_sign * c->prm.r_col_diff_sign * ve2;  // src/unit_1.cu:1228-1229
+    const double r2s = c->prm.r_col_same_sign * c->prm.r_col_same_sign * ve2;
+    const bool keep = n_ub >= (c->tune.col_share_min > 0 ? c->tune.col_share_min : kColGridShareMin);
+    const bool shared = keep && r2d == r2s;
+    const int64_t passes = keep ? (shared ? 2 : 1) * c->prm.n_col : 1;  // passes over one grid
+    const int64_t pcap = std::min<int64_t>(2 * passes * n_ub, (int64_t)INT32_MAX - n_ub - 1);
+    const int64_t max_cells = std::max<int64_t>(1 << 20, 8 * n_ub);
+    if (c->col.ensure((size_t)n_ub, st) || c->vals.ensure((size_t)max_cells, st) ||
+        c->cnode.ensure((size_t)(n_ub + 2 * pcap), st) || c->gridp.ensure(sizeof(Grid), st) ||
+        c->dcount.ensure(16, st))
+        return 1;
+    Grid *G = (Grid *)c->gridp.p;
+    int32_t *head = c->vals.p, *next = c->cnode.p, *pslot = c->cnode.p + n_ub + pcap;
+    int32_t *pcount = (int32_t *)(c->dcount.p + 12);
+    const GridPush gp{G, head, next, pslot, pcount, (int32_t *)(c->dcount.p + 3), n_ub, pcap};
+    const unsigned nb = (unsigned)((n_ub + 255) / 256);
+    for (int kind = 1; kind <= 2; ++kind) {
+        const double r2 = kind == 1 ? r2d : r2s;
+        for (int64_t cc = 0; cc < c->prm.n_col; ++cc) {
+            ColBufs cb{};
+            if (col_bufs(c, n_ub, true, cb)) return 1;
+            if (!keep || (cc == 0 && (kind == 1 || !shared))) {
+                // heads cleared, pool emptied, the pass's work arrays prepared
+                // and every live slot linked at its cell
+                hipLaunchKernelGGL(k_grid_clear, dim3(1024), dim3(256), 0, st, head, c->red.p + 4, r2, max_cells,
+                                   G, n_ub, cb, (const int64_t *)step_dn(c), (const int64_t *)nullptr, pcount);
+                hipLaunchKernelGGL(k_grid_nodes, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, step_dn(c), G,
+                                   n_ub, head, next);
+            } else {
+                hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, cb);
+            }
+            // after a pass of the same kind, the other slot holds the count before it
+            const int64_t *dn = step_dn(c);
+            const int64_t *dn_prev = cc > 0 ? c->dcount.p + (c->dn_slot == 1 ? 8 : 1) : nullptr;
+            if (kind == 1)
+                hipLaunchKernelGGL(k_setka_nodes<1>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn,
+                                   dn_prev, G, r2, c->prm.max_ve_g, head, next, n_ub, pslot, c->col.p, cb);
+            else
+                hipLaunchKernelGGL(k_setka_nodes<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn,
+                                   dn_prev, G, r2, c->prm.max_ve_g, head, next, n_ub, pslot, c->col.p, cb);
+            VX_CHECK(hipGetLastError());
+            const GridPush *push = keep ? &gp : nullptr;  // a rebuilt grid needs no pushes
+            if (launch_collapse(c, kind, n_ub, true, push) || launch_compact(c, n_ub, col_merged(c), push)) return 1;
+        }
+    }
+    return 0;
+}
 }  // namespace vx

@@ -189,8 +189,8 @@ class Case:
         check(lib().vx_case_autotune(self._h, float(vinf[0]), float(vinf[1]), C.byref(t)))
         return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
-    def set_tuning(self, tpt_bs=0, tpt_diff=0, leaf_off=-1):
-        t = Tuning(tpt_bs=tpt_bs, tpt_diff=tpt_diff, leaf_off=leaf_off)
+    def set_tuning(self, tpt_bs=0, tpt_diff=0, leaf_off=-1, col_share_min=0):
+        t = Tuning(tpt_bs=tpt_bs, tpt_diff=tpt_diff, leaf_off=leaf_off, col_share_min=col_share_min)
         check(lib().vx_case_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:
