@@ -311,28 +311,40 @@ struct WriteIndex {  // out[pos] = i for flagged items (ordered selection of ind
 };
 
 // ---- LSD radix sort: 32-bit keys, 32-bit values, 8-bit digits ---------------------------
-// Four stable passes.  Per pass: the histogram launch counts each tile's
-// digits (one LDS add per run of equal digits in a wave, found with 8
-// ballots) and stores them digit-major, hist[d * ntiles + tile]; one
-// exclusive scan of that array (scan_values) gives every (digit, tile) its
-// output offset, and the scan's total, n, lands after it, so the tile's count
-// of digit d is off[d * ntiles + tile + 1] - off[d * ntiles + tile].  The
-// scatter launch ranks each key within its tile (the wave's earlier lanes
-// with the same digit, plus the tile's earlier waves and rounds), places the
-// tile in LDS sorted by digit, and writes it out in that order, so each
-// digit's run of the tile goes to consecutive addresses.  Stable: equal keys
-// keep their input order.  No work-group waits on another.
+// Four stable passes of one launch each, after one launch that counts every
+// pass's digits over all the keys.  A pass takes tiles of kSortTile keys; a
+// tile counts its digits (one LDS add per run of equal digits in a wave,
+// found with 8 ballots), publishes the counts in its status words, ranks its
+// keys in LDS by digit (the wave's earlier lanes with the same digit, plus
+// the tile's earlier waves and rounds) and writes them out digit run by digit
+// run.  A run's output offset is the global count of the smaller digits plus
+// the digit's count in the earlier tiles, which the tile reads back from
+// their status words: a word holds a tile's own count (kStAgg) or its count
+// including every earlier tile (kStInc); the look-back adds own counts until
+// it meets an inclusive one, then publishes its own inclusive count.
+// Progress: a work-group takes its tile by ticket, so every tile it reads
+// belongs to a work-group that took its ticket earlier and is resident, and
+// a tile publishes its own counts before it reads any other tile's, so each
+// read waits at most until an earlier tile has counted.  A read still
+// unpublished after kSpinMax polls (which this argument excludes) sets *err,
+// and the tile publishes anyway, so a failure ends as a reported error.
+// The status words are read and written with relaxed atomics at agent scope:
+// each carries its count itself, and no other data is passed through them.
+// Stable: equal keys keep their input order.
 constexpr int kRadixBits = 8;
 constexpr int kBins = 1 << kRadixBits;
+constexpr int kSortPasses = 32 / kRadixBits;
 constexpr int kSortIPT = 8;                 // keys per thread per tile
 constexpr int kSortTile = kT * kSortIPT;
+constexpr int64_t kSpinMax = 1 << 20;
+constexpr uint64_t kStAgg = 1ull << 62, kStInc = 2ull << 62, kStMask = 3ull << 62, kStCount = 0xffffffffull;
 
 inline int64_t sort_tiles_of(int64_t n) { return n > 0 ? (n + kSortTile - 1) / kSortTile : 0; }
-// int32 entries of scratch radix_sort_pairs needs for n keys: the offsets
-// (plus the total) and the offset scan's tile sums
+// int32 entries of scratch radix_sort_pairs needs for n keys: every pass's
+// status words (64-bit, [pass][tile][digit]), digit counts ([pass][digit])
+// and ticket
 inline int64_t radix_tmp_size(int64_t n) {
-    const int64_t h = (int64_t)kBins * sort_tiles_of(n);
-    return h + 1 + tiles_of(h) + 1;
+    return 2 * (int64_t)kSortPasses * kBins * sort_tiles_of(n) + kSortPasses * kBins + kSortPasses;
 }
 
 // Lanes of this wave whose digit equals this lane's (among lanes with ok).
@@ -346,61 +358,94 @@ __device__ __forceinline__ uint64_t digit_peers(int d, bool ok) {
     return peers;
 }
 
+// Every pass's digit counts over all n keys, added into ghist (zeroed).
 template <int DUMMY = 0>
-__global__ __launch_bounds__(kT) void k_radix_hist(const uint32_t *__restrict__ key, int64_t n, int shift,
-                                                   int32_t *__restrict__ hist, int64_t ntiles) {
-    __shared__ int32_t h[kBins];
-    h[threadIdx.x] = 0;
+__global__ __launch_bounds__(kT) void k_radix_counts(const uint32_t *__restrict__ key, int64_t n,
+                                                     int32_t *__restrict__ ghist) {
+    __shared__ int32_t h[kSortPasses][kBins];
+    for (int p = 0; p < kSortPasses; ++p) h[p][threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
-#pragma unroll 2
     for (int r = 0; r < kSortIPT; ++r) {
         const int64_t i = base + r * kT + threadIdx.x;
         const bool ok = i < n;
-        const int d = ok ? (int)((key[i] >> shift) & (kBins - 1)) : 0;
-        const uint64_t peers = digit_peers(d, ok);
-        if (ok && (peers & lanes_below()) == 0) atomicAdd(&h[d], (int32_t)__popcll(peers));
+        const uint32_t k = ok ? key[i] : 0u;
+#pragma unroll
+        for (int p = 0; p < kSortPasses; ++p) {
+            const int d = (int)((k >> (p * kRadixBits)) & (kBins - 1));
+            const uint64_t peers = digit_peers(d, ok);
+            if (ok && (peers & lanes_below()) == 0) atomicAdd(&h[p][d], (int32_t)__popcll(peers));
+        }
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    for (int p = 0; p < kSortPasses; ++p) {
+        const int32_t v = h[p][threadIdx.x];
+        if (v) atomicAdd(&ghist[p * kBins + threadIdx.x], v);
+    }
 }
 
+// One pass: kin/vin by the digit at shift into kout/vout.  ghist: this
+// pass's digit counts; ticket: zeroed; status: ntiles x kBins zeroed words.
 template <int DUMMY = 0>
-__global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
-                                                      int64_t n, int shift, const int32_t *__restrict__ off,
-                                                      int64_t ntiles, uint32_t *__restrict__ kout,
-                                                      int32_t *__restrict__ vout) {
+__global__ __launch_bounds__(kT) void k_radix_onesweep(const uint32_t *__restrict__ kin,
+                                                       const int32_t *__restrict__ vin, int64_t n, int shift,
+                                                       const int32_t *__restrict__ ghist, int32_t *__restrict__ ticket,
+                                                       uint64_t *__restrict__ status, uint32_t *__restrict__ kout,
+                                                       int32_t *__restrict__ vout, int32_t *__restrict__ err) {
     __shared__ uint32_t sk[kSortTile];   // the tile sorted by digit
     __shared__ int32_t sv[kSortTile];
+    __shared__ int32_t cnt[kBins];       // the tile's count of each digit
     __shared__ int32_t loc[kBins];       // first slot of each digit in sk
-    __shared__ int32_t gbase[kBins];     // first output position of each digit's run
     __shared__ int32_t run[kBins];       // next free slot of each digit in sk
+    __shared__ int32_t gbase[kBins];     // first output position of each digit's run
     __shared__ int32_t wcnt[kW][kBins];
-    __shared__ int32_t ws[kW];
-    const int w = threadIdx.x >> 6;
-    {
-        const int64_t q = (int64_t)threadIdx.x * ntiles + blockIdx.x;
-        const int32_t g = off[q], cnt = off[q + 1] - g;
-        const int32_t inc = wave_inclusive(cnt, OpSum());
-        if (lane_id() == 63) ws[w] = inc;
-        __syncthreads();
-        int32_t pre = inc - cnt;
-        for (int k = 0; k < w; ++k) pre += ws[k];
-        loc[threadIdx.x] = pre;
-        run[threadIdx.x] = pre;
-        gbase[threadIdx.x] = g;
+    __shared__ int32_t ws[2][kW];
+    __shared__ int32_t tile_s;
+    const int w = threadIdx.x >> 6, dd = threadIdx.x;  // thread dd handles digit dd in the per-digit steps
+    if (threadIdx.x == 0) tile_s = atomicAdd(ticket, 1);
+    cnt[dd] = 0;
+    __syncthreads();
+    const int64_t tile = tile_s;
+    const int64_t base = tile * kSortTile;
+    uint32_t kr[kSortIPT];
+    int32_t vr[kSortIPT];
+#pragma unroll
+    for (int r = 0; r < kSortIPT; ++r) {
+        const int64_t i = base + r * kT + threadIdx.x;
+        kr[r] = i < n ? kin[i] : 0u;
+        vr[r] = i < n ? vin[i] : 0;
     }
-    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortIPT; ++r) {
+        const bool ok = base + r * kT + threadIdx.x < n;
+        const int d = (int)((kr[r] >> shift) & (kBins - 1));
+        const uint64_t peers = digit_peers(d, ok);
+        if (ok && (peers & lanes_below()) == 0) atomicAdd(&cnt[d], (int32_t)__popcll(peers));
+    }
+    __syncthreads();
+    const int32_t c = cnt[dd];
+    uint64_t *own = status + tile * kBins + dd;
+    __hip_atomic_store(own, (tile == 0 ? kStInc : kStAgg) | (uint64_t)(uint32_t)c, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    // the digit's first slot in sk, and the global count of the smaller digits
+    const int32_t g = ghist[dd];
+    const int32_t inc = wave_inclusive(c, OpSum()), ginc = wave_inclusive(g, OpSum());
+    if (lane_id() == 63) {
+        ws[0][w] = inc;
+        ws[1][w] = ginc;
+    }
+    __syncthreads();
+    int32_t pre = inc - c, gpre = ginc - g;
+    for (int k = 0; k < w; ++k) {
+        pre += ws[0][k];
+        gpre += ws[1][k];
+    }
+    loc[dd] = pre;
+    run[dd] = pre;
     for (int r = 0; r < kSortIPT; ++r) {
         for (int k = 0; k < kW; ++k) wcnt[k][threadIdx.x] = 0;
-        const int64_t i = base + r * kT + threadIdx.x;
-        const bool ok = i < n;
-        uint32_t k = 0;
-        int32_t v = 0;
-        if (ok) {
-            k = kin[i];
-            v = vin[i];
-        }
+        const bool ok = base + r * kT + threadIdx.x < n;
+        const uint32_t k = kr[r];
         const int d = (int)((k >> shift) & (kBins - 1));
         const uint64_t peers = digit_peers(d, ok);
         const int32_t below = (int32_t)__popcll(peers & lanes_below());
@@ -411,7 +456,7 @@ __global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict
             int32_t pos = run[d] + below;
             for (int q = 0; q < w; ++q) pos += wcnt[q][d];
             sk[pos] = k;
-            sv[pos] = v;
+            sv[pos] = vr[r];
         }
         __syncthreads();
         int32_t add = 0;
@@ -419,6 +464,28 @@ __global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict
         for (int q = 0; q < kW; ++q) add += wcnt[q][threadIdx.x];
         run[threadIdx.x] += add;
     }
+    // the digit's count in the earlier tiles
+    int64_t excl = 0;
+    if (tile > 0) {
+        int64_t t = tile - 1, polls = 0;
+        for (;;) {
+            const uint64_t sw = __hip_atomic_load(status + t * kBins + dd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t st = sw & kStMask;
+            if (st == 0) {
+                if (++polls > kSpinMax) {
+                    if (err) atomicOr(err, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += (int64_t)(sw & kStCount);
+            if (st == kStInc || t == 0) break;
+            --t;
+        }
+        __hip_atomic_store(own, kStInc | (uint64_t)(uint32_t)(excl + c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    gbase[dd] = (int32_t)(gpre + excl);
     __syncthreads();
     const int64_t m = n - base < kSortTile ? n - base : kSortTile;
     for (int j = threadIdx.x; j < m; j += kT) {
@@ -431,24 +498,24 @@ __global__ __launch_bounds__(kT) void k_radix_scatter(const uint32_t *__restrict
 }
 
 // Sorts (k0, v0) of n items; k1, v1 are the same size.  The result ends in
-// k0, v0 (four passes).  tmp must hold radix_tmp_size(n) int32 entries.
+// k0, v0 (four passes).  tmp must hold radix_tmp_size(n) int32 entries (64-bit
+// aligned); *err (if given) gets bit 1 when a look-back gave up.
 inline hipError_t radix_sort_pairs(uint32_t *k0, uint32_t *k1, int32_t *v0, int32_t *v1, int64_t n, int32_t *tmp,
-                                   hipStream_t st) {
+                                   int32_t *err, hipStream_t st) {
     const int64_t nt = sort_tiles_of(n);
     if (nt == 0) return hipSuccess;
-    const int64_t h = (int64_t)kBins * nt;
-    int32_t *off = tmp, *stmp = tmp + h + 1;
+    uint64_t *status = reinterpret_cast<uint64_t *>(tmp);
+    int32_t *ghist = tmp + 2 * (int64_t)kSortPasses * kBins * nt;
+    int32_t *ticket = ghist + kSortPasses * kBins;
+    const hipError_t e = fill(tmp, radix_tmp_size(n), 0, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_radix_counts<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)k0, n, ghist);
     uint32_t *ka = k0, *kb = k1;
     int32_t *va = v0, *vb = v1;
-    for (int pass = 0; pass < 32 / kRadixBits; ++pass) {
-        const int shift = pass * kRadixBits;
-        hipLaunchKernelGGL(k_radix_hist<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka, n, shift, off,
-                           nt);
-        // in place: each offset is read and then written by the same thread
-        const hipError_t e = scan_values<false>(ArrayValue{off}, h, OpSum(), WriteScan{off}, stmp, off + h, st);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_radix_scatter<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka,
-                           (const int32_t *)va, n, shift, (const int32_t *)off, nt, kb, vb);
+    for (int pass = 0; pass < kSortPasses; ++pass) {
+        hipLaunchKernelGGL(k_radix_onesweep<0>, dim3((unsigned)nt), dim3(kT), 0, st, (const uint32_t *)ka,
+                           (const int32_t *)va, n, pass * kRadixBits, (const int32_t *)(ghist + pass * kBins),
+                           ticket + pass, status + (int64_t)pass * kBins * nt, kb, vb, err);
         uint32_t *tk = ka; ka = kb; kb = tk;
         int32_t *tv = va; va = vb; vb = tv;
     }

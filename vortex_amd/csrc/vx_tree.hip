@@ -71,7 +71,7 @@ struct TreeBufs {
     DBuf<double2> cd;
     DBuf<float2> cf;
     DBuf<NodeRec> rec;
-    DBuf<unsigned long long> cnt;
+    DBuf<unsigned long long> cnt;   // counters, zeroed by each build; slot 15 (int32): the key sort's error flag
     DBuf<unsigned char> tmp, tmp2;  // scan scratch of the build's two concurrent chains
     bool sd_fresh = false;          // sd holds d in sorted order, as the last Biot-Savart pass wrote it
     int64_t s = 0;                         // slots keyed and sorted by the last build
@@ -883,10 +883,11 @@ __global__ __launch_bounds__(1024) void k_upper_top(int32_t root, int64_t nb, in
 __global__ void k_tree_counts(const int32_t *__restrict__ gscan, const int32_t *__restrict__ gflag,
                               const int32_t *__restrict__ bscan, const int32_t *__restrict__ bstart,
                               const int32_t *__restrict__ uscan, const int32_t *__restrict__ upper, int64_t m, int big,
-                              int64_t *__restrict__ out) {
+                              const int32_t *__restrict__ sort_err, int64_t *__restrict__ out) {
     out[0] = (int64_t)gscan[m - 1] + gflag[m - 1];
     out[1] = big ? (int64_t)bscan[m - 1] + bstart[m - 1] : 1;
     out[2] = big ? (int64_t)uscan[m - 2] + upper[m - 2] : 0;
+    out[3] = *sort_err;
 }
 
 __global__ void k_pack_nodes(int64_t nn, const double *__restrict__ ncx, const double *__restrict__ ncy,
@@ -2516,7 +2517,8 @@ static int tree_build_a(vx_case *c, int64_t s, hipStream_t st, Births B) {
     // stable LSD radix sort of (key, index); the sorted pairs end in keys and
     // idx, which then take the names keys_alt and perm
     if (t.tmp.ensure(sizeof(int32_t) * (size_t)prim::radix_tmp_size(s), st)) return 1;
-    VX_CHECK(prim::radix_sort_pairs(t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p, s, (int32_t *)t.tmp.p, st));
+    VX_CHECK(prim::radix_sort_pairs(t.keys.p, t.keys_alt.p, t.idx.p, t.perm.p, s, (int32_t *)t.tmp.p,
+                                    (int32_t *)(t.cnt.p + 15), st));
     std::swap(t.keys, t.keys_alt);
     std::swap(t.idx, t.perm);
     return 0;
@@ -2592,10 +2594,14 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
     if (t.sizes.ensure(4, st)) return 1;
     VX_CHECK(hipStreamWaitEvent(st, c->aux_done[0], 0));
     hipLaunchKernelGGL(k_tree_counts, dim3(1), dim3(1), 0, st, t.gscan.p, t.gflag.p, t.bscan.p, t.bstart.p, t.uscan.p,
-                       t.upper.p, m, (int)(m > kLeaf), t.sizes.p);
-    VX_CHECK(hipMemcpyAsync(c->h_count, t.sizes.p, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+                       t.upper.p, m, (int)(m > kLeaf), (const int32_t *)(t.cnt.p + 15), t.sizes.p);
+    VX_CHECK(hipMemcpyAsync(c->h_count, t.sizes.p, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     VX_CHECK(hipEventRecord(t.ev, st));
     VX_CHECK(hipEventSynchronize(t.ev));
+    if (c->h_count[3]) {
+        set_error("tree key sort: a tile's look-back did not finish");
+        return 1;
+    }
     t.ngroup = c->h_count[0];
     nb = c->h_count[1];
     nu = c->h_count[2];
