@@ -325,20 +325,30 @@ __global__ void k_bucket_first(const int32_t *__restrict__ bstart, const int32_t
     if (t < m && bstart[t]) first[bscan[t]] = (int32_t)t;
 }
 
-__global__ void k_bucket_count(const int32_t *__restrict__ first, int64_t nb, int64_t m,
+// sz: the device counts of k_tree_counts (sz[1] buckets, sz[2] upper nodes).
+__global__ void k_bucket_count(const int32_t *__restrict__ first, const int64_t *__restrict__ sz, int64_t m,
                                int32_t *__restrict__ count) {
+    const int64_t nb = sz[1];
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b < nb) count[b] = (int32_t)((b + 1 < nb ? (int64_t)first[b + 1] : m) - first[b]);
+}
+
+// parent[0, nn) = -1 for the nn = sz[1] + sz[2] nodes.
+__global__ void k_parent_init(const int64_t *__restrict__ sz, int32_t *__restrict__ parent) {
+    const int64_t nn = sz[1] + sz[2];
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
+        parent[u] = -1;
 }
 
 __global__ void k_upper_nodes(const int32_t *__restrict__ lo, const int32_t *__restrict__ hi,
                               const int32_t *__restrict__ left, const int32_t *__restrict__ right,
                               const int32_t *__restrict__ upper, const int32_t *__restrict__ uscan,
-                              const int32_t *__restrict__ bscan, int64_t m, int64_t nb, int32_t *__restrict__ first,
-                              int32_t *__restrict__ count, int32_t *__restrict__ child0,
+                              const int32_t *__restrict__ bscan, int64_t m, const int64_t *__restrict__ sz,
+                              int32_t *__restrict__ first, int32_t *__restrict__ count, int32_t *__restrict__ child0,
                               int32_t *__restrict__ child1, int32_t *__restrict__ parent) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m - 1 || !upper[i]) return;
+    const int64_t nb = sz[1];
     const int32_t u = (int32_t)(nb + uscan[i]);
     const int32_t ch[2] = {left[i], right[i]};
     int32_t id[2];
@@ -454,36 +464,43 @@ __device__ __forceinline__ int64_t bucket_n(int32_t u, int64_t nb, int64_t m, co
 
 struct CutNode {
     const int32_t *first, *count, *bscan, *parent;
-    int64_t nb, m;
+    const int64_t *sz;  // device counts: sz[1] buckets
+    int64_t m;
     __device__ bool operator()(int32_t u) const {
+        const int64_t nb = sz[1];
         if (bucket_n(u, nb, m, first, count, bscan) > kCut) return false;
         const int32_t p = parent[u];
         return p < 0 || bucket_n(p, nb, m, first, count, bscan) > kCut;
     }
 };
 
-// The cut nodes (CutNode) among nodes [0, nn), appended to cut with one
-// atomic per work-group on *ncut (zero on entry); their order is arbitrary,
-// each one is processed on its own.
-__global__ __launch_bounds__(256) void k_cut_list(CutNode pred, int64_t nn, int32_t *__restrict__ cut,
-                                                  unsigned long long *__restrict__ ncut) {
+// The cut nodes (CutNode) among the nn = sz[1] + sz[2] nodes, appended to
+// cut with one atomic per work-group and round on *ncut (zero on entry);
+// their order is arbitrary, each one is processed on its own.  Grid-stride
+// over the device count, so the launch need not know it.
+__global__ __launch_bounds__(256) void k_cut_list(CutNode pred, const int64_t *__restrict__ sz,
+                                                  int32_t *__restrict__ cut, unsigned long long *__restrict__ ncut) {
     __shared__ int wcnt[4];
     __shared__ unsigned long long base;
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nn = sz[1] + sz[2];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const bool hit = u < nn && pred((int32_t)u);
-    const uint64_t mk = __ballot(hit);
-    if (lane == 0) wcnt[w] = __popcll(mk);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-        base = tot ? atomicAdd(ncut, (unsigned long long)tot) : 0ull;
-    }
-    __syncthreads();
-    if (hit) {
-        unsigned long long k = base + __popcll(mk & ((1ull << lane) - 1));
-        for (int q = 0; q < w; ++q) k += wcnt[q];
-        cut[k] = (int32_t)u;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < nn; b0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t u = b0 + threadIdx.x;
+        const bool hit = u < nn && pred((int32_t)u);
+        const uint64_t mk = __ballot(hit);
+        if (lane == 0) wcnt[w] = __popcll(mk);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+            base = tot ? atomicAdd(ncut, (unsigned long long)tot) : 0ull;
+        }
+        __syncthreads();
+        if (hit) {
+            unsigned long long k = base + __popcll(mk & ((1ull << lane) - 1));
+            for (int q = 0; q < w; ++q) k += wcnt[q];
+            cut[k] = (int32_t)u;
+        }
+        __syncthreads();  // wcnt and base are rewritten by the next round
     }
 }
 
@@ -2590,13 +2607,39 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
         VX_CHECK(prim::select_flags(prim::ArrayFlag{t.upper.p}, m - 1, prim::WriteScan{t.uscan.p}, tt,
                                     (int32_t *)nullptr, st));
     }
-    // the group, bucket and upper-node counts in one read-back
-    if (t.sizes.ensure(4, st)) return 1;
+    // the group, bucket and upper-node counts in one read-back; the node
+    // links and the cut list, which need only the device counts, are queued
+    // before the host waits for it, their arrays sized for the bound
+    // nn <= 2 m - 1
+    const int64_t nnb = 2 * m;
+    if (t.sizes.ensure(4, st) || t.first.ensure(nnb, st) || t.count.ensure(nnb, st) || t.child0.ensure(nnb, st) ||
+        t.child1.ensure(nnb, st) || t.parent.ensure(nnb, st) || t.arr.ensure((size_t)nnb, st))
+        return 1;
     VX_CHECK(hipStreamWaitEvent(st, c->aux_done[0], 0));
     hipLaunchKernelGGL(k_tree_counts, dim3(1), dim3(1), 0, st, t.gscan.p, t.gflag.p, t.bscan.p, t.bstart.p, t.uscan.p,
                        t.upper.p, m, (int)(m > kLeaf), (const int32_t *)(t.cnt.p + 15), t.sizes.p);
     VX_CHECK(hipMemcpyAsync(c->h_count, t.sizes.p, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     VX_CHECK(hipEventRecord(t.ev, st));
+    hipLaunchKernelGGL(k_parent_init, dim3((unsigned)std::min<int64_t>(1024, (nnb + 255) / 256)), dim3(256), 0, st,
+                       (const int64_t *)t.sizes.p, t.parent.p);
+    if (m > kLeaf) {
+        hipLaunchKernelGGL(k_bucket_first, dim3(gm), dim3(256), 0, st, t.bstart.p, t.bscan.p, m, t.first.p);
+        hipLaunchKernelGGL(k_bucket_count, dim3(gm), dim3(256), 0, st, t.first.p, (const int64_t *)t.sizes.p, m,
+                           t.count.p);
+        hipLaunchKernelGGL(k_upper_nodes, dim3(gm), dim3(256), 0, st, t.rlo.p, t.rhi.p, t.rleft.p, t.rright.p,
+                           t.upper.p, t.uscan.p, t.bscan.p, m, (const int64_t *)t.sizes.p, t.first.p, t.count.p,
+                           t.child0.p, t.child1.p, t.parent.p);
+    } else {
+        const int32_t one[2] = {0, (int32_t)m};
+        VX_CHECK(hipMemcpyAsync(t.first.p, &one[0], sizeof(int32_t), hipMemcpyHostToDevice, st));
+        VX_CHECK(hipMemcpyAsync(t.count.p, &one[1], sizeof(int32_t), hipMemcpyHostToDevice, st));
+    }
+    // cut nodes, in any order (count in cnt[10], cleared with the counters)
+    int32_t *cut = t.arr.p;
+    unsigned long long *ncut = t.cnt.p + 10;
+    hipLaunchKernelGGL(k_cut_list, dim3((unsigned)std::min<int64_t>(1024, (nnb + 255) / 256)), dim3(256), 0, st,
+                       CutNode{t.first.p, t.count.p, t.bscan.p, t.parent.p, (const int64_t *)t.sizes.p, m},
+                       (const int64_t *)t.sizes.p, cut, ncut);
     VX_CHECK(hipEventSynchronize(t.ev));
     if (c->h_count[3]) {
         set_error("tree key sort: a tile's look-back did not finish");
@@ -2610,30 +2653,10 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
     t.root = (int32_t)(nu > 0 ? nb : 0);
     const int64_t nn = t.nnode;
     if (t.ncx.ensure(nn, st) || t.ncy.ensure(nn, st) || t.nr.ensure(nn, st) || t.nbb.ensure(nn, st) ||
-        t.child0.ensure(nn, st) || t.child1.ensure(nn, st) || t.first.ensure(nn, st) || t.count.ensure(nn, st) ||
-        t.parent.ensure(nn, st) || t.toplist.ensure(nn, st) || t.rec.ensure(nn, st) || t.cd.ensure(nn * order, st) ||
+        t.toplist.ensure(nn, st) || t.rec.ensure(nn, st) || t.cd.ensure(nn * order, st) ||
         t.cf.ensure(nn * order, st))
         return 1;
-    VX_CHECK(prim::fill(t.parent.p, nn, -1, st));
-    if (m > kLeaf) {
-        hipLaunchKernelGGL(k_bucket_first, dim3(gm), dim3(256), 0, st, t.bstart.p, t.bscan.p, m, t.first.p);
-        hipLaunchKernelGGL(k_bucket_count, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, t.first.p, nb, m,
-                           t.count.p);
-        hipLaunchKernelGGL(k_upper_nodes, dim3(gm), dim3(256), 0, st, t.rlo.p, t.rhi.p, t.rleft.p, t.rright.p,
-                           t.upper.p, t.uscan.p, t.bscan.p, m, nb, t.first.p, t.count.p, t.child0.p, t.child1.p,
-                           t.parent.p);
-    } else {
-        const int32_t one[2] = {0, (int32_t)m};
-        VX_CHECK(hipMemcpyAsync(t.first.p, &one[0], sizeof(int32_t), hipMemcpyHostToDevice, st));
-        VX_CHECK(hipMemcpyAsync(t.count.p, &one[1], sizeof(int32_t), hipMemcpyHostToDevice, st));
-    }
     const unsigned lb = (unsigned)((nb + 3) / 4);
-    // cut nodes, in any order (count in cnt[10], cleared with the counters)
-    if (t.arr.ensure((size_t)nn, st)) return 1;
-    int32_t *cut = t.arr.p;
-    unsigned long long *ncut = t.cnt.p + 10;
-    hipLaunchKernelGGL(k_cut_list, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st,
-                       CutNode{t.first.p, t.count.p, t.bscan.p, t.parent.p, nb, m}, nn, cut, ncut);
     const unsigned gc = (unsigned)std::min<int64_t>(nb, 2048);
     VX_CHECK(hipStreamWaitEvent(st, c->aux_done[1], 0));  // the gathered particles
     if (order == 32) {
