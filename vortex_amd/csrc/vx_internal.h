@@ -351,7 +351,9 @@ int launch_live_box(vx_case *c);                       // box of g != 0 in [0, *
 int launch_setka(vx_case *c, int kind, int64_t n_ub, bool prepare, bool after_pass = false);
 // default targets per thread of the direct kernels for n_tgt targets (vx_pairwise.hip)
 int direct_tpt_default(bool f32, int64_t n_tgt);
-int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared, const GridPush *gp = nullptr);
+// With reset, the collapse work arrays are left prepared for the next pass.
+int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared, const GridPush *gp = nullptr,
+                    bool reset = false);
 // Every collapse pass of a step (both kinds, n_col each, with their
 // compactions), from 2^17 slots on over a grid kept from pass to pass.
 // Collapse error flag in dcount[3] (bit 2: the grid's node pool was full).

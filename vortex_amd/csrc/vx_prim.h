@@ -126,34 +126,53 @@ __global__ __launch_bounds__(1024) void k_tile_scan(int32_t *__restrict__ tot, i
 // and maxima do not depend on the order of reduction.
 constexpr int64_t kFuseTiles = 2048;
 
-template <bool SUMS, class Op, class TT>
-__device__ __forceinline__ int32_t tile_start(const int32_t *__restrict__ tile, int64_t nt, Op op, TT *total) {
+// With GRAND, *grand also gets the reduction of every tile: with SUMS from
+// the tiles' totals, else from *total (which k_tile_scan wrote).
+template <bool SUMS, bool GRAND = false, class Op, class TT>
+__device__ __forceinline__ int32_t tile_start(const int32_t *__restrict__ tile, int64_t nt, Op op, TT *total,
+                                              int32_t *grand = nullptr) {
     if constexpr (!SUMS) {
+        if constexpr (GRAND) *grand = (int32_t)*total;
         return tile[blockIdx.x];
     } else {
-        __shared__ int32_t ws[kW];
-        int32_t a = Op::id;
-        for (int64_t k = threadIdx.x; k < (int64_t)blockIdx.x; k += kT) a = op(a, tile[k]);
+        __shared__ int32_t ws[2][kW];
+        int32_t a = Op::id, b = Op::id;
+        const int64_t lim = GRAND ? nt : (int64_t)blockIdx.x;
+        for (int64_t k = threadIdx.x; k < lim; k += kT) {
+            const int32_t v = tile[k];
+            if (k < (int64_t)blockIdx.x) a = op(a, v);
+            if constexpr (GRAND) b = op(b, v);
+        }
         a = wave_reduce(a, op);
-        if (lane_id() == 0) ws[threadIdx.x >> 6] = a;
+        if constexpr (GRAND) b = wave_reduce(b, op);
+        if (lane_id() == 0) {
+            ws[0][threadIdx.x >> 6] = a;
+            ws[1][threadIdx.x >> 6] = b;
+        }
         __syncthreads();
-        int32_t pre = Op::id;
+        int32_t pre = Op::id, all = Op::id;
 #pragma unroll
-        for (int q = 0; q < kW; ++q) pre = op(pre, ws[q]);
+        for (int q = 0; q < kW; ++q) {
+            pre = op(pre, ws[0][q]);
+            all = op(all, ws[1][q]);
+        }
+        if constexpr (GRAND) *grand = all;
         if (total && threadIdx.x == 0 && (int64_t)blockIdx.x == nt - 1) *total = (TT)op(pre, tile[blockIdx.x]);
         return pre;
     }
 }
 
 // Phase 3: each flagged item's rank among the flagged items (its output
-// position); the writer sees every item i < n with its flag.
-template <class F, class Wr, bool SUMS = false, class TT = int32_t>
+// position); the writer sees every item i < n with its flag, and with GRAND
+// also the count of all flagged items.
+template <class F, class Wr, bool SUMS = false, class TT = int32_t, bool GRAND = false>
 __global__ __launch_bounds__(kT) void k_flag_apply(F f, int64_t n, const int32_t *__restrict__ tile_off, Wr wr,
                                                    TT *total = nullptr) {
     __shared__ int32_t wc[2][kW];
     const int64_t base = (int64_t)blockIdx.x * kTile;
     const int w = threadIdx.x >> 6;
-    int32_t run = tile_start<SUMS>(tile_off, (n + kTile - 1) / kTile, OpSum(), total);
+    int32_t grand = 0;
+    int32_t run = tile_start<SUMS, GRAND>(tile_off, (n + kTile - 1) / kTile, OpSum(), total, &grand);
     for (int r = 0; r < kIPT; ++r) {
         const int64_t i = base + r * kT + threadIdx.x;
         const int fl = i < n ? f(i) : 0;
@@ -167,7 +186,12 @@ __global__ __launch_bounds__(kT) void k_flag_apply(F f, int64_t n, const int32_t
             off += q < w ? v : 0;
             all += v;
         }
-        if (i < n) wr(i, off + (int32_t)__popcll(bal & lanes_below()), fl);
+        if (i < n) {
+            if constexpr (GRAND)
+                wr(i, off + (int32_t)__popcll(bal & lanes_below()), fl, grand);
+            else
+                wr(i, off + (int32_t)__popcll(bal & lanes_below()), fl);
+        }
         run += all;
     }
 }
@@ -240,6 +264,24 @@ inline hipError_t select_flags(F f, int64_t n, Wr wr, int32_t *tmp, TT *total, h
     if (nt > 0)
         hipLaunchKernelGGL((k_flag_apply<F, Wr>), dim3((unsigned)nt), dim3(kT), 0, st, f, n, (const int32_t *)tmp, wr,
                            (int32_t *)nullptr);
+    return hipGetLastError();
+}
+
+// select_flags whose writer also takes the count of all flagged items,
+// wr(i, pos, flag, count); total (device, not null) receives that count.
+template <class F, class Wr>
+inline hipError_t select_flags_counted(F f, int64_t n, Wr wr, int32_t *tmp, int32_t *total, hipStream_t st) {
+    const int64_t nt = tiles_of(n);
+    if (nt > 0) hipLaunchKernelGGL(k_flag_count<F>, dim3((unsigned)nt), dim3(kT), 0, st, f, n, tmp);
+    if (nt > 0 && nt <= kFuseTiles) {
+        hipLaunchKernelGGL((k_flag_apply<F, Wr, true, int32_t, true>), dim3((unsigned)nt), dim3(kT), 0, st, f, n,
+                           (const int32_t *)tmp, wr, total);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_tile_scan<int32_t, OpSum>), dim3(1), dim3(1024), 0, st, tmp, nt, total, OpSum());
+    if (nt > 0)
+        hipLaunchKernelGGL((k_flag_apply<F, Wr, false, int32_t, true>), dim3((unsigned)nt), dim3(kT), 0, st, f, n,
+                           (const int32_t *)tmp, wr, total);
     return hipGetLastError();
 }
 

@@ -362,28 +362,27 @@ struct LiveFlag {
 
 __device__ __forceinline__ bool compact_skip(const int64_t *merged) { return merged && *merged == 0; }
 
-// s = exclusive scan of the flags; n1 = total live (s[n-1] + f[n-1]), written
-// to the other count slot *dn1 by thread 0.
-__global__ void k_compact_lists(const double *__restrict__ g, const int32_t *__restrict__ s,
-                                const int64_t *__restrict__ dn, int64_t *__restrict__ dn1,
-                                int32_t *__restrict__ holes, int32_t *__restrict__ donors,
-                                const int64_t *__restrict__ merged) {
-    const int64_t n = *dn;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (compact_skip(merged)) {
-        if (i == 0) *dn1 = n;
-        return;
+// The writer of the live-flag scan: s[i] = pos, the live entries before i;
+// n1 = the count of live entries, written to the other count slot *dn1; the
+// holes of [0, n1) by rank, the live entries of [n1, n) by rank from the top.
+struct CompactLists {
+    int32_t *s, *holes, *donors;
+    const int64_t *dn, *merged;
+    int64_t *dn1;
+    __device__ void operator()(int64_t i, int32_t pos, int f, int32_t n1) const {
+        if (compact_skip(merged)) {
+            if (i == 0) *dn1 = *dn;
+            return;
+        }
+        s[i] = pos;
+        if (i == 0) *dn1 = n1;
+        if (i < n1) {
+            if (!f) holes[i - pos] = (int32_t)i;  // rank among the holes of [0, n1)
+        } else if (f) {
+            donors[n1 - pos - 1] = (int32_t)i;    // rank from the top among live entries of [n1, n)
+        }
     }
-    const int64_t n1 = n > 0 ? (int64_t)s[n - 1] + live_at(g, n - 1, n) : 0;
-    if (i == 0) *dn1 = n1;
-    if (i >= n) return;
-    const int32_t f = live_at(g, i, n);
-    if (i < n1) {
-        if (!f) holes[i - s[i]] = (int32_t)i;  // rank among the holes of [0, n1)
-    } else if (f) {
-        donors[n1 - s[i] - 1] = (int32_t)i;    // rank from the top among live entries of [n1, n)
-    }
-}
+};
 
 __global__ void k_compact_gather(const double *__restrict__ x, const double *__restrict__ y,
                                  const double *__restrict__ g, const int32_t *__restrict__ s,
@@ -447,14 +446,13 @@ int launch_compact(vx_case *c, int64_t n_ub, const int64_t *merged, const GridPu
         c->tg.ensure((size_t)(2 * n_ub), st) || c->dcount.ensure(16, st))
         return 1;
     int64_t *dn = step_dn(c), *dn1 = c->dcount.p + (c->dn_slot == 1 ? 8 : 1);
-    const unsigned nb = (unsigned)((n_ub + 255) / 256);
-    // exclusive scan of the live flags (ballot and popcount per wave)
+    // exclusive scan of the live flags (ballot and popcount per wave), which
+    // also lists the holes and the donors
     if (c->ptmp.ensure((size_t)prim::tiles_of(n_ub) + 1, st)) return 1;
-    VX_CHECK(prim::select_flags(LiveFlag{c->g.p, dn, merged}, n_ub, prim::WriteScan{c->scan.p}, c->ptmp.p,
-                                (int32_t *)nullptr, st));
     int32_t *holes = c->lists.p, *donors = c->lists.p + n_ub;
-    hipLaunchKernelGGL(k_compact_lists, dim3(nb), dim3(256), 0, st, c->g.p, c->scan.p, dn, dn1, holes, donors,
-                       merged);
+    VX_CHECK(prim::select_flags_counted(LiveFlag{c->g.p, dn, merged}, n_ub,
+                                        CompactLists{c->scan.p, holes, donors, dn, merged, dn1}, c->ptmp.p,
+                                        (int32_t *)(c->dcount.p + 13), st));
     const unsigned nb2 = (unsigned)((2 * n_ub + 255) / 256);
     hipLaunchKernelGGL(k_compact_gather, dim3(nb2), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->scan.p, dn, dn1,
                        holes, donors, c->tx.p, c->ty.p, c->tg.p, merged);
@@ -494,18 +492,20 @@ struct ColBufs {
 // work-groups.
 static int64_t col_seg_cap(int64_t n_ub) { return ((n_ub + 255) / 256 + kColSeg - 1) / kColSeg * 256; }
 
+// lists holds cand, rest, pcd and state; a compaction's scan (c->scan) and its
+// holes and donors (c->lists below 2 n_ub <= ncap + n_ub) leave pc, pcd,
+// state and first alone, so a pass can leave them prepared for the next.
 static int col_bufs(vx_case *c, int64_t n_ub, bool segmented, ColBufs &b) {
     hipStream_t st = c->stream;
     const int64_t cap = col_seg_cap(n_ub), ncap = kColSeg * cap;
-    if (c->flags.ensure((size_t)n_ub, st) || c->scan.ensure((size_t)n_ub, st) ||
-        c->lists.ensure((size_t)(ncap + 2 * n_ub), st) || c->misc.ensure((size_t)n_ub, st) ||
-        c->dcount.ensure(16, st) || c->colseg.ensure(kColSeg, st))
+    if (c->flags.ensure((size_t)n_ub, st) || c->lists.ensure((size_t)(ncap + 3 * n_ub), st) ||
+        c->misc.ensure((size_t)n_ub, st) || c->dcount.ensure(16, st) || c->colseg.ensure(kColSeg, st))
         return 1;
     b.pc = c->flags.p;
-    b.state = c->scan.p;
     b.cand = c->lists.p;
     b.rest = c->lists.p + ncap;
     b.pcd = c->lists.p + ncap + n_ub;
+    b.state = c->lists.p + ncap + 2 * n_ub;
     b.first = c->misc.p;
     b.ncs = c->colseg.p;
     b.nrest = (int32_t *)(c->dcount.p + 4);
@@ -867,60 +867,78 @@ struct ColCandidate {
     __device__ bool operator()(int32_t i) const { return i < *dn && col[i] >= 0; }
 };
 
-// Leaves of the forest (no candidate points at them: pc 0) are active, as
-// nothing precedes them, and are published at once: their target is marked
-// killed, its first merger recorded and its pending count pcd lowered.  pc
-// itself is only read, so every thread sees the counts from before any
-// publication.  No candidate points at a leaf, so its state slot is written
-// by this thread only.  The other candidates are listed in rest.
-__global__ __launch_bounds__(256) void k_col_leaves(const int32_t *__restrict__ col, ColBufs cb,
-                                                    int64_t *__restrict__ merged) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) {
-        // candidates listed by the search: none means no merge in this pass
-        int64_t k = 0;
-        for (int q = 0; q < cb.nseg; ++q) k += cb.ncs[q];
-        *merged = k;
-    }
-    const int32_t i = col_cand(cb, t);
-    const bool leaf = i >= 0 && cb.pc[i] == 0;
-    if (leaf) {
-        const int32_t j = col[i];
-        cb.state[i] = 1 | 16;
-        atomicOr(&cb.state[j], 4);
-        atomicMin(&cb.first[j], i);
-        atomicSub(&cb.pcd[j], 1);
-    }
-    const uint64_t m = __ballot(i >= 0 && !leaf);
-    if (m == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int f = __builtin_ctzll(m);
-    int base = 0;
-    if (lane == f) base = atomicAdd(cb.nrest, __popcll(m));
-    base = __shfl(base, f, 64);
-    if (i >= 0 && !leaf) cb.rest[base + __popcll(m & ((1ull << lane) - 1))] = i;
-}
-
-// state bits: 1 active, 2 inactive, 4 killed by an active child, 16 published.
-// The remaining (non-leaf) candidates are decided here in one work-group: a
-// candidate is decided once pcd, its count of unpublished candidates pointing
-// at it, reaches 0.
-// Every access goes through L2 atomics (no stale per-CU L1 lines).  Each round
-// decides at least the lowest undecided candidate (its children have lower
-// indices and were published in earlier rounds), so ncand + 1 rounds always
-// suffice; the loop bound is a guard, reported through *err.
-__global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict__ col, const int32_t *__restrict__ cand,
-                                                      const int32_t *__restrict__ ncand_p,
-                                                      int32_t *__restrict__ pcd, int32_t *__restrict__ state,
-                                                      int32_t *__restrict__ first, int32_t *__restrict__ err) {
+// The pass's resolution and merge, in one work-group, phase after phase
+// (state bits: 1 active, 2 inactive, 4 killed by an active child, 16
+// published; every access to state, first and pcd goes through L2 atomics,
+// so no stale per-CU L1 line is read):
+//  1. Leaves of the forest (no candidate points at them: pc 0) are active,
+//     as nothing precedes them, and are published at once: their target is
+//     marked killed, its first merger recorded and its pending count pcd
+//     lowered.  pc itself is only read, so every leaf sees the counts from
+//     before any publication.  The other candidates are listed in rest.
+//  2. The rest are decided in rounds: a candidate is decided once pcd, its
+//     count of unpublished candidates pointing at it, reaches 0.  Each round
+//     decides at least the lowest undecided candidate (its children have
+//     lower indices and were published in earlier rounds), so ncand + 1
+//     rounds always suffice; the loop bound is a guard, reported through
+//     *err (bit 1).
+//  3. Each active candidate i merges with j = COL_i: the smallest active
+//     candidate of j (first[j] == i) with j as it was, then kills j; the
+//     others with the killed j (1e10, 1e10, 0).  No two threads touch one
+//     slot: an active i is never the target of an active candidate (that
+//     would have made it inactive), so x[i] is read and written by thread i
+//     alone, and x[j] by j's first merger alone.  A kept grid gets a node
+//     for each survivor at its new cell.
+//  4. With reset, every slot the pass touched (the candidates and their
+//     targets) gets its prepared work-array values back and the list counts
+//     are zeroed, so the next pass's search needs no preparation.
+// The candidates are the cb.nseg segments of the search's lists; *merged
+// receives their count.
+template <int KIND>
+__global__ __launch_bounds__(1024) void k_col_pass(double *__restrict__ x, double *__restrict__ y,
+                                                   double *__restrict__ g, const int32_t *__restrict__ col, ColBufs cb,
+                                                   int64_t *__restrict__ merged, int32_t *__restrict__ err,
+                                                   GridPush gp, int reset) {
+    __shared__ int64_t segoff[kColSeg + 1];
+    __shared__ int32_t nrest;
     __shared__ int undecided;
-    const int64_t ncand = (int64_t)*ncand_p;
-    int64_t round = 0;
-    for (;; ++round) {
+    if (threadIdx.x == 0) {
+        int64_t k = 0;
+        for (int q = 0; q < cb.nseg; ++q) {
+            segoff[q] = k;
+            k += cb.ncs[q];
+        }
+        segoff[cb.nseg] = k;
+        *merged = k;
+        nrest = 0;
+    }
+    __syncthreads();
+    const int64_t K = segoff[cb.nseg];
+    auto cand = [&](int64_t t) -> int32_t {
+        int q = 0;
+        while (segoff[q + 1] <= t) ++q;
+        return cb.cand[q * cb.cap + (t - segoff[q])];
+    };
+    int32_t *__restrict__ state = cb.state, *__restrict__ first = cb.first, *__restrict__ pcd = cb.pcd;
+    for (int64_t t = threadIdx.x; t < K; t += blockDim.x) {
+        const int32_t i = cand(t);
+        if (cb.pc[i] == 0) {
+            const int32_t j = col[i];
+            atomicOr(&state[i], 1 | 16);
+            atomicOr(&state[j], 4);
+            atomicMin(&first[j], i);
+            atomicSub(&pcd[j], 1);
+        } else {
+            cb.rest[atomicAdd(&nrest, 1)] = i;
+        }
+    }
+    __syncthreads();
+    const int64_t ncand = nrest;
+    for (int64_t round = 0;; ++round) {
         if (threadIdx.x == 0) undecided = 0;
         __syncthreads();
         for (int64_t t = threadIdx.x; t < ncand; t += blockDim.x) {
-            const int32_t i = cand[t];
+            const int32_t i = cb.rest[t];
             const int32_t s = atomicOr(&state[i], 0);
             if (s & 3) continue;
             if (atomicAdd(&pcd[i], 0) == 0)
@@ -930,7 +948,7 @@ __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict_
         }
         __syncthreads();
         for (int64_t t = threadIdx.x; t < ncand; t += blockDim.x) {
-            const int32_t i = cand[t];
+            const int32_t i = cb.rest[t];
             const int32_t s = atomicOr(&state[i], 0);
             if ((s & 3) && !(s & 16)) {
                 const int32_t j = col[i];
@@ -947,64 +965,66 @@ __global__ __launch_bounds__(1024) void k_col_resolve(const int32_t *__restrict_
         __syncthreads();
         if (!more) break;
         if (round > ncand + 1) {
-            if (threadIdx.x == 0) *err = 1;
+            if (threadIdx.x == 0) atomicOr(err, 1);
             break;
         }
     }
+    for (int64_t t = threadIdx.x; t < K; t += blockDim.x) {
+        const int32_t i = cand(t);
+        if (!(atomicOr(&state[i], 0) & 1)) continue;
+        const int32_t j = col[i];
+        const bool fst = atomicOr(&first[j], 0) == i;
+        double xj = 1e+10, yj = 1e+10, gj = 0.0;
+        if (fst) {
+            xj = x[j];
+            yj = y[j];
+            gj = g[j];
+        }
+        const double xi = x[i], yi = y[i], gi = g[i];
+        double nx, ny, ng;
+        if (KIND == 1) {
+            const double mn = fabs(gi) + fabs(gj);
+            nx = (xi * fabs(gi) + xj * fabs(gj)) / fabs(mn);
+            ny = (yi * fabs(gi) + yj * fabs(gj)) / fabs(mn);
+            ng = gi + gj;
+        } else {
+            ng = gi + gj;
+            nx = (xi * gi + xj * gj) / ng;
+            ny = (yi * gi + yj * gj) / ng;
+        }
+        x[i] = nx;
+        y[i] = ny;
+        g[i] = ng;
+        if (fst) {
+            x[j] = 1e+10;
+            y[j] = 1e+10;
+            g[j] = 0.0;
+        }
+        if (gp.grid) grid_push(gp, i, nx, ny);  // the survivor may have left its cell
+    }
+    if (!reset) return;
+    __syncthreads();
+    for (int64_t t = threadIdx.x; t < K; t += blockDim.x) {
+        const int32_t i = cand(t), j = col[i];
+        cb.pc[i] = 0;
+        pcd[i] = 0;
+        state[i] = 0;
+        first[i] = INT32_MAX;
+        cb.pc[j] = 0;
+        pcd[j] = 0;
+        state[j] = 0;
+        first[j] = INT32_MAX;
+    }
+    if (threadIdx.x < cb.nseg) cb.ncs[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *cb.nrest = 0;
 }
 
-// Each active candidate i merges with j = COL_i: the smallest active
-// candidate of j (first[j] == i) with j as it was, then kills j; the others
-// with the killed j (1e10, 1e10, 0).  No two threads touch one slot: an
-// active i is never the target of an active candidate (that would have made
-// it inactive), so x[i] is read and written by thread i alone, and x[j] by
-// j's first merger alone.
-template <int KIND>
-__global__ void k_col_merge(double *__restrict__ x, double *__restrict__ y, double *__restrict__ g,
-                            const int32_t *__restrict__ col, ColBufs cb, GridPush gp) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t i = col_cand(cb, t);
-    if (i < 0) return;
-    const int32_t *__restrict__ state = cb.state, *__restrict__ first = cb.first;
-    if (!(state[i] & 1)) return;
-    const int32_t j = col[i];
-    const bool fst = first[j] == i;
-    double xj = 1e+10, yj = 1e+10, gj = 0.0;
-    if (fst) {
-        xj = x[j];
-        yj = y[j];
-        gj = g[j];
-    }
-    const double xi = x[i], yi = y[i], gi = g[i];
-    double nx, ny, ng;
-    if (KIND == 1) {
-        const double mn = fabs(gi) + fabs(gj);
-        nx = (xi * fabs(gi) + xj * fabs(gj)) / fabs(mn);
-        ny = (yi * fabs(gi) + yj * fabs(gj)) / fabs(mn);
-        ng = gi + gj;
-    } else {
-        ng = gi + gj;
-        nx = (xi * gi + xj * gj) / ng;
-        ny = (yi * gi + yj * gj) / ng;
-    }
-    x[i] = nx;
-    y[i] = ny;
-    g[i] = ng;
-    if (fst) {
-        x[j] = 1e+10;
-        y[j] = 1e+10;
-        g[j] = 0.0;
-    }
-    if (gp.grid) grid_push(gp, i, nx, ny);  // the survivor may have left its cell
-}
-
-int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared, const GridPush *gp) {
+int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared, const GridPush *gp, bool reset) {
     if (n_ub <= 0) return 0;
     hipStream_t st = c->stream;
     ColBufs cb{};
     if (col_bufs(c, n_ub, prepared, cb)) return 1;
     const unsigned nb = (unsigned)((n_ub + 255) / 256);
-    const unsigned nbc = (unsigned)(((int64_t)cb.nseg * cb.cap + 255) / 256);  // over the candidate slots
     if (!prepared) {
         hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, cb);
         hipLaunchKernelGGL(k_col_count, dim3(nb), dim3(256), 0, st, c->col.p, step_dn(c), cb);
@@ -1012,14 +1032,14 @@ int launch_collapse(vx_case *c, int kind, int64_t n_ub, bool prepared, const Gri
         if (c->ptmp.ensure((size_t)prim::tiles_of(n_ub) + 1, st)) return 1;
         VX_CHECK(prim::select_flags(prim::pred_flag(pred), n_ub, prim::WriteIndex{cb.cand}, c->ptmp.p, cb.ncs, st));
     }
-    hipLaunchKernelGGL(k_col_leaves, dim3(nbc), dim3(256), 0, st, c->col.p, cb, col_merged(c));
-    hipLaunchKernelGGL(k_col_resolve, dim3(1), dim3(1024), 0, st, c->col.p, cb.rest, cb.nrest, cb.pcd, cb.state,
-                       cb.first, (int32_t *)(c->dcount.p + 3));
     const GridPush push = gp ? *gp : GridPush{};
+    int32_t *err = (int32_t *)(c->dcount.p + 3);
     if (kind == 1)
-        hipLaunchKernelGGL(k_col_merge<1>, dim3(nbc), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb, push);
+        hipLaunchKernelGGL(k_col_pass<1>, dim3(1), dim3(1024), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb,
+                           col_merged(c), err, push, (int)reset);
     else
-        hipLaunchKernelGGL(k_col_merge<2>, dim3(nbc), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb, push);
+        hipLaunchKernelGGL(k_col_pass<2>, dim3(1), dim3(1024), 0, st, c->x.p, c->y.p, c->g.p, c->col.p, cb,
+                           col_merged(c), err, push, (int)reset);
     VX_CHECK(hipGetLastError());
     return 0;
 }
@@ -1071,9 +1091,7 @@ int launch_step_collapse(vx_case *c, int64_t n_ub) {
                                    G, n_ub, cb, (const int64_t *)step_dn(c), (const int64_t *)nullptr, pcount);
                 hipLaunchKernelGGL(k_grid_nodes, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, step_dn(c), G,
                                    n_ub, head, next);
-            } else {
-                hipLaunchKernelGGL(k_col_prepare, dim3(nb), dim3(256), 0, st, n_ub, cb);
-            }
+            }  // else the work arrays were left prepared by the last pass (k_col_pass with reset)
             // after a pass of the same kind, the other slot holds the count before it
             const int64_t *dn = step_dn(c);
             const int64_t *dn_prev = cc > 0 ? c->dcount.p + (c->dn_slot == 1 ? 8 : 1) : nullptr;
@@ -1084,8 +1102,10 @@ int launch_step_collapse(vx_case *c, int64_t n_ub) {
                 hipLaunchKernelGGL(k_setka_nodes<2>, dim3(nb), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, dn,
                                    dn_prev, G, r2, c->prm.max_ve_g, head, next, n_ub, pslot, c->col.p, cb);
             VX_CHECK(hipGetLastError());
-            const GridPush *push = keep ? &gp : nullptr;  // a rebuilt grid needs no pushes
-            if (launch_collapse(c, kind, n_ub, true, push) || launch_compact(c, n_ub, col_merged(c), push)) return 1;
+            // a rebuilt grid needs no pushes, and its clear prepares the work arrays
+            const GridPush *push = keep ? &gp : nullptr;
+            if (launch_collapse(c, kind, n_ub, true, push, keep) || launch_compact(c, n_ub, col_merged(c), push))
+                return 1;
         }
     }
     return 0;
