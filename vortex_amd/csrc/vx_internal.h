@@ -75,8 +75,10 @@ struct RhsSeg {
     int32_t first, count;
 };
 
-// Growable device buffer.  Growth is geometric (at least 1.25x), so sizes
-// that creep up from step to step reallocate O(log) times, not every step.
+// Growable device buffer.  The first allocation takes an eighth more than
+// asked and growth is geometric (at least 1.25x), so sizes that creep up
+// from step to step (a vortex count near its start) reallocate O(log) times,
+// and rarely early in a run.
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -84,6 +86,7 @@ struct DBuf {
     int ensure(size_t want, hipStream_t s, bool keep = false) {
         if (want <= n) return 0;
         size_t cap = want < 1024 ? 1024 : want;
+        if (!n) cap += cap / 8;
         if (n && cap < n + n / 4) cap = n + n / 4;
         T *q = nullptr;
         hipError_t e = hipMalloc((void **)&q, cap * sizeof(T));
