@@ -95,6 +95,8 @@ static int case_init(vx_case *c, const vx_params *prm, const vx_panel *panels, i
     VX_CHECK(hipGetDevice(&c->device));
     VX_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     VX_CHECK(hipHostMalloc((void **)&c->h_count, 128, hipHostMallocDefault));
+    VX_CHECK(hipHostMalloc((void **)&c->h_dev, 128, hipHostMallocMapped | hipHostMallocCoherent));
+    VX_CHECK(hipHostGetDevicePointer((void **)&c->d_dev, c->h_dev, 0));
     VX_CHECK(hipHostMalloc((void **)&c->h_red, 64, hipHostMallocDefault));
     for (auto &e : c->ev) VX_CHECK(hipEventCreate(&e));
     // panels -> SoA, one allocation: 13 double columns + rp
@@ -195,6 +197,7 @@ int vx_case_destroy(vx_case *c) {
     if (c->segs) (void)hipFree(c->segs);
     if (c->bpos) (void)hipFree(c->bpos);
     if (c->h_count) (void)hipHostFree(c->h_count);
+    if (c->h_dev) (void)hipHostFree(c->h_dev);
     if (c->h_red) (void)hipHostFree(c->h_red);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);

@@ -179,6 +179,10 @@ struct vx_case {
     vx::DBuf<double> wstage;                          // their velocity terms, per list slot
     int dn_slot = 1;                                  // dcount slot of the step's current count (1 or 8)
     int64_t *h_count = nullptr;                       // pinned read-back slots (16)
+    // pinned, mapped and coherent slots that kernels store into themselves
+    // (system-scope stores), read by the host after an event: h_dev on the
+    // host, d_dev the same memory on the device (16 slots)
+    int64_t *h_dev = nullptr, *d_dev = nullptr;
     double *h_red = nullptr;                          // pinned reduction read-back
     // probes
     vx::DBuf<double> px, py, pu, pv, pg;

@@ -184,9 +184,12 @@ __global__ __launch_bounds__(256) void k_incl_box(const double *__restrict__ x, 
 }
 
 // Box of the included particles and their count (cnt[0]) from the block partials.
+// The included count also goes straight to the host: *hout, pinned and
+// coherent, by a system-scope store (the host reads it after an event that
+// follows this kernel).
 __global__ __launch_bounds__(256) void k_box_final(const double4 *__restrict__ part, const int32_t *__restrict__ pcount,
                                                    int64_t nb, double4 *__restrict__ box,
-                                                   unsigned long long *__restrict__ cnt) {
+                                                   unsigned long long *__restrict__ cnt, int64_t *__restrict__ hout) {
     __shared__ double r[4][256];
     __shared__ long long rc[256];
     double a[4] = {1e300, 1e300, 1e300, 1e300};
@@ -208,6 +211,8 @@ __global__ __launch_bounds__(256) void k_box_final(const double4 *__restrict__ p
     }
     if (threadIdx.x == 0) {
         cnt[0] = (unsigned long long)rc[0];
+        __hip_atomic_store(hout, (int64_t)rc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
         double4 b = make_double4(r[0][0], -r[1][0], r[2][0], -r[3][0]);
         if (!(b.x <= b.y)) b = make_double4(0.0, 1.0, 0.0, 1.0);
         box[0] = b;
@@ -900,11 +905,15 @@ __global__ __launch_bounds__(1024) void k_upper_top(int32_t root, int64_t nb, in
 __global__ void k_tree_counts(const int32_t *__restrict__ gscan, const int32_t *__restrict__ gflag,
                               const int32_t *__restrict__ bscan, const int32_t *__restrict__ bstart,
                               const int32_t *__restrict__ uscan, const int32_t *__restrict__ upper, int64_t m, int big,
-                              const int32_t *__restrict__ sort_err, int64_t *__restrict__ out) {
+                              const int32_t *__restrict__ sort_err, int64_t *__restrict__ out,
+                              int64_t *__restrict__ hout) {
     out[0] = (int64_t)gscan[m - 1] + gflag[m - 1];
     out[1] = big ? (int64_t)bscan[m - 1] + bstart[m - 1] : 1;
     out[2] = big ? (int64_t)uscan[m - 2] + upper[m - 2] : 0;
     out[3] = *sort_err;
+    // and straight to the host (pinned, coherent; read after an event)
+    for (int q = 0; q < 4; ++q) __hip_atomic_store(&hout[q], out[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __threadfence_system();
 }
 
 __global__ void k_pack_nodes(int64_t nn, const double *__restrict__ ncx, const double *__restrict__ ncy,
@@ -2524,10 +2533,10 @@ static int tree_build_a(vx_case *c, int64_t s, hipStream_t st, Births B) {
     const double4 dbox = make_double4(c->prm.x_min, c->prm.x_max, c->prm.y_min, c->prm.y_max);
     hipLaunchKernelGGL(k_incl_box, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, c->g.p, s, dbox, t.incl.p,
                        t.part.p, t.pcount.p, B);
-    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, st, t.part.p, t.pcount.p, nbs, t.part.p + nbs, t.cnt.p);
-    // the included count is read back while the keys are sorted
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, st, t.part.p, t.pcount.p, nbs, t.part.p + nbs, t.cnt.p,
+                       c->d_dev);
+    // the included count reaches the host (c->h_dev[0]) while the keys are sorted
     if (!t.ev) VX_CHECK(hipEventCreateWithFlags(&t.ev, hipEventDisableTiming));
-    VX_CHECK(hipMemcpyAsync(c->h_count, t.cnt.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     VX_CHECK(hipEventRecord(t.ev, st));
     hipLaunchKernelGGL(k_tree_keys, dim3((unsigned)nbs), dim3(256), 0, st, c->x.p, c->y.p, t.incl.p, s,
                        t.part.p + nbs, t.keys.p, t.idx.p, B);
@@ -2545,7 +2554,7 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
     TreeBufs &t = *c->tree;
     const int order = t.order;
     VX_CHECK(hipEventSynchronize(t.ev));
-    const int64_t m = (int64_t)*c->h_count;
+    const int64_t m = c->h_dev[0];
     t.m = m;
     t.nleaf = 0;
     t.ngroup = 0;
@@ -2617,8 +2626,8 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
         return 1;
     VX_CHECK(hipStreamWaitEvent(st, c->aux_done[0], 0));
     hipLaunchKernelGGL(k_tree_counts, dim3(1), dim3(1), 0, st, t.gscan.p, t.gflag.p, t.bscan.p, t.bstart.p, t.uscan.p,
-                       t.upper.p, m, (int)(m > kLeaf), (const int32_t *)(t.cnt.p + 15), t.sizes.p);
-    VX_CHECK(hipMemcpyAsync(c->h_count, t.sizes.p, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+                       t.upper.p, m, (int)(m > kLeaf), (const int32_t *)(t.cnt.p + 15), t.sizes.p,
+                       c->d_dev + 4);
     VX_CHECK(hipEventRecord(t.ev, st));
     hipLaunchKernelGGL(k_parent_init, dim3((unsigned)std::min<int64_t>(1024, (nnb + 255) / 256)), dim3(256), 0, st,
                        (const int64_t *)t.sizes.p, t.parent.p);
@@ -2641,13 +2650,14 @@ static int tree_build_b(vx_case *c, hipStream_t st, Births B) {
                        CutNode{t.first.p, t.count.p, t.bscan.p, t.parent.p, (const int64_t *)t.sizes.p, m},
                        (const int64_t *)t.sizes.p, cut, ncut);
     VX_CHECK(hipEventSynchronize(t.ev));
-    if (c->h_count[3]) {
+    const int64_t *hc = c->h_dev + 4;
+    if (hc[3]) {
         set_error("tree key sort: a tile's look-back did not finish");
         return 1;
     }
-    t.ngroup = c->h_count[0];
-    nb = c->h_count[1];
-    nu = c->h_count[2];
+    t.ngroup = hc[0];
+    nb = hc[1];
+    nu = hc[2];
     t.nleaf = nb;
     t.nnode = nb + nu;
     t.root = (int32_t)(nu > 0 ? nb : 0);
